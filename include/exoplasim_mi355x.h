@@ -1,0 +1,186 @@
+/*
+ * exoplasim_mi355x: C-ABI of the MI355X-native ExoPlaSim time step.
+ *
+ * One handle is one engine on one GPU holding one or more members
+ * (planets). Every member's prognostic state, tables and scratch live in
+ * HBM; the host passes plain pointers and sizes only.
+ *
+ * The engine-internal seam this library replaces is the reference WebGPU
+ * worker's use of SETUP / STEPPER / LAYOUT / RESTART
+ * (src/model/climate/gcm/runner/webgpu/gcm.worker.ts:53-388): each entry
+ * point below cites the reference call it stands in for.
+ *
+ * Errors: every call returns 0 on success and a negative EXO_E_* code on
+ * failure; exo_last_error() returns the message of the last failure on the
+ * calling thread, worded like the reference's worker errors
+ * (gcm.worker.ts:47-51, :317-323; checkpoint/restart/index.ts:50-66).
+ * Calls on one handle are not re-entrant; different handles are
+ * independent.
+ */
+#ifndef EXOPLASIM_MI355X_H
+#define EXOPLASIM_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXO_OK 0
+#define EXO_E_ARG -1      /* invalid argument or configuration */
+#define EXO_E_DEVICE -2   /* HIP runtime failure */
+#define EXO_E_STATE -3    /* call out of order (e.g. step before init) */
+#define EXO_E_UNSTABLE -4 /* non-finite value in the published state */
+#define EXO_E_FORMAT -5   /* restart bytes do not match this layout */
+
+/* Precision of the Legendre contractions. */
+#define EXO_PRECISION_FP32 0       /* plain form: FP32 throughout */
+#define EXO_PRECISION_BF16_SPLIT 1 /* BF16 hi/lo split operands, FP32 acc. */
+
+/*
+ * Planet and run configuration: the subset of the reference's GcmConfig
+ * (src/model/climate/gcm/types.ts:28-62) that the synthetic workloads
+ * vary, plus the seeded terrain and initial-state generator.
+ */
+typedef struct exo_config {
+  int truncation;              /* 21, 42 or 63 */
+  int levels;                  /* 5 or 10 */
+  double timestep_minutes;     /* GcmConfig.timestepMinutes */
+  double radius_earth;         /* GcmConfig.radiusEarth (x 6371220 m) */
+  double gravity;              /* GcmConfig.gravityMs2 */
+  double rotation_period_days; /* GcmConfig.rotationPeriodDays */
+  double pressure_bar;         /* surface pressure (atmosphere resolve) */
+  double gas_constant;         /* J/(kg K) (atmosphere resolve) */
+  double co2_ppm;              /* atmosphere resolve */
+  double flux_wm2;             /* GcmConfig.fluxWm2 */
+  double obliquity_deg;        /* GcmConfig.obliquityDeg */
+  double eccentricity;         /* GcmConfig.eccentricity */
+  double year_days;            /* GcmConfig.yearDays */
+  double ocean_depth_m;        /* GcmConfig.oceanDepthM */
+  int ozone;                   /* GcmConfig.atmosphere.ozone */
+  /* Seeded synthetic planet (no reference counterpart: the reference
+   * reads terrain rasters and .sra climatologies). */
+  double land_fraction;        /* fraction of grid points that are land */
+  double terrain_max_m;        /* highest land, metres */
+  double initial_temperature;  /* isothermal atmosphere, K */
+  double initial_humidity;     /* kg/kg */
+  double noise_kelvin;         /* std of grid-point temperature noise, K */
+  double sea_surface_k;        /* SST climatology, K */
+  uint64_t terrain_seed;
+  uint64_t noise_seed;
+} exo_config;
+
+/* Fills *cfg with the defaults of a T21 L10 Earth-like synthetic planet. */
+void exo_config_default(exo_config *cfg);
+
+/* Last error message of the calling thread ("" if none). */
+const char *exo_last_error(void);
+
+typedef struct exo_engine exo_engine;
+
+/*
+ * Creates an engine on HIP device `device` with room for `max_members`
+ * members. Stands in for GCM_DEVICE.request + KERNEL.context
+ * (device/index.ts:6-35, kernel/index.ts:17-87).
+ */
+int exo_create(int device, int max_members, exo_engine **out);
+void exo_destroy(exo_engine *e);
+
+/*
+ * Adds a member configured from *cfg (FP64 set-up on the host: pumamod,
+ * Gaussian grid, Legendre and Fourier tables, initpm/initsi/makebm/setzt,
+ * physics tables, seeded initial state) and uploads it; the member then
+ * runs the reference's nkits start-up steps on its first exo_step calls.
+ * Stands in for SETUP.state + SETUP.schedule + STEPPER.context
+ * (setup/index.ts:434-477, stepper/index.ts:94-119).
+ * `precision` is EXO_PRECISION_FP32 or EXO_PRECISION_BF16_SPLIT.
+ * Writes the member id to *member.
+ */
+int exo_add_member(exo_engine *e, const exo_config *cfg, int precision,
+                   int *member);
+
+/* Removes a member and frees its device memory. */
+int exo_remove_member(exo_engine *e, int member);
+
+/*
+ * Advances every active member by `steps` model steps in one enqueue
+ * (start-up phases included); returns when the work is queued. Stands in
+ * for STEPPER.values + STEPPER.batch + queue.submit (stepper/index.ts:
+ * 203-334).
+ */
+int exo_step(exo_engine *e, int steps);
+
+/* Blocks until queued steps have finished. Stands in for mapAsync. */
+int exo_sync(exo_engine *e);
+
+/* Model step counter of a member (nstep, counted from the start step). */
+int exo_member_step(exo_engine *e, int member, int64_t *step);
+
+/* Arena layout: total floats and one slot's offset/length by name.
+ * LAYOUT.create / LAYOUT.constants (layout/index.ts:22-53). */
+int exo_arena_floats(exo_engine *e, int member, size_t *floats);
+int exo_slot(exo_engine *e, int member, const char *name, size_t *offset,
+             size_t *floats);
+/* Name of slot `index` in arena order, or NULL past the end. */
+const char *exo_slot_name(int index);
+
+/* Field readback / write (LAYOUT.read / LAYOUT.write, layout/index.ts:
+ * 66-108). `floats` must equal the slot's length. */
+int exo_read_field(exo_engine *e, int member, const char *name, float *out,
+                   size_t floats);
+int exo_write_field(exo_engine *e, int member, const char *name,
+                    const float *values, size_t floats);
+
+/* FP64 set-up tables and constants of a member, as computed on the host
+ * (for checkers and diagnostics). */
+int exo_read_table(exo_engine *e, int member, const char *name, double *out,
+                   size_t count);
+int exo_table_count(exo_engine *e, int member, const char *name,
+                    size_t *count);
+int exo_read_constants(exo_engine *e, int member, int phase, double *out,
+                       size_t count);
+const char *exo_constant_name(int index);
+/* Number of constant phases (start-up steps + main loop) and of start-up
+ * steps (SETUP.schedule, setup/index.ts:455-477). */
+int exo_phase_count(exo_engine *e, int member, int *phases, int *startup);
+int exo_dims(exo_engine *e, int member, int *dims4);
+
+/* Per-step calendar values (16 doubles) for model step `step`
+ * (STEPPER.values, stepper/index.ts:281-334). */
+int exo_step_values(exo_engine *e, int member, int64_t step, double *out16);
+
+/*
+ * Snapshot readback: packs the reference's published field set (101
+ * fields at L10, 76 at L5, in fields/index.ts:81-132 order) and copies it
+ * to `out` (fields x NHOR floats). Fails with EXO_E_UNSTABLE and the
+ * reference's "model became unstable at step N" message when any value
+ * is not finite (gcm.worker.ts:317-323).
+ */
+int exo_read_snapshot(exo_engine *e, int member, float *out, size_t floats);
+
+/* Area-weighted global means (GCM_GRID.globalMean, grid/index.ts:99-111)
+ * of surface temperature (K) and TOA net flux (W/m^2). */
+int exo_global_means(exo_engine *e, int member, double *tsurf,
+                     double *toa_net);
+
+/*
+ * Checkpoint in the reference's GPU restart layout
+ * (checkpoint/restart/index.ts:11-79): "GCM-WEBGPU-STATE\n" magic, a JSON
+ * index of [name, floats] pairs in arena offset order, then the f32
+ * payload. exo_checkpoint_size gives the byte count; exo_checkpoint_save
+ * writes it; exo_checkpoint_restore rejects a mismatched layout with the
+ * reference's decode errors.
+ */
+int exo_checkpoint_size(exo_engine *e, int member, size_t *bytes);
+int exo_checkpoint_save(exo_engine *e, int member, void *out, size_t bytes);
+int exo_checkpoint_restore(exo_engine *e, int member, const void *bytes,
+                           size_t size);
+
+/* Device-side timing of the last exo_step call (ms, HIP events). */
+int exo_last_step_ms(exo_engine *e, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,131 @@
+"""Drives the FP64 CPU oracle (oracle/liboracle.so) over a member's FP64
+set-up as the engine computed it, so the oracle and the GPU start from the
+same inputs. Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class _Member(ctypes.Structure):
+    _fields_ = [
+        ("ntru", ctypes.c_int), ("nlat", ctypes.c_int), ("nlon", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("arena", ctypes.POINTER(ctypes.c_double)),
+        ("tables", ctypes.POINTER(ctypes.c_double)),
+        ("constants", ctypes.POINTER(ctypes.c_double)),
+        ("slot_offsets", ctypes.POINTER(ctypes.c_long)),
+        ("table_offsets", ctypes.POINTER(ctypes.c_long)),
+    ]
+
+
+def oracle_library():
+    if not os.path.exists(ORACLE_LIB):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                       capture_output=True)
+    lib = ctypes.CDLL(ORACLE_LIB)
+    for n in ("oracle_slot_names", "oracle_table_names", "oracle_constant_names"):
+        getattr(lib, n).restype = ctypes.POINTER(ctypes.c_char_p)
+    lib.oracle_step.argtypes = [ctypes.POINTER(_Member), ctypes.c_int]
+    lib.oracle_physics.argtypes = [ctypes.POINTER(_Member)]
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.oracle_transform_inverse.argtypes = [ctypes.POINTER(_Member), D, D, ctypes.c_int, ctypes.c_int]
+    lib.oracle_transform_forward.argtypes = [ctypes.POINTER(_Member), D, D, ctypes.c_int]
+    return lib
+
+
+def _names(fn):
+    out = []
+    p = fn()
+    i = 0
+    while p[i] is not None:
+        out.append(p[i].decode())
+        i += 1
+    return out
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OracleRun:
+    """An FP64 copy of one engine member, stepped by the oracle."""
+
+    def __init__(self, engine, member, table_overrides=None):
+        self.lib = oracle_library()
+        self.engine = engine
+        self.member = member
+        self.dims = engine.dims(member)
+        self.nphase, self.nstartup = engine.phases(member)
+        self.layout = {name: (off, n) for name, off, n in engine.layout(member)}
+        self.arena = engine.read_table(member, "initial").copy()
+        self.table_names = _names(self.lib.oracle_table_names)
+        self.const_names = _names(self.lib.oracle_constant_names)
+        slot_names = _names(self.lib.oracle_slot_names)
+        self.slot_off = np.array([self.layout[n][0] for n in slot_names], dtype=np.int64)
+        overrides = table_overrides or {}
+        parts, offs, at = [], [], 0
+        self.table_at = {}
+        for n in self.table_names:
+            t = overrides.get(n)
+            if t is None:
+                t = engine.read_table(member, n)
+            parts.append(np.asarray(t, dtype=np.float64))
+            offs.append(at)
+            self.table_at[n] = (at, len(t))
+            at += len(t)
+        self.tables = np.concatenate(parts)
+        self.table_off = np.array(offs, dtype=np.int64)
+        self.bm1 = [engine.read_table(member, f"bm1_phase{p}") for p in range(self.nphase)]
+        self.consts = []
+        for p in range(self.nphase):
+            k = engine.constants(member, p)
+            self.consts.append(np.array([k[n] for n in self.const_names], dtype=np.float64))
+        self.phase = 0
+        self.step_no = None
+
+    def _struct(self, consts):
+        m = _Member()
+        m.ntru, m.nlat, m.nlon, m.nlev = self.dims
+        m.arena = _dptr(self.arena)
+        m.tables = _dptr(self.tables)
+        m.constants = _dptr(consts)
+        m.slot_offsets = self.slot_off.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+        m.table_offsets = self.table_off.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+        return m
+
+    def field(self, name):
+        off, n = self.layout[name]
+        return self.arena[off:off + n]
+
+    def step(self, steps=1, physics=False):
+        for _ in range(steps):
+            p = min(self.phase, self.nphase - 1)
+            at, n = self.table_at["bm1"]
+            self.tables[at:at + n] = self.bm1[p]
+            consts = self.consts[p]
+            m = self._struct(consts)
+            self.lib.oracle_step(ctypes.byref(m), 1 if physics else 0)
+            self.phase += 1
+
+    def inverse(self, sp, planes, dmu=False):
+        consts = self.consts[-1]
+        m = self._struct(consts)
+        nlat, nlon = self.dims[1], self.dims[2]
+        sp = np.ascontiguousarray(sp, dtype=np.float64)
+        out = np.zeros(planes * nlat * nlon)
+        self.lib.oracle_transform_inverse(ctypes.byref(m), _dptr(sp), _dptr(out), planes, 1 if dmu else 0)
+        return out
+
+    def forward(self, grid, planes):
+        consts = self.consts[-1]
+        m = self._struct(consts)
+        t = self.dims[0]
+        grid = np.ascontiguousarray(grid, dtype=np.float64)
+        out = np.zeros(planes * (t + 1) * (t + 2))
+        self.lib.oracle_transform_forward(ctypes.byref(m), _dptr(grid), _dptr(out), planes)
+        return out

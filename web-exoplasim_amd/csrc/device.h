@@ -1,0 +1,47 @@
+// Device-side view of the batch: one descriptor per member (planet) and a
+// launch plan that maps a flat workgroup index to (member, tile). Every
+// kernel of the step takes both, so one launch advances every member of a
+// mixed T21/T42/T63, L5/L10 batch; a single run is a batch of one.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "model.h"
+
+#define EXO_MAX_MEMBERS 64
+
+struct DevMember {
+  int ntru, nlat, nlon, nlev;
+  int ncsp, nrsp, nfc, nhor;
+  int precision;  // 0 plain FP32, 1 BF16 hi/lo split Legendre
+  int pad_[3];
+  float* arena;
+  const float* tab;
+  // Split operand tables for the BF16 form (hi, lo planes of qi..qv).
+  const unsigned short* tab_hi;
+  const unsigned short* tab_lo;
+  int slot[SLOT_COUNT];
+  int tabo[TAB_COUNT];
+  float k[CON_COUNT];
+};
+
+// first[i] is the first workgroup of member i; first[n] the total.
+struct LaunchPlan {
+  int n;
+  int member[EXO_MAX_MEMBERS];  // descriptor index of each planned member
+  int first[EXO_MAX_MEMBERS + 1];
+};
+
+__device__ __forceinline__ int plan_find(const LaunchPlan& p, int block, int* tile) {
+  int lo = 0, hi = p.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.first[mid] <= block) lo = mid;
+    else hi = mid - 1;
+  }
+  *tile = block - p.first[lo];
+  return p.member[lo];
+}
+
+#define SLOTP(M, name) ((M).arena + (M).slot[SLOT_##name])
+#define TABP(M, name) ((M).tab + (M).tabo[TAB_##name])
+#define KON(M, name) ((M).k[CON_##name])

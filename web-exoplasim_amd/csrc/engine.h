@@ -1,0 +1,154 @@
+// Engine internals shared by the step builders: member bookkeeping, the
+// engine, and the launch helper that plans one launch over all active
+// members.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <initializer_list>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "device.h"
+#include "setup.h"
+#include "transforms.h"
+
+extern "C" __global__ void k_sp2gp(const DevMember*, LaunchPlan, InvJobs);
+extern "C" __global__ void k_gp2fc(const DevMember*, LaunchPlan, FwdJobs);
+extern "C" __global__ void k_fc2sp(const DevMember*, LaunchPlan, LegJobs);
+extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan, int);
+extern "C" __global__ void k_mean_mode(const DevMember*, LaunchPlan, int);
+extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
+
+namespace exo {
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+struct MemberDev {
+  Member host;
+  int precision = 0;
+  float* d_arena = nullptr;
+  float* d_tab = nullptr;
+  float* d_consts = nullptr;  // phases x CON_COUNT
+  float* d_bm1 = nullptr;     // phases x bm1 count
+  int loaded_phase = 0;       // phase whose constants/bm1 are on the device
+  int phase = 0;              // phase of the next step
+  int64_t step = 0;           // nstep of the next main-loop step
+  int64_t carbon_count = 0;
+  int desc = -1;              // descriptor index
+  bool active = false;
+};
+
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int max_members = 0;
+  DevMember* d_desc = nullptr;
+  std::vector<DevMember> h_desc;
+  std::vector<std::unique_ptr<MemberDev>> members;
+  float* h_params = nullptr;
+  float* d_params = nullptr;
+  size_t params_cap = 0;
+  hipEvent_t params_done = nullptr, t0 = nullptr, t1 = nullptr;
+  bool timed = false;
+  int publish_every = 0;
+};
+
+inline int tiles_column(const DevMember& d) { return (d.nhor + 63) / 64; }
+inline int tiles_coeff(const DevMember& d) { return (d.nrsp + 63) / 64; }
+inline int tiles_levels(const DevMember& d) { return (d.nlev * d.nhor + 63) / 64; }
+inline int tiles_lev_coeff(const DevMember& d) { return (d.nlev * d.nrsp + 63) / 64; }
+
+template <typename Jobs>
+int job_planes_total(const Jobs& jobs, int nlev) {
+  int n = 0;
+  for (int j = 0; j < jobs.n; j++) n += jobs.j[j].per_level ? nlev : 1;
+  return n;
+}
+
+class Stepper {
+ public:
+  Stepper(Engine& e, const std::vector<int>& active) : e_(e), active_(active) {}
+
+  template <typename F>
+  LaunchPlan plan(F tiles) const {
+    LaunchPlan p{};
+    p.n = (int)active_.size();
+    int at = 0;
+    for (int i = 0; i < p.n; i++) {
+      p.member[i] = active_[i];
+      p.first[i] = at;
+      at += tiles(e_.h_desc[active_[i]]);
+    }
+    p.first[p.n] = at;
+    return p;
+  }
+
+  void column(void (*k)(const DevMember*, LaunchPlan)) {
+    const LaunchPlan p = plan(tiles_column);
+    hipLaunchKernelGGL(k, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p);
+  }
+  void coeff(void (*k)(const DevMember*, LaunchPlan)) {
+    const LaunchPlan p = plan(tiles_coeff);
+    hipLaunchKernelGGL(k, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p);
+  }
+  void levels(void (*k)(const DevMember*, LaunchPlan)) {
+    const LaunchPlan p = plan(tiles_levels);
+    hipLaunchKernelGGL(k, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p);
+  }
+  void mkdheat(int stage) {
+    const LaunchPlan p = plan(tiles_levels);
+    hipLaunchKernelGGL(k_mkdheat, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, stage);
+  }
+  void mean_mode(int slot) {
+    const LaunchPlan p = plan(tiles_lev_coeff);
+    hipLaunchKernelGGL(k_mean_mode, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, slot);
+  }
+  void inv(const InvJobs& jobs) {
+    const LaunchPlan p = plan([&](const DevMember& d) { return d.nlat * job_planes_total(jobs, d.nlev); });
+    hipLaunchKernelGGL(k_sp2gp, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
+  }
+  void fwd(const FwdJobs& jobs) {
+    const LaunchPlan p = plan([&](const DevMember& d) { return d.nlat * job_planes_total(jobs, d.nlev); });
+    hipLaunchKernelGGL(k_gp2fc, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
+  }
+  void leg(const LegJobs& jobs) {
+    const LaunchPlan p =
+        plan([&](const DevMember& d) { return ((d.ncsp + 63) / 64) * job_planes_total(jobs, d.nlev); });
+    hipLaunchKernelGGL(k_fc2sp, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
+  }
+  void load_step(const float* d_params) {
+    const LaunchPlan p = plan([](const DevMember&) { return 1; });
+    hipLaunchKernelGGL(k_load_step, dim3(p.n), dim3(64), 0, e_.stream, e_.d_desc, p, d_params);
+  }
+
+ private:
+  Engine& e_;
+  const std::vector<int>& active_;
+};
+
+
+inline InvJob inv_plain(int table, int src, int dst, int per_level) {
+  return InvJob{INV_PLAIN, table, src, -1, dst, -1, per_level, 0};
+}
+inline InvJob inv_winds(int d, int z, int u, int v, int plavor) {
+  return InvJob{INV_DV2UV, -1, d, z, u, v, 1, plavor};
+}
+inline FwdJob fwd_job(int src, int dst, int per_level) { return FwdJob{src, dst, per_level}; }
+inline LegJob leg_plain(int src, int dst, int per_level) { return LegJob{LEG_PLAIN, -1, -1, -1, src, dst, -1, per_level}; }
+
+
+template <typename Jobs, typename Job>
+Jobs jobs_of(std::initializer_list<Job> list) {
+  Jobs j{};
+  for (const Job& x : list) j.j[j.n++] = x;
+  return j;
+}
+
+void enqueue_physics(Engine& e, const std::vector<int>& active);
+void enqueue_step(Engine& e, const std::vector<int>& active);
+
+}  // namespace exo

@@ -1,0 +1,628 @@
+// Engine runtime and C-ABI of the MI355X ExoPlaSim step.
+//
+// An engine owns one HIP stream on one device and a table of members. Each
+// member's FP32 state arena, FP32 tables, per-phase constants and bm1 stay
+// resident in HBM; the host only builds set-up data once, computes the
+// per-step calendar values, and enqueues launches. Every launch covers
+// every active member (LaunchPlan), so a batch of planets of mixed
+// truncation and level count advances with one launch per kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/exoplasim_mi355x.h"
+#include "device.h"
+#include "engine.h"
+#include "setup.h"
+#include "transforms.h"
+
+static thread_local std::string g_error;
+
+static int fail(int code, const std::string& message) {
+  g_error = message;
+  return code;
+}
+
+#define HIP_OR_THROW(expr)                                                              \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      throw exo::DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e));        \
+  } while (0)
+
+extern "C" __global__ void k_calcgp(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_spectrala(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_spectrald_diffusion(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_spectrald_update(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_physics_in(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_physics_out(const DevMember*, LaunchPlan);
+
+// Writes each member's 16 per-step values into its arena's table slots
+// (the params-to-state copies of STEPPER.batch, stepper/index.ts:242-255).
+extern "C" __global__ __launch_bounds__(64) void k_load_step(const DevMember* __restrict__ mems, LaunchPlan plan,
+                                                              const float* __restrict__ params) {
+  const int i = blockIdx.x;  // one block per planned member
+  if (i >= plan.n || threadIdx.x >= EXO_STEP_VALUES) return;
+  const DevMember& M = mems[plan.member[i]];
+  const float v = params[i * EXO_STEP_VALUES + threadIdx.x];
+  const int t = threadIdx.x;
+  int slot, at;
+  if (t < 5) {
+    slot = SLOT_radiation_step;
+    at = t;
+  } else if (t < 12) {
+    slot = SLOT_surface_step;
+    at = t - 5;
+  } else if (t < 13) {
+    slot = SLOT_carbon_step;
+    at = 0;
+  } else {
+    slot = SLOT_snapshot_step;
+    at = t - 13;
+  }
+  M.arena[M.slot[slot] + at] = v;
+}
+
+namespace exo {
+
+// master's step order (stepper/index.ts:125-162) for the dynamical core and
+// gridpointd's frame: gridpointa, spectrala, physics in, physics out,
+// spectrald.
+void enqueue_step(Engine& e, const std::vector<int>& active) {
+  Stepper s(e, active);
+  // gridpointa (dynamics/index.ts:171-230)
+  s.inv(jobs_of<InvJobs, InvJob>({
+      inv_winds(SLOT_sd, SLOT_sz, SLOT_gu, SLOT_gv, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sd, SLOT_gd, 1),
+      inv_plain(TAB_legendre_qi, SLOT_st, SLOT_gt, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sz, SLOT_gz, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sq, SLOT_gq, 1),
+      InvJob{INV_DLAMBDA, TAB_legendre_qi, SLOT_sp, -1, SLOT_gp, SLOT_gpmt, 0, 0},
+      inv_plain(TAB_legendre_qj, SLOT_sp, SLOT_gpj, 0),
+  }));
+  s.column(k_calcgp);
+  s.fwd(jobs_of<FwdJobs, FwdJob>({
+      fwd_job(SLOT_gtn, SLOT_fgtn, 1), fwd_job(SLOT_gqn, SLOT_fgqn, 1), fwd_job(SLOT_gut, SLOT_fgut, 1),
+      fwd_job(SLOT_gvt, SLOT_fgvt, 1), fwd_job(SLOT_guz, SLOT_fguz, 1), fwd_job(SLOT_gvz, SLOT_fgvz, 1),
+      fwd_job(SLOT_gke, SLOT_fgke, 1), fwd_job(SLOT_gvpp, SLOT_fgvpp, 0), fwd_job(SLOT_guq, SLOT_fguq, 1),
+      fwd_job(SLOT_gvq, SLOT_fgvq, 1),
+  }));
+  s.leg(jobs_of<LegJobs, LegJob>({
+      leg_plain(SLOT_fgvpp, SLOT_spt, 0),
+      LegJob{LEG_UV2DV, SLOT_fgvz, SLOT_fguz, SLOT_fgke, -1, SLOT_sdt, SLOT_szt, 1},
+      LegJob{LEG_QTEND, SLOT_fgut, SLOT_fgvt, -1, SLOT_fgtn, SLOT_stt, -1, 1},
+      LegJob{LEG_QTEND, SLOT_fguq, SLOT_fgvq, -1, SLOT_fgqn, SLOT_sqt, -1, 1},
+  }));
+  // spectrala
+  s.coeff(k_spectrala);
+  // gridpointd's frame in (dynamics/index.ts:299-320)
+  s.inv(jobs_of<InvJobs, InvJob>({
+      inv_winds(SLOT_sd, SLOT_sz, SLOT_gu, SLOT_gv, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sq, SLOT_gq, 1),
+      inv_plain(TAB_legendre_qi, SLOT_st, SLOT_gt, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sp, SLOT_gp, 0),
+  }));
+  s.column(k_physics_in);
+  enqueue_physics(e, active);
+  // gridpointd's frame out (dynamics/index.ts:321-344)
+  s.levels(k_physics_out);
+  s.fwd(jobs_of<FwdJobs, FwdJob>({
+      fwd_job(SLOT_gtdt, SLOT_fgtdt, 1),
+      fwd_job(SLOT_gudt, SLOT_fgudt, 1),
+      fwd_job(SLOT_gvdt, SLOT_fgvdt, 1),
+      fwd_job(SLOT_gqdt, SLOT_fgqdt, 1),
+  }));
+  s.leg(jobs_of<LegJobs, LegJob>({
+      leg_plain(SLOT_fgtdt, SLOT_stt, 1),
+      leg_plain(SLOT_fgqdt, SLOT_sqt, 1),
+      LegJob{LEG_UV2DV, SLOT_fgudt, SLOT_fgvdt, -1, -1, SLOT_sdt, SLOT_szt, 1},
+  }));
+  // spectrald (dynamics/index.ts:239-296)
+  s.coeff(k_spectrald_diffusion);
+  s.inv(jobs_of<InvJobs, InvJob>({
+      inv_winds(SLOT_sdp, SLOT_szp, SLOT_zu, SLOT_zv, 1),
+      inv_winds(SLOT_zdsdp1, SLOT_zdszp1, SLOT_zdu, SLOT_zdv, 0),
+  }));
+  s.mkdheat(1);
+  s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
+  s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt1, 1)}));
+  s.inv(jobs_of<InvJobs, InvJob>({inv_winds(SLOT_zdsdp2, SLOT_zdszp2, SLOT_zdu, SLOT_zdv, 0)}));
+  s.mkdheat(2);
+  s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdekin, SLOT_fzdekin, 1)}));
+  s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdekin, SLOT_zsdef, 1)}));
+  s.mean_mode(SLOT_zsdef);
+  s.inv(jobs_of<InvJobs, InvJob>({inv_plain(TAB_legendre_qi, SLOT_zsdef, SLOT_zdekin, 1)}));
+  s.mkdheat(3);
+  s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
+  s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt2, 1)}));
+  s.coeff(k_spectrald_update);
+}
+
+}  // namespace exo
+
+// ------------------------------------------------------------------ C-ABI
+
+struct exo_engine {
+  exo::Engine e;
+};
+
+using exo::Engine;
+using exo::MemberDev;
+
+static MemberDev* member_of(exo_engine* h, int id) {
+  if (!h || id < 0 || id >= (int)h->e.members.size() || !h->e.members[id]) return nullptr;
+  return h->e.members[id].get();
+}
+
+extern "C" {
+
+void exo_config_default(exo_config* cfg) {
+  if (cfg) exo::config_default(cfg);
+}
+
+const char* exo_last_error(void) { return g_error.c_str(); }
+
+int exo_create(int device, int max_members, exo_engine** out) {
+  if (!out || max_members < 1 || max_members > EXO_MAX_MEMBERS)
+    return fail(EXO_E_ARG, "max_members must be in 1.." + std::to_string(EXO_MAX_MEMBERS));
+  auto h = std::make_unique<exo_engine>();
+  Engine& e = h->e;
+  if (device < 0) {
+    // Host-only engine: FP64 set-up and its readback, no device work.
+    e.device = -1;
+    e.max_members = max_members;
+    e.h_desc.assign(max_members, DevMember{});
+    *out = h.release();
+    return EXO_OK;
+  }
+  try {
+    int count = 0;
+    HIP_OR_THROW(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(EXO_E_ARG, "no HIP device " + std::to_string(device));
+    e.device = device;
+    HIP_OR_THROW(hipSetDevice(device));
+    HIP_OR_THROW(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    e.max_members = max_members;
+    HIP_OR_THROW(hipMalloc(&e.d_desc, sizeof(DevMember) * max_members));
+    e.h_desc.assign(max_members, DevMember{});
+    HIP_OR_THROW(hipEventCreateWithFlags(&e.params_done, hipEventDisableTiming));
+    HIP_OR_THROW(hipEventCreate(&e.t0));
+    HIP_OR_THROW(hipEventCreate(&e.t1));
+  } catch (const std::exception& ex) {
+    return fail(EXO_E_DEVICE, ex.what());
+  }
+  *out = h.release();
+  return EXO_OK;
+}
+
+void exo_destroy(exo_engine* h) {
+  if (!h) return;
+  Engine& e = h->e;
+  if (e.device < 0) {
+    delete h;
+    return;
+  }
+  hipSetDevice(e.device);
+  if (e.stream) hipStreamSynchronize(e.stream);
+  for (auto& m : e.members)
+    if (m) {
+      hipFree(m->d_arena);
+      hipFree(m->d_tab);
+      hipFree(m->d_consts);
+      hipFree(m->d_bm1);
+    }
+  hipFree(e.d_desc);
+  hipFree(e.d_params);
+  if (e.h_params) hipHostFree(e.h_params);
+  if (e.params_done) hipEventDestroy(e.params_done);
+  if (e.t0) hipEventDestroy(e.t0);
+  if (e.t1) hipEventDestroy(e.t1);
+  if (e.stream) hipStreamDestroy(e.stream);
+  delete h;
+}
+
+int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* member) {
+  if (!h || !cfg || !member) return fail(EXO_E_ARG, "null argument");
+  if (precision != EXO_PRECISION_FP32 && precision != EXO_PRECISION_BF16_SPLIT)
+    return fail(EXO_E_ARG, "unknown precision");
+  Engine& e = h->e;
+  int desc = -1;
+  std::vector<bool> used(e.max_members, false);
+  for (auto& m : e.members)
+    if (m) used[m->desc] = true;
+  for (int i = 0; i < e.max_members; i++)
+    if (!used[i]) {
+      desc = i;
+      break;
+    }
+  if (desc < 0) return fail(EXO_E_ARG, "engine is full");
+  auto md = std::make_unique<MemberDev>();
+  try {
+    exo::build_member(*cfg, md->host);
+  } catch (const std::exception& ex) {
+    return fail(EXO_E_ARG, ex.what());
+  }
+  if (e.device < 0) {
+    md->desc = desc;
+    md->step = md->host.start_step;
+    md->active = false;
+  } else try {
+    HIP_OR_THROW(hipSetDevice(e.device));
+    exo::Member& m = md->host;
+    md->precision = precision;
+    const size_t nphase = m.constants.size();
+    const size_t nbm1 = m.tlayout.count[TAB_bm1];
+    std::vector<float> arena(m.initial.begin(), m.initial.end());
+    std::vector<float> tab(m.tables.begin(), m.tables.end());
+    std::vector<float> consts(nphase * CON_COUNT), bm1(nphase * nbm1);
+    for (size_t p = 0; p < nphase; p++) {
+      for (int c = 0; c < CON_COUNT; c++) consts[p * CON_COUNT + c] = (float)m.constants[p][c];
+      for (size_t i = 0; i < nbm1; i++) bm1[p * nbm1 + i] = (float)m.bm1[p][i];
+    }
+    HIP_OR_THROW(hipMalloc(&md->d_arena, arena.size() * sizeof(float)));
+    HIP_OR_THROW(hipMalloc(&md->d_tab, tab.size() * sizeof(float)));
+    HIP_OR_THROW(hipMalloc(&md->d_consts, consts.size() * sizeof(float)));
+    HIP_OR_THROW(hipMalloc(&md->d_bm1, bm1.size() * sizeof(float)));
+    HIP_OR_THROW(hipMemcpy(md->d_arena, arena.data(), arena.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OR_THROW(hipMemcpy(md->d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OR_THROW(hipMemcpy(md->d_consts, consts.data(), consts.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OR_THROW(hipMemcpy(md->d_bm1, bm1.data(), bm1.size() * sizeof(float), hipMemcpyHostToDevice));
+    DevMember d{};
+    d.ntru = m.dims.ntru;
+    d.nlat = m.dims.nlat;
+    d.nlon = m.dims.nlon;
+    d.nlev = m.dims.nlev;
+    d.ncsp = exo_ncsp(d.ntru);
+    d.nrsp = exo_nrsp(d.ntru);
+    d.nfc = exo_nfc(d.ntru);
+    d.nhor = d.nlat * d.nlon;
+    d.precision = precision;
+    d.arena = md->d_arena;
+    d.tab = md->d_tab;
+    for (int s = 0; s < SLOT_COUNT; s++) d.slot[s] = (int)m.layout.offset[s];
+    for (int t = 0; t < TAB_COUNT; t++) d.tabo[t] = (int)m.tlayout.offset[t];
+    for (int c = 0; c < CON_COUNT; c++) d.k[c] = consts[c];
+    e.h_desc[desc] = d;
+    HIP_OR_THROW(hipMemcpy(e.d_desc + desc, &d, sizeof(DevMember), hipMemcpyHostToDevice));
+    md->desc = desc;
+    md->loaded_phase = 0;
+    md->phase = 0;
+    md->step = m.start_step;
+    md->carbon_count = 0;
+    md->active = true;
+  } catch (const std::exception& ex) {
+    hipFree(md->d_arena);
+    hipFree(md->d_tab);
+    hipFree(md->d_consts);
+    hipFree(md->d_bm1);
+    return fail(EXO_E_DEVICE, ex.what());
+  }
+  int id = -1;
+  for (size_t i = 0; i < e.members.size(); i++)
+    if (!e.members[i]) {
+      id = (int)i;
+      break;
+    }
+  if (id < 0) {
+    id = (int)e.members.size();
+    e.members.emplace_back();
+  }
+  e.members[id] = std::move(md);
+  *member = id;
+  return EXO_OK;
+}
+
+int exo_remove_member(exo_engine* h, int id) {
+  MemberDev* m = member_of(h, id);
+  if (!m) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  Engine& e = h->e;
+  if (e.device < 0) {
+    e.members[id].reset();
+    return EXO_OK;
+  }
+  hipSetDevice(e.device);
+  hipStreamSynchronize(e.stream);
+  hipFree(m->d_arena);
+  hipFree(m->d_tab);
+  hipFree(m->d_consts);
+  hipFree(m->d_bm1);
+  e.members[id].reset();
+  return EXO_OK;
+}
+
+int exo_step(exo_engine* h, int steps) {
+  if (!h || steps < 0) return fail(EXO_E_ARG, "invalid step count");
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "host-only engine cannot step");
+  std::vector<MemberDev*> act;
+  std::vector<int> desc;
+  for (auto& m : e.members)
+    if (m && m->active) {
+      act.push_back(m.get());
+      desc.push_back(m->desc);
+    }
+  if (act.empty() || steps == 0) return EXO_OK;
+  try {
+    HIP_OR_THROW(hipSetDevice(e.device));
+    const size_t per_step = act.size() * EXO_STEP_VALUES;
+    const size_t need = per_step * (size_t)steps;
+    if (e.params_cap > 0) HIP_OR_THROW(hipEventSynchronize(e.params_done));
+    if (need > e.params_cap) {
+      if (e.h_params) HIP_OR_THROW(hipHostFree(e.h_params));
+      if (e.d_params) HIP_OR_THROW(hipFree(e.d_params));
+      HIP_OR_THROW(hipHostMalloc(&e.h_params, need * sizeof(float), hipHostMallocDefault));
+      HIP_OR_THROW(hipMalloc(&e.d_params, need * sizeof(float)));
+      e.params_cap = need;
+    }
+    // Calendar values for every step of the call, then one upload.
+    std::vector<int> phase(act.size());
+    std::vector<int64_t> stepno(act.size()), carbon(act.size());
+    for (size_t i = 0; i < act.size(); i++) {
+      phase[i] = act[i]->phase;
+      stepno[i] = act[i]->step;
+      carbon[i] = act[i]->carbon_count;
+    }
+    std::vector<std::vector<int>> phase_at(steps, std::vector<int>(act.size()));
+    for (int k = 0; k < steps; k++)
+      for (size_t i = 0; i < act.size(); i++) {
+        const exo::Member& m = act[i]->host;
+        const bool startup = phase[i] < m.startup_steps;
+        double v[EXO_STEP_VALUES];
+        const int publish = (!startup && e.publish_every > 0 && stepno[i] % e.publish_every == 0) ? 1 : 0;
+        exo::step_values(m, stepno[i], carbon[i], publish, v);
+        for (int c = 0; c < EXO_STEP_VALUES; c++) e.h_params[k * per_step + i * EXO_STEP_VALUES + c] = (float)v[c];
+        carbon[i] = v[12] != 0 ? 0 : carbon[i] + 1;
+        phase_at[k][i] = phase[i];
+        if (startup)
+          phase[i]++;
+        else
+          stepno[i]++;
+      }
+    HIP_OR_THROW(hipMemcpyAsync(e.d_params, e.h_params, need * sizeof(float), hipMemcpyHostToDevice, e.stream));
+    HIP_OR_THROW(hipEventRecord(e.params_done, e.stream));
+    HIP_OR_THROW(hipEventRecord(e.t0, e.stream));
+    for (int k = 0; k < steps; k++) {
+      for (size_t i = 0; i < act.size(); i++) {
+        MemberDev* m = act[i];
+        const int p = phase_at[k][i];
+        if (p != m->loaded_phase) {
+          const size_t nbm1 = m->host.tlayout.count[TAB_bm1];
+          HIP_OR_THROW(hipMemcpyAsync(reinterpret_cast<char*>(e.d_desc + m->desc) + offsetof(DevMember, k),
+                                      m->d_consts + (size_t)p * CON_COUNT, sizeof(float) * CON_COUNT,
+                                      hipMemcpyDeviceToDevice, e.stream));
+          HIP_OR_THROW(hipMemcpyAsync(m->d_tab + m->host.tlayout.offset[TAB_bm1], m->d_bm1 + (size_t)p * nbm1,
+                                      sizeof(float) * nbm1, hipMemcpyDeviceToDevice, e.stream));
+          for (int c = 0; c < CON_COUNT; c++) e.h_desc[m->desc].k[c] = (float)m->host.constants[p][c];
+          m->loaded_phase = p;
+        }
+      }
+      exo::Stepper(e, desc).load_step(e.d_params + k * per_step);
+      exo::enqueue_step(e, desc);
+    }
+    HIP_OR_THROW(hipEventRecord(e.t1, e.stream));
+    HIP_OR_THROW(hipGetLastError());
+    e.timed = true;
+    for (size_t i = 0; i < act.size(); i++) {
+      act[i]->phase = phase[i];
+      act[i]->step = stepno[i];
+      act[i]->carbon_count = carbon[i];
+    }
+  } catch (const std::exception& ex) {
+    return fail(EXO_E_DEVICE, ex.what());
+  }
+  return EXO_OK;
+}
+
+int exo_sync(exo_engine* h) {
+  if (!h) return fail(EXO_E_ARG, "null engine");
+  if (h->e.device < 0) return EXO_OK;
+  hipError_t err = hipStreamSynchronize(h->e.stream);
+  if (err != hipSuccess) return fail(EXO_E_DEVICE, std::string("GPU error: ") + hipGetErrorString(err));
+  return EXO_OK;
+}
+
+int exo_last_step_ms(exo_engine* h, double* ms) {
+  if (!h || !ms) return fail(EXO_E_ARG, "null argument");
+  if (!h->e.timed) return fail(EXO_E_STATE, "no step has been timed");
+  float f = 0;
+  hipError_t err = hipEventElapsedTime(&f, h->e.t0, h->e.t1);
+  if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  *ms = f;
+  return EXO_OK;
+}
+
+int exo_member_step(exo_engine* h, int id, int64_t* step) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !step) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  *step = m->step;
+  return EXO_OK;
+}
+
+const char* exo_slot_name(int index) {
+  static const char* names[] = {
+#define EXO_SLOT_NAME(name, shape, count) #name,
+      EXO_FIELDS(EXO_SLOT_NAME)
+#undef EXO_SLOT_NAME
+  };
+  if (index < 0 || index >= SLOT_COUNT) return nullptr;
+  return names[index];
+}
+
+static int slot_index(const char* name) {
+  if (!name) return -1;
+  for (int i = 0; i < SLOT_COUNT; i++)
+    if (std::strcmp(exo_slot_name(i), name) == 0) return i;
+  return -1;
+}
+
+static const char* table_name(int index) {
+  static const char* names[] = {
+#define EXO_TABLE_NAME(name, count) #name,
+      EXO_TABLES(EXO_TABLE_NAME)
+#undef EXO_TABLE_NAME
+  };
+  if (index < 0 || index >= TAB_COUNT) return nullptr;
+  return names[index];
+}
+
+static int table_index(const char* name) {
+  if (!name) return -1;
+  for (int i = 0; i < TAB_COUNT; i++)
+    if (std::strcmp(table_name(i), name) == 0) return i;
+  return -1;
+}
+
+const char* exo_constant_name(int index) {
+  static const char* names[] = {
+#define EXO_CONST_NAME(name) #name,
+      EXO_CONSTANTS(EXO_CONST_NAME)
+#undef EXO_CONST_NAME
+  };
+  if (index < 0 || index >= CON_COUNT) return nullptr;
+  return names[index];
+}
+
+int exo_arena_floats(exo_engine* h, int id, size_t* floats) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !floats) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  *floats = m->host.layout.total;
+  return EXO_OK;
+}
+
+int exo_slot(exo_engine* h, int id, const char* name, size_t* offset, size_t* floats) {
+  MemberDev* m = member_of(h, id);
+  if (!m) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  const int s = slot_index(name);
+  if (s < 0) return fail(EXO_E_ARG, std::string("no field ") + (name ? name : "(null)"));
+  if (offset) *offset = m->host.layout.offset[s];
+  if (floats) *floats = m->host.layout.floats[s];
+  return EXO_OK;
+}
+
+int exo_read_field(exo_engine* h, int id, const char* name, float* out, size_t floats) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !out) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  const int s = slot_index(name);
+  if (s < 0) return fail(EXO_E_ARG, std::string("no field ") + (name ? name : "(null)"));
+  if (floats != m->host.layout.floats[s])
+    return fail(EXO_E_ARG, std::string(name) + " needs " + std::to_string(m->host.layout.floats[s]) +
+                               " values, got " + std::to_string(floats));
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
+  hipSetDevice(e.device);
+  hipError_t err = hipMemcpyAsync(out, m->d_arena + m->host.layout.offset[s], floats * sizeof(float),
+                                  hipMemcpyDeviceToHost, e.stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+  if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  return EXO_OK;
+}
+
+int exo_write_field(exo_engine* h, int id, const char* name, const float* values, size_t floats) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !values) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  const int s = slot_index(name);
+  if (s < 0) return fail(EXO_E_ARG, std::string("no field ") + (name ? name : "(null)"));
+  if (floats != m->host.layout.floats[s])
+    return fail(EXO_E_ARG, std::string(name) + " needs " + std::to_string(m->host.layout.floats[s]) +
+                               " values, got " + std::to_string(floats));
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
+  hipSetDevice(e.device);
+  hipError_t err = hipMemcpyAsync(m->d_arena + m->host.layout.offset[s], values, floats * sizeof(float),
+                                  hipMemcpyHostToDevice, e.stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+  if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  return EXO_OK;
+}
+
+int exo_table_count(exo_engine* h, int id, const char* name, size_t* count) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !count) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  if (name && std::strcmp(name, "qc_raw") == 0) {
+    *count = m->host.qc_raw.size();
+    return EXO_OK;
+  }
+  if (name && std::strcmp(name, "qi_raw") == 0) {
+    *count = m->host.qi_raw.size();
+    return EXO_OK;
+  }
+  if (name && std::strcmp(name, "land_mask") == 0) {
+    *count = m->host.land_mask.size();
+    return EXO_OK;
+  }
+  if (name && std::strcmp(name, "initial") == 0) {
+    *count = m->host.initial.size();
+    return EXO_OK;
+  }
+  if (name && std::strncmp(name, "bm1_phase", 9) == 0) {
+    const int p = std::atoi(name + 9);
+    if (p < 0 || p >= (int)m->host.bm1.size()) return fail(EXO_E_ARG, "phase out of range");
+    *count = m->host.bm1[p].size();
+    return EXO_OK;
+  }
+  const int t = table_index(name);
+  if (t < 0) return fail(EXO_E_ARG, std::string("no table ") + (name ? name : "(null)"));
+  *count = m->host.tlayout.count[t];
+  return EXO_OK;
+}
+
+int exo_read_table(exo_engine* h, int id, const char* name, double* out, size_t count) {
+  size_t n = 0;
+  int rc = exo_table_count(h, id, name, &n);
+  if (rc) return rc;
+  if (count != n || !out) return fail(EXO_E_ARG, std::string(name) + " needs " + std::to_string(n) + " values");
+  MemberDev* m = member_of(h, id);
+  const double* src = nullptr;
+  if (std::strcmp(name, "qc_raw") == 0) src = m->host.qc_raw.data();
+  else if (std::strcmp(name, "qi_raw") == 0) src = m->host.qi_raw.data();
+  else if (std::strcmp(name, "land_mask") == 0) src = m->host.land_mask.data();
+  else if (std::strcmp(name, "initial") == 0) src = m->host.initial.data();
+  else if (std::strncmp(name, "bm1_phase", 9) == 0) src = m->host.bm1[std::atoi(name + 9)].data();
+  else src = &m->host.tables[m->host.tlayout.offset[table_index(name)]];
+  std::memcpy(out, src, n * sizeof(double));
+  return EXO_OK;
+}
+
+int exo_read_constants(exo_engine* h, int id, int phase, double* out, size_t count) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !out) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  const int nph = (int)m->host.constants.size();
+  if (phase < 0 || phase >= nph) return fail(EXO_E_ARG, "phase out of range");
+  if (count != (size_t)CON_COUNT) return fail(EXO_E_ARG, "constants need " + std::to_string(CON_COUNT));
+  for (int c = 0; c < CON_COUNT; c++) out[c] = m->host.constants[phase][c];
+  return EXO_OK;
+}
+
+int exo_phase_count(exo_engine* h, int id, int* phases, int* startup) {
+  MemberDev* m = member_of(h, id);
+  if (!m) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  if (phases) *phases = (int)m->host.constants.size();
+  if (startup) *startup = m->host.startup_steps;
+  return EXO_OK;
+}
+
+int exo_dims(exo_engine* h, int id, int* dims4) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !dims4) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  dims4[0] = m->host.dims.ntru;
+  dims4[1] = m->host.dims.nlat;
+  dims4[2] = m->host.dims.nlon;
+  dims4[3] = m->host.dims.nlev;
+  return EXO_OK;
+}
+
+int exo_step_values(exo_engine* h, int id, int64_t step, double* out16) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !out16) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  exo::step_values(m->host, step, 0, 0, out16);
+  return EXO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,443 @@
+// Dynamical core of the ExoPlaSim step on gfx950: gridpointa's column
+// work, the semi-implicit spectrala, spectrald's implicit diffusion and
+// time filter, mkdheat's grid-point stages, and gridpointd's physics frame.
+// One lane per grid column or per real spectral coefficient; the level
+// count is a template parameter so the column lives in registers.
+#include "device.h"
+
+namespace {
+
+template <int NL>
+__device__ void calcgp_column(const DevMember& M, int h) {
+  // dynamics/wgsl/calcgp.wgsl:14-147
+  const int NHOR = M.nhor;
+  float* A = M.arena;
+  const float* T = M.tab;
+  auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NHOR + h]; };
+  auto tab = [&](int t, int i) { return T[M.tabo[t] + i]; };
+  const float PSURF = M.k[CON_psurf], WW = M.k[CON_ww], CV = M.k[CON_cv];
+  const float AKAP = M.k[CON_akap], RDBRV = M.k[CON_rdbrv], ADV = M.k[CON_adv];
+  const float rcsq = tab(TAB_rcsq, h);
+  const float ps = expf(A[M.slot[SLOT_gp] + h]);
+  A[M.slot[SLOT_gp] + h] = ps;
+  const float gpm = A[M.slot[SLOT_gpmt] + h];
+  const float gpj = A[M.slot[SLOT_gpj] + h];
+  float gu[NL], gv[NL], gt[NL], gd[NL], gz[NL], gq[NL], zvgpg[NL], ztv1[NL], ztv2[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    gu[l] = lev(SLOT_gu, l);
+    gv[l] = lev(SLOT_gv, l);
+    gt[l] = lev(SLOT_gt, l);
+    gd[l] = lev(SLOT_gd, l);
+    gz[l] = lev(SLOT_gz, l);
+    gq[l] = lev(SLOT_gq, l);
+    zvgpg[l] = rcsq * (gu[l] * gpm + gv[l] * gpj);
+    const float zq = fmaxf(gq[l] / ps, 0.0f);
+    const float t0 = tab(TAB_t0, l);
+    const float tvirt = (gt[l] + t0) * (1.0f + (1.0f / RDBRV - 1.0f) * zq);
+    ztv1[l] = tvirt - t0;
+    ztv2[l] = tvirt / (1.0f + ADV * zq) - t0;
+  }
+  float zsdotp[NL];
+  float zsumd = tab(TAB_dsigma, 0) * gd[0];
+  float gvp = tab(TAB_dsigma, 0) * zvgpg[0];
+  zsdotp[0] = zsumd + gvp;
+#pragma unroll
+  for (int l = 1; l < NL - 1; l++) {
+    zsumd = zsumd + tab(TAB_dsigma, l) * gd[l];
+    gvp = gvp + tab(TAB_dsigma, l) * zvgpg[l];
+    zsdotp[l] = zsumd + gvp;
+  }
+  zsumd = zsumd + tab(TAB_dsigma, NL - 1) * gd[NL - 1];
+  gvp = gvp + tab(TAB_dsigma, NL - 1) * zvgpg[NL - 1];
+  float gtd[NL], gqm[NL], gud[NL], gvd[NL];
+#pragma unroll
+  for (int l = 0; l < NL - 1; l++) {
+    zsdotp[l] = tab(TAB_sigmah, l) * (zsumd + gvp) - zsdotp[l];
+    gtd[l] = zsdotp[l] * (gt[l + 1] - gt[l]);
+    gqm[l] = zsdotp[l] * (gq[l + 1] + gq[l]);
+    gud[l] = zsdotp[l] * (gu[l + 1] - gu[l]);
+    gvd[l] = zsdotp[l] * (gv[l + 1] - gv[l]);
+  }
+  float gtn[NL], gqn[NL], guz[NL], gvz[NL], dw[NL];
+  zsumd = zvgpg[0] * tab(TAB_dsigma, 0);
+  const float rdsig0 = tab(TAB_rdsig, 0);
+  gtn[0] = gt[0] * gd[0] - AKAP * ztv2[0] * gd[0] -
+           rdsig0 * (gtd[0] + tab(TAB_t01s2, 0) * (tab(TAB_sigmah, 0) * gvp - zsumd));
+  gqn[0] = -rdsig0 * gqm[0];
+  guz[0] = -gu[0] * gz[0] - gpj * ztv1[0] - rdsig0 * gvd[0];
+  gvz[0] = gv[0] * gz[0] - gpm * ztv1[0] - rdsig0 * gud[0];
+  dw[0] = gd[0] * ps * PSURF * WW;
+#pragma unroll
+  for (int l = 1; l < NL; l++) {
+    const int column = l * NL;
+    float ztpta = tab(TAB_c, column) * zvgpg[0];
+    float ztptb = tab(TAB_c, column) * (zvgpg[0] + gd[0]);
+    for (int j = 1; j <= l; j++) {
+      ztpta = ztpta + tab(TAB_c, column + j) * zvgpg[j];
+      ztptb = ztptb + tab(TAB_c, column + j) * (zvgpg[j] + gd[j]);
+    }
+    const float rdsig = tab(TAB_rdsig, l);
+    const float t01s2 = tab(TAB_t01s2, l), t01s2m = tab(TAB_t01s2, l - 1);
+    const float sigmah = tab(TAB_sigmah, l), sigmahm = tab(TAB_sigmah, l - 1);
+    const float tkp = tab(TAB_tkp, l);
+    if (l < NL - 1) {
+      zsumd = zsumd + zvgpg[l] * tab(TAB_dsigma, l);
+      gtn[l] = gt[l] * gd[l] + AKAP * ztv2[l] * (zvgpg[l] - ztptb) + tkp * (zvgpg[l] - ztpta) -
+               rdsig * (gtd[l] + gtd[l - 1] + gvp * (t01s2 * sigmah + t01s2m * sigmahm) -
+                        zsumd * (t01s2m + t01s2) + zvgpg[l] * tab(TAB_dsigma, l) * t01s2m);
+      gqn[l] = -rdsig * (gqm[l] - gqm[l - 1]);
+      guz[l] = -gu[l] * gz[l] - gpj * ztv1[l] - rdsig * (gvd[l] + gvd[l - 1]);
+      gvz[l] = gv[l] * gz[l] - gpm * ztv1[l] - rdsig * (gud[l] + gud[l - 1]);
+    } else {
+      gtn[l] = gt[l] * gd[l] + AKAP * ztv2[l] * (zvgpg[l] - ztptb) + tkp * (zvgpg[l] - ztpta) -
+               rdsig * (gtd[NL - 2] + t01s2m * (sigmahm * gvp - zsumd));
+      gqn[l] = rdsig * gqm[NL - 2];
+      guz[l] = -gu[l] * gz[l] - gpj * ztv1[l] - rdsig * gvd[NL - 2];
+      gvz[l] = gv[l] * gz[l] - gpm * ztv1[l] - rdsig * gud[NL - 2];
+    }
+    dw[l] = (zvgpg[l] - ztptb) * ps * PSURF * WW;
+  }
+  const float scale = sqrtf(rcsq);
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    float gsum = 0;
+    for (int k = 0; k < NL; k++) gsum = gsum + tab(TAB_g, l * NL + k) * (ztv1[k] - gt[k]);
+    const float gphi = gsum * 2.0f / rcsq;
+    lev(SLOT_gtn, l) = gtn[l];
+    lev(SLOT_gqn, l) = gqn[l];
+    lev(SLOT_guz, l) = guz[l];
+    lev(SLOT_gvz, l) = gvz[l];
+    lev(SLOT_dw, l) = dw[l];
+    lev(SLOT_gut, l) = gu[l] * gt[l];
+    lev(SLOT_gvt, l) = gv[l] * gt[l];
+    lev(SLOT_gke, l) = gu[l] * gu[l] + gv[l] * gv[l] + gphi;
+    lev(SLOT_guq, l) = gu[l] * gq[l];
+    lev(SLOT_gvq, l) = gv[l] * gq[l];
+    lev(SLOT_du0, l) = CV * gu[l] * scale;
+    lev(SLOT_dv0, l) = CV * gv[l] * scale;
+  }
+  A[M.slot[SLOT_gvpp] + h] = gvp;
+  A[M.slot[SLOT_dp0] + h] = PSURF * ps;
+}
+
+template <int NL>
+__device__ void spectrala_coeff(const DevMember& M, int s) {
+  // dynamics/wgsl/spectrala.wgsl:15-109
+  const int NRSP = M.nrsp;
+  float* A = M.arena;
+  const float* T = M.tab;
+  auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
+  auto tab = [&](int t, int i) { return T[M.tabo[t] + i]; };
+  const float DELT = M.k[CON_delt], DELT2 = M.k[CON_delt2];
+  const float PNU = M.k[CON_pnu], PNU21 = M.k[CON_pnu21];
+  const bool FILTER = M.k[CON_filter] != 0.0f;
+  const float apm = A[M.slot[SLOT_spm] + s];
+  float adm[NL], azm[NL], atm[NL], aqm[NL], adt[NL], stt[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    adm[l] = lev(SLOT_sdm, l);
+    azm[l] = lev(SLOT_szm, l);
+    atm[l] = lev(SLOT_stm, l);
+    aqm[l] = lev(SLOT_sqm, l);
+    adt[l] = lev(SLOT_sdt, l);
+    stt[l] = lev(SLOT_stt, l);
+  }
+  float zgt[NL], zgm[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    float gt = 0, gm = 0;
+    for (int k = 0; k < NL; k++) {
+      gt = gt + tab(TAB_g, l * NL + k) * stt[k];
+      gm = gm + tab(TAB_g, l * NL + k) * atm[k];
+    }
+    zgt[l] = gt;
+    zgm[l] = gm;
+  }
+  float spt = A[M.slot[SLOT_spt] + s];
+  const float spm = apm;
+  const float sop = tab(TAB_sop, s);
+  const int jn = (int)tab(TAB_nindex, s);
+  float sdt[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    float zsum = 0;
+    if (jn > 0) {
+      const float zq = 1.0f / (float)(jn * (jn + 1));
+      const float* bm1 = T + M.tabo[TAB_bm1] + ((jn - 1) * NL + l) * NL;
+      for (int l2 = 0; l2 < NL; l2++) {
+        const float z0 = tab(TAB_t0, l2);
+        const float zt = zgt[l2] - z0 * spt;
+        const float zm = zgm[l2] + z0 * spm;
+        const float za = adt[l2] * zq + sop;
+        const float zb = adm[l2] * zq;
+        const float zz = zb + DELT * (zm + za + DELT * zt);
+        zsum = zsum + zz * bm1[l2];
+      }
+    }
+    sdt[l] = s < 2 ? 0.0f : zsum;
+  }
+#pragma unroll
+  for (int l = 0; l < NL; l++) spt = spt + tab(TAB_dsigma, l) * sdt[l];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    float tau = 0;
+    for (int k = 0; k < NL; k++) tau = tau + tab(TAB_tau, l * NL + k) * sdt[k];
+    stt[l] = stt[l] - tau;
+  }
+  const float spp = A[M.slot[SLOT_spp] + s];
+  if (FILTER) A[M.slot[SLOT_spm] + s] = PNU21 * spp + PNU * apm;
+  const float sppNew = s < 2 ? 0.0f : apm - DELT2 * spt;
+  A[M.slot[SLOT_spp] + s] = sppNew;
+  A[M.slot[SLOT_sp] + s] = sppNew;
+  A[M.slot[SLOT_spt] + s] = spt;
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    if (FILTER) {
+      lev(SLOT_sdm, l) = PNU21 * lev(SLOT_sdp, l) + PNU * adm[l];
+      lev(SLOT_szm, l) = PNU21 * lev(SLOT_szp, l) + PNU * azm[l];
+      lev(SLOT_stm, l) = PNU21 * lev(SLOT_stp, l) + PNU * atm[l];
+      lev(SLOT_sqm, l) = PNU21 * lev(SLOT_sqp, l) + PNU * aqm[l];
+    }
+    const float sdp = 2.0f * sdt[l] - adm[l];
+    const float stp = DELT2 * stt[l] + atm[l];
+    const float szp = DELT2 * lev(SLOT_szt, l) + azm[l];
+    const float sqp = DELT2 * lev(SLOT_sqt, l) + aqm[l];
+    lev(SLOT_sdt, l) = sdt[l];
+    lev(SLOT_stt, l) = stt[l];
+    lev(SLOT_sdp, l) = sdp;
+    lev(SLOT_stp, l) = stp;
+    lev(SLOT_szp, l) = szp;
+    lev(SLOT_sqp, l) = sqp;
+    lev(SLOT_sd, l) = sdp;
+    lev(SLOT_st, l) = stp;
+    lev(SLOT_sz, l) = szp;
+    lev(SLOT_sq, l) = sqp;
+  }
+}
+
+template <int NL>
+__device__ void diffusion_coeff(const DevMember& M, int s) {
+  // dynamics/wgsl/spectrald-diffusion.wgsl:12-60
+  const int NRSP = M.nrsp;
+  float* A = M.arena;
+  const float* T = M.tab;
+  auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
+  const float DELT2 = M.k[CON_delt2], PLAVOR = M.k[CON_plavor];
+  if (s < 2) A[M.slot[SLOT_spp] + s] = 0.0f;
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    const float szp = lev(SLOT_szp, l) + DELT2 * lev(SLOT_szt, l);
+    const float stp = lev(SLOT_stp, l) + DELT2 * lev(SLOT_stt, l);
+    const float sdp = lev(SLOT_sdp, l) + DELT2 * lev(SLOT_sdt, l);
+    const float sqp = lev(SLOT_sqp, l) + DELT2 * lev(SLOT_sqt, l);
+    const float sakpp = T[M.tabo[TAB_sakpp] + l * NRSP + s];
+    const float damp = T[M.tabo[TAB_damp] + l];
+    const float tdisst = T[M.tabo[TAB_tdisst] + l];
+    const float tdissq = T[M.tabo[TAB_tdissq] + l];
+    const float tdissd = T[M.tabo[TAB_tdissd] + l];
+    const float tdissz = T[M.tabo[TAB_tdissz] + l];
+    const float tfrc = T[M.tabo[TAB_tfrc] + l];
+    const float srp = T[M.tabo[TAB_srp] + l * NRSP + s];
+    const float stt = ((-damp - tdisst * sakpp) * stp + damp * srp) / (1.0f + DELT2 * (damp + tdisst * sakpp));
+    const float sqt = -tdissq * sakpp * sqp / (1.0f + DELT2 * tdissq * sakpp);
+    const float zsdt1 = -tfrc * sdp / (1.0f + DELT2 * tfrc);
+    const float zsdt2 = -tdissd * sakpp * sdp / (1.0f + DELT2 * tdissd * sakpp);
+    float zszt1 = -tfrc * szp / (1.0f + DELT2 * tfrc);
+    float zszt2 = -tdissz * sakpp * szp / (1.0f + DELT2 * tdissz * sakpp);
+    float szt = zszt1 + zszt2;
+    if (s == 2) {
+      zszt1 = zszt1 + PLAVOR * tfrc / (1.0f + DELT2 * tfrc);
+      zszt2 = zszt2 + PLAVOR * tdissz * sakpp / (1.0f + DELT2 * tdissz * sakpp);
+      szt = szt + PLAVOR * (tfrc / (1.0f + DELT2 * tfrc) + tdissz * sakpp / (1.0f + DELT2 * tdissz * sakpp));
+    }
+    lev(SLOT_szp, l) = szp;
+    lev(SLOT_stp, l) = stp;
+    lev(SLOT_sdp, l) = sdp;
+    lev(SLOT_sqp, l) = sqp;
+    lev(SLOT_stt, l) = stt;
+    lev(SLOT_sqt, l) = sqt;
+    lev(SLOT_sdt, l) = zsdt1 + zsdt2;
+    lev(SLOT_szt, l) = szt;
+    lev(SLOT_zdsdp1, l) = zsdt1 * DELT2;
+    lev(SLOT_zdszp1, l) = zszt1 * DELT2;
+    lev(SLOT_zdsdp2, l) = zsdt2 * DELT2;
+    lev(SLOT_zdszp2, l) = zszt2 * DELT2;
+  }
+}
+
+template <int NL>
+__device__ void update_coeff(const DevMember& M, int s) {
+  // dynamics/wgsl/spectrald-update.wgsl:12-45
+  const int NRSP = M.nrsp;
+  float* A = M.arena;
+  auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
+  const float DELT2 = M.k[CON_delt2], PNU = M.k[CON_pnu];
+  const bool FILTER = M.k[CON_filter] != 0.0f;
+  const float spp = A[M.slot[SLOT_spp] + s];
+  if (FILTER) A[M.slot[SLOT_spm] + s] = A[M.slot[SLOT_spm] + s] + PNU * spp;
+  A[M.slot[SLOT_sp] + s] = spp;
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    const float stt = lev(SLOT_stt, l) + lev(SLOT_zstt1, l) + lev(SLOT_zstt2, l);
+    lev(SLOT_stt, l) = stt;
+    const float szp = lev(SLOT_szp, l) + DELT2 * lev(SLOT_szt, l);
+    const float stp = lev(SLOT_stp, l) + DELT2 * stt;
+    const float sdp = lev(SLOT_sdp, l) + DELT2 * lev(SLOT_sdt, l);
+    const float sqp = lev(SLOT_sqp, l) + DELT2 * lev(SLOT_sqt, l);
+    lev(SLOT_szp, l) = szp;
+    lev(SLOT_stp, l) = stp;
+    lev(SLOT_sdp, l) = sdp;
+    lev(SLOT_sqp, l) = sqp;
+    if (FILTER) {
+      lev(SLOT_szm, l) = lev(SLOT_szm, l) + PNU * szp;
+      lev(SLOT_stm, l) = lev(SLOT_stm, l) + PNU * stp;
+      lev(SLOT_sdm, l) = lev(SLOT_sdm, l) + PNU * sdp;
+      lev(SLOT_sqm, l) = lev(SLOT_sqm, l) + PNU * sqp;
+    }
+    lev(SLOT_sz, l) = szp;
+    lev(SLOT_st, l) = stp;
+    lev(SLOT_sd, l) = sdp;
+    lev(SLOT_sq, l) = sqp;
+  }
+}
+
+template <int NL>
+__device__ void physics_in_column(const DevMember& M, int h) {
+  // dynamics/wgsl/physics-in.wgsl:8-34
+  const int NHOR = M.nhor;
+  float* A = M.arena;
+  const float PSURF = M.k[CON_psurf], CV = M.k[CON_cv], CT = M.k[CON_ct];
+  const float dp = PSURF * expf(A[M.slot[SLOT_gp] + h]);
+  A[M.slot[SLOT_dp] + h] = dp;
+  const float scale = sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
+#pragma unroll
+  for (int l = 0; l < NL + 1; l++) {
+    const int at = l * NHOR + h;
+    A[M.slot[SLOT_dudt] + at] = 0.0f;
+    A[M.slot[SLOT_dvdt] + at] = 0.0f;
+    A[M.slot[SLOT_dtdt] + at] = 0.0f;
+    A[M.slot[SLOT_dqdt] + at] = 0.0f;
+    if (l < NL) {
+      A[M.slot[SLOT_gudt] + at] = 0.0f;
+      A[M.slot[SLOT_gvdt] + at] = 0.0f;
+      A[M.slot[SLOT_gtdt] + at] = 0.0f;
+      A[M.slot[SLOT_gqdt] + at] = 0.0f;
+      A[M.slot[SLOT_du] + at] = CV * A[M.slot[SLOT_gu] + at] * scale;
+      A[M.slot[SLOT_dv] + at] = CV * A[M.slot[SLOT_gv] + at] * scale;
+      A[M.slot[SLOT_dt] + at] = CT * (A[M.slot[SLOT_gt] + at] + M.tab[M.tabo[TAB_t0] + l]);
+      A[M.slot[SLOT_dq] + at] = A[M.slot[SLOT_gq] + at] * PSURF / dp;
+    }
+  }
+}
+
+}  // namespace
+
+#define EXO_LEVEL_DISPATCH(fn, M, i) \
+  do {                               \
+    if ((M).nlev == 10)              \
+      fn<10>(M, i);                  \
+    else                             \
+      fn<5>(M, i);                   \
+  } while (0)
+
+extern "C" __global__ __launch_bounds__(64) void k_calcgp(const DevMember* __restrict__ mems, LaunchPlan plan) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int h = tile * 64 + threadIdx.x;
+  if (h >= M.nhor) return;
+  EXO_LEVEL_DISPATCH(calcgp_column, M, h);
+}
+
+extern "C" __global__ __launch_bounds__(64) void k_spectrala(const DevMember* __restrict__ mems, LaunchPlan plan) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int s = tile * 64 + threadIdx.x;
+  if (s >= M.nrsp) return;
+  EXO_LEVEL_DISPATCH(spectrala_coeff, M, s);
+}
+
+extern "C" __global__ __launch_bounds__(64) void k_spectrald_diffusion(const DevMember* __restrict__ mems,
+                                                                        LaunchPlan plan) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int s = tile * 64 + threadIdx.x;
+  if (s >= M.nrsp) return;
+  EXO_LEVEL_DISPATCH(diffusion_coeff, M, s);
+}
+
+extern "C" __global__ __launch_bounds__(64) void k_spectrald_update(const DevMember* __restrict__ mems,
+                                                                     LaunchPlan plan) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int s = tile * 64 + threadIdx.x;
+  if (s >= M.nrsp) return;
+  EXO_LEVEL_DISPATCH(update_coeff, M, s);
+}
+
+extern "C" __global__ __launch_bounds__(64) void k_physics_in(const DevMember* __restrict__ mems, LaunchPlan plan) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int h = tile * 64 + threadIdx.x;
+  if (h >= M.nhor) return;
+  EXO_LEVEL_DISPATCH(physics_in_column, M, h);
+}
+
+// dynamics/wgsl/physics-out.wgsl:9-21, per level and point.
+extern "C" __global__ __launch_bounds__(64) void k_physics_out(const DevMember* __restrict__ mems, LaunchPlan plan) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int i = tile * 64 + threadIdx.x;
+  if (i >= M.nlev * M.nhor) return;
+  const int h = i % M.nhor;
+  float* A = M.arena;
+  const float PSURF = M.k[CON_psurf], CV = M.k[CON_cv], CT = M.k[CON_ct], WW = M.k[CON_ww];
+  const float scale = sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
+  A[M.slot[SLOT_gudt] + i] = A[M.slot[SLOT_dudt] + i] / scale / CV / WW + A[M.slot[SLOT_gudt] + i];
+  A[M.slot[SLOT_gvdt] + i] = A[M.slot[SLOT_dvdt] + i] / scale / CV / WW + A[M.slot[SLOT_gvdt] + i];
+  A[M.slot[SLOT_gtdt] + i] = A[M.slot[SLOT_dtdt] + i] / CT / WW + A[M.slot[SLOT_gtdt] + i];
+  A[M.slot[SLOT_gqdt] + i] = A[M.slot[SLOT_dqdt] + i] * A[M.slot[SLOT_dp] + h] / WW / PSURF + A[M.slot[SLOT_gqdt] + i];
+}
+
+// dynamics/wgsl/mkdheat.wgsl:17-44, per level and point.
+extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __restrict__ mems, LaunchPlan plan,
+                                                            int stage) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int i = tile * 64 + threadIdx.x;
+  if (i >= M.nlev * M.nhor) return;
+  const int l = i / M.nhor, h = i % M.nhor;
+  float* A = M.arena;
+  const float CV = M.k[CON_cv], CT = M.k[CON_ct], WW = M.k[CON_ww], GA = M.k[CON_ga];
+  const float ACPD = M.k[CON_acpd], ADV = M.k[CON_adv], DELTSEC2 = M.k[CON_deltsec2];
+  const float heat = 1.0f + ADV * A[M.slot[SLOT_dq] + i];
+  const float dsig = M.tab[M.tabo[TAB_dsigma] + l];
+  if (stage == 3) {
+    const float dp = A[M.slot[SLOT_dp] + h];
+    const float dtdt = -A[M.slot[SLOT_zdekin] + i] * 0.5f / ACPD / heat / dp * GA / dsig;
+    A[M.slot[SLOT_zdtdt] + i] = dtdt / CT / WW;
+    return;
+  }
+  const float scale = CV * sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
+  const float u = A[M.slot[SLOT_zu] + i] * scale;
+  const float v = A[M.slot[SLOT_zv] + i] * scale;
+  const float du = A[M.slot[SLOT_zdu] + i] * scale;
+  const float dv = A[M.slot[SLOT_zdv] + i] * scale;
+  const float change = du * (2.0f * u + du) + dv * (2.0f * v + dv);
+  if (stage == 1) {
+    const float dtdt = -change / DELTSEC2 * 0.5f / ACPD / heat;
+    A[M.slot[SLOT_zdtdt] + i] = dtdt / CT / WW;
+  } else {
+    A[M.slot[SLOT_zdekin] + i] = change / DELTSEC2 * A[M.slot[SLOT_dp] + h] / GA * dsig;
+  }
+}
+
+// dynamics/wgsl/mean-mode.wgsl:4-13: keep only Re mode (0, 0) of a slot.
+extern "C" __global__ __launch_bounds__(64) void k_mean_mode(const DevMember* __restrict__ mems, LaunchPlan plan,
+                                                              int target) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int i = tile * 64 + threadIdx.x;
+  if (i >= M.nlev * M.nrsp) return;
+  if (i % M.nrsp != 0) M.arena[M.slot[target] + i] = 0.0f;
+}

@@ -1,0 +1,274 @@
+"""Python host binding of libexoplasim_mi355x.so (ctypes over the C-ABI in
+include/exoplasim_mi355x.h).
+
+`Engine` holds one engine on one GPU with its members (planets); the
+methods map one to one onto the C entry points. The library is loaded from
+this package directory; a missing or unloadable library raises at once —
+there is no CPU fallback for the step.
+"""
+import ctypes
+import json
+import os
+import struct
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libexoplasim_mi355x.so")
+
+EXO_PRECISION_FP32 = 0
+EXO_PRECISION_BF16_SPLIT = 1
+
+ERRORS = {-1: "argument", -2: "device", -3: "state", -4: "unstable", -5: "format"}
+
+
+class ExoError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class ExoConfig(ctypes.Structure):
+    """exo_config of include/exoplasim_mi355x.h."""
+
+    _fields_ = [
+        ("truncation", ctypes.c_int),
+        ("levels", ctypes.c_int),
+        ("timestep_minutes", ctypes.c_double),
+        ("radius_earth", ctypes.c_double),
+        ("gravity", ctypes.c_double),
+        ("rotation_period_days", ctypes.c_double),
+        ("pressure_bar", ctypes.c_double),
+        ("gas_constant", ctypes.c_double),
+        ("co2_ppm", ctypes.c_double),
+        ("flux_wm2", ctypes.c_double),
+        ("obliquity_deg", ctypes.c_double),
+        ("eccentricity", ctypes.c_double),
+        ("year_days", ctypes.c_double),
+        ("ocean_depth_m", ctypes.c_double),
+        ("ozone", ctypes.c_int),
+        ("land_fraction", ctypes.c_double),
+        ("terrain_max_m", ctypes.c_double),
+        ("initial_temperature", ctypes.c_double),
+        ("initial_humidity", ctypes.c_double),
+        ("noise_kelvin", ctypes.c_double),
+        ("sea_surface_k", ctypes.c_double),
+        ("terrain_seed", ctypes.c_uint64),
+        ("noise_seed", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def library():
+    """Loads the C-ABI library once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ExoError(-2, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    S = ctypes.c_size_t
+    D = ctypes.POINTER(ctypes.c_double)
+    F = ctypes.POINTER(ctypes.c_float)
+    sig = {
+        "exo_config_default": (None, [ctypes.POINTER(ExoConfig)]),
+        "exo_last_error": (ctypes.c_char_p, []),
+        "exo_create": (I, [I, I, ctypes.POINTER(P)]),
+        "exo_destroy": (None, [P]),
+        "exo_add_member": (I, [P, ctypes.POINTER(ExoConfig), I, ctypes.POINTER(I)]),
+        "exo_remove_member": (I, [P, I]),
+        "exo_step": (I, [P, I]),
+        "exo_sync": (I, [P]),
+        "exo_member_step": (I, [P, I, ctypes.POINTER(ctypes.c_int64)]),
+        "exo_arena_floats": (I, [P, I, ctypes.POINTER(S)]),
+        "exo_slot": (I, [P, I, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
+        "exo_slot_name": (ctypes.c_char_p, [I]),
+        "exo_read_field": (I, [P, I, ctypes.c_char_p, F, S]),
+        "exo_write_field": (I, [P, I, ctypes.c_char_p, F, S]),
+        "exo_read_table": (I, [P, I, ctypes.c_char_p, D, S]),
+        "exo_table_count": (I, [P, I, ctypes.c_char_p, ctypes.POINTER(S)]),
+        "exo_read_constants": (I, [P, I, I, D, S]),
+        "exo_constant_name": (ctypes.c_char_p, [I]),
+        "exo_phase_count": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "exo_dims": (I, [P, I, ctypes.POINTER(I)]),
+        "exo_step_values": (I, [P, I, ctypes.c_int64, D]),
+        "exo_last_step_ms": (I, [P, D]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names the C-ABI header declares (for the load/export check)."""
+    return [n for n in _header_functions()]
+
+
+def _header_functions():
+    import re
+    path = os.path.join(os.path.dirname(HERE), "include", "exoplasim_mi355x.h")
+    text = open(path).read()
+    return sorted(set(re.findall(r"\b(exo_[a-z0-9_]+)\s*\(", text)))
+
+
+def default_config(**overrides):
+    cfg = ExoConfig()
+    library().exo_config_default(ctypes.byref(cfg))
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _check(rc):
+    if rc != 0:
+        raise ExoError(rc, library().exo_last_error().decode())
+
+
+def slot_names():
+    lib = library()
+    out = []
+    i = 0
+    while True:
+        n = lib.exo_slot_name(i)
+        if n is None:
+            return out
+        out.append(n.decode())
+        i += 1
+
+
+def constant_names():
+    lib = library()
+    out = []
+    i = 0
+    while True:
+        n = lib.exo_constant_name(i)
+        if n is None:
+            return out
+        out.append(n.decode())
+        i += 1
+
+
+class Engine:
+    """One engine (device < 0: host-only set-up, no stepping)."""
+
+    def __init__(self, device=0, max_members=1):
+        self.lib = library()
+        h = ctypes.c_void_p()
+        _check(self.lib.exo_create(device, max_members, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.exo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # members ------------------------------------------------------------
+    def add_member(self, cfg=None, precision=EXO_PRECISION_FP32):
+        cfg = cfg or default_config()
+        mid = ctypes.c_int()
+        _check(self.lib.exo_add_member(self.h, ctypes.byref(cfg), precision, ctypes.byref(mid)))
+        return mid.value
+
+    def remove_member(self, member):
+        _check(self.lib.exo_remove_member(self.h, member))
+
+    def dims(self, member):
+        d = (ctypes.c_int * 4)()
+        _check(self.lib.exo_dims(self.h, member, d))
+        return tuple(d)
+
+    def phases(self, member):
+        p, s = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.exo_phase_count(self.h, member, ctypes.byref(p), ctypes.byref(s)))
+        return p.value, s.value
+
+    # stepping -----------------------------------------------------------
+    def step(self, steps=1):
+        _check(self.lib.exo_step(self.h, steps))
+
+    def sync(self):
+        _check(self.lib.exo_sync(self.h))
+
+    def last_step_ms(self):
+        v = ctypes.c_double()
+        _check(self.lib.exo_last_step_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def member_step(self, member):
+        v = ctypes.c_int64()
+        _check(self.lib.exo_member_step(self.h, member, ctypes.byref(v)))
+        return v.value
+
+    # layout and fields --------------------------------------------------
+    def arena_floats(self, member):
+        n = ctypes.c_size_t()
+        _check(self.lib.exo_arena_floats(self.h, member, ctypes.byref(n)))
+        return n.value
+
+    def slot(self, member, name):
+        off, n = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(self.lib.exo_slot(self.h, member, name.encode(), ctypes.byref(off), ctypes.byref(n)))
+        return off.value, n.value
+
+    def layout(self, member):
+        return [(name,) + self.slot(member, name) for name in slot_names()]
+
+    def read_field(self, member, name):
+        _, n = self.slot(member, name)
+        out = np.empty(n, dtype=np.float32)
+        _check(self.lib.exo_read_field(self.h, member, name.encode(),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
+        return out
+
+    def write_field(self, member, name, values):
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        _check(self.lib.exo_write_field(self.h, member, name.encode(),
+                                        values.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), values.size))
+
+    def read_table(self, member, name):
+        n = ctypes.c_size_t()
+        _check(self.lib.exo_table_count(self.h, member, name.encode(), ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.float64)
+        _check(self.lib.exo_read_table(self.h, member, name.encode(),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n.value))
+        return out
+
+    def constants(self, member, phase):
+        names = constant_names()
+        out = np.empty(len(names), dtype=np.float64)
+        _check(self.lib.exo_read_constants(self.h, member, phase,
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(names)))
+        return dict(zip(names, out.tolist()))
+
+    def step_values(self, member, step):
+        out = np.empty(16, dtype=np.float64)
+        _check(self.lib.exo_step_values(self.h, member, step,
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def read_arena(self, member):
+        """Every slot, in arena order, as one float32 array."""
+        parts = [self.read_field(member, name) for name in slot_names()]
+        return np.concatenate(parts)
