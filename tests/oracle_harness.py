@@ -87,6 +87,7 @@ class OracleRun:
             self.consts.append(np.array([k[n] for n in self.const_names], dtype=np.float64))
         self.phase = 0
         self.step_no = None
+        self.carbon_interval = int(round(engine.read_table(member, "calendar")[0]))
 
     def _struct(self, consts):
         m = _Member()
@@ -102,9 +103,29 @@ class OracleRun:
         off, n = self.layout[name]
         return self.arena[off:off + n]
 
+    def _load_step_values(self):
+        """STEPPER.values into the arena's per-step slots, as the engine's
+        params-to-state copy does (stepper/index.ts:242-255)."""
+        if self.step_no is None:
+            self.step_no = self.engine.member_step(self.member)
+            self.carbon = 0
+        vals = self.engine.step_values(self.member, self.step_no)
+        interval = self.carbon_interval
+        weathers = (self.carbon + 1) % interval == 0
+        vals[12] = 1.0 if weathers else 0.0
+        self.carbon = 0 if weathers else self.carbon + 1
+        vals[15] = 0.0
+        for name, lo, hi in (("radiation_step", 0, 5), ("surface_step", 5, 12),
+                             ("carbon_step", 12, 13), ("snapshot_step", 13, 16)):
+            off, n = self.layout[name]
+            self.arena[off:off + n] = vals[lo:hi]
+
     def step(self, steps=1, physics=False):
         for _ in range(steps):
             p = min(self.phase, self.nphase - 1)
+            self._load_step_values()
+            if self.phase >= self.nstartup:
+                self.step_no += 1
             at, n = self.table_at["bm1"]
             self.tables[at:at + n] = self.bm1[p]
             consts = self.consts[p]

@@ -561,6 +561,10 @@ int exo_table_count(exo_engine* h, int id, const char* name, size_t* count) {
     *count = m->host.initial.size();
     return EXO_OK;
   }
+  if (name && std::strcmp(name, "calendar") == 0) {
+    *count = 4;
+    return EXO_OK;
+  }
   if (name && std::strncmp(name, "bm1_phase", 9) == 0) {
     const int p = std::atoi(name + 9);
     if (p < 0 || p >= (int)m->host.bm1.size()) return fail(EXO_E_ARG, "phase out of range");
@@ -579,6 +583,14 @@ int exo_read_table(exo_engine* h, int id, const char* name, double* out, size_t 
   if (rc) return rc;
   if (count != n || !out) return fail(EXO_E_ARG, std::string(name) + " needs " + std::to_string(n) + " values");
   MemberDev* m = member_of(h, id);
+  if (std::strcmp(name, "calendar") == 0) {
+    const exo::Calendar& c = m->host.cal;
+    out[0] = c.carbon_interval;
+    out[1] = c.steps_per_day;
+    out[2] = (double)m->host.start_step;
+    out[3] = c.output_window;
+    return EXO_OK;
+  }
   const double* src = nullptr;
   if (std::strcmp(name, "qc_raw") == 0) src = m->host.qc_raw.data();
   else if (std::strcmp(name, "qi_raw") == 0) src = m->host.qi_raw.data();
