@@ -1,7 +1,7 @@
 /*
  * FP64 CPU oracle of the ExoPlaSim model step: TEST INFRASTRUCTURE ONLY.
  *
- * A plain-C restatement, in double precision and in the reference's own
+ * A plain-C restatement, in real precision and in the reference's own
  * dispatch order, of every WGSL kernel of one model step
  * (src/model/climate/gcm/stepper/index.ts:125-162): the Legendre and
  * Fourier transforms with their Fourier intermediates, gridpointa,
@@ -17,7 +17,7 @@
 #include "exo_oracle.h"
 
 
-#include <math.h>
+#include <tgmath.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -31,14 +31,14 @@ const char* const* oracle_constant_names(void) { return oracle_const_names_; }
 /* ------------------------------------------------------------ transforms */
 
 /* sp2fc / sp2fcdmu: legendre/wgsl/inverse.wgsl:6-27 */
-static void sp2fc(const M* v, const double* sp, double* fc, int planes, const double* q) {
+static void sp2fc(const M* v, const real* sp, real* fc, int planes, const real* q) {
   for (int p = 0; p < planes; p++)
     for (int lat = 0; lat < v->NLAT; lat++)
       for (int m = 0; m < v->NTP1; m++) {
         const int first = m * (2 * v->NTP1 - m + 1) / 2;
-        double re = 0, im = 0;
+        real re = 0, im = 0;
         for (int w = first; w < first + v->NTP1 - m; w++) {
-          const double qq = q[lat * v->NCSP + w];
+          const real qq = q[lat * v->NCSP + w];
           re += qq * sp[p * v->NRSP + 2 * w];
           im += qq * sp[p * v->NRSP + 2 * w + 1];
         }
@@ -59,18 +59,18 @@ static void mode_of(int w, int ntp1, int* mo, int* no) {
 }
 
 /* fc2sp: legendre/wgsl/forward.wgsl:7-38 */
-static void fc2sp(const M* v, const double* fc, double* sp, int planes, const double* q) {
+static void fc2sp(const M* v, const real* fc, real* sp, int planes, const real* q) {
   for (int p = 0; p < planes; p++)
     for (int w = 0; w < v->NCSP; w++) {
       int m, n;
       mode_of(w, v->NTP1, &m, &n);
-      const double sign = (m + n) % 2 == 0 ? 1.0 : -1.0;
-      const double* f = fc + (long)p * v->NLAT * v->NFC + 2 * m;
-      double re = 0, im = 0;
+      const real sign = (m + n) % 2 == 0 ? 1.0 : -1.0;
+      const real* f = fc + (long)p * v->NLAT * v->NFC + 2 * m;
+      real re = 0, im = 0;
       for (int l = 0; l < v->NLAT / 2; l++) {
-        const double qq = q[l * v->NCSP + w];
-        const double* no = f + l * v->NFC;
-        const double* so = f + (v->NLAT - 1 - l) * v->NFC;
+        const real qq = q[l * v->NCSP + w];
+        const real* no = f + l * v->NFC;
+        const real* so = f + (v->NLAT - 1 - l) * v->NFC;
         re += qq * (no[0] + sign * so[0]);
         im += qq * (no[1] + sign * so[1]);
       }
@@ -80,54 +80,54 @@ static void fc2sp(const M* v, const double* fc, double* sp, int planes, const do
 }
 
 /* fc2gp: fourier/wgsl/inverse.wgsl:4-18 */
-static void fc2gp(const M* v, const double* fc, double* gp, int planes) {
-  const double* inv = TB(*v, fourier_inverse);
+static void fc2gp(const M* v, const real* fc, real* gp, int planes) {
+  const real* inv = TB(*v, fourier_inverse);
   for (int row = 0; row < planes * v->NLAT; row++)
     for (int j = 0; j < v->NLON; j++) {
-      double sum = 0;
+      real sum = 0;
       for (int k = 0; k < v->NFC; k++) sum += inv[k * v->NLON + j] * fc[(long)row * v->NFC + k];
       gp[(long)row * v->NLON + j] = sum;
     }
 }
 
 /* gp2fc: fourier/wgsl/forward.wgsl:4-18 */
-static void gp2fc(const M* v, const double* gp, double* fc, int planes) {
-  const double* fwd = TB(*v, fourier_forward);
+static void gp2fc(const M* v, const real* gp, real* fc, int planes) {
+  const real* fwd = TB(*v, fourier_forward);
   for (int row = 0; row < planes * v->NLAT; row++)
     for (int k = 0; k < v->NFC; k++) {
-      double sum = 0;
+      real sum = 0;
       for (int j = 0; j < v->NLON; j++) sum += fwd[j * v->NFC + k] * gp[(long)row * v->NLON + j];
       fc[(long)row * v->NFC + k] = sum;
     }
 }
 
 /* dlambda: fourier/wgsl/dlambda.wgsl:5-17 */
-static void dlambda(const M* v, const double* src, double* dst, int planes) {
+static void dlambda(const M* v, const real* src, real* dst, int planes) {
   for (int row = 0; row < planes * v->NLAT; row++)
     for (int m = 0; m < v->NTP1; m++) {
       const long at = (long)row * v->NFC + 2 * m;
-      const double re = src[at], im = src[at + 1];
+      const real re = src[at], im = src[at + 1];
       dst[at] = -im * m;
       dst[at + 1] = re * m;
     }
 }
 
 /* dv2uv: legendre/wgsl/dv2uv.wgsl:9-43 */
-static void dv2uv(const M* v, const double* d, const double* z, double* u, double* vv, double plavor) {
-  const double* qu = TB(*v, legendre_qu);
-  const double* qv = TB(*v, legendre_qv);
+static void dv2uv(const M* v, const real* d, const real* z, real* u, real* vv, real plavor) {
+  const real* qu = TB(*v, legendre_qu);
+  const real* qv = TB(*v, legendre_qv);
   for (int p = 0; p < v->NLEV; p++)
     for (int lat = 0; lat < v->NLAT; lat++)
       for (int m = 0; m < v->NTP1; m++) {
         const int first = m * (2 * v->NTP1 - m + 1) / 2;
-        const double* dd = d + p * v->NRSP;
-        const double* zz = z + p * v->NRSP;
-        double ure = 0, uim = 0, vre = 0, vim = 0;
+        const real* dd = d + p * v->NRSP;
+        const real* zz = z + p * v->NRSP;
+        real ure = 0, uim = 0, vre = 0, vim = 0;
         for (int w = first; w < first + v->NTP1 - m; w++) {
-          const double a = qu[lat * v->NCSP + w], b = qv[lat * v->NCSP + w];
-          const double zre = w == 1 ? zz[2 * w] - plavor : zz[2 * w];
-          const double zim = zz[2 * w + 1];
-          const double dre = dd[2 * w], dim = dd[2 * w + 1];
+          const real a = qu[lat * v->NCSP + w], b = qv[lat * v->NCSP + w];
+          const real zre = w == 1 ? zz[2 * w] - plavor : zz[2 * w];
+          const real zim = zz[2 * w + 1];
+          const real dre = dd[2 * w], dim = dd[2 * w + 1];
           ure = ure + b * zre + a * dim;
           uim = uim + b * zim - a * dre;
           vre = vre + a * zim - b * dre;
@@ -142,26 +142,26 @@ static void dv2uv(const M* v, const double* d, const double* z, double* u, doubl
 }
 
 /* uv2dv / mktend: legendre/wgsl/uv2dv.wgsl:11-63 */
-static void uv2dv(const M* v, const double* U, const double* V, const double* K, double* D, double* Z) {
-  const double* qe = TB(*v, legendre_qe);
-  const double* qm = TB(*v, legendre_qm);
-  const double* qq = TB(*v, legendre_qq);
+static void uv2dv(const M* v, const real* U, const real* V, const real* K, real* D, real* Z) {
+  const real* qe = TB(*v, legendre_qe);
+  const real* qm = TB(*v, legendre_qm);
+  const real* qq = TB(*v, legendre_qq);
   for (int p = 0; p < v->NLEV; p++)
     for (int w = 0; w < v->NCSP; w++) {
       int m, n;
       mode_of(w, v->NTP1, &m, &n);
-      const double s = (m + n) % 2 == 0 ? 1.0 : -1.0;
+      const real s = (m + n) % 2 == 0 ? 1.0 : -1.0;
       const long col = (long)p * v->NLAT * v->NFC + 2 * m;
-      double dre = 0, dim = 0, zre = 0, zim = 0;
+      real dre = 0, dim = 0, zre = 0, zim = 0;
       for (int l = 0; l < v->NLAT / 2; l++) {
         const long no = col + l * v->NFC, so = col + (v->NLAT - 1 - l) * v->NFC;
-        const double e = qe[l * v->NCSP + w], mm = qm[l * v->NCSP + w];
-        const double dure = U[no] - s * U[so], duim = U[no + 1] - s * U[so + 1];
-        const double dvre = V[no] - s * V[so], dvim = V[no + 1] - s * V[so + 1];
-        const double sure = U[no] + s * U[so], suim = U[no + 1] + s * U[so + 1];
-        const double svre = V[no] + s * V[so], svim = V[no + 1] + s * V[so + 1];
+        const real e = qe[l * v->NCSP + w], mm = qm[l * v->NCSP + w];
+        const real dure = U[no] - s * U[so], duim = U[no + 1] - s * U[so + 1];
+        const real dvre = V[no] - s * V[so], dvim = V[no + 1] - s * V[so + 1];
+        const real sure = U[no] + s * U[so], suim = U[no + 1] + s * U[so + 1];
+        const real svre = V[no] + s * V[so], svim = V[no + 1] + s * V[so + 1];
         if (K) {
-          const double c = qq[l * v->NCSP + w];
+          const real c = qq[l * v->NCSP + w];
           dre = dre + c * (K[no] + s * K[so]);
           dim = dim + c * (K[no + 1] + s * K[so + 1]);
         }
@@ -179,20 +179,20 @@ static void uv2dv(const M* v, const double* U, const double* V, const double* K,
 }
 
 /* qtend: legendre/wgsl/qtend.wgsl:9-46 */
-static void qtend(const M* v, const double* U, const double* V, const double* S, double* out) {
-  const double* qe = TB(*v, legendre_qe);
-  const double* qc = TB(*v, legendre_qc);
-  const double* qm = TB(*v, legendre_qm);
+static void qtend(const M* v, const real* U, const real* V, const real* S, real* out) {
+  const real* qe = TB(*v, legendre_qe);
+  const real* qc = TB(*v, legendre_qc);
+  const real* qm = TB(*v, legendre_qm);
   for (int p = 0; p < v->NLEV; p++)
     for (int w = 0; w < v->NCSP; w++) {
       int m, n;
       mode_of(w, v->NTP1, &m, &n);
-      const double s = (m + n) % 2 == 0 ? 1.0 : -1.0;
+      const real s = (m + n) % 2 == 0 ? 1.0 : -1.0;
       const long col = (long)p * v->NLAT * v->NFC + 2 * m;
-      double re = 0, im = 0;
+      real re = 0, im = 0;
       for (int l = 0; l < v->NLAT / 2; l++) {
         const long no = col + l * v->NFC, so = col + (v->NLAT - 1 - l) * v->NFC;
-        const double e = qe[l * v->NCSP + w], c = qc[l * v->NCSP + w], mm = qm[l * v->NCSP + w];
+        const real e = qe[l * v->NCSP + w], c = qc[l * v->NCSP + w], mm = qm[l * v->NCSP + w];
         re = re + e * (V[no] - s * V[so]) + c * (S[no] + s * S[so]) + mm * (U[no + 1] + s * U[so + 1]);
         im = im + e * (V[no + 1] - s * V[so + 1]) + c * (S[no + 1] + s * S[so + 1]) - mm * (U[no] + s * U[so]);
       }
@@ -206,17 +206,17 @@ static void qtend(const M* v, const double* U, const double* V, const double* S,
 /* calcgp: dynamics/wgsl/calcgp.wgsl:14-147 */
 static void calcgp(const M* v) {
   const int L = v->NLEV, NH = v->NHOR;
-  const double PSURF = KK(*v, psurf), WW = KK(*v, ww), CV = KK(*v, cv);
-  const double AKAP = KK(*v, akap), RDBRV = KK(*v, rdbrv), ADV = KK(*v, adv);
-  const double *ds = TB(*v, dsigma), *sh = TB(*v, sigmah), *rd = TB(*v, rdsig);
-  const double *t01 = TB(*v, t01s2), *tkp = TB(*v, tkp), *c = TB(*v, c), *g = TB(*v, g), *t0 = TB(*v, t0);
-  double gu[16], gv[16], gt[16], gd[16], gz[16], gq[16], zvgpg[16], ztv1[16], ztv2[16];
-  double zsdotp[16], gtd[16], gqm[16], gud[16], gvd[16], gtn[16], gqn[16], guz[16], gvz[16], dw[16];
+  const real PSURF = KK(*v, psurf), WW = KK(*v, ww), CV = KK(*v, cv);
+  const real AKAP = KK(*v, akap), RDBRV = KK(*v, rdbrv), ADV = KK(*v, adv);
+  const real *ds = TB(*v, dsigma), *sh = TB(*v, sigmah), *rd = TB(*v, rdsig);
+  const real *t01 = TB(*v, t01s2), *tkp = TB(*v, tkp), *c = TB(*v, c), *g = TB(*v, g), *t0 = TB(*v, t0);
+  real gu[16], gv[16], gt[16], gd[16], gz[16], gq[16], zvgpg[16], ztv1[16], ztv2[16];
+  real zsdotp[16], gtd[16], gqm[16], gud[16], gvd[16], gtn[16], gqn[16], guz[16], gvz[16], dw[16];
   for (int h = 0; h < NH; h++) {
-    const double rcsq = TB(*v, rcsq)[h];
-    const double ps = exp(A(*v, gp)[h]);
+    const real rcsq = TB(*v, rcsq)[h];
+    const real ps = exp(A(*v, gp)[h]);
     A(*v, gp)[h] = ps;
-    const double gpm = A(*v, gpmt)[h], gpj = A(*v, gpj)[h];
+    const real gpm = A(*v, gpmt)[h], gpj = A(*v, gpj)[h];
     for (int l = 0; l < L; l++) {
       gu[l] = A(*v, gu)[l * NH + h];
       gv[l] = A(*v, gv)[l * NH + h];
@@ -225,13 +225,13 @@ static void calcgp(const M* v) {
       gz[l] = A(*v, gz)[l * NH + h];
       gq[l] = A(*v, gq)[l * NH + h];
       zvgpg[l] = rcsq * (gu[l] * gpm + gv[l] * gpj);
-      const double zq = fmax(gq[l] / ps, 0.0);
-      const double tvirt = (gt[l] + t0[l]) * (1.0 + (1.0 / RDBRV - 1.0) * zq);
+      const real zq = fmax(gq[l] / ps, 0.0);
+      const real tvirt = (gt[l] + t0[l]) * (1.0 + (1.0 / RDBRV - 1.0) * zq);
       ztv1[l] = tvirt - t0[l];
       ztv2[l] = tvirt / (1.0 + ADV * zq) - t0[l];
     }
-    double zsumd = ds[0] * gd[0];
-    double gvp = ds[0] * zvgpg[0];
+    real zsumd = ds[0] * gd[0];
+    real gvp = ds[0] * zvgpg[0];
     zsdotp[0] = zsumd + gvp;
     for (int l = 1; l < L - 1; l++) {
       zsumd = zsumd + ds[l] * gd[l];
@@ -255,8 +255,8 @@ static void calcgp(const M* v) {
     dw[0] = gd[0] * ps * PSURF * WW;
     for (int l = 1; l < L; l++) {
       const int column = l * L;
-      double ztpta = c[column] * zvgpg[0];
-      double ztptb = c[column] * (zvgpg[0] + gd[0]);
+      real ztpta = c[column] * zvgpg[0];
+      real ztptb = c[column] * (zvgpg[0] + gd[0]);
       for (int j = 1; j <= l; j++) {
         ztpta = ztpta + c[column + j] * zvgpg[j];
         ztptb = ztptb + c[column + j] * (zvgpg[j] + gd[j]);
@@ -278,11 +278,11 @@ static void calcgp(const M* v) {
       }
       dw[l] = (zvgpg[l] - ztptb) * ps * PSURF * WW;
     }
-    const double scale = sqrt(rcsq);
+    const real scale = sqrt(rcsq);
     for (int l = 0; l < L; l++) {
-      double gsum = 0;
+      real gsum = 0;
       for (int k = 0; k < L; k++) gsum = gsum + g[l * L + k] * (ztv1[k] - gt[k]);
-      const double gphi = gsum * 2.0 / rcsq;
+      const real gphi = gsum * 2.0 / rcsq;
       const long at = (long)l * NH + h;
       A(*v, gtn)[at] = gtn[l];
       A(*v, gqn)[at] = gqn[l];
@@ -305,12 +305,12 @@ static void calcgp(const M* v) {
 /* spectrala: dynamics/wgsl/spectrala.wgsl:15-109 */
 static void spectrala(const M* v) {
   const int L = v->NLEV, NR = v->NRSP;
-  const double DELT = KK(*v, delt), DELT2 = KK(*v, delt2), PNU = KK(*v, pnu), PNU21 = KK(*v, pnu21);
+  const real DELT = KK(*v, delt), DELT2 = KK(*v, delt2), PNU = KK(*v, pnu), PNU21 = KK(*v, pnu21);
   const int FILTER = KK(*v, filter) != 0;
-  const double *g = TB(*v, g), *tau = TB(*v, tau), *bm1 = TB(*v, bm1), *t0 = TB(*v, t0), *ds = TB(*v, dsigma);
-  double adm[16], azm[16], atm[16], aqm[16], adt[16], stt[16], zgt[16], zgm[16], sdt[16];
+  const real *g = TB(*v, g), *tau = TB(*v, tau), *bm1 = TB(*v, bm1), *t0 = TB(*v, t0), *ds = TB(*v, dsigma);
+  real adm[16], azm[16], atm[16], aqm[16], adt[16], stt[16], zgt[16], zgm[16], sdt[16];
   for (int s = 0; s < NR; s++) {
-    const double apm = A(*v, spm)[s];
+    const real apm = A(*v, spm)[s];
     for (int l = 0; l < L; l++) {
       adm[l] = A(*v, sdm)[l * NR + s];
       azm[l] = A(*v, szm)[l * NR + s];
@@ -320,7 +320,7 @@ static void spectrala(const M* v) {
       stt[l] = A(*v, stt)[l * NR + s];
     }
     for (int l = 0; l < L; l++) {
-      double gt = 0, gm = 0;
+      real gt = 0, gm = 0;
       for (int k = 0; k < L; k++) {
         gt = gt + g[l * L + k] * stt[k];
         gm = gm + g[l * L + k] * atm[k];
@@ -328,20 +328,20 @@ static void spectrala(const M* v) {
       zgt[l] = gt;
       zgm[l] = gm;
     }
-    double spt = A(*v, spt)[s];
-    const double spm = apm;
-    const double sop = TB(*v, sop)[s];
+    real spt = A(*v, spt)[s];
+    const real spm = apm;
+    const real sop = TB(*v, sop)[s];
     const int jn = (int)TB(*v, nindex)[s];
     for (int l = 0; l < L; l++) {
-      double zsum = 0;
+      real zsum = 0;
       if (jn > 0) {
-        const double zq = 1.0 / (double)(jn * (jn + 1));
+        const real zq = 1.0 / (real)(jn * (jn + 1));
         for (int l2 = 0; l2 < L; l2++) {
-          const double zt = zgt[l2] - t0[l2] * spt;
-          const double zm = zgm[l2] + t0[l2] * spm;
-          const double za = adt[l2] * zq + sop;
-          const double zb = adm[l2] * zq;
-          const double zz = zb + DELT * (zm + za + DELT * zt);
+          const real zt = zgt[l2] - t0[l2] * spt;
+          const real zm = zgm[l2] + t0[l2] * spm;
+          const real za = adt[l2] * zq + sop;
+          const real zb = adm[l2] * zq;
+          const real zz = zb + DELT * (zm + za + DELT * zt);
           zsum = zsum + zz * bm1[((jn - 1) * L + l) * L + l2];
         }
       }
@@ -349,13 +349,13 @@ static void spectrala(const M* v) {
     }
     for (int l = 0; l < L; l++) spt = spt + ds[l] * sdt[l];
     for (int l = 0; l < L; l++) {
-      double t = 0;
+      real t = 0;
       for (int k = 0; k < L; k++) t = t + tau[l * L + k] * sdt[k];
       stt[l] = stt[l] - t;
     }
-    const double spp = A(*v, spp)[s];
+    const real spp = A(*v, spp)[s];
     if (FILTER) A(*v, spm)[s] = PNU21 * spp + PNU * apm;
-    const double sppn = s < 2 ? 0.0 : apm - DELT2 * spt;
+    const real sppn = s < 2 ? 0.0 : apm - DELT2 * spt;
     A(*v, spp)[s] = sppn;
     A(*v, sp)[s] = sppn;
     A(*v, spt)[s] = spt;
@@ -367,10 +367,10 @@ static void spectrala(const M* v) {
         A(*v, stm)[at] = PNU21 * A(*v, stp)[at] + PNU * atm[l];
         A(*v, sqm)[at] = PNU21 * A(*v, sqp)[at] + PNU * aqm[l];
       }
-      const double sdp = 2.0 * sdt[l] - adm[l];
-      const double stp = DELT2 * stt[l] + atm[l];
-      const double szp = DELT2 * A(*v, szt)[at] + azm[l];
-      const double sqp = DELT2 * A(*v, sqt)[at] + aqm[l];
+      const real sdp = 2.0 * sdt[l] - adm[l];
+      const real stp = DELT2 * stt[l] + atm[l];
+      const real szp = DELT2 * A(*v, szt)[at] + azm[l];
+      const real sqp = DELT2 * A(*v, sqt)[at] + aqm[l];
       A(*v, sdt)[at] = sdt[l];
       A(*v, stt)[at] = stt[l];
       A(*v, sdp)[at] = sdp;
@@ -388,26 +388,26 @@ static void spectrala(const M* v) {
 /* spectrald up to mkdheat: dynamics/wgsl/spectrald-diffusion.wgsl:12-60 */
 static void spectrald_diffusion(const M* v) {
   const int L = v->NLEV, NR = v->NRSP;
-  const double DELT2 = KK(*v, delt2), PLAVOR = KK(*v, plavor);
+  const real DELT2 = KK(*v, delt2), PLAVOR = KK(*v, plavor);
   for (int s = 0; s < NR; s++) {
     if (s < 2) A(*v, spp)[s] = 0.0;
     for (int l = 0; l < L; l++) {
       const long at = (long)l * NR + s;
-      const double szp = A(*v, szp)[at] + DELT2 * A(*v, szt)[at];
-      const double stp = A(*v, stp)[at] + DELT2 * A(*v, stt)[at];
-      const double sdp = A(*v, sdp)[at] + DELT2 * A(*v, sdt)[at];
-      const double sqp = A(*v, sqp)[at] + DELT2 * A(*v, sqt)[at];
-      const double sakpp = TB(*v, sakpp)[at];
-      const double damp = TB(*v, damp)[l], tdisst = TB(*v, tdisst)[l], tdissq = TB(*v, tdissq)[l];
-      const double tdissd = TB(*v, tdissd)[l], tdissz = TB(*v, tdissz)[l], tfrc = TB(*v, tfrc)[l];
-      const double stt = ((-damp - tdisst * sakpp) * stp + damp * TB(*v, srp)[at]) /
+      const real szp = A(*v, szp)[at] + DELT2 * A(*v, szt)[at];
+      const real stp = A(*v, stp)[at] + DELT2 * A(*v, stt)[at];
+      const real sdp = A(*v, sdp)[at] + DELT2 * A(*v, sdt)[at];
+      const real sqp = A(*v, sqp)[at] + DELT2 * A(*v, sqt)[at];
+      const real sakpp = TB(*v, sakpp)[at];
+      const real damp = TB(*v, damp)[l], tdisst = TB(*v, tdisst)[l], tdissq = TB(*v, tdissq)[l];
+      const real tdissd = TB(*v, tdissd)[l], tdissz = TB(*v, tdissz)[l], tfrc = TB(*v, tfrc)[l];
+      const real stt = ((-damp - tdisst * sakpp) * stp + damp * TB(*v, srp)[at]) /
                          (1.0 + DELT2 * (damp + tdisst * sakpp));
-      const double sqt = -tdissq * sakpp * sqp / (1.0 + DELT2 * tdissq * sakpp);
-      const double zsdt1 = -tfrc * sdp / (1.0 + DELT2 * tfrc);
-      const double zsdt2 = -tdissd * sakpp * sdp / (1.0 + DELT2 * tdissd * sakpp);
-      double zszt1 = -tfrc * szp / (1.0 + DELT2 * tfrc);
-      double zszt2 = -tdissz * sakpp * szp / (1.0 + DELT2 * tdissz * sakpp);
-      double szt = zszt1 + zszt2;
+      const real sqt = -tdissq * sakpp * sqp / (1.0 + DELT2 * tdissq * sakpp);
+      const real zsdt1 = -tfrc * sdp / (1.0 + DELT2 * tfrc);
+      const real zsdt2 = -tdissd * sakpp * sdp / (1.0 + DELT2 * tdissd * sakpp);
+      real zszt1 = -tfrc * szp / (1.0 + DELT2 * tfrc);
+      real zszt2 = -tdissz * sakpp * szp / (1.0 + DELT2 * tdissz * sakpp);
+      real szt = zszt1 + zszt2;
       if (s == 2) {
         zszt1 = zszt1 + PLAVOR * tfrc / (1.0 + DELT2 * tfrc);
         zszt2 = zszt2 + PLAVOR * tdissz * sakpp / (1.0 + DELT2 * tdissz * sakpp);
@@ -432,21 +432,21 @@ static void spectrald_diffusion(const M* v) {
 /* mkdheat grid-point stages: dynamics/wgsl/mkdheat.wgsl:17-44 */
 static void mkdheat(const M* v, int stage) {
   const int NH = v->NHOR;
-  const double CV = KK(*v, cv), CT = KK(*v, ct), WW = KK(*v, ww), GA = KK(*v, ga);
-  const double ACPD = KK(*v, acpd), ADV = KK(*v, adv), DELTSEC2 = KK(*v, deltsec2);
+  const real CV = KK(*v, cv), CT = KK(*v, ct), WW = KK(*v, ww), GA = KK(*v, ga);
+  const real ACPD = KK(*v, acpd), ADV = KK(*v, adv), DELTSEC2 = KK(*v, deltsec2);
   for (long i = 0; i < (long)v->NLEV * NH; i++) {
     const int l = (int)(i / NH), h = (int)(i % NH);
-    const double heat = 1.0 + ADV * A(*v, dq)[i];
-    const double dsig = TB(*v, dsigma)[l];
+    const real heat = 1.0 + ADV * A(*v, dq)[i];
+    const real dsig = TB(*v, dsigma)[l];
     if (stage == 3) {
-      const double dtdt = -A(*v, zdekin)[i] * 0.5 / ACPD / heat / A(*v, dp)[h] * GA / dsig;
+      const real dtdt = -A(*v, zdekin)[i] * 0.5 / ACPD / heat / A(*v, dp)[h] * GA / dsig;
       A(*v, zdtdt)[i] = dtdt / CT / WW;
       continue;
     }
-    const double scale = CV * sqrt(TB(*v, rcsq)[h]);
-    const double u = A(*v, zu)[i] * scale, vv = A(*v, zv)[i] * scale;
-    const double du = A(*v, zdu)[i] * scale, dv = A(*v, zdv)[i] * scale;
-    const double change = du * (2.0 * u + du) + dv * (2.0 * vv + dv);
+    const real scale = CV * sqrt(TB(*v, rcsq)[h]);
+    const real u = A(*v, zu)[i] * scale, vv = A(*v, zv)[i] * scale;
+    const real du = A(*v, zdu)[i] * scale, dv = A(*v, zdv)[i] * scale;
+    const real change = du * (2.0 * u + du) + dv * (2.0 * vv + dv);
     if (stage == 1)
       A(*v, zdtdt)[i] = (-change / DELTSEC2 * 0.5 / ACPD / heat) / CT / WW;
     else
@@ -457,20 +457,20 @@ static void mkdheat(const M* v, int stage) {
 /* spectrald after mkdheat: dynamics/wgsl/spectrald-update.wgsl:12-45 */
 static void spectrald_update(const M* v) {
   const int L = v->NLEV, NR = v->NRSP;
-  const double DELT2 = KK(*v, delt2), PNU = KK(*v, pnu);
+  const real DELT2 = KK(*v, delt2), PNU = KK(*v, pnu);
   const int FILTER = KK(*v, filter) != 0;
   for (int s = 0; s < NR; s++) {
-    const double spp = A(*v, spp)[s];
+    const real spp = A(*v, spp)[s];
     if (FILTER) A(*v, spm)[s] = A(*v, spm)[s] + PNU * spp;
     A(*v, sp)[s] = spp;
     for (int l = 0; l < L; l++) {
       const long at = (long)l * NR + s;
-      const double stt = A(*v, stt)[at] + A(*v, zstt1)[at] + A(*v, zstt2)[at];
+      const real stt = A(*v, stt)[at] + A(*v, zstt1)[at] + A(*v, zstt2)[at];
       A(*v, stt)[at] = stt;
-      const double szp = A(*v, szp)[at] + DELT2 * A(*v, szt)[at];
-      const double stp = A(*v, stp)[at] + DELT2 * stt;
-      const double sdp = A(*v, sdp)[at] + DELT2 * A(*v, sdt)[at];
-      const double sqp = A(*v, sqp)[at] + DELT2 * A(*v, sqt)[at];
+      const real szp = A(*v, szp)[at] + DELT2 * A(*v, szt)[at];
+      const real stp = A(*v, stp)[at] + DELT2 * stt;
+      const real sdp = A(*v, sdp)[at] + DELT2 * A(*v, sdt)[at];
+      const real sqp = A(*v, sqp)[at] + DELT2 * A(*v, sqt)[at];
       A(*v, szp)[at] = szp;
       A(*v, stp)[at] = stp;
       A(*v, sdp)[at] = sdp;
@@ -492,11 +492,11 @@ static void spectrald_update(const M* v) {
 /* physics frame in: dynamics/wgsl/physics-in.wgsl:8-34 */
 static void physics_in(const M* v) {
   const int L = v->NLEV, NH = v->NHOR;
-  const double PSURF = KK(*v, psurf), CV = KK(*v, cv), CT = KK(*v, ct);
+  const real PSURF = KK(*v, psurf), CV = KK(*v, cv), CT = KK(*v, ct);
   for (int h = 0; h < NH; h++) {
-    const double dp = PSURF * exp(A(*v, gp)[h]);
+    const real dp = PSURF * exp(A(*v, gp)[h]);
     A(*v, dp)[h] = dp;
-    const double scale = sqrt(TB(*v, rcsq)[h]);
+    const real scale = sqrt(TB(*v, rcsq)[h]);
     for (int l = 0; l <= L; l++) {
       const long at = (long)l * NH + h;
       A(*v, dudt)[at] = 0;
@@ -520,10 +520,10 @@ static void physics_in(const M* v) {
 /* physics frame out: dynamics/wgsl/physics-out.wgsl:9-21 */
 static void physics_out(const M* v) {
   const int NH = v->NHOR;
-  const double PSURF = KK(*v, psurf), CV = KK(*v, cv), CT = KK(*v, ct), WW = KK(*v, ww);
+  const real PSURF = KK(*v, psurf), CV = KK(*v, cv), CT = KK(*v, ct), WW = KK(*v, ww);
   for (long i = 0; i < (long)v->NLEV * NH; i++) {
     const int h = (int)(i % NH);
-    const double scale = sqrt(TB(*v, rcsq)[h]);
+    const real scale = sqrt(TB(*v, rcsq)[h]);
     A(*v, gudt)[i] = A(*v, dudt)[i] / scale / CV / WW + A(*v, gudt)[i];
     A(*v, gvdt)[i] = A(*v, dvdt)[i] / scale / CV / WW + A(*v, gvdt)[i];
     A(*v, gtdt)[i] = A(*v, dtdt)[i] / CT / WW + A(*v, gtdt)[i];
@@ -533,18 +533,18 @@ static void physics_out(const M* v) {
 
 /* ----------------------------------------------------------------- step */
 
-void oracle_transform_inverse(const oracle_member* m, const double* sp, double* grid, int planes, int dmu) {
+void oracle_transform_inverse(const oracle_member* m, const real* sp, real* grid, int planes, int dmu) {
   M v = view(m);
-  const double* q = dmu ? TB(v, legendre_qj) : TB(v, legendre_qi);
-  double* fc = (double*)malloc(sizeof(double) * (size_t)planes * v.NLAT * v.NFC);
+  const real* q = dmu ? TB(v, legendre_qj) : TB(v, legendre_qi);
+  real* fc = (real*)malloc(sizeof(real) * (size_t)planes * v.NLAT * v.NFC);
   sp2fc(&v, sp, fc, planes, q);
   fc2gp(&v, fc, grid, planes);
   free(fc);
 }
 
-void oracle_transform_forward(const oracle_member* m, const double* grid, double* sp, int planes) {
+void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp, int planes) {
   M v = view(m);
-  double* fc = (double*)malloc(sizeof(double) * (size_t)planes * v.NLAT * v.NFC);
+  real* fc = (real*)malloc(sizeof(real) * (size_t)planes * v.NLAT * v.NFC);
   gp2fc(&v, grid, fc, planes);
   fc2sp(&v, fc, sp, planes, TB(v, legendre_qc));
   free(fc);
@@ -553,7 +553,7 @@ void oracle_transform_forward(const oracle_member* m, const double* grid, double
 /* gridpointa: dynamics/index.ts:171-230 */
 static void gridpointa(const M* v) {
   const int L = v->NLEV;
-  const double *qi = TB(*v, legendre_qi), *qj = TB(*v, legendre_qj), *qc = TB(*v, legendre_qc);
+  const real *qi = TB(*v, legendre_qi), *qj = TB(*v, legendre_qj), *qc = TB(*v, legendre_qc);
   dv2uv(v, A(*v, sd), A(*v, sz), A(*v, fgu), A(*v, fgv), KK(*v, plavor));
   sp2fc(v, A(*v, sd), A(*v, fgd), L, qi);
   sp2fc(v, A(*v, st), A(*v, fgt), L, qi);
@@ -591,7 +591,7 @@ static void gridpointa(const M* v) {
 /* spectrald: dynamics/index.ts:239-296 */
 static void spectrald(const M* v) {
   const int L = v->NLEV;
-  const double *qi = TB(*v, legendre_qi), *qc = TB(*v, legendre_qc);
+  const real *qi = TB(*v, legendre_qi), *qc = TB(*v, legendre_qc);
   spectrald_diffusion(v);
   dv2uv(v, A(*v, sdp), A(*v, szp), A(*v, fzu), A(*v, fzv), KK(*v, plavor));
   fc2gp(v, A(*v, fzu), A(*v, zu), L);
@@ -621,7 +621,7 @@ static void spectrald(const M* v) {
 void oracle_step(const oracle_member* m, int with_physics) {
   M v = view(m);
   const int L = v.NLEV;
-  const double *qi = TB(v, legendre_qi), *qc = TB(v, legendre_qc);
+  const real *qi = TB(v, legendre_qi), *qc = TB(v, legendre_qc);
   gridpointa(&v);
   spectrala(&v);
   /* gridpointd's frame in: dynamics/index.ts:299-320 */

@@ -6,16 +6,25 @@
 #ifndef EXO_ORACLE_H
 #define EXO_ORACLE_H
 
-/* One member: dimensions, a double arena, double tables and the current
+/* The oracle's arithmetic type: double for the FP64 oracle (liboracle.so);
+ * liboracle32.so builds the same source with real = float and
+ * single-precision constants, a sequential FP32 emulation of the
+ * reference's step used to size parity tolerances. */
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+typedef ORACLE_REAL real;
+
+/* One member: dimensions, a real arena, real tables and the current
  * phase's constants. slot_offsets / table_offsets / constants are indexed
  * by the position of each name in oracle_slot_names() /
  * oracle_table_names() / oracle_constant_names(); the caller resolves the
  * names against its own layout. */
 typedef struct {
   int ntru, nlat, nlon, nlev;
-  double* arena;
-  const double* tables;
-  const double* constants;
+  real* arena;
+  const real* tables;
+  const real* constants;
   const long* slot_offsets;
   const long* table_offsets;
 } oracle_member;
@@ -30,8 +39,8 @@ void oracle_step(const oracle_member* m, int with_physics);
 /* gridpointd's parameterizations alone (fixer .. carbon). */
 void oracle_physics(const oracle_member* m);
 /* Set-up transforms over `planes` planes (qi or qj when dmu). */
-void oracle_transform_inverse(const oracle_member* m, const double* sp, double* grid, int planes, int dmu);
-void oracle_transform_forward(const oracle_member* m, const double* grid, double* sp, int planes);
+void oracle_transform_inverse(const oracle_member* m, const real* sp, real* grid, int planes, int dmu);
+void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp, int planes);
 
 #ifdef EXO_ORACLE_INTERNAL
 #define SLOTS(X)                                                             \
@@ -142,9 +151,9 @@ static const char* const oracle_const_names_[] = {
 
 typedef struct {
   int T, NTP1, NLAT, NLON, NLEV, NHOR, NCSP, NRSP, NFC;
-  double* a;
-  const double* t;
-  const double* k;
+  real* a;
+  const real* t;
+  const real* k;
   const long* so;
   const long* to;
 } M;

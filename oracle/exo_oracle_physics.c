@@ -1,13 +1,13 @@
 /*
  * FP64 CPU oracle of gridpointd's parameterizations: TEST INFRASTRUCTURE
  * ONLY (see exo_oracle.c). Each routine restates one reference WGSL kernel
- * (file:line given) in double precision, in the reference's dispatch order
+ * (file:line given) in real precision, in the reference's dispatch order
  * (stepper/index.ts:148-161): fixer, fluxstep, radstep, rainstep,
  * surfstep, carbonstep. The reference's f32 "precise_*" helpers
  * (shared/gpu/math/wgsl/precise.wgsl) are libm's log/exp/pow here.
  */
 #define EXO_ORACLE_INTERNAL
-#include <math.h>
+#include <tgmath.h>
 #include <stdlib.h>
 
 #include "exo_oracle.h"
@@ -17,13 +17,13 @@
 #define K(s) KK(*v, s)
 #define LV(s, l, h) (F(s)[(long)(l) * NH + (h)])
 
-static double minf_(double a, double b) { return a < b ? a : b; }
-static double maxf_(double a, double b) { return a > b ? a : b; }
+static real minf_(real a, real b) { return a < b ? a : b; }
+static real maxf_(real a, real b) { return a > b ? a : b; }
 #define MIN(a, b) minf_((a), (b))
 #define MAX(a, b) maxf_((a), (b))
 
-static double powi(double x, int n) {
-  double r = 1;
+static real powi(real x, int n) {
+  real r = 1;
   for (int i = 0; i < n; i++) r *= x;
   return r;
 }
@@ -34,15 +34,15 @@ static double powi(double x, int n) {
 static void fixer_column(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
   for (int h = 0; h < NH; h++) {
-    double neg = 0, pos = 0;
+    real neg = 0, pos = 0;
     for (int l = 0; l < L; l++) {
-      const double q = LV(dq, l, h) * T(dsigma)[l];
+      const real q = LV(dq, l, h) * T(dsigma)[l];
       if (LV(dq, l, h) < 0) neg += q;
       if (LV(dq, l, h) > 0) pos += q;
     }
-    double zneg = -neg * F(dp)[h] * T(zgw)[h];
-    double zpos = pos * F(dp)[h] * T(zgw)[h];
-    double factor = 1;
+    real zneg = -neg * F(dp)[h] * T(zgw)[h];
+    real zpos = pos * F(dp)[h] * T(zgw)[h];
+    real factor = 1;
     const int fixed = zneg > 0 && zpos >= zneg;
     if (fixed) {
       factor = (zpos - zneg) / zpos;
@@ -50,7 +50,7 @@ static void fixer_column(const M* v) {
       zpos = zpos * factor;
     }
     for (int l = 0; l < L; l++) {
-      const double q = LV(dq, l, h);
+      const real q = LV(dq, l, h);
       LV(fixq, l, h) = fixed ? MAX(q * factor, 0.0) : q;
     }
     F(fixneg)[h] = zneg;
@@ -63,13 +63,13 @@ static void fixer_row(const M* v) {
   const int NH = v->NHOR, L = v->NLEV, NLON = v->NLON;
   for (int row = 0; row < v->NLAT; row++) {
     const int first = row * NLON;
-    double zneg = 0;
+    real zneg = 0;
     for (int j = 0; j < NLON; j++) zneg += F(fixneg)[first + j];
     if (zneg <= 0) continue;
-    double zpos = 0;
+    real zpos = 0;
     for (int j = 0; j < NLON; j++) zpos += F(fixpos)[first + j];
     if (zpos < zneg) continue;
-    const double factor = (zpos - zneg) / zpos;
+    const real factor = (zpos - zneg) / zpos;
     for (int j = 0; j < NLON; j++) {
       for (int l = 0; l < L; l++) LV(fixq, l, first + j) = MAX(LV(fixq, l, first + j) * factor, 0.0);
       F(fixneg)[first + j] = 0;
@@ -79,8 +79,8 @@ static void fixer_row(const M* v) {
 }
 
 /* reduction/wgsl/sum.wgsl:12-34 (exact-order sum in FP64) */
-static void global_sum(const M* v, const double* a, const double* b, double* out) {
-  double s1 = 0, s2 = 0;
+static void global_sum(const M* v, const real* a, const real* b, real* out) {
+  real s1 = 0, s2 = 0;
   for (int h = 0; h < v->NHOR; h++) {
     s1 += a[h];
     s2 += b[h];
@@ -92,10 +92,10 @@ static void global_sum(const M* v, const double* a, const double* b, double* out
 /* physics/fixer/wgsl/apply.wgsl:4-23 */
 static void fixer_apply(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double zzneg = F(fixsum)[0], zzpos = F(fixsum)[1];
+  const real zzneg = F(fixsum)[0], zzpos = F(fixsum)[1];
   for (int h = 0; h < NH; h++)
     for (int l = 0; l < L; l++) {
-      double q = LV(fixq, l, h);
+      real q = LV(fixq, l, h);
       if (zzneg > 0) q = MAX(q * ((zzpos - zzneg) / zzpos), 0.0);
       LV(gqdt, l, h) = LV(gqdt, l, h) + F(dp)[h] * (q - LV(dq, l, h)) / K(delt2) / K(psurf);
       LV(dq, l, h) = q;
@@ -107,86 +107,86 @@ static void fixer_apply(const M* v) {
 /* physics/surface-flux/wgsl/surflx.wgsl:30-128 */
 static void surflx(const M* v) {
   const int NH = v->NHOR, L = v->NLEV, low = L - 1;
-  const double VK = 0.4, GA = K(ga), GASCON = K(gascon), DELTSEC2 = K(deltsec2);
-  const double moisture = 1.0 / K(rdbrv) - 1.0;
+  const real VK = 0.4, GA = K(ga), GASCON = K(gascon), DELTSEC2 = K(deltsec2);
+  const real moisture = 1.0 / K(rdbrv) - 1.0;
   for (int h = 0; h < NH; h++) {
-    const double u = LV(du, low, h), vv = LV(dv, low, h), t = LV(dt, low, h), q = LV(dq, low, h);
-    const double ts = LV(dt, L, h), qs = LV(dq, L, h);
-    const double dp = F(dp)[h], dls = F(dls)[h], dsigma = T(dsigma)[low];
-    const double zfac = T(flux_sigfac)[0], zlnsig = T(flux_lnsig)[0];
-    double dtsa = t * zfac * (1.0 + moisture * q);
-    const double zabsu2 = MAX(K(zumin), u * u + vv * vv);
-    const double znl = -GASCON * 0.5 * (t + dtsa) * zlnsig / GA;
-    const double zbz0 = znl / F(dz0)[h];
-    const double zri = GA * znl / (zabsu2 * dtsa) * (dtsa - ts * (1.0 + moisture * qs));
-    const double zkblnz = VK / log(zbz0 + 1.0);
-    const double zkblnz2 = zkblnz * zkblnz;
-    double zrifm, zrifh;
+    const real u = LV(du, low, h), vv = LV(dv, low, h), t = LV(dt, low, h), q = LV(dq, low, h);
+    const real ts = LV(dt, L, h), qs = LV(dq, L, h);
+    const real dp = F(dp)[h], dls = F(dls)[h], dsigma = T(dsigma)[low];
+    const real zfac = T(flux_sigfac)[0], zlnsig = T(flux_lnsig)[0];
+    real dtsa = t * zfac * (1.0 + moisture * q);
+    const real zabsu2 = MAX(K(zumin), u * u + vv * vv);
+    const real znl = -GASCON * 0.5 * (t + dtsa) * zlnsig / GA;
+    const real zbz0 = znl / F(dz0)[h];
+    const real zri = GA * znl / (zabsu2 * dtsa) * (dtsa - ts * (1.0 + moisture * qs));
+    const real zkblnz = VK / log(zbz0 + 1.0);
+    const real zkblnz2 = zkblnz * zkblnz;
+    real zrifm, zrifh;
     if (zri <= 0) {
-      const double zdenom = 1.0 + 3.0 * K(vdiff_c) * K(vdiff_b) * sqrt(-1.0 * zri * (zbz0 + 1.0)) * zkblnz2;
+      const real zdenom = 1.0 + 3.0 * K(vdiff_c) * K(vdiff_b) * sqrt(-1.0 * zri * (zbz0 + 1.0)) * zkblnz2;
       zrifm = 1.0 - 2.0 * K(vdiff_b) * zri / zdenom;
       if (dls < 1.0) {
-        const double zdth = -(dtsa - ts * (1.0 + moisture * qs));
+        const real zdth = -(dtsa - ts * (1.0 + moisture * qs));
         zrifh = pow(1.0 + pow(0.0016 * pow(zdth, 1.0 / 3.0) / sqrt(zabsu2) / zkblnz2, 1.25), 0.8);
       } else {
         zrifh = 1.0 - 3.0 * K(vdiff_b) * zri / zdenom;
       }
     } else {
-      const double zdenom = sqrt(1.0 + K(vdiff_d) * zri);
+      const real zdenom = sqrt(1.0 + K(vdiff_d) * zri);
       zrifh = 1.0 / (1.0 + 3.0 * K(vdiff_b) * zri * zdenom);
       zrifm = 1.0 / (1.0 + 2.0 * K(vdiff_b) * zri / zdenom);
     }
-    const double ztrans = sqrt(zabsu2) * zkblnz2;
-    const double transh = ztrans * zrifh, transm = ztrans * zrifm;
+    const real ztrans = sqrt(zabsu2) * zkblnz2;
+    const real transh = ztrans * zrifh, transm = ztrans * zrifm;
     F(dtransh)[h] = transh;
     F(dtransm)[h] = transm;
-    const double zkonst1 = GA * DELTSEC2 / (GASCON * dsigma);
-    const double zkonst2 = dsigma / DELTSEC2 / GA;
-    const double zkdiffm = zkonst1 * transm / ts;
-    const double zun = u / (1.0 + zkdiffm), zvn = vv / (1.0 + zkdiffm);
+    const real zkonst1 = GA * DELTSEC2 / (GASCON * dsigma);
+    const real zkonst2 = dsigma / DELTSEC2 / GA;
+    const real zkdiffm = zkonst1 * transm / ts;
+    const real zun = u / (1.0 + zkdiffm), zvn = vv / (1.0 + zkdiffm);
     LV(dudt, low, h) = LV(dudt, low, h) + (zun - u) / DELTSEC2;
     LV(dvdt, low, h) = LV(dvdt, low, h) + (zvn - vv) / DELTSEC2;
-    double dtdt = LV(dtdt, low, h);
+    real dtdt = LV(dtdt, low, h);
     if (K(ndheat) != 0)
       dtdt = dtdt + -(zun * zun - u * u + zvn * zvn - vv * vv) / DELTSEC2 * 0.5 / K(acpd) / (1.0 + K(adv) * q);
-    const double taux = dp * zkonst2 * zkdiffm * zun, tauy = dp * zkonst2 * zkdiffm * zvn;
+    const real taux = dp * zkonst2 * zkdiffm * zun, tauy = dp * zkonst2 * zkdiffm * zvn;
     F(dtaux)[h] = taux;
     F(dtauy)[h] = tauy;
-    const double ustar = sqrt(sqrt(taux * taux + tauy * tauy) * GASCON * ts / dp);
+    const real ustar = sqrt(sqrt(taux * taux + tauy * tauy) * GASCON * ts / dp);
     F(dust3)[h] = ustar * ustar * ustar;
-    const double zkdiffh = zkonst1 * transh / ts;
-    const double ztn = (t + zkdiffh * ts) / (1.0 + zkdiffh * zfac);
+    const real zkdiffh = zkonst1 * transh / ts;
+    const real ztn = (t + zkdiffh * ts) / (1.0 + zkdiffh * zfac);
     dtdt = dtdt + (ztn - t) / DELTSEC2;
     LV(dtdt, low, h) = dtdt;
-    const double moist = MAX(1.0, 1.0 + K(adv) * q);
+    const real moist = MAX(1.0, 1.0 + K(adv) * q);
     F(dshfl)[h] = zkonst2 * zkdiffh * dp * K(acpd) * moist * (ztn * zfac - ts);
     F(dshdt)[h] = -1.0 * zkonst2 * zkdiffh * dp * K(acpd) * moist;
     dtsa = ztn * zfac;
     F(dtsa)[h] = dtsa;
-    const double zkdiffq = F(drhs)[h] * zkonst1 * transh / ts;
-    double zqn = MAX(q, (q + zkdiffq * qs) / (1.0 + zkdiffq));
+    const real zkdiffq = F(drhs)[h] * zkonst1 * transh / ts;
+    real zqn = MAX(q, (q + zkdiffq * qs) / (1.0 + zkdiffq));
     if (dls > 0) zqn = MIN(zqn, F(dwatc)[h] / K(deltsec) * 1000.0 / dp / zkonst2 + q);
     LV(dqdt, low, h) = LV(dqdt, low, h) + (zqn - q) / DELTSEC2;
-    const double devap = -dp * zkonst2 / 1000.0 * (zqn - q);
+    const real devap = -dp * zkonst2 / 1000.0 * (zqn - q);
     F(devap)[h] = devap;
-    const double latent = (ts > K(tmelt) || dls < 0.5) ? K(alv) : K(als);
-    const double dlhfl = devap * latent * 1000.0;
+    const real latent = (ts > K(tmelt) || dls < 0.5) ? K(alv) : K(als);
+    const real dlhfl = devap * latent * 1000.0;
     F(dlhfl)[h] = dlhfl;
-    const double curvature = (ts - K(ra4)) * (ts - K(ra4));
-    const double dlhdt = -1.0 * latent * zkdiffq * zkonst2 * dp * K(ra2) * (K(tmelt) - K(ra4)) * qs / curvature;
+    const real curvature = (ts - K(ra4)) * (ts - K(ra4));
+    const real dlhdt = -1.0 * latent * zkdiffq * zkonst2 * dp * K(ra2) * (K(tmelt) - K(ra4)) * qs / curvature;
     F(dlhdt)[h] = dlhfl == 0.0 ? 0.0 : dlhdt;
   }
 }
 
 /* physics/diffusion/wgsl/implicit.wgsl:4-26 */
-static void diffuse(const M* v, double* x, const double* k) {
+static void diffuse(const M* v, real* x, const real* k) {
   const int L = v->NLEV;
-  double ebs[16];
-  const double* ds = T(dsigma);
+  real ebs[16];
+  const real* ds = T(dsigma);
   ebs[0] = k[0] / (ds[0] + k[0]);
   x[0] = ds[0] * x[0] / (ds[0] + k[0]);
   for (int l = 1; l < L - 1; l++) {
-    const double den = ds[l] + k[l] + k[l - 1] * (1.0 - ebs[l - 1]);
+    const real den = ds[l] + k[l] + k[l - 1] * (1.0 - ebs[l - 1]);
     ebs[l] = k[l] / den;
     x[l] = (x[l] * ds[l] + k[l - 1] * x[l - 1]) / den;
   }
@@ -195,14 +195,14 @@ static void diffuse(const M* v, double* x, const double* k) {
 }
 
 /* physics/diffusion/wgsl/implicit.wgsl:28-47 */
-static void diffuse_potential(const M* v, double* x, const double* k, const double* skap) {
+static void diffuse_potential(const M* v, real* x, const real* k, const real* skap) {
   const int L = v->NLEV;
-  double ebs[16];
-  const double* ds = T(dsigma);
+  real ebs[16];
+  const real* ds = T(dsigma);
   ebs[0] = k[0] / (ds[0] + k[0] / skap[0]);
   x[0] = ds[0] * x[0] / (ds[0] + k[0] / skap[0]);
   for (int l = 1; l < L - 1; l++) {
-    const double den = ds[l] + (k[l] + k[l - 1] * (1.0 - ebs[l - 1] / skap[l - 1])) / skap[l];
+    const real den = ds[l] + (k[l] + k[l - 1] * (1.0 - ebs[l - 1] / skap[l - 1])) / skap[l];
     ebs[l] = k[l] / den;
     x[l] = (x[l] * ds[l] + k[l - 1] / skap[l - 1] * x[l - 1]) / den;
   }
@@ -215,10 +215,10 @@ static void diffuse_potential(const M* v, double* x, const double* k, const doub
 /* physics/surface-flux/wgsl/vdiff.wgsl:22-120 */
 static void vdiff(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double DELTSEC2 = K(deltsec2), GA = K(ga), GASCON = K(gascon);
-  const double moisture = 1.0 / K(rdbrv) - 1.0;
-  double zt[16], zq[16], zu[16], zv[16], zabsu[16], zskap[16], zskaph[16], kh[16], km[16];
-  double zun[16], zvn[16], zke[16], zken[16], zqn[16], ztn[16];
+  const real DELTSEC2 = K(deltsec2), GA = K(ga), GASCON = K(gascon);
+  const real moisture = 1.0 / K(rdbrv) - 1.0;
+  real zt[16], zq[16], zu[16], zv[16], zabsu[16], zskap[16], zskaph[16], kh[16], km[16];
+  real zun[16], zvn[16], zke[16], zken[16], zqn[16], ztn[16];
   for (int h = 0; h < NH; h++) {
     for (int l = 0; l < L; l++) {
       zt[l] = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
@@ -229,28 +229,28 @@ static void vdiff(const M* v) {
       zskap[l] = T(vdiff_skap)[l];
       zskaph[l] = T(vdiff_skaph)[l];
     }
-    const double zkonst1 = GA * DELTSEC2 / GASCON;
-    const double zkonst2 = zkonst1 * GA / GASCON;
-    const double zkonst3 = zkonst2 * GA / GASCON;
+    const real zkonst1 = GA * DELTSEC2 / GASCON;
+    const real zkonst2 = zkonst1 * GA / GASCON;
+    const real zkonst3 = zkonst2 * GA / GASCON;
     for (int l = 0; l < L - 1; l++) {
       const int lp = l + 1;
-      const double sigmah = T(sigmah)[l], dsigma = T(dsigma)[l], dsigmap = T(dsigma)[lp];
-      const double zmixh2 = T(vdiff_mixh2)[l], zmixm2 = T(vdiff_mixm2)[l], zrdsig = T(vdiff_rdsig)[l];
-      const double zfac = zkonst3 * T(vdiff_sigmah3)[l];
-      const double zzkfac = T(vdiff_kfac)[l];
-      const double zrthl = (dsigmap + dsigma) / (zt[l] * dsigmap + zt[lp] * dsigma);
-      const double zdvds = MAX(K(zumin), fabs(zabsu[l] - zabsu[lp])) * zrdsig;
-      const double zdthds =
+      const real sigmah = T(sigmah)[l], dsigma = T(dsigma)[l], dsigmap = T(dsigma)[lp];
+      const real zmixh2 = T(vdiff_mixh2)[l], zmixm2 = T(vdiff_mixm2)[l], zrdsig = T(vdiff_rdsig)[l];
+      const real zfac = zkonst3 * T(vdiff_sigmah3)[l];
+      const real zzkfac = T(vdiff_kfac)[l];
+      const real zrthl = (dsigmap + dsigma) / (zt[l] * dsigmap + zt[lp] * dsigma);
+      const real zdvds = MAX(K(zumin), fabs(zabsu[l] - zabsu[lp])) * zrdsig;
+      const real zdthds =
           (zt[l] * (1.0 + moisture * zq[l]) / zskap[l] - zt[lp] * (1.0 + moisture * zq[lp]) / zskap[lp]) * zrdsig;
-      const double zri = zskaph[l] * GASCON * zdthds / (sigmah * zdvds * zdvds);
-      double zrifh, zrifm;
+      const real zri = zskaph[l] * GASCON * zdthds / (sigmah * zdvds * zdvds);
+      real zrifh, zrifm;
       if (zri <= 0) {
-        const double zdh = 1.0 + 3.0 * K(vdiff_c) * K(vdiff_b) * sqrt(-1.0 * zri) * zzkfac * zmixh2;
-        const double zdm = 1.0 + 3.0 * K(vdiff_c) * K(vdiff_b) * sqrt(-1.0 * zri) * zzkfac * zmixm2;
+        const real zdh = 1.0 + 3.0 * K(vdiff_c) * K(vdiff_b) * sqrt(-1.0 * zri) * zzkfac * zmixh2;
+        const real zdm = 1.0 + 3.0 * K(vdiff_c) * K(vdiff_b) * sqrt(-1.0 * zri) * zzkfac * zmixm2;
         zrifh = 1.0 - 3.0 * K(vdiff_b) * zri / zdh;
         zrifm = 1.0 - 2.0 * K(vdiff_b) * zri / zdm;
       } else {
-        const double zdenom = sqrt(1.0 + K(vdiff_d) * zri);
+        const real zdenom = sqrt(1.0 + K(vdiff_d) * zri);
         zrifh = 1.0 / (1.0 + 3.0 * K(vdiff_b) * zri * zdenom);
         zrifm = 1.0 / (1.0 + 2.0 * K(vdiff_b) * zri / zdenom);
       }
@@ -271,7 +271,7 @@ static void vdiff(const M* v) {
       for (int l = 0; l < L; l++) zke[l] = zken[l] = 0.5 * zabsu[l] * zabsu[l];
       diffuse(v, zken, km);
       for (int l = 0; l < L; l++) {
-        const double zdtdt = -((zun[l] * zun[l] - zu[l] * zu[l] + zvn[l] * zvn[l] - zv[l] * zv[l]) / DELTSEC2 -
+        const real zdtdt = -((zun[l] * zun[l] - zu[l] * zu[l] + zvn[l] * zvn[l] - zv[l] * zv[l]) / DELTSEC2 -
                                (zken[l] - zke[l]) / DELTSEC2) *
                              0.5 / K(acpd) / (1.0 + K(adv) * LV(dq, l, h));
         LV(dtdt, l, h) = LV(dtdt, l, h) + zdtdt;
@@ -291,16 +291,16 @@ static void vdiff(const M* v) {
 
 /* physics/radiation/wgsl/solang.wgsl:10-32 */
 static void solang(const M* v) {
-  const double PI = 3.14159265359, TWOPI = PI + PI;
-  const double ZRLON = TWOPI / v->NLON;
-  const double* rs = F(radiation_step);
+  const real PI = 3.14159265359, TWOPI = PI + PI;
+  const real ZRLON = TWOPI / v->NLON;
+  const real* rs = F(radiation_step);
   for (int h = 0; h < v->NHOR; h++) {
     const int lat = h / v->NLON, lon = h % v->NLON;
-    double hangle = rs[2] + lon * ZRLON - PI;
+    real hangle = rs[2] + lon * ZRLON - PI;
     if (hangle < -PI) hangle = hangle + TWOPI;
     if (hangle > PI) hangle = hangle - TWOPI;
     if (K(rad_synchronous) != 0) hangle = hangle + PI;
-    const double muz = rs[0] * T(sid)[lat] + T(cola)[lat] * rs[1] * cos(hangle);
+    const real muz = rs[0] * T(sid)[lat] + T(cola)[lat] * rs[1] * cos(hangle);
     F(gmu0)[h] = muz > K(zdawn) ? muz : 0.0;
   }
 }
@@ -308,15 +308,15 @@ static void solang(const M* v) {
 /* physics/radiation/wgsl/mko3.wgsl:15-37 */
 static void mko3(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double ZFO3 = 100.0 / 2.14, GA = K(ga), GASCON = K(gascon);
+  const real ZFO3 = 100.0 / 2.14, GA = K(ga), GASCON = K(gascon);
   for (int h = 0; h < NH; h++) {
-    const double sid = T(sid)[h / v->NLON];
-    const double za = K(a0o3) + K(a1o3) * fabs(sid) + K(aco3) * sid * F(radiation_step)[4];
-    const double dp = F(dp)[h];
-    double o3t = za, zh = 0;
+    const real sid = T(sid)[h / v->NLON];
+    const real za = K(a0o3) + K(a1o3) * fabs(sid) + K(aco3) * sid * F(radiation_step)[4];
+    const real dp = F(dp)[h];
+    real o3t = za, zh = 0;
     for (int l = L - 1; l >= 1; l--) {
       zh = zh - LV(dt, l, h) * GASCON / GA * T(rad_lnsig)[l];
-      const double o3 = -(za + za * K(zconst_o3)) / (1.0 + exp((zh - K(bo3)) / K(co3))) + o3t;
+      const real o3 = -(za + za * K(zconst_o3)) / (1.0 + exp((zh - K(bo3)) / K(co3))) + o3t;
       LV(dqo3, l, h) = o3 * GA / (ZFO3 * T(dsigma)[l] * dp);
       o3t = o3t - o3;
     }
@@ -324,30 +324,30 @@ static void mko3(const M* v) {
   }
 }
 
-static double ozone_absorption(const M* v, double o3) {
+static real ozone_absorption(const M* v, real o3) {
   return (0.02118 * o3 / (1.0 + 0.042 * o3 + 0.000323 * (o3 * o3)) + 1.082 * o3 / pow(1.0 + 138.6 * o3, 0.805) +
           0.0658 * o3 / (1.0 + (103.6 * o3) * (103.6 * o3) * (103.6 * o3))) /
          K(zsolar1);
 }
 
-static double vapour_absorption(const M* v, double wv) {
+static real vapour_absorption(const M* v, real wv) {
   return 2.9 * wv / (pow(1.0 + 141.5 * wv, 0.635) + 5.925 * wv) / K(zsolar2);
 }
 
 /* physics/radiation/wgsl/swr.wgsl:45-276 */
 static void swr(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double ZERO = 1.0e-6, ZBETTA = 1.66, ZMBAR = 1.9, ZFO3 = 100.0 / 2.14, ZMU00 = 0.5;
-  const double INV_LN10 = 0.43429448190325176, GA = K(ga);
-  double rcl1[16], rcl1s[16], rcl2[16], rcl2s[16], tcl2[16], tcl2s[16];
-  double o3l[16], xo3l[16], wvl[16], ywvl[16];
-  double t1[17], t2[17], r1s[17], r2s[17], tb1[16], tb1u[16], rb1[16], rb1s[16];
-  double tb2[16], tb2u[16], rb2[16], rb2s[16], rl1[17], rl2[17], rl1s[17], rl2s[17];
+  const real ZERO = 1.0e-6, ZBETTA = 1.66, ZMBAR = 1.9, ZFO3 = 100.0 / 2.14, ZMU00 = 0.5;
+  const real INV_LN10 = 0.43429448190325176, GA = K(ga);
+  real rcl1[16], rcl1s[16], rcl2[16], rcl2s[16], tcl2[16], tcl2s[16];
+  real o3l[16], xo3l[16], wvl[16], ywvl[16];
+  real t1[17], t2[17], r1s[17], r2s[17], tb1[16], tb1u[16], rb1[16], rb1s[16];
+  real tb2[16], tb2u[16], rb2[16], rb2s[16], rl1[17], rl2[17], rl1s[17], rl2s[17];
   for (int h = 0; h < NH; h++) {
-    const double mu0 = F(gmu0)[h];
-    const double gdist2 = F(radiation_step)[3];
-    const double ftop1 = K(zsolar1) * K(gsol0) * gdist2 * mu0;
-    const double ftop2 = K(zsolar2) * K(gsol0) * gdist2 * mu0;
+    const real mu0 = F(gmu0)[h];
+    const real gdist2 = F(radiation_step)[3];
+    const real ftop1 = K(zsolar1) * K(gsol0) * gdist2 * mu0;
+    const real ftop2 = K(zsolar2) * K(gsol0) * gdist2 * mu0;
     if (!(ftop1 + ftop2 > ZERO)) {
       for (int l = 0; l <= L; l++) {
         LV(dfu, l, h) = 0;
@@ -356,27 +356,27 @@ static void swr(const M* v) {
       }
       continue;
     }
-    const double dp = F(dp)[h];
-    double cs = 1.0;
+    const real dp = F(dp)[h];
+    real cs = 1.0;
     for (int l = 0; l < L; l++) {
-      const double cc = LV(dcc, l, h);
+      const real cc = LV(dcc, l, h);
       rcl1[l] = rcl1s[l] = rcl2[l] = rcl2s[l] = 0;
       tcl2[l] = tcl2s[l] = 1;
       if (cc > 0) {
-        const double lwp = MIN(1000.0, 1000.0 * LV(dql, l, h) * dp / GA * T(dsigma)[l]);
-        const double tau = 2.0 * pow(log(lwp + 1.5) * INV_LN10, 3.9);
-        const double zlog = log(1000.0 / tau);
-        double b2 = K(zb4) / log(3.0 + 0.1 * tau);
-        double om0 = MIN(0.9999, 1.0 - K(zb5) * zlog);
-        double un = 1.0 - om0;
-        double uz = un + 2.0 * b2 * om0;
-        double u = sqrt(uz / un);
-        double ex = exp(MIN(25.0, tau * sqrt(uz * un) / ZMU00));
-        double r = (u + 1.0) * (u + 1.0) * ex - (u - 1.0) * (u - 1.0) / ex;
+        const real lwp = MIN(1000.0, 1000.0 * LV(dql, l, h) * dp / GA * T(dsigma)[l]);
+        const real tau = 2.0 * pow(log(lwp + 1.5) * INV_LN10, 3.9);
+        const real zlog = log(1000.0 / tau);
+        real b2 = K(zb4) / log(3.0 + 0.1 * tau);
+        real om0 = MIN(0.9999, 1.0 - K(zb5) * zlog);
+        real un = 1.0 - om0;
+        real uz = un + 2.0 * b2 * om0;
+        real u = sqrt(uz / un);
+        real ex = exp(MIN(25.0, tau * sqrt(uz * un) / ZMU00));
+        real r = (u + 1.0) * (u + 1.0) * ex - (u - 1.0) * (u - 1.0) / ex;
         rcl1s[l] = 1.0 - 1.0 / (1.0 + K(zb3) * tau);
         tcl2s[l] = 4.0 * u / r;
         rcl2s[l] = (u * u - 1.0) / r * (ex - 1.0 / ex);
-        const double b1 = K(tswr1) * sqrt(mu0);
+        const real b1 = K(tswr1) * sqrt(mu0);
         b2 = K(tswr2) * sqrt(mu0) / log(3.0 + 0.1 * tau);
         om0 = MIN(0.9999, 1.0 - K(tswr3) * mu0 * mu0 * zlog);
         un = 1.0 - om0;
@@ -384,45 +384,45 @@ static void swr(const M* v) {
         u = sqrt(uz / un);
         ex = exp(MIN(25.0, tau * sqrt(uz * un) / mu0));
         r = (u + 1.0) * (u + 1.0) * ex - (u - 1.0) * (u - 1.0) / ex;
-        const double direct1 = 1.0 - 1.0 / (1.0 + b1 * tau / mu0);
-        const double directT = 4.0 * u / r;
-        const double direct2 = (u * u - 1.0) / r * (ex - 1.0 / ex);
+        const real direct1 = 1.0 - 1.0 / (1.0 + b1 * tau / mu0);
+        const real directT = 4.0 * u / r;
+        const real direct2 = (u * u - 1.0) / r * (ex - 1.0 / ex);
         rcl1[l] = cs * direct1 + (1.0 - cs) * rcl1s[l];
         tcl2[l] = cs * directT + (1.0 - cs) * tcl2s[l];
         rcl2[l] = cs * direct2 + (1.0 - cs) * rcl2s[l];
       }
       cs = cs * (1.0 - cc);
     }
-    const double zm = 35.0 / sqrt(1.0 + 1224.0 * mu0 * mu0);
-    double o3t = 0, xo3t = 0, wvt = 0, ywvt = 0;
+    const real zm = 35.0 / sqrt(1.0 + 1224.0 * mu0 * mu0);
+    real o3t = 0, xo3t = 0, wvt = 0, ywvt = 0;
     cs = 1.0;
     for (int l = 0; l < L; l++) {
-      const double dsg = T(dsigma)[l];
-      const double o3 = ZFO3 * dsg * dp * LV(dqo3, l, h) / GA;
+      const real dsg = T(dsigma)[l];
+      const real o3 = ZFO3 * dsg * dp * LV(dqo3, l, h) / GA;
       o3t = o3t + o3;
       xo3t = cs * (xo3t + zm * o3) + (1.0 - cs) * (xo3t + ZMBAR * o3);
       o3l[l] = o3t;
       xo3l[l] = xo3t;
-      const double wv = 0.1 * dsg * LV(dq, l, h) * dp / GA * sqrt(273.0 / LV(dt, l, h)) * T(sigma)[l] * dp / 100000.0;
+      const real wv = 0.1 * dsg * LV(dq, l, h) * dp / GA * sqrt(273.0 / LV(dt, l, h)) * T(sigma)[l] * dp / 100000.0;
       wvt = wvt + wv;
       ywvt = cs * (ywvt + zm * wv) + (1.0 - cs) * (ywvt + ZBETTA * wv);
       wvl[l] = wvt;
       ywvl[l] = ywvt;
       cs = cs * (1.0 - LV(dcc, l, h));
     }
-    double ta1 = 1, ta1s = 1, ra1 = 0, ra1s = 0, ta2 = 1, ta2s = 1, ra2 = 0, ra2s = 0;
-    double to3t = 1.0;
-    double to3tu = 1.0 - ozone_absorption(v, xo3t + ZMBAR * o3t);
-    double twvt = 1.0;
-    double twvtu = 1.0 - vapour_absorption(v, ywvt + ZBETTA * wvt);
-    const double ccn = LV(dcc, L - 1, h);
-    const double scf = K(rcoeff) * dp / 101100.0 * 9.80665 / GA;
-    const double diffuseRayleigh = 1.0 - exp(scf * K(ln_diffuse_rayleigh));
-    const double rcsuSurface = diffuseRayleigh * (1.0 - ccn);
-    const double rcsSurface =
+    real ta1 = 1, ta1s = 1, ra1 = 0, ra1s = 0, ta2 = 1, ta2s = 1, ra2 = 0, ra2s = 0;
+    real to3t = 1.0;
+    real to3tu = 1.0 - ozone_absorption(v, xo3t + ZMBAR * o3t);
+    real twvt = 1.0;
+    real twvtu = 1.0 - vapour_absorption(v, ywvt + ZBETTA * wvt);
+    const real ccn = LV(dcc, L - 1, h);
+    const real scf = K(rcoeff) * dp / 101100.0 * 9.80665 / GA;
+    const real diffuseRayleigh = 1.0 - exp(scf * K(ln_diffuse_rayleigh));
+    const real rcsuSurface = diffuseRayleigh * (1.0 - ccn);
+    const real rcsSurface =
         (1.0 - exp(scf * log(1.0 - 0.219 / (1.0 + 0.816 * mu0)))) * cs + diffuseRayleigh * (1.0 - cs - ccn);
     for (int l = 0; l < L; l++) {
-      const double cc = LV(dcc, l, h);
+      const real cc = LV(dcc, l, h);
       t1[l] = ta1;
       t2[l] = ta2;
       r1s[l] = ra1s;
@@ -430,30 +430,30 @@ static void swr(const M* v) {
       const int surface = l == L - 1;
       rb1[l] = (surface ? rcsSurface : 0.0) + rcl1[l] * cc;
       rb1s[l] = (surface ? rcsuSurface : 0.0) + rcl1s[l] * cc;
-      double o3 = xo3l[l];
-      const double to3 = (1.0 - ozone_absorption(v, o3)) / to3t;
+      real o3 = xo3l[l];
+      const real to3 = (1.0 - ozone_absorption(v, o3)) / to3t;
       to3t = to3t * to3;
       o3 = xo3t + ZMBAR * (o3t - o3l[l]);
-      const double to3u = to3tu / (1.0 - ozone_absorption(v, o3));
+      const real to3u = to3tu / (1.0 - ozone_absorption(v, o3));
       to3tu = to3tu / to3u;
       tb1[l] = 1.0 - (1.0 - to3) * (1.0 - cc) - rb1[l];
       tb1u[l] = 1.0 - (1.0 - to3u) * (1.0 - cc) - rb1s[l];
-      const double m1 = 1.0 / (1.0 - ra1s * rb1s[l]);
+      const real m1 = 1.0 / (1.0 - ra1s * rb1s[l]);
       ra1 = ra1 + ta1 * rb1[l] * ta1s * m1;
       ta1 = ta1 * tb1[l] * m1;
       ra1s = rb1s[l] + tb1u[l] * ra1s * tb1[l] * m1;
       ta1s = tb1u[l] * ta1s * m1;
       rb2[l] = rcl2[l] * cc;
       rb2s[l] = rcl2s[l] * cc;
-      double wv = ywvl[l];
-      const double twv = (1.0 - vapour_absorption(v, wv)) / twvt;
+      real wv = ywvl[l];
+      const real twv = (1.0 - vapour_absorption(v, wv)) / twvt;
       twvt = twvt * twv;
       wv = ywvt + ZBETTA * (wvt - wvl[l]);
-      const double twvu = twvtu / (1.0 - vapour_absorption(v, wv));
+      const real twvu = twvtu / (1.0 - vapour_absorption(v, wv));
       twvtu = twvtu / twvu;
       tb2[l] = 1.0 - (1.0 - twv) * (1.0 - cc) - (1.0 - tcl2[l]) * cc;
       tb2u[l] = 1.0 - (1.0 - twvu) * (1.0 - cc) - (1.0 - tcl2s[l]) * cc;
-      const double m2 = 1.0 / (1.0 - ra2s * rb2s[l]);
+      const real m2 = 1.0 / (1.0 - ra2s * rb2s[l]);
       ra2 = ra2 + ta2 * rb2[l] * ta2s * m2;
       ta2 = ta2 * tb2[l] * m2;
       ra2s = rb2s[l] + tb2u[l] * ra2s * tb2[l] * m2;
@@ -463,17 +463,17 @@ static void swr(const M* v) {
     t2[L] = ta2;
     r1s[L] = ra1s;
     r2s[L] = ra2s;
-    const double dalb = F(dalb)[h], salb1 = F(dsalb)[h], salb2 = F(dsalb)[NH + h];
+    const real dalb = F(dalb)[h], salb1 = F(dsalb)[h], salb2 = F(dsalb)[NH + h];
     const int nst = K(nstartemp) != 0;
     ra1s = nst ? salb1 : dalb;
     ra2s = nst ? salb2 : dalb;
-    const double dls = F(dls)[h], dicec = F(dicec)[h];
-    const double water = (1.0 - dls) * (1.0 - dicec) * MIN(0.05 / (mu0 + 0.15), 0.15);
-    const double nalb1 = dls * salb1 + (1.0 - dls) * dicec * salb1 + water;
-    const double nalb2 = dls * salb2 + (1.0 - dls) * dicec * salb2 + water;
+    const real dls = F(dls)[h], dicec = F(dicec)[h];
+    const real water = (1.0 - dls) * (1.0 - dicec) * MIN(0.05 / (mu0 + 0.15), 0.15);
+    const real nalb1 = dls * salb1 + (1.0 - dls) * dicec * salb1 + water;
+    const real nalb2 = dls * salb2 + (1.0 - dls) * dicec * salb2 + water;
     F(dsalb)[h] = nalb1;
     F(dsalb)[NH + h] = nalb2;
-    const double albedo = nst ? K(zsolars1) * nalb1 + K(zsolars2) * nalb2 : dls * dalb + (1.0 - dls) * dicec * dalb + water;
+    const real albedo = nst ? K(zsolars1) * nalb1 + K(zsolars2) * nalb2 : dls * dalb + (1.0 - dls) * dicec * dalb + water;
     F(dalb)[h] = albedo;
     ra1 = nst ? nalb1 : albedo;
     ra2 = nst ? nalb2 : albedo;
@@ -482,10 +482,10 @@ static void swr(const M* v) {
       rl2[l + 1] = ra2;
       rl1s[l + 1] = ra1s;
       rl2s[l + 1] = ra2s;
-      const double d1 = 1.0 - ra1s * rb1s[l];
+      const real d1 = 1.0 - ra1s * rb1s[l];
       ra1 = rb1[l] + tb1[l] * ra1 * tb1u[l] / d1;
       ra1s = rb1s[l] + tb1u[l] * ra1s * tb1u[l] / d1;
-      const double d2 = 1.0 - ra2s * rb2s[l];
+      const real d2 = 1.0 - ra2s * rb2s[l];
       ra2 = rb2[l] + tb2[l] * ra2 * tb2u[l] / d2;
       ra2s = rb2s[l] + tb2u[l] * ra2s * tb2u[l] / d2;
     }
@@ -494,10 +494,10 @@ static void swr(const M* v) {
     rl1s[0] = ra1s;
     rl2s[0] = ra2s;
     for (int l = 0; l <= L; l++) {
-      const double m1 = 1.0 / (1.0 - r1s[l] * rl1s[l]);
-      const double m2 = 1.0 / (1.0 - r2s[l] * rl2s[l]);
-      const double fu = -t1[l] * rl1[l] * m1 * ftop1 + -t2[l] * rl2[l] * m2 * ftop2;
-      const double fd = t1[l] * m1 * ftop1 + t2[l] * m2 * ftop2;
+      const real m1 = 1.0 / (1.0 - r1s[l] * rl1s[l]);
+      const real m2 = 1.0 / (1.0 - r2s[l] * rl2s[l]);
+      const real fu = -t1[l] * rl1[l] * m1 * ftop1 + -t2[l] * rl2[l] * m2 * ftop2;
+      const real fd = t1[l] * m1 * ftop1 + t2[l] * m2 * ftop2;
       LV(dfu, l, h) = fu;
       LV(dfd, l, h) = fd;
       LV(dswfl, l, h) = fu + fd;
@@ -508,15 +508,15 @@ static void swr(const M* v) {
 /* physics/radiation/wgsl/lwr.wgsl:36-179 */
 static void lwr(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double ZERO = 1.0e-6, ZFH2O = 0.1, ZFO3 = 100.0 / 2.14;
-  double st4[17], st4h[17], bu[18], bd[18], q[16], qo3[16], qco2[16], taucc0[16];
-  double dftu[17], dftd[17], taucs[16], tau[16], tausf[16];
-  const double* sg = T(sigma);
-  const double* sh = T(sigmah);
+  const real ZERO = 1.0e-6, ZFH2O = 0.1, ZFO3 = 100.0 / 2.14;
+  real st4[17], st4h[17], bu[18], bd[18], q[16], qo3[16], qco2[16], taucc0[16];
+  real dftu[17], dftd[17], taucs[16], tau[16], tausf[16];
+  const real* sg = T(sigma);
+  const real* sh = T(sigmah);
   for (int h = 0; h < NH; h++) {
-    const double dp = F(dp)[h], ps2 = dp * dp;
+    const real dp = F(dp)[h], ps2 = dp * dp;
     for (int l = 0; l <= L; l++) {
-      const double t = LV(dt, l, h), t2 = t * t;
+      const real t = LV(dt, l, h), t2 = t * t;
       st4[l] = K(sbk) * (t2 * t2);
     }
     st4h[0] = st4[0] - (st4[0] - st4[1]) * sg[0] / (sg[0] - sg[1]);
@@ -528,11 +528,11 @@ static void lwr(const M* v) {
     else
       st4h[L] = st4[n] + (st4[n] - st4[n - 1]) * (1.0 - sg[n]) / (sg[n] - sg[n - 1]);
     bd[0] = 0;
-    const double dls = F(dls)[h];
-    const double eps = dls + 0.98 * (1.0 - dls);
+    const real dls = F(dls)[h];
+    const real eps = dls + 0.98 * (1.0 - dls);
     bu[L + 1] = eps * st4[L];
     for (int l = 0; l < L; l++) {
-      const double sfac = T(lwr_zzf1)[l] * ps2;
+      const real sfac = T(lwr_zzf1)[l] * ps2;
       q[l] = ZFH2O * sfac * LV(dq, l, h);
       qo3[l] = ZFO3 * sfac * LV(dqo3, l, h);
       qco2[l] = K(zfco2_zzf2) * sfac * LV(dqco2, l, h);
@@ -540,39 +540,39 @@ static void lwr(const M* v) {
     }
     for (int l = 0; l <= L; l++) dftu[l] = dftd[l] = 0;
     for (int j = 0; j < L; j++) {
-      double taucc = 1.0, sumwv = 0, sumo3 = 0, sumco2 = 0;
+      real taucc = 1.0, sumwv = 0, sumo3 = 0, sumco2 = 0;
       for (int k = j; k < L; k++) {
         sumwv = sumwv + q[k];
         sumo3 = sumo3 + qo3[k];
         sumco2 = sumco2 + qco2[k];
-        double ah2o;
+        real ah2o;
         if (sumwv <= 0.01)
           ah2o = 0.846 * pow(sumwv + 3.59e-5, 0.243) - K(zh2o0a);
         else
           ah2o = 0.24 * log10(sumwv + 0.01) + K(zah2oc);
         if (K(th2oc) > 0) ah2o = MIN(ah2o + (1.0 - exp(-K(th2oc) * sumwv)), 1.0);
-        double aco2;
+        real aco2;
         if (sumco2 <= 1.0)
           aco2 = 0.0676 * pow(sumco2 + 0.01022, 0.421) - K(zco20);
         else
           aco2 = 0.0546 * log10(sumco2) + K(zaco2c);
-        double th2o;
+        real th2o;
         if (sumwv <= 2.0)
           th2o = 1.0 - (0.832 * pow(sumwv + 0.0286, 0.26) - K(zh2o0));
         else
           th2o = MAX(0.0, K(zth2oc) - 0.1196 * log(sumwv - 0.6931));
-        double ao3;
+        real ao3;
         if (sumo3 <= 0.01)
           ao3 = 0.209 * pow(sumo3 + 7.0e-5, 0.436) - K(zao30);
         else
           ao3 = 0.0212 * log10(sumo3) + K(zao3c);
-        const double cs = MIN(1.0 - ZERO, MAX(ZERO, 1.0 - ah2o - ao3 - aco2 * th2o));
+        const real cs = MIN(1.0 - ZERO, MAX(ZERO, 1.0 - ah2o - ao3 - aco2 * th2o));
         taucs[k] = cs;
         taucc = taucc * taucc0[k];
         tau[k] = cs * taucc;
       }
       if (j == 0) {
-        double tau0 = 1.0 - ZERO;
+        real tau0 = 1.0 - ZERO;
         for (int k = 0; k < L; k++) {
           tau0 = taucs[k] / tau0 * taucc0[k] * K(tpofmt);
           tau0 = MIN(1.0 - ZERO, MAX(ZERO, tau0));
@@ -588,7 +588,7 @@ static void lwr(const M* v) {
       }
       dftu[j] = dftu[j] - bu[j + 1];
       dftd[j] = dftd[j] + bd[j];
-      const double bdl = bd[j] - bd[j + 1];
+      const real bdl = bd[j] - bd[j + 1];
       for (int k = j; k < L; k++) {
         dftu[j] = dftu[j] - (bu[k + 2] - bu[k + 1]) * tau[k];
         dftd[k + 1] = dftd[k + 1] + bdl * tau[k];
@@ -597,7 +597,7 @@ static void lwr(const M* v) {
     }
     dftu[L] = dftu[L] - bu[L + 1];
     dftd[L] = dftd[L] + bd[L];
-    const double reflected = (1.0 - eps) * dftd[L];
+    const real reflected = (1.0 - eps) * dftd[L];
     for (int l = 0; l < L; l++) dftu[l] = dftu[l] - tausf[l] * reflected;
     for (int l = 0; l <= L; l++) {
       LV(dftu, l, h) = dftu[l];
@@ -610,14 +610,14 @@ static void lwr(const M* v) {
 /* physics/radiation/wgsl/radstep.wgsl:11-32 */
 static void radstep(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double GA = K(ga), GASCON = K(gascon);
+  const real GA = K(ga), GASCON = K(gascon);
   for (int h = 0; h < NH; h++) {
     for (int l = 0; l <= L; l++) LV(dflux, l, h) = LV(dlwfl, l, h) + LV(dswfl, l, h);
-    const double dp = F(dp)[h];
+    const real dp = F(dp)[h];
     for (int l = 0; l < L; l++) {
-      const double zdh = -LV(dt, l, h) * GASCON / GA * T(rad_lnsig)[l];
-      const double den = T(dsigma)[l] * dp * K(acpd) * (1.0 + K(adv) * LV(dq, l, h));
-      const double flux = LV(dflux, l, h), below = LV(dflux, l + 1, h);
+      const real zdh = -LV(dt, l, h) * GASCON / GA * T(rad_lnsig)[l];
+      const real den = T(dsigma)[l] * dp * K(acpd) * (1.0 + K(adv) * LV(dq, l, h));
+      const real flux = LV(dflux, l, h), below = LV(dflux, l + 1, h);
       LV(dtdt, l, h) = LV(dtdt, l, h) - GA * (below - flux) / den;
       LV(dtdtswr, l, h) = -GA * (LV(dswfl, l + 1, h) - LV(dswfl, l, h)) / den;
       LV(dtdtlwr, l, h) = -GA * (LV(dlwfl, l + 1, h) - LV(dlwfl, l, h)) / den;
@@ -629,37 +629,37 @@ static void radstep(const M* v) {
 /* --------------------------------------------------------- precipitation */
 
 /* physics/precipitation/wgsl/saturation.wgsl:15-24 */
-static double sat_capped(const M* v, double t, double p) {
+static real sat_capped(const M* v, real t, real p) {
   return MIN(K(rdbrv) * K(ra1) * exp(K(ra2) * (t - K(tmelt)) / (t - K(ra4))) / p, K(rdbrv));
 }
-static double sat_divided(const M* v, double t, double p) {
-  const double q = sat_capped(v, t, p);
+static real sat_divided(const M* v, real t, real p) {
+  const real q = sat_capped(v, t, p);
   return q / (1.0 - (1.0 / K(rdbrv) - 1.0) * q);
 }
 
 typedef struct {
-  double t, q, cor;
+  real t, q, cor;
   int saturated;
 } Parcel;
 
 /* kuo.wgsl:22-47 */
-static Parcel condense(const M* v, double t0, double q0, double p) {
-  const double MOIST = 1.0 / K(rdbrv) - 1.0;
-  double s = sat_capped(v, t0, p);
-  double cor = 1.0 / (1.0 - MOIST * s);
+static Parcel condense(const M* v, real t0, real q0, real p) {
+  const real MOIST = 1.0 / K(rdbrv) - 1.0;
+  real s = sat_capped(v, t0, p);
+  real cor = 1.0 / (1.0 - MOIST * s);
   s = s * cor;
   Parcel r = {t0, q0, cor, 0};
   if (!(s < q0)) return r;
-  double t = t0, q = q0;
-  const double lcp = (t0 < K(tmelt) ? K(als) : K(alv)) / (K(acpd) * (1.0 + K(adv) * q0));
+  real t = t0, q = q0;
+  const real lcp = (t0 < K(tmelt) ? K(als) : K(alv)) / (K(acpd) * (1.0 + K(adv) * q0));
   for (int i = 0; i < 2; i++) {
     if (i == 1) {
       s = sat_capped(v, t, p);
       cor = 1.0 / (1.0 - MOIST * s);
       s = s * cor;
     }
-    const double dqsdt = K(ra2) * (K(tmelt) - K(ra4)) * s * cor / ((t - K(ra4)) * (t - K(ra4)));
-    const double rain = (q - s) / (1.0 + lcp * dqsdt);
+    const real dqsdt = K(ra2) * (K(tmelt) - K(ra4)) * s * cor / ((t - K(ra4)) * (t - K(ra4)));
+    const real rain = (q - s) / (1.0 + lcp * dqsdt);
     t = t + lcp * rain;
     q = q - rain;
   }
@@ -673,17 +673,17 @@ static Parcel condense(const M* v, double t0, double q0, double p) {
 /* physics/precipitation/wgsl/kuo.wgsl:53-298 */
 static void kuo(const M* v) {
   const int NH = v->NHOR, L = v->NLEV, NLEM = L - 1;
-  const double DELTSEC2 = K(deltsec2), ALV = K(alv), ACPD = K(acpd), ADV = K(adv);
-  const double MOIST = 1.0 / K(rdbrv) - 1.0;
-  const double *ds = T(dsigma), *sg = T(sigma), *skap = T(vdiff_skap), *zf = T(kuo_zfac);
-  double zdqt[16], zqe[16], zte[16], zdqtot[16], zalcpe[16], zacpe[16], zdqdt[16], zdtdt[16];
-  double zkdiff[16], zqn[16], ztn[16];
+  const real DELTSEC2 = K(deltsec2), ALV = K(alv), ACPD = K(acpd), ADV = K(adv);
+  const real MOIST = 1.0 / K(rdbrv) - 1.0;
+  const real *ds = T(dsigma), *sg = T(sigma), *skap = T(vdiff_skap), *zf = T(kuo_zfac);
+  real zdqt[16], zqe[16], zte[16], zdqtot[16], zalcpe[16], zacpe[16], zdqdt[16], zdtdt[16];
+  real zkdiff[16], zqn[16], ztn[16];
   for (int h = 0; h < NH; h++) {
-    const double dp = F(dp)[h];
+    const real dp = F(dp)[h];
     for (int l = 0; l < L; l++) {
       zdqt[l] = LV(dqt, l, h) * K(psurf) * K(ww) / dp;
       LV(dqt, l, h) = zdqt[l];
-      const double dq = LV(dq, l, h), dqdt = LV(dqdt, l, h);
+      const real dq = LV(dq, l, h), dqdt = LV(dqdt, l, h);
       zdqtot[l] = (zdqt[l] + dqdt) * DELTSEC2;
       zqe[l] = dq + dqdt * DELTSEC2;
       zte[l] = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
@@ -693,21 +693,21 @@ static void kuo(const M* v) {
       zdtdt[l] = 0;
     }
     int ilift = -1, itop = -1;
-    double zi = 0, zpt = 0, zpq = 0, zbeta = 0, zpbeta = 0, ztp = 0, zqp = 0;
+    real zi = 0, zpt = 0, zpq = 0, zbeta = 0, zpbeta = 0, ztp = 0, zqp = 0;
     const int n = L - 1;
-    const double dsn = ds[n];
-    const double ztsurf = LV(dt, L, h) * skap[n];
+    const real dsn = ds[n];
+    const real ztsurf = LV(dt, L, h) * skap[n];
     if (zdqtot[n] > 0 && ztsurf > zte[n]) {
-      const double p = dp * sg[n];
-      const double zqsnl = sat_divided(v, zte[n], p);
-      const double zqnew = LV(dq, L, h) * MIN(1.0, zqe[n] / zqsnl);
+      const real p = dp * sg[n];
+      const real zqsnl = sat_divided(v, zte[n], p);
+      const real zqnew = LV(dq, L, h) * MIN(1.0, zqe[n] / zqsnl);
       const Parcel parcel = condense(v, ztsurf, zqnew, p);
       if (parcel.saturated) {
         ilift = L + 1;
         itop = L;
         ztp = parcel.t;
         zqp = parcel.q;
-        const double zqsa = sat_divided(v, zte[n], p);
+        const real zqsa = sat_divided(v, zte[n], p);
         zdqdt[n] = MAX(zqsa - zqe[n], 0.0);
         zdtdt[n] = ztp - zte[n];
         zpt = zdtdt[n] * dsn * zacpe[n];
@@ -720,35 +720,35 @@ static void kuo(const M* v) {
     for (int l = NLEM - 1; l >= 0; l--) {
       const int lp = l + 1;
       if (ilift == -1 && zdqtot[lp] > 0) {
-        const double p = dp * sg[l];
+        const real p = dp * sg[l];
         const Parcel parcel = condense(v, zte[lp] * zf[l], zqe[lp], p);
         if (parcel.saturated && parcel.t > zte[l]) {
-          const double d = ds[l], dspp = ds[lp];
+          const real d = ds[l], dspp = ds[lp];
           ilift = l + 2;
           itop = l + 1;
           ztp = parcel.t;
           zqp = parcel.q;
-          const double zqsa = sat_divided(v, zte[l], p);
+          const real zqsa = sat_divided(v, zte[l], p);
           zdqdt[l] = MAX(zqsa - zqe[l], 0.0);
           zdtdt[l] = ztp - zte[l];
           zpt = zdtdt[l] * d * zacpe[l];
           zpq = zdqdt[l] * d * ALV;
           zbeta = MIN(1.0, zqe[l] / zqsa) * d;
           zpbeta = d;
-          const double pp = dp * sg[lp];
-          double zqsnl = sat_divided(v, zte[lp], pp);
+          const real pp = dp * sg[lp];
+          real zqsnl = sat_divided(v, zte[lp], pp);
           if (zqe[lp] > zqsnl) {
-            double t = zte[lp], q = zqe[lp];
-            const double lcp = (t < K(tmelt) ? K(als) : ALV) / (ACPD * (1.0 + ADV * q));
-            double cor = parcel.cor;
+            real t = zte[lp], q = zqe[lp];
+            const real lcp = (t < K(tmelt) ? K(als) : ALV) / (ACPD * (1.0 + ADV * q));
+            real cor = parcel.cor;
             for (int i = 0; i < 2; i++) {
               if (i == 1) {
                 zqsnl = sat_capped(v, t, pp);
                 cor = 1.0 / (1.0 - MOIST * zqsnl);
                 zqsnl = zqsnl * cor;
               }
-              const double dqsdt = K(ra2) * (K(tmelt) - K(ra4)) * zqsnl * cor / ((t - K(ra4)) * (t - K(ra4)));
-              const double rain = (q - zqsnl) / (1.0 + lcp * dqsdt);
+              const real dqsdt = K(ra2) * (K(tmelt) - K(ra4)) * zqsnl * cor / ((t - K(ra4)) * (t - K(ra4)));
+              const real rain = (q - zqsnl) / (1.0 + lcp * dqsdt);
               t = t + lcp * rain;
               q = q - rain;
             }
@@ -765,14 +765,14 @@ static void kuo(const M* v) {
     }
     for (int l = NLEM - 1; l >= 0; l--) {
       if (itop == l + 2) {
-        const double p = dp * sg[l];
+        const real p = dp * sg[l];
         const Parcel parcel = condense(v, ztp * zf[l], zqp, p);
         if (parcel.saturated && parcel.t > zte[l]) {
-          const double d = ds[l];
+          const real d = ds[l];
           itop = l + 1;
           ztp = parcel.t;
           zqp = parcel.q;
-          const double zqsa = sat_divided(v, zte[l], p);
+          const real zqsa = sat_divided(v, zte[l], p);
           zdqdt[l] = MAX(zqsa - zqe[l], 0.0);
           zdtdt[l] = ztp - zte[l];
           zpt = zdtdt[l] * d * zacpe[l] + zpt;
@@ -784,19 +784,19 @@ static void kuo(const M* v) {
       }
     }
     zi = MAX(0.0, zi);
-    double zrhm = 0;
+    real zrhm = 0;
     if (zpbeta > 0) zrhm = zbeta / zpbeta;
-    double beta = 1.0;
+    real beta = 1.0;
     if (zrhm >= K(rhbeta)) beta = powi((1.0 - zrhm) / (1.0 - K(rhbeta)), (int)K(nbeta));
-    double zat = 0, zaq = 0;
+    real zat = 0, zaq = 0;
     if (zpt > 0) zat = zi * (1.0 - beta) / zpt;
     if (zpq > 0) zaq = zi * beta / zpq;
     int shallow = 0;
     if (itop > 0 && (dp * sg[itop - 1] > K(pdeep) || zi <= 0)) shallow = 1;
-    double icctot = 0;
+    real icctot = 0;
     for (int l = 0; l < L; l++) {
       const int level = l + 1;
-      double dprcl = 0, icclev = 0;
+      real dprcl = 0, icclev = 0;
       if (ilift >= level && itop <= level && zi > 0 && !shallow) {
         LV(dqdt, l, h) = zaq * zdqdt[l] / DELTSEC2 - zdqt[l];
         LV(dtdt, l, h) = zat * zdtdt[l] / DELTSEC2 + LV(dtdt, l, h);
@@ -813,8 +813,8 @@ static void kuo(const M* v) {
     }
     F(icctot)[h] = icctot;
     if (!shallow) continue;
-    const double zkonst1 = K(ga) * DELTSEC2 / K(gascon);
-    const double zkonst2 = K(rkshallow) * zkonst1 * K(ga) / K(gascon);
+    const real zkonst1 = K(ga) * DELTSEC2 / K(gascon);
+    const real zkonst2 = K(rkshallow) * zkonst1 * K(ga) / K(gascon);
     int kktop = 1;
     for (int level = 2; level <= NLEM; level++)
       if (dp * sg[level - 1] < K(pdeepth)) kktop = level;
@@ -822,18 +822,18 @@ static void kuo(const M* v) {
     for (int l = 0; l < L; l++) zkdiff[l] = 0;
     for (int l = 0; l < NLEM; l++) {
       const int lp = l + 1, level = l + 1;
-      const double shh = T(sigmah)[l];
-      const double zfac = zkonst2 * shh * shh * T(vdiff_rdsig)[l];
-      const double d = ds[l], dspp = ds[lp];
-      const double zrthl = (dspp + d) / (zte[l] * dspp + zte[lp] * d);
+      const real shh = T(sigmah)[l];
+      const real zfac = zkonst2 * shh * shh * T(vdiff_rdsig)[l];
+      const real d = ds[l], dspp = ds[lp];
+      const real zrthl = (dspp + d) / (zte[l] * dspp + zte[lp] * d);
       if (level >= ktop && level < ilift) zkdiff[l] = zfac * zrthl * zrthl;
       if (level + 1 == ktop) {
-        double zzqs = sat_capped(v, zte[l], dp * sg[l]);
+        real zzqs = sat_capped(v, zte[l], dp * sg[l]);
         zzqs = zzqs * (1.0 / (1.0 - MOIST * zzqs));
-        double zzqsp = sat_capped(v, zte[lp], dp * sg[lp]);
+        real zzqsp = sat_capped(v, zte[lp], dp * sg[lp]);
         zzqsp = zzqsp * (1.0 / (1.0 - MOIST * zzqsp));
-        const double zrh = zqe[l] / zzqs, zrhp = zqe[lp] / zzqsp;
-        const double zzf = fabs((zrh - 0.8) / 0.2 * (zrhp - zrh));
+        const real zrh = zqe[l] / zzqs, zrhp = zqe[lp] / zzqsp;
+        const real zzf = fabs((zrh - 0.8) / 0.2 * (zrhp - zrh));
         zkdiff[l] = zfac * zrthl * zrthl * zzf;
       }
     }
@@ -852,10 +852,10 @@ static void kuo(const M* v) {
 /* physics/precipitation/wgsl/mkdca.wgsl:13-97 */
 static void mkdca(const M* v) {
   const int NH = v->NHOR, L = v->NLEV, NLEM = L - 1;
-  const double DELTSEC2 = K(deltsec2);
-  double zt[16], zth[16], zq[16], zqn[16];
+  const real DELTSEC2 = K(deltsec2);
+  real zt[16], zth[16], zq[16], zqn[16];
   int mixed[16];
-  const double *skap = T(vdiff_skap), *ds = T(dsigma);
+  const real *skap = T(vdiff_skap), *ds = T(dsigma);
   for (int h = 0; h < NH; h++) {
     for (int l = 0; l < L; l++) {
       zt[l] = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
@@ -864,11 +864,11 @@ static void mkdca(const M* v) {
       zqn[l] = zq[l];
       mixed[l] = 0;
     }
-    double limited = 0;
+    real limited = 0;
     int passes = 0;
     for (;;) {
       int unstable = 0, ibase = 0, itop = L;
-      double ztht = 0, zqt = 0, zsum1 = 0, zsum2 = 0, zsumq1 = 0, zsumq2 = 0;
+      real ztht = 0, zqt = 0, zsum1 = 0, zsum2 = 0, zsumq1 = 0, zsumq2 = 0;
       for (int l = 0; l < NLEM; l++) {
         const int lp = l + 1;
         if (zth[l] < zth[lp]) {
@@ -920,7 +920,7 @@ static void mkdca(const M* v) {
 
 /* physics/precipitation/wgsl/mkdca-limit.wgsl:5-22 */
 static void mkdca_limit(const M* v) {
-  double flag = 0;
+  real flag = 0;
   for (int h = 0; h < v->NHOR; h++) flag = MAX(flag, F(dca_limited)[h]);
   F(dca_skip)[0] = flag;
 }
@@ -928,33 +928,33 @@ static void mkdca_limit(const M* v) {
 /* physics/precipitation/wgsl/mklsp.wgsl:3-39 */
 static void mklsp(const M* v, int h) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double DELTSEC2 = K(deltsec2), MOIST = 1.0 / K(rdbrv) - 1.0;
-  const double dp = F(dp)[h];
+  const real DELTSEC2 = K(deltsec2), MOIST = 1.0 / K(rdbrv) - 1.0;
+  const real dp = F(dp)[h];
   for (int l = 0; l < L; l++) {
     LV(dprsll, l, h) = 0;
-    double dprll = 0;
-    const double dq = LV(dq, l, h);
-    const double zq = dq + LV(dqdt, l, h) * DELTSEC2;
-    const double zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
-    const double zlcpe = K(alv) / (K(acpd) * (1.0 + K(adv) * dq));
-    const double p = dp * T(sigma)[l];
-    double zqsat = sat_capped(v, zt, p);
-    double zcor = 1.0 / (1.0 - MOIST * zqsat);
+    real dprll = 0;
+    const real dq = LV(dq, l, h);
+    const real zq = dq + LV(dqdt, l, h) * DELTSEC2;
+    const real zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
+    const real zlcpe = K(alv) / (K(acpd) * (1.0 + K(adv) * dq));
+    const real p = dp * T(sigma)[l];
+    real zqsat = sat_capped(v, zt, p);
+    real zcor = 1.0 / (1.0 - MOIST * zqsat);
     zqsat = zqsat * zcor;
     if (zqsat < 0) zqsat = 0;
     if (zq > zqsat) {
-      const double c = K(ra2) * (K(tmelt) - K(ra4));
-      double zdqdt = (zqsat - zq) / (1.0 + zlcpe * c * zqsat * zcor / ((zt - K(ra4)) * (zt - K(ra4))));
-      double ztn = zt - zdqdt * zlcpe;
-      double zqn = zq + zdqdt;
+      const real c = K(ra2) * (K(tmelt) - K(ra4));
+      real zdqdt = (zqsat - zq) / (1.0 + zlcpe * c * zqsat * zcor / ((zt - K(ra4)) * (zt - K(ra4))));
+      real ztn = zt - zdqdt * zlcpe;
+      real zqn = zq + zdqdt;
       zqsat = sat_capped(v, ztn, p);
       zcor = 1.0 / (1.0 - MOIST * zqsat);
       zqsat = zqsat * zcor;
       zdqdt = (zqsat - zqn) / (1.0 + zlcpe * c * zqsat * zcor / ((ztn - K(ra4)) * (ztn - K(ra4))));
       ztn = ztn - zdqdt * zlcpe;
       zqn = zqn + zdqdt;
-      const double dqdt = (zqn - zq) / DELTSEC2;
-      const double dtdt = (ztn - zt) / DELTSEC2;
+      const real dqdt = (zqn - zq) / DELTSEC2;
+      const real dtdt = (ztn - zt) / DELTSEC2;
       LV(dqdt, l, h) = LV(dqdt, l, h) + dqdt;
       LV(dtdt, l, h) = LV(dtdt, l, h) + dtdt;
       dprll = -MIN(dqdt, 0.0) * T(dsigma)[l];
@@ -964,9 +964,9 @@ static void mklsp(const M* v, int h) {
 }
 
 /* mkrain.wgsl:8-15 */
-static double evaporate(const M* v, double* flux, double deficit, double zlcpe, double zqsat, double zcor, double zt,
-                        double ds) {
-  const double zdqdt = MIN(K(gamma) * deficit * ds / K(deltsec2) /
+static real evaporate(const M* v, real* flux, real deficit, real zlcpe, real zqsat, real zcor, real zt,
+                        real ds) {
+  const real zdqdt = MIN(K(gamma) * deficit * ds / K(deltsec2) /
                                (1.0 + zlcpe * K(ra2) * (K(tmelt) - K(ra4)) * zqsat * zcor / ((zt - K(ra4)) * (zt - K(ra4)))),
                            *flux);
   *flux = *flux - zdqdt;
@@ -976,22 +976,22 @@ static double evaporate(const M* v, double* flux, double deficit, double zlcpe, 
 /* physics/precipitation/wgsl/mkrain.wgsl:20-95 */
 static void mkrain(const M* v, int h) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double DELTSEC2 = K(deltsec2), MOIST = 1.0 / K(rdbrv) - 1.0;
-  const double ALV = K(alv), ALS = K(als), ACPD = K(acpd), ADV = K(adv), TMELT = K(tmelt);
-  const double dp = F(dp)[h];
-  double zprc = 0, zprl = 0, zprsc = 0, zprsl = 0;
+  const real DELTSEC2 = K(deltsec2), MOIST = 1.0 / K(rdbrv) - 1.0;
+  const real ALV = K(alv), ALS = K(als), ACPD = K(acpd), ADV = K(adv), TMELT = K(tmelt);
+  const real dp = F(dp)[h];
+  real zprc = 0, zprl = 0, zprsc = 0, zprsl = 0;
   for (int l = 0; l < L; l++) {
-    const double ds = T(dsigma)[l];
-    const double dq = LV(dq, l, h);
-    double zdtdt = 0, zdqdtl = 0, zdqdts = 0;
+    const real ds = T(dsigma)[l];
+    const real dq = LV(dq, l, h);
+    real zdtdt = 0, zdqdtl = 0, zdqdts = 0;
     zprc = zprc + LV(dprcl, l, h) - LV(dprscl, l, h);
     zprl = zprl + LV(dprll, l, h) - LV(dprsll, l, h);
     zprsc = zprsc + LV(dprscl, l, h);
     zprsl = zprsl + LV(dprsll, l, h);
     const int surface = l == L - 1 ? 2 : 1;
     for (int below = 0; below < surface; below++) {
-      const double temperature = LV(dt, l + below, h);
-      const double zlcp = (ALS - ALV) / (ACPD * (1.0 + ADV * dq));
+      const real temperature = LV(dt, l + below, h);
+      const real zlcp = (ALS - ALV) / (ACPD * (1.0 + ADV * dq));
       if (zprsc + zprsl > 0 && temperature > TMELT) {
         zdtdt = -(zprsc + zprsl) / ds * zlcp + zdtdt;
         zprc = zprsc + zprc;
@@ -1006,33 +1006,33 @@ static void mkrain(const M* v, int h) {
         zprl = 0;
       }
     }
-    const double zq = dq + LV(dqdt, l, h) * DELTSEC2;
-    const double zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
-    double zqsat = sat_capped(v, zt, dp * T(sigma)[l]);
-    const double zcor = 1.0 / (1.0 - MOIST * zqsat);
+    const real zq = dq + LV(dqdt, l, h) * DELTSEC2;
+    const real zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
+    real zqsat = sat_capped(v, zt, dp * T(sigma)[l]);
+    const real zcor = 1.0 / (1.0 - MOIST * zqsat);
     zqsat = zqsat * zcor;
     if (zqsat < 0) zqsat = 0;
     if (zq < zqsat) {
-      const double deficit = zqsat - zq;
-      const double water = ALV / (ACPD * (1.0 + ADV * dq));
-      const double ice = ALS / (ACPD * (1.0 + ADV * dq));
+      const real deficit = zqsat - zq;
+      const real water = ALV / (ACPD * (1.0 + ADV * dq));
+      const real ice = ALS / (ACPD * (1.0 + ADV * dq));
       if (zprl > 0) {
-        const double e = evaporate(v, &zprl, deficit, water, zqsat, zcor, zt, ds);
+        const real e = evaporate(v, &zprl, deficit, water, zqsat, zcor, zt, ds);
         zdtdt = zdtdt - e * water;
         zdqdtl = zdqdtl + e;
       }
       if (zprc > 0) {
-        const double e = evaporate(v, &zprc, deficit, water, zqsat, zcor, zt, ds);
+        const real e = evaporate(v, &zprc, deficit, water, zqsat, zcor, zt, ds);
         zdtdt = zdtdt - e * water;
         zdqdtl = zdqdtl + e;
       }
       if (zprsl > 0) {
-        const double e = evaporate(v, &zprsl, deficit, ice, zqsat, zcor, zt, ds);
+        const real e = evaporate(v, &zprsl, deficit, ice, zqsat, zcor, zt, ds);
         zdtdt = zdtdt - e * ice;
         zdqdts = zdqdts + e;
       }
       if (zprsc > 0) {
-        const double e = evaporate(v, &zprsc, deficit, ice, zqsat, zcor, zt, ds);
+        const real e = evaporate(v, &zprsc, deficit, ice, zqsat, zcor, zt, ds);
         zdtdt = zdtdt - e * ice;
         zdqdts = zdqdts + e;
       }
@@ -1048,42 +1048,42 @@ static void mkrain(const M* v, int h) {
 /* physics/precipitation/wgsl/mkclouds.wgsl:18-86 */
 static void mkclouds(const M* v, int h) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double ZCLMAX = 1.0, ZCCA = 0.245, ZCCB = 0.125, ZCCMAX = 0.8, ZCCMIN = 0.05;
-  const double DELTSEC2 = K(deltsec2), GA = K(ga), GASCON = K(gascon);
-  const double dp = F(dp)[h];
-  double cover[16], zzf[16];
-  double zcctot = ZCCMIN, zccconv = ZCCMIN;
-  const double dprc = F(dprc)[h];
+  const real ZCLMAX = 1.0, ZCCA = 0.245, ZCCB = 0.125, ZCCMAX = 0.8, ZCCMIN = 0.05;
+  const real DELTSEC2 = K(deltsec2), GA = K(ga), GASCON = K(gascon);
+  const real dp = F(dp)[h];
+  real cover[16], zzf[16];
+  real zcctot = ZCCMIN, zccconv = ZCCMIN;
+  const real dprc = F(dprc)[h];
   if (dprc > 0) {
     zcctot = ZCCA + ZCCB * log(dprc * K(zrfac));
     zcctot = MIN(ZCCMAX, MAX(ZCCMIN, zcctot));
-    const double levels = F(icctot)[h];
+    const real levels = F(icctot)[h];
     zccconv = levels == 0.0 ? 1.0 : 1.0 - pow(1.0 - zcctot, 1.0 / levels);
   }
   for (int l = 0; l < L; l++) {
-    const double zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
-    const double zq = LV(dq, l, h) + LV(dqdt, l, h) * DELTSEC2;
-    const double zqsat = sat_divided(v, zt, dp * T(sigma)[l]);
-    double zrh = zq / zqsat;
-    const double icclev = LV(icclev, l, h);
-    double dcc = 0;
+    const real zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
+    const real zq = LV(dq, l, h) + LV(dqdt, l, h) * DELTSEC2;
+    const real zqsat = sat_divided(v, zt, dp * T(sigma)[l]);
+    real zrh = zq / zqsat;
+    const real icclev = LV(icclev, l, h);
+    real dcc = 0;
     if (icclev > 0) dcc = zccconv;
     if (icclev == 2.0 && zcctot >= 0.3) dcc = MIN(dcc + 2.0 * (zcctot - 0.3), ZCLMAX);
     zrh = zrh * (1.0 - dcc);
-    const double zrcrit = T(rcrit)[l];
-    double zwfac = 1.0;
+    const real zrcrit = T(rcrit)[l];
+    real zwfac = 1.0;
     if (T(sigma)[l] > 0.8 && K(clwfac) > 0) {
-      const double dw = LV(dw, l, h);
+      const real dw = LV(dw, l, h);
       zwfac = K(clwfac) * MAX(0.0, K(clwcrit1) - dw);
       if (K(clwcrit2) > dw) zwfac = 1.0;
     }
-    const double excess = MAX(0.0, (zrh - zrcrit) / (1.0 - zrcrit));
+    const real excess = MAX(0.0, (zrh - zrcrit) / (1.0 - zrcrit));
     cover[l] = MIN(dcc + zwfac * (excess * excess), ZCLMAX);
   }
-  double dqvi = 0, zzh = 0;
+  real dqvi = 0, zzh = 0;
   for (int l = L - 1; l >= 1; l--) {
     dqvi = LV(dq, l, h) * T(dsigma)[l] + dqvi;
-    const double zdh = -LV(dt, l, h) * GASCON / GA * T(rad_lnsig)[l];
+    const real zdh = -LV(dt, l, h) * GASCON / GA * T(rad_lnsig)[l];
     zzf[l] = zzh + zdh * 0.5;
     zzh = zzh + zdh;
   }
@@ -1092,9 +1092,9 @@ static void mkclouds(const M* v, int h) {
   dqvi = dqvi * dp / GA;
   F(dqvi)[h] = dqvi;
   zzh = 700.0 * log(1.0 + dqvi);
-  double clear = 1.0;
+  real clear = 1.0;
   for (int l = 0; l < L; l++) {
-    double dql = 0;
+    real dql = 0;
     if (zzh > 0 && cover[l] > 0) {
       dql = 0.00021 * exp(-zzf[l] / zzh) * GASCON * LV(dt, l, h) / (T(sigma)[l] * dp);
       dql = MAX(dql, 1.0e-9);
@@ -1125,15 +1125,15 @@ static void rain_column(const M* v) {
 /* ----------------------------------------------------------------- land */
 
 typedef struct {
-  double ctop, soilz1, diff1, cap1;
+  real ctop, soilz1, diff1, cap1;
 } SnowLayer;
 
 /* physics/land/wgsl/soil.wgsl:63-77 */
-static SnowLayer snow_layer(const M* v, double snowz, double cap0, double diff0, double soilz0) {
+static SnowLayer snow_layer(const M* v, real snowz, real cap0, real diff0, real soilz0) {
   SnowLayer r;
-  const double zsntop = MIN(snowz * 1000.0 / K(rhosnow), K(dztop));
+  const real zsntop = MIN(snowz * 1000.0 / K(rhosnow), K(dztop));
   r.ctop = cap0 * K(snowcap) * K(dztop) / (cap0 * zsntop + K(snowcap) * (K(dztop) - zsntop));
-  const double zsnowz = MIN(snowz * 1000.0 / K(rhosnow), 1.0) - zsntop;
+  const real zsnowz = MIN(snowz * 1000.0 / K(rhosnow), 1.0) - zsntop;
   r.soilz1 = soilz0 + zsnowz;
   r.diff1 = diff0 * K(snowdiff) * r.soilz1 / (K(snowdiff) * soilz0 + diff0 * zsnowz);
   r.cap1 = cap0 * K(snowcap) * r.soilz1 / (cap0 * zsnowz + K(snowcap) * soilz0);
@@ -1141,18 +1141,18 @@ static SnowLayer snow_layer(const M* v, double snowz, double cap0, double diff0,
 }
 
 /* soil.wgsl:84-91 */
-static double surface_temperature(const M* v, SnowLayer layer, double dtsm, double flux, double soil1) {
-  const double conduction = 2.0 * layer.diff1;
+static real surface_temperature(const M* v, SnowLayer layer, real dtsm, real flux, real soil1) {
+  const real conduction = 2.0 * layer.diff1;
   return (layer.ctop * K(dztop) * dtsm / K(deltsec) + flux + conduction * soil1 / layer.soilz1) /
          (layer.ctop * K(dztop) / K(deltsec) + conduction / layer.soilz1);
 }
 
 /* soil.wgsl:23-55 */
-static void mktsoil(const M* v, int h, double ftop, const double* soilz, const double* cap, const double* diff) {
+static void mktsoil(const M* v, int h, real ftop, const real* soilz, const real* cap, const real* diff) {
   const int NH = v->NHOR, NS = 5;
-  double zdiff[5], zcap[5], zebs[5], ztn[5];
+  real zdiff[5], zcap[5], zebs[5], ztn[5];
   const int n = NS - 1;
-  const double DELTSEC = K(deltsec);
+  const real DELTSEC = K(deltsec);
   zdiff[n - 1] = 2.0 * diff[n] * diff[n - 1] / (diff[n] * soilz[n - 1] + diff[n - 1] * soilz[n]);
   zcap[n] = cap[n] * soilz[n] / DELTSEC;
   zebs[n] = 1.0 / (zcap[n] + zdiff[n - 1]);
@@ -1174,36 +1174,36 @@ static void mktsoil(const M* v, int h, double ftop, const double* soilz, const d
 /* physics/land/wgsl/soil.wgsl:95-185 */
 static void soil(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double DELTSEC = K(deltsec), TMELT = K(tmelt), ALS = K(als), ALV = K(alv);
+  const real DELTSEC = K(deltsec), TMELT = K(tmelt), ALS = K(als), ALV = K(alv);
   for (int h = 0; h < NH; h++) {
-    const double dls = F(dls)[h];
+    const real dls = F(dls)[h];
     const int land = dls > 0;
-    const double snowStart = F(dsnowz)[h];
-    double dwater = 0, dsmelt = 0;
+    const real snowStart = F(dsnowz)[h];
+    real dwater = 0, dsmelt = 0;
     if (snowStart == 0.0) F(dsnowt)[h] = TMELT;
     if (land) {
-      const double glac = F(dglac)[h];
-      double soilz[5], cap[5], diff[5];
+      const real glac = F(dglac)[h];
+      real soilz[5], cap[5], diff[5];
       for (int l = 0; l < 5; l++) {
         cap[l] = K(sicecap) * glac + K(soilcap) * (1.0 - glac);
         diff[l] = K(sicediff) * glac + K(soildiff) * (1.0 - glac);
         soilz[l] = T(dsoilz)[l];
       }
-      const double zhfla = F(dshfl)[h] + F(dlhfl)[h] + LV(dflux, L, h);
-      const double devap = F(devap)[h];
-      double zdsnowz = 0;
+      const real zhfla = F(dshfl)[h] + F(dlhfl)[h] + LV(dflux, L, h);
+      const real devap = F(devap)[h];
+      real zdsnowz = 0;
       if (F(dprs)[h] > 0) zdsnowz = F(dprs)[h];
       if (snowStart > 0) zdsnowz = zdsnowz + devap;
-      double zsnowz = MAX(0.0, snowStart + zdsnowz * DELTSEC);
+      real zsnowz = MAX(0.0, snowStart + zdsnowz * DELTSEC);
       zdsnowz = (zsnowz - snowStart) / DELTSEC;
       dwater = devap + F(dprl)[h] + F(dprc)[h] - zdsnowz;
-      double dsnowz = zsnowz;
-      const double dtsm = F(land_dtsm)[h];
-      const double soil1 = LV(dsoilt, 0, h);
+      real dsnowz = zsnowz;
+      const real dtsm = F(land_dtsm)[h];
+      const real soil1 = LV(dsoilt, 0, h);
       SnowLayer layer = snow_layer(v, dsnowz, cap[0], diff[0], soilz[0]);
-      double dts = surface_temperature(v, layer, dtsm, zhfla, soil1);
-      double zhfls = 2.0 * layer.diff1 / layer.soilz1 * (dts - soil1);
-      double zhflm = 0;
+      real dts = surface_temperature(v, layer, dtsm, zhfla, soil1);
+      real zhfls = 2.0 * layer.diff1 / layer.soilz1 * (dts - soil1);
+      real zhflm = 0;
       if (dsnowz > 0 && dts > TMELT) {
         dts = TMELT;
         zhflm = MAX(0.0, zhfla - layer.ctop * K(dztop) * (dts - dtsm) / DELTSEC +
@@ -1241,10 +1241,10 @@ static void soil(const M* v) {
     }
     F(dwater)[h] = dwater;
     F(dsmelt)[h] = dsmelt;
-    double drunoff = 0;
+    real drunoff = 0;
     if (land) {
-      const double dwatc = F(dwatc)[h] + DELTSEC * dwater;
-      const double dwmax = F(dwmax)[h];
+      const real dwatc = F(dwatc)[h] + DELTSEC * dwater;
+      const real dwmax = F(dwmax)[h];
       drunoff = MAX(0.0, dwatc - dwmax) / DELTSEC;
       F(dwatc)[h] = MAX(MIN(dwmax, dwatc), 0.0);
     }
@@ -1253,23 +1253,23 @@ static void soil(const M* v) {
 }
 
 /* physics/land/wgsl/mkradv.wgsl:1-59 */
-static double river(const M* v, int h) { return F(driver)[h] * F(darea)[h] * F(dls)[h]; }
+static real river(const M* v, int h) { return F(driver)[h] * F(darea)[h] * F(dls)[h]; }
 
-static void eastward(const M* v, int row, int lon, double* out, double* into) {
+static void eastward(const M* v, int row, int lon, real* out, real* into) {
   const int h = row + lon;
   const int next = row + (lon + 1) % v->NLON;
-  const double u = F(duroff)[h];
-  const double water = u > 0 ? river(v, h) : river(v, next);
+  const real u = F(duroff)[h];
+  const real water = u > 0 ? river(v, h) : river(v, next);
   *out = u * water / F(darea)[h];
   *into = u * water / F(darea)[next];
 }
 
 static void mkradv(const M* v) {
   const int NH = v->NHOR, NLON = v->NLON;
-  double* out = (double*)malloc(sizeof(double) * NH);
+  real* out = (real*)malloc(sizeof(real) * NH);
   for (int h = 0; h < NH; h++) {
     const int lat = h / NLON, lon = h % NLON, row = lat * NLON;
-    double runoff = F(drunoff)[h], o, in;
+    real runoff = F(drunoff)[h], o, in;
     if (lon == 0) {
       eastward(v, row, 0, &o, &in);
       runoff = runoff - o;
@@ -1281,17 +1281,17 @@ static void mkradv(const M* v) {
       eastward(v, row, lon, &o, &in);
       runoff = runoff - o;
     }
-    const double area = F(darea)[h];
-    const double vv = F(dvroff)[h];
-    const double below = lat + 1 < v->NLAT ? river(v, h + NLON) : 0.0;
+    const real area = F(darea)[h];
+    const real vv = F(dvroff)[h];
+    const real below = lat + 1 < v->NLAT ? river(v, h + NLON) : 0.0;
     if (vv < 0)
       runoff = runoff + vv * river(v, h) / area;
     else
       runoff = runoff + vv * below / area;
-    double fromAbove = 0;
+    real fromAbove = 0;
     if (lat > 0) {
       const int above = h - NLON;
-      const double va = F(dvroff)[above];
+      const real va = F(dvroff)[above];
       fromAbove = va < 0 ? -va * river(v, above) : -va * river(v, h);
     }
     out[h] = runoff + fromAbove / area;
@@ -1301,20 +1301,20 @@ static void mkradv(const M* v) {
 }
 
 /* land/wgsl/surface.wgsl:43-49: momint over a 0..13 climatology at nstep+1 */
-static double clim_next(const M* v, const double* field, int h) {
-  const double* ss = F(surface_step);
-  const double w = ss[5];
+static real clim_next(const M* v, const real* field, int h) {
+  const real* ss = F(surface_step);
+  const real w = ss[5];
   return (1.0 - w) * field[(long)ss[3] * v->NHOR + h] + w * field[(long)ss[4] * v->NHOR + h];
 }
 
-static double snow_albedo(const M* v, double forest, double maxF, double maxB, double minF, double minB, double ts) {
-  const double zalbmax = forest * maxF + (1.0 - forest) * maxB;
-  const double zalbmin = forest * minF + (1.0 - forest) * minB;
-  const double zdalb = (zalbmax - zalbmin) * (ts - 263.16) / (K(tmelt) - 263.16);
+static real snow_albedo(const M* v, real forest, real maxF, real maxB, real minF, real minB, real ts) {
+  const real zalbmax = forest * maxF + (1.0 - forest) * maxB;
+  const real zalbmin = forest * minF + (1.0 - forest) * minB;
+  const real zdalb = (zalbmax - zalbmin) * (ts - 263.16) / (K(tmelt) - 263.16);
   return MAX(zalbmin, MIN(zalbmax, zalbmax - zdalb));
 }
 
-static double glacier_albedo(const M* v, double mx, double mn, double ts) {
+static real glacier_albedo(const M* v, real mx, real mn, real ts) {
   return MAX(mn, MIN(mx, mx - (mx - mn) * (ts - 263.16) / (K(tmelt) - 263.16)));
 }
 
@@ -1323,18 +1323,18 @@ static double glacier_albedo(const M* v, double mx, double mn, double ts) {
 static void land_surface(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
   for (int h = 0; h < NH; h++) {
-    const double dls = F(dls)[h];
+    const real dls = F(dls)[h];
     if (dls > 0) F(driver)[h] = F(driver)[h] + F(drunoff)[h] * K(deltsec);
-    const double dts = F(land_dts)[h];
-    double albclim = clim_next(v, F(dalbcl), h);
-    double albclim1 = clim_next(v, F(dalbcl1), h);
-    double albclim2 = clim_next(v, F(dalbcl2), h);
+    const real dts = F(land_dts)[h];
+    real albclim = clim_next(v, F(dalbcl), h);
+    real albclim1 = clim_next(v, F(dalbcl1), h);
+    real albclim2 = clim_next(v, F(dalbcl2), h);
     if (K(nwetsoil) != 0 && dls > 0.5) {
       if (dts > 273.15) {
-        const double bf = F(dwater)[h] / K(wsmax);
-        const double al = 1.0 / 5.2 * pow(pow(MAX(bf, 1.0e-3), (1.0 - 4.0) / 4.0) - 1.0, 1.0 / 4.0);
-        const double dry = exp(-powi(bf * 25.0, 6));
-        const double wet = exp(-powi((1.0 - bf) * 30.0, 9));
+        const real bf = F(dwater)[h] / K(wsmax);
+        const real al = 1.0 / 5.2 * pow(pow(MAX(bf, 1.0e-3), (1.0 - 4.0) / 4.0) - 1.0, 1.0 / 4.0);
+        const real dry = exp(-powi(bf * 25.0, 6));
+        const real wet = exp(-powi((1.0 - bf) * 30.0, 9));
         albclim = K(alblandmax) * dry + al * (1.0 - dry - wet) + K(doceanalb1) * wet;
         albclim1 = albclim;
         albclim2 = K(alblandmax) * dry + al * (1.0 - dry - wet) + K(doceanalb2) * wet;
@@ -1349,16 +1349,16 @@ static void land_surface(const M* v) {
     F(dalbclim2)[h] = albclim2;
     if (dls > 0) {
       F(land_dtsm)[h] = dts;
-      double dqs = K(rdbrv) * K(ra1) * exp(K(ra2) * (dts - K(tmelt)) / (dts - K(ra4))) / F(dp)[h];
+      real dqs = K(rdbrv) * K(ra1) * exp(K(ra2) * (dts - K(tmelt)) / (dts - K(ra4))) / F(dp)[h];
       dqs = dqs / (1.0 - (1.0 / K(rdbrv) - 1.0) * dqs);
       F(land_dqs)[h] = dqs;
-      const double dsnow = F(dsnowz)[h];
+      const real dsnow = F(dsnowz)[h];
       F(dsnow)[h] = dsnow;
       if (dsnow > 0) {
-        const double forest = F(dforest)[h];
-        const double s0 = snow_albedo(v, forest, K(albsmaxf), K(albsmax), K(albsminf), K(albsmin), dts);
-        const double s1 = snow_albedo(v, forest, K(albsmaxf1), K(albsmax1), K(albsminf1), K(albsmin1), dts);
-        const double s2 = snow_albedo(v, forest, K(albsmaxf2), K(albsmax2), K(albsminf2), K(albsmin2), dts);
+        const real forest = F(dforest)[h];
+        const real s0 = snow_albedo(v, forest, K(albsmaxf), K(albsmax), K(albsminf), K(albsmin), dts);
+        const real s1 = snow_albedo(v, forest, K(albsmaxf1), K(albsmax1), K(albsminf1), K(albsmin1), dts);
+        const real s2 = snow_albedo(v, forest, K(albsmaxf2), K(albsmax2), K(albsminf2), K(albsmin2), dts);
         F(dalb)[h] = albclim + (s0 - albclim) * dsnow / (dsnow + 0.01);
         F(dsalb)[h] = albclim1 + (s1 - albclim1) * dsnow / (dsnow + 0.01);
         F(dsalb)[NH + h] = albclim2 + (s2 - albclim2) * dsnow / (dsnow + 0.01);
@@ -1367,7 +1367,7 @@ static void land_surface(const M* v) {
         F(dalb)[h] = albclim;
         F(dsalb)[h] = albclim1;
         F(dsalb)[NH + h] = albclim2;
-        const double dwmax = F(dwmax)[h];
+        const real dwmax = F(dwmax)[h];
         if (dwmax > 0) F(drhs)[h] = MIN(1.0, F(dwatc)[h] / (K(drhsfull) * dwmax));
       }
       F(dtsoil)[h] = LV(dsoilt, 0, h);
@@ -1390,57 +1390,57 @@ static void land_surface(const M* v) {
 /* -------------------------------------------------------- sea ice, ocean */
 
 /* sea-ice/wgsl/ice.wgsl:21-28: offset 0 for nstep, 3 for nstep + 1 */
-static double clim_at(const M* v, const double* field, int h, int offset) {
-  const double* ss = F(surface_step);
-  const double w = ss[offset + 2];
+static real clim_at(const M* v, const real* field, int h, int offset) {
+  const real* ss = F(surface_step);
+  const real w = ss[offset + 2];
   return (1.0 - w) * field[(long)ss[offset] * v->NHOR + h] + w * field[(long)ss[offset + 1] * v->NHOR + h];
 }
 
 /* physics/sea-ice/wgsl/ice.wgsl:36-225 */
 static void sea_ice(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
-  const double XDT = K(xdt), XMIND = K(xmind), XMAXD = K(xmaxd), CICEMIN = K(cicemin);
-  const double TFREEZE = K(tfreeze), TMELT = K(tmelt), CRHOI = K(crhoi), CRHOS = K(crhos);
-  const double CRHOSN = K(crhosn), CPI = K(cpi), CKAPI = K(ckapi), CKAPSN = K(ckapsn);
-  const double CLFI = K(clfi), CLFSN = K(clfsn);
+  const real XDT = K(xdt), XMIND = K(xmind), XMAXD = K(xmaxd), CICEMIN = K(cicemin);
+  const real TFREEZE = K(tfreeze), TMELT = K(tmelt), CRHOI = K(crhoi), CRHOS = K(crhos);
+  const real CRHOSN = K(crhosn), CPI = K(cpi), CKAPI = K(ckapi), CKAPSN = K(ckapsn);
+  const real CLFI = K(clfi), CLFSN = K(clfsn);
   for (int h = 0; h < NH; h++) {
     const int sea = F(dls)[h] < 0.5;
-    double xheat = 0, xprs = 0;
+    real xheat = 0, xprs = 0;
     if (sea) {
-      const double naccua = F(sea_naccua)[h] + 1.0;
+      const real naccua = F(sea_naccua)[h] + 1.0;
       xheat = (F(dshfl)[h] + LV(dswfl, L, h) + LV(dlwfl, L, h) + F(dlhfl)[h] + F(sea_cheata)[h]) / naccua;
       xprs = (F(dprs)[h] + F(sea_cprsa)[h]) / naccua;
     }
     F(sea_cheata)[h] = 0;
     F(sea_cprsa)[h] = 0;
     F(sea_naccua)[h] = 0;
-    const double zrhoilfdt = CRHOI * CLFI / XDT;
+    const real zrhoilfdt = CRHOI * CLFI / XDT;
     F(xclssto)[h] = clim_at(v, F(xclsst), h, 0);
     F(xclsst2)[h] = clim_at(v, F(xclsst), h, 3);
-    const double cliced = MAX(clim_at(v, F(xcliced), h, 3), 0.0);
-    double clicec = MAX(clim_at(v, F(xclicec), h, 3), 0.0);
+    const real cliced = MAX(clim_at(v, F(xcliced), h, 3), 0.0);
+    real clicec = MAX(clim_at(v, F(xclicec), h, 3), 0.0);
     if (cliced <= 0) clicec = 0;
     F(xcliced2)[h] = cliced;
     F(xclicec2)[h] = clicec;
-    const double xls = F(xls)[h], xsst = F(xsst)[h];
-    double xiced = F(xiced)[h], xsnow = F(xsnow)[h], xicecc = F(xicecc)[h];
-    double xcflux = F(xcflux)[h], xcfluxf = F(xcfluxf)[h];
+    const real xls = F(xls)[h], xsst = F(xsst)[h];
+    real xiced = F(xiced)[h], xsnow = F(xsnow)[h], xicecc = F(xicecc)[h];
+    real xcflux = F(xcflux)[h], xcfluxf = F(xcfluxf)[h];
     F(ice_snowold)[h] = xsnow;
-    const double zicedold = xls < 0.5 ? xiced : 0.0;
-    double ximelt = 0, xsmelt = 0, xsmflx = 0, xscflx = 0, xsndch = 0, xcfluxr = 0;
+    const real zicedold = xls < 0.5 ? xiced : 0.0;
+    real ximelt = 0, xsmelt = 0, xsmflx = 0, xscflx = 0, xsndch = 0, xcfluxr = 0;
     if (F(xicec)[h] >= CICEMIN) {
       xsnow = xsnow + xprs * XDT;
       xprs = 0;
     }
-    double xts = F(xts)[h], xqmelt = 0;
-    const double ztso = MIN(TMELT, xts);
-    const double zhsnow = 1.0e3 / CRHOSN * xsnow;
+    real xts = F(xts)[h], xqmelt = 0;
+    const real ztso = MIN(TMELT, xts);
+    const real zhsnow = 1.0e3 / CRHOSN * xsnow;
     if (xiced >= XMIND) {
-      const double zcpdt = XMIND * CRHOI * CPI / XDT;
-      const double zckap = (CKAPSN * zhsnow + CKAPI * xiced) / (zhsnow + xiced);
-      double zkapz = zckap / (xiced + zhsnow);
+      const real zcpdt = XMIND * CRHOI * CPI / XDT;
+      const real zckap = (CKAPSN * zhsnow + CKAPI * xiced) / (zhsnow + xiced);
+      real zkapz = zckap / (xiced + zhsnow);
       if (xiced >= XMAXD && XMAXD >= 0) zkapz = 0;
-      const double zflx = xheat + zkapz * xsst;
+      const real zflx = xheat + zkapz * xsst;
       xts = (zcpdt * ztso + zflx) / (zcpdt + zkapz);
       if (xts > TMELT) {
         xqmelt = xheat + zkapz * (xsst - TMELT) - zcpdt * (TMELT - ztso);
@@ -1456,8 +1456,8 @@ static void sea_ice(const M* v) {
       xcflux = xheat;
       xcfluxf = 0;
     } else {
-      const double zhsnowc = 1.0e3 / CRHOSN * xsnow;
-      double zckap = (CKAPSN * zhsnowc + CKAPI * xiced) / (zhsnowc + xiced);
+      const real zhsnowc = 1.0e3 / CRHOSN * xsnow;
+      real zckap = (CKAPSN * zhsnowc + CKAPI * xiced) / (zhsnowc + xiced);
       if (XMAXD >= 0 && xiced >= XMAXD) zckap = 0;
       xcflux = zckap * (xts - xsst) / (zhsnowc + xiced);
       xcfluxf = zckap * (TFREEZE - xsst) / (zhsnowc + xiced);
@@ -1468,7 +1468,7 @@ static void sea_ice(const M* v) {
       xsmflx = xsmflx + xprs * 1000.0 * CLFSN;
       xprs = 0;
     }
-    double zqmelt = xqmelt;
+    real zqmelt = xqmelt;
     if (xiced > 0) {
       if (xqmelt > xsnow * 1000.0 * CLFSN / XDT) {
         xsmelt = xsmelt + xsnow / XDT;
@@ -1489,9 +1489,9 @@ static void sea_ice(const M* v) {
       xsnow = 0;
     }
     if (xls < 0.5 && 1.0e3 * xsnow > (CRHOS - CRHOI) * xiced) {
-      const double zhice = (1.0e3 * xsnow + CRHOI * xiced) / CRHOS;
-      const double zdice = zhice - xiced;
-      const double zdsnow = -zdice * CRHOI / 1.0e3;
+      const real zhice = (1.0e3 * xsnow + CRHOI * xiced) / CRHOS;
+      const real zdice = zhice - xiced;
+      const real zdsnow = -zdice * CRHOI / 1.0e3;
       xsnow = xsnow + zdsnow;
       xscflx = zdsnow * 1.0e3 * CLFSN / XDT;
       xsmelt = xsmelt - zdsnow / XDT;
@@ -1513,16 +1513,16 @@ static void sea_ice(const M* v) {
     if (xiced < zicedold) xicecc = MAX(xicecc + xicecc * (xiced - zicedold) / (2.0 * zicedold), 0.0);
     if (xiced <= 0) xicecc = 0;
     if (XMAXD >= 0 && xiced > XMAXD) {
-      const double zcflux = (xiced - XMAXD) * zrhoilfdt;
+      const real zcflux = (xiced - XMAXD) * zrhoilfdt;
       xiced = XMAXD;
       xcfluxr = xcfluxr + zcflux;
       ximelt = ximelt + zcflux;
       xsndch = xsndch + zcflux * 1000.0 / CRHOI / zrhoilfdt;
     }
-    double zflx = F(ice_zflx)[h];
+    real zflx = F(ice_zflx)[h];
     if (xls < 1.0) zflx = MAX(0.0, XMAXD - xiced) * zrhoilfdt;
     F(ice_zflx)[h] = zflx;
-    const double xgw = F(xgw)[h];
+    const real xgw = F(xgw)[h];
     F(ice_part1)[h] = xls < 1.0 ? xcfluxr * xgw : 0.0;
     F(ice_part2)[h] = xls < 1.0 ? zflx * xgw : 0.0;
     F(xiced)[h] = xiced;
@@ -1540,12 +1540,12 @@ static void sea_ice(const M* v) {
 }
 
 /* physics/ocean/wgsl/oceanstep.wgsl:11-46 */
-static void oceanstep(const M* v, int h, double heat) {
+static void oceanstep(const M* v, int h, real heat) {
   F(yclsst2)[h] = clim_at(v, F(yclsst), h, 3);
-  const double yls = F(yls)[h], yiced = F(xiced)[h], ymld = F(ymld)[h], yfldo = F(yfldo)[h];
-  const double zcpsdt = K(dtmix) / (K(crhos) * K(cps));
-  const double ysst = F(ysst)[h];
-  double zsst = ysst, yiflux = 0;
+  const real yls = F(yls)[h], yiced = F(xiced)[h], ymld = F(ymld)[h], yfldo = F(yfldo)[h];
+  const real zcpsdt = K(dtmix) / (K(crhos) * K(cps));
+  const real ysst = F(ysst)[h];
+  real zsst = ysst, yiflux = 0;
   if (yls < 1.0 && yiced <= 0) zsst = zsst + (heat + yfldo) * zcpsdt / ymld;
   if (yls < 1.0 && yiced > 0) yiflux = yiflux - yfldo - heat;
   if (ysst < K(tfreeze) && yls < 1.0) {
@@ -1553,7 +1553,7 @@ static void oceanstep(const M* v, int h, double heat) {
     zsst = K(tfreeze);
   }
   if (yiced > 0 && yls < 1.0) {
-    const double zflx = (K(tfreeze) - zsst) * ymld / zcpsdt;
+    const real zflx = (K(tfreeze) - zsst) * ymld / zcpsdt;
     zsst = zsst + zflx * zcpsdt / ymld;
     yiflux = yiflux + zflx;
   }
@@ -1565,22 +1565,22 @@ static void oceanstep(const M* v, int h, double heat) {
 
 /* physics/sea-ice/wgsl/finish.wgsl:8-46 */
 static void ice_finish(const M* v, int h) {
-  const double xls = F(xls)[h];
-  double xcflux = F(xcflux)[h], xcfluxr = F(xcfluxr)[h];
-  const double s1 = F(ice_sums)[0], s2 = F(ice_sums)[1];
+  const real xls = F(xls)[h];
+  real xcflux = F(xcflux)[h], xcfluxr = F(xcfluxr)[h];
+  const real s1 = F(ice_sums)[0], s2 = F(ice_sums)[1];
   if (s1 > 0 && s2 > 0 && xls < 1.0) {
-    double zflx = F(ice_zflx)[h];
-    const double zfac = s1 / s2;
+    real zflx = F(ice_zflx)[h];
+    const real zfac = s1 / s2;
     if (zfac <= 1.0) zflx = zflx * zfac;
     xcfluxr = xcfluxr - zflx;
     xcflux = xcflux - zflx;
   }
   F(xcfluxr)[h] = xcfluxr;
-  const double xiced = F(xiced)[h];
-  const double xicec = F(xicec)[h] >= K(thicec) ? 1.0 : 0.0;
-  double xsnow = F(xsnow)[h], xsmelt = F(xsmelt)[h];
+  const real xiced = F(xiced)[h];
+  const real xicec = F(xicec)[h] >= K(thicec) ? 1.0 : 0.0;
+  real xsnow = F(xsnow)[h], xsmelt = F(xsmelt)[h];
   if (xiced <= 0 && xsnow > 0) {
-    const double zcflux = xsnow * 1000.0 * K(clfsn) / K(xdt);
+    const real zcflux = xsnow * 1000.0 * K(clfsn) / K(xdt);
     xsmelt = xsmelt + xsnow / K(xdt);
     F(xsmflx)[h] = F(xsmflx)[h] + zcflux;
     xcflux = xcflux - zcflux;
@@ -1606,8 +1606,8 @@ static void seastep(const M* v) {
     else
       F(sea_naccua)[h] = F(sea_naccua)[h] + 1.0;
     if (!(F(dls)[h] < 0.5)) continue;
-    const double dts = F(xts)[h];
-    const double dicec = F(xicec)[h];
+    const real dts = F(xts)[h];
+    const real dicec = F(xicec)[h];
     if (coupled) {
       F(sea_dsst)[h] = F(xsst)[h];
       F(sea_dts)[h] = dts;
@@ -1621,14 +1621,14 @@ static void seastep(const M* v) {
       F(sea_cheata)[h] = F(dshfl)[h] + LV(dswfl, L, h) + LV(dlwfl, L, h) + F(dlhfl)[h] + F(sea_cheata)[h];
       F(sea_cprsa)[h] = F(dprs)[h] + F(sea_cprsa)[h];
     }
-    const double dp = F(dp)[h];
+    const real dp = F(dp)[h];
     LV(dt, L, h) = dts;
-    double dqs = K(rdbrv) * K(ra1) * exp(K(ra2) * (dts - K(tmelt)) / (dts - K(ra4))) / dp;
+    real dqs = K(rdbrv) * K(ra1) * exp(K(ra2) * (dts - K(tmelt)) / (dts - K(ra4))) / dp;
     dqs = dqs / (1.0 - (1.0 / K(rdbrv) - 1.0) * dqs);
     F(sea_dqs)[h] = dqs;
     LV(dq, L, h) = dqs;
-    const double taux = F(dtaux)[h], tauy = F(dtauy)[h];
-    const double zz0 = MAX(K(charnock) * sqrt(taux * taux + tauy * tauy) * K(gascon) * dts / (K(ga) * dp), K(dz0sea));
+    const real taux = F(dtaux)[h], tauy = F(dtauy)[h];
+    const real zz0 = MAX(K(charnock) * sqrt(taux * taux + tauy * tauy) * K(gascon) * dts / (K(ga) * dp), K(dz0sea));
     F(drhs)[h] = K(drhssea) * (1.0 - dicec) + K(drhsice) * dicec;
     F(dalb)[h] = K(albsea) * (1.0 - dicec) + dicec * MIN(K(albice), 0.5 + 0.025 * (273.0 - dts));
     F(dsalb)[h] = K(doceanalb1) * (1.0 - dicec) + dicec * MIN(K(dicealbmx1), K(dicealbmn1) + 0.025 * (273.0 - dts));
@@ -1641,18 +1641,18 @@ static void seastep(const M* v) {
 static void carbonstep(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
   for (int h = 0; h < NH; h++) {
-    const double precip = (F(dprc)[h] + F(dprl)[h]) * 3.154e9;
+    const real precip = (F(dprc)[h] + F(dprl)[h]) * 3.154e9;
     F(localprecip)[h] = precip;
     F(localavgprecip)[h] = F(localavgprecip)[h] + precip * K(timeweight);
     if (F(carbon_step)[0] == 0.0) continue;
-    const double tsurf = LV(dt, L, h);
+    const real tsurf = LV(dt, L, h);
     if (K(ncarbon) != 0) {
-      double weathering = 0;
+      real weathering = 0;
       if (F(dls)[h] > 0.5 && tsurf >= 273.15) {
-        const double pco2 = F(co2)[0] * F(dp)[h] * 1.0e-5;
+        const real pco2 = F(co2)[0] * F(dp)[h] * 1.0e-5;
         weathering = pow(pco2 / K(co2earth), K(cbeta)) * exp(K(kact) * (tsurf - 288.0)) * pow(precip / K(pearth), 0.65);
         if (K(nsupply) != 0) {
-          const double wmaxp = MAX(K(wmax) / K(vearth), 2.0e-15);
+          const real wmaxp = MAX(K(wmax) / K(vearth), 2.0e-15);
           weathering = wmaxp * (1.0 - exp(-weathering / wmaxp));
         }
       }

@@ -7,34 +7,48 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_LIBS = {"fp64": os.path.join(ROOT, "oracle", "liboracle.so"),
+               "fp32": os.path.join(ROOT, "oracle", "liboracle32.so")}
+CTYPES = {"fp64": (ctypes.c_double, np.float64), "fp32": (ctypes.c_float, np.float32)}
 
 
-class _Member(ctypes.Structure):
-    _fields_ = [
-        ("ntru", ctypes.c_int), ("nlat", ctypes.c_int), ("nlon", ctypes.c_int),
-        ("nlev", ctypes.c_int),
-        ("arena", ctypes.POINTER(ctypes.c_double)),
-        ("tables", ctypes.POINTER(ctypes.c_double)),
-        ("constants", ctypes.POINTER(ctypes.c_double)),
-        ("slot_offsets", ctypes.POINTER(ctypes.c_long)),
-        ("table_offsets", ctypes.POINTER(ctypes.c_long)),
-    ]
+def _member_struct(ct):
+    class _Member(ctypes.Structure):
+        _fields_ = [
+            ("ntru", ctypes.c_int), ("nlat", ctypes.c_int), ("nlon", ctypes.c_int),
+            ("nlev", ctypes.c_int),
+            ("arena", ctypes.POINTER(ct)),
+            ("tables", ctypes.POINTER(ct)),
+            ("constants", ctypes.POINTER(ct)),
+            ("slot_offsets", ctypes.POINTER(ctypes.c_long)),
+            ("table_offsets", ctypes.POINTER(ctypes.c_long)),
+        ]
+    return _Member
 
 
-def oracle_library():
-    if not os.path.exists(ORACLE_LIB):
+_MEMBER = {p: _member_struct(CTYPES[p][0]) for p in CTYPES}
+_LIBS = {}
+
+
+def oracle_library(precision="fp64"):
+    if precision in _LIBS:
+        return _LIBS[precision]
+    path = ORACLE_LIBS[precision]
+    if not os.path.exists(path):
         import subprocess
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
                        capture_output=True)
-    lib = ctypes.CDLL(ORACLE_LIB)
+    lib = ctypes.CDLL(path)
+    ct = CTYPES[precision][0]
+    Mem = _MEMBER[precision]
     for n in ("oracle_slot_names", "oracle_table_names", "oracle_constant_names"):
         getattr(lib, n).restype = ctypes.POINTER(ctypes.c_char_p)
-    lib.oracle_step.argtypes = [ctypes.POINTER(_Member), ctypes.c_int]
-    lib.oracle_physics.argtypes = [ctypes.POINTER(_Member)]
-    D = ctypes.POINTER(ctypes.c_double)
-    lib.oracle_transform_inverse.argtypes = [ctypes.POINTER(_Member), D, D, ctypes.c_int, ctypes.c_int]
-    lib.oracle_transform_forward.argtypes = [ctypes.POINTER(_Member), D, D, ctypes.c_int]
+    lib.oracle_step.argtypes = [ctypes.POINTER(Mem), ctypes.c_int]
+    lib.oracle_physics.argtypes = [ctypes.POINTER(Mem)]
+    P = ctypes.POINTER(ct)
+    lib.oracle_transform_inverse.argtypes = [ctypes.POINTER(Mem), P, P, ctypes.c_int, ctypes.c_int]
+    lib.oracle_transform_forward.argtypes = [ctypes.POINTER(Mem), P, P, ctypes.c_int]
+    _LIBS[precision] = lib
     return lib
 
 
@@ -48,21 +62,24 @@ def _names(fn):
     return out
 
 
-def _dptr(a):
-    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+def _ptr(a):
+    ct = ctypes.c_float if a.dtype == np.float32 else ctypes.c_double
+    return a.ctypes.data_as(ctypes.POINTER(ct))
 
 
 class OracleRun:
     """An FP64 copy of one engine member, stepped by the oracle."""
 
-    def __init__(self, engine, member, table_overrides=None):
-        self.lib = oracle_library()
+    def __init__(self, engine, member, table_overrides=None, precision="fp64"):
+        self.precision = precision
+        self.dtype = CTYPES[precision][1]
+        self.lib = oracle_library(precision)
         self.engine = engine
         self.member = member
         self.dims = engine.dims(member)
         self.nphase, self.nstartup = engine.phases(member)
         self.layout = {name: (off, n) for name, off, n in engine.layout(member)}
-        self.arena = engine.read_table(member, "initial").copy()
+        self.arena = engine.read_table(member, "initial").astype(self.dtype)
         self.table_names = _names(self.lib.oracle_table_names)
         self.const_names = _names(self.lib.oracle_constant_names)
         slot_names = _names(self.lib.oracle_slot_names)
@@ -74,27 +91,27 @@ class OracleRun:
             t = overrides.get(n)
             if t is None:
                 t = engine.read_table(member, n)
-            parts.append(np.asarray(t, dtype=np.float64))
+            parts.append(np.asarray(t, dtype=self.dtype))
             offs.append(at)
             self.table_at[n] = (at, len(t))
             at += len(t)
         self.tables = np.concatenate(parts)
         self.table_off = np.array(offs, dtype=np.int64)
-        self.bm1 = [engine.read_table(member, f"bm1_phase{p}") for p in range(self.nphase)]
+        self.bm1 = [engine.read_table(member, f"bm1_phase{p}").astype(self.dtype) for p in range(self.nphase)]
         self.consts = []
         for p in range(self.nphase):
             k = engine.constants(member, p)
-            self.consts.append(np.array([k[n] for n in self.const_names], dtype=np.float64))
+            self.consts.append(np.array([k[n] for n in self.const_names], dtype=self.dtype))
         self.phase = 0
         self.step_no = None
         self.carbon_interval = int(round(engine.read_table(member, "calendar")[0]))
 
     def _struct(self, consts):
-        m = _Member()
+        m = _MEMBER[self.precision]()
         m.ntru, m.nlat, m.nlon, m.nlev = self.dims
-        m.arena = _dptr(self.arena)
-        m.tables = _dptr(self.tables)
-        m.constants = _dptr(consts)
+        m.arena = _ptr(self.arena)
+        m.tables = _ptr(self.tables)
+        m.constants = _ptr(consts)
         m.slot_offsets = self.slot_off.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
         m.table_offsets = self.table_off.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
         return m
@@ -118,7 +135,7 @@ class OracleRun:
         for name, lo, hi in (("radiation_step", 0, 5), ("surface_step", 5, 12),
                              ("carbon_step", 12, 13), ("snapshot_step", 13, 16)):
             off, n = self.layout[name]
-            self.arena[off:off + n] = vals[lo:hi]
+            self.arena[off:off + n] = vals[lo:hi].astype(self.dtype)
 
     def step(self, steps=1, physics=False):
         for _ in range(steps):
@@ -137,16 +154,16 @@ class OracleRun:
         consts = self.consts[-1]
         m = self._struct(consts)
         nlat, nlon = self.dims[1], self.dims[2]
-        sp = np.ascontiguousarray(sp, dtype=np.float64)
-        out = np.zeros(planes * nlat * nlon)
-        self.lib.oracle_transform_inverse(ctypes.byref(m), _dptr(sp), _dptr(out), planes, 1 if dmu else 0)
+        sp = np.ascontiguousarray(sp, dtype=self.dtype)
+        out = np.zeros(planes * nlat * nlon, dtype=self.dtype)
+        self.lib.oracle_transform_inverse(ctypes.byref(m), _ptr(sp), _ptr(out), planes, 1 if dmu else 0)
         return out
 
     def forward(self, grid, planes):
         consts = self.consts[-1]
         m = self._struct(consts)
         t = self.dims[0]
-        grid = np.ascontiguousarray(grid, dtype=np.float64)
-        out = np.zeros(planes * (t + 1) * (t + 2))
-        self.lib.oracle_transform_forward(ctypes.byref(m), _dptr(grid), _dptr(out), planes)
+        grid = np.ascontiguousarray(grid, dtype=self.dtype)
+        out = np.zeros(planes * (t + 1) * (t + 2), dtype=self.dtype)
+        self.lib.oracle_transform_forward(ctypes.byref(m), _ptr(grid), _ptr(out), planes)
         return out

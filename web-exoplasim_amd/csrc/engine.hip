@@ -108,11 +108,13 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
       inv_plain(TAB_legendre_qi, SLOT_sq, SLOT_gq, 1),
       inv_plain(TAB_legendre_qi, SLOT_st, SLOT_gt, 1),
       inv_plain(TAB_legendre_qi, SLOT_sp, SLOT_gp, 0),
+      // rainstep's mkdqtgp (physics/precipitation/index.ts:103-113)
+      inv_plain(TAB_legendre_qi, SLOT_sqt, SLOT_dqt, 1),
   }));
   s.column(k_physics_in);
+  // parameterizations and the frame out's tendency conversion
   enqueue_physics(e, active);
   // gridpointd's frame out (dynamics/index.ts:321-344)
-  s.levels(k_physics_out);
   s.fwd(jobs_of<FwdJobs, FwdJob>({
       fwd_job(SLOT_gtdt, SLOT_fgtdt, 1),
       fwd_job(SLOT_gudt, SLOT_fgudt, 1),
