@@ -111,6 +111,10 @@ int exo_remove_member(exo_engine *e, int member);
  */
 int exo_step(exo_engine *e, int steps);
 
+/* Publish a snapshot every `every` main-loop steps (0: never), as
+ * GcmRunRequest.publishEvery (runner/types.ts:47-67). */
+int exo_set_publish_every(exo_engine *e, int every);
+
 /* Blocks until queued steps have finished. Stands in for mapAsync. */
 int exo_sync(exo_engine *e);
 

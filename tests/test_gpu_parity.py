@@ -11,8 +11,9 @@ seeded inputs. Two gates:
   verify-dynamics.mjs:56-59, :179-181): per field, rel-RMS(GPU - FP64) <=
   2 x rel-RMS(FP32 emulation - FP64) + 1e-6, where the FP32 emulation is
   the same oracle source compiled with float arithmetic
-  (oracle/liboracle32.so). Threshold physics makes single-precision
-  trajectories diverge, which this rule measures instead of hiding.
+  (oracle/liboracle32.so), both RMS-averaged over every step of the run.
+  Threshold physics makes single-precision trajectories diverge, which
+  this rule measures instead of hiding.
 """
 import numpy as np
 import pytest
@@ -73,20 +74,42 @@ def test_first_step_within_1e5(exo, trunc, levels, minutes, land):
 
 @pytest.mark.parametrize("trunc,levels,minutes,land", CASES)
 def test_steps_within_fp32_envelope(exo, trunc, levels, minutes, land):
-    steps = 6 if trunc < 63 else 4
-    gpu, g, host, o64, o32 = _runs(exo, trunc, levels, minutes, land, steps)
+    """Per field, the RMS over steps 1..N of the GPU's relative error must
+    stay within twice the same RMS of the FP32 emulation's (single-step
+    samples of either fluctuate by 10x as rounding errors cross
+    thresholds)."""
+    steps = 8 if trunc < 63 else 5
+    cfg = exo.default_config(truncation=trunc, levels=levels, timestep_minutes=minutes,
+                             land_fraction=land)
+    gpu = exo.Engine(0, 1)
+    host = exo.Engine(-1, 1)
     try:
+        g = gpu.add_member(cfg)
+        h = host.add_member(cfg)
+        o64 = OracleRun(host, h)
+        o32 = OracleRun(host, h, precision="fp32")
+        names = STATE + PHYSICS
+        eg = {n: [] for n in names}
+        ec = {n: [] for n in names}
+        for _ in range(steps):
+            gpu.step(1)
+            gpu.sync()
+            o64.step(1, physics=True)
+            o32.step(1, physics=True)
+            for n in names:
+                a = gpu.read_field(g, n).astype(np.float64)
+                b = o64.field(n)
+                assert np.all(np.isfinite(a)), n
+                eg[n].append(_rms(a, b))
+                ec[n].append(_rms(o32.field(n).astype(np.float64), b))
         bad = []
-        for name in STATE + PHYSICS:
-            a = gpu.read_field(g, name).astype(np.float64)
-            b = o64.field(name)
-            c = o32.field(name).astype(np.float64)
-            assert np.all(np.isfinite(a)), name
-            ea, ec = _rms(a, b), _rms(c, b)
-            ok = ea <= 2 * ec + 1e-6
-            print(f"T{trunc} L{levels} {steps} steps {name}: GPU {ea:.2e} FP32-emulation {ec:.2e} {'ok' if ok else 'FAIL'}")
+        for n in names:
+            ga = np.sqrt(np.mean(np.square(eg[n])))
+            ca = np.sqrt(np.mean(np.square(ec[n])))
+            ok = ga <= 2 * ca + 1e-6
+            print(f"T{trunc} L{levels} steps 1..{steps} {n}: GPU {ga:.2e} FP32-emulation {ca:.2e} {'ok' if ok else 'FAIL'}")
             if not ok:
-                bad.append((name, ea, ec))
+                bad.append((n, ga, ca))
         assert not bad, bad
     finally:
         gpu.close()

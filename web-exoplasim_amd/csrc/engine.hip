@@ -38,6 +38,8 @@ static int fail(int code, const std::string& message) {
   } while (0)
 
 extern "C" __global__ void k_calcgp(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_pack(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_global_means(const DevMember*, LaunchPlan, double*);
 extern "C" __global__ void k_spectrala(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_spectrald_diffusion(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_spectrald_update(const DevMember*, LaunchPlan);
@@ -407,6 +409,12 @@ int exo_step(exo_engine* h, int steps) {
       }
       exo::Stepper(e, desc).load_step(e.d_params + k * per_step);
       exo::enqueue_step(e, desc);
+      // gcm_publish_step's pack runs after every main-loop step
+      // (stepper/index.ts:256-260); the start-up steps skip it.
+      std::vector<int> main_desc;
+      for (size_t i = 0; i < act.size(); i++)
+        if (phase_at[k][i] >= act[i]->host.startup_steps) main_desc.push_back(desc[i]);
+      if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
     }
     HIP_OR_THROW(hipEventRecord(e.t1, e.stream));
     HIP_OR_THROW(hipGetLastError());
@@ -636,6 +644,203 @@ int exo_step_values(exo_engine* h, int id, int64_t step, double* out16) {
   MemberDev* m = member_of(h, id);
   if (!m || !out16) return fail(EXO_E_ARG, "no member " + std::to_string(id));
   exo::step_values(m->host, step, 0, 0, out16);
+  return EXO_OK;
+}
+
+
+int exo_set_publish_every(exo_engine* h, int every) {
+  if (!h || every < 0) return fail(EXO_E_ARG, "publish interval must be >= 0");
+  h->e.publish_every = every;
+  return EXO_OK;
+}
+
+int exo_read_snapshot(exo_engine* h, int id, float* out, size_t floats) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !out) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  const size_t n = m->host.layout.floats[SLOT_snapshot];
+  if (floats != n) return fail(EXO_E_ARG, "snapshot needs " + std::to_string(n) + " values");
+  int rc = exo_read_field(h, id, "snapshot", out, floats);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(out[i]))
+      return fail(EXO_E_UNSTABLE, "The model became unstable at step " + std::to_string(m->step - 1));
+  return EXO_OK;
+}
+
+int exo_global_means(exo_engine* h, int id, double* tsurf, double* toa) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !tsurf || !toa) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
+  try {
+    HIP_OR_THROW(hipSetDevice(e.device));
+    double* d_out = nullptr;
+    HIP_OR_THROW(hipMallocAsync((void**)&d_out, 2 * sizeof(double), e.stream));
+    std::vector<int> one{m->desc};
+    const LaunchPlan p = exo::Stepper(e, one).plan([](const DevMember&) { return 1; });
+    hipLaunchKernelGGL(k_global_means, dim3(1), dim3(256), 0, e.stream, e.d_desc, p, d_out);
+    double host[2];
+    HIP_OR_THROW(hipMemcpyAsync(host, d_out, sizeof(host), hipMemcpyDeviceToHost, e.stream));
+    HIP_OR_THROW(hipFreeAsync(d_out, e.stream));
+    HIP_OR_THROW(hipStreamSynchronize(e.stream));
+    *tsurf = host[0];
+    *toa = host[1];
+  } catch (const std::exception& ex) {
+    return fail(EXO_E_DEVICE, ex.what());
+  }
+  return EXO_OK;
+}
+
+// ------------------------------------------------------------ checkpoint
+// GPU restart layout of checkpoint/restart/index.ts:11-79: magic, uint32
+// little-endian JSON length, {"fields":[[name,floats],...],"step":N,
+// "carbonStep":M}, zero padding to 4 bytes, then the f32 payload in arena
+// offset order.
+
+static const char kMagic[] = "GCM-WEBGPU-STATE\n";
+
+static std::string checkpoint_header(const MemberDev* m) {
+  std::string j = "{\"fields\":[";
+  for (int s = 0; s < SLOT_COUNT; s++) {
+    if (s) j += ",";
+    j += "[\"";
+    j += exo_slot_name(s);
+    j += "\",";
+    j += std::to_string(m->host.layout.floats[s]);
+    j += "]";
+  }
+  j += "],\"step\":" + std::to_string(m->step) + ",\"carbonStep\":" + std::to_string(m->carbon_count) + "}";
+  return j;
+}
+
+int exo_checkpoint_size(exo_engine* h, int id, size_t* bytes) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !bytes) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  if (m->phase < m->host.startup_steps)
+    return fail(EXO_E_STATE, "checkpoints start after the start-up steps");
+  const size_t head = sizeof(kMagic) - 1 + 4 + checkpoint_header(m).size();
+  *bytes = (head + 3) / 4 * 4 + m->host.layout.total * 4;
+  return EXO_OK;
+}
+
+int exo_checkpoint_save(exo_engine* h, int id, void* out, size_t bytes) {
+  size_t need = 0;
+  int rc = exo_checkpoint_size(h, id, &need);
+  if (rc) return rc;
+  if (!out || bytes != need) return fail(EXO_E_ARG, "checkpoint needs " + std::to_string(need) + " bytes");
+  MemberDev* m = member_of(h, id);
+  const std::string json = checkpoint_header(m);
+  unsigned char* p = static_cast<unsigned char*>(out);
+  std::memset(p, 0, need);
+  const size_t ml = sizeof(kMagic) - 1;
+  std::memcpy(p, kMagic, ml);
+  const uint32_t len = (uint32_t)json.size();
+  for (int b = 0; b < 4; b++) p[ml + b] = (unsigned char)((len >> (8 * b)) & 0xff);
+  std::memcpy(p + ml + 4, json.data(), json.size());
+  const size_t start = (ml + 4 + json.size() + 3) / 4 * 4;
+  Engine& e = h->e;
+  hipSetDevice(e.device);
+  hipError_t err = hipMemcpyAsync(p + start, m->d_arena, m->host.layout.total * 4, hipMemcpyDeviceToHost, e.stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+  if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  return EXO_OK;
+}
+
+// Minimal reader of the header JSON this format writes.
+namespace {
+struct JsonCursor {
+  const char* p;
+  const char* end;
+  void ws() {
+    while (p < end && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) p++;
+  }
+  bool lit(char c) {
+    ws();
+    if (p < end && *p == c) {
+      p++;
+      return true;
+    }
+    return false;
+  }
+  bool str(std::string& out) {
+    ws();
+    if (p >= end || *p != '"') return false;
+    p++;
+    out.clear();
+    while (p < end && *p != '"') {
+      if (*p == '\\') return false;
+      out += *p++;
+    }
+    if (p >= end) return false;
+    p++;
+    return true;
+  }
+  bool num(long long& out) {
+    ws();
+    const char* s = p;
+    if (p < end && (*p == '-' || *p == '+')) p++;
+    while (p < end && *p >= '0' && *p <= '9') p++;
+    if (p == s) return false;
+    out = std::stoll(std::string(s, p));
+    return true;
+  }
+};
+}  // namespace
+
+int exo_checkpoint_restore(exo_engine* h, int id, const void* bytes, size_t size) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !bytes) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  const unsigned char* p = static_cast<const unsigned char*>(bytes);
+  const size_t ml = sizeof(kMagic) - 1;
+  if (size < ml + 4 || std::memcmp(p, kMagic, ml) != 0)
+    return fail(EXO_E_FORMAT, "This checkpoint was not saved by the WebGPU model");
+  const uint32_t len = p[ml] | (p[ml + 1] << 8) | (p[ml + 2] << 16) | ((uint32_t)p[ml + 3] << 24);
+  const char* js = reinterpret_cast<const char*>(p + ml + 4);
+  if (ml + 4 + (size_t)len > size) return fail(EXO_E_FORMAT, "checkpoint header is truncated");
+  static const char* mismatch = "This checkpoint's model state does not match this version of the model";
+  JsonCursor c{js, js + len};
+  std::string key, name;
+  long long step = 0, carbon = 0;
+  bool fields_ok = false;
+  if (!c.lit('{')) return fail(EXO_E_FORMAT, mismatch);
+  do {
+    if (!c.str(key) || !c.lit(':')) return fail(EXO_E_FORMAT, mismatch);
+    if (key == "fields") {
+      if (!c.lit('[')) return fail(EXO_E_FORMAT, mismatch);
+      int s = 0;
+      if (!c.lit(']')) {
+        do {
+          long long n = 0;
+          if (!c.lit('[') || !c.str(name) || !c.lit(',') || !c.num(n) || !c.lit(']'))
+            return fail(EXO_E_FORMAT, mismatch);
+          if (s >= SLOT_COUNT || name != exo_slot_name(s) || (size_t)n != m->host.layout.floats[s])
+            return fail(EXO_E_FORMAT, mismatch);
+          s++;
+        } while (c.lit(','));
+        if (!c.lit(']')) return fail(EXO_E_FORMAT, mismatch);
+      }
+      fields_ok = s == SLOT_COUNT;
+    } else if (key == "step") {
+      if (!c.num(step)) return fail(EXO_E_FORMAT, mismatch);
+    } else if (key == "carbonStep") {
+      if (!c.num(carbon)) return fail(EXO_E_FORMAT, mismatch);
+    } else {
+      return fail(EXO_E_FORMAT, mismatch);
+    }
+  } while (c.lit(','));
+  if (!c.lit('}') || !fields_ok) return fail(EXO_E_FORMAT, mismatch);
+  const size_t start = (ml + 4 + len + 3) / 4 * 4;
+  if (size != start + m->host.layout.total * 4) return fail(EXO_E_FORMAT, "checkpoint payload is truncated");
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
+  hipSetDevice(e.device);
+  hipError_t err = hipMemcpyAsync(m->d_arena, p + start, m->host.layout.total * 4, hipMemcpyHostToDevice, e.stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+  if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  // A restart skips the start-up steps (gcm.worker.ts:110-130).
+  m->step = step;
+  m->carbon_count = carbon;
+  m->phase = m->host.startup_steps;
   return EXO_OK;
 }
 

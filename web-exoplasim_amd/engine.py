@@ -97,6 +97,12 @@ def library():
         "exo_dims": (I, [P, I, ctypes.POINTER(I)]),
         "exo_step_values": (I, [P, I, ctypes.c_int64, D]),
         "exo_last_step_ms": (I, [P, D]),
+        "exo_set_publish_every": (I, [P, I]),
+        "exo_read_snapshot": (I, [P, I, F, S]),
+        "exo_global_means": (I, [P, I, D, D]),
+        "exo_checkpoint_size": (I, [P, I, ctypes.POINTER(S)]),
+        "exo_checkpoint_save": (I, [P, I, ctypes.c_void_p, S]),
+        "exo_checkpoint_restore": (I, [P, I, ctypes.c_void_p, S]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -267,6 +273,33 @@ class Engine:
         _check(self.lib.exo_step_values(self.h, member, step,
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
+
+    def set_publish_every(self, every):
+        _check(self.lib.exo_set_publish_every(self.h, every))
+
+    def read_snapshot(self, member):
+        _, n = self.slot(member, "snapshot")
+        out = np.empty(n, dtype=np.float32)
+        _check(self.lib.exo_read_snapshot(self.h, member,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
+        nhor = self.dims(member)[1] * self.dims(member)[2]
+        return out.reshape(-1, nhor)
+
+    def global_means(self, member):
+        t, f = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.exo_global_means(self.h, member, ctypes.byref(t), ctypes.byref(f)))
+        return t.value, f.value
+
+    def checkpoint(self, member):
+        n = ctypes.c_size_t()
+        _check(self.lib.exo_checkpoint_size(self.h, member, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(self.lib.exo_checkpoint_save(self.h, member, buf, n.value))
+        return buf.raw
+
+    def restore(self, member, data):
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(self.lib.exo_checkpoint_restore(self.h, member, buf, len(data)))
 
     def read_arena(self, member):
         """Every slot, in arena order, as one float32 array."""
