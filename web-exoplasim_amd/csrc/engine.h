@@ -14,9 +14,10 @@
 #include "setup.h"
 #include "transforms.h"
 
-extern "C" __global__ void k_sp2gp(const DevMember*, LaunchPlan, InvJobs);
-extern "C" __global__ void k_gp2fc(const DevMember*, LaunchPlan, FwdJobs);
-extern "C" __global__ void k_fc2sp(const DevMember*, LaunchPlan, LegJobs);
+extern "C" __global__ void k_leg_inv(const DevMember*, LaunchPlan, InvJobs);
+extern "C" __global__ void k_dft_inv(const DevMember*, LaunchPlan, InvJobs);
+extern "C" __global__ void k_dft_fwd(const DevMember*, LaunchPlan, FwdJobs);
+extern "C" __global__ void k_leg_fwd(const DevMember*, LaunchPlan, LegJobs);
 extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan, int);
 extern "C" __global__ void k_mean_mode(const DevMember*, LaunchPlan, int);
 extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
@@ -108,17 +109,18 @@ class Stepper {
     hipLaunchKernelGGL(k_mean_mode, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, slot);
   }
   void inv(const InvJobs& jobs) {
-    const LaunchPlan p = plan([&](const DevMember& d) { return d.nlat * job_planes_total(jobs, d.nlev); });
-    hipLaunchKernelGGL(k_sp2gp, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
+    const LaunchPlan pl = plan([&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
+    hipLaunchKernelGGL(k_leg_inv, dim3(pl.first[pl.n]), dim3(256), 0, e_.stream, e_.d_desc, pl, jobs);
+    const LaunchPlan pd = plan([&](const DevMember& d) { return d.nlat * jobs.n; });
+    hipLaunchKernelGGL(k_dft_inv, dim3(pd.first[pd.n]), dim3(256), 0, e_.stream, e_.d_desc, pd, jobs);
   }
   void fwd(const FwdJobs& jobs) {
-    const LaunchPlan p = plan([&](const DevMember& d) { return d.nlat * job_planes_total(jobs, d.nlev); });
-    hipLaunchKernelGGL(k_gp2fc, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
+    const LaunchPlan p = plan([&](const DevMember& d) { return d.nlat * jobs.n; });
+    hipLaunchKernelGGL(k_dft_fwd, dim3(p.first[p.n]), dim3(256), 0, e_.stream, e_.d_desc, p, jobs);
   }
   void leg(const LegJobs& jobs) {
-    const LaunchPlan p =
-        plan([&](const DevMember& d) { return ((d.ncsp + 63) / 64) * job_planes_total(jobs, d.nlev); });
-    hipLaunchKernelGGL(k_fc2sp, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
+    const LaunchPlan p = plan([&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
+    hipLaunchKernelGGL(k_leg_fwd, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
   }
   void load_step(const float* d_params) {
     const LaunchPlan p = plan([](const DevMember&) { return 1; });
@@ -131,11 +133,11 @@ class Stepper {
 };
 
 
-inline InvJob inv_plain(int table, int src, int dst, int per_level) {
-  return InvJob{INV_PLAIN, table, src, -1, dst, -1, per_level, 0};
+inline InvJob inv_plain(int table, int src, int four, int dst, int per_level) {
+  return InvJob{INV_PLAIN, table, src, -1, four, -1, dst, -1, per_level, 0};
 }
-inline InvJob inv_winds(int d, int z, int u, int v, int plavor) {
-  return InvJob{INV_DV2UV, -1, d, z, u, v, 1, plavor};
+inline InvJob inv_winds(int d, int z, int fu, int fv, int u, int v, int plavor) {
+  return InvJob{INV_DV2UV, -1, d, z, fu, fv, u, v, 1, plavor};
 }
 inline FwdJob fwd_job(int src, int dst, int per_level) { return FwdJob{src, dst, per_level}; }
 inline LegJob leg_plain(int src, int dst, int per_level) { return LegJob{LEG_PLAIN, -1, -1, -1, src, dst, -1, per_level}; }

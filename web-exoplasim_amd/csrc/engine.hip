@@ -81,13 +81,13 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
   Stepper s(e, active);
   // gridpointa (dynamics/index.ts:171-230)
   s.inv(jobs_of<InvJobs, InvJob>({
-      inv_winds(SLOT_sd, SLOT_sz, SLOT_gu, SLOT_gv, 1),
-      inv_plain(TAB_legendre_qi, SLOT_sd, SLOT_gd, 1),
-      inv_plain(TAB_legendre_qi, SLOT_st, SLOT_gt, 1),
-      inv_plain(TAB_legendre_qi, SLOT_sz, SLOT_gz, 1),
-      inv_plain(TAB_legendre_qi, SLOT_sq, SLOT_gq, 1),
-      InvJob{INV_DLAMBDA, TAB_legendre_qi, SLOT_sp, -1, SLOT_gp, SLOT_gpmt, 0, 0},
-      inv_plain(TAB_legendre_qj, SLOT_sp, SLOT_gpj, 0),
+      inv_winds(SLOT_sd, SLOT_sz, SLOT_fgu, SLOT_fgv, SLOT_gu, SLOT_gv, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sd, SLOT_fgd, SLOT_gd, 1),
+      inv_plain(TAB_legendre_qi, SLOT_st, SLOT_fgt, SLOT_gt, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sz, SLOT_fgz, SLOT_gz, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sq, SLOT_fgq, SLOT_gq, 1),
+      InvJob{INV_DLAMBDA, TAB_legendre_qi, SLOT_sp, -1, SLOT_fgp, SLOT_fgpmt, SLOT_gp, SLOT_gpmt, 0, 0},
+      inv_plain(TAB_legendre_qj, SLOT_sp, SLOT_fgpj, SLOT_gpj, 0),
   }));
   s.column(k_calcgp);
   s.fwd(jobs_of<FwdJobs, FwdJob>({
@@ -106,12 +106,12 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
   s.coeff(k_spectrala);
   // gridpointd's frame in (dynamics/index.ts:299-320)
   s.inv(jobs_of<InvJobs, InvJob>({
-      inv_winds(SLOT_sd, SLOT_sz, SLOT_gu, SLOT_gv, 1),
-      inv_plain(TAB_legendre_qi, SLOT_sq, SLOT_gq, 1),
-      inv_plain(TAB_legendre_qi, SLOT_st, SLOT_gt, 1),
-      inv_plain(TAB_legendre_qi, SLOT_sp, SLOT_gp, 0),
+      inv_winds(SLOT_sd, SLOT_sz, SLOT_fgu, SLOT_fgv, SLOT_gu, SLOT_gv, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sq, SLOT_fgq, SLOT_gq, 1),
+      inv_plain(TAB_legendre_qi, SLOT_st, SLOT_fgt, SLOT_gt, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sp, SLOT_fgp, SLOT_gp, 0),
       // rainstep's mkdqtgp (physics/precipitation/index.ts:103-113)
-      inv_plain(TAB_legendre_qi, SLOT_sqt, SLOT_dqt, 1),
+      inv_plain(TAB_legendre_qi, SLOT_sqt, SLOT_fdqt, SLOT_dqt, 1),
   }));
   s.column(k_physics_in);
   // parameterizations and the frame out's tendency conversion
@@ -131,18 +131,18 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
   // spectrald (dynamics/index.ts:239-296)
   s.coeff(k_spectrald_diffusion);
   s.inv(jobs_of<InvJobs, InvJob>({
-      inv_winds(SLOT_sdp, SLOT_szp, SLOT_zu, SLOT_zv, 1),
-      inv_winds(SLOT_zdsdp1, SLOT_zdszp1, SLOT_zdu, SLOT_zdv, 0),
+      inv_winds(SLOT_sdp, SLOT_szp, SLOT_fzu, SLOT_fzv, SLOT_zu, SLOT_zv, 1),
+      inv_winds(SLOT_zdsdp1, SLOT_zdszp1, SLOT_fzdu, SLOT_fzdv, SLOT_zdu, SLOT_zdv, 0),
   }));
   s.mkdheat(1);
   s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
   s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt1, 1)}));
-  s.inv(jobs_of<InvJobs, InvJob>({inv_winds(SLOT_zdsdp2, SLOT_zdszp2, SLOT_zdu, SLOT_zdv, 0)}));
+  s.inv(jobs_of<InvJobs, InvJob>({inv_winds(SLOT_zdsdp2, SLOT_zdszp2, SLOT_fzdu, SLOT_fzdv, SLOT_zdu, SLOT_zdv, 0)}));
   s.mkdheat(2);
   s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdekin, SLOT_fzdekin, 1)}));
   s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdekin, SLOT_zsdef, 1)}));
   s.mean_mode(SLOT_zsdef);
-  s.inv(jobs_of<InvJobs, InvJob>({inv_plain(TAB_legendre_qi, SLOT_zsdef, SLOT_zdekin, 1)}));
+  s.inv(jobs_of<InvJobs, InvJob>({inv_plain(TAB_legendre_qi, SLOT_zsdef, SLOT_fzdekin, SLOT_zdekin, 1)}));
   s.mkdheat(3);
   s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
   s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt2, 1)}));
