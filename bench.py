@@ -1,10 +1,18 @@
 """Benchmark of the MI355X ExoPlaSim step: simulated days per second.
 
-Default workload (N=1): one T63 L10 planet at a 30-minute step in the plain
-FP32 form (48 steps = one simulated day), state resident in HBM. A "step"
-is one model time step of the whole planet. Prints one JSON line.
+Default workload (N=1): the 64-planet mixed ensemble (32 x T21 L5,
+24 x T42 L10, 8 x T63 L10; 45-minute steps; seeded rotation, insolation,
+pressure and land fraction) advanced together, one batched step per
+"step", state resident in HBM. `value` is the aggregate simulated days/s
+of the batch. Beside it, the plain FP32 single-run form (one T63 L10
+planet, 30-minute step) is timed on the same GPU for the batched-vs-plain
+ratio, the dominant kernel's launches are timed with HIP events for the
+roofline figure, and the FP64 CPU oracle is timed on a bounded sample of
+the same workload. Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain|batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload batched|plain] [--probe KERNEL]
+                    [--no-cpu-baseline] [--no-plain]
 """
 import argparse
 import json
@@ -15,54 +23,140 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "simulated days/s at T63 L10, batched vs plain (1 GPU); p50/p99 run latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def _package():
+    import __graft_entry__ as entry
+    return entry._package()
+
+
+def run_gpu(exo, cfgs, steps, warmup, probe):
+    """Steps every config together; returns (wall s, device ms, probe)."""
+    eng = exo.Engine(0, len(cfgs))
+    try:
+        ids = [eng.add_member(c) for c in cfgs]
+        eng.step(warmup)
+        eng.sync()
+        if probe:
+            eng.probe_kernel(probe, 65536)
+        t0 = time.perf_counter()
+        eng.step(steps)
+        eng.sync()
+        wall = time.perf_counter() - t0
+        dev_ms = eng.last_step_ms()
+        probed = eng.probe_result() if probe else None
+        if probe:
+            eng.probe_kernel(None)
+        for i in ids:
+            ts, _ = eng.global_means(i)
+            if not (100.0 < ts < 500.0):
+                raise RuntimeError(f"member {i} left the physical range: mean Ts {ts}")
+        return wall, dev_ms, probed
+    finally:
+        eng.close()
+
+
+def cpu_baseline(exo, cfgs, seconds_per_case):
+    """FP64 oracle steps on the host, one planet of each (truncation,
+    levels) group of the workload, scaled to the workload's aggregate
+    simulated days/s on one core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_harness import OracleRun
+    groups = {}
+    for c in cfgs:
+        groups.setdefault((c.truncation, c.levels, c.timestep_minutes), []).append(c)
+    host = exo.Engine(-1, len(groups))
+    cpu_s_per_day = 0.0
+    days = 0.0
+    sample = []
+    try:
+        for (trunc, levels, minutes), members in groups.items():
+            m = host.add_member(members[0])
+            run = OracleRun(host, m)
+            n = 0
+            t0 = time.perf_counter()
+            while True:
+                run.step(1, physics=True)
+                n += 1
+                el = time.perf_counter() - t0
+                if el >= seconds_per_case or n >= 64:
+                    break
+            per_step = el / n
+            spd = 1440.0 / minutes
+            cpu_s_per_day += per_step * spd * len(members)
+            days += len(members)
+            sample.append(f"T{trunc} L{levels}: {n} steps, {per_step * 1e3:.1f} ms/step")
+    finally:
+        host.close()
+    return {"value": days / cpu_s_per_day, "unit": "simulated days/s", "cores": 1, "kind": "port",
+            "sample": "FP64 C oracle (oracle/liboracle.so), one planet per group of the workload, "
+                      "scaled by group size: " + "; ".join(sample)}
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=480)
-    ap.add_argument("--warmup", type=int, default=48)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
+    ap.add_argument("--steps", type=int, default=320)
+    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every member of the workload runs on the "
-              "one GPU of this box, and no multi-GPU figure is reported", file=sys.stderr)
+        print("bench.py measures one GPU: every member of the workload runs on the one GPU "
+              "of this box, and no multi-GPU figure is reported", file=sys.stderr)
         sys.exit(2)
-    import __graft_entry__ as entry
-    exo = entry._package()
+    exo = _package()
+    from web_exoplasim_amd import workloads, roofline
 
-    cfg = exo.default_config(truncation=63, levels=10, timestep_minutes=30)
-    steps_per_day = 48
-    eng = exo.Engine(0, 1)
-    m = eng.add_member(cfg)
-    eng.step(args.warmup)
-    eng.sync()
-    t0 = time.perf_counter()
-    eng.step(args.steps)
-    eng.sync()
-    wall = time.perf_counter() - t0
-    dev_ms = eng.last_step_ms()
-    ms_per_step = wall * 1000.0 / args.steps
-    days = args.steps / steps_per_day
+    if args.workload == "batched":
+        cfgs = workloads.ensemble(exo)
+        workload = {"workload": "64-planet mixed ensemble: 32 x T21 L5, 24 x T42 L10, 8 x T63 L10, "
+                                "45-min step, seeded parameters (seed 64)",
+                    "members": len(cfgs), "timestep_minutes": 45, "form": "plain FP32"}
+    else:
+        cfgs = [workloads.plain(exo, 63, 10, 30)]
+        workload = {"workload": "plain FP32 single planet T63 L10, 30-min step",
+                    "truncation": 63, "levels": 10, "timestep_minutes": 30, "members": 1}
+    probe = roofline.DOMINANT[args.workload] if args.probe == "auto" else (args.probe or None)
+    wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probe)
+    days = sum(args.steps / workloads.steps_per_day(c.timestep_minutes) for c in cfgs)
+    value = days / wall
     line = {
-        "metric": "simulated days/s at T63 L10, batched vs plain (1 GPU); p50/p99 run latency",
-        "value": days / wall,
+        "metric": METRIC,
+        "value": value,
         "unit": "simulated days/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall * 1000.0 / args.steps,
         "device_ms_per_step": dev_ms / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic seeded planet (SH terrain, isothermal 250 K start)",
-        "config": {"workload": "plain FP32 single planet T63 L10, 30-min step",
-                   "truncation": 63, "levels": 10, "timestep_minutes": 30, "members": 1},
-        "roofline": None,
-        "cpu_baseline": None,
+        "data": "synthetic seeded planets (SH terrain, isothermal 250 K start)",
+        "config": workload,
     }
+    if args.workload == "batched" and not args.no_plain:
+        plain_cfg = workloads.plain(exo, 63, 10, 30)
+        pw, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None)
+        plain_value = (480 / 48) / pw
+        line["plain_single"] = {"value": plain_value, "unit": "simulated days/s",
+                                "ms_per_step": pw * 1000.0 / 480,
+                                "config": "plain FP32 single planet T63 L10, 30-min step"}
+        line["batched_vs_plain"] = value / plain_value
+    if probed:
+        total_ms, launches, missed = probed
+        line["roofline"] = roofline.figure(exo, probe, cfgs, total_ms, launches, missed, args.steps)
+    else:
+        line["roofline"] = None
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(exo, cfgs, args.cpu_seconds)
     print(json.dumps(line))
 
 

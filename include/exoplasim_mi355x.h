@@ -181,8 +181,22 @@ int exo_checkpoint_save(exo_engine *e, int member, void *out, size_t bytes);
 int exo_checkpoint_restore(exo_engine *e, int member, const void *bytes,
                            size_t size);
 
-/* Device-side timing of the last exo_step call (ms, HIP events). */
+/* Device-side timing of the last exo_step call (ms, HIP events). Replaces
+   the timestamp queries of STEPPER.batch (src/model/climate/gcm/stepper/
+   index.ts:175-276). */
 int exo_last_step_ms(exo_engine *e, double *ms);
+
+/* Kernel-duration probe: from this call on, every launch of `kernel` (a
+   kernel name such as "k_physics_a") is bracketed by a pair of HIP events
+   on the engine's stream, for up to max_launches launches (1..65536;
+   later launches are counted in `missed`). kernel = NULL stops it. The
+   reference has no per-kernel timing; this serves the benchmark's
+   roofline figures. */
+int exo_probe_kernel(exo_engine *e, const char *kernel, int max_launches);
+/* Summed duration (ms) and count of the probed launches so far; waits for
+   the last one. */
+int exo_probe_result(exo_engine *e, double *total_ms, int *launches,
+                     int *missed);
 
 #ifdef __cplusplus
 }

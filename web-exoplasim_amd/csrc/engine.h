@@ -21,6 +21,21 @@ extern "C" __global__ void k_leg_fwd(const DevMember*, LaunchPlan, LegJobs);
 extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan, int);
 extern "C" __global__ void k_mean_mode(const DevMember*, LaunchPlan, int);
 extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
+extern "C" __global__ void k_calcgp(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_spectrala(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_spectrald_diffusion(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_spectrald_update(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_physics_in(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_fixer_column(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_fixer_reduce(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_physics_a(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_dca_limit(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_physics_b(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_mkradv(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_physics_c(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_ice_sums(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_physics_d(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_pack(const DevMember*, LaunchPlan);
 
 namespace exo {
 
@@ -43,6 +58,15 @@ struct MemberDev {
   bool active = false;
 };
 
+// Times every launch of one kernel with a pair of HIP events recorded on
+// the engine's stream around it (begin[i], end[i] for launch i).
+struct KernelProbe {
+  const void* kernel = nullptr;
+  std::vector<hipEvent_t> begin, end;
+  size_t used = 0;
+  size_t missed = 0;  // launches beyond the pool
+};
+
 struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -56,7 +80,21 @@ struct Engine {
   hipEvent_t params_done = nullptr, t0 = nullptr, t1 = nullptr;
   bool timed = false;
   int publish_every = 0;
+  KernelProbe probe;
 };
+
+// Every launch of the step goes through here, so the probe can bracket
+// the kernel it times.
+template <typename... Params, typename... Args>
+inline void launch(Engine& e, void (*k)(Params...), dim3 grid, dim3 block, Args... args) {
+  KernelProbe& p = e.probe;
+  const bool timed = p.kernel == reinterpret_cast<const void*>(k);
+  const bool room = timed && p.used < p.begin.size();
+  if (timed && !room) p.missed++;
+  if (room) hipEventRecord(p.begin[p.used], e.stream);
+  hipLaunchKernelGGL(k, grid, block, 0, e.stream, args...);
+  if (room) hipEventRecord(p.end[p.used++], e.stream);
+}
 
 inline int tiles_column(const DevMember& d) { return (d.nhor + 63) / 64; }
 inline int tiles_coeff(const DevMember& d) { return (d.nrsp + 63) / 64; }
@@ -90,41 +128,41 @@ class Stepper {
 
   void column(void (*k)(const DevMember*, LaunchPlan)) {
     const LaunchPlan p = plan(tiles_column);
-    hipLaunchKernelGGL(k, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p);
+    launch(e_, k, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
   }
   void coeff(void (*k)(const DevMember*, LaunchPlan)) {
     const LaunchPlan p = plan(tiles_coeff);
-    hipLaunchKernelGGL(k, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p);
+    launch(e_, k, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
   }
   void levels(void (*k)(const DevMember*, LaunchPlan)) {
     const LaunchPlan p = plan(tiles_levels);
-    hipLaunchKernelGGL(k, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p);
+    launch(e_, k, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
   }
   void mkdheat(int stage) {
     const LaunchPlan p = plan(tiles_levels);
-    hipLaunchKernelGGL(k_mkdheat, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, stage);
+    launch(e_, k_mkdheat, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, stage);
   }
   void mean_mode(int slot) {
     const LaunchPlan p = plan(tiles_lev_coeff);
-    hipLaunchKernelGGL(k_mean_mode, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, slot);
+    launch(e_, k_mean_mode, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, slot);
   }
   void inv(const InvJobs& jobs) {
     const LaunchPlan pl = plan([&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
-    hipLaunchKernelGGL(k_leg_inv, dim3(pl.first[pl.n]), dim3(256), 0, e_.stream, e_.d_desc, pl, jobs);
+    launch(e_, k_leg_inv, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
     const LaunchPlan pd = plan([&](const DevMember& d) { return d.nlat * jobs.n; });
-    hipLaunchKernelGGL(k_dft_inv, dim3(pd.first[pd.n]), dim3(256), 0, e_.stream, e_.d_desc, pd, jobs);
+    launch(e_, k_dft_inv, dim3(pd.first[pd.n]), dim3(256), e_.d_desc, pd, jobs);
   }
   void fwd(const FwdJobs& jobs) {
     const LaunchPlan p = plan([&](const DevMember& d) { return d.nlat * jobs.n; });
-    hipLaunchKernelGGL(k_dft_fwd, dim3(p.first[p.n]), dim3(256), 0, e_.stream, e_.d_desc, p, jobs);
+    launch(e_, k_dft_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
   }
   void leg(const LegJobs& jobs) {
     const LaunchPlan p = plan([&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
-    hipLaunchKernelGGL(k_leg_fwd, dim3(p.first[p.n]), dim3(64), 0, e_.stream, e_.d_desc, p, jobs);
+    launch(e_, k_leg_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
   }
   void load_step(const float* d_params) {
     const LaunchPlan p = plan([](const DevMember&) { return 1; });
-    hipLaunchKernelGGL(k_load_step, dim3(p.n), dim3(64), 0, e_.stream, e_.d_desc, p, d_params);
+    launch(e_, k_load_step, dim3(p.n), dim3(64), e_.d_desc, p, d_params);
   }
 
  private:

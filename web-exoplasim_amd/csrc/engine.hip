@@ -37,14 +37,7 @@ static int fail(int code, const std::string& message) {
       throw exo::DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e));        \
   } while (0)
 
-extern "C" __global__ void k_calcgp(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_pack(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_global_means(const DevMember*, LaunchPlan, double*);
-extern "C" __global__ void k_spectrala(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_spectrald_diffusion(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_spectrald_update(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_physics_in(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_physics_out(const DevMember*, LaunchPlan);
 
 // Writes each member's 16 per-step values into its arena's table slots
 // (the params-to-state copies of STEPPER.batch, stepper/index.ts:242-255).
@@ -228,6 +221,8 @@ void exo_destroy(exo_engine* h) {
   if (e.params_done) hipEventDestroy(e.params_done);
   if (e.t0) hipEventDestroy(e.t0);
   if (e.t1) hipEventDestroy(e.t1);
+  for (hipEvent_t ev : e.probe.begin) hipEventDestroy(ev);
+  for (hipEvent_t ev : e.probe.end) hipEventDestroy(ev);
   if (e.stream) hipStreamDestroy(e.stream);
   delete h;
 }
@@ -445,6 +440,71 @@ int exo_last_step_ms(exo_engine* h, double* ms) {
   hipError_t err = hipEventElapsedTime(&f, h->e.t0, h->e.t1);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
   *ms = f;
+  return EXO_OK;
+}
+
+namespace {
+
+// Kernels the probe can time, by name.
+const std::pair<const char*, const void*> kProbeKernels[] = {
+#define EXO_PROBE(k) {#k, reinterpret_cast<const void*>(k)},
+    EXO_PROBE(k_load_step) EXO_PROBE(k_leg_inv) EXO_PROBE(k_dft_inv) EXO_PROBE(k_dft_fwd) EXO_PROBE(k_leg_fwd)
+    EXO_PROBE(k_calcgp) EXO_PROBE(k_spectrala) EXO_PROBE(k_spectrald_diffusion) EXO_PROBE(k_spectrald_update)
+    EXO_PROBE(k_mkdheat) EXO_PROBE(k_mean_mode) EXO_PROBE(k_physics_in) EXO_PROBE(k_fixer_column)
+    EXO_PROBE(k_fixer_reduce) EXO_PROBE(k_physics_a) EXO_PROBE(k_dca_limit) EXO_PROBE(k_physics_b)
+    EXO_PROBE(k_mkradv) EXO_PROBE(k_physics_c) EXO_PROBE(k_ice_sums) EXO_PROBE(k_physics_d) EXO_PROBE(k_pack)
+#undef EXO_PROBE
+};
+
+}  // namespace
+
+int exo_probe_kernel(exo_engine* h, const char* kernel, int max_launches) {
+  if (!h) return fail(EXO_E_ARG, "null engine");
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "a host-only engine runs no kernels");
+  if (!kernel) {
+    e.probe.kernel = nullptr;
+    return EXO_OK;
+  }
+  if (max_launches <= 0 || max_launches > (1 << 16)) return fail(EXO_E_ARG, "max_launches must be in 1..65536");
+  const void* k = nullptr;
+  for (const auto& entry : kProbeKernels)
+    if (std::strcmp(entry.first, kernel) == 0) k = entry.second;
+  if (!k) return fail(EXO_E_ARG, std::string("no kernel ") + kernel);
+  try {
+    HIP_OR_THROW(hipSetDevice(e.device));
+    HIP_OR_THROW(hipStreamSynchronize(e.stream));
+    while (e.probe.begin.size() < (size_t)max_launches) {
+      hipEvent_t b, t;
+      HIP_OR_THROW(hipEventCreate(&b));
+      HIP_OR_THROW(hipEventCreate(&t));
+      e.probe.begin.push_back(b);
+      e.probe.end.push_back(t);
+    }
+  } catch (const std::exception& ex) {
+    return fail(EXO_E_DEVICE, ex.what());
+  }
+  e.probe.kernel = k;
+  e.probe.used = 0;
+  e.probe.missed = 0;
+  return EXO_OK;
+}
+
+int exo_probe_result(exo_engine* h, double* total_ms, int* launches, int* missed) {
+  if (!h || !total_ms || !launches || !missed) return fail(EXO_E_ARG, "null argument");
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "a host-only engine runs no kernels");
+  double sum = 0;
+  for (size_t i = 0; i < e.probe.used; i++) {
+    float f = 0;
+    hipError_t err = hipEventSynchronize(e.probe.end[i]);
+    if (err == hipSuccess) err = hipEventElapsedTime(&f, e.probe.begin[i], e.probe.end[i]);
+    if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+    sum += f;
+  }
+  *total_ms = sum;
+  *launches = (int)e.probe.used;
+  *missed = (int)e.probe.missed;
   return EXO_OK;
 }
 

@@ -383,22 +383,6 @@ extern "C" __global__ __launch_bounds__(64) void k_physics_in(const DevMember* _
   EXO_LEVEL_DISPATCH(physics_in_column, M, h);
 }
 
-// dynamics/wgsl/physics-out.wgsl:9-21, per level and point.
-extern "C" __global__ __launch_bounds__(64) void k_physics_out(const DevMember* __restrict__ mems, LaunchPlan plan) {
-  int tile;
-  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int i = tile * 64 + threadIdx.x;
-  if (i >= M.nlev * M.nhor) return;
-  const int h = i % M.nhor;
-  float* A = M.arena;
-  const float PSURF = M.k[CON_psurf], CV = M.k[CON_cv], CT = M.k[CON_ct], WW = M.k[CON_ww];
-  const float scale = sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
-  A[M.slot[SLOT_gudt] + i] = A[M.slot[SLOT_dudt] + i] / scale / CV / WW + A[M.slot[SLOT_gudt] + i];
-  A[M.slot[SLOT_gvdt] + i] = A[M.slot[SLOT_dvdt] + i] / scale / CV / WW + A[M.slot[SLOT_gvdt] + i];
-  A[M.slot[SLOT_gtdt] + i] = A[M.slot[SLOT_dtdt] + i] / CT / WW + A[M.slot[SLOT_gtdt] + i];
-  A[M.slot[SLOT_gqdt] + i] = A[M.slot[SLOT_dqdt] + i] * A[M.slot[SLOT_dp] + h] / WW / PSURF + A[M.slot[SLOT_gqdt] + i];
-}
-
 // dynamics/wgsl/mkdheat.wgsl:17-44, per level and point.
 extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __restrict__ mems, LaunchPlan plan,
                                                             int stage) {

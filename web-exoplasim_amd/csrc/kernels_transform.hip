@@ -28,7 +28,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxLat = 96, kMaxK = 64, kMaxLev = 10, kMaxFC = 128, kMaxLon = 192;
 constexpr int kStride = kMaxK + 1;  // odd LDS row stride: no bank conflicts
-constexpr int kLegFwdThreads = 64;  // k_leg_fwd: one wave, lane = mode (K <= kMaxK)
+constexpr int kLegFwdThreads = 256;  // k_leg_fwd: 4 waves, lane = mode (K <= kMaxK = 64)
 
 __device__ __forceinline__ int mode_first(int m, int ntp1) { return m * (2 * ntp1 - m + 1) / 2; }
 
@@ -55,72 +55,53 @@ __device__ __forceinline__ void read_vec(const float* src, float* out) {
   }
 }
 
+// sp2fc / dv2uv of wavenumber m. Lanes own latitudes; the kThreads / nlat
+// lane groups (2 at T63, 4 at T42, 8 at T21) split the modes n
+// (group g sums k = g, g + G, ...), then group 0 adds the other groups'
+// partial sums in group order.
 template <int P>
-__device__ void leg_inv_plain(const DevMember& M, const InvJob& job, int m, float* qa, float* sa) {
+__device__ void leg_inv(const DevMember& M, const InvJob& job, int m, float* qab, float* sa, float* sb) {
   const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
   const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
-  const float* q = M.tab + M.tabo[job.table];
-  const float* S = M.arena + M.slot[job.src0];
+  const bool winds = job.kind == INV_DV2UV;
+  float* qa = qab;
+  float* qb = qab + kMaxLat * kStride;
+  const float* q0 = M.tab + M.tabo[winds ? (int)TAB_legendre_qu : job.table];
+  const float* q1 = M.tab + M.tabo[TAB_legendre_qv];
   for (int i = threadIdx.x; i < nlat * K; i += kThreads) {
     const int lat = i / K, k = i % K;
-    qa[lat * kStride + k] = q[lat * ncsp + w0 + k];
+    qa[lat * kStride + k] = q0[lat * ncsp + w0 + k];
+    if (winds) qb[lat * kStride + k] = q1[lat * ncsp + w0 + k];
   }
-  for (int i = threadIdx.x; i < P * 2 * K; i += kThreads) {
-    const int p = i / (2 * K), r = i % (2 * K);
-    sa[(r >> 1) * SV(P) + 2 * p + (r & 1)] = S[p * nrsp + 2 * w0 + r];
-  }
-  __syncthreads();
-  float* F = M.arena + M.slot[job.four0];
-  for (int lat = threadIdx.x; lat < nlat; lat += kThreads) {
-    float acc[SV(P)];
-#pragma unroll
-    for (int i = 0; i < SV(P); i++) acc[i] = 0.0f;
-    for (int k = 0; k < K; k++) {
-      const float qq = qa[lat * kStride + k];
-      float s[SV(P)];
-      read_vec<P>(sa + k * SV(P), s);
-#pragma unroll
-      for (int i = 0; i < 2 * P; i++) acc[i] += qq * s[i];
-    }
-#pragma unroll
-    for (int p = 0; p < P; p++) {
-      float* f = F + (p * nlat + lat) * nfc + 2 * m;
-      f[0] = acc[2 * p];
-      f[1] = acc[2 * p + 1];
-    }
-  }
-}
-
-template <int P>
-__device__ void leg_inv_winds(const DevMember& M, const InvJob& job, int m, float* qa, float* qb, float* sa,
-                              float* sb) {
-  const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
-  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
-  const float* qu = M.tab + M.tabo[TAB_legendre_qu];
-  const float* qv = M.tab + M.tabo[TAB_legendre_qv];
-  const float* D = M.arena + M.slot[job.src0];
-  const float* Z = M.arena + M.slot[job.src1];
+  const float* S0 = M.arena + M.slot[job.src0];
+  const float* S1 = winds ? M.arena + M.slot[job.src1] : nullptr;
   const float plavor = job.plavor ? M.k[CON_plavor] : 0.0f;
-  for (int i = threadIdx.x; i < nlat * K; i += kThreads) {
-    const int lat = i / K, k = i % K;
-    qa[lat * kStride + k] = qu[lat * ncsp + w0 + k];
-    qb[lat * kStride + k] = qv[lat * ncsp + w0 + k];
-  }
   for (int i = threadIdx.x; i < P * 2 * K; i += kThreads) {
     const int p = i / (2 * K), r = i % (2 * K);
     const int at = (r >> 1) * SV(P) + 2 * p + (r & 1);
-    sa[at] = D[p * nrsp + 2 * w0 + r];
+    sa[at] = S0[p * nrsp + 2 * w0 + r];
     // zeta without the planetary vorticity in Re mode (0, 1)
-    sb[at] = Z[p * nrsp + 2 * w0 + r] - ((w0 + (r >> 1) == 1 && (r & 1) == 0) ? plavor : 0.0f);
+    if (winds) sb[at] = S1[p * nrsp + 2 * w0 + r] - ((w0 + (r >> 1) == 1 && (r & 1) == 0) ? plavor : 0.0f);
   }
   __syncthreads();
-  float* U = M.arena + M.slot[job.four0];
-  float* V = M.arena + M.slot[job.four1];
-  for (int lat = threadIdx.x; lat < nlat; lat += kThreads) {
-    float ure[P], uim[P], vre[P], vim[P];
+  const int G = kThreads / nlat;
+  const int g = threadIdx.x / nlat, lat = threadIdx.x % nlat;
+  const bool live = g < G;
+  // plain: acc[2p + part]; winds: U in acc[2p + part], V in acc[2P + 2p + part]
+  float acc[4 * P];
 #pragma unroll
-    for (int p = 0; p < P; p++) ure[p] = uim[p] = vre[p] = vim[p] = 0.0f;
-    for (int k = 0; k < K; k++) {
+  for (int i = 0; i < 4 * P; i++) acc[i] = 0.0f;
+  const int nacc = winds ? 4 * P : 2 * P;
+  if (live && !winds) {
+    for (int k = g; k < K; k += G) {
+      const float qq = qa[lat * kStride + k];
+      float v[SV(P)];
+      read_vec<P>(sa + k * SV(P), v);
+#pragma unroll
+      for (int i = 0; i < 2 * P; i++) acc[i] += qq * v[i];
+    }
+  } else if (live) {
+    for (int k = g; k < K; k += G) {
       const float u = qa[lat * kStride + k], v = qb[lat * kStride + k];
       float d[SV(P)], z[SV(P)];
       read_vec<P>(sa + k * SV(P), d);
@@ -129,20 +110,44 @@ __device__ void leg_inv_winds(const DevMember& M, const InvJob& job, int m, floa
       for (int p = 0; p < P; p++) {
         const float dre = d[2 * p], dim = d[2 * p + 1];
         const float zre = z[2 * p], zim = z[2 * p + 1];
-        ure[p] = ure[p] + v * zre + u * dim;
-        uim[p] = uim[p] + v * zim - u * dre;
-        vre[p] = vre[p] + u * zim - v * dre;
-        vim[p] = vim[p] - u * zre - v * dim;
+        acc[2 * p] = acc[2 * p] + v * zre + u * dim;
+        acc[2 * p + 1] = acc[2 * p + 1] + v * zim - u * dre;
+        acc[2 * P + 2 * p] = acc[2 * P + 2 * p] + u * zim - v * dre;
+        acc[2 * P + 2 * p + 1] = acc[2 * P + 2 * p + 1] - u * zre - v * dim;
       }
     }
+  }
+  // the table tiles are dead now: their LDS takes the partial sums
+  constexpr int RS = 4 * P + 1;
+  __syncthreads();
+  if (live && g > 0) {
+    float* r = qab + ((g - 1) * nlat + lat) * RS;
+#pragma unroll
+    for (int i = 0; i < 4 * P; i++)
+      if (i < nacc) r[i] = acc[i];
+  }
+  __syncthreads();
+  if (!live || g != 0) return;
+  for (int j = 1; j < G; j++) {
+    const float* r = qab + ((j - 1) * nlat + lat) * RS;
+#pragma unroll
+    for (int i = 0; i < 4 * P; i++)
+      if (i < nacc) acc[i] += r[i];
+  }
+  float* F0 = M.arena + M.slot[job.four0];
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    float* f = F0 + (p * nlat + lat) * nfc + 2 * m;
+    f[0] = acc[2 * p];
+    f[1] = acc[2 * p + 1];
+  }
+  if (winds) {
+    float* F1 = M.arena + M.slot[job.four1];
 #pragma unroll
     for (int p = 0; p < P; p++) {
-      float* fu = U + (p * nlat + lat) * nfc + 2 * m;
-      float* fv = V + (p * nlat + lat) * nfc + 2 * m;
-      fu[0] = ure[p];
-      fu[1] = uim[p];
-      fv[0] = vre[p];
-      fv[1] = vim[p];
+      float* f = F1 + (p * nlat + lat) * nfc + 2 * m;
+      f[0] = acc[2 * P + 2 * p];
+      f[1] = acc[2 * P + 2 * p + 1];
     }
   }
 }
@@ -291,8 +296,11 @@ __device__ void load_column(const DevMember& M, const int* slots, int fields, in
   }
 }
 
+// Partial sums of waves 1..3 of k_leg_fwd, per lane: odd row stride.
+constexpr int kRedStride = 4 * kMaxLev + 1;
+
 template <int P>
-__device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin) {
+__device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin, float* red) {
   const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1), nrsp = M.nrsp;
   const int ncsp = M.ncsp, half = nlat / 2;
   int slots[3];
@@ -310,19 +318,23 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin
   }
   load_column<P>(M, slots, fields, m, fin);
   __syncthreads();
-  // lane k owns mode w0 + k for every level; its table column is read
-  // straight from memory (consecutive lanes, consecutive modes)
-  const int k = threadIdx.x;
-  if (k >= K) return;
+  // lane k owns mode w0 + k for every level; wave wv sums latitude pairs
+  // l = wv, wv + 4, ... Table columns come straight from memory
+  // (consecutive lanes, consecutive modes).
+  const int k = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool live = k < K;
   const float s = (k & 1) ? -1.0f : 1.0f;  // parity of m + n = 2m + k
-  const int w = w0 + k;
+  const int w = w0 + (live ? k : 0);
   auto row = [&](int f, int lat) { return fin + (f * nlat + lat) * SV(P); };
-  if (job.kind == LEG_PLAIN) {
-    const float* q = M.tab + M.tabo[TAB_legendre_qc] + w;
-    float acc[SV(P)];
+  // PLAIN / QTEND: acc[2p + part]; UV2DV: D in acc[2p + part], zeta in
+  // acc[2P + 2p + part]
+  float acc[4 * P];
 #pragma unroll
-    for (int i = 0; i < SV(P); i++) acc[i] = 0.0f;
-    for (int l = 0; l < half; l++) {
+  for (int i = 0; i < 4 * P; i++) acc[i] = 0.0f;
+  const int nacc = job.kind == LEG_UV2DV ? 4 * P : 2 * P;
+  if (live && job.kind == LEG_PLAIN) {
+    const float* q = M.tab + M.tabo[TAB_legendre_qc] + w;
+    for (int l = wv; l < half; l += 4) {
       const float qq = q[l * ncsp];
       float nv[SV(P)], sv[SV(P)];
       read_vec<P>(row(0, l), nv);
@@ -330,21 +342,12 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin
 #pragma unroll
       for (int i = 0; i < 2 * P; i++) acc[i] += qq * (nv[i] + s * sv[i]);
     }
-    float* out = M.arena + M.slot[job.dst0] + 2 * w;
-#pragma unroll
-    for (int p = 0; p < P; p++) {
-      out[p * nrsp] = acc[2 * p];
-      out[p * nrsp + 1] = acc[2 * p + 1];
-    }
-  } else if (job.kind == LEG_UV2DV) {
+  } else if (live && job.kind == LEG_UV2DV) {
     const bool kin = job.ke >= 0;
     const float* qe = M.tab + M.tabo[TAB_legendre_qe] + w;
     const float* qm = M.tab + M.tabo[TAB_legendre_qm] + w;
     const float* qq = M.tab + M.tabo[TAB_legendre_qq] + w;
-    float dre[P], dim[P], zre[P], zim[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) dre[p] = dim[p] = zre[p] = zim[p] = 0.0f;
-    for (int l = 0; l < half; l++) {
+    for (int l = wv; l < half; l += 4) {
       const int so = nlat - 1 - l;
       const float e = qe[l * ncsp], mm = qm[l * ncsp];
       float un[SV(P)], us[SV(P)], vn[SV(P)], vs[SV(P)];
@@ -358,10 +361,7 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin
         read_vec<P>(row(2, l), kn);
         read_vec<P>(row(2, so), ks);
 #pragma unroll
-        for (int p = 0; p < P; p++) {
-          dre[p] = dre[p] + c * (kn[2 * p] + s * ks[2 * p]);
-          dim[p] = dim[p] + c * (kn[2 * p + 1] + s * ks[2 * p + 1]);
-        }
+        for (int i = 0; i < 2 * P; i++) acc[i] = acc[i] + c * (kn[i] + s * ks[i]);
       }
 #pragma unroll
       for (int p = 0; p < P; p++) {
@@ -373,29 +373,17 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin
         const float suim = un[2 * p + 1] + s * us[2 * p + 1];
         const float svre = vn[2 * p] + s * vs[2 * p];
         const float svim = vn[2 * p + 1] + s * vs[2 * p + 1];
-        dre[p] = dre[p] - e * dvre - mm * suim;
-        dim[p] = dim[p] - e * dvim + mm * sure;
-        zre[p] = zre[p] + e * dure - mm * svim;
-        zim[p] = zim[p] + e * duim + mm * svre;
+        acc[2 * p] = acc[2 * p] - e * dvre - mm * suim;
+        acc[2 * p + 1] = acc[2 * p + 1] - e * dvim + mm * sure;
+        acc[2 * P + 2 * p] = acc[2 * P + 2 * p] + e * dure - mm * svim;
+        acc[2 * P + 2 * p + 1] = acc[2 * P + 2 * p + 1] + e * duim + mm * svre;
       }
     }
-    float* o0 = M.arena + M.slot[job.dst0] + 2 * w;
-    float* o1 = M.arena + M.slot[job.dst1] + 2 * w;
-#pragma unroll
-    for (int p = 0; p < P; p++) {
-      o0[p * nrsp] = dre[p];
-      o0[p * nrsp + 1] = dim[p];
-      o1[p * nrsp] = zre[p];
-      o1[p * nrsp + 1] = zim[p];
-    }
-  } else {
+  } else if (live) {
     const float* qe = M.tab + M.tabo[TAB_legendre_qe] + w;
     const float* qc = M.tab + M.tabo[TAB_legendre_qc] + w;
     const float* qm = M.tab + M.tabo[TAB_legendre_qm] + w;
-    float re[P], im[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) re[p] = im[p] = 0.0f;
-    for (int l = 0; l < half; l++) {
+    for (int l = wv; l < half; l += 4) {
       const int so = nlat - 1 - l;
       const float e = qe[l * ncsp], c = qc[l * ncsp], mm = qm[l * ncsp];
       float un[SV(P)], us[SV(P)], vn[SV(P)], vs[SV(P)], sn[SV(P)], ss[SV(P)];
@@ -407,17 +395,42 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin
       read_vec<P>(row(2, so), ss);
 #pragma unroll
       for (int p = 0; p < P; p++) {
-        re[p] = re[p] + e * (vn[2 * p] - s * vs[2 * p]) + c * (sn[2 * p] + s * ss[2 * p]) +
-                mm * (un[2 * p + 1] + s * us[2 * p + 1]);
-        im[p] = im[p] + e * (vn[2 * p + 1] - s * vs[2 * p + 1]) + c * (sn[2 * p + 1] + s * ss[2 * p + 1]) -
-                mm * (un[2 * p] + s * us[2 * p]);
+        acc[2 * p] = acc[2 * p] + e * (vn[2 * p] - s * vs[2 * p]) + c * (sn[2 * p] + s * ss[2 * p]) +
+                     mm * (un[2 * p + 1] + s * us[2 * p + 1]);
+        acc[2 * p + 1] = acc[2 * p + 1] + e * (vn[2 * p + 1] - s * vs[2 * p + 1]) +
+                         c * (sn[2 * p + 1] + s * ss[2 * p + 1]) - mm * (un[2 * p] + s * us[2 * p]);
       }
     }
-    float* out = M.arena + M.slot[job.dst0] + 2 * w;
+  }
+  // waves 1..3 hand their partial sums to wave 0, which adds them in wave
+  // order (a fixed order: bit-reproducible)
+  if (wv > 0 && live) {
+    float* r = red + ((wv - 1) * 64 + k) * kRedStride;
+#pragma unroll
+    for (int i = 0; i < 4 * P; i++)
+      if (i < nacc) r[i] = acc[i];
+  }
+  __syncthreads();
+  if (wv != 0 || !live) return;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const float* r = red + (j * 64 + k) * kRedStride;
+#pragma unroll
+    for (int i = 0; i < 4 * P; i++)
+      if (i < nacc) acc[i] += r[i];
+  }
+  float* o0 = M.arena + M.slot[job.dst0] + 2 * w;
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    o0[p * nrsp] = acc[2 * p];
+    o0[p * nrsp + 1] = acc[2 * p + 1];
+  }
+  if (job.kind == LEG_UV2DV) {
+    float* o1 = M.arena + M.slot[job.dst1] + 2 * w;
 #pragma unroll
     for (int p = 0; p < P; p++) {
-      out[p * nrsp] = re[p];
-      out[p * nrsp + 1] = im[p];
+      o1[p * nrsp] = acc[2 * P + 2 * p];
+      o1[p * nrsp + 1] = acc[2 * P + 2 * p + 1];
     }
   }
 }
@@ -436,21 +449,19 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin
 
 extern "C" __global__ __launch_bounds__(256) void k_leg_inv(const DevMember* __restrict__ mems, LaunchPlan plan,
                                                              InvJobs jobs) {
-  __shared__ float qa[kMaxLat * kStride];
-  __shared__ float qb[kMaxLat * kStride];
+  // qu and qv tiles; after the sums, the partial sums of lane groups 1..7
+  // ((8 - 1) x 32 lanes x (4 x 10 + 1) floats at T21 fits)
+  __shared__ float qab[2 * kMaxLat * kStride];
   __shared__ __align__(16) float sa[kMaxK * SV(kMaxLev)];
   __shared__ __align__(16) float sb[kMaxK * SV(kMaxLev)];
+  static_assert(7 * 32 * (4 * kMaxLev + 1) <= 2 * kMaxLat * kStride, "partial sums exceed the table tiles");
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
   const int ntp1 = M.ntru + 1;
   const int m = tile % ntp1;
   const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
   const int P = job.per_level ? M.nlev : 1;
-#define CALL(PP)                                                    \
-  if (job.kind == INV_DV2UV)                                        \
-    leg_inv_winds<PP>(M, job, m, qa, qb, sa, sb);                   \
-  else                                                              \
-    leg_inv_plain<PP>(M, job, m, qa, sa)
+#define CALL(PP) leg_inv<PP>(M, job, m, qab, sa, sb)
   EXO_PLANES_DISPATCH(P, CALL);
 #undef CALL
 }
@@ -487,13 +498,14 @@ extern "C" __global__ __launch_bounds__(256) void k_dft_fwd(const DevMember* __r
 extern "C" __global__ __launch_bounds__(kLegFwdThreads) void k_leg_fwd(const DevMember* __restrict__ mems, LaunchPlan plan,
                                                             LegJobs jobs) {
   __shared__ __align__(16) float fin[3 * kMaxLat * SV(kMaxLev)];
+  __shared__ float red[3 * 64 * kRedStride];
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
   const int ntp1 = M.ntru + 1;
   const int m = tile % ntp1;
   const LegJob job = jobs.j[job_of(jobs, tile / ntp1)];
   const int P = job.per_level ? M.nlev : 1;
-#define CALL(PP) leg_fwd<PP>(M, job, m, fin)
+#define CALL(PP) leg_fwd<PP>(M, job, m, fin, red)
   EXO_PLANES_DISPATCH(P, CALL);
 #undef CALL
 }

@@ -1844,13 +1844,13 @@ void enqueue_physics(Engine& e, const std::vector<int>& active) {
   Stepper s(e, active);
   const LaunchPlan one = s.plan([](const DevMember&) { return 1; });
   s.column(k_fixer_column);
-  hipLaunchKernelGGL(k_fixer_reduce, dim3(one.n), dim3(256), 0, e.stream, e.d_desc, one);
+  launch(e, k_fixer_reduce, dim3(one.n), dim3(256), e.d_desc, one);
   s.column(k_physics_a);
-  hipLaunchKernelGGL(k_dca_limit, dim3(one.n), dim3(256), 0, e.stream, e.d_desc, one);
+  launch(e, k_dca_limit, dim3(one.n), dim3(256), e.d_desc, one);
   s.column(k_physics_b);
   s.column(k_mkradv);
   s.column(k_physics_c);
-  hipLaunchKernelGGL(k_ice_sums, dim3(one.n), dim3(256), 0, e.stream, e.d_desc, one);
+  launch(e, k_ice_sums, dim3(one.n), dim3(256), e.d_desc, one);
   s.column(k_physics_d);
 }
 

@@ -97,6 +97,8 @@ def library():
         "exo_dims": (I, [P, I, ctypes.POINTER(I)]),
         "exo_step_values": (I, [P, I, ctypes.c_int64, D]),
         "exo_last_step_ms": (I, [P, D]),
+        "exo_probe_kernel": (I, [P, ctypes.c_char_p, I]),
+        "exo_probe_result": (I, [P, D, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "exo_set_publish_every": (I, [P, I]),
         "exo_read_snapshot": (I, [P, I, F, S]),
         "exo_global_means": (I, [P, I, D, D]),
@@ -221,6 +223,16 @@ class Engine:
         v = ctypes.c_double()
         _check(self.lib.exo_last_step_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def probe_kernel(self, kernel, max_launches=4096):
+        """Times each later launch of `kernel` with HIP events (None stops)."""
+        _check(self.lib.exo_probe_kernel(self.h, kernel.encode() if kernel else None, max_launches))
+
+    def probe_result(self):
+        """(total ms, launches, missed launches) of the probed kernel."""
+        t, n, miss = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.exo_probe_result(self.h, ctypes.byref(t), ctypes.byref(n), ctypes.byref(miss)))
+        return t.value, n.value, miss.value
 
     def member_step(self, member):
         v = ctypes.c_int64()

@@ -1,0 +1,80 @@
+"""Algorithmic work of the step's kernels, for roofline figures.
+
+Counts are per model step and member, from the transform plane counts of
+one step (the job lists the step enqueues):
+
+* inverse Legendre (k_leg_inv): 8L+3 scalar planes (sp2fc/sp2fcdmu) and
+  5L dv2uv levels; forward Legendre (k_leg_fwd): 5L+1 scalar planes
+  (fc2sp), L uv2dv levels with kinetic energy, L without, 2L qtend
+  levels. FLOPs use the north/south-symmetric count with an FMA as 2:
+  2 NLAT NCSP per scalar plane, 8 per dv2uv level, 10 per uv2dv+KE level,
+  8 per uv2dv level, 6 per qtend level. Bound: the FP32 vector peak.
+* zonal DFTs (k_dft_inv, k_dft_fwd): 18L+4 inverse and 16L+1 forward rows
+  of every latitude; each row reads or writes NFC Fourier and NLON grid
+  floats once: 4 (NLON + NFC) bytes. Bound: HBM.
+* column physics, first fused pass (k_physics_a: fixer, surflx, vdiff,
+  solang, mko3, swr, lwr, radstep, kuo, mkdca): the distinct arena floats
+  per column its routines read (17L + 14: winds, T, q and their
+  tendencies, radiation and cloud inputs, surface scalars) plus those
+  they write (23L + 17: tendencies, fluxes, heating rates, convective
+  outputs, surface fluxes), counted from the kernel source: 4 (40L + 31)
+  bytes per column, 1,724 B at L10 and 924 B at L5. Bound: HBM.
+"""
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+# The kernel bench.py times by default: the one with the largest share of
+# device time in that workload's rocprofv3 kernel trace.
+DOMINANT = {"batched": "k_physics_a", "plain": "k_physics_a"}
+
+
+def _dims(cfg):
+    trunc, levels = cfg.truncation, cfg.levels
+    nlat = {21: 32, 42: 64, 63: 96}[trunc]
+    nlon = 2 * nlat
+    ntp1 = trunc + 1
+    ncsp = ntp1 * (ntp1 + 1) // 2
+    nfc = 2 * ntp1
+    return nlat, nlon, levels, ncsp, nfc
+
+
+def work_per_step(kernel, cfg):
+    """(units, algorithmic work) of one model step of one member."""
+    nlat, nlon, L, ncsp, nfc = _dims(cfg)
+    if kernel == "k_leg_inv":
+        return "flop", nlat * ncsp * (2 * (8 * L + 3) + 8 * 5 * L)
+    if kernel == "k_leg_fwd":
+        return "flop", nlat * ncsp * (2 * (5 * L + 1) + 10 * L + 8 * L + 6 * 2 * L)
+    if kernel == "k_dft_inv":
+        return "byte", 4 * (nlon + nfc) * nlat * (18 * L + 4)
+    if kernel == "k_dft_fwd":
+        return "byte", 4 * (nlon + nfc) * nlat * (16 * L + 1)
+    if kernel == "k_physics_a":
+        return "byte", 4 * (40 * L + 31) * nlat * nlon
+    raise KeyError(f"no algorithmic work model for {kernel}")
+
+
+LAUNCHES_PER_STEP = {"k_leg_inv": 5, "k_dft_inv": 5, "k_leg_fwd": 5, "k_dft_fwd": 5, "k_physics_a": 1}
+
+
+def figure(exo, kernel, cfgs, total_ms, launches, missed, steps):
+    """The bench line's roofline object for `kernel` over `steps` steps of
+    `cfgs`, from the probed launch durations."""
+    units, per_step = None, 0
+    for c in cfgs:
+        units, w = work_per_step(kernel, c)
+        per_step += w
+    expected = LAUNCHES_PER_STEP[kernel] * steps
+    if launches + missed != expected:
+        raise RuntimeError(f"{kernel}: {launches + missed} launches seen, {expected} expected")
+    per_launch = per_step / LAUNCHES_PER_STEP[kernel]
+    avg_s = total_ms / launches / 1e3
+    if units == "flop":
+        achieved = per_launch / avg_s / 1e12
+        peak, unit, bound = FP32_PEAK_TFLOPS, "TFLOP/s", "mfma"
+    else:
+        achieved = per_launch / avg_s / 1e9
+        peak, unit, bound = HBM_PEAK_GBS, "GB/s", "hbm"
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+            "traffic": None, "kernel": kernel, "avg_launch_us": avg_s * 1e6, "launches": launches,
+            "algorithmic_per_launch": per_launch, "algorithmic_unit": units}
