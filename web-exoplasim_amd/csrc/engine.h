@@ -28,7 +28,10 @@ extern "C" __global__ void k_spectrald_update(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_physics_in(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_fixer_column(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_fixer_reduce(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_physics_a(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_surface_fluxes(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_shortwave(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_longwave(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_convection(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_dca_limit(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_physics_b(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_mkradv(const DevMember*, LaunchPlan);

@@ -2,14 +2,18 @@
 // gfx950, fused one lane per grid column. The reference dispatches 23
 // kernels between the physics frame in and out (fixer 4, fluxstep 2,
 // radstep 5, rainstep 6, surfstep 6); here the per-column routines run
-// back to back inside a few kernels whose boundaries are only where a
-// routine needs every column of the planet first:
+// back to back inside a few kernels whose boundaries are where a routine
+// needs every column of the planet first, or where splitting a pass lets
+// each part run at a higher occupancy than their union:
 //
 //   k_fixer_column   fixer's column pass (physics/fixer/wgsl/column.wgsl)
-//   k_fixer_reduce   latitude rows then the global sum, one workgroup per
-//                    member (fixer/wgsl/row.wgsl, reduction/wgsl/sum.wgsl)
-//   k_physics_a      fixer apply, surflx, vdiff, solang, mko3, swr, lwr,
-//                    radstep, kuo + mkshallow, mkdca's column pass
+//   k_fixer_reduce   latitude rows (one wave per row) then the global sum,
+//                    one workgroup per member (fixer/wgsl/row.wgsl,
+//                    reduction/wgsl/sum.wgsl)
+//   k_surface_fluxes fixer apply, surflx, vdiff
+//   k_shortwave      solang, mko3, swr
+//   k_longwave       lwr, radstep
+//   k_convection     kuo + mkshallow, mkdca's column pass
 //   k_dca_limit      mkdca's global give-up flag (mkdca-limit.wgsl)
 //   k_physics_b      rain column (mkdca apply, mklsp, mkrain, mkclouds),
 //                    soil and snow (land/wgsl/soil.wgsl)
@@ -38,7 +42,7 @@ __device__ __forceinline__ float powi(float x, int n) {
 // --------------------------------------------------------------- fixer
 
 template <int NL>
-__device__ void fixer_column(const DevMember& M, int h) {
+__device__ __forceinline__ void fixer_column(const DevMember& M, int h) {
   // physics/fixer/wgsl/column.wgsl:4-36
   const int NH = M.nhor;
   float neg = 0, pos = 0;
@@ -68,7 +72,7 @@ __device__ void fixer_column(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void fixer_apply(const DevMember& M, int h) {
+__device__ __forceinline__ void fixer_apply(const DevMember& M, int h) {
   // physics/fixer/wgsl/apply.wgsl:4-23
   const int NH = M.nhor;
   const float zzneg = F(fixsum)[0], zzpos = F(fixsum)[1];
@@ -85,7 +89,7 @@ __device__ void fixer_apply(const DevMember& M, int h) {
 // -------------------------------------------------------- surface flux
 
 template <int NL>
-__device__ void surflx(const DevMember& M, int h) {
+__device__ __forceinline__ void surflx(const DevMember& M, int h) {
   // physics/surface-flux/wgsl/surflx.wgsl:30-128
   const int NH = M.nhor, low = NL - 1;
   const float VK = 0.4f, GA = K(ga), GASCON = K(gascon), DELTSEC2 = K(deltsec2);
@@ -158,7 +162,7 @@ __device__ void surflx(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void diffuse(const DevMember& M, float* x, const float* k) {
+__device__ __forceinline__ void diffuse(const DevMember& M, float* x, const float* k) {
   // physics/diffusion/wgsl/implicit.wgsl:4-26
   float ebs[NL];
   const float* ds = T(dsigma);
@@ -176,7 +180,7 @@ __device__ void diffuse(const DevMember& M, float* x, const float* k) {
 }
 
 template <int NL>
-__device__ void diffuse_potential(const DevMember& M, float* x, const float* k, const float* skap) {
+__device__ __forceinline__ void diffuse_potential(const DevMember& M, float* x, const float* k, const float* skap) {
   // physics/diffusion/wgsl/implicit.wgsl:28-47
   float ebs[NL];
   const float* ds = T(dsigma);
@@ -196,7 +200,7 @@ __device__ void diffuse_potential(const DevMember& M, float* x, const float* k, 
 }
 
 template <int NL>
-__device__ void vdiff(const DevMember& M, int h) {
+__device__ __forceinline__ void vdiff(const DevMember& M, int h) {
   // physics/surface-flux/wgsl/vdiff.wgsl:22-120
   const int NH = M.nhor;
   const float DELTSEC2 = K(deltsec2), GA = K(ga), GASCON = K(gascon);
@@ -286,7 +290,7 @@ __device__ void vdiff(const DevMember& M, int h) {
 
 // ----------------------------------------------------------- radiation
 
-__device__ void solang(const DevMember& M, int h) {
+__device__ __forceinline__ void solang(const DevMember& M, int h) {
   // physics/radiation/wgsl/solang.wgsl:10-32
   const float PI = 3.14159265359f, TWOPI = PI + PI;
   const float ZRLON = TWOPI / M.nlon;
@@ -301,7 +305,7 @@ __device__ void solang(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void mko3(const DevMember& M, int h) {
+__device__ __forceinline__ void mko3(const DevMember& M, int h) {
   // physics/radiation/wgsl/mko3.wgsl:15-37
   const int NH = M.nhor;
   const float ZFO3 = 100.0f / 2.14f, GA = K(ga), GASCON = K(gascon);
@@ -330,7 +334,7 @@ __device__ __forceinline__ float vapour_absorption(const DevMember& M, float wv)
 }
 
 template <int NL>
-__device__ void swr(const DevMember& M, int h) {
+__device__ __forceinline__ void swr(const DevMember& M, int h) {
   // physics/radiation/wgsl/swr.wgsl:45-276
   const int NH = M.nhor;
   const float ZERO = 1.0e-6f, ZBETTA = 1.66f, ZMBAR = 1.9f, ZFO3 = 100.0f / 2.14f, ZMU00 = 0.5f;
@@ -507,7 +511,7 @@ __device__ void swr(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void lwr(const DevMember& M, int h) {
+__device__ __forceinline__ void lwr(const DevMember& M, int h) {
   // physics/radiation/wgsl/lwr.wgsl:36-179
   const int NH = M.nhor;
   const float ZERO = 1.0e-6f, ZFH2O = 0.1f, ZFO3 = 100.0f / 2.14f;
@@ -615,7 +619,7 @@ __device__ void lwr(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void radstep(const DevMember& M, int h) {
+__device__ __forceinline__ void radstep(const DevMember& M, int h) {
   // physics/radiation/wgsl/radstep.wgsl:11-32
   const int NH = M.nhor;
   const float GA = K(ga), GASCON = K(gascon);
@@ -653,7 +657,7 @@ struct Parcel {
   bool saturated;
 };
 
-__device__ Parcel condense(const DevMember& M, float t0, float q0, float p) {
+__device__ __forceinline__ Parcel condense(const DevMember& M, float t0, float q0, float p) {
   // physics/precipitation/wgsl/kuo.wgsl:22-47
   const float MOIST = 1.0f / K(rdbrv) - 1.0f;
   float s = sat_capped(M, t0, p);
@@ -677,7 +681,7 @@ __device__ Parcel condense(const DevMember& M, float t0, float q0, float p) {
 }
 
 template <int NL>
-__device__ void kuo(const DevMember& M, int h) {
+__device__ __forceinline__ void kuo(const DevMember& M, int h) {
   // physics/precipitation/wgsl/kuo.wgsl:53-298
   const int NH = M.nhor, NLEM = NL - 1;
   const float DELTSEC2 = K(deltsec2), ALV = K(alv), ACPD = K(acpd), ADV = K(adv);
@@ -867,7 +871,7 @@ __device__ void kuo(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void mkdca(const DevMember& M, int h) {
+__device__ __forceinline__ void mkdca(const DevMember& M, int h) {
   // physics/precipitation/wgsl/mkdca.wgsl:13-97
   const int NH = M.nhor, NLEM = NL - 1;
   const float DELTSEC2 = K(deltsec2);
@@ -941,7 +945,7 @@ __device__ void mkdca(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void mklsp(const DevMember& M, int h) {
+__device__ __forceinline__ void mklsp(const DevMember& M, int h) {
   // physics/precipitation/wgsl/mklsp.wgsl:3-39
   const int NH = M.nhor;
   const float DELTSEC2 = K(deltsec2), MOIST = 1.0f / K(rdbrv) - 1.0f;
@@ -991,7 +995,7 @@ __device__ __forceinline__ float evaporate(const DevMember& M, float* flux, floa
 }
 
 template <int NL>
-__device__ void mkrain(const DevMember& M, int h) {
+__device__ __forceinline__ void mkrain(const DevMember& M, int h) {
   // physics/precipitation/wgsl/mkrain.wgsl:20-95
   const int NH = M.nhor;
   const float DELTSEC2 = K(deltsec2), MOIST = 1.0f / K(rdbrv) - 1.0f;
@@ -1065,7 +1069,7 @@ __device__ void mkrain(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void mkclouds(const DevMember& M, int h) {
+__device__ __forceinline__ void mkclouds(const DevMember& M, int h) {
   // physics/precipitation/wgsl/mkclouds.wgsl:18-86
   const int NH = M.nhor;
   const float ZCLMAX = 1.0f, ZCCA = 0.245f, ZCCB = 0.125f, ZCCMAX = 0.8f, ZCCMIN = 0.05f;
@@ -1136,7 +1140,7 @@ struct SnowLayer {
   float ctop, soilz1, diff1, cap1;
 };
 
-__device__ SnowLayer snow_layer(const DevMember& M, float snowz, float cap0, float diff0, float soilz0) {
+__device__ __forceinline__ SnowLayer snow_layer(const DevMember& M, float snowz, float cap0, float diff0, float soilz0) {
   // physics/land/wgsl/soil.wgsl:63-77
   SnowLayer r;
   const float zsntop = fminf(snowz * 1000.0f / K(rhosnow), K(dztop));
@@ -1148,14 +1152,14 @@ __device__ SnowLayer snow_layer(const DevMember& M, float snowz, float cap0, flo
   return r;
 }
 
-__device__ float surface_temperature(const DevMember& M, SnowLayer layer, float dtsm, float flux, float soil1) {
+__device__ __forceinline__ float surface_temperature(const DevMember& M, SnowLayer layer, float dtsm, float flux, float soil1) {
   // physics/land/wgsl/soil.wgsl:84-91
   const float conduction = 2.0f * layer.diff1;
   return (layer.ctop * K(dztop) * dtsm / K(deltsec) + flux + conduction * soil1 / layer.soilz1) /
          (layer.ctop * K(dztop) / K(deltsec) + conduction / layer.soilz1);
 }
 
-__device__ void mktsoil(const DevMember& M, int h, float ftop, const float* soilz, const float* cap,
+__device__ __forceinline__ void mktsoil(const DevMember& M, int h, float ftop, const float* soilz, const float* cap,
                         const float* diff) {
   // physics/land/wgsl/soil.wgsl:23-55
   const int NH = M.nhor, NS = 5, n = NS - 1;
@@ -1183,7 +1187,7 @@ __device__ void mktsoil(const DevMember& M, int h, float ftop, const float* soil
 }
 
 template <int NL>
-__device__ void soil(const DevMember& M, int h) {
+__device__ __forceinline__ void soil(const DevMember& M, int h) {
   // physics/land/wgsl/soil.wgsl:95-185
   const int NH = M.nhor;
   const float DELTSEC = K(deltsec), TMELT = K(tmelt), ALS = K(als), ALV = K(alv);
@@ -1277,7 +1281,7 @@ __device__ __forceinline__ void eastward(const DevMember& M, int row, int lon, f
   *into = u * water / F(darea)[next];
 }
 
-__device__ void mkradv(const DevMember& M, int h) {
+__device__ __forceinline__ void mkradv(const DevMember& M, int h) {
   // physics/land/wgsl/mkradv.wgsl:24-59; each lane writes only its own
   // drunoff and reads only driver, so the update needs no staging.
   const int NLON = M.nlon;
@@ -1331,7 +1335,7 @@ __device__ __forceinline__ float glacier_albedo(const DevMember& M, float mx, fl
 }
 
 template <int NL>
-__device__ void land_surface(const DevMember& M, int h) {
+__device__ __forceinline__ void land_surface(const DevMember& M, int h) {
   // physics/land/wgsl/surface.wgsl:67-142 (vegstep and glacierstep are
   // inert here: nveg = 0, nglacier = 0)
   const int NH = M.nhor;
@@ -1401,7 +1405,7 @@ __device__ void land_surface(const DevMember& M, int h) {
 // --------------------------------------------------- sea ice and ocean
 
 template <int NL>
-__device__ void sea_ice(const DevMember& M, int h) {
+__device__ __forceinline__ void sea_ice(const DevMember& M, int h) {
   // physics/sea-ice/wgsl/ice.wgsl:36-225
   const int NH = M.nhor;
   const float XDT = K(xdt), XMIND = K(xmind), XMAXD = K(xmaxd), CICEMIN = K(cicemin);
@@ -1542,7 +1546,7 @@ __device__ void sea_ice(const DevMember& M, int h) {
   F(xsndch)[h] = xsndch;
 }
 
-__device__ void oceanstep(const DevMember& M, int h, float heat) {
+__device__ __forceinline__ void oceanstep(const DevMember& M, int h, float heat) {
   // physics/ocean/wgsl/oceanstep.wgsl:11-46
   F(yclsst2)[h] = clim_at(M, F(yclsst), h, 3);
   const float yls = F(yls)[h], yiced = F(xiced)[h], ymld = F(ymld)[h], yfldo = F(yfldo)[h];
@@ -1566,7 +1570,7 @@ __device__ void oceanstep(const DevMember& M, int h, float heat) {
   F(xoheat)[h] = -yiflux;
 }
 
-__device__ void ice_finish(const DevMember& M, int h) {
+__device__ __forceinline__ void ice_finish(const DevMember& M, int h) {
   // physics/sea-ice/wgsl/finish.wgsl:8-46
   const float xls = F(xls)[h];
   float xcflux = F(xcflux)[h], xcfluxr = F(xcfluxr)[h];
@@ -1600,7 +1604,7 @@ __device__ void ice_finish(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void seastep(const DevMember& M, int h) {
+__device__ __forceinline__ void seastep(const DevMember& M, int h) {
   // physics/sea/wgsl/seastep.wgsl:32-75
   const int NH = M.nhor;
   const bool coupled = K(ice_coupled) != 0;
@@ -1641,7 +1645,7 @@ __device__ void seastep(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void carbonstep(const DevMember& M, int h) {
+__device__ __forceinline__ void carbonstep(const DevMember& M, int h) {
   // physics/carbon/wgsl/carbonstep.wgsl:22-50
   const int NH = M.nhor;
   const float precip = (F(dprc)[h] + F(dprl)[h]) * 3.154e9f;
@@ -1666,7 +1670,7 @@ __device__ void carbonstep(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void physics_out(const DevMember& M, int h) {
+__device__ __forceinline__ void physics_out(const DevMember& M, int h) {
   // dynamics/wgsl/physics-out.wgsl:9-21, this column's levels
   const int NH = M.nhor;
   const float PSURF = K(psurf), CV = K(cv), CT = K(ct), WW = K(ww);
@@ -1681,22 +1685,37 @@ __device__ void physics_out(const DevMember& M, int h) {
   }
 }
 
+// The first physics pass runs as four column kernels, so that each gets
+// the registers of its own routines (one fused kernel holds the union and
+// runs one wave per SIMD).
 template <int NL>
-__device__ void physics_a(const DevMember& M, int h) {
+__device__ __forceinline__ void surface_fluxes(const DevMember& M, int h) {
   fixer_apply<NL>(M, h);
   surflx<NL>(M, h);
   vdiff<NL>(M, h);
+}
+
+template <int NL>
+__device__ __forceinline__ void shortwave(const DevMember& M, int h) {
   solang(M, h);
   if (K(ozone) != 0) mko3<NL>(M, h);
   swr<NL>(M, h);
+}
+
+template <int NL>
+__device__ __forceinline__ void longwave(const DevMember& M, int h) {
   lwr<NL>(M, h);
   radstep<NL>(M, h);
+}
+
+template <int NL>
+__device__ __forceinline__ void convection(const DevMember& M, int h) {
   kuo<NL>(M, h);
   mkdca<NL>(M, h);
 }
 
 template <int NL>
-__device__ void physics_b(const DevMember& M, int h) {
+__device__ __forceinline__ void physics_b(const DevMember& M, int h) {
   // physics/precipitation/wgsl/column.wgsl:12-27, then soilstep
   const int NH = M.nhor;
   if (F(dca_skip)[0] == 0.0f) {
@@ -1713,13 +1732,13 @@ __device__ void physics_b(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ void physics_c(const DevMember& M, int h) {
+__device__ __forceinline__ void physics_c(const DevMember& M, int h) {
   land_surface<NL>(M, h);
   if (K(ice_coupled) != 0) sea_ice<NL>(M, h);
 }
 
 template <int NL>
-__device__ void physics_d(const DevMember& M, int h) {
+__device__ __forceinline__ void physics_d(const DevMember& M, int h) {
   seastep<NL>(M, h);
   carbonstep<NL>(M, h);
   physics_out<NL>(M, h);
@@ -1728,7 +1747,7 @@ __device__ void physics_d(const DevMember& M, int h) {
 // Deterministic sum of two grid fields of one member by one workgroup:
 // each lane sums a strided share in order, then a fixed tree
 // (reduction/wgsl/sum.wgsl:12-34).
-__device__ void block_sum2(const DevMember& M, const float* a, const float* b, float* out) {
+__device__ __forceinline__ void block_sum2(const DevMember& M, const float* a, const float* b, float* out) {
   __shared__ float sa[256], sb[256];
   float x = 0, y = 0;
   for (int h = threadIdx.x; h < M.nhor; h += 256) {
@@ -1758,9 +1777,11 @@ __device__ void block_sum2(const DevMember& M, const float* a, const float* b, f
 
 }  // namespace
 
-#define EXO_COLUMN_KERNEL(kname, fn)                                                           \
-  extern "C" __global__ __launch_bounds__(64) void kname(const DevMember* __restrict__ mems,   \
-                                                          LaunchPlan plan) {                   \
+// One lane per column; `waves` is the occupancy the register allocation
+// must allow (waves per SIMD).
+#define EXO_COLUMN_KERNEL_W(kname, fn, waves)                                                  \
+  extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves)))      \
+  void kname(const DevMember* __restrict__ mems, LaunchPlan plan) {                            \
     int tile;                                                                                  \
     const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];                             \
     const int h = tile * 64 + threadIdx.x;                                                     \
@@ -1770,9 +1791,13 @@ __device__ void block_sum2(const DevMember& M, const float* a, const float* b, f
     else                                                                                       \
       fn<5>(M, h);                                                                             \
   }
+#define EXO_COLUMN_KERNEL(kname, fn) EXO_COLUMN_KERNEL_W(kname, fn, 1)
 
 EXO_COLUMN_KERNEL(k_fixer_column, fixer_column)
-EXO_COLUMN_KERNEL(k_physics_a, physics_a)
+EXO_COLUMN_KERNEL_W(k_surface_fluxes, surface_fluxes, 2)
+EXO_COLUMN_KERNEL(k_shortwave, shortwave)
+EXO_COLUMN_KERNEL(k_longwave, longwave)
+EXO_COLUMN_KERNEL(k_convection, convection)
 EXO_COLUMN_KERNEL(k_physics_b, physics_b)
 EXO_COLUMN_KERNEL(k_physics_c, physics_c)
 EXO_COLUMN_KERNEL(k_physics_d, physics_d)
@@ -1794,16 +1819,24 @@ extern "C" __global__ __launch_bounds__(256) void k_fixer_reduce(const DevMember
   float* fixq = M.arena + M.slot[SLOT_fixq];
   float* fixneg = M.arena + M.slot[SLOT_fixneg];
   float* fixpos = M.arena + M.slot[SLOT_fixpos];
-  for (int row = threadIdx.x; row < M.nlat; row += 256) {
+  // one wave per latitude row: lanes sum strided shares, then a fixed
+  // butterfly (the same order on every run)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int row = wave; row < M.nlat; row += 4) {
     const int first = row * NLON;
-    float zneg = 0;
-    for (int j = 0; j < NLON; j++) zneg += fixneg[first + j];
-    if (zneg <= 0) continue;
-    float zpos = 0;
-    for (int j = 0; j < NLON; j++) zpos += fixpos[first + j];
-    if (zpos < zneg) continue;
+    float zneg = 0, zpos = 0;
+    for (int j = lane; j < NLON; j += 64) {
+      zneg += fixneg[first + j];
+      zpos += fixpos[first + j];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      zneg += __shfl_xor(zneg, o);
+      zpos += __shfl_xor(zpos, o);
+    }
+    if (zneg <= 0 || zpos < zneg) continue;
     const float factor = (zpos - zneg) / zpos;
-    for (int j = 0; j < NLON; j++) {
+    for (int j = lane; j < NLON; j += 64) {
       for (int l = 0; l < L; l++) fixq[l * NH + first + j] = fmaxf(fixq[l * NH + first + j] * factor, 0.0f);
       fixneg[first + j] = 0;
       fixpos[first + j] = fixpos[first + j] * factor;
@@ -1845,7 +1878,10 @@ void enqueue_physics(Engine& e, const std::vector<int>& active) {
   const LaunchPlan one = s.plan([](const DevMember&) { return 1; });
   s.column(k_fixer_column);
   launch(e, k_fixer_reduce, dim3(one.n), dim3(256), e.d_desc, one);
-  s.column(k_physics_a);
+  s.column(k_surface_fluxes);
+  s.column(k_shortwave);
+  s.column(k_longwave);
+  s.column(k_convection);
   launch(e, k_dca_limit, dim3(one.n), dim3(256), e.d_desc, one);
   s.column(k_physics_b);
   s.column(k_mkradv);

@@ -12,20 +12,23 @@ one step (the job lists the step enqueues):
 * zonal DFTs (k_dft_inv, k_dft_fwd): 18L+4 inverse and 16L+1 forward rows
   of every latitude; each row reads or writes NFC Fourier and NLON grid
   floats once: 4 (NLON + NFC) bytes. Bound: HBM.
-* column physics, first fused pass (k_physics_a: fixer, surflx, vdiff,
-  solang, mko3, swr, lwr, radstep, kuo, mkdca): the distinct arena floats
-  per column its routines read (17L + 14: winds, T, q and their
-  tendencies, radiation and cloud inputs, surface scalars) plus those
-  they write (23L + 17: tendencies, fluxes, heating rates, convective
-  outputs, surface fluxes), counted from the kernel source: 4 (40L + 31)
-  bytes per column, 1,724 B at L10 and 924 B at L5. Bound: HBM.
+* column physics (HBM): the distinct arena floats per column a kernel's
+  routines read from outside the kernel plus those they write, counted
+  from the kernel source, 4 bytes each:
+    k_surface_fluxes (fixer apply, surflx, vdiff): reads 10L + 7, writes
+      6L + 11 -> 16L + 18 floats;
+    k_shortwave (solang, mko3, swr): reads 5L + 7, writes 4L + 7 ->
+      9L + 14;
+    k_longwave (lwr, radstep): reads 8L + 4, writes 8L + 4 -> 16L + 8;
+    k_convection (kuo, mkshallow, mkdca's column pass): reads 5L + 3,
+      writes 8L + 2 -> 13L + 5.
 """
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 # The kernel bench.py times by default: the one with the largest share of
 # device time in that workload's rocprofv3 kernel trace.
-DOMINANT = {"batched": "k_physics_a", "plain": "k_physics_a"}
+DOMINANT = {"batched": "k_leg_inv", "plain": "k_leg_inv"}
 
 
 def _dims(cfg):
@@ -49,12 +52,15 @@ def work_per_step(kernel, cfg):
         return "byte", 4 * (nlon + nfc) * nlat * (18 * L + 4)
     if kernel == "k_dft_fwd":
         return "byte", 4 * (nlon + nfc) * nlat * (16 * L + 1)
-    if kernel == "k_physics_a":
-        return "byte", 4 * (40 * L + 31) * nlat * nlon
+    column = {"k_surface_fluxes": 16 * L + 18, "k_shortwave": 9 * L + 14, "k_longwave": 16 * L + 8,
+              "k_convection": 13 * L + 5}
+    if kernel in column:
+        return "byte", 4 * column[kernel] * nlat * nlon
     raise KeyError(f"no algorithmic work model for {kernel}")
 
 
-LAUNCHES_PER_STEP = {"k_leg_inv": 5, "k_dft_inv": 5, "k_leg_fwd": 5, "k_dft_fwd": 5, "k_physics_a": 1}
+LAUNCHES_PER_STEP = {"k_leg_inv": 5, "k_dft_inv": 5, "k_leg_fwd": 5, "k_dft_fwd": 5, "k_surface_fluxes": 1,
+                     "k_shortwave": 1, "k_longwave": 1, "k_convection": 1}
 
 
 def figure(exo, kernel, cfgs, total_ms, launches, missed, steps):
