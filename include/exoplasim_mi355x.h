@@ -120,6 +120,10 @@ int exo_sync(exo_engine *e);
 
 /* Model step counter of a member (nstep, counted from the start step). */
 int exo_member_step(exo_engine *e, int member, int64_t *step);
+/* Calendar date {year, month, day, hour, minute} of the member's step
+   counter, as GcmSnapshot.date (CALENDAR date, calendar/date/index.ts:
+   46-82). */
+int exo_member_date(exo_engine *e, int member, int *date5);
 
 /* Arena layout: total floats and one slot's offset/length by name.
  * LAYOUT.create / LAYOUT.constants (layout/index.ts:22-53). */
@@ -187,7 +191,7 @@ int exo_checkpoint_restore(exo_engine *e, int member, const void *bytes,
 int exo_last_step_ms(exo_engine *e, double *ms);
 
 /* Kernel-duration probe: from this call on, every launch of `kernel` (a
-   kernel name such as "k_physics_a") is bracketed by a pair of HIP events
+   kernel name such as "k_leg_inv") is bracketed by a pair of HIP events
    on the engine's stream, for up to max_launches launches (1..65536;
    later launches are counted in `missed`). kernel = NULL stops it. The
    reference has no per-kernel timing; this serves the benchmark's

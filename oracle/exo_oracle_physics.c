@@ -5,6 +5,10 @@
  * (stepper/index.ts:148-161): fixer, fluxstep, radstep, rainstep,
  * surfstep, carbonstep. The reference's f32 "precise_*" helpers
  * (shared/gpu/math/wgsl/precise.wgsl) are libm's log/exp/pow here.
+ *
+ * Parity unpinned: no independent check (closed form, library routine or
+ * reference golden vector) pins these routines; they are held only to the
+ * GPU kernels written from the same reading of the reference WGSL.
  */
 #define EXO_ORACLE_INTERNAL
 #include <tgmath.h>

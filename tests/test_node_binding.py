@@ -1,0 +1,90 @@
+"""The Node-API binding (web-exoplasim_amd/node): it loads, exports the
+engine calls, maps configs and errors as the C-ABI does, and GcmRun
+drives a GPU run through snapshot, checkpoint and stop."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE_DIR = os.path.join(ROOT, "web-exoplasim_amd", "node")
+ADDON = os.path.join(NODE_DIR, "exoplasim_napi.node")
+
+pytestmark = pytest.mark.skipif(shutil.which("node") is None or not os.path.exists(ADDON),
+                                reason="node or the built addon is absent")
+
+
+def _node(script, timeout=120):
+    proc = subprocess.run(["node", "-e", script], capture_output=True, text=True, timeout=timeout,
+                          cwd=NODE_DIR)
+    assert proc.returncode == 0, proc.stderr
+    return json.loads(proc.stdout.strip().splitlines()[-1])
+
+
+def test_addon_exports_and_host_engine():
+    out = _node("""
+const a = require('./exoplasim_napi.node')
+const e = a.create(-1, 2)
+const id = a.addMember(e, {truncation: 42, levels: 10, land_fraction: 0.3}, a.PRECISION_FP32)
+const errors = []
+for (const bad of [{truncation: 22}, {bogus: 1}, {levels: 'x'}]) {
+  try { a.addMember(e, bad, 0) } catch (x) { errors.push(x.message) }
+}
+try { a.step(e, 1) } catch (x) { errors.push('step: ' + x.message) }
+const r = {keys: Object.keys(a), dims: a.dims(e, id), date: a.memberDate(e, id),
+           defaults: a.configDefault(), errors}
+a.destroy(e)
+console.log(JSON.stringify(r))
+""")
+    for name in ("create", "addMember", "step", "sync", "readSnapshot", "readField", "globalMeans",
+                 "checkpoint", "restore", "memberDate"):
+        assert name in out["keys"]
+    assert out["dims"] == [42, 64, 128, 10]
+    assert out["date"] == {"year": 1, "month": 1, "day": 1, "hour": 0, "minute": 0}
+    assert out["defaults"]["truncation"] == 21
+    assert out["errors"][0] == "T22 is not built"
+    assert out["errors"][1] == "unknown config field bogus"
+    assert out["errors"][2] == "levels must be a number"
+    assert len(out["errors"]) == 4
+
+
+def test_engine_config_mapping():
+    out = _node("""
+const {engineConfig} = require('./runner.js')
+const c = engineConfig({truncation: 63, layers: 5, timestepMinutes: 30, radiusEarth: 1.5, gravityMs2: 9.0,
+  rotationPeriodDays: 2, fluxWm2: 1000, obliquityDeg: 10, eccentricity: 0.1, yearDays: 200, oceanDepthM: 20,
+  atmosphere: {ozone: false, pressureBar: 2, gasConstant: null}}, {landFraction: 0.2, terrainSeed: 9})
+console.log(JSON.stringify(c))
+""")
+    assert out == {"truncation": 63, "levels": 5, "timestep_minutes": 30, "radius_earth": 1.5, "gravity": 9.0,
+                   "rotation_period_days": 2, "flux_wm2": 1000, "obliquity_deg": 10, "eccentricity": 0.1,
+                   "year_days": 200, "ocean_depth_m": 20, "ozone": 0, "pressure_bar": 2, "land_fraction": 0.2,
+                   "terrain_seed": 9}
+
+
+@pytest.mark.gpu
+def test_gcm_run_snapshots_checkpoint_stop():
+    out = _node("""
+const {start} = require('./runner.js')
+const statuses = [], snaps = []
+let restart = null
+const run = start({config: {truncation: 21, layers: 10, timestepMinutes: 45}, publishEvery: 4,
+  restart: null, startPaused: false,
+  onStatus: (u) => { statuses.push(u.status); if (u.status === 'finished' || u.status === 'failed') done(u) },
+  onDevice: () => {},
+  onSnapshot: (s) => { snaps.push({step: s.step, n: s.values.length, finite: s.values.every(Number.isFinite),
+                                   date: s.date}); if (snaps.length === 3) run.checkpoint() },
+  onCheckpoint: (c) => { restart = c.restart.byteLength; run.stop() }})
+function done(u) {
+  console.log(JSON.stringify({statuses, snaps, restart, message: u.message}))
+}
+""", timeout=300)
+    assert out["statuses"][:2] == ["loading", "running"], out
+    assert out["statuses"][-1] == "finished", out
+    assert "checkpointing" in out["statuses"]
+    assert len(out["snaps"]) >= 3
+    assert all(s["finite"] for s in out["snaps"])
+    assert out["snaps"][0]["n"] == 101 * 2048
+    assert out["restart"] > 2429594 * 4

@@ -2,6 +2,9 @@
 csrc/ compiled for gfx950 with hipcc, then linked into one shared library
 whose exported symbols are the C-ABI of include/exoplasim_mi355x.h.
 
+When Node's headers are present (/usr/include/node/node_api.h), it also
+builds the Node-API binding node/exoplasim_napi.node over that library.
+
 Usage: python web-exoplasim_amd/build.py [--jobs N]
 """
 import argparse
@@ -62,7 +65,31 @@ def build(jobs=8, verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
+    addon = build_node_addon()
+    if verbose and addon:
+        print(f"built {addon}")
     return LIB
+
+
+NODE_INCLUDE = "/usr/include/node"
+ADDON_SRC = os.path.join(HERE, "node", "exoplasim_napi.cc")
+ADDON = os.path.join(HERE, "node", "exoplasim_napi.node")
+
+
+def build_node_addon():
+    """The Node-API binding, linked against the library with an rpath to
+    it; skipped (returns None) where Node's headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    tmp = ADDON + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall",
+           "-DNODE_GYP_MODULE_NAME=exoplasim_napi", f"-I{NODE_INCLUDE}", ADDON_SRC,
+           "-o", tmp, f"-L{HERE}", "-lexoplasim_mi355x", "-Wl,-rpath,$ORIGIN/.."]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("Node addon build failed:\n" + proc.stderr)
+    os.replace(tmp, ADDON)
+    return ADDON
 
 
 if __name__ == "__main__":

@@ -229,8 +229,9 @@ void exo_destroy(exo_engine* h) {
 
 int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* member) {
   if (!h || !cfg || !member) return fail(EXO_E_ARG, "null argument");
-  if (precision != EXO_PRECISION_FP32 && precision != EXO_PRECISION_BF16_SPLIT)
-    return fail(EXO_E_ARG, "unknown precision");
+  if (precision == EXO_PRECISION_BF16_SPLIT)
+    return fail(EXO_E_ARG, "the split-operand Legendre form is not available in this build");
+  if (precision != EXO_PRECISION_FP32) return fail(EXO_E_ARG, "unknown precision");
   Engine& e = h->e;
   int desc = -1;
   std::vector<bool> used(e.max_members, false);
@@ -506,6 +507,13 @@ int exo_probe_result(exo_engine* h, double* total_ms, int* launches, int* missed
   *total_ms = sum;
   *launches = (int)e.probe.used;
   *missed = (int)e.probe.missed;
+  return EXO_OK;
+}
+
+int exo_member_date(exo_engine* h, int id, int* date5) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !date5) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  exo::member_date(m->host, m->step, date5);
   return EXO_OK;
 }
 

@@ -802,6 +802,15 @@ void config_default(exo_config* c) {
   c->noise_seed = 14963991;
 }
 
+void member_date(const Member& m, int64_t step, int out[5]) {
+  const Date d = date_of(step, m.cal.steps_per_day, m.cal.days_per_year, m.cal.step_minutes);
+  out[0] = d.year;
+  out[1] = d.month;
+  out[2] = d.day;
+  out[3] = d.hour;
+  out[4] = d.minute;
+}
+
 void build_member(const exo_config& cfg, Member& m) {
   if (cfg.levels != 5 && cfg.levels != 10) throw std::runtime_error("Choose 5 or 10 atmospheric layers");
   if (!(cfg.timestep_minutes > 0)) throw std::runtime_error("timestep must be positive");

@@ -66,6 +66,9 @@ void config_default(exo_config* cfg);
 // Builds everything the device needs for one member; throws
 // std::runtime_error on an invalid configuration.
 void build_member(const exo_config& cfg, Member& m);
+// Calendar date {year, month, day, hour, minute} of main-loop step `step`
+// (calendar/date/index.ts:46-82, 360-day calendar).
+void member_date(const Member& m, int64_t step, int out[5]);
 Layout make_layout(const ExoDims& d, int nsnap);
 TableLayout make_table_layout(const ExoDims& d);
 // STEPPER.values for model step `step` into 16 doubles: radiation_step[5],

@@ -84,6 +84,7 @@ def library():
         "exo_step": (I, [P, I]),
         "exo_sync": (I, [P]),
         "exo_member_step": (I, [P, I, ctypes.POINTER(ctypes.c_int64)]),
+        "exo_member_date": (I, [P, I, ctypes.POINTER(I)]),
         "exo_arena_floats": (I, [P, I, ctypes.POINTER(S)]),
         "exo_slot": (I, [P, I, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
         "exo_slot_name": (ctypes.c_char_p, [I]),
@@ -238,6 +239,12 @@ class Engine:
         v = ctypes.c_int64()
         _check(self.lib.exo_member_step(self.h, member, ctypes.byref(v)))
         return v.value
+
+    def member_date(self, member):
+        """(year, month, day, hour, minute) of the member's step counter."""
+        d = (ctypes.c_int * 5)()
+        _check(self.lib.exo_member_date(self.h, member, d))
+        return tuple(d)
 
     # layout and fields --------------------------------------------------
     def arena_floats(self, member):

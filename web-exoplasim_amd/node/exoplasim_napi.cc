@@ -1,0 +1,359 @@
+// Node-API binding of libexoplasim_mi355x.so: the engine calls the
+// reference's worker makes (runner/webgpu/gcm.worker.ts) as synchronous
+// functions over the C-ABI. A failed call throws an Error carrying
+// exo_last_error()'s text.
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/exoplasim_mi355x.h"
+
+namespace {
+
+#define NAPI_OK(call)                                           \
+  do {                                                          \
+    if ((call) != napi_ok) {                                    \
+      napi_throw_error(env, nullptr, "Node-API call failed");   \
+      return nullptr;                                           \
+    }                                                           \
+  } while (0)
+
+#define EXO_OK_OR_THROW(call)                                   \
+  do {                                                          \
+    if ((call) != EXO_OK) {                                     \
+      napi_throw_error(env, nullptr, exo_last_error());         \
+      return nullptr;                                           \
+    }                                                           \
+  } while (0)
+
+struct Args {
+  size_t n = 8;
+  napi_value v[8];
+};
+
+bool get_args(napi_env env, napi_callback_info info, Args& a, size_t need) {
+  if (napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr) != napi_ok) return false;
+  if (a.n < need) {
+    napi_throw_type_error(env, nullptr, "missing arguments");
+    return false;
+  }
+  return true;
+}
+
+exo_engine* engine_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "not an engine handle (or destroyed)");
+    return nullptr;
+  }
+  return static_cast<exo_engine*>(p);
+}
+
+int32_t int_of(napi_env env, napi_value v) {
+  int32_t x = 0;
+  napi_get_value_int32(env, v, &x);
+  return x;
+}
+
+napi_value number(napi_env env, double x) {
+  napi_value r;
+  napi_create_double(env, x, &r);
+  return r;
+}
+
+napi_value float_array(napi_env env, const std::vector<float>& data) {
+  napi_value buf, arr;
+  void* p = nullptr;
+  if (napi_create_arraybuffer(env, data.size() * sizeof(float), &p, &buf) != napi_ok) return nullptr;
+  if (!data.empty()) std::memcpy(p, data.data(), data.size() * sizeof(float));
+  if (napi_create_typedarray(env, napi_float32_array, data.size(), buf, 0, &arr) != napi_ok) return nullptr;
+  return arr;
+}
+
+// exo_config fields by their C names; the JS side maps GcmConfig onto them.
+struct Field {
+  const char* name;
+  char kind;  // i: int, d: double, u: uint64
+  size_t offset;
+};
+#define EXO_FIELD(k, n) {#n, k, offsetof(exo_config, n)}
+const Field kConfigFields[] = {
+    EXO_FIELD('i', truncation), EXO_FIELD('i', levels), EXO_FIELD('d', timestep_minutes),
+    EXO_FIELD('d', radius_earth), EXO_FIELD('d', gravity), EXO_FIELD('d', rotation_period_days),
+    EXO_FIELD('d', pressure_bar), EXO_FIELD('d', gas_constant), EXO_FIELD('d', co2_ppm),
+    EXO_FIELD('d', flux_wm2), EXO_FIELD('d', obliquity_deg), EXO_FIELD('d', eccentricity),
+    EXO_FIELD('d', year_days), EXO_FIELD('d', ocean_depth_m), EXO_FIELD('i', ozone),
+    EXO_FIELD('d', land_fraction), EXO_FIELD('d', terrain_max_m), EXO_FIELD('d', initial_temperature),
+    EXO_FIELD('d', initial_humidity), EXO_FIELD('d', noise_kelvin), EXO_FIELD('d', sea_surface_k),
+    EXO_FIELD('u', terrain_seed), EXO_FIELD('u', noise_seed),
+};
+#undef EXO_FIELD
+
+napi_value config_default(napi_env env, napi_callback_info) {
+  exo_config c;
+  exo_config_default(&c);
+  napi_value obj;
+  NAPI_OK(napi_create_object(env, &obj));
+  for (const Field& f : kConfigFields) {
+    const char* at = reinterpret_cast<const char*>(&c) + f.offset;
+    double x = f.kind == 'i' ? *reinterpret_cast<const int*>(at)
+               : f.kind == 'd' ? *reinterpret_cast<const double*>(at)
+                               : (double)*reinterpret_cast<const uint64_t*>(at);
+    NAPI_OK(napi_set_named_property(env, obj, f.name, number(env, x)));
+  }
+  return obj;
+}
+
+// Defaults overlaid with the object's numeric properties; unknown keys throw.
+bool config_from(napi_env env, napi_value obj, exo_config& c) {
+  exo_config_default(&c);
+  napi_value keys;
+  uint32_t n = 0;
+  if (napi_get_property_names(env, obj, &keys) != napi_ok || napi_get_array_length(env, keys, &n) != napi_ok)
+    return false;
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value k, v;
+    napi_get_element(env, keys, i, &k);
+    char name[64] = {0};
+    size_t len = 0;
+    napi_get_value_string_utf8(env, k, name, sizeof(name), &len);
+    const Field* f = nullptr;
+    for (const Field& x : kConfigFields)
+      if (std::strcmp(x.name, name) == 0) f = &x;
+    if (!f) {
+      napi_throw_error(env, nullptr, (std::string("unknown config field ") + name).c_str());
+      return false;
+    }
+    napi_get_property(env, obj, k, &v);
+    char* at = reinterpret_cast<char*>(&c) + f->offset;
+    if (f->kind == 'u') {
+      bool lossless = false;
+      uint64_t x = 0;
+      if (napi_get_value_bigint_uint64(env, v, &x, &lossless) != napi_ok) {
+        double d = 0;
+        napi_get_value_double(env, v, &d);
+        x = (uint64_t)d;
+      }
+      *reinterpret_cast<uint64_t*>(at) = x;
+    } else {
+      double d = 0;
+      if (napi_get_value_double(env, v, &d) != napi_ok) {
+        napi_throw_type_error(env, nullptr, (std::string(name) + " must be a number").c_str());
+        return false;
+      }
+      if (f->kind == 'i')
+        *reinterpret_cast<int*>(at) = (int)d;
+      else
+        *reinterpret_cast<double*>(at) = d;
+    }
+  }
+  return true;
+}
+
+napi_value create(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = nullptr;
+  EXO_OK_OR_THROW(exo_create(int_of(env, a.v[0]), int_of(env, a.v[1]), &e));
+  napi_value r;
+  NAPI_OK(napi_create_external(env, e, nullptr, nullptr, &r));
+  return r;
+}
+
+napi_value destroy(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  exo_destroy(e);
+  return nullptr;
+}
+
+napi_value add_member(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  exo_config c;
+  if (!config_from(env, a.v[1], c)) return nullptr;
+  int id = -1;
+  EXO_OK_OR_THROW(exo_add_member(e, &c, int_of(env, a.v[2]), &id));
+  return number(env, id);
+}
+
+napi_value remove_member(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  EXO_OK_OR_THROW(exo_remove_member(e, int_of(env, a.v[1])));
+  return nullptr;
+}
+
+napi_value step(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  EXO_OK_OR_THROW(exo_step(e, int_of(env, a.v[1])));
+  return nullptr;
+}
+
+napi_value sync(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  EXO_OK_OR_THROW(exo_sync(e));
+  return nullptr;
+}
+
+napi_value set_publish_every(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  EXO_OK_OR_THROW(exo_set_publish_every(e, int_of(env, a.v[1])));
+  return nullptr;
+}
+
+napi_value member_step(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  int64_t s = 0;
+  EXO_OK_OR_THROW(exo_member_step(e, int_of(env, a.v[1]), &s));
+  return number(env, (double)s);
+}
+
+napi_value member_date(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  int d[5];
+  EXO_OK_OR_THROW(exo_member_date(e, int_of(env, a.v[1]), d));
+  napi_value obj;
+  NAPI_OK(napi_create_object(env, &obj));
+  const char* names[5] = {"year", "month", "day", "hour", "minute"};
+  for (int i = 0; i < 5; i++) NAPI_OK(napi_set_named_property(env, obj, names[i], number(env, d[i])));
+  return obj;
+}
+
+napi_value dims(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  int d[4];
+  EXO_OK_OR_THROW(exo_dims(e, int_of(env, a.v[1]), d));
+  napi_value arr;
+  NAPI_OK(napi_create_array_with_length(env, 4, &arr));
+  for (uint32_t i = 0; i < 4; i++) NAPI_OK(napi_set_element(env, arr, i, number(env, d[i])));
+  return arr;
+}
+
+napi_value read_field(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  char name[64] = {0};
+  size_t len = 0;
+  napi_get_value_string_utf8(env, a.v[2], name, sizeof(name), &len);
+  size_t off = 0, n = 0;
+  const int id = int_of(env, a.v[1]);
+  EXO_OK_OR_THROW(exo_slot(e, id, name, &off, &n));
+  std::vector<float> out(n);
+  EXO_OK_OR_THROW(exo_read_field(e, id, name, out.data(), n));
+  return float_array(env, out);
+}
+
+napi_value read_snapshot(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  size_t off = 0, n = 0;
+  EXO_OK_OR_THROW(exo_slot(e, id, "snapshot", &off, &n));
+  std::vector<float> out(n);
+  EXO_OK_OR_THROW(exo_read_snapshot(e, id, out.data(), n));
+  return float_array(env, out);
+}
+
+napi_value global_means(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  double ts = 0, toa = 0;
+  EXO_OK_OR_THROW(exo_global_means(e, int_of(env, a.v[1]), &ts, &toa));
+  napi_value obj;
+  NAPI_OK(napi_create_object(env, &obj));
+  NAPI_OK(napi_set_named_property(env, obj, "surfaceTemperatureK", number(env, ts)));
+  NAPI_OK(napi_set_named_property(env, obj, "toaNetWm2", number(env, toa)));
+  return obj;
+}
+
+napi_value checkpoint(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  size_t bytes = 0;
+  EXO_OK_OR_THROW(exo_checkpoint_size(e, id, &bytes));
+  napi_value buf;
+  void* p = nullptr;
+  NAPI_OK(napi_create_arraybuffer(env, bytes, &p, &buf));
+  EXO_OK_OR_THROW(exo_checkpoint_save(e, id, p, bytes));
+  return buf;
+}
+
+napi_value restore(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  void* p = nullptr;
+  size_t bytes = 0;
+  if (napi_get_arraybuffer_info(env, a.v[2], &p, &bytes) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "restart must be an ArrayBuffer");
+    return nullptr;
+  }
+  EXO_OK_OR_THROW(exo_checkpoint_restore(e, int_of(env, a.v[1]), p, bytes));
+  return nullptr;
+}
+
+napi_value init(napi_env env, napi_value exports) {
+  const std::pair<const char*, napi_callback> fns[] = {
+      {"configDefault", config_default}, {"create", create},
+      {"destroy", destroy},              {"addMember", add_member},
+      {"removeMember", remove_member},   {"step", step},
+      {"sync", sync},                    {"setPublishEvery", set_publish_every},
+      {"memberStep", member_step},       {"memberDate", member_date},
+      {"dims", dims},                    {"readField", read_field},
+      {"readSnapshot", read_snapshot},   {"globalMeans", global_means},
+      {"checkpoint", checkpoint},        {"restore", restore},
+  };
+  for (const auto& f : fns) {
+    napi_value fn;
+    if (napi_create_function(env, f.first, NAPI_AUTO_LENGTH, f.second, nullptr, &fn) != napi_ok) return nullptr;
+    if (napi_set_named_property(env, exports, f.first, fn) != napi_ok) return nullptr;
+  }
+  napi_value v;
+  napi_create_int32(env, EXO_PRECISION_FP32, &v);
+  napi_set_named_property(env, exports, "PRECISION_FP32", v);
+  napi_create_int32(env, EXO_PRECISION_BF16_SPLIT, &v);
+  napi_set_named_property(env, exports, "PRECISION_BF16_SPLIT", v);
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)
