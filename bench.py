@@ -24,8 +24,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "simulated days/s at T63 L10, batched vs plain (1 GPU); p50/p99 run latency"
-HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
 
 
 def _package():
@@ -96,6 +94,17 @@ def cpu_baseline(exo, cfgs, seconds_per_case):
                       "scaled by group size: " + "; ".join(sample)}
 
 
+def copy_calibration():
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,7 +115,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three device copies of 256 MiB (the known-size kernel that "
+                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every member of the workload runs on the one GPU "
               "of this box, and no multi-GPU figure is reported", file=sys.stderr)

@@ -1,36 +1,25 @@
-// Spectral transforms of the ExoPlaSim step on gfx950, plain FP32 form.
+// Zonal DFTs of the ExoPlaSim step on gfx950, plain FP32 form (the
+// Legendre half of each transform is in kernels_legendre.hip).
 //
 // The reference runs one WGSL invocation per output value and rereads the
-// Legendre and DFT tables for every plane (legendre/wgsl/*.wgsl,
-// fourier/wgsl/*.wgsl). Here each workgroup owns one zonal wavenumber m
-// (Legendre) or one latitude row (DFT) of one job and every level of it:
-// the table block is staged in LDS once and reused for all levels, the
-// spectral/Fourier operands come in as contiguous runs, and each lane
-// keeps one accumulator per level in registers.
+// DFT tables for every plane (fourier/wgsl/*.wgsl). Here each workgroup
+// owns one latitude row of one job and every level of it: the Fourier
+// coefficients come in as per-wavenumber level vectors in LDS, the
+// twiddles are built once per row, and each lane keeps one accumulator
+// per level in registers.
 //
-//  k_leg_inv  sp2fc / sp2fcdmu / dv2uv for wavenumber m: F(lat, p, m) =
-//             sum_n q(lat, w(m, n)) S(p, w) over every latitude and level
-//             (legendre/wgsl/inverse.wgsl:6-27, dv2uv.wgsl:9-43).
 //  k_dft_inv  fc2gp (+ d/dlambda) of one latitude row, every level
 //             (fourier/wgsl/inverse.wgsl:4-18, dlambda.wgsl:5-17).
 //  k_dft_fwd  gp2fc of one latitude row, every level
 //             (fourier/wgsl/forward.wgsl:4-18).
-//  k_leg_fwd  fc2sp / uv2dv / mktend / qtend for wavenumber m over the
-//             northern/southern row pairs (legendre/wgsl/forward.wgsl:7-38,
-//             uv2dv.wgsl:11-63, qtend.wgsl:9-46).
 //
-// Sums run over n (inverse) and over row pairs (forward) in the reference's
-// order. Every kernel walks a LaunchPlan so one launch serves a mixed batch.
+// Every kernel walks a LaunchPlan so one launch serves a mixed batch.
 #include "transforms.h"
 
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxLat = 96, kMaxK = 64, kMaxLev = 10, kMaxFC = 128, kMaxLon = 192;
-constexpr int kStride = kMaxK + 1;  // odd LDS row stride: no bank conflicts
-constexpr int kLegFwdThreads = 256;  // k_leg_fwd: 4 waves, lane = mode (K <= kMaxK = 64)
-
-__device__ __forceinline__ int mode_first(int m, int ntp1) { return m * (2 * ntp1 - m + 1) / 2; }
+constexpr int kMaxLev = 10, kMaxFC = 128, kMaxLon = 192;
 
 template <typename Jobs>
 __device__ __forceinline__ int job_of(const Jobs& jobs, int index) {
@@ -52,103 +41,6 @@ __device__ __forceinline__ void read_vec(const float* src, float* out) {
     out[4 * i + 1] = t.y;
     out[4 * i + 2] = t.z;
     out[4 * i + 3] = t.w;
-  }
-}
-
-// sp2fc / dv2uv of wavenumber m. Lanes own latitudes; the kThreads / nlat
-// lane groups (2 at T63, 4 at T42, 8 at T21) split the modes n
-// (group g sums k = g, g + G, ...), then group 0 adds the other groups'
-// partial sums in group order.
-template <int P>
-__device__ void leg_inv(const DevMember& M, const InvJob& job, int m, float* qab, float* sa, float* sb) {
-  const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
-  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
-  const bool winds = job.kind == INV_DV2UV;
-  float* qa = qab;
-  float* qb = qab + kMaxLat * kStride;
-  const float* q0 = M.tab + M.tabo[winds ? (int)TAB_legendre_qu : job.table];
-  const float* q1 = M.tab + M.tabo[TAB_legendre_qv];
-  for (int i = threadIdx.x; i < nlat * K; i += kThreads) {
-    const int lat = i / K, k = i % K;
-    qa[lat * kStride + k] = q0[lat * ncsp + w0 + k];
-    if (winds) qb[lat * kStride + k] = q1[lat * ncsp + w0 + k];
-  }
-  const float* S0 = M.arena + M.slot[job.src0];
-  const float* S1 = winds ? M.arena + M.slot[job.src1] : nullptr;
-  const float plavor = job.plavor ? M.k[CON_plavor] : 0.0f;
-  for (int i = threadIdx.x; i < P * 2 * K; i += kThreads) {
-    const int p = i / (2 * K), r = i % (2 * K);
-    const int at = (r >> 1) * SV(P) + 2 * p + (r & 1);
-    sa[at] = S0[p * nrsp + 2 * w0 + r];
-    // zeta without the planetary vorticity in Re mode (0, 1)
-    if (winds) sb[at] = S1[p * nrsp + 2 * w0 + r] - ((w0 + (r >> 1) == 1 && (r & 1) == 0) ? plavor : 0.0f);
-  }
-  __syncthreads();
-  const int G = kThreads / nlat;
-  const int g = threadIdx.x / nlat, lat = threadIdx.x % nlat;
-  const bool live = g < G;
-  // plain: acc[2p + part]; winds: U in acc[2p + part], V in acc[2P + 2p + part]
-  float acc[4 * P];
-#pragma unroll
-  for (int i = 0; i < 4 * P; i++) acc[i] = 0.0f;
-  const int nacc = winds ? 4 * P : 2 * P;
-  if (live && !winds) {
-    for (int k = g; k < K; k += G) {
-      const float qq = qa[lat * kStride + k];
-      float v[SV(P)];
-      read_vec<P>(sa + k * SV(P), v);
-#pragma unroll
-      for (int i = 0; i < 2 * P; i++) acc[i] += qq * v[i];
-    }
-  } else if (live) {
-    for (int k = g; k < K; k += G) {
-      const float u = qa[lat * kStride + k], v = qb[lat * kStride + k];
-      float d[SV(P)], z[SV(P)];
-      read_vec<P>(sa + k * SV(P), d);
-      read_vec<P>(sb + k * SV(P), z);
-#pragma unroll
-      for (int p = 0; p < P; p++) {
-        const float dre = d[2 * p], dim = d[2 * p + 1];
-        const float zre = z[2 * p], zim = z[2 * p + 1];
-        acc[2 * p] = acc[2 * p] + v * zre + u * dim;
-        acc[2 * p + 1] = acc[2 * p + 1] + v * zim - u * dre;
-        acc[2 * P + 2 * p] = acc[2 * P + 2 * p] + u * zim - v * dre;
-        acc[2 * P + 2 * p + 1] = acc[2 * P + 2 * p + 1] - u * zre - v * dim;
-      }
-    }
-  }
-  // the table tiles are dead now: their LDS takes the partial sums
-  constexpr int RS = 4 * P + 1;
-  __syncthreads();
-  if (live && g > 0) {
-    float* r = qab + ((g - 1) * nlat + lat) * RS;
-#pragma unroll
-    for (int i = 0; i < 4 * P; i++)
-      if (i < nacc) r[i] = acc[i];
-  }
-  __syncthreads();
-  if (!live || g != 0) return;
-  for (int j = 1; j < G; j++) {
-    const float* r = qab + ((j - 1) * nlat + lat) * RS;
-#pragma unroll
-    for (int i = 0; i < 4 * P; i++)
-      if (i < nacc) acc[i] += r[i];
-  }
-  float* F0 = M.arena + M.slot[job.four0];
-#pragma unroll
-  for (int p = 0; p < P; p++) {
-    float* f = F0 + (p * nlat + lat) * nfc + 2 * m;
-    f[0] = acc[2 * p];
-    f[1] = acc[2 * p + 1];
-  }
-  if (winds) {
-    float* F1 = M.arena + M.slot[job.four1];
-#pragma unroll
-    for (int p = 0; p < P; p++) {
-      float* f = F1 + (p * nlat + lat) * nfc + 2 * m;
-      f[0] = acc[2 * P + 2 * p];
-      f[1] = acc[2 * P + 2 * p + 1];
-    }
   }
 }
 
@@ -282,159 +174,6 @@ __device__ void dft_fwd(const DevMember& M, const FwdJob& job, int lat, float* r
   }
 }
 
-// Fourier coefficient m of every row of `fields` Fourier slots into LDS as
-// level vectors: fin[(f * nlat + lat) * SV(P) + 2 p + part].
-template <int P>
-__device__ void load_column(const DevMember& M, const int* slots, int fields, int m, float* fin) {
-  const int nlat = M.nlat, nfc = M.nfc;
-  const int per = P * nlat * 2;
-  for (int i = threadIdx.x; i < fields * per; i += kLegFwdThreads) {
-    const int f = i / per, r = i % per;
-    const int p = r / (nlat * 2), lat = (r >> 1) % nlat, part = r & 1;
-    fin[(f * nlat + lat) * SV(P) + 2 * p + part] =
-        M.arena[M.slot[slots[f]] + (p * nlat + lat) * nfc + 2 * m + part];
-  }
-}
-
-// Partial sums of waves 1..3 of k_leg_fwd, per lane: odd row stride.
-constexpr int kRedStride = 4 * kMaxLev + 1;
-
-template <int P>
-__device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin, float* red) {
-  const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1), nrsp = M.nrsp;
-  const int ncsp = M.ncsp, half = nlat / 2;
-  int slots[3];
-  int fields = 0;
-  if (job.kind == LEG_PLAIN) {
-    slots[fields++] = job.src;
-  } else if (job.kind == LEG_UV2DV) {
-    slots[fields++] = job.u;
-    slots[fields++] = job.v;
-    if (job.ke >= 0) slots[fields++] = job.ke;
-  } else {
-    slots[fields++] = job.u;
-    slots[fields++] = job.v;
-    slots[fields++] = job.src;
-  }
-  load_column<P>(M, slots, fields, m, fin);
-  __syncthreads();
-  // lane k owns mode w0 + k for every level; wave wv sums latitude pairs
-  // l = wv, wv + 4, ... Table columns come straight from memory
-  // (consecutive lanes, consecutive modes).
-  const int k = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const bool live = k < K;
-  const float s = (k & 1) ? -1.0f : 1.0f;  // parity of m + n = 2m + k
-  const int w = w0 + (live ? k : 0);
-  auto row = [&](int f, int lat) { return fin + (f * nlat + lat) * SV(P); };
-  // PLAIN / QTEND: acc[2p + part]; UV2DV: D in acc[2p + part], zeta in
-  // acc[2P + 2p + part]
-  float acc[4 * P];
-#pragma unroll
-  for (int i = 0; i < 4 * P; i++) acc[i] = 0.0f;
-  const int nacc = job.kind == LEG_UV2DV ? 4 * P : 2 * P;
-  if (live && job.kind == LEG_PLAIN) {
-    const float* q = M.tab + M.tabo[TAB_legendre_qc] + w;
-    for (int l = wv; l < half; l += 4) {
-      const float qq = q[l * ncsp];
-      float nv[SV(P)], sv[SV(P)];
-      read_vec<P>(row(0, l), nv);
-      read_vec<P>(row(0, nlat - 1 - l), sv);
-#pragma unroll
-      for (int i = 0; i < 2 * P; i++) acc[i] += qq * (nv[i] + s * sv[i]);
-    }
-  } else if (live && job.kind == LEG_UV2DV) {
-    const bool kin = job.ke >= 0;
-    const float* qe = M.tab + M.tabo[TAB_legendre_qe] + w;
-    const float* qm = M.tab + M.tabo[TAB_legendre_qm] + w;
-    const float* qq = M.tab + M.tabo[TAB_legendre_qq] + w;
-    for (int l = wv; l < half; l += 4) {
-      const int so = nlat - 1 - l;
-      const float e = qe[l * ncsp], mm = qm[l * ncsp];
-      float un[SV(P)], us[SV(P)], vn[SV(P)], vs[SV(P)];
-      read_vec<P>(row(0, l), un);
-      read_vec<P>(row(0, so), us);
-      read_vec<P>(row(1, l), vn);
-      read_vec<P>(row(1, so), vs);
-      if (kin) {
-        const float c = qq[l * ncsp];
-        float kn[SV(P)], ks[SV(P)];
-        read_vec<P>(row(2, l), kn);
-        read_vec<P>(row(2, so), ks);
-#pragma unroll
-        for (int i = 0; i < 2 * P; i++) acc[i] = acc[i] + c * (kn[i] + s * ks[i]);
-      }
-#pragma unroll
-      for (int p = 0; p < P; p++) {
-        const float dure = un[2 * p] - s * us[2 * p];
-        const float duim = un[2 * p + 1] - s * us[2 * p + 1];
-        const float dvre = vn[2 * p] - s * vs[2 * p];
-        const float dvim = vn[2 * p + 1] - s * vs[2 * p + 1];
-        const float sure = un[2 * p] + s * us[2 * p];
-        const float suim = un[2 * p + 1] + s * us[2 * p + 1];
-        const float svre = vn[2 * p] + s * vs[2 * p];
-        const float svim = vn[2 * p + 1] + s * vs[2 * p + 1];
-        acc[2 * p] = acc[2 * p] - e * dvre - mm * suim;
-        acc[2 * p + 1] = acc[2 * p + 1] - e * dvim + mm * sure;
-        acc[2 * P + 2 * p] = acc[2 * P + 2 * p] + e * dure - mm * svim;
-        acc[2 * P + 2 * p + 1] = acc[2 * P + 2 * p + 1] + e * duim + mm * svre;
-      }
-    }
-  } else if (live) {
-    const float* qe = M.tab + M.tabo[TAB_legendre_qe] + w;
-    const float* qc = M.tab + M.tabo[TAB_legendre_qc] + w;
-    const float* qm = M.tab + M.tabo[TAB_legendre_qm] + w;
-    for (int l = wv; l < half; l += 4) {
-      const int so = nlat - 1 - l;
-      const float e = qe[l * ncsp], c = qc[l * ncsp], mm = qm[l * ncsp];
-      float un[SV(P)], us[SV(P)], vn[SV(P)], vs[SV(P)], sn[SV(P)], ss[SV(P)];
-      read_vec<P>(row(0, l), un);
-      read_vec<P>(row(0, so), us);
-      read_vec<P>(row(1, l), vn);
-      read_vec<P>(row(1, so), vs);
-      read_vec<P>(row(2, l), sn);
-      read_vec<P>(row(2, so), ss);
-#pragma unroll
-      for (int p = 0; p < P; p++) {
-        acc[2 * p] = acc[2 * p] + e * (vn[2 * p] - s * vs[2 * p]) + c * (sn[2 * p] + s * ss[2 * p]) +
-                     mm * (un[2 * p + 1] + s * us[2 * p + 1]);
-        acc[2 * p + 1] = acc[2 * p + 1] + e * (vn[2 * p + 1] - s * vs[2 * p + 1]) +
-                         c * (sn[2 * p + 1] + s * ss[2 * p + 1]) - mm * (un[2 * p] + s * us[2 * p]);
-      }
-    }
-  }
-  // waves 1..3 hand their partial sums to wave 0, which adds them in wave
-  // order (a fixed order: bit-reproducible)
-  if (wv > 0 && live) {
-    float* r = red + ((wv - 1) * 64 + k) * kRedStride;
-#pragma unroll
-    for (int i = 0; i < 4 * P; i++)
-      if (i < nacc) r[i] = acc[i];
-  }
-  __syncthreads();
-  if (wv != 0 || !live) return;
-#pragma unroll
-  for (int j = 0; j < 3; j++) {
-    const float* r = red + (j * 64 + k) * kRedStride;
-#pragma unroll
-    for (int i = 0; i < 4 * P; i++)
-      if (i < nacc) acc[i] += r[i];
-  }
-  float* o0 = M.arena + M.slot[job.dst0] + 2 * w;
-#pragma unroll
-  for (int p = 0; p < P; p++) {
-    o0[p * nrsp] = acc[2 * p];
-    o0[p * nrsp + 1] = acc[2 * p + 1];
-  }
-  if (job.kind == LEG_UV2DV) {
-    float* o1 = M.arena + M.slot[job.dst1] + 2 * w;
-#pragma unroll
-    for (int p = 0; p < P; p++) {
-      o1[p * nrsp] = acc[2 * P + 2 * p];
-      o1[p * nrsp + 1] = acc[2 * P + 2 * p + 1];
-    }
-  }
-}
-
 }  // namespace
 
 #define EXO_PLANES_DISPATCH(P, call) \
@@ -446,25 +185,6 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* fin
     else                             \
       call(1);                       \
   } while (0)
-
-extern "C" __global__ __launch_bounds__(256) void k_leg_inv(const DevMember* __restrict__ mems, LaunchPlan plan,
-                                                             InvJobs jobs) {
-  // qu and qv tiles; after the sums, the partial sums of lane groups 1..7
-  // ((8 - 1) x 32 lanes x (4 x 10 + 1) floats at T21 fits)
-  __shared__ float qab[2 * kMaxLat * kStride];
-  __shared__ __align__(16) float sa[kMaxK * SV(kMaxLev)];
-  __shared__ __align__(16) float sb[kMaxK * SV(kMaxLev)];
-  static_assert(7 * 32 * (4 * kMaxLev + 1) <= 2 * kMaxLat * kStride, "partial sums exceed the table tiles");
-  int tile;
-  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int ntp1 = M.ntru + 1;
-  const int m = tile % ntp1;
-  const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
-  const int P = job.per_level ? M.nlev : 1;
-#define CALL(PP) leg_inv<PP>(M, job, m, qab, sa, sb)
-  EXO_PLANES_DISPATCH(P, CALL);
-#undef CALL
-}
 
 extern "C" __global__ __launch_bounds__(256) void k_dft_inv(const DevMember* __restrict__ mems, LaunchPlan plan,
                                                              InvJobs jobs) {
@@ -495,17 +215,3 @@ extern "C" __global__ __launch_bounds__(256) void k_dft_fwd(const DevMember* __r
 #undef CALL
 }
 
-extern "C" __global__ __launch_bounds__(kLegFwdThreads) void k_leg_fwd(const DevMember* __restrict__ mems, LaunchPlan plan,
-                                                            LegJobs jobs) {
-  __shared__ __align__(16) float fin[3 * kMaxLat * SV(kMaxLev)];
-  __shared__ float red[3 * 64 * kRedStride];
-  int tile;
-  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int ntp1 = M.ntru + 1;
-  const int m = tile % ntp1;
-  const LegJob job = jobs.j[job_of(jobs, tile / ntp1)];
-  const int P = job.per_level ? M.nlev : 1;
-#define CALL(PP) leg_fwd<PP>(M, job, m, fin, red)
-  EXO_PLANES_DISPATCH(P, CALL);
-#undef CALL
-}
