@@ -12,13 +12,15 @@
 struct DevMember {
   int ntru, nlat, nlon, nlev;
   int ncsp, nrsp, nfc, nhor;
-  int precision;  // 0 plain FP32, 1 BF16 hi/lo split Legendre
+  int precision;  // 0 plain FP32, 1 split-operand Legendre
   int pad_[3];
   float* arena;
+  // The member's tables; tab + tabo[t] for every table t from zgw on.
   const float* tab;
-  // Split operand tables for the BF16 form (hi, lo planes of qi..qv).
-  const unsigned short* tab_hi;
-  const unsigned short* tab_lo;
+  // The transform tables (Fourier and Legendre, tabo below zgw), one copy
+  // shared by every member of the same truncation: ltab + tabo[t].
+  const float* ltab;
+  const void* reserved_;
   int slot[SLOT_COUNT];
   int tabo[TAB_COUNT];
   float k[CON_COUNT];

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <initializer_list>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -50,7 +51,9 @@ struct MemberDev {
   Member host;
   int precision = 0;
   float* d_arena = nullptr;
-  float* d_tab = nullptr;
+  float* d_tab_alloc = nullptr;  // the member's own tables (zgw on)
+  float* d_tab = nullptr;        // base such that d_tab + tabo[t] addresses table t
+  int shared_key = -1;           // truncation of its shared transform tables
   float* d_consts = nullptr;  // phases x CON_COUNT
   float* d_bm1 = nullptr;     // phases x bm1 count
   int loaded_phase = 0;       // phase whose constants/bm1 are on the device
@@ -70,8 +73,15 @@ struct KernelProbe {
   size_t missed = 0;  // launches beyond the pool
 };
 
+// Fourier and Legendre tables of one truncation, shared by its members.
+struct SharedTables {
+  float* d = nullptr;
+  int refs = 0;
+};
+
 struct Engine {
   int device = 0;
+  std::map<int, SharedTables> transform_tables;
   hipStream_t stream = nullptr;
   int max_members = 0;
   DevMember* d_desc = nullptr;

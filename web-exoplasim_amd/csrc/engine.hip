@@ -39,6 +39,20 @@ static int fail(int code, const std::string& message) {
 
 extern "C" __global__ void k_global_means(const DevMember*, LaunchPlan, double*);
 
+namespace {
+
+// Drops a member's reference to its truncation's transform tables.
+void release_shared_tables(exo::Engine& e, int key) {
+  auto it = e.transform_tables.find(key);
+  if (it == e.transform_tables.end()) return;
+  if (--it->second.refs <= 0) {
+    hipFree(it->second.d);
+    e.transform_tables.erase(it);
+  }
+}
+
+}  // namespace
+
 // Writes each member's 16 per-step values into its arena's table slots
 // (the params-to-state copies of STEPPER.batch, stepper/index.ts:242-255).
 extern "C" __global__ __launch_bounds__(64) void k_load_step(const DevMember* __restrict__ mems, LaunchPlan plan,
@@ -211,10 +225,11 @@ void exo_destroy(exo_engine* h) {
   for (auto& m : e.members)
     if (m) {
       hipFree(m->d_arena);
-      hipFree(m->d_tab);
+      hipFree(m->d_tab_alloc);
       hipFree(m->d_consts);
       hipFree(m->d_bm1);
     }
+  for (auto& entry : e.transform_tables) hipFree(entry.second.d);
   hipFree(e.d_desc);
   hipFree(e.d_params);
   if (e.h_params) hipHostFree(e.h_params);
@@ -266,12 +281,24 @@ int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* mem
       for (int c = 0; c < CON_COUNT; c++) consts[p * CON_COUNT + c] = (float)m.constants[p][c];
       for (size_t i = 0; i < nbm1; i++) bm1[p * nbm1 + i] = (float)m.bm1[p][i];
     }
+    // the transform tables (every table before zgw) are one copy per
+    // truncation; the member's block holds the rest
+    const size_t shared_floats = m.tlayout.offset[TAB_zgw];
+    exo::SharedTables& shared = e.transform_tables[m.dims.ntru];
+    if (!shared.d) {
+      HIP_OR_THROW(hipMalloc(&shared.d, shared_floats * sizeof(float)));
+      HIP_OR_THROW(hipMemcpy(shared.d, tab.data(), shared_floats * sizeof(float), hipMemcpyHostToDevice));
+    }
+    shared.refs++;
+    md->shared_key = m.dims.ntru;
     HIP_OR_THROW(hipMalloc(&md->d_arena, arena.size() * sizeof(float)));
-    HIP_OR_THROW(hipMalloc(&md->d_tab, tab.size() * sizeof(float)));
+    HIP_OR_THROW(hipMalloc(&md->d_tab_alloc, (tab.size() - shared_floats) * sizeof(float)));
+    md->d_tab = reinterpret_cast<float*>(reinterpret_cast<uintptr_t>(md->d_tab_alloc) - shared_floats * sizeof(float));
     HIP_OR_THROW(hipMalloc(&md->d_consts, consts.size() * sizeof(float)));
     HIP_OR_THROW(hipMalloc(&md->d_bm1, bm1.size() * sizeof(float)));
     HIP_OR_THROW(hipMemcpy(md->d_arena, arena.data(), arena.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIP_OR_THROW(hipMemcpy(md->d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OR_THROW(hipMemcpy(md->d_tab_alloc, tab.data() + shared_floats, (tab.size() - shared_floats) * sizeof(float),
+                           hipMemcpyHostToDevice));
     HIP_OR_THROW(hipMemcpy(md->d_consts, consts.data(), consts.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_OR_THROW(hipMemcpy(md->d_bm1, bm1.data(), bm1.size() * sizeof(float), hipMemcpyHostToDevice));
     DevMember d{};
@@ -286,6 +313,7 @@ int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* mem
     d.precision = precision;
     d.arena = md->d_arena;
     d.tab = md->d_tab;
+    d.ltab = shared.d;
     for (int s = 0; s < SLOT_COUNT; s++) d.slot[s] = (int)m.layout.offset[s];
     for (int t = 0; t < TAB_COUNT; t++) d.tabo[t] = (int)m.tlayout.offset[t];
     for (int c = 0; c < CON_COUNT; c++) d.k[c] = consts[c];
@@ -299,9 +327,10 @@ int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* mem
     md->active = true;
   } catch (const std::exception& ex) {
     hipFree(md->d_arena);
-    hipFree(md->d_tab);
+    hipFree(md->d_tab_alloc);
     hipFree(md->d_consts);
     hipFree(md->d_bm1);
+    release_shared_tables(e, md->shared_key);
     return fail(EXO_E_DEVICE, ex.what());
   }
   int id = -1;
@@ -330,9 +359,10 @@ int exo_remove_member(exo_engine* h, int id) {
   hipSetDevice(e.device);
   hipStreamSynchronize(e.stream);
   hipFree(m->d_arena);
-  hipFree(m->d_tab);
+  hipFree(m->d_tab_alloc);
   hipFree(m->d_consts);
   hipFree(m->d_bm1);
+  release_shared_tables(e, m->shared_key);
   e.members[id].reset();
   return EXO_OK;
 }

@@ -159,7 +159,7 @@ __device__ void leg_inv(const DevMember& M, const InvJob& job, int m, float* As,
   for (int t = 0; t < kMaxTiles; t++) acc[t] = f4{0, 0, 0, 0};
   const int blocks = winds ? 2 : 1;
   for (int blk = 0; blk < blocks; blk++) {
-    const float* q = M.tab + M.tabo[winds ? (blk == 0 ? (int)TAB_legendre_qu : (int)TAB_legendre_qv) : job.table];
+    const float* q = M.ltab + M.tabo[winds ? (blk == 0 ? (int)TAB_legendre_qu : (int)TAB_legendre_qv) : job.table];
     // A[lat][k] and B[k][c], column c = (level, part) of the block's
     // arrangement; lanes walk the modes k, so a table row and a plane's
     // coefficients each load as one run
@@ -262,7 +262,7 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* As,
       [&](int fp, int x, float v) { raw[fp * 2 * nlat + x] = v; });
   auto F = [&](int f, int p, int lat, int part) { return raw[(f * P + p) * 2 * nlat + 2 * lat + part]; };
   for (int blk = 0; blk < blocks; blk++) {
-    const float* q = M.tab + M.tabo[tables[blk]];
+    const float* q = M.ltab + M.tabo[tables[blk]];
     // A[k][lat]: the folded table's row l and its mirror nlat - 1 - l
     const int dsign = diff[blk];
     stage(

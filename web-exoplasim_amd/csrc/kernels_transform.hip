@@ -51,7 +51,7 @@ __device__ __forceinline__ void read_vec(const float* src, float* out) {
 // FP32 cos/sin), and every lane walks its (m x) mod NLON index.
 __device__ void load_twiddles(const DevMember& M, float* ctw, float* stw) {
   const int nlon = M.nlon;
-  const float* inv = M.tab + M.tabo[TAB_fourier_inverse];
+  const float* inv = M.ltab + M.tabo[TAB_fourier_inverse];
   for (int x = threadIdx.x; x < nlon; x += kThreads) {
     ctw[x] = 0.5f * inv[2 * nlon + x];
     stw[x] = -0.5f * inv[3 * nlon + x];
