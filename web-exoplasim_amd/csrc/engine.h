@@ -162,11 +162,22 @@ class Stepper {
   void inv(const InvJobs& jobs) {
     const LaunchPlan pl = plan([&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
     launch(e_, k_leg_inv, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
-    const LaunchPlan pd = plan([&](const DevMember& d) { return d.nlat * jobs.n; });
+    // k_dft_inv: 64 rows (output x level x latitude) of one job per workgroup
+    const LaunchPlan pd = plan([&](const DevMember& d) {
+      int blocks = 0;
+      for (int j = 0; j < jobs.n; j++)
+        blocks += ((jobs.j[j].kind == INV_PLAIN ? 1 : 2) * job_planes(jobs.j[j].per_level, d.nlev) * d.nlat + 63) / 64;
+      return blocks;
+    });
     launch(e_, k_dft_inv, dim3(pd.first[pd.n]), dim3(256), e_.d_desc, pd, jobs);
   }
   void fwd(const FwdJobs& jobs) {
-    const LaunchPlan p = plan([&](const DevMember& d) { return d.nlat * jobs.n; });
+    // 64 rows (level x latitude) of one job per workgroup
+    const LaunchPlan p = plan([&](const DevMember& d) {
+      int blocks = 0;
+      for (int j = 0; j < jobs.n; j++) blocks += (job_planes(jobs.j[j].per_level, d.nlev) * d.nlat + 63) / 64;
+      return blocks;
+    });
     launch(e_, k_dft_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
   }
   void leg(const LegJobs& jobs) {

@@ -4,10 +4,10 @@
 // A transform runs in two launches, as the reference's sp2fc/fc2gp and
 // gp2fc/fc2sp pairs do, with the Fourier coefficients in the arena's
 // Fourier slots between them:
-//   inverse: k_leg_inv (one workgroup per zonal wavenumber m and job, every
-//            latitude and level of that m staged in LDS) then k_dft_inv
-//            (one workgroup per latitude row and job, every level);
-//   forward: k_dft_fwd (per row) then k_leg_fwd (per m).
+//   inverse: k_leg_inv (one workgroup per zonal wavenumber m and job,
+//            kernels_legendre.hip) then k_dft_inv (64 latitude rows of one
+//            job per workgroup, kernels_dft.hip);
+//   forward: k_dft_fwd (64 rows) then k_leg_fwd (per m).
 #pragma once
 #include "device.h"
 
