@@ -10,8 +10,15 @@ ratio, the dominant kernel's launches are timed with HIP events for the
 roofline figure, and the FP64 CPU oracle is timed on a bounded sample of
 the same workload. Prints one JSON line.
 
+The serving loop (continuous batching of Poisson-arriving runs, 4 runs/s,
+T21/T42/T63 weights 0.5/0.35/0.15, L5 or L10, 1-360 simulated days each,
+ensemble parameters) runs after it over a --serve-seconds arrival window
+and reports p50/p99 run-completion latency and aggregate days/s in the
+line's "serving" object; --workload serving makes it the measured value.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload batched|plain] [--probe KERNEL]
+                    [--workload batched|plain|serving] [--probe KERNEL]
+                    [--serve-seconds S] [--no-serving]
                     [--no-cpu-baseline] [--no-plain]
 """
 import argparse
@@ -94,6 +101,29 @@ def cpu_baseline(exo, cfgs, seconds_per_case):
                       "scaled by group size: " + "; ".join(sample)}
 
 
+def run_serving(exo, seconds, rate=4.0, seed=5):
+    """The serving loop over a `seconds` arrival window."""
+    import numpy as np
+    from web_exoplasim_amd import workloads
+    reqs = workloads.serving_requests(exo, seed=seed, rate=rate, seconds=seconds)
+    spd = workloads.steps_per_day(45)
+    runs = [(t, cfg, max(1, int(round(days * spd)))) for t, cfg, days in reqs]
+    with exo.Engine(0, 64) as eng:
+        results, wall = eng.serve(runs, max_chunk_steps=32, prepare_threads=8)
+    ok = [r for r in results if r["status"] == 0]
+    if len(ok) != len(results):
+        raise RuntimeError(f"{len(results) - len(ok)} served runs failed")
+    lat = np.array([r["finished_s"] - r["arrival_s"] for r in ok])
+    queued = np.array([r["admitted_s"] - r["arrival_s"] for r in ok])
+    days = sum(r["steps"] for r in ok) / spd
+    return {"value": days / wall, "unit": "simulated days/s", "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "p50_admit_wait_s": float(np.percentile(queued, 50)),
+            "runs": len(ok), "arrival_window_s": seconds, "arrival_rate_per_s": rate, "wall_s": wall,
+            "simulated_days": days,
+            "config": "Poisson arrivals, T21/T42/T63 weights 0.5/0.35/0.15, L5 or L10, length U[1,360] days, "
+                      "45-min step, ensemble parameters (seed 5); up to 64 resident runs"}
+
+
 def copy_calibration():
     import torch
     n = 64 * 1024 * 1024
@@ -110,7 +140,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=320)
     ap.add_argument("--warmup", type=int, default=32)
-    ap.add_argument("--workload", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "serving"])
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -128,6 +160,16 @@ def main():
         sys.exit(2)
     exo = _package()
     from web_exoplasim_amd import workloads, roofline
+
+    if args.workload == "serving":
+        sv = run_serving(exo, args.serve_seconds)
+        print(json.dumps({
+            "metric": METRIC, "value": sv["value"], "unit": "simulated days/s", "n_gpus": 1,
+            "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic seeded planets",
+            "config": {"workload": "serving loop: " + sv["config"], "arrival_window_s": args.serve_seconds},
+            "serving": sv, "roofline": None, "cpu_baseline": None}))
+        return
 
     if args.workload == "batched":
         cfgs = workloads.ensemble(exo)
@@ -166,6 +208,8 @@ def main():
                                 "ms_per_step": pw * 1000.0 / 480,
                                 "config": "plain FP32 single planet T63 L10, 30-min step"}
         line["batched_vs_plain"] = value / plain_value
+    if args.workload == "batched" and not args.no_serving:
+        line["serving"] = run_serving(exo, args.serve_seconds)
     if probed:
         total_ms, launches, missed = probed
         line["roofline"] = roofline.figure(exo, probe, cfgs, total_ms, launches, missed, args.steps)

@@ -185,6 +185,47 @@ int exo_checkpoint_save(exo_engine *e, int member, void *out, size_t bytes);
 int exo_checkpoint_restore(exo_engine *e, int member, const void *bytes,
                            size_t size);
 
+/*
+ * Serving loop (continuous batching). Runs arrive over time
+ * (arrival_s, seconds after the call starts); host threads run each
+ * arrived run's FP64 set-up; runs are admitted first-fit in arrival order
+ * while a descriptor is free and the resident runs' device memory stays
+ * within hbm_budget_bytes (0: 90% of the device's free memory); all
+ * resident runs advance together, each retiring at exactly its `steps`
+ * (an advance lasts as long as the shortest remaining run, at most
+ * max_chunk_steps), and runs that arrive meanwhile join at the next
+ * advance. Each result holds the run's times and its global means at the
+ * end. The engine must have no members; it has none again on return. A
+ * run computes the same alone and in the mix. The reference serves one run
+ * per page (runner/index.ts:61-78); this batched loop has no counterpart
+ * there.
+ */
+typedef struct {
+  exo_config config;
+  int precision;
+  int reserved;
+  double arrival_s;
+  int64_t steps; /* model steps, start-up steps included */
+} exo_request;
+
+typedef struct {
+  double arrival_s, admitted_s, finished_s; /* seconds after the call starts */
+  double tsurf, toa_net;                    /* global means at the end */
+  int64_t steps;                            /* steps run */
+  int status;                               /* EXO_OK or the failure code */
+  int reserved;
+} exo_result;
+
+typedef struct {
+  size_t hbm_budget_bytes;
+  int max_chunk_steps; /* 0: 32 */
+  int prepare_threads; /* 0: 4 */
+} exo_serve_options;
+
+int exo_serve(exo_engine *e, const exo_request *requests, int count,
+              const exo_serve_options *options, exo_result *results,
+              double *wall_s);
+
 /* Device-side timing of the last exo_step call (ms, HIP events). Replaces
    the timestamp queries of STEPPER.batch (src/model/climate/gcm/stepper/
    index.ts:175-276). */

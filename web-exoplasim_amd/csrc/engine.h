@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "device.h"
@@ -44,6 +45,10 @@ extern "C" __global__ void k_pack(const DevMember*, LaunchPlan);
 namespace exo {
 
 struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+// Every descriptor of the engine is taken.
+struct FullError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
@@ -215,4 +220,22 @@ Jobs jobs_of(std::initializer_list<Job> list) {
 void enqueue_physics(Engine& e, const std::vector<int>& active);
 void enqueue_step(Engine& e, const std::vector<int>& active);
 
+// Member life cycle. prepare_member runs the FP64 host set-up (no device
+// work; safe to call from several threads); admit_member uploads it into a
+// free descriptor and returns its id (FullError when none is free);
+// member_bytes is the device memory admitting it takes, its truncation's
+// shared tables included when no member holds them yet; drop_member frees
+// it after the queued work.
+std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision);
+// Sets the calling thread's exo_last_error() text; returns `code`.
+int set_error(int code, const std::string& message);
+size_t member_bytes(const Engine& e, const MemberDev& md);
+int admit_member(Engine& e, std::unique_ptr<MemberDev> md);
+void drop_member(Engine& e, int id);
+
 }  // namespace exo
+
+// The C-ABI's engine handle.
+struct exo_engine {
+  exo::Engine e;
+};

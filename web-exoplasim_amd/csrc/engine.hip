@@ -30,6 +30,10 @@ static int fail(int code, const std::string& message) {
   return code;
 }
 
+namespace exo {
+int set_error(int code, const std::string& message) { return fail(code, message); }
+}  // namespace exo
+
 #define HIP_OR_THROW(expr)                                                              \
   do {                                                                                  \
     hipError_t _e = (expr);                                                             \
@@ -158,11 +162,139 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
 
 }  // namespace exo
 
-// ------------------------------------------------------------------ C-ABI
+namespace exo {
 
-struct exo_engine {
-  exo::Engine e;
-};
+std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision) {
+  if (precision == EXO_PRECISION_BF16_SPLIT)
+    throw std::runtime_error("the split-operand Legendre form is not available in this build");
+  if (precision != EXO_PRECISION_FP32) throw std::runtime_error("unknown precision");
+  auto md = std::make_unique<MemberDev>();
+  build_member(cfg, md->host);
+  md->precision = precision;
+  md->step = md->host.start_step;
+  return md;
+}
+
+size_t member_bytes(const Engine& e, const MemberDev& md) {
+  const Member& m = md.host;
+  const size_t shared = m.tlayout.offset[TAB_zgw];
+  size_t bytes = m.layout.total + (m.tlayout.total - shared) + m.constants.size() * CON_COUNT +
+                 m.constants.size() * m.tlayout.count[TAB_bm1];
+  if (!e.transform_tables.count(m.dims.ntru)) bytes += shared;
+  return bytes * sizeof(float);
+}
+
+int admit_member(Engine& e, std::unique_ptr<MemberDev> md) {
+  int desc = -1;
+  std::vector<bool> used(e.max_members, false);
+  for (auto& x : e.members)
+    if (x) used[x->desc] = true;
+  for (int i = 0; i < e.max_members; i++)
+    if (!used[i]) {
+      desc = i;
+      break;
+    }
+  if (desc < 0) throw FullError("engine is full");
+  if (e.device < 0) {
+    md->desc = desc;
+    md->active = false;
+  } else {
+    try {
+      HIP_OR_THROW(hipSetDevice(e.device));
+      Member& m = md->host;
+      const size_t nphase = m.constants.size();
+      const size_t nbm1 = m.tlayout.count[TAB_bm1];
+      std::vector<float> arena(m.initial.begin(), m.initial.end());
+      std::vector<float> tab(m.tables.begin(), m.tables.end());
+      std::vector<float> consts(nphase * CON_COUNT), bm1(nphase * nbm1);
+      for (size_t p = 0; p < nphase; p++) {
+        for (int c = 0; c < CON_COUNT; c++) consts[p * CON_COUNT + c] = (float)m.constants[p][c];
+        for (size_t i = 0; i < nbm1; i++) bm1[p * nbm1 + i] = (float)m.bm1[p][i];
+      }
+      // the transform tables (every table before zgw) are one copy per
+      // truncation; the member's block holds the rest
+      const size_t shared_floats = m.tlayout.offset[TAB_zgw];
+      SharedTables& shared = e.transform_tables[m.dims.ntru];
+      if (!shared.d) {
+        HIP_OR_THROW(hipMalloc(&shared.d, shared_floats * sizeof(float)));
+        HIP_OR_THROW(hipMemcpy(shared.d, tab.data(), shared_floats * sizeof(float), hipMemcpyHostToDevice));
+      }
+      shared.refs++;
+      md->shared_key = m.dims.ntru;
+      HIP_OR_THROW(hipMalloc(&md->d_arena, arena.size() * sizeof(float)));
+      HIP_OR_THROW(hipMalloc(&md->d_tab_alloc, (tab.size() - shared_floats) * sizeof(float)));
+      md->d_tab =
+          reinterpret_cast<float*>(reinterpret_cast<uintptr_t>(md->d_tab_alloc) - shared_floats * sizeof(float));
+      HIP_OR_THROW(hipMalloc(&md->d_consts, consts.size() * sizeof(float)));
+      HIP_OR_THROW(hipMalloc(&md->d_bm1, bm1.size() * sizeof(float)));
+      HIP_OR_THROW(hipMemcpy(md->d_arena, arena.data(), arena.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIP_OR_THROW(hipMemcpy(md->d_tab_alloc, tab.data() + shared_floats,
+                             (tab.size() - shared_floats) * sizeof(float), hipMemcpyHostToDevice));
+      HIP_OR_THROW(hipMemcpy(md->d_consts, consts.data(), consts.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIP_OR_THROW(hipMemcpy(md->d_bm1, bm1.data(), bm1.size() * sizeof(float), hipMemcpyHostToDevice));
+      DevMember d{};
+      d.ntru = m.dims.ntru;
+      d.nlat = m.dims.nlat;
+      d.nlon = m.dims.nlon;
+      d.nlev = m.dims.nlev;
+      d.ncsp = exo_ncsp(d.ntru);
+      d.nrsp = exo_nrsp(d.ntru);
+      d.nfc = exo_nfc(d.ntru);
+      d.nhor = d.nlat * d.nlon;
+      d.precision = md->precision;
+      d.arena = md->d_arena;
+      d.tab = md->d_tab;
+      d.ltab = shared.d;
+      for (int sl = 0; sl < SLOT_COUNT; sl++) d.slot[sl] = (int)m.layout.offset[sl];
+      for (int t = 0; t < TAB_COUNT; t++) d.tabo[t] = (int)m.tlayout.offset[t];
+      for (int c = 0; c < CON_COUNT; c++) d.k[c] = consts[c];
+      e.h_desc[desc] = d;
+      HIP_OR_THROW(hipMemcpy(e.d_desc + desc, &d, sizeof(DevMember), hipMemcpyHostToDevice));
+      md->desc = desc;
+      md->loaded_phase = 0;
+      md->phase = 0;
+      md->carbon_count = 0;
+      md->active = true;
+    } catch (...) {
+      hipFree(md->d_arena);
+      hipFree(md->d_tab_alloc);
+      hipFree(md->d_consts);
+      hipFree(md->d_bm1);
+      release_shared_tables(e, md->shared_key);
+      throw;
+    }
+  }
+  int id = -1;
+  for (size_t i = 0; i < e.members.size(); i++)
+    if (!e.members[i]) {
+      id = (int)i;
+      break;
+    }
+  if (id < 0) {
+    id = (int)e.members.size();
+    e.members.emplace_back();
+  }
+  e.members[id] = std::move(md);
+  return id;
+}
+
+void drop_member(Engine& e, int id) {
+  MemberDev* m = e.members[id].get();
+  if (e.device >= 0) {
+    hipSetDevice(e.device);
+    hipStreamSynchronize(e.stream);
+    hipFree(m->d_arena);
+    hipFree(m->d_tab_alloc);
+    hipFree(m->d_consts);
+    hipFree(m->d_bm1);
+    release_shared_tables(e, m->shared_key);
+  }
+  e.members[id].reset();
+}
+
+}  // namespace exo
+
+// ------------------------------------------------------------------ C-ABI
 
 using exo::Engine;
 using exo::MemberDev;
@@ -244,126 +376,26 @@ void exo_destroy(exo_engine* h) {
 
 int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* member) {
   if (!h || !cfg || !member) return fail(EXO_E_ARG, "null argument");
-  if (precision == EXO_PRECISION_BF16_SPLIT)
-    return fail(EXO_E_ARG, "the split-operand Legendre form is not available in this build");
-  if (precision != EXO_PRECISION_FP32) return fail(EXO_E_ARG, "unknown precision");
-  Engine& e = h->e;
-  int desc = -1;
-  std::vector<bool> used(e.max_members, false);
-  for (auto& m : e.members)
-    if (m) used[m->desc] = true;
-  for (int i = 0; i < e.max_members; i++)
-    if (!used[i]) {
-      desc = i;
-      break;
-    }
-  if (desc < 0) return fail(EXO_E_ARG, "engine is full");
-  auto md = std::make_unique<MemberDev>();
+  std::unique_ptr<MemberDev> md;
   try {
-    exo::build_member(*cfg, md->host);
+    md = exo::prepare_member(*cfg, precision);
   } catch (const std::exception& ex) {
     return fail(EXO_E_ARG, ex.what());
   }
-  if (e.device < 0) {
-    md->desc = desc;
-    md->step = md->host.start_step;
-    md->active = false;
-  } else try {
-    HIP_OR_THROW(hipSetDevice(e.device));
-    exo::Member& m = md->host;
-    md->precision = precision;
-    const size_t nphase = m.constants.size();
-    const size_t nbm1 = m.tlayout.count[TAB_bm1];
-    std::vector<float> arena(m.initial.begin(), m.initial.end());
-    std::vector<float> tab(m.tables.begin(), m.tables.end());
-    std::vector<float> consts(nphase * CON_COUNT), bm1(nphase * nbm1);
-    for (size_t p = 0; p < nphase; p++) {
-      for (int c = 0; c < CON_COUNT; c++) consts[p * CON_COUNT + c] = (float)m.constants[p][c];
-      for (size_t i = 0; i < nbm1; i++) bm1[p * nbm1 + i] = (float)m.bm1[p][i];
-    }
-    // the transform tables (every table before zgw) are one copy per
-    // truncation; the member's block holds the rest
-    const size_t shared_floats = m.tlayout.offset[TAB_zgw];
-    exo::SharedTables& shared = e.transform_tables[m.dims.ntru];
-    if (!shared.d) {
-      HIP_OR_THROW(hipMalloc(&shared.d, shared_floats * sizeof(float)));
-      HIP_OR_THROW(hipMemcpy(shared.d, tab.data(), shared_floats * sizeof(float), hipMemcpyHostToDevice));
-    }
-    shared.refs++;
-    md->shared_key = m.dims.ntru;
-    HIP_OR_THROW(hipMalloc(&md->d_arena, arena.size() * sizeof(float)));
-    HIP_OR_THROW(hipMalloc(&md->d_tab_alloc, (tab.size() - shared_floats) * sizeof(float)));
-    md->d_tab = reinterpret_cast<float*>(reinterpret_cast<uintptr_t>(md->d_tab_alloc) - shared_floats * sizeof(float));
-    HIP_OR_THROW(hipMalloc(&md->d_consts, consts.size() * sizeof(float)));
-    HIP_OR_THROW(hipMalloc(&md->d_bm1, bm1.size() * sizeof(float)));
-    HIP_OR_THROW(hipMemcpy(md->d_arena, arena.data(), arena.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIP_OR_THROW(hipMemcpy(md->d_tab_alloc, tab.data() + shared_floats, (tab.size() - shared_floats) * sizeof(float),
-                           hipMemcpyHostToDevice));
-    HIP_OR_THROW(hipMemcpy(md->d_consts, consts.data(), consts.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIP_OR_THROW(hipMemcpy(md->d_bm1, bm1.data(), bm1.size() * sizeof(float), hipMemcpyHostToDevice));
-    DevMember d{};
-    d.ntru = m.dims.ntru;
-    d.nlat = m.dims.nlat;
-    d.nlon = m.dims.nlon;
-    d.nlev = m.dims.nlev;
-    d.ncsp = exo_ncsp(d.ntru);
-    d.nrsp = exo_nrsp(d.ntru);
-    d.nfc = exo_nfc(d.ntru);
-    d.nhor = d.nlat * d.nlon;
-    d.precision = precision;
-    d.arena = md->d_arena;
-    d.tab = md->d_tab;
-    d.ltab = shared.d;
-    for (int s = 0; s < SLOT_COUNT; s++) d.slot[s] = (int)m.layout.offset[s];
-    for (int t = 0; t < TAB_COUNT; t++) d.tabo[t] = (int)m.tlayout.offset[t];
-    for (int c = 0; c < CON_COUNT; c++) d.k[c] = consts[c];
-    e.h_desc[desc] = d;
-    HIP_OR_THROW(hipMemcpy(e.d_desc + desc, &d, sizeof(DevMember), hipMemcpyHostToDevice));
-    md->desc = desc;
-    md->loaded_phase = 0;
-    md->phase = 0;
-    md->step = m.start_step;
-    md->carbon_count = 0;
-    md->active = true;
+  try {
+    *member = exo::admit_member(h->e, std::move(md));
+  } catch (const exo::FullError& ex) {
+    return fail(EXO_E_ARG, ex.what());
   } catch (const std::exception& ex) {
-    hipFree(md->d_arena);
-    hipFree(md->d_tab_alloc);
-    hipFree(md->d_consts);
-    hipFree(md->d_bm1);
-    release_shared_tables(e, md->shared_key);
     return fail(EXO_E_DEVICE, ex.what());
   }
-  int id = -1;
-  for (size_t i = 0; i < e.members.size(); i++)
-    if (!e.members[i]) {
-      id = (int)i;
-      break;
-    }
-  if (id < 0) {
-    id = (int)e.members.size();
-    e.members.emplace_back();
-  }
-  e.members[id] = std::move(md);
-  *member = id;
   return EXO_OK;
 }
 
 int exo_remove_member(exo_engine* h, int id) {
   MemberDev* m = member_of(h, id);
   if (!m) return fail(EXO_E_ARG, "no member " + std::to_string(id));
-  Engine& e = h->e;
-  if (e.device < 0) {
-    e.members[id].reset();
-    return EXO_OK;
-  }
-  hipSetDevice(e.device);
-  hipStreamSynchronize(e.stream);
-  hipFree(m->d_arena);
-  hipFree(m->d_tab_alloc);
-  hipFree(m->d_consts);
-  hipFree(m->d_bm1);
-  release_shared_tables(e, m->shared_key);
-  e.members[id].reset();
+  exo::drop_member(h->e, id);
   return EXO_OK;
 }
 

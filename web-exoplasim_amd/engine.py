@@ -58,6 +58,26 @@ class ExoConfig(ctypes.Structure):
     ]
 
 
+class ExoRequest(ctypes.Structure):
+    """exo_request: one run of the serving loop."""
+
+    _fields_ = [("config", ExoConfig), ("precision", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("arrival_s", ctypes.c_double), ("steps", ctypes.c_int64)]
+
+
+class ExoResult(ctypes.Structure):
+    """exo_result: a served run's times and its global means at the end."""
+
+    _fields_ = [("arrival_s", ctypes.c_double), ("admitted_s", ctypes.c_double),
+                ("finished_s", ctypes.c_double), ("tsurf", ctypes.c_double), ("toa_net", ctypes.c_double),
+                ("steps", ctypes.c_int64), ("status", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class ExoServeOptions(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_chunk_steps", ctypes.c_int),
+                ("prepare_threads", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -99,6 +119,8 @@ def library():
         "exo_step_values": (I, [P, I, ctypes.c_int64, D]),
         "exo_last_step_ms": (I, [P, D]),
         "exo_probe_kernel": (I, [P, ctypes.c_char_p, I]),
+        "exo_serve": (I, [P, ctypes.POINTER(ExoRequest), I, ctypes.POINTER(ExoServeOptions),
+                          ctypes.POINTER(ExoResult), D]),
         "exo_probe_result": (I, [P, D, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "exo_set_publish_every": (I, [P, I]),
         "exo_read_snapshot": (I, [P, I, F, S]),
@@ -224,6 +246,24 @@ class Engine:
         v = ctypes.c_double()
         _check(self.lib.exo_last_step_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def serve(self, requests, hbm_budget_bytes=0, max_chunk_steps=32, prepare_threads=4):
+        """Runs the serving loop over [(arrival_s, config, steps)] (FP32
+        form); returns (results as dicts in request order, wall seconds)."""
+        n = len(requests)
+        req = (ExoRequest * max(n, 1))()
+        for i, (arrival, cfg, steps) in enumerate(requests):
+            req[i].config = cfg
+            req[i].precision = EXO_PRECISION_FP32
+            req[i].arrival_s = float(arrival)
+            req[i].steps = int(steps)
+        res = (ExoResult * max(n, 1))()
+        opt = ExoServeOptions(hbm_budget_bytes, max_chunk_steps, prepare_threads)
+        wall = ctypes.c_double()
+        _check(self.lib.exo_serve(self.h, req, n, ctypes.byref(opt), res, ctypes.byref(wall)))
+        out = [{name: getattr(res[i], name) for name, _ in ExoResult._fields_ if name != "reserved"}
+               for i in range(n)]
+        return out, wall.value
 
     def probe_kernel(self, kernel, max_launches=4096):
         """Times each later launch of `kernel` with HIP events (None stops)."""
