@@ -38,11 +38,11 @@ def _package():
     return entry._package()
 
 
-def run_gpu(exo, cfgs, steps, warmup, probe):
+def run_gpu(exo, cfgs, steps, warmup, probe, precision=0):
     """Steps every config together; returns (wall s, device ms, probe)."""
     eng = exo.Engine(0, len(cfgs))
     try:
-        ids = [eng.add_member(c) for c in cfgs]
+        ids = [eng.add_member(c, precision=precision) for c in cfgs]
         eng.step(warmup)
         eng.sync()
         if probe:
@@ -141,6 +141,9 @@ def main():
     ap.add_argument("--steps", type=int, default=320)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--workload", default="batched", choices=["batched", "plain", "serving"])
+    ap.add_argument("--form", default="plain", choices=["split", "plain"],
+                    help="operand form of the batched ensemble's Legendre products (the other form is "
+                         "timed beside it)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--probe", default="auto")
@@ -171,17 +174,23 @@ def main():
             "serving": sv, "roofline": None, "cpu_baseline": None}))
         return
 
+    precision = 0
     if args.workload == "batched":
         cfgs = workloads.ensemble(exo)
+        precision = exo.EXO_PRECISION_SPLIT if args.form == "split" else exo.EXO_PRECISION_FP32
         workload = {"workload": "64-planet mixed ensemble: 32 x T21 L5, 24 x T42 L10, 8 x T63 L10, "
                                 "45-min step, seeded parameters (seed 64)",
-                    "members": len(cfgs), "timestep_minutes": 45, "form": "plain FP32"}
+                    "members": len(cfgs), "timestep_minutes": 45,
+                    "form": ("split-operand Legendre (FP16 hi/lo, FP32 accumulate)" if args.form == "split"
+                             else "plain FP32")}
     else:
         cfgs = [workloads.plain(exo, 63, 10, 30)]
         workload = {"workload": "plain FP32 single planet T63 L10, 30-min step",
                     "truncation": 63, "levels": 10, "timestep_minutes": 30, "members": 1}
     probe = roofline.DOMINANT[args.workload] if args.probe == "auto" else (args.probe or None)
-    wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probe)
+    if probe and precision == exo.EXO_PRECISION_SPLIT and probe in ("k_leg_inv", "k_leg_fwd"):
+        probe += "_split"
+    wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probe, precision)
     days = sum(args.steps / workloads.steps_per_day(c.timestep_minutes) for c in cfgs)
     value = days / wall
     line = {
@@ -201,6 +210,13 @@ def main():
         "config": workload,
     }
     if args.workload == "batched" and not args.no_plain:
+        other = exo.EXO_PRECISION_FP32 if precision == exo.EXO_PRECISION_SPLIT else exo.EXO_PRECISION_SPLIT
+        ow, _, _ = run_gpu(exo, cfgs, args.steps, args.warmup, None, other)
+        name = "batched_plain_fp32" if other == exo.EXO_PRECISION_FP32 else "batched_split_fp16"
+        line[name] = {"value": days / ow, "unit": "simulated days/s", "ms_per_step": ow * 1000.0 / args.steps,
+                      "form": ("plain FP32" if other == exo.EXO_PRECISION_FP32
+                               else "split-operand Legendre (FP16 hi/lo, FP32 accumulate)")}
+    if args.workload == "batched" and not args.no_plain:
         plain_cfg = workloads.plain(exo, 63, 10, 30)
         pw, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None)
         plain_value = (480 / 48) / pw
@@ -208,6 +224,8 @@ def main():
                                 "ms_per_step": pw * 1000.0 / 480,
                                 "config": "plain FP32 single planet T63 L10, 30-min step"}
         line["batched_vs_plain"] = value / plain_value
+        if "batched_split_fp16" in line:
+            line["batched_split_vs_plain"] = line["batched_split_fp16"]["value"] / plain_value
     if args.workload == "batched" and not args.no_serving:
         line["serving"] = run_serving(exo, args.serve_seconds)
     if probed:

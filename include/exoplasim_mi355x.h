@@ -36,7 +36,7 @@ extern "C" {
 
 /* Precision of the Legendre contractions. */
 #define EXO_PRECISION_FP32 0       /* plain form: FP32 throughout */
-#define EXO_PRECISION_BF16_SPLIT 1 /* BF16 hi/lo split operands, FP32 acc. */
+#define EXO_PRECISION_SPLIT 1 /* Legendre on FP16 hi/lo split operands, FP32 acc. */
 
 /*
  * Planet and run configuration: the subset of the reference's GcmConfig
@@ -94,7 +94,7 @@ void exo_destroy(exo_engine *e);
  * runs the reference's nkits start-up steps on its first exo_step calls.
  * Stands in for SETUP.state + SETUP.schedule + STEPPER.context
  * (setup/index.ts:434-477, stepper/index.ts:94-119).
- * `precision` is EXO_PRECISION_FP32 or EXO_PRECISION_BF16_SPLIT.
+ * `precision` is EXO_PRECISION_FP32 or EXO_PRECISION_SPLIT.
  * Writes the member id to *member.
  */
 int exo_add_member(exo_engine *e, const exo_config *cfg, int precision,

@@ -5,6 +5,6 @@ behind the C-ABI of include/exoplasim_mi355x.h); this package holds the
 Python host binding (engine.py) and the runner that mirrors the reference's
 engine interface (runner.py).
 """
-from .engine import (EXO_PRECISION_BF16_SPLIT, EXO_PRECISION_FP32, Engine,  # noqa: F401
+from .engine import (EXO_PRECISION_SPLIT, EXO_PRECISION_FP32, Engine,  # noqa: F401
                      ExoConfig, ExoError, constant_names, default_config,
                      library, slot_names)

@@ -20,7 +20,10 @@ struct DevMember {
   // The transform tables (Fourier and Legendre, tabo below zgw), one copy
   // shared by every member of the same truncation: ltab + tabo[t].
   const float* ltab;
-  const void* reserved_;
+  // Power-of-two scale of each Legendre table (qc, qe, qm, qq, qi, qj, qu,
+  // qv) that puts its largest magnitude in [2^12, 2^13): the split form's
+  // FP16 operands.
+  float lscale[8];
   int slot[SLOT_COUNT];
   int tabo[TAB_COUNT];
   float k[CON_COUNT];

@@ -20,6 +20,8 @@ extern "C" __global__ void k_leg_inv(const DevMember*, LaunchPlan, InvJobs);
 extern "C" __global__ void k_dft_inv(const DevMember*, LaunchPlan, InvJobs);
 extern "C" __global__ void k_dft_fwd(const DevMember*, LaunchPlan, FwdJobs);
 extern "C" __global__ void k_leg_fwd(const DevMember*, LaunchPlan, LegJobs);
+extern "C" __global__ void k_leg_inv_split(const DevMember*, LaunchPlan, InvJobs);
+extern "C" __global__ void k_leg_fwd_split(const DevMember*, LaunchPlan, LegJobs);
 extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan, int);
 extern "C" __global__ void k_mean_mode(const DevMember*, LaunchPlan, int);
 extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
@@ -82,6 +84,7 @@ struct KernelProbe {
 struct SharedTables {
   float* d = nullptr;
   int refs = 0;
+  float lscale[8] = {};  // DevMember::lscale
 };
 
 struct Engine {
@@ -165,8 +168,12 @@ class Stepper {
     launch(e_, k_mean_mode, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, slot);
   }
   void inv(const InvJobs& jobs) {
-    const LaunchPlan pl = plan([&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
-    launch(e_, k_leg_inv, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
+    // the Legendre half per operand form (plain FP32, split FP16)
+    for (int form = 0; form < 2; form++) {
+      const LaunchPlan pl = plan([&](const DevMember& d) { return d.precision == form ? (d.ntru + 1) * jobs.n : 0; });
+      if (pl.first[pl.n] > 0)
+        launch(e_, form ? k_leg_inv_split : k_leg_inv, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
+    }
     // k_dft_inv: 64 rows (output x level x latitude) of one job per workgroup
     const LaunchPlan pd = plan([&](const DevMember& d) {
       int blocks = 0;
@@ -186,8 +193,11 @@ class Stepper {
     launch(e_, k_dft_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
   }
   void leg(const LegJobs& jobs) {
-    const LaunchPlan p = plan([&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
-    launch(e_, k_leg_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
+    for (int form = 0; form < 2; form++) {
+      const LaunchPlan p = plan([&](const DevMember& d) { return d.precision == form ? (d.ntru + 1) * jobs.n : 0; });
+      if (p.first[p.n] > 0)
+        launch(e_, form ? k_leg_fwd_split : k_leg_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
+    }
   }
   void load_step(const float* d_params) {
     const LaunchPlan p = plan([](const DevMember&) { return 1; });

@@ -165,9 +165,8 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
 namespace exo {
 
 std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision) {
-  if (precision == EXO_PRECISION_BF16_SPLIT)
-    throw std::runtime_error("the split-operand Legendre form is not available in this build");
-  if (precision != EXO_PRECISION_FP32) throw std::runtime_error("unknown precision");
+  if (precision != EXO_PRECISION_FP32 && precision != EXO_PRECISION_SPLIT)
+    throw std::runtime_error("unknown precision");
   auto md = std::make_unique<MemberDev>();
   build_member(cfg, md->host);
   md->precision = precision;
@@ -218,6 +217,14 @@ int admit_member(Engine& e, std::unique_ptr<MemberDev> md) {
       if (!shared.d) {
         HIP_OR_THROW(hipMalloc(&shared.d, shared_floats * sizeof(float)));
         HIP_OR_THROW(hipMemcpy(shared.d, tab.data(), shared_floats * sizeof(float), hipMemcpyHostToDevice));
+        for (int t = 0; t < 8; t++) {
+          const size_t off = m.tlayout.offset[TAB_legendre_qc + t], n = m.tlayout.count[TAB_legendre_qc + t];
+          float mx = 0.0f;
+          for (size_t x = 0; x < n; x++) mx = std::max(mx, std::fabs(tab[off + x]));
+          int e = 0;
+          std::frexp(mx, &e);
+          shared.lscale[t] = mx > 0.0f ? std::ldexp(1.0f, 13 - e) : 1.0f;
+        }
       }
       shared.refs++;
       md->shared_key = m.dims.ntru;
@@ -245,6 +252,7 @@ int admit_member(Engine& e, std::unique_ptr<MemberDev> md) {
       d.arena = md->d_arena;
       d.tab = md->d_tab;
       d.ltab = shared.d;
+      for (int t = 0; t < 8; t++) d.lscale[t] = shared.lscale[t];
       for (int sl = 0; sl < SLOT_COUNT; sl++) d.slot[sl] = (int)m.layout.offset[sl];
       for (int t = 0; t < TAB_COUNT; t++) d.tabo[t] = (int)m.tlayout.offset[t];
       for (int c = 0; c < CON_COUNT; c++) d.k[c] = consts[c];
@@ -512,6 +520,7 @@ namespace {
 const std::pair<const char*, const void*> kProbeKernels[] = {
 #define EXO_PROBE(k) {#k, reinterpret_cast<const void*>(k)},
     EXO_PROBE(k_load_step) EXO_PROBE(k_leg_inv) EXO_PROBE(k_dft_inv) EXO_PROBE(k_dft_fwd) EXO_PROBE(k_leg_fwd)
+    EXO_PROBE(k_leg_inv_split) EXO_PROBE(k_leg_fwd_split)
     EXO_PROBE(k_calcgp) EXO_PROBE(k_spectrala) EXO_PROBE(k_spectrald_diffusion) EXO_PROBE(k_spectrald_update)
     EXO_PROBE(k_mkdheat) EXO_PROBE(k_mean_mode) EXO_PROBE(k_physics_in) EXO_PROBE(k_fixer_column)
     EXO_PROBE(k_fixer_reduce) EXO_PROBE(k_surface_fluxes) EXO_PROBE(k_shortwave) EXO_PROBE(k_longwave)

@@ -132,6 +132,77 @@ __device__ __forceinline__ Col fwd_col(int kind, int blk, int c, int NC) {
   return col_of(0, p, q ? 0 : 1, q ? -1.0f : 1.0f, kRawLat);
 }
 
+// ---------------------------------------------- split-operand form
+// With M.precision = 1 the products run on v_mfma_f32_16x16x32_f16 with
+// every operand split into FP16 hi + lo parts, x s = hi + lo (hi the
+// nearest FP16, lo the FP16 of the remainder), as hi.hi + hi.lo + lo.hi
+// with FP32 accumulation: 22 significant bits per operand against FP32's
+// 24. The scale s is a power of two per Legendre table (set once, from
+// the whole table) and per staged input row (level and part of one field
+// at wavenumber m), chosen so the largest magnitude lands in
+// [2^12, 2^13), well inside FP16's range; the sums are unscaled exactly.
+// Operands are split once, while staged, into hi and lo arrays laid out
+// so each lane's 8 reduction values are one 16-byte LDS read.
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split1(float x, _Float16* hi, _Float16* lo) {
+  const _Float16 h = (_Float16)x;
+  *hi = h;
+  *lo = (_Float16)(x - (float)h);
+}
+
+// Power of two putting a largest magnitude in [2^12, 2^13); 1 for zero.
+__device__ __forceinline__ float pow2_scale(float maxabs) {
+  if (!(maxabs > 0.0f)) return 1.0f;
+  int e;
+  frexpf(maxabs, &e);
+  e = e < -100 ? -100 : e;
+  return ldexpf(1.0f, 13 - e);
+}
+
+// Split staging of input rows: each wave owns rows fp = wave, wave + 4, ...
+// (field and level), its lanes walk the row's `cols` values x = 2 v + part
+// (v = mode or latitude); the row's scale per part comes from a wave
+// reduction over the lanes of that part, then the values go to
+// hi/lo[(fp * 2 + part) * stride + v].
+template <typename Load>
+__device__ __forceinline__ void stage_rows_split(int rows, int cols, int stride, Load load, _Float16* hi,
+                                                 _Float16* lo, float* scale) {
+  const int lane = threadIdx.x & 63;
+  for (int fp = threadIdx.x >> 6; fp < rows; fp += 4) {
+    float v[3];
+    float mx = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int x = lane + 64 * j;
+      v[j] = x < cols ? load(fp, x) : 0.0f;
+      mx = fmaxf(mx, fabsf(v[j]));
+    }
+    // lanes of one parity hold one part
+#pragma unroll
+    for (int o = 32; o > 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    const float s = pow2_scale(mx);
+    if (lane < 2) scale[fp * 2 + lane] = s;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int x = lane + 64 * j;
+      if (x < cols) {
+        const int at = (fp * 2 + (x & 1)) * stride + (x >> 1);
+        split1(v[j] * s, hi + at, lo + at);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ h8 load_h8(const _Float16* p) { return *reinterpret_cast<const h8*>(p); }
+
+__device__ __forceinline__ f4 mfma_split(const h8& ahi, const h8& alo, const h8& bhi, const h8& blo, f4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blo, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, c, 0, 0, 0);
+}
+
 // ------------------------------------------------------------- inverse
 
 template <int P>
@@ -292,16 +363,234 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* Ah,
   }
 }
 
+
+// ------------------------------------------------- split-operand bodies
+
+// Where a column's operand row sits in the split input rows, and its sign.
+struct SCol {
+  int row;
+  float sign;
+};
+
+__device__ __forceinline__ SCol scol(int field, int p, int part, float sign, int P) {
+  return SCol{(field * P + p) * 2 + part, sign};
+}
+
+template <int P>
+__device__ __forceinline__ SCol inv_scol(bool winds, int blk, int c, int NC) {
+  if (c >= NC) return SCol{0, 0.0f};
+  if (!winds) return scol(0, c >> 1, c & 1, 1.0f, P);
+  const int p = c >> 2, q = c & 3;
+  if (blk == 0) return scol(q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? -1.0f : 1.0f, P);
+  return scol((q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? -1.0f : 1.0f, P);
+}
+
+template <int P>
+__device__ __forceinline__ SCol fwd_scol(int kind, int blk, int c, int NC) {
+  if (c >= NC) return SCol{0, 0.0f};
+  if (kind == LEG_PLAIN) return scol(0, c >> 1, c & 1, 1.0f, P);
+  if (kind == LEG_UV2DV) {
+    const int p = c >> 2, q = c & 3;
+    if (blk == 0) return scol((q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? 1.0f : -1.0f, P);
+    if (blk == 1) return scol(q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? 1.0f : -1.0f, P);
+    return q < 2 ? scol(2, p, q, 1.0f, P) : SCol{0, 0.0f};
+  }
+  const int p = c >> 1, q = c & 1;
+  if (blk == 0) return scol(1, p, q, 1.0f, P);
+  if (blk == 1) return scol(2, p, q, 1.0f, P);
+  return scol(0, p, q ? 0 : 1, q ? -1.0f : 1.0f, P);
+}
+
+// LDS rows of the split operands: 8 halves of padding keep 16-byte
+// alignment and spread the rows over the banks.
+constexpr int kSInvA = kMaxK + 8;     // inverse A row: modes
+constexpr int kSFwd = kMaxLat + 8;    // forward A row and input row: latitudes
+constexpr int kSInvB = kMaxK + 8;     // inverse input row: modes
+
+struct SplitInvLds {
+  _Float16 ahi[kMaxLat * kSInvA], alo[kMaxLat * kSInvA];
+  _Float16 bhi[2 * kMaxLev * 2 * kSInvB], blo[2 * kMaxLev * 2 * kSInvB];
+  float bscale[2 * kMaxLev * 2];
+};
+
+struct SplitFwdLds {
+  _Float16 ahi[kMaxK * kSFwd], alo[kMaxK * kSFwd];
+  _Float16 bhi[3 * kMaxLev * 2 * kSFwd], blo[3 * kMaxLev * 2 * kSFwd];
+  float bscale[3 * kMaxLev * 2];
+};
+
+template <int P>
+__device__ void leg_inv_split(const DevMember& M, const InvJob& job, int m, SplitInvLds& L) {
+  const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
+  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
+  const bool winds = job.kind == INV_DV2UV;
+  const int NC = winds ? 4 * P : 2 * P;
+  const int NT = (NC + 15) / 16, MT = nlat / 16;
+  const int R = (K + 31) & ~31;  // FP16 MFMA depth
+  const int nf = winds ? 2 : 1;
+  const float plavor = job.plavor ? M.k[CON_plavor] : 0.0f;
+  // the job's coefficients of m (zeta without the planetary vorticity in
+  // Re mode (0, 1)), zeros past K, split per row
+  stage_rows_split(
+      nf * P, 2 * R, kSInvB,
+      [&](int fp, int x) {
+        const int f = fp / P, p = fp % P, k = x >> 1;
+        if (k >= K) return 0.0f;
+        const float v = M.arena[M.slot[f ? job.src1 : job.src0] + p * nrsp + 2 * w0 + x];
+        return (winds && f == 1 && w0 + k == 1 && (x & 1) == 0) ? v - plavor : v;
+      },
+      L.bhi, L.blo, L.bscale);
+  f4 acc[kMaxTiles];
+#pragma unroll
+  for (int t = 0; t < kMaxTiles; t++) acc[t] = f4{0, 0, 0, 0};
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const int blocks = winds ? 2 : 1;
+  for (int blk = 0; blk < blocks; blk++) {
+    const int table = winds ? (blk == 0 ? (int)TAB_legendre_qu : (int)TAB_legendre_qv) : job.table;
+    const float* q = M.ltab + M.tabo[table];
+    const float sa = M.lscale[table - TAB_legendre_qc];
+    stage(
+        nlat, R, [&](int lat, int k) { return k < K ? q[lat * ncsp + w0 + k] * sa : 0.0f; },
+        [&](int lat, int k, float v) { split1(v, &L.ahi[lat * kSInvA + k], &L.alo[lat * kSInvA + k]); });
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kMaxTiles; t++) {
+      const int tile = wave + 4 * t;
+      if (tile < MT * NT) {
+        const int mt = tile / NT, nt = tile % NT;
+        const SCol col = inv_scol<P>(winds, blk, nt * 16 + i, NC);
+        const int arow = (mt * 16 + i) * kSInvA + 8 * g, brow = col.row * kSInvB + 8 * g;
+        f4 c = f4{0, 0, 0, 0};
+        for (int r = 0; r < R; r += 32)
+          c = mfma_split(load_h8(L.ahi + arow + r), load_h8(L.alo + arow + r), load_h8(L.bhi + brow + r),
+                         load_h8(L.blo + brow + r), c);
+        acc[t] += c * (col.sign / (sa * L.bscale[col.row]));
+      }
+    }
+    __syncthreads();
+  }
+  float* F0 = M.arena + M.slot[job.four0];
+  float* F1 = winds ? M.arena + M.slot[job.four1] : nullptr;
+#pragma unroll
+  for (int t = 0; t < kMaxTiles; t++) {
+    const int tile = wave + 4 * t;
+    if (tile < MT * NT) {
+      const int mt = tile / NT, nt = tile % NT;
+      const int c = nt * 16 + i;
+      if (c < NC) {
+        float* base = winds ? ((c & 2) ? F1 : F0) : F0;
+        const int p = winds ? c >> 2 : c >> 1, part = c & 1;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int lat = mt * 16 + 4 * g + r;
+          base[(p * nlat + lat) * nfc + 2 * m + part] = acc[t][r];
+        }
+      }
+    }
+  }
+}
+
+template <int P>
+__device__ void leg_fwd_split(const DevMember& M, const LegJob& job, int m, SplitFwdLds& L) {
+  const int nlat = M.nlat, half = nlat / 2, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
+  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
+  const bool uv = job.kind == LEG_UV2DV;
+  const int NC = uv ? 4 * P : 2 * P;
+  const int NT = (NC + 15) / 16, MT = (K + 15) / 16;
+  int tables[3], diff[3], blocks = 0;
+  if (job.kind == LEG_PLAIN) {
+    tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
+  } else if (uv) {
+    tables[blocks] = TAB_legendre_qe, diff[blocks++] = 1;
+    tables[blocks] = TAB_legendre_qm, diff[blocks++] = 0;
+    if (job.ke >= 0) tables[blocks] = TAB_legendre_qq, diff[blocks++] = 0;
+  } else {
+    tables[blocks] = TAB_legendre_qe, diff[blocks++] = 1;
+    tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qm, diff[blocks++] = 0;
+  }
+  const int fields[3] = {job.kind == LEG_PLAIN ? job.src : job.u, job.v, uv ? job.ke : job.src};
+  const int nf = job.kind == LEG_PLAIN ? 1 : (uv && job.ke < 0) ? 2 : 3;
+  stage_rows_split(
+      nf * P, 2 * nlat, kSFwd,
+      [&](int fp, int x) {
+        const int f = fp / P, p = fp % P;
+        return M.arena[M.slot[fields[f]] + (p * nlat + (x >> 1)) * nfc + 2 * m + (x & 1)];
+      },
+      L.bhi, L.blo, L.bscale);
+  f4 acc[kMaxTiles];
+#pragma unroll
+  for (int t = 0; t < kMaxTiles; t++) acc[t] = f4{0, 0, 0, 0};
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  for (int blk = 0; blk < blocks; blk++) {
+    const float* q = M.ltab + M.tabo[tables[blk]];
+    const float sa = M.lscale[tables[blk] - TAB_legendre_qc];
+    const int dsign = diff[blk];
+    // A[k][lat] over every latitude: southern rows (-1)^(n-m) q (negated
+    // again for qe*), exact in FP16
+    stage(
+        half, MT * 16, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] * sa : 0.0f; },
+        [&](int l, int k, float v) {
+          const float s = ((k & 1) ? -1.0f : 1.0f) * (dsign ? -1.0f : 1.0f);
+          _Float16 h, lo;
+          split1(v, &h, &lo);
+          L.ahi[k * kSFwd + l] = h;
+          L.alo[k * kSFwd + l] = lo;
+          L.ahi[k * kSFwd + nlat - 1 - l] = (_Float16)s * h;
+          L.alo[k * kSFwd + nlat - 1 - l] = (_Float16)s * lo;
+        });
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kMaxTiles; t++) {
+      const int tile = wave + 4 * t;
+      if (tile < MT * NT) {
+        const int mt = tile / NT, nt = tile % NT;
+        const SCol col = fwd_scol<P>(job.kind, blk, nt * 16 + i, NC);
+        const int arow = (mt * 16 + i) * kSFwd + 8 * g, brow = col.row * kSFwd + 8 * g;
+        f4 c = f4{0, 0, 0, 0};
+        for (int r = 0; r < nlat; r += 32)
+          c = mfma_split(load_h8(L.ahi + arow + r), load_h8(L.alo + arow + r), load_h8(L.bhi + brow + r),
+                         load_h8(L.blo + brow + r), c);
+        acc[t] += c * (col.sign / (sa * L.bscale[col.row]));
+      }
+    }
+    __syncthreads();
+  }
+  float* O0 = M.arena + M.slot[job.dst0];
+  float* O1 = uv ? M.arena + M.slot[job.dst1] : nullptr;
+#pragma unroll
+  for (int t = 0; t < kMaxTiles; t++) {
+    const int tile = wave + 4 * t;
+    if (tile < MT * NT) {
+      const int mt = tile / NT, nt = tile % NT;
+      const int c = nt * 16 + i;
+      if (c < NC) {
+        float* base = uv ? ((c & 2) ? O1 : O0) : O0;
+        const int p = uv ? c >> 2 : c >> 1, part = c & 1;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int k = mt * 16 + 4 * g + r;
+          if (k < K) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
-#define EXO_PLANES_DISPATCH(P, call) \
-  do {                               \
-    if ((P) == 10)                   \
-      call(10);                      \
-    else if ((P) == 5)               \
-      call(5);                       \
-    else                             \
-      call(1);                       \
+// One kernel per operand form, so each is allocated only the registers and
+// LDS of its own path; the step launches each over the members of its form.
+#define EXO_P_DISPATCH(P, call) \
+  do {                          \
+    if ((P) == 10)              \
+      call(10);                 \
+    else if ((P) == 5)          \
+      call(5);                  \
+    else                        \
+      call(1);                  \
   } while (0)
 
 extern "C" __global__ __launch_bounds__(256) void k_leg_inv(const DevMember* __restrict__ mems, LaunchPlan plan,
@@ -311,11 +600,21 @@ extern "C" __global__ __launch_bounds__(256) void k_leg_inv(const DevMember* __r
   int tile;
   const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
   const int ntp1 = M.ntru + 1;
-  const int m = tile % ntp1;
   const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
-  const int P = job.per_level ? M.nlev : 1;
-#define CALL(PP) leg_inv<PP>(M, job, m, As, raw)
-  EXO_PLANES_DISPATCH(P, CALL);
+#define CALL(PP) leg_inv<PP>(M, job, tile % ntp1, As, raw)
+  EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
+#undef CALL
+}
+
+extern "C" __global__ __launch_bounds__(256) void k_leg_inv_split(const DevMember* __restrict__ mems,
+                                                                   LaunchPlan plan, InvJobs jobs) {
+  __shared__ __align__(16) SplitInvLds L;
+  int tile;
+  const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
+  const int ntp1 = M.ntru + 1;
+  const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
+#define CALL(PP) leg_inv_split<PP>(M, job, tile % ntp1, L)
+  EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
 
@@ -326,10 +625,20 @@ extern "C" __global__ __launch_bounds__(256) void k_leg_fwd(const DevMember* __r
   int tile;
   const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
   const int ntp1 = M.ntru + 1;
-  const int m = tile % ntp1;
   const LegJob job = jobs.j[job_of(jobs, tile / ntp1)];
-  const int P = job.per_level ? M.nlev : 1;
-#define CALL(PP) leg_fwd<PP>(M, job, m, Ah, raw)
-  EXO_PLANES_DISPATCH(P, CALL);
+#define CALL(PP) leg_fwd<PP>(M, job, tile % ntp1, Ah, raw)
+  EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
+#undef CALL
+}
+
+extern "C" __global__ __launch_bounds__(256) void k_leg_fwd_split(const DevMember* __restrict__ mems,
+                                                                   LaunchPlan plan, LegJobs jobs) {
+  __shared__ __align__(16) SplitFwdLds L;
+  int tile;
+  const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
+  const int ntp1 = M.ntru + 1;
+  const LegJob job = jobs.j[job_of(jobs, tile / ntp1)];
+#define CALL(PP) leg_fwd_split<PP>(M, job, tile % ntp1, L)
+  EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }

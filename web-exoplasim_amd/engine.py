@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libexoplasim_mi355x.so")
 
 EXO_PRECISION_FP32 = 0
-EXO_PRECISION_BF16_SPLIT = 1
+EXO_PRECISION_SPLIT = 1
 
 ERRORS = {-1: "argument", -2: "device", -3: "state", -4: "unstable", -5: "format"}
 

@@ -349,8 +349,8 @@ napi_value init(napi_env env, napi_value exports) {
   napi_value v;
   napi_create_int32(env, EXO_PRECISION_FP32, &v);
   napi_set_named_property(env, exports, "PRECISION_FP32", v);
-  napi_create_int32(env, EXO_PRECISION_BF16_SPLIT, &v);
-  napi_set_named_property(env, exports, "PRECISION_BF16_SPLIT", v);
+  napi_create_int32(env, EXO_PRECISION_SPLIT, &v);
+  napi_set_named_property(env, exports, "PRECISION_SPLIT", v);
   return exports;
 }
 

@@ -102,7 +102,7 @@ function start(request) {
 		try {
 			status("loading")
 			state.engine = addon.create(request.device === undefined ? 0 : request.device, 1)
-			const precision = request.precision === "split" ? addon.PRECISION_BF16_SPLIT : addon.PRECISION_FP32
+			const precision = request.precision === "split" ? addon.PRECISION_SPLIT : addon.PRECISION_FP32
 			state.id = addon.addMember(state.engine, engineConfig(request.config, request.planet), precision)
 			addon.setPublishEvery(state.engine, Math.max(1, request.publishEvery | 0))
 			if (request.restart) addon.restore(state.engine, state.id, request.restart)
