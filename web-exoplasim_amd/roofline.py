@@ -23,6 +23,8 @@ one step (the job lists the step enqueues):
     k_convection (kuo, mkshallow, mkdca's column pass): reads 5L + 3,
       writes 8L + 2 -> 13L + 5.
 """
+import os
+
 FP32_PEAK_TFLOPS = 157.3
 FP16_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA: the split form's bound (its 3 products per term not counted)
 HBM_PEAK_GBS = 8000.0
@@ -65,6 +67,24 @@ LAUNCHES_PER_STEP = {"k_leg_inv": 5, "k_dft_inv": 5, "k_leg_fwd": 5, "k_dft_fwd"
                      "k_shortwave": 1, "k_longwave": 1, "k_convection": 1}
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
+                               "ensemble64_fp32_hbm_traffic.md")
+
+
+def measured_traffic(kernel, path=TRAFFIC_PROFILE):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the ensemble (scaled by the copy
+    calibration it states), or None when it has no line for the kernel."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def figure(exo, kernel, cfgs, total_ms, launches, missed, steps):
     """The bench line's roofline object for `kernel` over `steps` steps of
     `cfgs`, from the probed launch durations."""
@@ -85,5 +105,7 @@ def figure(exo, kernel, cfgs, total_ms, launches, missed, steps):
         achieved = per_launch / avg_s / 1e9
         peak, unit, bound = HBM_PEAK_GBS, "GB/s", "hbm"
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
-            "traffic": None, "kernel": kernel, "avg_launch_us": avg_s * 1e6, "launches": launches,
+            "traffic": measured_traffic(kernel),
+            "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE) + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "copy-calibrated, averaged over a 5-step run)", "kernel": kernel, "avg_launch_us": avg_s * 1e6, "launches": launches,
             "algorithmic_per_launch": per_launch, "algorithmic_unit": units}
