@@ -37,7 +37,9 @@ extern "C" __global__ void k_shortwave(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_longwave(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_convection(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_dca_limit(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_physics_b(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_rain(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_clouds(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_soil(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_mkradv(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_physics_c(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_ice_sums(const DevMember*, LaunchPlan);

@@ -524,7 +524,7 @@ const std::pair<const char*, const void*> kProbeKernels[] = {
     EXO_PROBE(k_calcgp) EXO_PROBE(k_spectrala) EXO_PROBE(k_spectrald_diffusion) EXO_PROBE(k_spectrald_update)
     EXO_PROBE(k_mkdheat) EXO_PROBE(k_mean_mode) EXO_PROBE(k_physics_in) EXO_PROBE(k_fixer_column)
     EXO_PROBE(k_fixer_reduce) EXO_PROBE(k_surface_fluxes) EXO_PROBE(k_shortwave) EXO_PROBE(k_longwave)
-    EXO_PROBE(k_convection) EXO_PROBE(k_dca_limit) EXO_PROBE(k_physics_b)
+    EXO_PROBE(k_convection) EXO_PROBE(k_dca_limit) EXO_PROBE(k_rain) EXO_PROBE(k_clouds) EXO_PROBE(k_soil)
     EXO_PROBE(k_mkradv) EXO_PROBE(k_physics_c) EXO_PROBE(k_ice_sums) EXO_PROBE(k_physics_d) EXO_PROBE(k_pack)
 #undef EXO_PROBE
 };

@@ -15,8 +15,9 @@
 //   k_longwave       lwr, radstep
 //   k_convection     kuo + mkshallow, mkdca's column pass
 //   k_dca_limit      mkdca's global give-up flag (mkdca-limit.wgsl)
-//   k_physics_b      rain column (mkdca apply, mklsp, mkrain, mkclouds),
-//                    soil and snow (land/wgsl/soil.wgsl)
+//   k_rain           rain column: mkdca apply, mklsp, mkrain
+//   k_clouds         mkclouds
+//   k_soil           soil and snow (land/wgsl/soil.wgsl)
 //   k_mkradv         river routing stencil (land/wgsl/mkradv.wgsl)
 //   k_physics_c      land surface and albedo, sea ice up to getiflx
 //   k_ice_sums       getiflx's two global sums
@@ -1715,8 +1716,8 @@ __device__ __forceinline__ void convection(const DevMember& M, int h) {
 }
 
 template <int NL>
-__device__ __forceinline__ void physics_b(const DevMember& M, int h) {
-  // physics/precipitation/wgsl/column.wgsl:12-27, then soilstep
+__device__ __forceinline__ void rain(const DevMember& M, int h) {
+  // physics/precipitation/wgsl/column.wgsl:12-27 up to mkrain
   const int NH = M.nhor;
   if (F(dca_skip)[0] == 0.0f) {
 #pragma unroll
@@ -1727,7 +1728,15 @@ __device__ __forceinline__ void physics_b(const DevMember& M, int h) {
   }
   mklsp<NL>(M, h);
   mkrain<NL>(M, h);
+}
+
+template <int NL>
+__device__ __forceinline__ void clouds(const DevMember& M, int h) {
   mkclouds<NL>(M, h);
+}
+
+template <int NL>
+__device__ __forceinline__ void soil_step(const DevMember& M, int h) {
   soil<NL>(M, h);
 }
 
@@ -1798,7 +1807,9 @@ EXO_COLUMN_KERNEL_W(k_surface_fluxes, surface_fluxes, 2)
 EXO_COLUMN_KERNEL(k_shortwave, shortwave)
 EXO_COLUMN_KERNEL(k_longwave, longwave)
 EXO_COLUMN_KERNEL(k_convection, convection)
-EXO_COLUMN_KERNEL(k_physics_b, physics_b)
+EXO_COLUMN_KERNEL(k_rain, rain)
+EXO_COLUMN_KERNEL(k_clouds, clouds)
+EXO_COLUMN_KERNEL(k_soil, soil_step)
 EXO_COLUMN_KERNEL(k_physics_c, physics_c)
 EXO_COLUMN_KERNEL(k_physics_d, physics_d)
 
@@ -1883,7 +1894,9 @@ void enqueue_physics(Engine& e, const std::vector<int>& active) {
   s.column(k_longwave);
   s.column(k_convection);
   launch(e, k_dca_limit, dim3(one.n), dim3(256), e.d_desc, one);
-  s.column(k_physics_b);
+  s.column(k_rain);
+  s.column(k_clouds);
+  s.column(k_soil);
   s.column(k_mkradv);
   s.column(k_physics_c);
   launch(e, k_ice_sums, dim3(one.n), dim3(256), e.d_desc, one);
