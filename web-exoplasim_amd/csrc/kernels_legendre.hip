@@ -54,8 +54,10 @@ __host__ __device__ constexpr int a_stride(int r) { return (r + 29) / 32 * 32 + 
 // Staged inputs: inverse rawS[(f * kMaxLev + p) * kRawK + 2k + part]
 // (f = source field, k < K padded with zeros to a multiple of 4); forward
 // raw[(f * kMaxLev + p) * kRawLat + 2 lat + part].
-constexpr int kRawK = 2 * kMaxK;
-constexpr int kRawLat = 2 * kMaxLat;
+// Row strides are 4 mod 32: the 16 product columns one MFMA step reads
+// (8 rows x 2 parts) and its 2 row offsets then fall in 32 distinct banks.
+constexpr int kRawK = 2 * kMaxK + 4;
+constexpr int kRawLat = 2 * kMaxLat + 4;
 constexpr int kInvAs = kMaxLat * a_stride(kMaxK);
 constexpr int kInvRaw = 2 * kMaxLev * kRawK;
 constexpr int kFwdAs = kMaxK * a_stride(kMaxLat / 2);
