@@ -38,6 +38,8 @@ const char* const* oracle_constant_names(void);
 void oracle_step(const oracle_member* m, int with_physics);
 /* gridpointd's parameterizations alone (fixer .. carbon). */
 void oracle_physics(const oracle_member* m);
+/* One parameterization by name (see exo_oracle_physics.c); -1 if unknown. */
+int oracle_routine(const oracle_member* m, const char* name);
 /* Set-up transforms over `planes` planes (qi or qj when dmu). */
 void oracle_transform_inverse(const oracle_member* m, const real* sp, real* grid, int planes, int dmu);
 void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp, int planes);

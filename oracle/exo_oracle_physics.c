@@ -6,13 +6,22 @@
  * surfstep, carbonstep. The reference's f32 "precise_*" helpers
  * (shared/gpu/math/wgsl/precise.wgsl) are libm's log/exp/pow here.
  *
- * Parity unpinned: no independent check (closed form, library routine or
- * reference golden vector) pins these routines; they are held only to the
- * GPU kernels written from the same reading of the reference WGSL.
+ * Pinned by physics on crafted columns (tests/test_oracle_physics_pins.py):
+ * lwr, swr, solang, radstep, mkdca, mklsp, mkrain, vdiff and the fixer, each
+ * against laws their results must obey (Kirchhoff, flux and water
+ * conservation, backward-Euler diffusion, textbook latent heats, Tetens
+ * saturation, random cloud overlap).
+ *
+ * Parity unpinned: surflx, kuo/mkshallow, mkclouds, soil, rivers, land,
+ * sea ice, ocean, carbon and mko3. No closed form, library routine or
+ * reference golden vector checks them; they are held only to the GPU
+ * kernels written from the same reading of the reference WGSL, and by the
+ * physical range of day-scale runs.
  */
 #define EXO_ORACLE_INTERNAL
 #include <tgmath.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "exo_oracle.h"
 
@@ -1673,6 +1682,49 @@ static void carbonstep(const M* v) {
  * (stepper/index.ts:148-161; fixer/index.ts:39-73; surface-flux/index.ts:
  * 72-113; radiation/index.ts:151-234; precipitation/index.ts:71-160;
  * surface/index.ts:62-144; carbon/index.ts:32-45). */
+/* One parameterization by name, for the invariant checks of the oracle's
+ * routines; -1 for an unknown name. "fixer" runs the column, row, global
+ * and apply passes; "rain" runs mklsp, mkrain and mkclouds per column. */
+int oracle_routine(const oracle_member* m, const char* name) {
+  M vv = view(m);
+  const M* v = &vv;
+#define IS(s) (strcmp(name, s) == 0)
+  if (IS("fixer")) {
+    fixer_column(v);
+    fixer_row(v);
+    global_sum(v, F(fixneg), F(fixpos), F(fixsum));
+    fixer_apply(v);
+  } else if (IS("surflx")) {
+    surflx(v);
+  } else if (IS("vdiff")) {
+    vdiff(v);
+  } else if (IS("solang")) {
+    solang(v);
+  } else if (IS("swr")) {
+    swr(v);
+  } else if (IS("lwr")) {
+    lwr(v);
+  } else if (IS("radstep")) {
+    radstep(v);
+  } else if (IS("kuo")) {
+    kuo(v);
+  } else if (IS("mkdca")) {
+    mkdca(v);
+  } else if (IS("mklsp")) {
+    for (int h = 0; h < v->NHOR; h++) mklsp(v, h);
+  } else if (IS("mkrain")) {
+    for (int h = 0; h < v->NHOR; h++) mkrain(v, h);
+  } else if (IS("soil")) {
+    soil(v);
+  } else if (IS("seastep")) {
+    seastep(v);
+  } else {
+    return -1;
+  }
+#undef IS
+  return 0;
+}
+
 void oracle_physics(const oracle_member* m) {
   M vv = view(m);
   const M* v = &vv;

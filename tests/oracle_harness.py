@@ -45,6 +45,8 @@ def oracle_library(precision="fp64"):
         getattr(lib, n).restype = ctypes.POINTER(ctypes.c_char_p)
     lib.oracle_step.argtypes = [ctypes.POINTER(Mem), ctypes.c_int]
     lib.oracle_physics.argtypes = [ctypes.POINTER(Mem)]
+    lib.oracle_routine.argtypes = [ctypes.POINTER(Mem), ctypes.c_char_p]
+    lib.oracle_routine.restype = ctypes.c_int
     P = ctypes.POINTER(ct)
     lib.oracle_transform_inverse.argtypes = [ctypes.POINTER(Mem), P, P, ctypes.c_int, ctypes.c_int]
     lib.oracle_transform_forward.argtypes = [ctypes.POINTER(Mem), P, P, ctypes.c_int]
@@ -149,6 +151,21 @@ class OracleRun:
             m = self._struct(consts)
             self.lib.oracle_step(ctypes.byref(m), 1 if physics else 0)
             self.phase += 1
+
+    def routine(self, name, phase=None):
+        """Runs one parameterization of the oracle on the current arena
+        with the constants of `phase` (default: the main loop's)."""
+        consts = self.consts[-1 if phase is None else phase]
+        m = self._struct(consts)
+        if self.lib.oracle_routine(ctypes.byref(m), name.encode()) != 0:
+            raise KeyError(name)
+
+    def constant(self, name, phase=None):
+        return float(self.consts[-1 if phase is None else phase][self.const_names.index(name)])
+
+    def table(self, name):
+        at, n = self.table_at[name]
+        return self.tables[at:at + n]
 
     def inverse(self, sp, planes, dmu=False):
         consts = self.consts[-1]

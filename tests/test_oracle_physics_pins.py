@@ -1,0 +1,564 @@
+"""Pins of the oracle's parameterizations by what the physics fixes, on
+crafted columns (CPU). Each check is chosen so that a plausible slip in a
+routine (a dropped term, a wrong sign or index, a wrong weight, a swapped
+coefficient) fails it; planting such slips one at a time in a copy of the
+oracle fails at least one test each.
+
+* lwr: over an isothermal column on a black surface the upward flux is
+  sigma T^4 at every level whatever the absorbers (Kirchhoff); a dry,
+  clear, CO2- and ozone-free column is transparent; opaque clouds of
+  covers c1, c2 overlap at random (downward flux (1 - (1-c1)(1-c2))
+  sigma T^4 below them) and an ocean of emissivity 0.98 reflects the rest;
+  the Stefan-Boltzmann constant is the physical one.
+* swr: no sunlight below the horizon; the downward flux at the top is the
+  insolation S0 d^-2 mu0; a column without absorbers passes the same net
+  flux through every level; absorbers and cloud layers only take flux
+  away; clouds raise the planetary albedo.
+* solang: mu0 is the spherical-trigonometry zenith cosine on the
+  Gauss-Legendre latitudes.
+* radstep: the column's heating is the net flux convergence.
+* mkdca: dry adjustment conserves column enthalpy and moisture, leaves a
+  statically stable column, and is idempotent.
+* mklsp: the member's gas constants follow the ideal-gas relations;
+  condensation releases the textbook latent heat into moist air and
+  brings the level to saturation by the Tetens (Murray) formula.
+* mkrain: falling water is conserved between evaporation and the ground;
+  evaporation cools by L_v, melting and freezing exchange L_f.
+* vdiff: implicit diffusion with closed ends conserves u, v and q, is a
+  backward-Euler step, mixes moisture with the heat mixing length in
+  neutral air and damps heat more than momentum in stable air (Louis).
+* fixer: global moisture is conserved and no humidity stays negative.
+
+The remaining parameterizations (surflx, kuo/mkshallow, mkclouds, soil,
+rivers, land, sea ice, ocean, carbon, mko3) have no such pin here.
+Each test states the tolerance it needs: FP64 arithmetic, so conservation
+holds to rounding.
+"""
+import numpy as np
+import pytest
+
+from oracle_harness import OracleRun
+
+STEFAN_BOLTZMANN = 5.670374419e-8  # W m^-2 K^-4, CODATA 2018
+
+
+@pytest.fixture(scope="module")
+def column(exo):
+    host = exo.Engine(-1, 1)
+    m = host.add_member(exo.default_config(truncation=21, levels=10))
+    _, nlat, nlon, nlev = host.dims(m)
+    yield host, m, nlat, nlon, nlev
+    host.close()
+
+
+def _run(column):
+    host, m, nlat, nlon, nlev = column
+    return OracleRun(host, m), nlat * nlon, nlev
+
+
+def _planes(run, name, nh):
+    return run.field(name).reshape(-1, nh)
+
+
+def test_lwr_isothermal_upward_flux_is_sigma_t4(column):
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(1)
+    T0 = 271.3
+    _planes(run, "dt", nh)[:] = T0
+    run.field("dls")[:] = 1.0  # land: emissivity 1
+    run.field("dp")[:] = rng.uniform(6e4, 1.05e5, nh)
+    _planes(run, "dq", nh)[:] = rng.uniform(0, 0.02, (L + 1, nh))
+    _planes(run, "dqo3", nh)[:] = rng.uniform(0, 1e-5, (L, nh))
+    _planes(run, "dqco2", nh)[:] = rng.uniform(0, 1e-3, (L, nh))
+    _planes(run, "dcc", nh)[:] = rng.uniform(0, 1, (L + 1, nh))
+    _planes(run, "dql", nh)[:] = rng.uniform(0, 1e-4, (L + 1, nh))
+    run.routine("lwr")
+    sbk = run.constant("sbk")
+    assert abs(sbk / STEFAN_BOLTZMANN - 1) < 1e-3
+    up = _planes(run, "dftu", nh)
+    np.testing.assert_allclose(up, -sbk * T0 ** 4, rtol=1e-10)
+    # downward flux is the emission of the layers above: from 0 at the top
+    # up to at most sigma T^4
+    down = _planes(run, "dftd", nh)
+    assert np.all(np.abs(down[0]) < 1e-9)
+    assert np.all(down >= -1e-9) and np.all(down <= sbk * T0 ** 4 * (1 + 1e-9))
+
+
+def test_lwr_clear_dry_column_is_transparent(column):
+    run, nh, L = _run(column)
+    T0 = 250.0
+    _planes(run, "dt", nh)[:] = T0
+    run.field("dls")[:] = 0.0  # ocean: emissivity 0.98
+    for name in ("dq", "dcc", "dql"):
+        run.field(name)[:] = 0.0
+    for name in ("dqo3", "dqco2"):
+        run.field(name)[:] = 0.0
+    run.routine("lwr")
+    sbk = run.constant("sbk")
+    net = _planes(run, "dlwfl", nh)
+    np.testing.assert_allclose(net, -0.98 * sbk * T0 ** 4, rtol=2e-5)
+
+
+def test_lwr_black_clouds_random_overlap_over_ocean(column):
+    """Opaque cloud layers of cover c1, c2 in a transparent isothermal
+    column over an ocean of emissivity 0.98: below them the downward flux
+    is (1 - (1 - c1)(1 - c2)) sigma T^4 (random overlap), above them zero;
+    at the top the upward flux is the cloud emission plus the surface's
+    emission and reflection seen through the gaps."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(7)
+    T0 = 280.0
+    _planes(run, "dt", nh)[:] = T0
+    run.field("dls")[:] = 0.0
+    run.field("dp")[:] = 1e5
+    for name in ("dq", "dqo3", "dqco2", "dcc"):
+        run.field(name)[:] = 0.0
+    _planes(run, "dql", nh)[:] = 1e-3  # optical depth > 50 in every layer
+    cols = np.arange(nh)
+    k1 = rng.integers(1, L // 2, nh)
+    k2 = rng.integers(L // 2, L - 1, nh)
+    c1 = rng.uniform(0, 1, nh)
+    c2 = rng.uniform(0, 1, nh)
+    cc = _planes(run, "dcc", nh)
+    cc[k1, cols] = c1
+    cc[k2, cols] = c2
+    run.routine("lwr")
+    sbk = run.constant("sbk")
+    B = sbk * T0 ** 4
+    down = _planes(run, "dftd", nh)
+    up = -_planes(run, "dftu", nh)
+    level = np.arange(L + 1)[:, None]
+    cover = np.where(level > k1, c1, 0.0)
+    cover = 1 - (1 - cover) * (1 - np.where(level > k2, c2, 0.0))
+    # clear layers are not exactly transparent (band transmissivities are
+    # capped at 1 - 1e-6), hence 1e-5
+    np.testing.assert_allclose(down, cover * B, rtol=1e-5, atol=1e-5 * B)
+    eps = 0.98
+    gap = (1 - c1) * (1 - c2)
+    total = 1 - gap
+    expect_top = (total + gap * (eps + (1 - eps) * total)) * B
+    np.testing.assert_allclose(up[0], expect_top, rtol=1e-5)
+
+
+def _swr_column(run, nh, L, rng, mu0):
+    rs = run.field("radiation_step")
+    rs[3] = 1.03  # gdist2
+    run.field("gmu0")[:] = mu0
+    run.field("dp")[:] = 1e5
+    _planes(run, "dt", nh)[:] = 288.0 * np.append(run.table("sigma"), 1.0)[:, None] ** 0.19
+    # land without ice: the direct and diffuse surface albedos agree
+    run.field("dls")[:] = 1.0
+    run.field("dicec")[:] = 0.0
+    alb = rng.uniform(0.05, 0.6, nh)
+    run.field("dalb")[:] = alb
+    run.field("dsalb")[:] = np.concatenate([alb, alb])
+    return run.constant("gsol0") * rs[3] * mu0
+
+
+def _swr_net(run, nh):
+    run.routine("swr")
+    return _planes(run, "dswfl", nh).copy(), _planes(run, "dfd", nh).copy()
+
+
+def test_swr_conserves_in_a_transparent_column(column):
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(2)
+    assert run.constant("nstartemp") == 0
+    mu0 = rng.uniform(0.05, 1.0, nh)
+    mu0[::5] = 0.0  # night columns
+    top = _swr_column(run, nh, L, rng, mu0)
+    for name in ("dq", "dqo3", "dcc", "dql"):
+        run.field(name)[:] = 0.0
+    net, down = _swr_net(run, nh)
+    night = mu0 == 0
+    day = ~night
+    assert np.all(net[:, night] == 0) and np.all(down[:, night] == 0)
+    np.testing.assert_allclose(down[0, day], top[day], rtol=1e-12)
+    # no absorber anywhere: what enters at the top reaches the surface
+    np.testing.assert_allclose(net[:, day], np.broadcast_to(net[0, day], net[:, day].shape), rtol=1e-12)
+    alb = run.field("dalb")[day]
+    assert np.all(net[0, day] < top[day] * (1 - alb) * (1 + 1e-12))  # Rayleigh adds reflection
+    assert np.all(net[0, day] > 0)
+
+
+def test_swr_clear_absorbing_column_decreases_downwards(column):
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(3)
+    mu0 = rng.uniform(0.05, 1.0, nh)
+    top = _swr_column(run, nh, L, rng, mu0)
+    run.field("dcc")[:] = 0.0
+    run.field("dql")[:] = 0.0
+    _planes(run, "dq", nh)[:] = rng.uniform(0, 0.015, (L + 1, nh))
+    _planes(run, "dqo3", nh)[:] = rng.uniform(0, 1e-5, (L, nh))
+    net, down = _swr_net(run, nh)
+    np.testing.assert_allclose(down[0], top, rtol=1e-12)
+    assert np.all(np.diff(net, axis=0) <= 1e-12 * top)  # every layer absorbs
+    assert np.all(net[-1] > 0) and np.all(net[0] <= top)
+    # more vapour absorbs more
+    _planes(run, "dq", nh)[:] *= 2.0
+    net2, _ = _swr_net(run, nh)
+    assert np.all(net2[-1] < net[-1])
+
+
+def test_swr_cloudy_column_stays_within_insolation(column):
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(4)
+    mu0 = rng.uniform(0.05, 1.0, nh)
+    top = _swr_column(run, nh, L, rng, mu0)
+    _planes(run, "dq", nh)[:] = rng.uniform(0, 0.015, (L + 1, nh))
+    _planes(run, "dcc", nh)[:] = rng.uniform(0, 1, (L + 1, nh)) * (rng.uniform(0, 1, (L + 1, nh)) > 0.5)
+    _planes(run, "dql", nh)[:] = rng.uniform(0, 2e-4, (L + 1, nh))
+    net, down = _swr_net(run, nh)
+    np.testing.assert_allclose(down[0], top, rtol=1e-12)
+    assert np.all(net >= 0) and np.all(net <= top * (1 + 1e-12))
+    # the cloud-top reflectance raises the planetary albedo: less enters
+    # the column than without clouds. The cloudy fraction of a layer trades
+    # vapour absorption for cloud absorption, so a thin cloud may let in up
+    # to 1e-4 of the insolation more (seen: 6e-6).
+    run.field("dcc")[:] = 0.0
+    clear, _ = _swr_net(run, nh)
+    assert np.all(net[0] <= clear[0] + 1e-4 * top)
+    assert np.mean(net[0] < clear[0] - 1e-2 * top) > 0.5
+
+
+def test_swr_cloud_layer_absorbs(column):
+    """One cloud layer per column in a dry, ozone-free column: every layer
+    of the free atmosphere (above the lowest, whose Rayleigh layer has its
+    own direct and diffuse reflectances) passes on no more net flux than it
+    receives."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(8)
+    mu0 = rng.uniform(0.05, 1.0, nh)
+    top = _swr_column(run, nh, L, rng, mu0)
+    for name in ("dq", "dqo3", "dcc"):
+        run.field(name)[:] = 0.0
+    _planes(run, "dcc", nh)[rng.integers(0, L - 1, nh), np.arange(nh)] = rng.uniform(0.2, 1, nh)
+    _planes(run, "dql", nh)[:] = rng.uniform(1e-6, 5e-4, (L + 1, nh))
+    net, _ = _swr_net(run, nh)
+    assert np.all(np.diff(net[:L], axis=0) <= 1e-12 * top)
+    assert np.all(net[:L] >= 0)
+
+
+def test_solang_is_the_zenith_cosine(column):
+    run, nh, L = _run(column)
+    host, m, nlat, nlon, _ = column
+    delta = np.deg2rad(17.0)
+    rs = run.field("radiation_step")
+    rs[0], rs[1], rs[2] = np.sin(delta), np.cos(delta), 0.7
+    run.routine("solang")
+    phi = np.arcsin(np.polynomial.legendre.leggauss(nlat)[0][::-1])  # north to south
+    lam = 2 * np.pi * np.arange(nlon) / nlon
+    hour = rs[2] + lam - np.pi
+    mu = (np.sin(phi)[:, None] * np.sin(delta) + np.cos(phi)[:, None] * np.cos(delta) * np.cos(hour)[None, :])
+    got = run.field("gmu0").reshape(nlat, nlon)
+    dawn = run.constant("zdawn")
+    expect = np.where(mu > dawn, mu, 0.0)
+    np.testing.assert_allclose(got, expect, atol=1e-11)
+
+
+def _unstable_column(run, nh, L, rng):
+    sigma = run.table("sigma")
+    # a lapse rate of 12-14 K per scale height fraction, beyond the dry
+    # adiabat near the ground, with random kinks
+    t = 300.0 * sigma[:, None] ** 0.33 + rng.normal(0, 3.0, (L, nh))
+    t[-3:] += rng.uniform(2, 8, (3, nh))
+    return t
+
+
+def test_mkdca_conserves_and_stabilises(column):
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(3)
+    dt = _planes(run, "dt", nh)
+    dt[:L] = _unstable_column(run, nh, L, rng)
+    _planes(run, "dtdt", nh)[:] = 0.0
+    _planes(run, "dqdt", nh)[:] = 0.0
+    dq = _planes(run, "dq", nh)
+    dq[:L] = rng.uniform(1e-4, 1e-2, (L, nh))
+    run.routine("mkdca")
+    ds = run.table("dsigma")[:, None]
+    skap = run.table("vdiff_skap")[:, None]
+    dT = _planes(run, "dca_dtdt", nh)
+    dQ = _planes(run, "dca_dqdt", nh)
+    scale_t = np.sum(ds * np.abs(dT), axis=0) + 1e-30
+    scale_q = np.sum(ds * np.abs(dQ), axis=0) + 1e-30
+    assert np.all(np.abs(np.sum(ds * dT, axis=0)) <= 1e-12 * scale_t)
+    assert np.all(np.abs(np.sum(ds * dQ, axis=0)) <= 1e-12 * scale_q)
+    assert np.mean(np.any(dT != 0, axis=0)) > 0.5  # the columns were unstable
+    limited = run.field("dca_limited") != 0
+    dts = run.constant("deltsec2")
+    t_new = dt[:L] + dT * dts
+    theta = t_new / skap
+    stable = np.all(theta[:-1] >= theta[1:] - 1e-9, axis=0)
+    assert np.all(stable | limited)
+    # idempotent on the adjusted column
+    dt[:L] = t_new
+    dq[:L] = dq[:L] + dQ * dts
+    run.routine("mkdca")
+    again = _planes(run, "dca_dtdt", nh)
+    assert np.max(np.abs(again[:, ~limited])) < 1e-9
+
+
+def test_mklsp_latent_heat_and_saturation(column):
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(4)
+    sigma = run.table("sigma")
+    ps = rng.uniform(8e4, 1.03e5, nh)
+    run.field("dp")[:] = ps
+    t = 295.0 * sigma[:, None] ** 0.19 + rng.normal(0, 1, (L, nh))
+    p = sigma[:, None] * ps[None, :]
+    # the member's gas: dry-air constants from its gas constant by the
+    # ideal-gas relations, vapour and latent heat from the textbook
+    R, kappa = run.constant("gascon"), run.constant("akap")
+    assert abs(R - 8314.46 / 28.0134) < 1e-3  # N2 (the default atmosphere)
+    cp = run.constant("acpd")
+    assert abs(cp - R / kappa) < 1e-9 * cp
+    eps = run.constant("rdbrv")
+    assert abs(eps / (R / 461.5) - 1) < 2e-3  # R_v = 461.5 J/(kg K)
+    cpv = cp * (1 + run.constant("adv"))
+    assert abs(cpv / 1860.0 - 1) < 1e-2  # vapour cp
+    Lv = 2.501e6
+    assert abs(run.constant("alv") / Lv - 1) < 1e-3
+    es = 610.78 * np.exp(17.27 * (t - 273.16) / (t - 35.86))  # Tetens (Murray 1967)
+    qs = eps * es / (p - (1 - eps) * es)
+    ratio = rng.uniform(0.6, 1.5, (L, nh))
+    q = qs * ratio
+    _planes(run, "dt", nh)[:L] = t
+    _planes(run, "dq", nh)[:L] = q
+    _planes(run, "dtdt", nh)[:] = 0.0
+    _planes(run, "dqdt", nh)[:] = 0.0
+    run.routine("mklsp")
+    dts = run.constant("deltsec2")
+    dT = _planes(run, "dtdt", nh)[:L] * dts
+    dQ = _planes(run, "dqdt", nh)[:L] * dts
+    sub = ratio < 0.99
+    assert np.all(dT[sub] == 0) and np.all(dQ[sub] == 0)
+    sup = ratio > 1.01
+    assert np.all(dQ[sup] < 0)
+    # latent heat: dT = -L / (cp_moist) dq with cp_moist = cp (1 - q) + cpv q
+    # to first order in q
+    lcp = -dT[sup] / dQ[sup]
+    expect = Lv / (cp + (cpv - cp) * q[sup])
+    np.testing.assert_allclose(lcp, expect, rtol=2e-3)
+    # saturated at the new temperature (two Newton steps)
+    tn, qn = t[sup] + dT[sup], q[sup] + dQ[sup]
+    esn = 610.78 * np.exp(17.27 * (tn - 273.16) / (tn - 35.86))
+    qsn = eps * esn / (p[sup] - (1 - eps) * esn)
+    np.testing.assert_allclose(qn, qsn, rtol=2e-3)
+    # the condensed water falls: dprll = -dq/dt dsigma
+    prl = _planes(run, "dprll", nh)
+    np.testing.assert_allclose(prl[sup], -(dQ[sup] / dts) * np.broadcast_to(run.table("dsigma")[:, None],
+                                                                               (L, nh))[sup], rtol=1e-12)
+
+
+def test_vdiff_conserves_column_momentum_and_moisture(column):
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(5)
+    sigma = run.table("sigma")
+    _planes(run, "dt", nh)[:L] = 290.0 * sigma[:, None] ** 0.2 + rng.normal(0, 2, (L, nh))
+    _planes(run, "du", nh)[:L] = rng.normal(5, 10, (L, nh))
+    _planes(run, "dv", nh)[:L] = rng.normal(0, 8, (L, nh))
+    _planes(run, "dq", nh)[:L] = rng.uniform(1e-5, 1e-2, (L, nh))
+    for name in ("dudt", "dvdt", "dqdt", "dtdt"):
+        run.field(name)[:] = 0.0
+    run.routine("vdiff")
+    ds = run.table("dsigma")[:, None]
+    moved = 0
+    for name in ("dudt", "dvdt", "dqdt"):
+        x = _planes(run, name, nh)[:L]
+        scale = np.sum(ds * np.abs(x), axis=0) + 1e-30
+        assert np.all(np.abs(np.sum(ds * x, axis=0)) <= 1e-10 * scale), name
+        moved += np.count_nonzero(x)
+    assert moved > 0
+
+
+def _vdiff_coefficients(x, x_new, ds):
+    """Exchange coefficients K_l (interface l between layers l and l+1) of
+    a backward-Euler diffusion step ds_l (x'_l - x_l) = K_l (x'_{l+1} - x'_l)
+    - K_{l-1} (x'_l - x'_{l-1}) with closed ends, recovered from its input
+    and output: the flux through interface l is the cumulative change above
+    it. Also returns the flux through the bottom (zero for a closed end)."""
+    flux = np.cumsum(ds * (x_new - x), axis=0)
+    return flux[:-1] / (x_new[1:] - x_new[:-1]), flux[-1]
+
+
+def _vdiff_run(run, nh, L, rng, theta_v):
+    skap = run.table("vdiff_skap")[:, None]
+    f = np.sort(rng.uniform(-1, 1, (L, nh)), axis=0)  # strictly monotone: shear everywhere
+    u = 8.0 + 6.0 * f
+    q = 2e-3 * (1 + 0.5 * f)
+    moist = 1 / run.constant("rdbrv") - 1
+    _planes(run, "dt", nh)[:L] = theta_v * skap / (1 + moist * q)
+    _planes(run, "du", nh)[:L] = u
+    _planes(run, "dv", nh)[:L] = 0.0
+    _planes(run, "dq", nh)[:L] = q
+    for name in ("dudt", "dvdt", "dqdt", "dtdt"):
+        run.field(name)[:] = 0.0
+    run.routine("vdiff")
+    dts = run.constant("deltsec2")
+    ds = run.table("dsigma")[:, None]
+    km, bottom_u = _vdiff_coefficients(u, u + _planes(run, "dudt", nh)[:L] * dts, ds)
+    kh, bottom_q = _vdiff_coefficients(q, q + _planes(run, "dqdt", nh)[:L] * dts, ds)
+    assert np.all(np.abs(bottom_u) < 1e-9 * np.sum(ds * np.abs(u), axis=0))
+    assert np.all(np.abs(bottom_q) < 1e-9 * np.sum(ds * np.abs(q), axis=0))
+    assert np.all(km > 0) and np.all(kh > 0)
+    ratio = run.table("vdiff_mixh2")[:L - 1, None] / run.table("vdiff_mixm2")[:L - 1, None]
+    return kh / km, ratio
+
+
+def test_vdiff_neutral_column_mixes_with_the_mixing_lengths(column):
+    """Uniform virtual potential temperature (Richardson number 0): both
+    stability functions are 1, so moisture's exchange coefficient is
+    momentum's times the ratio of the squared heat and momentum mixing
+    lengths, interface by interface."""
+    run, nh, L = _run(column)
+    got, ratio = _vdiff_run(run, nh, L, np.random.default_rng(9), 290.0)
+    np.testing.assert_allclose(got, np.broadcast_to(ratio, got.shape), rtol=1e-8)
+
+
+def test_vdiff_stable_air_damps_heat_more_than_momentum(column):
+    """Louis (1979): in stable air (Richardson number > 0) the heat
+    stability function falls faster than the momentum one, so moisture's
+    coefficient drops below its neutral share of momentum's. (In unstable
+    air the two enhancements depend on the mixing lengths, which differ
+    here, so no order is fixed there.)"""
+    run, nh, L = _run(column)
+    height = np.arange(L - 1, -1, -1)[:, None]  # layer 0 is the top
+    got, ratio = _vdiff_run(run, nh, L, np.random.default_rng(10), 290.0 + 2.0 * height)
+    assert np.all(got < ratio * (1 - 1e-6))
+
+
+def test_fixer_conserves_global_moisture(column):
+    run, nh, L = _run(column)
+    host, m, nlat, nlon, _ = column
+    rng = np.random.default_rng(6)
+    q = rng.uniform(0, 5e-3, (L, nh))
+    neg = rng.uniform(0, 1, (L, nh)) < 0.08
+    q[neg] = -rng.uniform(0, 2e-4, neg.sum())
+    _planes(run, "dq", nh)[:L] = q
+    run.field("gqdt")[:] = 0.0
+    ps = rng.uniform(7e4, 1.03e5, nh)
+    run.field("dp")[:] = ps
+    ds = run.table("dsigma")[:, None]
+    w = run.table("zgw")
+    before = np.sum(w * ps * np.sum(ds * q, axis=0))
+    run.routine("fixer")
+    after_q = _planes(run, "dq", nh)[:L]
+    after = np.sum(w * ps * np.sum(ds * after_q, axis=0))
+    assert np.all(after_q >= 0)
+    assert abs(after / before - 1) < 1e-12
+
+
+def test_radstep_heating_is_the_flux_convergence(column):
+    """Radiative heating of the column's moist air (cp of dry air times
+    1 + (cpv/cp - 1) q) equals the net flux entering at the top minus the
+    net flux leaving at the surface, and the short- and longwave parts
+    add up to it."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(12)
+    ps = rng.uniform(6e4, 1.05e5, nh)
+    run.field("dp")[:] = ps
+    _planes(run, "dt", nh)[:] = rng.uniform(200, 300, (L + 1, nh))
+    q = rng.uniform(0, 0.02, (L + 1, nh))
+    _planes(run, "dq", nh)[:] = q
+    _planes(run, "dswfl", nh)[:] = rng.uniform(0, 800, (L + 1, nh))
+    _planes(run, "dlwfl", nh)[:] = rng.uniform(-400, 0, (L + 1, nh))
+    run.field("dtdt")[:] = 0.0
+    run.routine("radstep")
+    flux = _planes(run, "dswfl", nh) + _planes(run, "dlwfl", nh)
+    dtdt = _planes(run, "dtdt", nh)[:L]
+    ga, cp = run.constant("ga"), run.constant("acpd")
+    cpv = cp * (1 + run.constant("adv"))
+    mass = run.table("dsigma")[:, None] * ps / ga  # kg/m^2 per layer
+    heat = np.sum(mass * (cp * (1 - q[:L]) + cpv * q[:L]) * dtdt, axis=0)
+    # cp (1 + adv q) is the first-order form of cp (1 - q) + cpv q
+    scale = np.abs(flux).max(axis=0)
+    assert np.all(np.abs(heat - (flux[0] - flux[L])) <= 0.03 * scale)
+    exact = np.sum(mass * cp * (1 + run.constant("adv") * q[:L]) * dtdt, axis=0)
+    assert np.all(np.abs(exact - (flux[0] - flux[L])) <= 1e-12 * scale)
+    parts = _planes(run, "dtdtswr", nh)[:L] + _planes(run, "dtdtlwr", nh)[:L]
+    np.testing.assert_allclose(parts, dtdt, rtol=1e-12, atol=1e-18)
+
+
+def _saturation(run, t, p):
+    eps = run.constant("rdbrv")
+    es = 610.78 * np.exp(17.27 * (t - 273.16) / (t - 35.86))  # Tetens (Murray 1967)
+    return eps * es / (p - (1 - eps) * es)
+
+
+def _rain_column(run, nh, L, rng, t, humidity):
+    ps = rng.uniform(8e4, 1.03e5, nh)
+    run.field("dp")[:] = ps
+    p = run.table("sigma")[:, None] * ps
+    _planes(run, "dt", nh)[:] = t
+    q = humidity * _saturation(run, t[:L], p)
+    _planes(run, "dq", nh)[:L] = q
+    for name in ("dtdt", "dqdt", "dprcl", "dprll", "dprscl", "dprsll"):
+        run.field(name)[:] = 0.0
+    return ps, q
+
+
+def _rain_budget(run, nh, L, ps):
+    """Column water in (fluxes made per level) against water out (surface
+    precipitation and evaporation), in the flux units sigma s^-1."""
+    w = run.constant("ga") * 1000.0 / ps
+    made = np.sum(_planes(run, "dprcl", nh)[:L] + _planes(run, "dprll", nh)[:L], axis=0)
+    evap = np.sum(run.table("dsigma")[:, None] * _planes(run, "dqdt", nh)[:L], axis=0)
+    surface = (run.field("dprc") + run.field("dprl")) * w
+    return made, evap, surface, run.field("dprs") * w
+
+
+def test_mkrain_evaporation_conserves_water_and_cools(column):
+    """Rain falling through warm subsaturated air: what evaporates plus
+    what reaches the ground is what fell, the air is cooled by L_v per kg
+    evaporated, and no level is pushed past saturation."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(13)
+    t = 280.0 + 20.0 * np.append(run.table("sigma"), 1.0)[:, None]  # above freezing throughout
+    ps, q = _rain_column(run, nh, L, rng, np.broadcast_to(t, (L + 1, nh)).copy(), rng.uniform(0.3, 0.95, (L, nh)))
+    ds = run.table("dsigma")[:, None]
+    _planes(run, "dprll", nh)[:L] = rng.uniform(0, 1e-7, (L, nh)) * ds
+    _planes(run, "dprcl", nh)[:L] = rng.uniform(0, 1e-7, (L, nh)) * ds
+    run.routine("mkrain")
+    made, evap, surface, snow = _rain_budget(run, nh, L, ps)
+    np.testing.assert_allclose(evap + surface, made, rtol=1e-12)
+    assert np.all(evap > 0) and np.all(surface >= 0) and np.all(snow == 0)
+    dq = _planes(run, "dqdt", nh)[:L]
+    dT = _planes(run, "dtdt", nh)[:L]
+    cp, adv = run.constant("acpd"), run.constant("adv")
+    assert abs(run.constant("alv") / 2.501e6 - 1) < 1e-3
+    wet = dq > 0
+    np.testing.assert_allclose(-dT[wet] / dq[wet], (run.constant("alv") / (cp * (1 + adv * q)))[wet], rtol=1e-12)
+    dts = run.constant("deltsec2")
+    p = run.table("sigma")[:, None] * ps
+    after = _saturation(run, t[:L] + dT * dts, p)
+    assert np.all(q + dq * dts <= after * (1 + 2e-3))
+
+
+def test_mkrain_melting_and_freezing_exchange_latent_heat_of_fusion(column):
+    """Saturated air (nothing evaporates): snow made at the top falls into
+    air above freezing and melts there, rain made at the top falls into
+    air below freezing and freezes; either way the column's enthalpy
+    changes by L_f per kg, L_f = L_s - L_v the textbook 3.34e5 J/kg, and
+    all the water reaches the ground in the new phase."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(14)
+    ds = run.table("dsigma")[:, None]
+    cp, adv = run.constant("acpd"), run.constant("adv")
+    lf = run.constant("als") - run.constant("alv")
+    assert abs(lf / 3.34e5 - 1) < 5e-3
+    ga = run.constant("ga")
+    for warm in (True, False):
+        t = np.full((L + 1, nh), 283.0 if warm else 255.0)
+        ps, q = _rain_column(run, nh, L, rng, t, 1.05)
+        falling = rng.uniform(1e-9, 1e-7, nh)
+        _planes(run, "dprll", nh)[0] = falling
+        if warm:
+            _planes(run, "dprsll", nh)[0] = falling  # all of it snow
+        run.routine("mkrain")
+        made, evap, surface, snow = _rain_budget(run, nh, L, ps)
+        assert np.all(evap == 0)
+        np.testing.assert_allclose(surface, made, rtol=1e-12)
+        np.testing.assert_allclose(snow, 0.0 if warm else made, rtol=1e-12, atol=0)
+        dT = _planes(run, "dtdt", nh)[:L]
+        heat = np.sum(ds * cp * (1 + adv * q) * dT, axis=0)
+        np.testing.assert_allclose(heat, (-lf if warm else lf) * falling, rtol=1e-12)
