@@ -7,13 +7,14 @@
  * (shared/gpu/math/wgsl/precise.wgsl) are libm's log/exp/pow here.
  *
  * Pinned by physics on crafted columns (tests/test_oracle_physics_pins.py):
- * lwr, swr, solang, radstep, mkdca, mklsp, mkrain, vdiff and the fixer, each
- * against laws their results must obey (Kirchhoff, flux and water
- * conservation, backward-Euler diffusion, textbook latent heats, Tetens
- * saturation, random cloud overlap).
+ * lwr, swr, solang, mko3, radstep, kuo, mkdca, mklsp, mkrain, vdiff, soil,
+ * the ocean mixed layer and the fixer, each against laws their results
+ * must obey (Kirchhoff, energy and water conservation, backward-Euler
+ * diffusion, textbook latent heats, Tetens saturation, random cloud
+ * overlap, Green's ozone profile, Charnock's roughness).
  *
- * Parity unpinned: surflx, kuo/mkshallow, mkclouds, soil, rivers, land,
- * sea ice, ocean, carbon and mko3. No closed form, library routine or
+ * Parity unpinned: surflx, mkclouds, rivers, the land surface's albedos,
+ * the sea-ice model (sea_ice) and carbon. No closed form, library routine or
  * reference golden vector checks them; they are held only to the GPU
  * kernels written from the same reading of the reference WGSL, and by the
  * physical range of day-scale runs.
@@ -1700,6 +1701,8 @@ int oracle_routine(const oracle_member* m, const char* name) {
     vdiff(v);
   } else if (IS("solang")) {
     solang(v);
+  } else if (IS("mko3")) {
+    mko3(v);
   } else if (IS("swr")) {
     swr(v);
   } else if (IS("lwr")) {

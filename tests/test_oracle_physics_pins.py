@@ -28,9 +28,18 @@ oracle fails at least one test each.
   backward-Euler step, mixes moisture with the heat mixing length in
   neutral air and damps heat more than momentum in stable air (Louis).
 * fixer: global moisture is conserved and no humidity stays negative.
+* kuo: the cloud layers' moisture supply moistens or rains out (column
+  water conserved) and the rain's latent heat heats the column; the
+  shallow branch only redistributes.
+* soil: water (bucket, snow pack, runoff) is conserved, over snow the
+  pack sublimates, and on bare land the surface and soil layers store the
+  net surface heat flux through series conductances.
+* mko3: Green's (1964) ozone profile at hypsometric heights.
+* seastep: mixed-layer energy (with the flux handed to the ice), a
+  saturated surface air and Charnock's roughness.
 
-The remaining parameterizations (surflx, kuo/mkshallow, mkclouds, soil,
-rivers, land, sea ice, ocean, carbon, mko3) have no such pin here.
+The remaining parameterizations (surflx, mkclouds, rivers, the land
+surface's albedos, the sea-ice model, carbon) have no such pin here.
 Each test states the tolerance it needs: FP64 arithmetic, so conservation
 holds to rounding.
 """
@@ -562,3 +571,211 @@ def test_mkrain_melting_and_freezing_exchange_latent_heat_of_fusion(column):
         dT = _planes(run, "dtdt", nh)[:L]
         heat = np.sum(ds * cp * (1 + adv * q) * dT, axis=0)
         np.testing.assert_allclose(heat, (-lf if warm else lf) * falling, rtol=1e-12)
+
+
+def test_kuo_consumes_the_moisture_supply(column):
+    """Kuo convection: in a convecting column the moisture supply of the
+    cloud layers (convergence plus earlier tendencies) either moistens the
+    cloud or rains out, so the column's total moisture tendency plus the
+    convective rain is unchanged, and the rain's latent heat L_v is the
+    heating of the column's moist air. Where the supply is not positive the
+    shallow branch only redistributes heat and moisture (both column sums
+    stay). Crafted tropical soundings with convergence in the lowest four
+    layers; both branches occur."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(15)
+    sig = run.table("sigma")
+    ps = rng.uniform(9.5e4, 1.03e5, nh)
+    run.field("dp")[:] = ps
+    tsurf = rng.uniform(290, 305, nh)
+    t = tsurf * sig[:, None] ** rng.uniform(0.12, 0.2, nh)
+    _planes(run, "dt", nh)[:L] = t
+    _planes(run, "dt", nh)[L] = tsurf + rng.uniform(0, 3, nh)
+    q = _saturation(run, t, sig[:, None] * ps) * rng.uniform(0.5, 0.98, (L, nh))
+    _planes(run, "dq", nh)[:L] = q
+    _planes(run, "dq", nh)[L] = q[-1] * 1.05
+    run.field("dqt")[:] = 0.0
+    to_model = ps / (run.constant("psurf") * run.constant("ww"))
+    _planes(run, "dqt", nh)[L - 4:L] = rng.uniform(-0.2, 1, (4, nh)) * 1e-7 * to_model
+    dqdt0 = rng.uniform(-2e-9, 2e-9, (L, nh))
+    dtdt0 = rng.uniform(-1e-5, 1e-5, (L, nh))
+    _planes(run, "dqdt", nh)[:L] = dqdt0
+    _planes(run, "dtdt", nh)[:L] = dtdt0
+    run.routine("kuo")
+    ds = run.table("dsigma")[:, None]
+    dqt = _planes(run, "dqt", nh)[:L]  # in physical units now
+    dqdt = _planes(run, "dqdt", nh)[:L]
+    dtdt = _planes(run, "dtdt", nh)[:L]
+    rain = np.sum(_planes(run, "dprcl", nh)[:L], axis=0)
+    assert np.all(_planes(run, "dprscl", nh)[:L] == 0) and np.all(rain >= 0)
+    before = np.sum(ds * (dqt + dqdt0), axis=0)
+    after = np.sum(ds * (dqt + dqdt), axis=0) + rain
+    scale = np.sum(ds * (np.abs(dqt) + np.abs(dqdt0)), axis=0)
+    assert np.all(np.abs(after - before) <= 1e-12 * scale)
+    deep = rain > 0
+    shallow = ~deep & np.any(dtdt != dtdt0, axis=0)
+    assert deep.sum() > 200 and shallow.sum() > 0
+    cp, adv, lv = run.constant("acpd"), run.constant("adv"), run.constant("alv")
+    heat = np.sum(ds * cp * (1 + adv * q) * (dtdt - dtdt0), axis=0)
+    np.testing.assert_allclose(heat[deep], lv * rain[deep], rtol=1e-10)
+    dT = np.sum(ds * (dtdt - dtdt0), axis=0)
+    dQ = np.sum(ds * (dqdt - dqdt0), axis=0)
+    # the diffusion steps T and q themselves: rounding of T (300 K) and q
+    # over one time step
+    dts = run.constant("deltsec2")
+    assert np.all(np.abs(dT[shallow]) <= 1e-14 * 300.0 / dts)
+    assert np.all(np.abs(dQ[shallow]) <= 1e-14 * 0.03 / dts)
+
+
+def _land_column(run, nh, rng):
+    run.field("dls")[:] = 1.0
+    run.field("dglac")[:] = 0.0
+    run.field("dwmax")[:] = rng.uniform(0.1, 0.5, nh)
+    run.field("dwatc")[:] = run.field("dwmax") * rng.uniform(0.2, 1.0, nh)
+    run.field("land_dtsm")[:] = rng.uniform(265, 300, nh)
+    _planes(run, "dsoilt", nh)[:5] = rng.uniform(265, 300, (5, nh))
+    run.field("dshfl")[:] = rng.uniform(-80, 40, nh)
+    run.field("dlhfl")[:] = rng.uniform(-150, 0, nh)
+    run.field("dflux")[:] = 0.0
+    _planes(run, "dflux", nh)[-1] = rng.uniform(-50, 400, nh)
+
+
+def test_soil_conserves_water_and_heat(column):
+    """The land column: rain, snowfall and evaporation go into the bucket,
+    the snow pack and runoff with nothing lost; on snow-free land the
+    surface layer and the five soil layers (backward-Euler heat equation,
+    insulated bottom) store exactly the net surface heat flux."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(16)
+    _land_column(run, nh, rng)
+    snow = rng.uniform(0, 0.3, nh) * (rng.uniform(0, 1, nh) < 0.5)
+    run.field("dsnowz")[:] = snow
+    dts_s = run.constant("deltsec")
+    rain = rng.uniform(0, 3e-7, nh)
+    snowfall = rng.uniform(0, 1e-7, nh) * (rng.uniform(0, 1, nh) < 0.5)
+    run.field("dprl")[:] = rain + snowfall  # totals include their snow
+    run.field("dprc")[:] = rng.uniform(0, 2e-7, nh)
+    run.field("dprs")[:] = snowfall
+    run.field("devap")[:] = -rng.uniform(0, 5e-8, nh)
+    before_soil = _planes(run, "dsoilt", nh)[:5].copy()
+    before_t = run.field("land_dtsm").copy()
+    before_w = run.field("dwatc").copy()
+    water_in = (run.field("devap") + run.field("dprl") + run.field("dprc")) * dts_s
+    flux = (run.field("dshfl") + run.field("dlhfl") + _planes(run, "dflux", nh)[-1]).copy()
+    assert np.all(before_w + water_in >= 0)  # the bucket does not run dry
+    run.routine("soil")
+    stored = (run.field("dwatc") - before_w) + run.field("drunoff") * dts_s + (run.field("dsnowz") - snow)
+    np.testing.assert_allclose(stored, water_in, rtol=0, atol=1e-15)
+    assert np.all(run.field("dwatc") <= run.field("dwmax"))
+    assert np.any(run.field("drunoff") > 0)
+    # over snow, evaporation is sublimation from the pack: where nothing
+    # melts the pack gains snowfall and loses sublimation, and the bucket
+    # only rain
+    new_snow = run.field("dsnowz")
+    kept = (snow > 0) & (run.field("dsmelt") == 0) & (new_snow > 0)
+    assert kept.sum() > 50
+    np.testing.assert_allclose((new_snow - snow)[kept],
+                               ((snowfall + run.field("devap")) * dts_s)[kept], rtol=1e-9, atol=1e-16)
+    bare = (snow == 0) & (snowfall == 0)
+    assert bare.sum() > 100
+    cap = run.constant("soilcap")
+    soilz = run.table("dsoilz")[:, None]
+    heat = cap * (run.constant("dztop") * (run.field("land_dts") - before_t)
+                  + np.sum(soilz * (_planes(run, "dsoilt", nh)[:5] - before_soil), axis=0))
+    # rounding of the temperatures (300 K) times the column's heat capacity
+    atol = 1e-13 * cap * (run.constant("dztop") + soilz.sum()) * 300.0
+    np.testing.assert_allclose(heat[bare], flux[bare] * dts_s, rtol=1e-9, atol=atol)
+    # the heat flux between layers l and l+1 (the change stored below the
+    # interface) is the conductivity over the distance between the layer
+    # centres times the new temperature difference (backward Euler)
+    new = _planes(run, "dsoilt", nh)[:5]
+    below = np.cumsum((cap * soilz * (new - before_soil) / dts_s)[::-1], axis=0)[::-1][1:]
+    gap = (new[:-1] - new[1:])[:, bare]
+    conductance = below[:, bare] / gap
+    expect = np.broadcast_to(2 * run.constant("soildiff") / (soilz[:-1] + soilz[1:]), gap.shape)
+    clear = np.abs(gap) > 0.5  # well-conditioned quotients (K)
+    np.testing.assert_allclose(conductance[clear], expect[clear], rtol=1e-8)
+
+
+def test_mko3_is_the_green_profile(column):
+    """Ozone from Green's (1964) profile: the amount above height z is
+    a (1 + e^(-b/c)) / (1 + e^((z - b)/c)) with the column total a, at
+    hypsometric heights z from the layers' temperatures; mixing ratios
+    convert with the density of ozone at STP (2.14 kg/m^3, amounts in cm)."""
+    run, nh, L = _run(column)
+    host, m, nlat, nlon, _ = column
+    rng = np.random.default_rng(17)
+    ps = rng.uniform(6e4, 1.05e5, nh)
+    run.field("dp")[:] = ps
+    t = rng.uniform(190, 300, (L, nh))
+    _planes(run, "dt", nh)[:L] = t
+    season = 0.6
+    run.field("radiation_step")[4] = season
+    run.routine("mko3")
+    ga, R = run.constant("ga"), run.constant("gascon")
+    b, c = run.constant("bo3"), run.constant("co3")
+    mu = np.polynomial.legendre.leggauss(nlat)[0][::-1]  # sin(latitude), north to south
+    sinlat = np.repeat(mu, nlon)
+    total = run.constant("a0o3") + run.constant("a1o3") * np.abs(sinlat) + run.constant("aco3") * sinlat * season
+    sh = run.table("sigmah")
+    assert sh[L - 1] == 1.0
+    # height of each layer's top interface above the surface
+    thickness = R * t / ga * np.log(sh[:, None] / np.append(np.nan, sh[:-1])[:, None])
+    ztop = np.cumsum(thickness[::-1], axis=0)[::-1]
+    above = lambda z: total * (1 + np.exp(-b / c)) / (1 + np.exp((z - b) / c))
+    amount = _planes(run, "dqo3", nh)[:L] * (100.0 / 2.14) * run.table("dsigma")[:, None] * ps / ga
+    zbottom = np.vstack([ztop[1:], np.zeros((1, nh))])
+    np.testing.assert_allclose(amount[1:], above(zbottom[1:]) - above(ztop[1:]), rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(amount[0], above(zbottom[0]), rtol=1e-10)
+    np.testing.assert_allclose(amount.sum(axis=0), total, rtol=1e-12)
+
+
+def test_seastep_mixed_layer_energy_and_surface_air(column):
+    """Sea points: the mixed layer (sea water, rho c_p of 1030 kg/m^3 times
+    4180 J/(kg K)) warms by the heat it gets, and what freezing takes from
+    it is handed to the ice as flux, so rho c_p h dSST/dt plus the flux to
+    the ice is the net heat in. The surface air is saturated at the sea's
+    temperature (Tetens), and the open-water roughness is Charnock's
+    z0 = alpha u*^2 / g with u*^2 = |tau| / rho_air."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(18)
+    assert run.constant("ice_coupled") == 1
+    assert abs(run.constant("crhos") * run.constant("cps") / (1030 * 4180) - 1) < 1e-12
+    run.field("dls")[:] = 0.0
+    run.field("xls")[:] = 0.0
+    run.field("yls")[:] = 0.0
+    run.field("ice_sums")[:] = 0.0
+    run.field("xsnow")[:] = 0.0
+    heat = rng.uniform(-300, 300, nh)
+    run.field("xcflux")[:] = heat
+    fldo = rng.uniform(-30, 30, nh)
+    run.field("yfldo")[:] = fldo
+    tf = run.constant("tfreeze")
+    sst = rng.uniform(tf - 1.5, tf + 20, nh)
+    run.field("ysst")[:] = sst
+    ice = rng.uniform(0, 1, nh) < 0.3
+    run.field("xiced")[:] = np.where(ice, rng.uniform(0.1, 2, nh), 0.0)
+    run.field("xicec")[:] = np.where(ice, 1.0, 0.0)
+    mld = rng.uniform(20, 100, nh)
+    run.field("ymld")[:] = mld
+    ts = rng.uniform(265, 305, nh)
+    run.field("xts")[:] = ts
+    ps = rng.uniform(9e4, 1.04e5, nh)
+    run.field("dp")[:] = ps
+    taux, tauy = rng.normal(0, 0.3, nh), rng.normal(0, 0.3, nh)
+    run.field("dtaux")[:] = taux
+    run.field("dtauy")[:] = tauy
+    run.routine("seastep")
+    dtmix = run.constant("dtmix")
+    stored = 1030 * 4180 * mld * (run.field("ysst") - sst) / dtmix
+    to_ice = run.field("xoheat")
+    # rounding of the SST (300 K) times rho c_p h / dt (up to 1.6e5 W/m^2/K)
+    np.testing.assert_allclose(stored + to_ice, heat + fldo, rtol=1e-9, atol=1e-7)
+    assert np.all(run.field("ysst")[ice] == tf)
+    assert np.all(to_ice[~ice & (sst > tf + 1)] == 0)
+    eps = run.constant("rdbrv")
+    np.testing.assert_allclose(_planes(run, "dq", nh)[L], _saturation(run, ts, ps), rtol=2e-3)
+    np.testing.assert_allclose(_planes(run, "dt", nh)[L], ts, rtol=0)
+    rho = ps / (run.constant("gascon") * ts)
+    z0 = np.maximum(0.018 * np.hypot(taux, tauy) / rho / run.constant("ga"), run.constant("dz0sea"))
+    np.testing.assert_allclose(run.field("dz0")[~ice], z0[~ice], rtol=1e-12)
