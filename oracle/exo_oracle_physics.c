@@ -7,13 +7,13 @@
  * (shared/gpu/math/wgsl/precise.wgsl) are libm's log/exp/pow here.
  *
  * Pinned by physics on crafted columns (tests/test_oracle_physics_pins.py):
- * lwr, swr, solang, mko3, radstep, kuo, mkdca, mklsp, mkrain, vdiff, soil,
- * the ocean mixed layer and the fixer, each against laws their results
+ * surflx, lwr, swr, solang, mko3, radstep, kuo, mkdca, mklsp, mkrain, vdiff,
+ * soil, the ocean mixed layer and the fixer, each against laws their results
  * must obey (Kirchhoff, energy and water conservation, backward-Euler
  * diffusion, textbook latent heats, Tetens saturation, random cloud
- * overlap, Green's ozone profile, Charnock's roughness).
+ * overlap, Green's ozone profile, Charnock's roughness, the log law).
  *
- * Parity unpinned: surflx, mkclouds, rivers, the land surface's albedos,
+ * Parity unpinned: mkclouds, rivers, the land surface's albedos,
  * the sea-ice model (sea_ice) and carbon. No closed form, library routine or
  * reference golden vector checks them; they are held only to the GPU
  * kernels written from the same reading of the reference WGSL, and by the

@@ -37,9 +37,12 @@ oracle fails at least one test each.
 * mko3: Green's (1964) ozone profile at hypsometric heights.
 * seastep: mixed-layer energy (with the flux handed to the ice), a
   saturated surface air and Charnock's roughness.
+* surflx: the log law in neutral air, Louis's stability dependence, and
+  the lowest layer's momentum, enthalpy and water budgets against the
+  stress, sensible and latent heat fluxes and evaporation.
 
-The remaining parameterizations (surflx, mkclouds, rivers, the land
-surface's albedos, the sea-ice model, carbon) have no such pin here.
+The remaining parameterizations (mkclouds, rivers, the land surface's
+albedos, the sea-ice model, carbon) have no such pin here.
 Each test states the tolerance it needs: FP64 arithmetic, so conservation
 holds to rounding.
 """
@@ -779,3 +782,112 @@ def test_seastep_mixed_layer_energy_and_surface_air(column):
     rho = ps / (run.constant("gascon") * ts)
     z0 = np.maximum(0.018 * np.hypot(taux, tauy) / rho / run.constant("ga"), run.constant("dz0sea"))
     np.testing.assert_allclose(run.field("dz0")[~ice], z0[~ice], rtol=1e-12)
+
+
+def _surface_layer(run, nh, L, rng, ts):
+    ps = rng.uniform(8e4, 1.04e5, nh)
+    run.field("dp")[:] = ps
+    u, v = rng.normal(0, 8, nh), rng.normal(0, 8, nh)
+    t = rng.uniform(260, 300, nh)
+    q = rng.uniform(1e-3, 1.5e-2, nh)
+    for name, x in (("du", u), ("dv", v), ("dt", t), ("dq", q)):
+        _planes(run, name, nh)[L - 1] = x
+    for name in ("dudt", "dvdt", "dtdt", "dqdt"):
+        run.field(name)[:] = 0.0
+    _planes(run, "dt", nh)[L] = ts(t)
+    _planes(run, "dq", nh)[L] = rng.uniform(1e-3, 2e-2, nh)
+    run.field("dz0")[:] = 10 ** rng.uniform(-4, 0, nh)
+    run.field("drhs")[:] = 1.0
+    dry = rng.uniform(0, 1, nh) < 0.3  # nearly empty buckets
+    run.field("dwatc")[:] = np.where(dry, rng.uniform(0, 2e-4, nh), rng.uniform(0, 0.2, nh))
+    return ps, u, v, t, q
+
+
+def test_surflx_log_law_and_layer_budgets(column):
+    """Bulk surface fluxes into the lowest layer:
+    - in neutral air (equal virtual potential temperature of air and
+      surface) the transfer coefficient is the log law (kappa /
+      ln(1 + z/z0))^2 |u|, kappa = 0.4, z the hypsometric height of the
+      lowest level; stable air transfers less, unstable air more;
+    - the stress is rho_s C |u| u on the new wind (backward Euler), and it
+      is exactly the momentum the layer loses; u*^3 = (|tau| / rho_s)^1.5;
+    - the sensible heat flux is what the layer's moist air loses, beside
+      the kinetic energy the stress dissipates into heat;
+    - the evaporation is the water the layer gains, its latent heat the
+      textbook L_v (L_s over cold land), and land never evaporates more
+      than its bucket holds."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(19)
+    sig = run.table("sigma")[L - 1]
+    kappa_air = run.constant("akap")
+    assert run.table("flux_lnsig")[0] == pytest.approx(np.log(sig), rel=1e-12)
+    assert run.table("flux_sigfac")[0] == pytest.approx(sig ** -kappa_air, rel=1e-12)
+    moist = 1 / run.constant("rdbrv") - 1
+    R, ga = run.constant("gascon"), run.constant("ga")
+    for land in (1.0, 0.0):
+        run.field("dls")[:] = land
+        # neutral: the surface's virtual temperature equals the air's
+        # potential virtual temperature at the surface
+        ps, u, v, t, q = _surface_layer(run, nh, L, rng, lambda t: t)  # surface set below
+        qs = _planes(run, "dq", nh)[L].copy()
+        theta_v = t * sig ** -kappa_air * (1 + moist * q)
+        ts = theta_v / (1 + moist * qs)
+        _planes(run, "dt", nh)[L] = ts
+        run.routine("surflx")
+        z = R * 0.5 * (t + theta_v) * np.log(1 / sig) / ga
+        speed = np.sqrt(np.maximum(run.constant("zumin"), u * u + v * v))
+        log_law = speed * (0.4 / np.log(1 + z / run.field("dz0"))) ** 2
+        # the Richardson number is zero up to rounding (~1e-16), which the
+        # stability functions' square and cube roots lift to ~1e-7
+        np.testing.assert_allclose(run.field("dtransm"), log_law, rtol=1e-6)
+        np.testing.assert_allclose(run.field("dtransh"), log_law, rtol=1e-6)
+        # stable (air 3 K warmer than neutral) and unstable (3 K colder)
+        for shift, cmp in ((3.0, np.less), (-3.0, np.greater)):
+            _planes(run, "du", nh)[L - 1], _planes(run, "dv", nh)[L - 1] = u, v
+            _planes(run, "dt", nh)[L - 1] = t
+            _planes(run, "dq", nh)[L - 1] = q
+            _planes(run, "dt", nh)[L] = ts - shift
+            for name in ("dudt", "dvdt", "dtdt", "dqdt"):
+                run.field(name)[:] = 0.0
+            run.routine("surflx")
+            assert np.all(cmp(run.field("dtransm"), log_law)), (land, shift)
+            assert np.all(cmp(run.field("dtransh"), log_law)), (land, shift)
+        # very stable air (bulk Richardson number above 1): Louis's heat
+        # transfer falls to well below half of the momentum transfer
+        _planes(run, "dt", nh)[L] = ts - 15.0
+        run.routine("surflx")
+        theta_s = (ts - 15.0) * (1 + moist * qs)
+        ri = ga * z * (theta_v - theta_s) / (speed ** 2 * theta_v)
+        strong = ri > 1
+        assert strong.sum() > 100
+        assert np.all(run.field("dtransh")[strong] < 0.5 * run.field("dtransm")[strong])
+    # budgets on random (non-neutral) layers over land and sea
+    run.field("dls")[:] = (rng.uniform(0, 1, nh) < 0.5).astype(float)
+    ps, u, v, t, q = _surface_layer(run, nh, L, rng, lambda t: t + rng.uniform(-8, 8, t.shape))
+    ts = _planes(run, "dt", nh)[L].copy()
+    run.routine("surflx")
+    dts = run.constant("deltsec2")
+    mass = run.table("dsigma")[L - 1] * ps / ga
+    un = u + _planes(run, "dudt", nh)[L - 1] * dts
+    vn = v + _planes(run, "dvdt", nh)[L - 1] * dts
+    rho_s = ps / (R * ts)
+    trans_m = run.field("dtransm")
+    np.testing.assert_allclose(run.field("dtaux"), rho_s * trans_m * un, rtol=1e-10)
+    np.testing.assert_allclose(run.field("dtauy"), rho_s * trans_m * vn, rtol=1e-10)
+    np.testing.assert_allclose(mass * (un - u) / dts, -run.field("dtaux"), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(mass * (vn - v) / dts, -run.field("dtauy"), rtol=1e-9, atol=1e-12)
+    tau = np.hypot(run.field("dtaux"), run.field("dtauy"))
+    np.testing.assert_allclose(run.field("dust3"), (tau / rho_s) ** 1.5, rtol=1e-10)
+    cp, adv = run.constant("acpd"), run.constant("adv")
+    heating = mass * cp * (1 + adv * q) * _planes(run, "dtdt", nh)[L - 1]
+    dissipated = -mass * 0.5 * (un ** 2 - u ** 2 + vn ** 2 - v ** 2) / dts
+    assert run.constant("ndheat") != 0
+    np.testing.assert_allclose(heating, -run.field("dshfl") + dissipated, rtol=1e-8, atol=1e-6)
+    gained = mass * _planes(run, "dqdt", nh)[L - 1]
+    np.testing.assert_allclose(gained, -run.field("devap") * 1000.0, rtol=1e-10, atol=1e-16)
+    latent = np.where((ts > run.constant("tmelt")) | (run.field("dls") < 0.5), 2.501e6, 2.834e6)
+    np.testing.assert_allclose(run.field("dlhfl"), run.field("devap") * 1000.0 * latent, rtol=2e-3)
+    on_land = run.field("dls") > 0
+    bucket = run.field("dwatc") / run.constant("deltsec")
+    assert np.all(-run.field("devap")[on_land] <= bucket[on_land] * (1 + 1e-12))
+    assert np.any(-run.field("devap")[on_land] >= bucket[on_land] * (1 - 1e-12))  # the limit binds
