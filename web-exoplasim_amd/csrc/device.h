@@ -47,6 +47,31 @@ __device__ __forceinline__ int plan_find(const LaunchPlan& p, int block, int* ti
   return p.member[lo];
 }
 
+// Members one Legendre workgroup serves at most (FP32 form).
+constexpr int kLegGroup = 4;
+
+// Members grouped for the transform kernels: group i is list[start[i] ..
+// start[i] + count[i]), members of one truncation and level count (one set
+// of shared tables); first[i] is its first workgroup, first[n] the total.
+struct GroupPlan {
+  int n;
+  int first[EXO_MAX_MEMBERS + 1];
+  int start[EXO_MAX_MEMBERS];
+  int count[EXO_MAX_MEMBERS];
+  int list[EXO_MAX_MEMBERS];
+};
+
+__device__ __forceinline__ int group_find(const GroupPlan& p, int block, int* tile) {
+  int lo = 0, hi = p.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.first[mid] <= block) lo = mid;
+    else hi = mid - 1;
+  }
+  *tile = block - p.first[lo];
+  return lo;
+}
+
 #define SLOTP(M, name) ((M).arena + (M).slot[SLOT_##name])
 #define TABP(M, name) ((M).tab + (M).tabo[TAB_##name])
 #define KON(M, name) ((M).k[CON_##name])

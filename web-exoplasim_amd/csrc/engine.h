@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <initializer_list>
 #include <map>
@@ -16,10 +17,10 @@
 #include "setup.h"
 #include "transforms.h"
 
-extern "C" __global__ void k_leg_inv(const DevMember*, LaunchPlan, InvJobs);
+extern "C" __global__ void k_leg_inv(const DevMember*, GroupPlan, InvJobs);
 extern "C" __global__ void k_dft_inv(const DevMember*, LaunchPlan, InvJobs);
 extern "C" __global__ void k_dft_fwd(const DevMember*, LaunchPlan, FwdJobs);
-extern "C" __global__ void k_leg_fwd(const DevMember*, LaunchPlan, LegJobs);
+extern "C" __global__ void k_leg_fwd(const DevMember*, GroupPlan, LegJobs);
 extern "C" __global__ void k_leg_inv_split(const DevMember*, LaunchPlan, InvJobs);
 extern "C" __global__ void k_leg_fwd_split(const DevMember*, LaunchPlan, LegJobs);
 extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan, int);
@@ -149,6 +150,45 @@ class Stepper {
     return p;
   }
 
+  // Members of form `form` in groups of at most kLegGroup sharing
+  // transform tables and level count, in active order within a group;
+  // blocks(d) workgroups per group. The groups are dealt heaviest first
+  // (highest truncation, then most levels), so the long workgroups start
+  // first instead of trailing the launch.
+  template <typename F>
+  GroupPlan group_plan(int form, F blocks) const {
+    std::vector<std::vector<int>> groups;
+    std::vector<char> taken(active_.size(), 0);
+    for (size_t a = 0; a < active_.size(); a++) {
+      const DevMember& d = e_.h_desc[active_[a]];
+      if (taken[a] || d.precision != form) continue;
+      std::vector<int> g;
+      for (size_t b = a; b < active_.size() && (int)g.size() < kLegGroup; b++) {
+        const DevMember& o = e_.h_desc[active_[b]];
+        if (taken[b] || o.precision != form || o.ltab != d.ltab || o.nlev != d.nlev) continue;
+        taken[b] = 1;
+        g.push_back(active_[b]);
+      }
+      groups.push_back(std::move(g));
+    }
+    std::stable_sort(groups.begin(), groups.end(), [&](const std::vector<int>& x, const std::vector<int>& y) {
+      const DevMember &a = e_.h_desc[x[0]], &b = e_.h_desc[y[0]];
+      return a.ntru != b.ntru ? a.ntru > b.ntru : a.nlev > b.nlev;
+    });
+    GroupPlan p{};
+    int at = 0, listed = 0;
+    for (const std::vector<int>& g : groups) {
+      p.start[p.n] = listed;
+      p.count[p.n] = (int)g.size();
+      for (int id : g) p.list[listed++] = id;
+      p.first[p.n] = at;
+      at += blocks(e_.h_desc[g[0]]);
+      p.n++;
+    }
+    p.first[p.n] = at;
+    return p;
+  }
+
   void column(void (*k)(const DevMember*, LaunchPlan)) {
     const LaunchPlan p = plan(tiles_column);
     launch(e_, k, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
@@ -170,12 +210,12 @@ class Stepper {
     launch(e_, k_mean_mode, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, slot);
   }
   void inv(const InvJobs& jobs) {
-    // the Legendre half per operand form (plain FP32, split FP16)
-    for (int form = 0; form < 2; form++) {
-      const LaunchPlan pl = plan([&](const DevMember& d) { return d.precision == form ? (d.ntru + 1) * jobs.n : 0; });
-      if (pl.first[pl.n] > 0)
-        launch(e_, form ? k_leg_inv_split : k_leg_inv, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
-    }
+    // the Legendre half per operand form: plain FP32 over member groups,
+    // split FP16 per member
+    const GroupPlan pg = group_plan(0, [&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
+    if (pg.first[pg.n] > 0) launch(e_, k_leg_inv, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
+    const LaunchPlan pl = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
+    if (pl.first[pl.n] > 0) launch(e_, k_leg_inv_split, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
     // k_dft_inv: 64 rows (output x level x latitude) of one job per workgroup
     const LaunchPlan pd = plan([&](const DevMember& d) {
       int blocks = 0;
@@ -195,11 +235,10 @@ class Stepper {
     launch(e_, k_dft_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
   }
   void leg(const LegJobs& jobs) {
-    for (int form = 0; form < 2; form++) {
-      const LaunchPlan p = plan([&](const DevMember& d) { return d.precision == form ? (d.ntru + 1) * jobs.n : 0; });
-      if (p.first[p.n] > 0)
-        launch(e_, form ? k_leg_fwd_split : k_leg_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
-    }
+    const GroupPlan pg = group_plan(0, [&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
+    if (pg.first[pg.n] > 0) launch(e_, k_leg_fwd, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
+    const LaunchPlan p = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
+    if (p.first[p.n] > 0) launch(e_, k_leg_fwd_split, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
   }
   void load_step(const float* d_params) {
     const LaunchPlan p = plan([](const DevMember&) { return 1; });

@@ -35,7 +35,6 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kThreads = 256;
 constexpr int kMaxLat = 96, kMaxK = 64, kMaxLev = 10;
 // output tiles per wave: inverse dv2uv at T63 L10 is 6 x 3 tiles
 constexpr int kMaxTiles = 5;
@@ -51,17 +50,15 @@ __device__ __forceinline__ int job_of(const Jobs& jobs, int index) {
 // 2 mod 32 spreads them over 32 banks).
 __host__ __device__ constexpr int a_stride(int r) { return (r + 29) / 32 * 32 + 2; }
 
-// Staged inputs: inverse rawS[(f * kMaxLev + p) * kRawK + 2k + part]
-// (f = source field, k < K padded with zeros to a multiple of 4); forward
-// raw[(f * kMaxLev + p) * kRawLat + 2 lat + part].
+// Staged inputs: inverse raw[row * kRawK + 2k + part] (k < K padded with
+// zeros to a multiple of 4); forward raw[row * kRawLat + 2 lat + part],
+// rows over (member, field, level).
 // Row strides are 4 mod 32: the 16 product columns one MFMA step reads
 // (8 rows x 2 parts) and its 2 row offsets then fall in 32 distinct banks.
 constexpr int kRawK = 2 * kMaxK + 4;
 constexpr int kRawLat = 2 * kMaxLat + 4;
 constexpr int kInvAs = kMaxLat * a_stride(kMaxK);
-constexpr int kInvRaw = 2 * kMaxLev * kRawK;
 constexpr int kFwdAs = kMaxK * a_stride(kMaxLat / 2);
-constexpr int kFwdRaw = 3 * kMaxLev * kRawLat;
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
 // b mod 8). Within each run of 64 blocks, XCD x is given the 8 consecutive
@@ -75,23 +72,23 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
   return (run << 6) + (b & 7) * 8 + ((b >> 3) & 7);
 }
 
-// LDS staging of a rows x cols block: lane = column (in runs of 64), the 4
-// waves take rows r0, r0 + 4, ...; each lane issues 8 loads before it
+// LDS staging of a rows x cols block: lane = column (in runs of 64), the W
+// waves take rows r0, r0 + W, ...; each lane issues 8 loads before it
 // stores any, so one memory latency covers 8 rows.
-template <typename Load, typename Store>
+template <int W = 4, typename Load, typename Store>
 __device__ __forceinline__ void stage(int rows, int cols, Load load, Store store) {
   const int r0 = threadIdx.x >> 6;
   for (int c = threadIdx.x & 63; c < cols; c += 64) {
-    for (int base = r0; base < rows; base += 32) {
+    for (int base = r0; base < rows; base += 8 * W) {
       float v[8];
 #pragma unroll
       for (int j = 0; j < 8; j++) {
-        const int r = base + 4 * j;
+        const int r = base + W * j;
         v[j] = r < rows ? load(r, c) : 0.0f;
       }
 #pragma unroll
       for (int j = 0; j < 8; j++) {
-        const int r = base + 4 * j;
+        const int r = base + W * j;
         if (r < rows) store(r, c, v[j]);
       }
     }
@@ -100,38 +97,45 @@ __device__ __forceinline__ void stage(int rows, int cols, Load load, Store store
 
 // Where a product column's operand sits in the staged inputs (its offset
 // for row 0; rows step by 2 floats) and its sign; sign 0 is a zero column.
+// Staged input rows run over (member of the chunk, field, level):
+// row = (g nf + f) P + p.
 struct Col {
   int base;
   float sign;
 };
 
-__device__ __forceinline__ Col col_of(int field, int p, int part, float sign, int row_floats) {
-  return Col{(field * kMaxLev + p) * row_floats + part, sign};
+__device__ __forceinline__ Col gcol(int g, int nf, int P, int field, int p, int part, float sign, int row_floats) {
+  return Col{((g * nf + field) * P + p) * row_floats + part, sign};
 }
 
-// inverse: plain (S); dv2uv fields 0 = D, 1 = zeta, block 0 = qu, 1 = qv
-__device__ __forceinline__ Col inv_col(bool winds, int blk, int c, int NC) {
+// inverse: plain (S); dv2uv fields 0 = D, 1 = zeta, block 0 = qu, 1 = qv.
+// Column c of the chunk belongs to member c / NCm.
+__device__ __forceinline__ Col inv_col(bool winds, int blk, int c, int NC, int NCm, int P) {
   if (c >= NC) return Col{0, 0.0f};
-  if (!winds) return col_of(0, c >> 1, c & 1, 1.0f, kRawK);
+  const int g = c / NCm;
+  c -= g * NCm;
+  if (!winds) return gcol(g, 1, P, 0, c >> 1, c & 1, 1.0f, kRawK);
   const int p = c >> 2, q = c & 3;
-  if (blk == 0) return col_of(q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? -1.0f : 1.0f, kRawK);
-  return col_of((q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? -1.0f : 1.0f, kRawK);
+  if (blk == 0) return gcol(g, 2, P, q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? -1.0f : 1.0f, kRawK);
+  return gcol(g, 2, P, (q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? -1.0f : 1.0f, kRawK);
 }
 
 // forward: fields 0 = u (plain: the input), 1 = v, 2 = ke (qtend: s)
-__device__ __forceinline__ Col fwd_col(int kind, int blk, int c, int NC) {
+__device__ __forceinline__ Col fwd_col(int kind, int blk, int c, int NC, int NCm, int nf, int P) {
   if (c >= NC) return Col{0, 0.0f};
-  if (kind == LEG_PLAIN) return col_of(0, c >> 1, c & 1, 1.0f, kRawLat);
+  const int g = c / NCm;
+  c -= g * NCm;
+  if (kind == LEG_PLAIN) return gcol(g, nf, P, 0, c >> 1, c & 1, 1.0f, kRawLat);
   if (kind == LEG_UV2DV) {
     const int p = c >> 2, q = c & 3;
-    if (blk == 0) return col_of((q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? 1.0f : -1.0f, kRawLat);
-    if (blk == 1) return col_of(q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? 1.0f : -1.0f, kRawLat);
-    return q < 2 ? col_of(2, p, q, 1.0f, kRawLat) : Col{0, 0.0f};
+    if (blk == 0) return gcol(g, nf, P, (q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? 1.0f : -1.0f, kRawLat);
+    if (blk == 1) return gcol(g, nf, P, q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? 1.0f : -1.0f, kRawLat);
+    return q < 2 ? gcol(g, nf, P, 2, p, q, 1.0f, kRawLat) : Col{0, 0.0f};
   }
   const int p = c >> 1, q = c & 1;
-  if (blk == 0) return col_of(1, p, q, 1.0f, kRawLat);
-  if (blk == 1) return col_of(2, p, q, 1.0f, kRawLat);
-  return col_of(0, p, q ? 0 : 1, q ? -1.0f : 1.0f, kRawLat);
+  if (blk == 0) return gcol(g, nf, P, 1, p, q, 1.0f, kRawLat);
+  if (blk == 1) return gcol(g, nf, P, 2, p, q, 1.0f, kRawLat);
+  return gcol(g, nf, P, 0, p, q ? 0 : 1, q ? -1.0f : 1.0f, kRawLat);
 }
 
 // ---------------------------------------------- split-operand form
@@ -205,88 +209,174 @@ __device__ __forceinline__ f4 mfma_split(const h8& ahi, const h8& alo, const h8&
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, c, 0, 0, 0);
 }
 
+// ------------------------------------------------ member groups (FP32)
+// Members of one truncation and level count share the transform tables, so
+// one workgroup (8 waves) serves a group of them: it stages the table
+// columns of m once and runs the product for the members in chunks whose
+// inputs fit the staging buffer, the members' product columns side by
+// side. Every output element is the same sum, in the same order, whatever
+// the group, so a member computes the same alone and in a batch.
+
+constexpr int kGroupThreads = 512;
+constexpr int kGroupWaves = kGroupThreads / 64;
+constexpr int kGTiles = 4;     // output tiles per wave and chunk (<= 32 per chunk)
+constexpr int kInvRows = 40;   // staged input rows: inverse
+constexpr int kFwdRows = 30;   // staged input rows: forward
+// input rows of a whole group: members x fields x levels
+constexpr int kLegGroupRows = 128;
+static_assert(kLegGroup * 3 * kMaxLev <= kLegGroupRows, "group input rows");
+
+__device__ __forceinline__ int group_member(const GroupPlan& plan, int g, int i) {
+  return plan.list[plan.start[g] + i];
+}
+
+// Batched staging: element e = threadIdx.x + kGroupThreads j (j < N) is row
+// e >> LOGC, column e & (2^LOGC - 1) of a rows x cols block. load_batch
+// issues all N loads before anything waits on them; store_batch then
+// writes them to LDS, so a staged block costs one memory latency.
+template <int N, int LOGC, typename Load>
+__device__ __forceinline__ void load_batch(int rows, int cols, Load load, float* v) {
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int e = threadIdx.x + kGroupThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
+    v[j] = (r < rows && c < cols) ? load(r, c) : 0.0f;
+  }
+}
+
+template <int N, int LOGC, typename Store>
+__device__ __forceinline__ void store_batch(int rows, int cols, Store store, const float* v) {
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int e = threadIdx.x + kGroupThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
+    if (r < rows && c < cols) store(r, c, v[j]);
+  }
+}
+
+// batch sizes: inverse table nlat x 64 (<= 96 x 64), inputs rows x 128;
+// forward table half x 64 (<= 48 x 64), inputs rows x 256 (2 nlat <= 192)
+constexpr int kInvTabN = kMaxLat * 64 / kGroupThreads;
+constexpr int kInvRawN = kInvRows * 128 / kGroupThreads;
+constexpr int kFwdTabN = kMaxLat / 2 * 64 / kGroupThreads;
+constexpr int kFwdRawN = kFwdRows * 256 / kGroupThreads;
+static_assert(kMaxLat * 64 % kGroupThreads == 0 && kInvRows * 128 % kGroupThreads == 0, "batch sizes");
+static_assert(kMaxLat / 2 * 64 % kGroupThreads == 0 && kFwdRows * 256 % kGroupThreads == 0, "batch sizes");
+static_assert(2 * kMaxK <= 128 && 2 * kMaxLat <= 256 && kMaxK <= 64, "batch columns");
+
 // ------------------------------------------------------------- inverse
 
 template <int P>
-__device__ void leg_inv(const DevMember& M, const InvJob& job, int m, float* As, float* raw) {
+__device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int m, float* As,
+                        float* raw, const float** rowp, float* rowsub) {
+  const DevMember& M = mems[group_member(plan, grp, 0)];
+  const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
   const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
   const bool winds = job.kind == INV_DV2UV;
-  const int NC = winds ? 4 * P : 2 * P;
-  const int NT = (NC + 15) / 16, MT = nlat / 16;
+  const int nf = winds ? 2 : 1;
+  const int NCm = winds ? 4 * P : 2 * P;  // product columns of one member
+  const int chunk = max(1, min(count, kInvRows / (nf * P)));
+  const int MT = nlat / 16;
   const int R = (K + 3) & ~3;
   const int as = a_stride(R);
-  const int nf = winds ? 2 : 1;
-  const float plavor = job.plavor ? M.k[CON_plavor] : 0.0f;
-  // the job's coefficients of m (zeta without the planetary vorticity in
-  // Re mode (0, 1)), zeros past K
-  stage(
-      nf * P, 2 * R,
-      [&](int fp, int x) {
-        const int f = fp / P, p = fp % P, k = x >> 1;
-        if (k >= K) return 0.0f;
-        const float v = M.arena[M.slot[f ? job.src1 : job.src0] + p * nrsp + 2 * w0 + x];
-        return (winds && f == 1 && w0 + k == 1 && (x & 1) == 0) ? v - plavor : v;
-      },
-      [&](int fp, int x, float v) { raw[((fp / P) * kMaxLev + fp % P) * kRawK + x] = v; });
-  f4 acc[kMaxTiles];
-#pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) acc[t] = f4{0, 0, 0, 0};
+  const int blocks = winds ? 2 : 1;
+  const float* q0 = M.ltab + M.tabo[winds ? (int)TAB_legendre_qu : job.table];
+  const float* q1 = M.ltab + M.tabo[TAB_legendre_qv];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
-  const int blocks = winds ? 2 : 1;
-  for (int blk = 0; blk < blocks; blk++) {
-    const float* q = M.ltab + M.tabo[winds ? (blk == 0 ? (int)TAB_legendre_qu : (int)TAB_legendre_qv) : job.table];
-    stage(
-        nlat, R, [&](int lat, int k) { return k < K ? q[lat * ncsp + w0 + k] : 0.0f; },
-        [&](int lat, int k, float v) { As[lat * as + k] = v; });
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kMaxTiles; t++) {
-      const int tile = wave + 4 * t;
-      if (tile < MT * NT) {
-        const int mt = tile / NT, nt = tile % NT;
-        const Col col = inv_col(winds, blk, nt * 16 + i, NC);
-        const float* a = As + (mt * 16 + i) * as + kk;
-        const float* b = raw + col.base + 2 * kk;
-        f4 c = acc[t];
-        for (int r = 0; r < R; r += 4)
-          c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], col.sign * b[2 * r], c, 0, 0, 0);
-        acc[t] = c;
-      }
-    }
-    __syncthreads();
+  // where each input row of the group starts (member g, field f, level p
+  // at row (g nf + f) P + p), and the planetary vorticity it drops from
+  // Re mode (0, 1) (zeta rows of dv2uv)
+  if (threadIdx.x < count * nf * P) {
+    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
+    const DevMember& G = mems[group_member(plan, grp, g)];
+    rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp + 2 * w0;
+    rowsub[row] = (job.plavor && f == 1 && w0 <= 1 && 1 < w0 + K) ? G.k[CON_plavor] : 0.0f;
   }
-  float* F0 = M.arena + M.slot[job.four0];
-  float* F1 = winds ? M.arena + M.slot[job.four1] : nullptr;
+  // the table columns of m for the whole group (dv2uv: qu, qv)
 #pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) {
-    const int tile = wave + 4 * t;
-    if (tile < MT * NT) {
-      const int mt = tile / NT, nt = tile % NT;
-      const int c = nt * 16 + i;
-      if (c < NC) {
-        float* base = winds ? ((c & 2) ? F1 : F0) : F0;
-        const int p = winds ? c >> 2 : c >> 1, part = c & 1;
+  for (int blk = 0; blk < 2; blk++) {
+    if (blk < blocks) {
+      const float* q = blk ? q1 : q0;
+      float* A = As + blk * kInvAs;
+      float va[kInvTabN];
+      load_batch<kInvTabN, 6>(nlat, R, [&](int lat, int k) { return k < K ? q[lat * ncsp + w0 + k] : 0.0f; }, va);
+      store_batch<kInvTabN, 6>(nlat, R, [&](int lat, int k, float v) { A[lat * as + k] = v; }, va);
+    }
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < count; c0 += chunk) {
+    const int n = min(chunk, count - c0);
+    const int NC = n * NCm, NT = (NC + 15) / 16;
+    // the members' coefficients of m (zeta without the planetary vorticity
+    // in Re mode (0, 1)), zeros past K
+    const int r0 = c0 * nf * P;
+    const auto raw_load = [&](int row, int x) {
+      const int k = x >> 1;
+      if (k >= K) return 0.0f;
+      const float v = rowp[r0 + row][x];
+      return (w0 + k == 1 && (x & 1) == 0) ? v - rowsub[r0 + row] : v;
+    };
+    const auto raw_store = [&](int row, int x, float v) { raw[row * kRawK + x] = v; };
+    float vr[kInvRawN];
+    load_batch<kInvRawN, 7>(n * nf * P, 2 * R, raw_load, vr);
+    store_batch<kInvRawN, 7>(n * nf * P, 2 * R, raw_store, vr);
+    __syncthreads();
+    f4 acc[kGTiles];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int lat = mt * 16 + 4 * kk + r;
-          base[(p * nlat + lat) * nfc + 2 * m + part] = acc[t][r];
+    for (int t = 0; t < kGTiles; t++) acc[t] = f4{0, 0, 0, 0};
+    for (int blk = 0; blk < blocks; blk++) {
+      const float* A = As + blk * kInvAs;
+#pragma unroll
+      for (int t = 0; t < kGTiles; t++) {
+        const int tile = wave + kGroupWaves * t;
+        if (tile < MT * NT) {
+          const int mt = tile / NT, nt = tile % NT;
+          const Col col = inv_col(winds, blk, nt * 16 + i, NC, NCm, P);
+          const float* a = A + (mt * 16 + i) * as + kk;
+          const float* b = raw + col.base + 2 * kk;
+          f4 c = acc[t];
+          for (int r = 0; r < R; r += 4)
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], col.sign * b[2 * r], c, 0, 0, 0);
+          acc[t] = c;
         }
       }
     }
+#pragma unroll
+    for (int t = 0; t < kGTiles; t++) {
+      const int tile = wave + kGroupWaves * t;
+      if (tile < MT * NT) {
+        const int mt = tile / NT, nt = tile % NT;
+        int c = nt * 16 + i;
+        if (c < NC) {
+          const int g = c / NCm;
+          c -= g * NCm;
+          const DevMember& G = mems[group_member(plan, grp, c0 + g)];
+          float* base = G.arena + G.slot[winds && (c & 2) ? job.four1 : job.four0];
+          const int p = winds ? c >> 2 : c >> 1, part = c & 1;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int lat = mt * 16 + 4 * kk + r;
+            base[(p * nlat + lat) * nfc + 2 * m + part] = acc[t][r];
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next chunk restages the inputs
   }
 }
 
 // ------------------------------------------------------------- forward
 
 template <int P>
-__device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* Ah, float* raw) {
+__device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, const LegJob& job, int m, float* Ah,
+                        float* raw, const float** rowp) {
+  const DevMember& M = mems[group_member(plan, grp, 0)];
+  const int count = plan.count[grp];
   const int nlat = M.nlat, half = nlat / 2, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
   const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
   const bool uv = job.kind == LEG_UV2DV;
-  const int NC = uv ? 4 * P : 2 * P;
-  const int NT = (NC + 15) / 16, MT = (K + 15) / 16;
+  const int NCm = uv ? 4 * P : 2 * P;
+  const int MT = (K + 15) / 16;
   const int ash = a_stride(half);
   // blocks: table and whether its southern rows take the extra minus sign
   int tables[3], diff[3], blocks = 0;
@@ -301,70 +391,88 @@ __device__ void leg_fwd(const DevMember& M, const LegJob& job, int m, float* Ah,
     tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
     tables[blocks] = TAB_legendre_qm, diff[blocks++] = 0;
   }
-  // the job's Fourier coefficients m of every row and level, once
   const int fields[3] = {job.kind == LEG_PLAIN ? job.src : job.u, job.v, uv ? job.ke : job.src};
   const int nf = job.kind == LEG_PLAIN ? 1 : (uv && job.ke < 0) ? 2 : 3;
-  stage(
-      nf * P, 2 * nlat,
-      [&](int fp, int x) {
-        const int f = fp / P, p = fp % P;
-        return M.arena[M.slot[fields[f]] + (p * nlat + (x >> 1)) * nfc + 2 * m + (x & 1)];
-      },
-      [&](int fp, int x, float v) { raw[((fp / P) * kMaxLev + fp % P) * kRawLat + x] = v; });
-  f4 acc[kMaxTiles];
-#pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) acc[t] = f4{0, 0, 0, 0};
+  const int chunk = max(1, min(count, kFwdRows / (nf * P)));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
-  for (int blk = 0; blk < blocks; blk++) {
-    const float* q = M.ltab + M.tabo[tables[blk]];
-    // the northern half of A: Ah[k][l] = q(l, w0 + k)
-    stage(
-        half, MT * 16, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; },
-        [&](int l, int k, float v) { Ah[k * ash + l] = v; });
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kMaxTiles; t++) {
-      const int tile = wave + 4 * t;
-      if (tile < MT * NT) {
-        const int mt = tile / NT, nt = tile % NT;
-        const int k = mt * 16 + i;
-        // southern rows: (-1)^(n-m) q, negated again for qe*
-        const float south = ((k & 1) ? -1.0f : 1.0f) * (diff[blk] ? -1.0f : 1.0f);
-        const Col col = fwd_col(job.kind, blk, nt * 16 + i, NC);
-        const float* a = Ah + k * ash;
-        const float* b = raw + col.base + 2 * kk;
-        f4 c = acc[t];
-        for (int r = 0; r < half; r += 4)
-          c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r + kk], col.sign * b[2 * r], c, 0, 0, 0);
-        for (int r = half; r < nlat; r += 4)
-          c = __builtin_amdgcn_mfma_f32_16x16x4f32(south * a[nlat - 1 - r - kk], col.sign * b[2 * r], c, 0, 0, 0);
-        acc[t] = c;
-      }
-    }
-    __syncthreads();
+  // where each input row of the group starts: Fourier coefficient m of
+  // latitude 0 of member g, field f, level p at row (g nf + f) P + p
+  if (threadIdx.x < count * nf * P) {
+    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
+    const DevMember& G = mems[group_member(plan, grp, g)];
+    rowp[row] = G.arena + G.slot[fields[f]] + p * nlat * nfc + 2 * m;
   }
-  float* O0 = M.arena + M.slot[job.dst0];
-  float* O1 = uv ? M.arena + M.slot[job.dst1] : nullptr;
+  // the northern half of each table for the whole group, Ah[k][l] = q(l, w0 + k)
 #pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) {
-    const int tile = wave + 4 * t;
-    if (tile < MT * NT) {
-      const int mt = tile / NT, nt = tile % NT;
-      const int c = nt * 16 + i;
-      if (c < NC) {
-        float* base = uv ? ((c & 2) ? O1 : O0) : O0;
-        const int p = uv ? c >> 2 : c >> 1, part = c & 1;
+  for (int blk = 0; blk < 3; blk++) {
+    if (blk < blocks) {
+      const float* q = M.ltab + M.tabo[tables[blk]];
+      float* A = Ah + blk * kFwdAs;
+      float va[kFwdTabN];
+      load_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; }, va);
+      store_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k, float v) { A[k * ash + l] = v; }, va);
+    }
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < count; c0 += chunk) {
+    const int n = min(chunk, count - c0);
+    const int NC = n * NCm, NT = (NC + 15) / 16;
+    // the members' Fourier coefficients m of every row and level
+    const int r0 = c0 * nf * P;
+    const auto raw_load = [&](int row, int x) { return rowp[r0 + row][(x >> 1) * nfc + (x & 1)]; };
+    float vr[kFwdRawN];
+    load_batch<kFwdRawN, 8>(n * nf * P, 2 * nlat, raw_load, vr);
+    store_batch<kFwdRawN, 8>(n * nf * P, 2 * nlat, [&](int row, int x, float v) { raw[row * kRawLat + x] = v; }, vr);
+    __syncthreads();
+    f4 acc[kGTiles];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int k = mt * 16 + 4 * kk + r;
-          if (k < K) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
+    for (int t = 0; t < kGTiles; t++) acc[t] = f4{0, 0, 0, 0};
+    for (int blk = 0; blk < blocks; blk++) {
+      const float* A = Ah + blk * kFwdAs;
+#pragma unroll
+      for (int t = 0; t < kGTiles; t++) {
+        const int tile = wave + kGroupWaves * t;
+        if (tile < MT * NT) {
+          const int mt = tile / NT, nt = tile % NT;
+          const int k = mt * 16 + i;
+          // southern rows: (-1)^(n-m) q, negated again for qe*
+          const float south = ((k & 1) ? -1.0f : 1.0f) * (diff[blk] ? -1.0f : 1.0f);
+          const Col col = fwd_col(job.kind, blk, nt * 16 + i, NC, NCm, nf, P);
+          const float* a = A + k * ash;
+          const float* b = raw + col.base + 2 * kk;
+          f4 c = acc[t];
+          for (int r = 0; r < half; r += 4)
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r + kk], col.sign * b[2 * r], c, 0, 0, 0);
+          for (int r = half; r < nlat; r += 4)
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(south * a[nlat - 1 - r - kk], col.sign * b[2 * r], c, 0, 0, 0);
+          acc[t] = c;
         }
       }
     }
+#pragma unroll
+    for (int t = 0; t < kGTiles; t++) {
+      const int tile = wave + kGroupWaves * t;
+      if (tile < MT * NT) {
+        const int mt = tile / NT, nt = tile % NT;
+        int c = nt * 16 + i;
+        if (c < NC) {
+          const int g = c / NCm;
+          c -= g * NCm;
+          const DevMember& G = mems[group_member(plan, grp, c0 + g)];
+          float* base = G.arena + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
+          const int p = uv ? c >> 2 : c >> 1, part = c & 1;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int k = mt * 16 + 4 * kk + r;
+            if (k < K) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next chunk restages the inputs
   }
 }
-
 
 // ------------------------------------------------- split-operand bodies
 
@@ -595,15 +703,18 @@ __device__ void leg_fwd_split(const DevMember& M, const LegJob& job, int m, Spli
       call(1);                  \
   } while (0)
 
-extern "C" __global__ __launch_bounds__(256) void k_leg_inv(const DevMember* __restrict__ mems, LaunchPlan plan,
-                                                             InvJobs jobs) {
-  __shared__ float As[kInvAs];
-  __shared__ float raw[kInvRaw];
+extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_leg_inv(const DevMember* __restrict__ mems,
+                                                                       GroupPlan plan, InvJobs jobs) {
+  __shared__ float As[2 * kInvAs];
+  __shared__ float raw[kInvRows * kRawK];
+  __shared__ const float* rowp[kLegGroupRows];
+  __shared__ float rowsub[kLegGroupRows];
   int tile;
-  const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
-  const int ntp1 = M.ntru + 1;
-  const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
-#define CALL(PP) leg_inv<PP>(M, job, tile % ntp1, As, raw)
+  const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
+  const DevMember& M = mems[group_member(plan, grp, 0)];
+  // wavenumber-major: the long workgroups (small m) of every job first
+  const InvJob job = jobs.j[job_of(jobs, tile % jobs.n)];
+#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile / jobs.n, As, raw, rowp, rowsub)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
@@ -620,15 +731,17 @@ extern "C" __global__ __launch_bounds__(256) void k_leg_inv_split(const DevMembe
 #undef CALL
 }
 
-extern "C" __global__ __launch_bounds__(256) void k_leg_fwd(const DevMember* __restrict__ mems, LaunchPlan plan,
-                                                             LegJobs jobs) {
-  __shared__ float Ah[kFwdAs];
-  __shared__ float raw[kFwdRaw];
+extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_leg_fwd(const DevMember* __restrict__ mems,
+                                                                       GroupPlan plan, LegJobs jobs) {
+  __shared__ float Ah[3 * kFwdAs];
+  __shared__ float raw[kFwdRows * kRawLat];
+  __shared__ const float* rowp[kLegGroupRows];
   int tile;
-  const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
-  const int ntp1 = M.ntru + 1;
-  const LegJob job = jobs.j[job_of(jobs, tile / ntp1)];
-#define CALL(PP) leg_fwd<PP>(M, job, tile % ntp1, Ah, raw)
+  const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
+  const DevMember& M = mems[group_member(plan, grp, 0)];
+  // wavenumber-major: the long workgroups (small m) of every job first
+  const LegJob job = jobs.j[job_of(jobs, tile % jobs.n)];
+#define CALL(PP) leg_fwd<PP>(mems, plan, grp, job, tile / jobs.n, Ah, raw, rowp)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
