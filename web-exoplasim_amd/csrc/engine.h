@@ -18,8 +18,8 @@
 #include "transforms.h"
 
 extern "C" __global__ void k_leg_inv(const DevMember*, GroupPlan, InvJobs);
-extern "C" __global__ void k_dft_inv(const DevMember*, LaunchPlan, InvJobs);
-extern "C" __global__ void k_dft_fwd(const DevMember*, LaunchPlan, FwdJobs);
+extern "C" __global__ void k_dft_inv(const DevMember*, GroupPlan, InvJobs);
+extern "C" __global__ void k_dft_fwd(const DevMember*, GroupPlan, FwdJobs);
 extern "C" __global__ void k_leg_fwd(const DevMember*, GroupPlan, LegJobs);
 extern "C" __global__ void k_leg_inv_split(const DevMember*, LaunchPlan, InvJobs);
 extern "C" __global__ void k_leg_fwd_split(const DevMember*, LaunchPlan, LegJobs);
@@ -150,22 +150,22 @@ class Stepper {
     return p;
   }
 
-  // Members of form `form` in groups of at most kLegGroup sharing
-  // transform tables and level count, in active order within a group;
-  // blocks(d) workgroups per group. The groups are dealt heaviest first
-  // (highest truncation, then most levels), so the long workgroups start
-  // first instead of trailing the launch.
-  template <typename F>
-  GroupPlan group_plan(int form, F blocks) const {
+  // The members include(d) selects, in groups of at most kLegGroup sharing
+  // transform tables and level count (active order within a group);
+  // blocks(d, count) workgroups per group. The groups are dealt heaviest
+  // first (highest truncation, then most levels), so the long workgroups
+  // start first instead of trailing the launch.
+  template <typename I, typename F>
+  GroupPlan group_plan(I include, F blocks) const {
     std::vector<std::vector<int>> groups;
     std::vector<char> taken(active_.size(), 0);
     for (size_t a = 0; a < active_.size(); a++) {
       const DevMember& d = e_.h_desc[active_[a]];
-      if (taken[a] || d.precision != form) continue;
+      if (taken[a] || !include(d)) continue;
       std::vector<int> g;
       for (size_t b = a; b < active_.size() && (int)g.size() < kLegGroup; b++) {
         const DevMember& o = e_.h_desc[active_[b]];
-        if (taken[b] || o.precision != form || o.ltab != d.ltab || o.nlev != d.nlev) continue;
+        if (taken[b] || !include(o) || o.ltab != d.ltab || o.nlev != d.nlev) continue;
         taken[b] = 1;
         g.push_back(active_[b]);
       }
@@ -182,11 +182,19 @@ class Stepper {
       p.count[p.n] = (int)g.size();
       for (int id : g) p.list[listed++] = id;
       p.first[p.n] = at;
-      at += blocks(e_.h_desc[g[0]]);
+      at += blocks(e_.h_desc[g[0]], (int)g.size());
       p.n++;
     }
     p.first[p.n] = at;
     return p;
+  }
+
+  // The zonal DFTs run in FP32 for both forms: every member, rows(d) rows
+  // per member, 128 rows per workgroup.
+  template <typename F>
+  GroupPlan dft_plan(F rows) const {
+    return group_plan([](const DevMember&) { return true; },
+                      [&](const DevMember& d, int count) { return (count * rows(d) + 127) / 128; });
   }
 
   void column(void (*k)(const DevMember*, LaunchPlan)) {
@@ -212,30 +220,33 @@ class Stepper {
   void inv(const InvJobs& jobs) {
     // the Legendre half per operand form: plain FP32 over member groups,
     // split FP16 per member
-    const GroupPlan pg = group_plan(0, [&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
+    const GroupPlan pg = group_plan([](const DevMember& d) { return d.precision == 0; },
+                                    [&](const DevMember& d, int) { return (d.ntru + 1) * jobs.n; });
     if (pg.first[pg.n] > 0) launch(e_, k_leg_inv, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
     const LaunchPlan pl = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
     if (pl.first[pl.n] > 0) launch(e_, k_leg_inv_split, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
-    // k_dft_inv: 64 rows (output x level x latitude) of one job per workgroup
-    const LaunchPlan pd = plan([&](const DevMember& d) {
-      int blocks = 0;
+    // k_dft_inv: 128 rows (member x job x output x level x latitude) of a
+    // group per workgroup, over every member of both forms
+    const GroupPlan pd = dft_plan([&](const DevMember& d) {
+      int rows = 0;
       for (int j = 0; j < jobs.n; j++)
-        blocks += ((jobs.j[j].kind == INV_PLAIN ? 1 : 2) * job_planes(jobs.j[j].per_level, d.nlev) * d.nlat + 63) / 64;
-      return blocks;
+        rows += (jobs.j[j].kind == INV_PLAIN ? 1 : 2) * job_planes(jobs.j[j].per_level, d.nlev) * d.nlat;
+      return rows;
     });
-    launch(e_, k_dft_inv, dim3(pd.first[pd.n]), dim3(256), e_.d_desc, pd, jobs);
+    launch(e_, k_dft_inv, dim3(pd.first[pd.n]), dim3(512), e_.d_desc, pd, jobs);
   }
   void fwd(const FwdJobs& jobs) {
-    // 64 rows (level x latitude) of one job per workgroup
-    const LaunchPlan p = plan([&](const DevMember& d) {
-      int blocks = 0;
-      for (int j = 0; j < jobs.n; j++) blocks += (job_planes(jobs.j[j].per_level, d.nlev) * d.nlat + 63) / 64;
-      return blocks;
+    // 128 rows (member x job x level x latitude) of a group per workgroup
+    const GroupPlan p = dft_plan([&](const DevMember& d) {
+      int rows = 0;
+      for (int j = 0; j < jobs.n; j++) rows += job_planes(jobs.j[j].per_level, d.nlev) * d.nlat;
+      return rows;
     });
-    launch(e_, k_dft_fwd, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
+    launch(e_, k_dft_fwd, dim3(p.first[p.n]), dim3(512), e_.d_desc, p, jobs);
   }
   void leg(const LegJobs& jobs) {
-    const GroupPlan pg = group_plan(0, [&](const DevMember& d) { return (d.ntru + 1) * jobs.n; });
+    const GroupPlan pg = group_plan([](const DevMember& d) { return d.precision == 0; },
+                                    [&](const DevMember& d, int) { return (d.ntru + 1) * jobs.n; });
     if (pg.first[pg.n] > 0) launch(e_, k_leg_fwd, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
     const LaunchPlan p = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
     if (p.first[p.n] > 0) launch(e_, k_leg_fwd_split, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);

@@ -5,31 +5,71 @@
 // FOURIER.tables builds, fourier/index.ts:19-36):
 //   fc2gp (fourier/wgsl/inverse.wgsl:4-18):  g[x] = sum_k F[k] inv[k][x]
 //   gp2fc (fourier/wgsl/forward.wgsl:4-18):  F[k] = sum_x g[x] fwd[x][k]
-// and d/dlambda (dlambda.wgsl:5-17) is fc2gp of (-m Im F, m Re F). Rows of
-// one job are contiguous in the arena (level-major planes of latitude
-// rows), so a job is one [rows x K] x [K x N] product with the table as
-// the right operand, shared by every row, job and member of a truncation.
+// and d/dlambda (dlambda.wgsl:5-17) is fc2gp of (-m Im F, m Re F); k = 2m
+// + part. Half a period on, wavenumber m's table entries change sign by
+// (-1)^m (inv[k][x + N/2] = (-1)^m inv[k][x]), so the inverse product is
+// split by the parity of m into two half-width ones: E[x] = sum over even
+// m, O[x] = sum over odd m (x < N/2); g[x] = E + O, g[x + N/2] = E - O:
+// the same terms as the full product, in another order, for half its
+// multiply-adds. (The forward product, split the same way, needs twice
+// the staged rows and ran slower; it stays whole.) Rows of one job are
+// contiguous in the arena (level-major
+// planes of latitude rows); the tables are the right operand, shared by
+// every row, job and member of a truncation.
 //
-// A workgroup takes 64 rows of one job (16 per wave) and all N columns;
-// the reduction runs in chunks of 32: the rows' chunk and the table's
-// chunk are staged in LDS, then each wave runs v_mfma_f32_16x16x4_f32
-// over its 16 rows and every 16-column tile.
+// A workgroup (8 waves) takes 128 rows. Rows run over the jobs of a launch
+// and the members of a group (members of one truncation and level count,
+// so one table): each row has its own source and destination pointers.
+// The reduction runs in chunks of 32: while the waves run
+// v_mfma_f32_16x16x4_f32 over their 16 rows and the column tiles of chunk
+// c from LDS, the loads of chunk c + 1 are in flight into registers.
 #include "transforms.h"
 
 namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kThreads = 256;
-constexpr int kRows = 64;   // rows per workgroup
+constexpr int kThreads = 512;
+constexpr int kRows = 128;  // rows per workgroup
 constexpr int kChunk = 32;  // reduction chunk
-constexpr int kMaxN = 192;  // NLON at T63 (inverse); NFC <= 128 (forward)
-constexpr int kMaxTilesN = kMaxN / 16;
+constexpr int kInvTiles = 96 / 16;  // inverse: N/2 <= 96 columns per half
+constexpr int kFwdTiles = 128 / 16;  // forward: NFC <= 128 columns
 
 __host__ __device__ constexpr int a_stride(int r) { return (r + 29) / 32 * 32 + 2; }
 __host__ __device__ constexpr int b_stride(int c) { return (c + 15) / 32 * 32 + 16; }
 constexpr int kAs = kRows * a_stride(kChunk);
-constexpr int kBs = kChunk * b_stride(kMaxN);
+constexpr int kInvBs = kChunk * b_stride(96);
+constexpr int kFwdBs = kChunk * b_stride(128);
+
+// Chunk staging through registers: element e = threadIdx.x + kThreads j is
+// row e >> LOGC, column e & (2^LOGC - 1).
+constexpr int kAN = kRows * kChunk / kThreads;  // A chunk: 128 rows x 32
+constexpr int kBN = kChunk * 128 / kThreads;    // B chunk: 32 x <= 128
+static_assert(kRows * kChunk % kThreads == 0 && kChunk * 128 % kThreads == 0, "chunk staging");
+
+template <int N, int LOGC, typename Load>
+__device__ __forceinline__ void load_regs(int rows, int cols, Load load, float* v) {
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int e = threadIdx.x + kThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
+    v[j] = (r < rows && c < cols) ? load(r, c) : 0.0f;
+  }
+}
+
+template <int N, int LOGC, typename Store>
+__device__ __forceinline__ void store_regs(int rows, int cols, Store store, const float* v) {
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int e = threadIdx.x + kThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
+    if (r < rows && c < cols) store(r, c, v[j]);
+  }
+}
+
+// Coefficient index k of the j-th coefficient of the wavenumbers of one
+// parity (0: m = 0, 2, ..; 1: m = 1, 3, ..).
+__device__ __forceinline__ int parity_k(int parity, int j) { return 4 * (j >> 1) + 2 * parity + (j & 1); }
+// Coefficients of one parity: 2 x the number of such m <= T.
+__device__ __forceinline__ int parity_count(int parity, int ntp1) { return 2 * ((ntp1 + 1 - parity) / 2); }
 
 // Rows of an inverse job: outputs x planes x latitudes.
 __device__ __forceinline__ int inv_rows(const DevMember& M, const InvJob& job) {
@@ -39,173 +79,221 @@ __device__ __forceinline__ int fwd_rows(const DevMember& M, const FwdJob& job) {
   return (job.per_level ? M.nlev : 1) * M.nlat;
 }
 
-// The job and the row block of workgroup `tile` of a member.
+// Row `row` of a group's launch: member g = row / (rows per member), then
+// the job and the row within it.
 template <typename Jobs, typename Rows>
-__device__ __forceinline__ int find_block(const Jobs& jobs, int tile, Rows rows, int* block) {
-  int j = 0;
+__device__ __forceinline__ int locate(const Jobs& jobs, int per_member, int row, Rows rows, int* g, int* r) {
+  *g = row / per_member;
+  int x = row - *g * per_member, j = 0;
   for (; j + 1 < jobs.n; j++) {
-    const int blocks = (rows(jobs.j[j]) + kRows - 1) / kRows;
-    if (tile < blocks) break;
-    tile -= blocks;
+    const int n = rows(jobs.j[j]);
+    if (x < n) break;
+    x -= n;
   }
-  *block = tile;
+  *r = x;
   return j;
 }
 
-// acc[nt] += A[rows 16 wave .. +16][0..R) x B[0..R)[16 nt .. +16], nt < NT
-__device__ __forceinline__ void mfma_rows(const float* As, const float* Bs, int bs, int R, int NT, f4* acc) {
+// acc[nt] += A[rows 16 wave .. +16][0..32) x B[0..32)[16 nt .. +16], nt < NT
+template <int MAXT>
+__device__ __forceinline__ void mfma_rows(const float* As, const float* Bs, int bs, int NT, f4* acc) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
   const float* a = As + (16 * wave + i) * a_stride(kChunk) + kk;
   const float* b = Bs + kk * bs + i;
-  for (int r = 0; r < R; r += 4) {
+#pragma unroll 2
+  for (int r = 0; r < kChunk; r += 4) {
     const float av = a[r];
 #pragma unroll
-    for (int nt = 0; nt < kMaxTilesN; nt++)
+    for (int nt = 0; nt < MAXT; nt++)
       if (nt < NT) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b[r * bs + 16 * nt], acc[nt], 0, 0, 0);
-  }
-}
-
-// Rows x columns of a chunk staged with lanes along the columns (32-wide
-// chunks take two rows per wave at a time), each lane with 8 loads in
-// flight before it stores.
-template <typename Load, typename Store>
-__device__ __forceinline__ void stage(int rows, int cols, Load load, Store store) {
-  const bool narrow = cols <= 32;
-  const int lane = threadIdx.x & 63;
-  const int r0 = narrow ? (threadIdx.x >> 5) : (threadIdx.x >> 6);
-  const int rstep = narrow ? 8 : 4;
-  for (int c = narrow ? (lane & 31) : lane; c < cols; c += 64) {
-    for (int base = r0; base < rows; base += 8 * rstep) {
-      float v[8];
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int r = base + rstep * j;
-        v[j] = r < rows ? load(r, c) : 0.0f;
-      }
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int r = base + rstep * j;
-        if (r < rows) store(r, c, v[j]);
-      }
-    }
   }
 }
 
 }  // namespace
 
-extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember* __restrict__ mems, LaunchPlan plan,
+extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember* __restrict__ mems, GroupPlan plan,
                                                                   InvJobs jobs) {
   __shared__ float As[kAs];
-  __shared__ float Bs[kBs];
-  // per row of the block: arena offset of its Fourier row (-1 past the
-  // job's rows), whether it is d/dlambda's, and its grid row's offset
-  __shared__ int src_at[kRows], dst_at[kRows], derived[kRows];
-  int tile, block;
-  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int j = find_block(jobs, tile, [&](const InvJob& x) { return inv_rows(M, x); }, &block);
-  const InvJob& job = jobs.j[j];
-  const int nlat = M.nlat, nlon = M.nlon, nfc = M.nfc;
-  const int plane_rows = (job.per_level ? M.nlev : 1) * nlat;  // rows of one output
-  const int rows = inv_rows(M, job);
+  __shared__ float Bs[kInvBs];
+  // per row of the block: its Fourier row (null past the group's rows),
+  // whether it is d/dlambda's, and its grid row
+  __shared__ const float* src[kRows];
+  __shared__ float* dst[kRows];
+  __shared__ int derived[kRows];
+  int block;
+  const int grp = group_find(plan, blockIdx.x, &block);
+  const DevMember& M = mems[plan.list[plan.start[grp]]];
+  const int nlat = M.nlat, nlon = M.nlon, nfc = M.nfc, ntp1 = M.ntru + 1;
+  const auto rows_of = [&](const InvJob& x) { return inv_rows(M, x); };
+  int per_member = 0;
+  for (int j = 0; j < jobs.n; j++) per_member += rows_of(jobs.j[j]);
+  const int total = plan.count[grp] * per_member;
   const int row0 = block * kRows;
-  const int K = nfc;  // a multiple of 4 at T21/T42/T63 (44, 86 -> 88 padded, 128)
-  const int KP = (K + 3) & ~3;
-  const int NT = nlon / 16;
-  const int bs = b_stride(nlon);
+  const int half = nlon / 2, NT = half / 16;
+  const int bs = b_stride(half);
+  const int KE = parity_count(0, ntp1), KO = parity_count(1, ntp1);
+  const int CE = (KE + kChunk - 1) / kChunk, chunks = CE + (KO + kChunk - 1) / kChunk;
   const float* inv = M.ltab + M.tabo[TAB_fourier_inverse];
-  f4 acc[kMaxTilesN];
-#pragma unroll
-  for (int nt = 0; nt < kMaxTilesN; nt++) acc[nt] = f4{0, 0, 0, 0};
   if (threadIdx.x < kRows) {
     const int row = row0 + threadIdx.x;
-    const int o = row / plane_rows, pr = row % plane_rows;
-    const int src = (job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0;
-    src_at[threadIdx.x] = row < rows ? M.slot[src] + pr * nfc : -1;
-    dst_at[threadIdx.x] = row < rows ? M.slot[o == 0 ? job.dst0 : job.dst1] + pr * nlon : -1;
-    derived[threadIdx.x] = job.kind == INV_DLAMBDA && o == 1;
+    const float* s = nullptr;
+    float* d = nullptr;
+    int dl = 0;
+    if (row < total) {
+      int g, r;
+      const InvJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
+      const DevMember& G = mems[plan.list[plan.start[grp] + g]];
+      const int plane_rows = (job.per_level ? M.nlev : 1) * nlat;  // rows of one output
+      const int o = r / plane_rows, pr = r - o * plane_rows;
+      s = G.arena + G.slot[(job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0] + pr * nfc;
+      d = G.arena + G.slot[o == 0 ? job.dst0 : job.dst1] + pr * nlon;
+      dl = job.kind == INV_DLAMBDA && o == 1;
+    }
+    src[threadIdx.x] = s;
+    dst[threadIdx.x] = d;
+    derived[threadIdx.x] = dl;
   }
   __syncthreads();
-  for (int k0 = 0; k0 < KP; k0 += kChunk) {
-    const int kc = KP - k0 < kChunk ? KP - k0 : kChunk;
-    // A: the block's Fourier rows (d/dlambda's built from its row)
-    stage(
-        kRows, kc,
-        [&](int r, int c) {
-          const int k = k0 + c, at = src_at[r];
-          if (at < 0 || k >= K) return 0.0f;
-          const float* F = M.arena + at;
-          if (derived[r]) {
-            const float m = (float)(k >> 1);
-            return (k & 1) ? m * F[k - 1] : -m * F[k + 1];
-          }
-          return F[k];
-        },
-        [&](int r, int c, float v) { As[r * a_stride(kChunk) + c] = v; });
-    stage(
-        kc, nlon, [&](int c, int x) { return k0 + c < K ? inv[(k0 + c) * nlon + x] : 0.0f; },
-        [&](int c, int x, float v) { Bs[c * bs + x] = v; });
+  // chunk c: parity c >= CE, coefficients j0 .. j0 + 32 of that parity
+  const auto a_load = [&](int c) {
+    const int parity = c >= CE, j0 = kChunk * (parity ? c - CE : c), n = parity ? KO : KE;
+    return [=](int r, int x) {
+      const int j = j0 + x;
+      const float* F = src[r];
+      if (!F || j >= n) return 0.0f;
+      const int k = parity_k(parity, j);
+      if (derived[r]) {
+        const float m = (float)(k >> 1);
+        return (k & 1) ? m * F[k - 1] : -m * F[k + 1];
+      }
+      return F[k];
+    };
+  };
+  const auto b_load = [&](int c) {
+    const int parity = c >= CE, j0 = kChunk * (parity ? c - CE : c), n = parity ? KO : KE;
+    return [=](int y, int x) { return j0 + y < n ? inv[parity_k(parity, j0 + y) * nlon + x] : 0.0f; };
+  };
+  f4 even[kInvTiles], odd[kInvTiles];
+#pragma unroll
+  for (int nt = 0; nt < kInvTiles; nt++) even[nt] = odd[nt] = f4{0, 0, 0, 0};
+  float va[kAN], vb[kBN];
+  load_regs<kAN, 5>(kRows, kChunk, a_load(0), va);
+  load_regs<kBN, 7>(kChunk, half, b_load(0), vb);
+  for (int c = 0; c < chunks; c++) {
+    store_regs<kAN, 5>(kRows, kChunk, [&](int r, int x, float v) { As[r * a_stride(kChunk) + x] = v; }, va);
+    store_regs<kBN, 7>(kChunk, half, [&](int y, int x, float v) { Bs[y * bs + x] = v; }, vb);
     __syncthreads();
-    mfma_rows(As, Bs, bs, kc, NT, acc);
+    if (c + 1 < chunks) {
+      load_regs<kAN, 5>(kRows, kChunk, a_load(c + 1), va);
+      load_regs<kBN, 7>(kChunk, half, b_load(c + 1), vb);
+    }
+    mfma_rows<kInvTiles>(As, Bs, bs, NT, c < CE ? even : odd);
     __syncthreads();
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int at = dst_at[16 * wave + 4 * (lane >> 4) + r];
-    if (at >= 0) {
-      float* G = M.arena + at;
+    float* G = dst[16 * wave + 4 * (lane >> 4) + r];
+    if (G) {
 #pragma unroll
-      for (int nt = 0; nt < kMaxTilesN; nt++)
-        if (nt < NT) G[16 * nt + (lane & 15)] = acc[nt][r];
+      for (int nt = 0; nt < kInvTiles; nt++) {
+        if (nt < NT) {
+          const int x = 16 * nt + (lane & 15);
+          G[x] = even[nt][r] + odd[nt][r];
+          G[x + half] = even[nt][r] - odd[nt][r];
+        }
+      }
     }
   }
 }
 
-extern "C" __global__ __launch_bounds__(kThreads) void k_dft_fwd(const DevMember* __restrict__ mems, LaunchPlan plan,
+extern "C" __global__ __launch_bounds__(kThreads) void k_dft_fwd(const DevMember* __restrict__ mems, GroupPlan plan,
                                                                   FwdJobs jobs) {
-  __shared__ float As[kAs];
-  __shared__ float Bs[kBs];
-  int tile, block;
-  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int j = find_block(jobs, tile, [&](const FwdJob& x) { return fwd_rows(M, x); }, &block);
-  const FwdJob& job = jobs.j[j];
-  const int nlon = M.nlon, nfc = M.nfc;
-  const int rows = fwd_rows(M, job);
+  // A: the half-row sums (even m) and differences (odd m); B: the table's
+  // coefficients of even m in columns 0..63 and of odd m in 64..127
+  __shared__ float As[2 * kAs];
+  __shared__ float Bs[kFwdBs];
+  __shared__ const float* src[kRows];
+  __shared__ float* dst[kRows];
+  int block;
+  const int grp = group_find(plan, blockIdx.x, &block);
+  const DevMember& M = mems[plan.list[plan.start[grp]]];
+  const int nlon = M.nlon, nfc = M.nfc, ntp1 = M.ntru + 1;
+  const auto rows_of = [&](const FwdJob& x) { return fwd_rows(M, x); };
+  int per_member = 0;
+  for (int j = 0; j < jobs.n; j++) per_member += rows_of(jobs.j[j]);
+  const int total = plan.count[grp] * per_member;
   const int row0 = block * kRows;
-  const int K = nlon;
-  const int NT = (nfc + 15) / 16;
-  const int bs = b_stride(NT * 16);
+  const int half = nlon / 2;  // a multiple of 32
+  const int KE = parity_count(0, ntp1), KO = parity_count(1, ntp1);
+  const int NTE = (KE + 15) / 16, NTO = (KO + 15) / 16;
+  constexpr int bs = b_stride(128);
   const float* fwd = M.ltab + M.tabo[TAB_fourier_forward];
-  const float* G = M.arena + M.slot[job.src];
-  f4 acc[kMaxTilesN];
+  if (threadIdx.x < kRows) {
+    const int row = row0 + threadIdx.x;
+    const float* s = nullptr;
+    float* d = nullptr;
+    if (row < total) {
+      int g, r;
+      const FwdJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
+      const DevMember& G = mems[plan.list[plan.start[grp] + g]];
+      s = G.arena + G.slot[job.src] + r * nlon;
+      d = G.arena + G.slot[job.dst] + r * nfc;
+    }
+    src[threadIdx.x] = s;
+    dst[threadIdx.x] = d;
+  }
+  __syncthreads();
+  const auto g_load = [&](int x0, int shift) {
+    return [=](int r, int x) {
+      const float* g = src[r];
+      return g ? g[x0 + shift + x] : 0.0f;
+    };
+  };
+  const auto b_load = [&](int x0) {
+    return [=](int y, int j) {
+      const int parity = j >> 6, jj = j & 63;
+      return jj < (parity ? KO : KE) ? fwd[(x0 + y) * nfc + parity_k(parity, jj)] : 0.0f;
+    };
+  };
+  f4 even[kFwdTiles / 2], odd[kFwdTiles / 2];
 #pragma unroll
-  for (int nt = 0; nt < kMaxTilesN; nt++) acc[nt] = f4{0, 0, 0, 0};
-  for (int x0 = 0; x0 < K; x0 += kChunk) {
-    stage(
-        kRows, kChunk,
-        [&](int r, int c) {
-          const int row = row0 + r;
-          return row < rows ? G[row * nlon + x0 + c] : 0.0f;
-        },
-        [&](int r, int c, float v) { As[r * a_stride(kChunk) + c] = v; });
-    stage(
-        kChunk, NT * 16, [&](int c, int k) { return k < nfc ? fwd[(x0 + c) * nfc + k] : 0.0f; },
-        [&](int c, int k, float v) { Bs[c * bs + k] = v; });
+  for (int nt = 0; nt < kFwdTiles / 2; nt++) even[nt] = odd[nt] = f4{0, 0, 0, 0};
+  float va[kAN], vc[kAN], vb[kBN];
+  load_regs<kAN, 5>(kRows, kChunk, g_load(0, 0), va);
+  load_regs<kAN, 5>(kRows, kChunk, g_load(0, half), vc);
+  load_regs<kBN, 7>(kChunk, 128, b_load(0), vb);
+  for (int x0 = 0; x0 < half; x0 += kChunk) {
+#pragma unroll
+    for (int j = 0; j < kAN; j++) {
+      const float a = va[j], c = vc[j];
+      va[j] = a + c;
+      vc[j] = a - c;
+    }
+    store_regs<kAN, 5>(kRows, kChunk, [&](int r, int x, float v) { As[r * a_stride(kChunk) + x] = v; }, va);
+    store_regs<kAN, 5>(kRows, kChunk, [&](int r, int x, float v) { As[kAs + r * a_stride(kChunk) + x] = v; }, vc);
+    store_regs<kBN, 7>(kChunk, 128, [&](int y, int j, float v) { Bs[y * bs + j] = v; }, vb);
     __syncthreads();
-    mfma_rows(As, Bs, bs, kChunk, NT, acc);
+    if (x0 + kChunk < half) {
+      load_regs<kAN, 5>(kRows, kChunk, g_load(x0 + kChunk, 0), va);
+      load_regs<kAN, 5>(kRows, kChunk, g_load(x0 + kChunk, half), vc);
+      load_regs<kBN, 7>(kChunk, 128, b_load(x0 + kChunk), vb);
+    }
+    mfma_rows<kFwdTiles / 2>(As, Bs, bs, NTE, even);
+    mfma_rows<kFwdTiles / 2>(As + kAs, Bs + 64, bs, NTO, odd);
     __syncthreads();
   }
-  float* F = M.arena + M.slot[job.dst];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int row = row0 + 16 * wave + 4 * (lane >> 4) + r;
-    if (row < rows) {
+    float* F = dst[16 * wave + 4 * (lane >> 4) + r];
+    if (F) {
 #pragma unroll
-      for (int nt = 0; nt < kMaxTilesN; nt++) {
-        const int k = 16 * nt + (lane & 15);
-        if (nt < NT && k < nfc) F[row * nfc + k] = acc[nt][r];
+      for (int nt = 0; nt < kFwdTiles / 2; nt++) {
+        const int j = 16 * nt + (lane & 15);
+        if (nt < NTE && j < KE) F[parity_k(0, j)] = even[nt][r];
+        if (nt < NTO && j < KO) F[parity_k(1, j)] = odd[nt][r];
       }
     }
   }
