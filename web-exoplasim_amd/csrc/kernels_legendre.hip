@@ -712,9 +712,11 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
-  // wavenumber-major: the long workgroups (small m) of every job first
-  const InvJob job = jobs.j[job_of(jobs, tile % jobs.n)];
-#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile / jobs.n, As, raw, rowp, rowsub)
+  // job-major: neighbouring wavenumbers of one job, whose Fourier rows
+  // share cache lines, go to one XCD (xcd_block); m = 0 (longest) first
+  const int ntp1 = M.ntru + 1;
+  const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
+#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile % ntp1, As, raw, rowp, rowsub)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
@@ -739,9 +741,11 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
-  // wavenumber-major: the long workgroups (small m) of every job first
-  const LegJob job = jobs.j[job_of(jobs, tile % jobs.n)];
-#define CALL(PP) leg_fwd<PP>(mems, plan, grp, job, tile / jobs.n, Ah, raw, rowp)
+  // job-major: neighbouring wavenumbers of one job, whose Fourier rows
+  // share cache lines, go to one XCD (xcd_block); m = 0 (longest) first
+  const int ntp1 = M.ntru + 1;
+  const LegJob job = jobs.j[job_of(jobs, tile / ntp1)];
+#define CALL(PP) leg_fwd<PP>(mems, plan, grp, job, tile % ntp1, Ah, raw, rowp)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
