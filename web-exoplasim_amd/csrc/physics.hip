@@ -40,6 +40,23 @@ __device__ __forceinline__ float powi(float x, int n) {
   return r;
 }
 
+// Logarithms and powers of positive, normal arguments on the hardware's
+// base-2 units (v_log_f32, v_exp_f32; about 1 ulp each) instead of the
+// library's correctly rounded routines, whose argument reduction (with a
+// full-precision division) costs several times as much. Used where the
+// routine's arguments are positive and normal (the longwave absorptivities'
+// column amounts + offsets, the shortwave's ozone and vapour paths and
+// cloud optical depths, the surface layer's roughness ratio and stability
+// terms); the results differ from the library's by a few ulp. Exponentials
+// of large arguments keep the library's expf, whose argument reduction the
+// plain exp2(x log2 e) would lose.
+__device__ __forceinline__ float log2_pos(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float pow_pos(float a, float b) { return __builtin_amdgcn_exp2f(b * log2_pos(a)); }
+__device__ __forceinline__ float log10_pos(float x) { return log2_pos(x) * 0.30102999566398120f; }
+__device__ __forceinline__ float log_pos(float x) { return log2_pos(x) * 0.69314718055994531f; }
+// e^x for x <= 0 down to the normal range: exp2(x log2 e)
+__device__ __forceinline__ float exp_neg(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
 // --------------------------------------------------------------- fixer
 
 template <int NL>
@@ -104,7 +121,7 @@ __device__ __forceinline__ void surflx(const DevMember& M, int h) {
   const float znl = -GASCON * 0.5f * (t + dtsa) * zlnsig / GA;
   const float zbz0 = znl / F(dz0)[h];
   const float zri = GA * znl / (zabsu2 * dtsa) * (dtsa - ts * (1.0f + moisture * qs));
-  const float zkblnz = VK / logf(zbz0 + 1.0f);
+  const float zkblnz = VK / log_pos(zbz0 + 1.0f);
   const float zkblnz2 = zkblnz * zkblnz;
   float zrifm, zrifh;
   if (zri <= 0) {
@@ -112,7 +129,8 @@ __device__ __forceinline__ void surflx(const DevMember& M, int h) {
     zrifm = 1.0f - 2.0f * K(vdiff_b) * zri / zdenom;
     if (dls < 1.0f) {
       const float zdth = -(dtsa - ts * (1.0f + moisture * qs));
-      zrifh = powf(1.0f + powf(0.0016f * powf(zdth, 1.0f / 3.0f) / sqrtf(zabsu2) / zkblnz2, 1.25f), 0.8f);
+      // zdth >= 0 here; pow_pos(0, 1/3) = exp2(-inf) = 0
+      zrifh = pow_pos(1.0f + pow_pos(0.0016f * pow_pos(zdth, 1.0f / 3.0f) / sqrtf(zabsu2) / zkblnz2, 1.25f), 0.8f);
     } else {
       zrifh = 1.0f - 3.0f * K(vdiff_b) * zri / zdenom;
     }
@@ -325,13 +343,13 @@ __device__ __forceinline__ void mko3(const DevMember& M, int h) {
 }
 
 __device__ __forceinline__ float ozone_absorption(const DevMember& M, float o3) {
-  return (0.02118f * o3 / (1.0f + 0.042f * o3 + 0.000323f * (o3 * o3)) + 1.082f * o3 / powf(1.0f + 138.6f * o3, 0.805f) +
+  return (0.02118f * o3 / (1.0f + 0.042f * o3 + 0.000323f * (o3 * o3)) + 1.082f * o3 / pow_pos(1.0f + 138.6f * o3, 0.805f) +
           0.0658f * o3 / (1.0f + (103.6f * o3) * (103.6f * o3) * (103.6f * o3))) /
          K(zsolar1);
 }
 
 __device__ __forceinline__ float vapour_absorption(const DevMember& M, float wv) {
-  return 2.9f * wv / (powf(1.0f + 141.5f * wv, 0.635f) + 5.925f * wv) / K(zsolar2);
+  return 2.9f * wv / (pow_pos(1.0f + 141.5f * wv, 0.635f) + 5.925f * wv) / K(zsolar2);
 }
 
 template <int NL>
@@ -339,7 +357,7 @@ __device__ __forceinline__ void swr(const DevMember& M, int h) {
   // physics/radiation/wgsl/swr.wgsl:45-276
   const int NH = M.nhor;
   const float ZERO = 1.0e-6f, ZBETTA = 1.66f, ZMBAR = 1.9f, ZFO3 = 100.0f / 2.14f, ZMU00 = 0.5f;
-  const float INV_LN10 = 0.43429448190325176f, GA = K(ga);
+  const float GA = K(ga);
   const float mu0 = F(gmu0)[h];
   const float gdist2 = F(radiation_step)[3];
   const float ftop1 = K(zsolar1) * K(gsol0) * gdist2 * mu0;
@@ -363,9 +381,9 @@ __device__ __forceinline__ void swr(const DevMember& M, int h) {
     tcl2[l] = tcl2s[l] = 1;
     if (cc[l] > 0) {
       const float lwp = fminf(1000.0f, 1000.0f * LV(dql, l, h) * dp / GA * T(dsigma)[l]);
-      const float tau = 2.0f * powf(logf(lwp + 1.5f) * INV_LN10, 3.9f);
-      const float zlog = logf(1000.0f / tau);
-      float b2 = K(zb4) / logf(3.0f + 0.1f * tau);
+      const float tau = 2.0f * pow_pos(log10_pos(lwp + 1.5f), 3.9f);
+      const float zlog = log_pos(1000.0f / tau);
+      float b2 = K(zb4) / log_pos(3.0f + 0.1f * tau);
       float om0 = fminf(0.9999f, 1.0f - K(zb5) * zlog);
       float un = 1.0f - om0;
       float uz = un + 2.0f * b2 * om0;
@@ -376,7 +394,7 @@ __device__ __forceinline__ void swr(const DevMember& M, int h) {
       tcl2s[l] = 4.0f * u / r;
       rcl2s[l] = (u * u - 1.0f) / r * (ex - 1.0f / ex);
       const float b1 = K(tswr1) * sqrtf(mu0);
-      b2 = K(tswr2) * sqrtf(mu0) / logf(3.0f + 0.1f * tau);
+      b2 = K(tswr2) * sqrtf(mu0) / log_pos(3.0f + 0.1f * tau);
       om0 = fminf(0.9999f, 1.0f - K(tswr3) * mu0 * mu0 * zlog);
       un = 1.0f - om0;
       uz = un + 2.0f * b2 * om0;
@@ -556,27 +574,28 @@ __device__ __forceinline__ void lwr(const DevMember& M, int h) {
       sumwv = sumwv + q[k];
       sumo3 = sumo3 + qo3[k];
       sumco2 = sumco2 + qco2[k];
+      // the column amounts are >= 0, so every argument below is >= 3.59e-5
       float ah2o;
       if (sumwv <= 0.01f)
-        ah2o = 0.846f * powf(sumwv + 3.59e-5f, 0.243f) - K(zh2o0a);
+        ah2o = 0.846f * pow_pos(sumwv + 3.59e-5f, 0.243f) - K(zh2o0a);
       else
-        ah2o = 0.24f * log10f(sumwv + 0.01f) + K(zah2oc);
+        ah2o = 0.24f * log10_pos(sumwv + 0.01f) + K(zah2oc);
       if (K(th2oc) > 0) ah2o = fminf(ah2o + (1.0f - expf(-K(th2oc) * sumwv)), 1.0f);
       float aco2;
       if (sumco2 <= 1.0f)
-        aco2 = 0.0676f * powf(sumco2 + 0.01022f, 0.421f) - K(zco20);
+        aco2 = 0.0676f * pow_pos(sumco2 + 0.01022f, 0.421f) - K(zco20);
       else
-        aco2 = 0.0546f * log10f(sumco2) + K(zaco2c);
+        aco2 = 0.0546f * log10_pos(sumco2) + K(zaco2c);
       float th2o;
       if (sumwv <= 2.0f)
-        th2o = 1.0f - (0.832f * powf(sumwv + 0.0286f, 0.26f) - K(zh2o0));
+        th2o = 1.0f - (0.832f * pow_pos(sumwv + 0.0286f, 0.26f) - K(zh2o0));
       else
-        th2o = fmaxf(0.0f, K(zth2oc) - 0.1196f * logf(sumwv - 0.6931f));
+        th2o = fmaxf(0.0f, K(zth2oc) - 0.1196f * log_pos(sumwv - 0.6931f));
       float ao3;
       if (sumo3 <= 0.01f)
-        ao3 = 0.209f * powf(sumo3 + 7.0e-5f, 0.436f) - K(zao30);
+        ao3 = 0.209f * pow_pos(sumo3 + 7.0e-5f, 0.436f) - K(zao30);
       else
-        ao3 = 0.0212f * log10f(sumo3) + K(zao3c);
+        ao3 = 0.0212f * log10_pos(sumo3) + K(zao3c);
       const float cs = fminf(1.0f - ZERO, fmaxf(ZERO, 1.0f - ah2o - ao3 - aco2 * th2o));
       taucs[k] = cs;
       taucc = taucc * taucc0[k];
