@@ -7,7 +7,7 @@
 // each part run at a higher occupancy than their union:
 //
 //   k_fixer_column   fixer's column pass (physics/fixer/wgsl/column.wgsl)
-//   k_fixer_reduce   latitude rows (one wave per row) then the global sum,
+//   k_fixer_reduce   latitude rows (one wave per row, 16 waves) then the global sum,
 //                    one workgroup per member (fixer/wgsl/row.wgsl,
 //                    reduction/wgsl/sum.wgsl)
 //   k_surface_fluxes fixer apply, surflx, vdiff
@@ -180,18 +180,21 @@ __device__ __forceinline__ void surflx(const DevMember& M, int h) {
   F(dlhdt)[h] = dlhfl == 0.0f ? 0.0f : dlhdt;
 }
 
+// The elimination divides each row by its pivot once (one reciprocal, then
+// products) where the reference divides twice; the results agree to an ulp.
 template <int NL>
 __device__ __forceinline__ void diffuse(const DevMember& M, float* x, const float* k) {
   // physics/diffusion/wgsl/implicit.wgsl:4-26
   float ebs[NL];
   const float* ds = T(dsigma);
-  ebs[0] = k[0] / (ds[0] + k[0]);
-  x[0] = ds[0] * x[0] / (ds[0] + k[0]);
+  float r = 1.0f / (ds[0] + k[0]);
+  ebs[0] = k[0] * r;
+  x[0] = ds[0] * x[0] * r;
 #pragma unroll
   for (int l = 1; l < NL - 1; l++) {
-    const float den = ds[l] + k[l] + k[l - 1] * (1.0f - ebs[l - 1]);
-    ebs[l] = k[l] / den;
-    x[l] = (x[l] * ds[l] + k[l - 1] * x[l - 1]) / den;
+    r = 1.0f / (ds[l] + k[l] + k[l - 1] * (1.0f - ebs[l - 1]));
+    ebs[l] = k[l] * r;
+    x[l] = (x[l] * ds[l] + k[l - 1] * x[l - 1]) * r;
   }
   x[NL - 1] = (x[NL - 1] * ds[NL - 1] + k[NL - 2] * x[NL - 2]) / (ds[NL - 1] + k[NL - 2] * (1.0f - ebs[NL - 2]));
 #pragma unroll
@@ -201,21 +204,24 @@ __device__ __forceinline__ void diffuse(const DevMember& M, float* x, const floa
 template <int NL>
 __device__ __forceinline__ void diffuse_potential(const DevMember& M, float* x, const float* k, const float* skap) {
   // physics/diffusion/wgsl/implicit.wgsl:28-47
-  float ebs[NL];
+  float ebs[NL], rs[NL];
   const float* ds = T(dsigma);
-  ebs[0] = k[0] / (ds[0] + k[0] / skap[0]);
-  x[0] = ds[0] * x[0] / (ds[0] + k[0] / skap[0]);
+#pragma unroll
+  for (int l = 0; l < NL; l++) rs[l] = 1.0f / skap[l];
+  float r = 1.0f / (ds[0] + k[0] * rs[0]);
+  ebs[0] = k[0] * r;
+  x[0] = ds[0] * x[0] * r;
 #pragma unroll
   for (int l = 1; l < NL - 1; l++) {
-    const float den = ds[l] + (k[l] + k[l - 1] * (1.0f - ebs[l - 1] / skap[l - 1])) / skap[l];
-    ebs[l] = k[l] / den;
-    x[l] = (x[l] * ds[l] + k[l - 1] / skap[l - 1] * x[l - 1]) / den;
+    r = 1.0f / (ds[l] + (k[l] + k[l - 1] * (1.0f - ebs[l - 1] * rs[l - 1])) * rs[l]);
+    ebs[l] = k[l] * r;
+    x[l] = (x[l] * ds[l] + k[l - 1] * rs[l - 1] * x[l - 1]) * r;
   }
   const int last = NL - 1;
-  x[last] = (x[last] * ds[last] + k[NL - 2] * x[NL - 2] / skap[NL - 2]) /
-            (ds[last] + k[NL - 2] / skap[last] * (1.0f - ebs[NL - 2] / skap[NL - 2]));
+  x[last] = (x[last] * ds[last] + k[NL - 2] * x[NL - 2] * rs[NL - 2]) /
+            (ds[last] + k[NL - 2] * rs[last] * (1.0f - ebs[NL - 2] * rs[NL - 2]));
 #pragma unroll
-  for (int l = NL - 2; l >= 0; l--) x[l] = x[l] + ebs[l] * x[l + 1] / skap[l + 1];
+  for (int l = NL - 2; l >= 0; l--) x[l] = x[l] + ebs[l] * x[l + 1] * rs[l + 1];
 }
 
 template <int NL>
@@ -1775,17 +1781,20 @@ __device__ __forceinline__ void physics_d(const DevMember& M, int h) {
 // Deterministic sum of two grid fields of one member by one workgroup:
 // each lane sums a strided share in order, then a fixed tree
 // (reduction/wgsl/sum.wgsl:12-34).
+// The planet-wide reductions run one workgroup of kReduceThreads per member.
+constexpr int kReduceThreads = 1024;
+
 __device__ __forceinline__ void block_sum2(const DevMember& M, const float* a, const float* b, float* out) {
-  __shared__ float sa[256], sb[256];
+  __shared__ float sa[kReduceThreads], sb[kReduceThreads];
   float x = 0, y = 0;
-  for (int h = threadIdx.x; h < M.nhor; h += 256) {
+  for (int h = threadIdx.x; h < M.nhor; h += kReduceThreads) {
     x += a[h];
     y += b[h];
   }
   sa[threadIdx.x] = x;
   sb[threadIdx.x] = y;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       sa[threadIdx.x] += sa[threadIdx.x + s];
       sb[threadIdx.x] += sb[threadIdx.x + s];
@@ -1842,8 +1851,8 @@ extern "C" __global__ __launch_bounds__(64) void k_mkradv(const DevMember* __res
 
 // fixer's latitude pass (fixer/wgsl/row.wgsl:4-33), then its global sum:
 // one workgroup per member.
-extern "C" __global__ __launch_bounds__(256) void k_fixer_reduce(const DevMember* __restrict__ mems,
-                                                                  LaunchPlan plan) {
+extern "C" __global__ __launch_bounds__(kReduceThreads) void k_fixer_reduce(const DevMember* __restrict__ mems,
+                                                                             LaunchPlan plan) {
   const DevMember& M = mems[plan.member[blockIdx.x]];
   const int NH = M.nhor, NLON = M.nlon, L = M.nlev;
   float* fixq = M.arena + M.slot[SLOT_fixq];
@@ -1852,7 +1861,7 @@ extern "C" __global__ __launch_bounds__(256) void k_fixer_reduce(const DevMember
   // one wave per latitude row: lanes sum strided shares, then a fixed
   // butterfly (the same order on every run)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int row = wave; row < M.nlat; row += 4) {
+  for (int row = wave; row < M.nlat; row += kReduceThreads / 64) {
     const int first = row * NLON;
     float zneg = 0, zpos = 0;
     for (int j = lane; j < NLON; j += 64) {
@@ -1877,15 +1886,16 @@ extern "C" __global__ __launch_bounds__(256) void k_fixer_reduce(const DevMember
 }
 
 // mkdca-limit.wgsl:5-22: dca_skip = 1 when any column gave up.
-extern "C" __global__ __launch_bounds__(256) void k_dca_limit(const DevMember* __restrict__ mems, LaunchPlan plan) {
+extern "C" __global__ __launch_bounds__(kReduceThreads) void k_dca_limit(const DevMember* __restrict__ mems,
+                                                                          LaunchPlan plan) {
   const DevMember& M = mems[plan.member[blockIdx.x]];
-  __shared__ float part[256];
+  __shared__ float part[kReduceThreads];
   const float* lim = M.arena + M.slot[SLOT_dca_limited];
   float flag = 0;
-  for (int h = threadIdx.x; h < M.nhor; h += 256) flag = fmaxf(flag, lim[h]);
+  for (int h = threadIdx.x; h < M.nhor; h += kReduceThreads) flag = fmaxf(flag, lim[h]);
   part[threadIdx.x] = flag;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) part[threadIdx.x] = fmaxf(part[threadIdx.x], part[threadIdx.x + s]);
     __syncthreads();
   }
@@ -1894,7 +1904,8 @@ extern "C" __global__ __launch_bounds__(256) void k_dca_limit(const DevMember* _
 
 // getiflx's global sums (physics/surface/index.ts:72-96), members whose
 // sea ice runs this step only.
-extern "C" __global__ __launch_bounds__(256) void k_ice_sums(const DevMember* __restrict__ mems, LaunchPlan plan) {
+extern "C" __global__ __launch_bounds__(kReduceThreads) void k_ice_sums(const DevMember* __restrict__ mems,
+                                                                         LaunchPlan plan) {
   const DevMember& M = mems[plan.member[blockIdx.x]];
   if (M.k[CON_ice_coupled] == 0.0f) return;
   block_sum2(M, M.arena + M.slot[SLOT_ice_part1], M.arena + M.slot[SLOT_ice_part2],
@@ -1907,18 +1918,18 @@ void enqueue_physics(Engine& e, const std::vector<int>& active) {
   Stepper s(e, active);
   const LaunchPlan one = s.plan([](const DevMember&) { return 1; });
   s.column(k_fixer_column);
-  launch(e, k_fixer_reduce, dim3(one.n), dim3(256), e.d_desc, one);
+  launch(e, k_fixer_reduce, dim3(one.n), dim3(kReduceThreads), e.d_desc, one);
   s.column(k_surface_fluxes);
   s.column(k_shortwave);
   s.column(k_longwave);
   s.column(k_convection);
-  launch(e, k_dca_limit, dim3(one.n), dim3(256), e.d_desc, one);
+  launch(e, k_dca_limit, dim3(one.n), dim3(kReduceThreads), e.d_desc, one);
   s.column(k_rain);
   s.column(k_clouds);
   s.column(k_soil);
   s.column(k_mkradv);
   s.column(k_physics_c);
-  launch(e, k_ice_sums, dim3(one.n), dim3(256), e.d_desc, one);
+  launch(e, k_ice_sums, dim3(one.n), dim3(kReduceThreads), e.d_desc, one);
   s.column(k_physics_d);
 }
 
