@@ -24,7 +24,7 @@ extern "C" __global__ void k_leg_fwd(const DevMember*, GroupPlan, LegJobs);
 extern "C" __global__ void k_leg_inv_split(const DevMember*, LaunchPlan, InvJobs);
 extern "C" __global__ void k_leg_fwd_split(const DevMember*, LaunchPlan, LegJobs);
 extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan, int);
-extern "C" __global__ void k_mean_mode(const DevMember*, LaunchPlan, int);
+extern "C" __global__ void k_mean_dekin(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
 extern "C" __global__ void k_calcgp(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_spectrala(const DevMember*, LaunchPlan);
@@ -123,7 +123,6 @@ inline void launch(Engine& e, void (*k)(Params...), dim3 grid, dim3 block, Args.
 inline int tiles_column(const DevMember& d) { return (d.nhor + 63) / 64; }
 inline int tiles_coeff(const DevMember& d) { return (d.nrsp + 63) / 64; }
 inline int tiles_levels(const DevMember& d) { return (d.nlev * d.nhor + 63) / 64; }
-inline int tiles_lev_coeff(const DevMember& d) { return (d.nlev * d.nrsp + 63) / 64; }
 
 template <typename Jobs>
 int job_planes_total(const Jobs& jobs, int nlev) {
@@ -209,13 +208,14 @@ class Stepper {
     const LaunchPlan p = plan(tiles_levels);
     launch(e_, k, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
   }
+  // one workgroup of `threads` per member and level
+  void member_levels(void (*k)(const DevMember*, LaunchPlan), int threads) {
+    const LaunchPlan p = plan([](const DevMember& d) { return d.nlev; });
+    launch(e_, k, dim3(p.first[p.n]), dim3(threads), e_.d_desc, p);
+  }
   void mkdheat(int stage) {
     const LaunchPlan p = plan(tiles_levels);
     launch(e_, k_mkdheat, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, stage);
-  }
-  void mean_mode(int slot) {
-    const LaunchPlan p = plan(tiles_lev_coeff);
-    launch(e_, k_mean_mode, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, slot);
   }
   void inv(const InvJobs& jobs) {
     // the Legendre half per operand form: plain FP32 over member groups,

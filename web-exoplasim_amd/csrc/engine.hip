@@ -150,11 +150,9 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
   s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt1, 1)}));
   s.inv(jobs_of<InvJobs, InvJob>({inv_winds(SLOT_zdsdp2, SLOT_zdszp2, SLOT_fzdu, SLOT_fzdv, SLOT_zdu, SLOT_zdv, 0)}));
   s.mkdheat(2);
-  s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdekin, SLOT_fzdekin, 1)}));
-  s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdekin, SLOT_zsdef, 1)}));
-  s.mean_mode(SLOT_zsdef);
-  s.inv(jobs_of<InvJobs, InvJob>({inv_plain(TAB_legendre_qi, SLOT_zsdef, SLOT_fzdekin, SLOT_zdekin, 1)}));
-  s.mkdheat(3);
+  // zdekin -> zsdef -> mean mode -> zdekin -> heating, in one pass: only
+  // spectral mode (0, 0) survives the mean-mode cut (k_mean_dekin)
+  s.member_levels(k_mean_dekin, 1024);
   s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
   s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt2, 1)}));
   s.coeff(k_spectrald_update);
@@ -522,7 +520,7 @@ const std::pair<const char*, const void*> kProbeKernels[] = {
     EXO_PROBE(k_load_step) EXO_PROBE(k_leg_inv) EXO_PROBE(k_dft_inv) EXO_PROBE(k_dft_fwd) EXO_PROBE(k_leg_fwd)
     EXO_PROBE(k_leg_inv_split) EXO_PROBE(k_leg_fwd_split)
     EXO_PROBE(k_calcgp) EXO_PROBE(k_spectrala) EXO_PROBE(k_spectrald_diffusion) EXO_PROBE(k_spectrald_update)
-    EXO_PROBE(k_mkdheat) EXO_PROBE(k_mean_mode) EXO_PROBE(k_physics_in) EXO_PROBE(k_fixer_column)
+    EXO_PROBE(k_mkdheat) EXO_PROBE(k_mean_dekin) EXO_PROBE(k_physics_in) EXO_PROBE(k_fixer_column)
     EXO_PROBE(k_fixer_reduce) EXO_PROBE(k_surface_fluxes) EXO_PROBE(k_shortwave) EXO_PROBE(k_longwave)
     EXO_PROBE(k_convection) EXO_PROBE(k_dca_limit) EXO_PROBE(k_rain) EXO_PROBE(k_clouds) EXO_PROBE(k_soil)
     EXO_PROBE(k_mkradv) EXO_PROBE(k_physics_c) EXO_PROBE(k_ice_sums) EXO_PROBE(k_physics_d) EXO_PROBE(k_pack)

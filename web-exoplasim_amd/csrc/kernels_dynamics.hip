@@ -416,12 +416,64 @@ extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __re
   }
 }
 
-// dynamics/wgsl/mean-mode.wgsl:4-13: keep only Re mode (0, 0) of a slot.
-extern "C" __global__ __launch_bounds__(64) void k_mean_mode(const DevMember* __restrict__ mems, LaunchPlan plan,
-                                                              int target) {
-  int tile;
-  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int i = tile * 64 + threadIdx.x;
-  if (i >= M.nlev * M.nrsp) return;
-  if (i % M.nrsp != 0) M.arena[M.slot[target] + i] = 0.0f;
+
+// mkdheat's dissipation of the mean kinetic-energy defect, stages 2b-3
+// (dynamics/index.ts:285-293): the defect zdekin (grid) is transformed to
+// spectral space (gp2fc then fc2sp with qc), cut to its global-mean mode
+// (mean-mode.wgsl:4-13), transformed back (sp2fc with qi then fc2gp), and
+// turned into heating (mkdheat.wgsl, stage 3). Only mode (0, 0) survives,
+// so the chain is, per level l:
+//   S = sum_lat,x qc(lat, 0) zdekin(lat, x) fwd(x, 0)
+//   zdekin(lat, x) = (qi(lat, 0) S) inv(0, x)
+// with the same tables; one workgroup per member and level computes it
+// (every other spectral coefficient of zsdef set to 0, as mean-mode
+// leaves it) and the heating.
+extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember* __restrict__ mems, LaunchPlan plan) {
+  int l;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &l)];
+  const int NH = M.nhor, nlat = M.nlat, nlon = M.nlon, ncsp = M.ncsp, nfc = M.nfc, nrsp = M.nrsp;
+  float* A = M.arena;
+  float* zdekin = A + M.slot[SLOT_zdekin] + l * NH;
+  const float* qc = M.ltab + M.tabo[TAB_legendre_qc];
+  const float* qi = M.ltab + M.tabo[TAB_legendre_qi];
+  const float* fwd = M.ltab + M.tabo[TAB_fourier_forward];
+  const float* inv = M.ltab + M.tabo[TAB_fourier_inverse];
+  const int half = nlat / 2;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // Fourier mode 0 of each row, weighted by its latitude's qc (mode (0, 0)
+  // is even about the equator: a southern row takes its northern twin's);
+  // waves take rows, lanes points
+  __shared__ float part[1024];
+  float s = 0.0f;
+  for (int lat = wave; lat < nlat; lat += 16) {
+    const float w = qc[(lat < half ? lat : nlat - 1 - lat) * ncsp];
+    for (int x = lane; x < nlon; x += 64) s += w * (zdekin[lat * nlon + x] * fwd[x * nfc]);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  const float S = part[0];
+  // zsdef keeps only Re mode (0, 0)
+  float* zsdef = A + M.slot[SLOT_zsdef] + l * nrsp;
+  for (int c = threadIdx.x; c < nrsp; c += 1024) zsdef[c] = c == 0 ? S : 0.0f;
+  // back to the grid, then the heating (mkdheat stage 3)
+  const float CT = M.k[CON_ct], WW = M.k[CON_ww], GA = M.k[CON_ga], ACPD = M.k[CON_acpd], ADV = M.k[CON_adv];
+  const float dsig = M.tab[M.tabo[TAB_dsigma] + l];
+  const float* dq = A + M.slot[SLOT_dq] + l * NH;
+  const float* dp = A + M.slot[SLOT_dp];
+  float* zdtdt = A + M.slot[SLOT_zdtdt] + l * NH;
+  for (int lat = wave; lat < nlat; lat += 16) {
+    const float f = qi[lat * ncsp] * S;
+    for (int x = lane; x < nlon; x += 64) {
+      const int h = lat * nlon + x;
+      const float e = f * inv[x];
+      zdekin[h] = e;
+      const float heat = 1.0f + ADV * dq[h];
+      const float dtdt = -e * 0.5f / ACPD / heat / dp[h] * GA / dsig;
+      zdtdt[h] = dtdt / CT / WW;
+    }
+  }
 }

@@ -3,13 +3,14 @@
 Counts are per model step and member, from the transform plane counts of
 one step (the job lists the step enqueues):
 
-* inverse Legendre (k_leg_inv): 8L+3 scalar planes (sp2fc/sp2fcdmu) and
-  5L dv2uv levels; forward Legendre (k_leg_fwd): 5L+1 scalar planes
+* inverse Legendre (k_leg_inv): 7L+3 scalar planes (sp2fc/sp2fcdmu) and
+  5L dv2uv levels; forward Legendre (k_leg_fwd): 4L+1 scalar planes
   (fc2sp), L uv2dv levels with kinetic energy, L without, 2L qtend
-  levels. FLOPs use the north/south-symmetric count with an FMA as 2:
+  levels. (The dissipation's mean-mode round trip is not a transform
+  here: k_mean_dekin computes its one surviving mode directly.) FLOPs use the north/south-symmetric count with an FMA as 2:
   2 NLAT NCSP per scalar plane, 8 per dv2uv level, 10 per uv2dv+KE level,
   8 per uv2dv level, 6 per qtend level. Bound: the FP32 vector peak.
-* zonal DFTs (k_dft_inv, k_dft_fwd): 18L+4 inverse and 16L+1 forward rows
+* zonal DFTs (k_dft_inv, k_dft_fwd): 17L+4 inverse and 15L+1 forward rows
   of every latitude; each row reads or writes NFC Fourier and NLON grid
   floats once: 4 (NLON + NFC) bytes. Bound: HBM.
 * column physics (HBM): the distinct arena floats per column a kernel's
@@ -49,13 +50,13 @@ def work_per_step(kernel, cfg):
     nlat, nlon, L, ncsp, nfc = _dims(cfg)
     kernel = kernel.replace("_split", "")
     if kernel == "k_leg_inv":
-        return "flop", nlat * ncsp * (2 * (8 * L + 3) + 8 * 5 * L)
+        return "flop", nlat * ncsp * (2 * (7 * L + 3) + 8 * 5 * L)
     if kernel == "k_leg_fwd":
-        return "flop", nlat * ncsp * (2 * (5 * L + 1) + 10 * L + 8 * L + 6 * 2 * L)
+        return "flop", nlat * ncsp * (2 * (4 * L + 1) + 10 * L + 8 * L + 6 * 2 * L)
     if kernel == "k_dft_inv":
-        return "byte", 4 * (nlon + nfc) * nlat * (18 * L + 4)
+        return "byte", 4 * (nlon + nfc) * nlat * (17 * L + 4)
     if kernel == "k_dft_fwd":
-        return "byte", 4 * (nlon + nfc) * nlat * (16 * L + 1)
+        return "byte", 4 * (nlon + nfc) * nlat * (15 * L + 1)
     column = {"k_surface_fluxes": 16 * L + 18, "k_shortwave": 9 * L + 14, "k_longwave": 16 * L + 8,
               "k_convection": 13 * L + 5}
     if kernel in column:
@@ -63,7 +64,7 @@ def work_per_step(kernel, cfg):
     raise KeyError(f"no algorithmic work model for {kernel}")
 
 
-LAUNCHES_PER_STEP = {"k_leg_inv": 5, "k_dft_inv": 5, "k_leg_fwd": 5, "k_dft_fwd": 5, "k_surface_fluxes": 1,
+LAUNCHES_PER_STEP = {"k_leg_inv": 4, "k_dft_inv": 4, "k_leg_fwd": 4, "k_dft_fwd": 4, "k_surface_fluxes": 1,
                      "k_shortwave": 1, "k_longwave": 1, "k_convection": 1}
 
 
