@@ -32,7 +32,9 @@ HBM_PEAK_GBS = 8000.0
 
 # The kernel bench.py times by default: the one with the largest share of
 # device time in that workload's rocprofv3 kernel trace.
-DOMINANT = {"batched": "k_leg_inv", "plain": "k_leg_inv"}
+# (ensemble: k_dft_inv 12.6%, k_leg_inv 12.4% of device time in
+# profiles/ensemble64_fp32_kernel_trace.md)
+DOMINANT = {"batched": "k_dft_inv", "plain": "k_leg_inv"}
 
 
 def _dims(cfg):
