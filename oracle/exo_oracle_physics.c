@@ -7,17 +7,20 @@
  * (shared/gpu/math/wgsl/precise.wgsl) are libm's log/exp/pow here.
  *
  * Pinned by physics on crafted columns (tests/test_oracle_physics_pins.py):
- * surflx, lwr, swr, solang, mko3, radstep, kuo, mkdca, mklsp, mkrain, vdiff,
- * soil, the ocean mixed layer and the fixer, each against laws their results
+ * every parameterization (surflx, vdiff, lwr, swr, solang, mko3, radstep,
+ * kuo, mkdca, mklsp, mkrain, mkclouds, soil, mkradv, land, sea_ice, the
+ * ocean mixed layer, carbon and the fixer), each against laws its results
  * must obey (Kirchhoff, energy and water conservation, backward-Euler
  * diffusion, textbook latent heats, Tetens saturation, random cloud
- * overlap, Green's ozone profile, Charnock's roughness, the log law).
+ * overlap, Green's ozone profile, Charnock's roughness, the log law,
+ * Slingo's cloud cover, the Walker-Hays-Kasting weathering law).
  *
- * Parity unpinned: mkclouds, rivers, the land surface's albedos,
- * the sea-ice model (sea_ice) and carbon. No closed form, library routine or
- * reference golden vector checks them; they are held only to the GPU
- * kernels written from the same reading of the reference WGSL, and by the
- * physical range of day-scale runs.
+ * Parity unpinned: the glacier and wet-soil albedo curves, the ice
+ * surface temperature's implicit solution, the weathering supply limit
+ * and the cloud liquid-water profile. No closed form, library routine or
+ * reference golden vector checks those details; they are held only to the
+ * GPU kernels written from the same reading of the reference WGSL, and by
+ * the physical range of day-scale runs.
  */
 #define EXO_ORACLE_INTERNAL
 #include <tgmath.h>
@@ -1719,6 +1722,16 @@ int oracle_routine(const oracle_member* m, const char* name) {
     for (int h = 0; h < v->NHOR; h++) mkrain(v, h);
   } else if (IS("soil")) {
     soil(v);
+  } else if (IS("mkradv")) {
+    mkradv(v);
+  } else if (IS("mkclouds")) {
+    for (int h = 0; h < v->NHOR; h++) mkclouds(v, h);
+  } else if (IS("land")) {
+    land_surface(v);
+  } else if (IS("sea_ice")) {
+    sea_ice(v);
+  } else if (IS("carbon")) {
+    carbonstep(v);
   } else if (IS("seastep")) {
     seastep(v);
   } else {

@@ -41,8 +41,19 @@ oracle fails at least one test each.
   the lowest layer's momentum, enthalpy and water budgets against the
   stress, sensible and latent heat fluxes and evaporation.
 
-The remaining parameterizations (mkclouds, rivers, the land surface's
-albedos, the sea-ice model, carbon) have no such pin here.
+* mkradv: river routing moves water between cells without making any.
+* sea_ice: thickness change = melt energy / (rho_i L_f), the snow-pack
+  budget, snow melt flux = melt x L_f of snow, ice surface <= melting point.
+* mkclouds: Slingo's convective cover spread by random overlap, the
+  stratiform cover from relative humidity, the column total, the vapour
+  path.
+* land: the river store gains the runoff, the snow-albedo weight
+  d / (d + 1 cm), saturated surface air.
+* carbon: the Walker-Hays-Kasting weathering law against the present Earth.
+
+Not pinned here: the glacier and wet-soil albedo curves, the ice surface
+temperature's implicit solution, the weathering supply limit (off by
+default) and the cloud liquid-water profile.
 Each test states the tolerance it needs: FP64 arithmetic, so conservation
 holds to rounding.
 """
@@ -493,7 +504,7 @@ def test_radstep_heating_is_the_flux_convergence(column):
 
 def _saturation(run, t, p):
     eps = run.constant("rdbrv")
-    es = 610.78 * np.exp(17.27 * (t - 273.16) / (t - 35.86))  # Tetens (Murray 1967)
+    es = 610.78 * np.exp(17.2693882 * (t - 273.16) / (t - 35.86))  # Tetens (Murray 1967)
     return eps * es / (p - (1 - eps) * es)
 
 
@@ -891,3 +902,212 @@ def test_surflx_log_law_and_layer_budgets(column):
     bucket = run.field("dwatc") / run.constant("deltsec")
     assert np.all(-run.field("devap")[on_land] <= bucket[on_land] * (1 + 1e-12))
     assert np.any(-run.field("devap")[on_land] >= bucket[on_land] * (1 - 1e-12))  # the limit binds
+
+
+def test_mkradv_moves_river_water_without_making_any(column):
+    """River routing (mkradv): water crosses each cell face out of one cell
+    and into its neighbour, weighted by the cell areas, so the area-weighted
+    sum of the runoff is unchanged except for what leaves through the
+    southern edge of the last latitude row."""
+    run, nh, L = _run(column)
+    host, m, nlat, nlon, _ = column
+    rng = np.random.default_rng(20)
+    run.field("dls")[:] = (rng.uniform(0, 1, nh) < 0.6).astype(float)
+    run.field("driver")[:] = rng.uniform(0, 50, nh)
+    area = rng.uniform(1e9, 5e10, nh)
+    run.field("darea")[:] = area
+    run.field("duroff")[:] = rng.uniform(-1e-6, 1e-6, nh)
+    run.field("dvroff")[:] = rng.uniform(-1e-6, 1e-6, nh)
+    before = rng.uniform(0, 1e-6, nh)
+    run.field("drunoff")[:] = before
+    run.routine("mkradv")
+    after = run.field("drunoff")
+    river = run.field("driver") * area * run.field("dls")
+    last = slice((nlat - 1) * nlon, nh)
+    v_last = run.field("dvroff")[last]
+    lost = np.sum(np.where(v_last < 0, -v_last * river[last], 0.0))
+    change = np.sum(area * (after - before))
+    scale = np.sum(area * np.abs(after - before))
+    assert abs(change + lost) <= 1e-12 * scale
+    assert np.count_nonzero(after != before) > nh // 2
+
+
+def test_sea_ice_mass_and_latent_heat_budgets(column):
+    """Sea ice: the thickness change is the melt energy over rho_i L_f (the
+    textbook 920 kg/m^3 and 3.28e5 J/kg of sea ice); the snow pack changes
+    by snowfall minus melt; the snow melt flux is the melt times the
+    latent heat of fusion of snow (3.337e5 J/kg); an ice surface never
+    exceeds the melting point, open water is at the sea temperature."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(21)
+    rho_i, lf_i, lf_s = run.constant("crhoi"), run.constant("clfi"), run.constant("clfsn")
+    assert abs(rho_i / 920 - 1) < 0.02 and abs(lf_i / 3.34e5 - 1) < 0.03 and abs(lf_s / 3.337e5 - 1) < 1e-3
+    xdt = run.constant("xdt")
+    run.field("dls")[:] = 0.0
+    run.field("xls")[:] = 0.0
+    run.field("sea_naccua")[:] = 0.0
+    run.field("sea_cheata")[:] = 0.0
+    run.field("sea_cprsa")[:] = 0.0
+    run.field("dshfl")[:] = rng.uniform(-60, 20, nh)
+    run.field("dlhfl")[:] = rng.uniform(-80, 0, nh)
+    _planes(run, "dswfl", nh)[L] = rng.uniform(0, 300, nh)
+    _planes(run, "dlwfl", nh)[L] = rng.uniform(-120, -20, nh)
+    snowfall = rng.uniform(0, 5e-8, nh) * (rng.uniform(0, 1, nh) < 0.6)
+    run.field("dprs")[:] = snowfall
+    tf = run.constant("tfreeze")
+    run.field("xsst")[:] = rng.uniform(tf - 0.5, tf + 3, nh)
+    ice = rng.uniform(0, 4, nh) * (rng.uniform(0, 1, nh) < 0.7)
+    run.field("xiced")[:] = ice
+    run.field("xicec")[:] = np.where(ice > 0, rng.uniform(0, 1, nh), 0.0)
+    run.field("xicecc")[:] = run.field("xicec")
+    snow = rng.uniform(0, 0.3, nh) * (ice > 0)
+    run.field("xsnow")[:] = snow
+    run.field("xts")[:] = rng.uniform(240, 275, nh)
+    run.field("xcflux")[:] = 0.0
+    run.field("xcfluxf")[:] = 0.0
+    run.field("xoflux")[:] = rng.uniform(-20, 20, nh)
+    run.routine("sea_ice")
+    new_ice = run.field("xiced")
+    melt = run.field("ximelt")
+    np.testing.assert_allclose(new_ice - ice, -melt * xdt / (rho_i * lf_i), rtol=0, atol=1e-8)
+    assert np.any(melt > 0) and np.any(melt < 0)  # both melting and freezing
+    smelt = run.field("xsmelt")
+    np.testing.assert_allclose(run.field("xsnow") - snow, (snowfall - smelt) * xdt, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(run.field("xsmflx"), smelt * 1000.0 * lf_s, rtol=1e-12, atol=1e-9)
+    covered = ice >= run.constant("xmind")
+    assert np.all(run.field("xts")[covered] <= run.constant("tmelt"))
+    np.testing.assert_array_equal(run.field("xts")[~covered], run.field("xsst")[~covered])  # open water
+    assert np.all(new_ice >= 0) and np.all(new_ice <= run.constant("xmaxd"))
+
+
+def test_mkclouds_slingo_cover_and_random_overlap(column):
+    """Clouds (mkclouds): convective cloud of total cover Slingo's
+    0.245 + 0.125 ln P (P in mm/day, clamped to [0.05, 0.8]) spread over
+    the convective levels so that their random overlap gives that total;
+    stratiform cover ((RH - RHc) / (1 - RHc))^2 above the critical
+    relative humidity (Slingo 1987), with Tetens saturation; the column
+    total is the random overlap of the layers; the column's vapour path is
+    sum q dsigma p_s / g."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(22)
+    sig = run.table("sigma")
+    ps = rng.uniform(9e4, 1.03e5, nh)
+    run.field("dp")[:] = ps
+    t = 290.0 * sig[:, None] ** 0.19 + rng.normal(0, 1, (L, nh))
+    _planes(run, "dt", nh)[:L] = t
+    _planes(run, "dtdt", nh)[:] = 0.0
+    _planes(run, "dqdt", nh)[:] = 0.0
+    rh = rng.uniform(0.3, 1.0, (L, nh))
+    qs = _saturation(run, t, sig[:, None] * ps)
+    _planes(run, "dq", nh)[:L] = rh * qs
+    conv = (rng.uniform(0, 1, (L, nh)) < 0.3) & (sig[:, None] > 0.4)
+    run.field("icclev")[:] = 0.0
+    _planes(run, "icclev", nh)[:L] = conv.astype(float)
+    n_conv = conv.sum(axis=0)
+    run.field("icctot")[:] = n_conv
+    rain = rng.uniform(0, 3e-7, nh) * (n_conv > 0)  # m/s
+    run.field("dprc")[:] = rain
+    run.field("dw")[:] = 0.0
+    assert run.constant("clwfac") <= 0  # no low-level vertical-velocity factor
+    run.routine("mkclouds")
+    cc = _planes(run, "dcc", nh)
+    total = 0.245 + 0.125 * np.log(np.maximum(rain, 1e-300) * 86400e3)  # mm/day
+    total = np.clip(total, 0.05, 0.8)
+    rhc = run.table("rcrit")[:, None]
+    strat = np.where(rh > rhc, ((rh - rhc) / (1 - rhc)) ** 2, 0.0)
+    pure_conv = conv & (rh < rhc)  # a convective level without stratiform cloud
+    cols = np.where((n_conv > 0) & np.all(~conv | pure_conv, axis=0))[0]
+    assert cols.size > 50
+    conv_cover = np.where(conv[:, cols], cc[:L, cols], 0.0)
+    np.testing.assert_allclose(1 - np.prod(1 - conv_cover, axis=0), total[cols], rtol=1e-6)
+    plain = ~conv & (rh > rhc + 0.05)
+    np.testing.assert_allclose(cc[:L][plain], np.minimum(strat[plain], 1.0), rtol=1e-5)
+    np.testing.assert_allclose(cc[L], 1 - np.prod(1 - cc[:L], axis=0), rtol=1e-12, atol=1e-15)
+    vapour = np.sum(rh * qs * run.table("dsigma")[:, None], axis=0) * ps / run.constant("ga")
+    np.testing.assert_allclose(run.field("dqvi"), vapour, rtol=1e-12)
+
+
+def test_land_surface_river_store_albedo_and_surface_air(column):
+    """Land surface: the river store gains the cell's runoff over the time
+    step; a snow pack blends the snow albedo into the ground's as
+    d / (d + 1 cm) (a convex weight: the ground's without snow, the snow's
+    under deep snow); the surface air takes the land temperature and its
+    Tetens saturation humidity."""
+    run, nh, L = _run(column)
+    rng = np.random.default_rng(23)
+    land = rng.uniform(0, 1, nh) < 0.6
+    run.field("dls")[:] = land.astype(float)
+    run.field("dglac")[:] = 0.0
+    river = rng.uniform(0, 10, nh)
+    run.field("driver")[:] = river
+    runoff = rng.uniform(0, 1e-6, nh)
+    run.field("drunoff")[:] = runoff
+    # pairs of columns (2i, 2i + 1) alike but for their snow depth: the
+    # snow albedo is then the same for both and cancels from the ratio
+    ts = np.repeat(rng.uniform(240, 300, nh // 2), 2)
+    run.field("land_dts")[:] = ts
+    ps = rng.uniform(8e4, 1.04e5, nh)
+    run.field("dp")[:] = ps
+    snow = 10 ** rng.uniform(-5, 1, nh)
+    snow[::4] = 0.0
+    run.field("dsnowz")[:] = snow
+    run.field("dforest")[:] = np.repeat(rng.uniform(0, 1, nh // 2), 2)
+    land[1::2] = land[::2]
+    run.field("dls")[:] = land.astype(float)
+    run.routine("land")
+    np.testing.assert_allclose(run.field("driver"), np.where(land, river + runoff * run.constant("deltsec"), river),
+                               rtol=1e-12)
+    ground = run.field("dalbclim")
+    alb = run.field("dalb")
+    bare = land & (snow == 0)
+    np.testing.assert_array_equal(alb[bare], ground[bare])
+    w = snow / (snow + 0.01)
+    even = np.arange(0, nh, 2)
+    both = land[even] & (snow[even] > 0) & (snow[even + 1] > 0)
+    a, b = even[both], even[both] + 1
+    assert a.size > 100 and np.all(ground[a] == ground[b])
+    np.testing.assert_allclose((alb[a] - ground[a]) * w[b], (alb[b] - ground[b]) * w[a], rtol=1e-9, atol=1e-15)
+    assert np.all((alb[land] > 0) & (alb[land] < 1))
+    np.testing.assert_array_equal(_planes(run, "dt", nh)[L][land], ts[land])
+    np.testing.assert_allclose(_planes(run, "dq", nh)[L][land], _saturation(run, ts, ps)[land], rtol=1e-12)
+
+
+def test_carbon_weathering_law(column):
+    """Carbon: silicate weathering follows Walker, Hays and Kasting (1981),
+    W = (pCO2 / pCO2_E)^beta exp(k (T - 288 K)) (P / P_E)^0.65, relative to
+    the present Earth (W = 1 there), scaled by the planet's land area
+    against Earth's (radius^2 over Earth's, over Earth's land fraction
+    0.29); none over the sea or below freezing; precipitation is converted
+    from m/s to cm/yr; the running means gain the time-weighted values."""
+    run, nh, L = _run(column)
+    if run.constant("ncarbon") == 0:
+        pytest.skip("carbon cycle off in this configuration")
+    rng = np.random.default_rng(24)
+    assert run.constant("nsupply") == 0
+    beta, k, P_E, co2_E = (run.constant(n) for n in ("cbeta", "kact", "pearth", "co2earth"))
+    run.field("carbon_step")[:] = 1.0
+    land = rng.uniform(0, 1, nh) < 0.6
+    run.field("dls")[:] = land.astype(float)
+    ts = rng.uniform(260, 310, nh)
+    _planes(run, "dt", nh)[L] = ts
+    ps = rng.uniform(8e4, 1.1e5, nh)
+    run.field("dp")[:] = ps
+    co2 = 400.0
+    run.field("co2")[:] = co2
+    seconds_per_year = 3.154e7  # 365.0 days
+    precip_cm = rng.uniform(10, 300, nh)
+    run.field("dprc")[:] = precip_cm / 100.0 / seconds_per_year * 0.3
+    run.field("dprl")[:] = precip_cm / 100.0 / seconds_per_year * 0.7
+    for name in ("localavgprecip", "localavgtemps", "localavgweather"):
+        run.field(name)[:] = 0.0
+    run.routine("carbon")
+    np.testing.assert_allclose(run.field("localprecip"), precip_cm, rtol=1e-12)
+    pco2 = co2 * ps * 1e-5  # ppm of the column, in units of Earth's 1 bar
+    W = (pco2 / co2_E) ** beta * np.exp(k * (ts - 288.0)) * (precip_cm / P_E) ** 0.65
+    area = run.constant("plarad") ** 2 / (6.371e6 ** 2 * 0.29)
+    warm = land & (ts >= 273.15)
+    np.testing.assert_allclose(run.field("localweathering")[warm], W[warm] * area, rtol=2e-3)
+    assert np.all(run.field("localweathering")[~warm] == 0)
+    tw = run.constant("timeweight")
+    np.testing.assert_allclose(run.field("localavgtemps"), ts * tw, rtol=1e-12)
+    np.testing.assert_allclose(run.field("localavgweather"), run.field("localweathering") * tw, rtol=1e-12)
