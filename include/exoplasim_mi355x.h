@@ -243,6 +243,67 @@ int exo_probe_kernel(exo_engine *e, const char *kernel, int max_launches);
 int exo_probe_result(exo_engine *e, double *total_ms, int *launches,
                      int *missed);
 
+/* ---- Averages and balance ------------------------------------------
+   Monthly means of a member's published snapshots, kept on the device:
+   GCM_AVERAGES.add / close / annual (src/model/climate/gcm/averages/
+   index.ts:26-137) with the month history of HISTORY_MONTHS (:21-23) and
+   the yearly global balances (yearly, :31-48), and ExoPlaSim's drift test
+   GCM_BALANCE.assess (src/model/climate/gcm/balance/index.ts:6-68). The
+   reference runs them on the host over read-back snapshots; here each
+   published snapshot is added into an FP64 month sum on the device
+   (exo_step), and only what is asked for is read back. */
+#define EXO_MONTHS_PER_YEAR 12
+#define EXO_HISTORY_MONTHS 132
+
+typedef struct {
+  int year, month; /* calendar month the mean covers */
+  int samples;     /* snapshots averaged */
+  int64_t step;    /* step of its last snapshot */
+} exo_month_info;
+
+typedef struct {
+  int year;
+  double toa_net_radiation;   /* W/m^2, global mean of the annual mean */
+  double surface_net_heat;    /* W/m^2 */
+  double surface_temperature; /* K */
+} exo_yearly_balance;
+
+#define EXO_BALANCE_SPINNING_UP 0
+#define EXO_BALANCE_DRIFTING 1
+#define EXO_BALANCE_BALANCED 2
+
+typedef struct {
+  int status;       /* EXO_BALANCE_* */
+  int years_needed; /* spinning up: years still missing */
+  double toa_drift_per_year, surface_drift_per_year; /* W/m^2 per year; NaN while spinning up */
+} exo_balance_assessment;
+
+/* Starts (enable = 1) or stops averaging for a member: starting allocates
+   its month sum and the month history on the device (HISTORY_MONTHS x
+   the snapshot size) and clears them; the member's next published
+   snapshots are averaged. Needs a publishing interval (exo_set_publish_
+   every). */
+int exo_set_averaging(exo_engine *e, int member, int enable);
+/* Closed monthly means held (<= EXO_HISTORY_MONTHS). */
+int exo_month_count(exo_engine *e, int member, int *count);
+/* Closed month `index` (0 = oldest held): its calendar month and, when
+   values is not NULL, its mean snapshot (`count` floats: fields x NHOR,
+   exo_read_snapshot's layout). Waits for queued steps. */
+int exo_read_month(exo_engine *e, int member, int index, exo_month_info *info,
+                   float *values, size_t count);
+/* GCM_AVERAGES.annual (averages/index.ts:103-137): the mean of the last 12
+   closed months, with the diurnal temperature range as the mean of the
+   months' daily ranges; EXO_E_STATE before 12 months have closed. */
+int exo_read_annual(exo_engine *e, int member, float *values, size_t count);
+/* The member's yearly balances so far (one per complete calendar year of
+   closed months), oldest first; *count = how many there are (up to max
+   written). */
+int exo_yearly_balances(exo_engine *e, int member, exo_yearly_balance *out,
+                        int max, int *count);
+/* GCM_BALANCE.assess over yearly balances, oldest first (host only). */
+int exo_balance_assess(const exo_yearly_balance *years, int count,
+                       exo_balance_assessment *out);
+
 #ifdef __cplusplus
 }
 #endif

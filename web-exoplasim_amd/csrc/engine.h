@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <deque>
 #include <initializer_list>
 #include <map>
 #include <memory>
@@ -52,9 +53,61 @@ namespace exo {
 struct DeviceError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+
+// Throws DeviceError with the call and HIP's message when a HIP call fails.
+#define HIP_OR_THROW(expr)                                                              \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      throw exo::DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e));        \
+  } while (0)
+
 // Every descriptor of the engine is taken.
 struct FullError : std::runtime_error {
   using std::runtime_error::runtime_error;
+};
+
+// Device-side averages of a member's published snapshots (averages.hip):
+// the open month's FP64 sum, the history of closed monthly means (a ring
+// of EXO_HISTORY_MONTHS slots) and the yearly balances (3 doubles each).
+constexpr int kAvgYearCapacity = 4096;
+struct Averager {
+  double* d_sum = nullptr;
+  float* d_months = nullptr;
+  double* d_years = nullptr;
+  double* d_weights = nullptr;  // FP64 Gaussian weight of every grid point
+  size_t floats = 0;  // snapshot floats: fields x NHOR
+  bool open = false;  // a month is being summed
+  int year = 0, month = 0, samples = 0;
+  int64_t step = 0;
+  int head = 0;                       // history slot of the next closed month
+  std::deque<exo_month_info> months;  // closed months held, oldest first
+  std::vector<int> years;             // years whose balance is in d_years
+  size_t slot_of(size_t index) const; // history slot of months[index]
+  ~Averager();
+};
+
+// Kernel arguments of the averaging launches.
+enum { kAvgOpen = 0, kAvgAdd = 1, kAvgClose = 2 };
+struct AvgLaunch {
+  int n;
+  int first[EXO_MAX_MEMBERS + 1];  // first workgroup of entry i
+  int mode[EXO_MAX_MEMBERS];
+  int samples[EXO_MAX_MEMBERS];  // kAvgClose: snapshots in the closing month
+  size_t count[EXO_MAX_MEMBERS];
+  const float* snap[EXO_MAX_MEMBERS];
+  double* sum[EXO_MAX_MEMBERS];
+  float* out[EXO_MAX_MEMBERS];  // kAvgClose: the month's history slot
+};
+constexpr int kYearBatch = 16;  // years per k_avg_year launch (kernel argument size)
+struct YearLaunch {
+  int n;
+  int desc[kYearBatch];
+  const float* months[kYearBatch];  // history base
+  const double* weights[kYearBatch];
+  size_t floats[kYearBatch];
+  int slot[kYearBatch][EXO_MONTHS_PER_YEAR];
+  double* out[kYearBatch];
 };
 
 struct MemberDev {
@@ -72,6 +125,7 @@ struct MemberDev {
   int64_t carbon_count = 0;
   int desc = -1;              // descriptor index
   bool active = false;
+  std::unique_ptr<Averager> avg;  // set while averaging (exo_set_averaging)
 };
 
 // Times every launch of one kernel with a pair of HIP events recorded on
@@ -294,6 +348,10 @@ int set_error(int code, const std::string& message);
 size_t member_bytes(const Engine& e, const MemberDev& md);
 int admit_member(Engine& e, std::unique_ptr<MemberDev> md);
 void drop_member(Engine& e, int id);
+// The averaging members that published at the step just enqueued (their
+// snapshot's step numbers): their month sums, closed months and yearly
+// balances (averages.hip).
+void enqueue_averages(Engine& e, const std::vector<MemberDev*>& members, const std::vector<int64_t>& steps);
 
 }  // namespace exo
 

@@ -34,12 +34,6 @@ namespace exo {
 int set_error(int code, const std::string& message) { return fail(code, message); }
 }  // namespace exo
 
-#define HIP_OR_THROW(expr)                                                              \
-  do {                                                                                  \
-    hipError_t _e = (expr);                                                             \
-    if (_e != hipSuccess)                                                               \
-      throw exo::DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e));        \
-  } while (0)
 
 extern "C" __global__ void k_global_means(const DevMember*, LaunchPlan, double*);
 
@@ -438,12 +432,19 @@ int exo_step(exo_engine* h, int steps) {
       carbon[i] = act[i]->carbon_count;
     }
     std::vector<std::vector<int>> phase_at(steps, std::vector<int>(act.size()));
+    // per step: the averaging members that publish at it, with their step
+    std::vector<std::vector<MemberDev*>> avg_at(steps);
+    std::vector<std::vector<int64_t>> avg_step(steps);
     for (int k = 0; k < steps; k++)
       for (size_t i = 0; i < act.size(); i++) {
         const exo::Member& m = act[i]->host;
         const bool startup = phase[i] < m.startup_steps;
         double v[EXO_STEP_VALUES];
         const int publish = (!startup && e.publish_every > 0 && stepno[i] % e.publish_every == 0) ? 1 : 0;
+        if (publish && act[i]->avg) {
+          avg_at[k].push_back(act[i]);
+          avg_step[k].push_back(stepno[i]);
+        }
         exo::step_values(m, stepno[i], carbon[i], publish, v);
         for (int c = 0; c < EXO_STEP_VALUES; c++) e.h_params[k * per_step + i * EXO_STEP_VALUES + c] = (float)v[c];
         carbon[i] = v[12] != 0 ? 0 : carbon[i] + 1;
@@ -479,6 +480,7 @@ int exo_step(exo_engine* h, int steps) {
       for (size_t i = 0; i < act.size(); i++)
         if (phase_at[k][i] >= act[i]->host.startup_steps) main_desc.push_back(desc[i]);
       if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
+      if (!avg_at[k].empty()) exo::enqueue_averages(e, avg_at[k], avg_step[k]);
     }
     HIP_OR_THROW(hipEventRecord(e.t1, e.stream));
     HIP_OR_THROW(hipGetLastError());

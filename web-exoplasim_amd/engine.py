@@ -78,6 +78,27 @@ class ExoServeOptions(ctypes.Structure):
                 ("prepare_threads", ctypes.c_int)]
 
 
+class ExoMonthInfo(ctypes.Structure):
+    """exo_month_info: the calendar month a closed monthly mean covers."""
+
+    _fields_ = [("year", ctypes.c_int), ("month", ctypes.c_int), ("samples", ctypes.c_int),
+                ("step", ctypes.c_int64)]
+
+
+class ExoYearlyBalance(ctypes.Structure):
+    """exo_yearly_balance: global means of one calendar year's annual mean."""
+
+    _fields_ = [("year", ctypes.c_int), ("toa_net_radiation", ctypes.c_double),
+                ("surface_net_heat", ctypes.c_double), ("surface_temperature", ctypes.c_double)]
+
+
+class ExoBalanceAssessment(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("years_needed", ctypes.c_int),
+                ("toa_drift_per_year", ctypes.c_double), ("surface_drift_per_year", ctypes.c_double)]
+
+
+BALANCE_STATUS = {0: "spinning-up", 1: "drifting", 2: "balanced"}
+
 _lib = None
 
 
@@ -128,6 +149,12 @@ def library():
         "exo_checkpoint_size": (I, [P, I, ctypes.POINTER(S)]),
         "exo_checkpoint_save": (I, [P, I, ctypes.c_void_p, S]),
         "exo_checkpoint_restore": (I, [P, I, ctypes.c_void_p, S]),
+        "exo_set_averaging": (I, [P, I, I]),
+        "exo_month_count": (I, [P, I, ctypes.POINTER(I)]),
+        "exo_read_month": (I, [P, I, I, ctypes.POINTER(ExoMonthInfo), F, S]),
+        "exo_read_annual": (I, [P, I, F, S]),
+        "exo_yearly_balances": (I, [P, I, ctypes.POINTER(ExoYearlyBalance), I, ctypes.POINTER(I)]),
+        "exo_balance_assess": (I, [ctypes.POINTER(ExoYearlyBalance), I, ctypes.POINTER(ExoBalanceAssessment)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -135,6 +162,24 @@ def library():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def balance_assess(years):
+    """GCM_BALANCE.assess over yearly balances (dicts or (toa, surface)
+    pairs), oldest first: {"status", "years_needed", "toa_drift_per_year",
+    "surface_drift_per_year"}. Host only."""
+    lib = library()
+    arr = (ExoYearlyBalance * max(1, len(years)))()
+    for i, y in enumerate(years):
+        if isinstance(y, dict):
+            arr[i] = ExoYearlyBalance(y.get("year", i), y["toa_net_radiation"], y["surface_net_heat"],
+                                      y.get("surface_temperature", 0.0))
+        else:
+            arr[i] = ExoYearlyBalance(i, y[0], y[1], 0.0)
+    out = ExoBalanceAssessment()
+    _check(lib.exo_balance_assess(arr, len(years), ctypes.byref(out)))
+    return {"status": BALANCE_STATUS[out.status], "years_needed": out.years_needed,
+            "toa_drift_per_year": out.toa_drift_per_year, "surface_drift_per_year": out.surface_drift_per_year}
 
 
 def exported_symbols():
@@ -343,6 +388,43 @@ class Engine:
                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
         nhor = self.dims(member)[1] * self.dims(member)[2]
         return out.reshape(-1, nhor)
+
+    # ---- averages and balance (GCM_AVERAGES / GCM_BALANCE on the device)
+
+    def set_averaging(self, member, enable=True):
+        _check(self.lib.exo_set_averaging(self.h, member, 1 if enable else 0))
+
+    def month_count(self, member):
+        n = ctypes.c_int()
+        _check(self.lib.exo_month_count(self.h, member, ctypes.byref(n)))
+        return n.value
+
+    def read_month(self, member, index):
+        """(info dict, mean snapshot (fields, NHOR)) of closed month `index`
+        (0 = oldest held)."""
+        _, n = self.slot(member, "snapshot")
+        info = ExoMonthInfo()
+        out = np.empty(n, dtype=np.float32)
+        _check(self.lib.exo_read_month(self.h, member, index, ctypes.byref(info),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
+        nhor = self.dims(member)[1] * self.dims(member)[2]
+        return ({"year": info.year, "month": info.month, "samples": info.samples, "step": info.step},
+                out.reshape(-1, nhor))
+
+    def read_annual(self, member):
+        _, n = self.slot(member, "snapshot")
+        out = np.empty(n, dtype=np.float32)
+        _check(self.lib.exo_read_annual(self.h, member, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
+        nhor = self.dims(member)[1] * self.dims(member)[2]
+        return out.reshape(-1, nhor)
+
+    def yearly_balances(self, member):
+        n = ctypes.c_int()
+        _check(self.lib.exo_yearly_balances(self.h, member, None, 0, ctypes.byref(n)))
+        out = (ExoYearlyBalance * max(1, n.value))()
+        _check(self.lib.exo_yearly_balances(self.h, member, out, n.value, ctypes.byref(n)))
+        return [{"year": y.year, "toa_net_radiation": y.toa_net_radiation, "surface_net_heat": y.surface_net_heat,
+                 "surface_temperature": y.surface_temperature} for y in out[:n.value]]
 
     def global_means(self, member):
         t, f = ctypes.c_double(), ctypes.c_double()
