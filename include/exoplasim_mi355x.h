@@ -300,6 +300,34 @@ int exo_read_annual(exo_engine *e, int member, float *values, size_t count);
    written). */
 int exo_yearly_balances(exo_engine *e, int member, exo_yearly_balance *out,
                         int max, int *count);
+/* The month being summed (GcmAverager.current, averages/index.ts:71-79):
+   its calendar month, samples and last step (samples = 0 when none is
+   open) and, when sum is not NULL, its FP64 sum (`count` doubles). With
+   the closed months and the yearly balances it is the averager a
+   checkpoint carries (GcmRunCheckpoint.averager, runner/types.ts:28-32). */
+int exo_read_accumulator(exo_engine *e, int member, exo_month_info *info,
+                         double *sum, size_t count);
+/* Replaces a member's averages with a saved averager (GcmRunRequest.
+   averager, runner/types.ts:58-59; gcm.worker.ts:197): n_months closed
+   months (oldest first, <= EXO_HISTORY_MONTHS, `floats` values each, laid
+   out one after another), n_years yearly balances, and the open month
+   (current = NULL or current->samples = 0: none open; else current_sum
+   holds `floats` doubles). Starts averaging when it is off. */
+int exo_load_averager(exo_engine *e, int member, const exo_month_info *months,
+                      const float *month_values, int n_months,
+                      const exo_yearly_balance *years, int n_years,
+                      const exo_month_info *current, const double *current_sum,
+                      size_t floats);
+#define EXO_CLIMATE_YEARS 10
+/* The climatology of GCM_AVERAGES.view (averages/index.ts:177-222): for
+   calendar month `month` (1..12), the mean over the last EXO_CLIMATE_YEARS
+   complete calendar years held (twelve distinct months of one year) of
+   that month's mean, summed in FP64 oldest year first and rounded to FP32.
+   *years = the years averaged (0: no complete year, nothing written);
+   info = that month's entry in the latest of them. */
+int exo_read_climatology(exo_engine *e, int member, int month,
+                         exo_month_info *info, float *values, size_t count,
+                         int *years);
 /* GCM_BALANCE.assess over yearly balances, oldest first (host only). */
 int exo_balance_assess(const exo_yearly_balance *years, int count,
                        exo_balance_assessment *out);

@@ -12,6 +12,10 @@ with it; nothing shipped imports it).
   heat and surface temperature). annual() is :103-137 (a Float32Array
   accumulating month / 12; the diurnal temperature range as the months'
   daily ranges, fields/index.ts:492-497).
+* climatology is GCM_AVERAGES.view's climatology (:177-222): the complete
+  calendar years held (twelve distinct months of one year), the last
+  CLIMATE_YEARS of them, each calendar month's mean over those years
+  (FP64 sum oldest year first, the mean rounded to FP32).
 * global_mean is GCM_GRID.globalMean (grid/index.ts:99-111): latitude
   weights, in FP64, rows in order.
 * assess is GCM_BALANCE.assess (balance/index.ts:6-68): ExoPlaSim's
@@ -22,7 +26,8 @@ import math
 import numpy as np
 
 MONTHS_PER_YEAR = 12
-HISTORY_MONTHS = 132
+CLIMATE_YEARS = 10
+HISTORY_MONTHS = (CLIMATE_YEARS + 1) * MONTHS_PER_YEAR
 SURFACE_TEMPERATURE, TOA_NET_RADIATION, SURFACE_NET_HEAT = 0, 16, 17
 DIURNAL_RANGE, DAILY_RANGE_SUM, DAILY_RANGE_DAYS = 37, 38, 39
 
@@ -98,6 +103,28 @@ class MonthlyAverager:
         values[DAILY_RANGE_DAYS] = 1.0
         values[DIURNAL_RANGE] = rng
         return values
+
+
+def climatology(months):
+    """months: [(info dict, FP32 mean)] oldest first. Returns (years,
+    [(info, FP32 mean)] for calendar months 1..12) or (0, [])."""
+    grouped = {}
+    for info, mean in months:
+        grouped.setdefault(info["year"], []).append((info, mean))
+    complete = [held for held in grouped.values()
+                if len(held) == MONTHS_PER_YEAR and len({i["month"] for i, _ in held}) == MONTHS_PER_YEAR]
+    complete.sort(key=lambda held: held[0][0]["year"])
+    complete = complete[-CLIMATE_YEARS:]
+    if not complete:
+        return 0, []
+    out = []
+    for month in range(1, MONTHS_PER_YEAR + 1):
+        samples = [next(m for m in held if m[0]["month"] == month) for held in complete]
+        total = np.zeros(samples[-1][1].shape, dtype=np.float64)
+        for _, mean in samples:
+            total += mean.astype(np.float64)
+        out.append((samples[-1][0], (total / len(samples)).astype(np.float32)))
+    return len(complete), out
 
 
 RUNNING_MEAN_YEARS, SLOPE_WINDOW, SLOPES_AVERAGED = 10, 5, 30

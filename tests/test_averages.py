@@ -113,3 +113,25 @@ def test_averaging_needs_a_device_engine(exo):
     assert host.month_count(m) == 0
     assert host.yearly_balances(m) == []
     host.close()
+
+
+def test_oracle_climatology_closed_forms():
+    # constant months worth 100 * year + month: the climatology of month k
+    # over the last ten complete years is 100 * (mean of those years) + k
+    nh = 6
+    months = []
+    for year in range(3, 16):
+        for month in range(1, 13):
+            months.append(({"year": year, "month": month, "samples": 1, "step": 12 * year + month},
+                           np.full((2, nh), 100.0 * year + month, dtype=np.float32)))
+    # year 16 holds twelve months with one repeated: not complete
+    for month in [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 11]:
+        months.append(({"year": 16, "month": month, "samples": 1, "step": 0},
+                       np.full((2, nh), -1.0, dtype=np.float32)))
+    years, climate = oracle_avg.climatology(months[-oracle_avg.HISTORY_MONTHS:])
+    # the history keeps 132 months: years 6..15 complete plus year 16's twelve
+    assert years == 10
+    for k, (info, mean) in enumerate(climate, start=1):
+        assert info["year"] == 15 and info["month"] == k
+        np.testing.assert_array_equal(mean, np.float32(100.0 * np.mean(np.arange(6, 16)) + k))
+    assert oracle_avg.climatology(months[:11]) == (0, [])

@@ -12,6 +12,11 @@ FP32 snapshots in the same order:
   FP64 and rounded to FP32 month by month);
 * the yearly balance (global means of the annual mean) agrees to 1e-12:
   the device sums its points in a tree, the oracle row by row.
+
+A saved averager (the checkpoint's: open month sum, closed months, yearly
+balances) loads back unchanged, and averaging continues from it as the
+oracle continues from the same state; a loaded history's climatology
+equals the oracle's bit for bit.
 """
 import numpy as np
 import pytest
@@ -60,8 +65,109 @@ def test_device_averages_match_the_oracle(exo):
                 assert got[key] == pytest.approx(want[key], rel=1e-12, abs=1e-12), key
         assert 150 < years[0]["surface_temperature"] < 400
         assert exo.balance_assess(years)["status"] == "spinning-up"
+        # one complete year: its climatology is that year's months
+        n_years, climate = oracle_avg.climatology(ref.months)
+        for month in (1, 12):
+            info, mean, count = eng.read_climatology(m, month)
+            assert count == n_years == 1 and info == climate[month - 1][0]
+            np.testing.assert_array_equal(mean, climate[month - 1][1].reshape(mean.shape))
         # stopping frees the history
         eng.set_averaging(m, False)
         assert eng.month_count(m) == 0
+    finally:
+        eng.close()
+
+
+def _run_days(eng, m, days, every, ref=None):
+    """Steps `days` publishing intervals; with ref, each new snapshot also
+    goes to the oracle (the one already in the slot was averaged before)."""
+    for _ in range(days):
+        eng.step(every)
+        if ref is not None:
+            eng.sync()
+            date = eng.member_date(m)
+            ref.add(eng.read_snapshot(m), date[0], date[1], eng.member_step(m) - 1)
+    eng.sync()
+
+
+def _same_averager(a, b):
+    assert (a["current"] is None) == (b["current"] is None)
+    if a["current"] is not None:
+        assert {k: v for k, v in a["current"].items() if k != "sum"} == \
+            {k: v for k, v in b["current"].items() if k != "sum"}
+        np.testing.assert_array_equal(a["current"]["sum"], b["current"]["sum"])
+    assert len(a["months"]) == len(b["months"])
+    for (ia, va), (ib, vb) in zip(a["months"], b["months"]):
+        assert ia == ib
+        np.testing.assert_array_equal(va, vb)
+    assert a["years"] == b["years"]
+
+
+def test_saved_averager_loads_and_continues(exo):
+    cfg = exo.default_config(truncation=21, levels=5, timestep_minutes=45)
+    eng = exo.Engine(0, 1)
+    try:
+        m = eng.add_member(cfg)
+        every = 24 * 60 // 45
+        eng.set_publish_every(every)
+        eng.set_averaging(m)
+        _, nlat, _, _ = eng.dims(m)
+        while eng.member_step(m) % every != 1:
+            eng.step(1)
+        _run_days(eng, m, 45, every)
+        saved = eng.averager(m)
+        assert saved["current"] is not None and len(saved["months"]) >= 1
+        print(f"saved: {len(saved['months'])} months, open month {saved['current']['month']} "
+              f"with {saved['current']['samples']} samples")
+        eng.set_averaging(m, False)
+        eng.load_averager(m, saved)
+        _same_averager(eng.averager(m), saved)
+        # the oracle continues from the same state
+        ref = oracle_avg.MonthlyAverager(np.polynomial.legendre.leggauss(nlat)[1])
+        cur = saved["current"]
+        ref.current = [cur["year"], cur["month"], cur["sum"].copy(), cur["samples"], cur["step"]]
+        ref.months = list(saved["months"])
+        _run_days(eng, m, 40, every, ref)
+        assert eng.month_count(m) == len(ref.months) > len(saved["months"])
+        for i, (info, mean) in enumerate(ref.months):
+            got_info, got = eng.read_month(m, i)
+            assert got_info == info, i
+            np.testing.assert_array_equal(got.reshape(-1), mean.reshape(-1))
+        info, total = eng.read_accumulator(m)
+        assert info == {"year": ref.current[0], "month": ref.current[1], "samples": ref.current[3],
+                        "step": ref.current[4]}
+        np.testing.assert_array_equal(total.reshape(-1), ref.current[2].reshape(-1))
+    finally:
+        eng.close()
+
+
+def test_loaded_history_climatology(exo):
+    cfg = exo.default_config(truncation=21, levels=5)
+    eng = exo.Engine(0, 1)
+    try:
+        m = eng.add_member(cfg)
+        _, n = eng.slot(m, "snapshot")
+        nhor = eng.dims(m)[1] * eng.dims(m)[2]
+        rng = np.random.default_rng(7)
+        months = []
+        for year in range(2, 15):  # the history keeps years 4..14; the climate the last ten
+            for month in range(1, 13):
+                months.append(({"year": year, "month": month, "samples": 30, "step": 360 * year + 30 * month},
+                               rng.normal(280.0, 30.0, size=(n // nhor, nhor)).astype(np.float32)))
+        months = months[-132:]
+        years = [{"year": y, "toa_net_radiation": 0.5 * y, "surface_net_heat": -0.25 * y,
+                  "surface_temperature": 280.0 + y} for y in range(4, 14)]
+        eng.load_averager(m, {"current": None, "months": months, "years": years})
+        assert eng.read_accumulator(m) == (None, None)
+        assert eng.yearly_balances(m) == years
+        n_years, climate = oracle_avg.climatology(months)
+        print(f"{len(months)} months held, {n_years} complete years")
+        assert n_years == 10 and climate[0][0]["year"] == 14
+        for month in range(1, 13):
+            info, mean, count = eng.read_climatology(m, month)
+            assert count == n_years and info == climate[month - 1][0]
+            np.testing.assert_array_equal(mean, climate[month - 1][1])
+        with pytest.raises(exo.ExoError):
+            eng.load_averager(m, {"current": None, "months": months + months[:1], "years": []})
     finally:
         eng.close()

@@ -88,3 +88,58 @@ function done(u) {
     assert all(s["finite"] for s in out["snaps"])
     assert out["snaps"][0]["n"] == 101 * 2048
     assert out["restart"] > 2429594 * 4
+
+
+@pytest.mark.gpu
+def test_gcm_run_averages_and_averager_checkpoint():
+    """onAverages reports newly closed months and the yearly balances
+    (gcm.worker.ts:324-341); the checkpoint's averager restarts a run
+    that carries on averaging from it."""
+    out = _node("""
+const {start} = require('./runner.js')
+const config = {truncation: 21, layers: 5, timestepMinutes: 45}
+const updates = []
+let first = null
+const summary = (a) => ({months: a.months.map((m) => [m.year, m.month, m.snapshot.step, m.snapshot.values.length]),
+  current: a.current && {year: a.current.year, month: a.current.month, samples: a.current.samples,
+                         sum: a.current.sum.length, f64: a.current.sum instanceof Float64Array,
+                         step: a.current.last.step},
+  years: a.years.length})
+const run1 = start({config, publishEvery: 32, restart: null, startPaused: false,
+  onStatus: (u) => { if (u.status === 'failed') { console.log(JSON.stringify({failed: u.message})) } },
+  onDevice: () => {}, onSnapshot: () => {},
+  onAverages: (u) => {
+    updates.push({months: u.months.map((m) => [m.year, m.month, m.snapshot.step, m.snapshot.date.day]),
+                  years: u.years.length})
+    if (updates.length === 2) run1.checkpoint()
+  },
+  onCheckpoint: (c) => { first = c; run1.stop(); second(c) }})
+function second(saved) {
+  const run2 = start({config, publishEvery: 32, restart: saved.restart, averager: saved.averager,
+    startPaused: false,
+    onStatus: (u) => { if (u.status === 'failed') { console.log(JSON.stringify({failed: u.message})) } },
+    onDevice: () => {}, onAverages: () => {},
+    onSnapshot: () => run2.checkpoint(),
+    onCheckpoint: (c) => {
+      run2.stop()
+      console.log(JSON.stringify({updates, first: summary(first.averager), second: summary(c.averager)}))
+    }})
+}
+""", timeout=300)
+    assert "failed" not in out, out
+    updates = out["updates"]
+    assert [len(u["months"]) for u in updates] == [1, 1]
+    (y1, m1, s1, d1), (y2, m2, s2, d2) = updates[0]["months"][0], updates[1]["months"][0]
+    assert (y1, m1 + 1) == (y2, m2) and s2 > s1 and d1 == d2 == 1
+    first, second = out["first"], out["second"]
+    assert [m[:3] for m in first["months"]] == [m[:3] for m in updates[0]["months"] + updates[1]["months"]]
+    assert all(m[3] == 76 * 2048 for m in first["months"])
+    cur = first["current"]
+    assert cur["f64"] and cur["sum"] == 76 * 2048 and cur["samples"] >= 1
+    # one more snapshot after the restart: added to the open month, or it closes that month
+    if second["current"]["month"] == cur["month"]:
+        assert second["months"] == first["months"]
+        assert second["current"]["samples"] == cur["samples"] + 1
+    else:
+        assert second["months"][:-1] == first["months"][-len(second["months"]) + 1:]
+        assert second["months"][-1][:2] == [cur["year"], cur["month"]]

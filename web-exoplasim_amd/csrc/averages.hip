@@ -18,6 +18,12 @@
 //                 fields, then their global means (FP64)
 //   k_avg_annual  GCM_AVERAGES.annual: every field's annual mean, the
 //                 diurnal range as the mean of the months' daily ranges
+//   k_avg_climate one calendar month's climatology over the last complete
+//                 years (GCM_AVERAGES.view, averages/index.ts:177-222)
+//
+// A checkpoint's averager (the open month's FP64 sum, the closed months and
+// the yearly balances) is read with exo_read_accumulator / exo_read_month /
+// exo_yearly_balances and put back with exo_load_averager.
 //
 // Sums keep the reference's arithmetic: FP64 month sums over FP32
 // snapshots, the mean rounded to FP32; annual means accumulate month / 12
@@ -25,6 +31,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <map>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -132,6 +140,23 @@ extern "C" __global__ __launch_bounds__(256) void k_avg_annual(exo::YearLaunch y
   }
 }
 
+// One calendar month's climatology: the month's means of `years` years
+// (history slots, oldest year first) summed in FP64, the sum / years
+// rounded to FP32.
+struct ClimateSlots {
+  int years;
+  int slot[EXO_CLIMATE_YEARS];
+};
+
+extern "C" __global__ __launch_bounds__(256) void k_avg_climate(const float* __restrict__ history, size_t floats,
+                                                              ClimateSlots c, float* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= floats) return;
+  double sum = 0.0;
+  for (int y = 0; y < c.years; y++) sum += (double)history[(size_t)c.slot[y] * floats + e];
+  out[e] = (float)(sum / c.years);
+}
+
 namespace exo {
 
 size_t Averager::slot_of(size_t index) const {
@@ -222,6 +247,23 @@ MemberDev* averaging_member(exo_engine* h, int id) {
   return h->e.members[id].get();
 }
 
+// A cleared averager on the member's device (current device set by the
+// caller): month sum, month history, yearly balances and the FP64 Gaussian
+// weights the balance's global means use (GCM_GRID.globalMean,
+// grid/index.ts:99-111), not the FP32 table. Throws on a failed allocation.
+std::unique_ptr<exo::Averager> new_averager(MemberDev* m) {
+  auto v = std::make_unique<exo::Averager>();
+  v->floats = m->host.layout.floats[SLOT_snapshot];
+  HIP_OR_THROW(hipMalloc(&v->d_sum, v->floats * sizeof(double)));
+  HIP_OR_THROW(hipMalloc(&v->d_months, (size_t)EXO_HISTORY_MONTHS * v->floats * sizeof(float)));
+  HIP_OR_THROW(hipMalloc(&v->d_years, 3 * sizeof(double) * exo::kAvgYearCapacity));
+  const size_t nhor = (size_t)m->host.dims.nlat * m->host.dims.nlon;
+  HIP_OR_THROW(hipMalloc(&v->d_weights, nhor * sizeof(double)));
+  HIP_OR_THROW(hipMemcpy(v->d_weights, &m->host.tables[m->host.tlayout.offset[TAB_zgw]], nhor * sizeof(double),
+                         hipMemcpyHostToDevice));
+  return v;
+}
+
 }  // namespace
 
 extern "C" int exo_set_averaging(exo_engine* h, int id, int enable) {
@@ -238,18 +280,8 @@ extern "C" int exo_set_averaging(exo_engine* h, int id, int enable) {
   try {
     HIP_OR_THROW(hipSetDevice(h->e.device));
     HIP_OR_THROW(hipStreamSynchronize(h->e.stream));
-    auto v = std::make_unique<exo::Averager>();
-    v->floats = m->host.layout.floats[SLOT_snapshot];
-    HIP_OR_THROW(hipMalloc(&v->d_sum, v->floats * sizeof(double)));
-    HIP_OR_THROW(hipMalloc(&v->d_months, (size_t)EXO_HISTORY_MONTHS * v->floats * sizeof(float)));
-    HIP_OR_THROW(hipMalloc(&v->d_years, 3 * sizeof(double) * exo::kAvgYearCapacity));
-    // the balance's global means weigh points by the FP64 Gaussian weights
-    // (GCM_GRID.globalMean, grid/index.ts:99-111), not the FP32 table
-    const int nhor = m->host.dims.nlat * m->host.dims.nlon;
-    HIP_OR_THROW(hipMalloc(&v->d_weights, (size_t)nhor * sizeof(double)));
-    HIP_OR_THROW(hipMemcpy(v->d_weights, &m->host.tables[m->host.tlayout.offset[TAB_zgw]], (size_t)nhor * sizeof(double),
-                           hipMemcpyHostToDevice));
-    m->avg = std::move(v);
+    m->avg.reset();
+    m->avg = new_averager(m);
   } catch (const std::exception& ex) {
     return exo::set_error(EXO_E_DEVICE, ex.what());
   }
@@ -330,6 +362,117 @@ extern "C" int exo_yearly_balances(exo_engine* h, int id, exo_yearly_balance* ou
     return exo::set_error(EXO_E_DEVICE, "reading the yearly balances failed");
   for (int i = 0; i < n; i++)
     out[i] = exo_yearly_balance{v.years[i], host[3 * i], host[3 * i + 1], host[3 * i + 2]};
+  return EXO_OK;
+}
+
+extern "C" int exo_read_accumulator(exo_engine* h, int id, exo_month_info* info, double* sum, size_t count) {
+  MemberDev* m = averaging_member(h, id);
+  if (!m || !info) return exo::set_error(EXO_E_ARG, "no member " + std::to_string(id));
+  if (!m->avg) return exo::set_error(EXO_E_STATE, "member " + std::to_string(id) + " is not averaging");
+  const exo::Averager& v = *m->avg;
+  *info = v.open ? exo_month_info{v.year, v.month, v.samples, v.step} : exo_month_info{0, 0, 0, 0};
+  if (!sum || !v.open) return EXO_OK;
+  if (count != v.floats) return exo::set_error(EXO_E_ARG, "the month sum has " + std::to_string(v.floats) + " values");
+  if (hipSetDevice(h->e.device) != hipSuccess ||
+      hipMemcpyAsync(sum, v.d_sum, count * sizeof(double), hipMemcpyDeviceToHost, h->e.stream) != hipSuccess ||
+      hipStreamSynchronize(h->e.stream) != hipSuccess)
+    return exo::set_error(EXO_E_DEVICE, "reading the month sum failed");
+  return EXO_OK;
+}
+
+extern "C" int exo_load_averager(exo_engine* h, int id, const exo_month_info* months, const float* month_values,
+                                 int n_months, const exo_yearly_balance* years, int n_years,
+                                 const exo_month_info* current, const double* current_sum, size_t floats) {
+  MemberDev* m = averaging_member(h, id);
+  if (!m) return exo::set_error(EXO_E_ARG, "no member " + std::to_string(id));
+  if (h->e.device < 0) return exo::set_error(EXO_E_STATE, "a host-only engine has no device averages");
+  const size_t want = m->host.layout.floats[SLOT_snapshot];
+  const bool open = current && current->samples > 0;
+  if (floats != want) return exo::set_error(EXO_E_ARG, "a month holds " + std::to_string(want) + " values");
+  if (n_months < 0 || n_months > EXO_HISTORY_MONTHS || (n_months > 0 && (!months || !month_values)))
+    return exo::set_error(EXO_E_ARG, "at most " + std::to_string(EXO_HISTORY_MONTHS) + " closed months");
+  if (n_years < 0 || n_years > exo::kAvgYearCapacity || (n_years > 0 && !years))
+    return exo::set_error(EXO_E_ARG, "at most " + std::to_string(exo::kAvgYearCapacity) + " yearly balances");
+  if (open && !current_sum) return exo::set_error(EXO_E_ARG, "an open month needs its sum");
+  try {
+    HIP_OR_THROW(hipSetDevice(h->e.device));
+    HIP_OR_THROW(hipStreamSynchronize(h->e.stream));
+    m->avg.reset();
+    auto v = new_averager(m);
+    if (n_months > 0)
+      HIP_OR_THROW(hipMemcpy(v->d_months, month_values, (size_t)n_months * floats * sizeof(float), hipMemcpyHostToDevice));
+    for (int i = 0; i < n_months; i++) v->months.push_back(months[i]);
+    v->head = n_months % EXO_HISTORY_MONTHS;
+    std::vector<double> balances(3 * (size_t)n_years);
+    for (int i = 0; i < n_years; i++) {
+      balances[3 * i] = years[i].toa_net_radiation;
+      balances[3 * i + 1] = years[i].surface_net_heat;
+      balances[3 * i + 2] = years[i].surface_temperature;
+      v->years.push_back(years[i].year);
+    }
+    if (n_years > 0)
+      HIP_OR_THROW(hipMemcpy(v->d_years, balances.data(), balances.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (open) {
+      HIP_OR_THROW(hipMemcpy(v->d_sum, current_sum, floats * sizeof(double), hipMemcpyHostToDevice));
+      v->open = true;
+      v->year = current->year;
+      v->month = current->month;
+      v->samples = current->samples;
+      v->step = current->step;
+    }
+    m->avg = std::move(v);
+  } catch (const std::exception& ex) {
+    m->avg.reset();
+    return exo::set_error(EXO_E_DEVICE, ex.what());
+  }
+  return EXO_OK;
+}
+
+extern "C" int exo_read_climatology(exo_engine* h, int id, int month, exo_month_info* info, float* values,
+                                    size_t count, int* years) {
+  MemberDev* m = averaging_member(h, id);
+  if (!m || !years || month < 1 || month > EXO_MONTHS_PER_YEAR)
+    return exo::set_error(EXO_E_ARG, "no member " + std::to_string(id) + " or month " + std::to_string(month));
+  if (!m->avg) return exo::set_error(EXO_E_STATE, "member " + std::to_string(id) + " is not averaging");
+  const exo::Averager& v = *m->avg;
+  // complete years: twelve closed months of one calendar year, all
+  // distinct (averages/index.ts:178-190), oldest first, the last ten
+  std::map<int, std::vector<size_t>> by_year;
+  for (size_t i = 0; i < v.months.size(); i++) by_year[v.months[i].year].push_back(i);
+  std::vector<size_t> chosen;  // history index of `month` in each complete year
+  for (const auto& [year, held] : by_year) {
+    if (held.size() != EXO_MONTHS_PER_YEAR) continue;
+    unsigned seen = 0;
+    size_t at = 0;
+    for (size_t i : held) {
+      seen |= 1u << v.months[i].month;
+      if (v.months[i].month == month) at = i;
+    }
+    if (seen == 0x1FFEu) chosen.push_back(at);
+  }
+  if (chosen.size() > EXO_CLIMATE_YEARS) chosen.erase(chosen.begin(), chosen.end() - EXO_CLIMATE_YEARS);
+  *years = (int)chosen.size();
+  if (chosen.empty()) return EXO_OK;
+  if (info) *info = v.months[chosen.back()];
+  if (!values) return EXO_OK;
+  if (count != v.floats) return exo::set_error(EXO_E_ARG, "a month needs " + std::to_string(v.floats) + " values");
+  exo::Engine& e = h->e;
+  try {
+    HIP_OR_THROW(hipSetDevice(e.device));
+    ClimateSlots c{};
+    c.years = (int)chosen.size();
+    for (int y = 0; y < c.years; y++) c.slot[y] = (int)v.slot_of(chosen[y]);
+    float* d_out = nullptr;
+    HIP_OR_THROW(hipMallocAsync((void**)&d_out, count * sizeof(float), e.stream));
+    hipLaunchKernelGGL(k_avg_climate, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, e.stream, v.d_months,
+                       v.floats, c, d_out);
+    HIP_OR_THROW(hipGetLastError());
+    HIP_OR_THROW(hipMemcpyAsync(values, d_out, count * sizeof(float), hipMemcpyDeviceToHost, e.stream));
+    HIP_OR_THROW(hipFreeAsync(d_out, e.stream));
+    HIP_OR_THROW(hipStreamSynchronize(e.stream));
+  } catch (const std::exception& ex) {
+    return exo::set_error(EXO_E_DEVICE, ex.what());
+  }
   return EXO_OK;
 }
 

@@ -78,6 +78,10 @@ class ExoServeOptions(ctypes.Structure):
                 ("prepare_threads", ctypes.c_int)]
 
 
+def _month_info(info):
+    return {"year": info.year, "month": info.month, "samples": info.samples, "step": info.step}
+
+
 class ExoMonthInfo(ctypes.Structure):
     """exo_month_info: the calendar month a closed monthly mean covers."""
 
@@ -155,6 +159,10 @@ def library():
         "exo_read_annual": (I, [P, I, F, S]),
         "exo_yearly_balances": (I, [P, I, ctypes.POINTER(ExoYearlyBalance), I, ctypes.POINTER(I)]),
         "exo_balance_assess": (I, [ctypes.POINTER(ExoYearlyBalance), I, ctypes.POINTER(ExoBalanceAssessment)]),
+        "exo_read_accumulator": (I, [P, I, ctypes.POINTER(ExoMonthInfo), D, S]),
+        "exo_load_averager": (I, [P, I, ctypes.POINTER(ExoMonthInfo), F, I, ctypes.POINTER(ExoYearlyBalance), I,
+                                  ctypes.POINTER(ExoMonthInfo), D, S]),
+        "exo_read_climatology": (I, [P, I, I, ctypes.POINTER(ExoMonthInfo), F, S, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -408,8 +416,7 @@ class Engine:
         _check(self.lib.exo_read_month(self.h, member, index, ctypes.byref(info),
                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
         nhor = self.dims(member)[1] * self.dims(member)[2]
-        return ({"year": info.year, "month": info.month, "samples": info.samples, "step": info.step},
-                out.reshape(-1, nhor))
+        return _month_info(info), out.reshape(-1, nhor)
 
     def read_annual(self, member):
         _, n = self.slot(member, "snapshot")
@@ -425,6 +432,68 @@ class Engine:
         _check(self.lib.exo_yearly_balances(self.h, member, out, n.value, ctypes.byref(n)))
         return [{"year": y.year, "toa_net_radiation": y.toa_net_radiation, "surface_net_heat": y.surface_net_heat,
                  "surface_temperature": y.surface_temperature} for y in out[:n.value]]
+
+    def read_accumulator(self, member):
+        """(info dict, FP64 sum (fields, NHOR)) of the month being summed, or
+        (None, None) when none is open (GcmAverager.current)."""
+        _, n = self.slot(member, "snapshot")
+        info = ExoMonthInfo()
+        _check(self.lib.exo_read_accumulator(self.h, member, ctypes.byref(info), None, 0))
+        if info.samples == 0:
+            return None, None
+        out = np.empty(n, dtype=np.float64)
+        _check(self.lib.exo_read_accumulator(self.h, member, ctypes.byref(info),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        nhor = self.dims(member)[1] * self.dims(member)[2]
+        return _month_info(info), out.reshape(-1, nhor)
+
+    def averager(self, member):
+        """The member's whole averager, as a checkpoint carries it
+        (GcmAverager: current, months, years)."""
+        info, total = self.read_accumulator(member)
+        months = [self.read_month(member, i) for i in range(self.month_count(member))]
+        return {"current": None if info is None else dict(info, sum=total),
+                "months": months, "years": self.yearly_balances(member)}
+
+    def load_averager(self, member, averager):
+        """Replaces the member's averages with `averager` (the dict
+        averager() returns)."""
+        _, n = self.slot(member, "snapshot")
+        months = averager.get("months") or []
+        years = averager.get("years") or []
+        current = averager.get("current")
+        infos = (ExoMonthInfo * max(1, len(months)))()
+        values = np.zeros((max(1, len(months)), n), dtype=np.float32)
+        for i, (info, mean) in enumerate(months):
+            infos[i] = ExoMonthInfo(info["year"], info["month"], info["samples"], info["step"])
+            values[i] = np.asarray(mean, dtype=np.float32).reshape(-1)
+        balances = (ExoYearlyBalance * max(1, len(years)))()
+        for i, y in enumerate(years):
+            balances[i] = ExoYearlyBalance(y["year"], y["toa_net_radiation"], y["surface_net_heat"],
+                                           y["surface_temperature"])
+        cur = ExoMonthInfo()
+        total = np.zeros(n, dtype=np.float64)
+        if current is not None:
+            cur = ExoMonthInfo(current["year"], current["month"], current["samples"], current["step"])
+            total = np.ascontiguousarray(np.asarray(current["sum"], dtype=np.float64).reshape(-1))
+        _check(self.lib.exo_load_averager(
+            self.h, member, infos, values.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(months),
+            balances, len(years), ctypes.byref(cur), total.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+
+    def read_climatology(self, member, month):
+        """(info dict, mean (fields, NHOR), years) of calendar month `month`
+        over the last complete years held; (None, None, 0) with none."""
+        _, n = self.slot(member, "snapshot")
+        info = ExoMonthInfo()
+        years = ctypes.c_int()
+        out = np.empty(n, dtype=np.float32)
+        _check(self.lib.exo_read_climatology(self.h, member, month, ctypes.byref(info),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n,
+                                             ctypes.byref(years)))
+        if years.value == 0:
+            return None, None, 0
+        nhor = self.dims(member)[1] * self.dims(member)[2]
+        return _month_info(info), out.reshape(-1, nhor), years.value
 
     def global_means(self, member):
         t, f = ctypes.c_double(), ctypes.c_double()

@@ -64,6 +64,43 @@ napi_value number(napi_env env, double x) {
   return r;
 }
 
+napi_value double_array(napi_env env, const std::vector<double>& data) {
+  napi_value buf, arr;
+  void* p = nullptr;
+  if (napi_create_arraybuffer(env, data.size() * sizeof(double), &p, &buf) != napi_ok) return nullptr;
+  if (!data.empty()) std::memcpy(p, data.data(), data.size() * sizeof(double));
+  if (napi_create_typedarray(env, napi_float64_array, data.size(), buf, 0, &arr) != napi_ok) return nullptr;
+  return arr;
+}
+
+// The data of a typed array of `type` (Float32Array / Float64Array), or
+// nullptr with a TypeError thrown.
+void* typed_data(napi_env env, napi_value v, napi_typedarray_type type, size_t& length) {
+  bool is_typed = false;
+  napi_typedarray_type t;
+  void* p = nullptr;
+  napi_value buf;
+  size_t offset = 0;
+  if (napi_is_typedarray(env, v, &is_typed) != napi_ok || !is_typed ||
+      napi_get_typedarray_info(env, v, &t, &length, &p, &buf, &offset) != napi_ok || t != type) {
+    napi_throw_type_error(env, nullptr, type == napi_float32_array ? "expected a Float32Array" : "expected a Float64Array");
+    return nullptr;
+  }
+  return p;
+}
+
+double number_of(napi_env env, napi_value obj, const char* key) {
+  napi_value v;
+  double x = 0;
+  if (napi_get_named_property(env, obj, key, &v) == napi_ok) napi_get_value_double(env, v, &x);
+  return x;
+}
+
+bool is_nullish(napi_env env, napi_value v) {
+  napi_valuetype t;
+  return napi_typeof(env, v, &t) != napi_ok || t == napi_null || t == napi_undefined;
+}
+
 napi_value float_array(napi_env env, const std::vector<float>& data) {
   napi_value buf, arr;
   void* p = nullptr;
@@ -330,6 +367,234 @@ napi_value restore(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---- averages and balance (GCM_AVERAGES / GCM_BALANCE) ----------------
+
+napi_value set_averaging(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  bool on = false;
+  napi_get_value_bool(env, a.v[2], &on);
+  EXO_OK_OR_THROW(exo_set_averaging(e, int_of(env, a.v[1]), on ? 1 : 0));
+  return nullptr;
+}
+
+napi_value month_count(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  int n = 0;
+  EXO_OK_OR_THROW(exo_month_count(e, int_of(env, a.v[1]), &n));
+  return number(env, n);
+}
+
+bool set_month_info(napi_env env, napi_value obj, const exo_month_info& m) {
+  return napi_set_named_property(env, obj, "year", number(env, m.year)) == napi_ok &&
+         napi_set_named_property(env, obj, "month", number(env, m.month)) == napi_ok &&
+         napi_set_named_property(env, obj, "samples", number(env, m.samples)) == napi_ok &&
+         napi_set_named_property(env, obj, "step", number(env, (double)m.step)) == napi_ok;
+}
+
+exo_month_info month_info_of(napi_env env, napi_value obj) {
+  return exo_month_info{(int)number_of(env, obj, "year"), (int)number_of(env, obj, "month"),
+                        (int)number_of(env, obj, "samples"), (int64_t)number_of(env, obj, "step")};
+}
+
+size_t snapshot_floats(napi_env env, exo_engine* e, int id) {
+  size_t off = 0, n = 0;
+  if (exo_slot(e, id, "snapshot", &off, &n) != EXO_OK) return 0;
+  return n;
+}
+
+// {year, month, samples, step, values: Float32Array} of closed month index;
+// readMonth(engine, id, index, false) leaves the values out.
+napi_value read_month(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  bool with_values = true;
+  if (a.n > 3) napi_get_value_bool(env, a.v[3], &with_values);
+  std::vector<float> out(with_values ? snapshot_floats(env, e, id) : 0);
+  exo_month_info m{};
+  EXO_OK_OR_THROW(exo_read_month(e, id, int_of(env, a.v[2]), &m, with_values ? out.data() : nullptr, out.size()));
+  napi_value obj;
+  NAPI_OK(napi_create_object(env, &obj));
+  if (!set_month_info(env, obj, m)) return nullptr;
+  if (with_values) NAPI_OK(napi_set_named_property(env, obj, "values", float_array(env, out)));
+  return obj;
+}
+
+napi_value read_annual(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  std::vector<float> out(snapshot_floats(env, e, id));
+  EXO_OK_OR_THROW(exo_read_annual(e, id, out.data(), out.size()));
+  return float_array(env, out);
+}
+
+napi_value balance_object(napi_env env, const exo_yearly_balance& y) {
+  napi_value obj;
+  if (napi_create_object(env, &obj) != napi_ok) return nullptr;
+  napi_set_named_property(env, obj, "year", number(env, y.year));
+  napi_set_named_property(env, obj, "toaNetRadiation", number(env, y.toa_net_radiation));
+  napi_set_named_property(env, obj, "surfaceNetHeat", number(env, y.surface_net_heat));
+  napi_set_named_property(env, obj, "surfaceTemperature", number(env, y.surface_temperature));
+  return obj;
+}
+
+// [GcmYearlyBalance] oldest first.
+napi_value yearly_balances(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  int n = 0;
+  EXO_OK_OR_THROW(exo_yearly_balances(e, id, nullptr, 0, &n));
+  std::vector<exo_yearly_balance> years(n);
+  if (n > 0) EXO_OK_OR_THROW(exo_yearly_balances(e, id, years.data(), n, &n));
+  napi_value arr;
+  NAPI_OK(napi_create_array_with_length(env, years.size(), &arr));
+  for (uint32_t i = 0; i < years.size(); i++) NAPI_OK(napi_set_element(env, arr, i, balance_object(env, years[i])));
+  return arr;
+}
+
+// null, or {year, month, samples, step, sum: Float64Array} of the open month.
+napi_value read_accumulator(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  exo_month_info m{};
+  EXO_OK_OR_THROW(exo_read_accumulator(e, id, &m, nullptr, 0));
+  napi_value obj;
+  if (m.samples == 0) {
+    NAPI_OK(napi_get_null(env, &obj));
+    return obj;
+  }
+  std::vector<double> sum(snapshot_floats(env, e, id));
+  EXO_OK_OR_THROW(exo_read_accumulator(e, id, &m, sum.data(), sum.size()));
+  NAPI_OK(napi_create_object(env, &obj));
+  if (!set_month_info(env, obj, m)) return nullptr;
+  NAPI_OK(napi_set_named_property(env, obj, "sum", double_array(env, sum)));
+  return obj;
+}
+
+// loadAverager(engine, id, months, years, current): months
+// [{year, month, samples, step, values: Float32Array}], years
+// [GcmYearlyBalance], current null or {year, month, samples, step,
+// sum: Float64Array}.
+napi_value load_averager(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 5)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  const size_t floats = snapshot_floats(env, e, id);
+  uint32_t n_months = 0, n_years = 0;
+  NAPI_OK(napi_get_array_length(env, a.v[2], &n_months));
+  NAPI_OK(napi_get_array_length(env, a.v[3], &n_years));
+  std::vector<exo_month_info> months(n_months);
+  std::vector<float> values((size_t)n_months * floats);
+  for (uint32_t i = 0; i < n_months; i++) {
+    napi_value m, v;
+    NAPI_OK(napi_get_element(env, a.v[2], i, &m));
+    months[i] = month_info_of(env, m);
+    NAPI_OK(napi_get_named_property(env, m, "values", &v));
+    size_t length = 0;
+    const void* p = typed_data(env, v, napi_float32_array, length);
+    if (!p) return nullptr;
+    if (length != floats) {
+      napi_throw_range_error(env, nullptr, ("a month holds " + std::to_string(floats) + " values").c_str());
+      return nullptr;
+    }
+    std::memcpy(values.data() + (size_t)i * floats, p, floats * sizeof(float));
+  }
+  std::vector<exo_yearly_balance> years(n_years);
+  for (uint32_t i = 0; i < n_years; i++) {
+    napi_value y;
+    NAPI_OK(napi_get_element(env, a.v[3], i, &y));
+    years[i] = exo_yearly_balance{(int)number_of(env, y, "year"), number_of(env, y, "toaNetRadiation"),
+                                  number_of(env, y, "surfaceNetHeat"), number_of(env, y, "surfaceTemperature")};
+  }
+  exo_month_info current{};
+  const double* sum = nullptr;
+  if (!is_nullish(env, a.v[4])) {
+    current = month_info_of(env, a.v[4]);
+    napi_value v;
+    NAPI_OK(napi_get_named_property(env, a.v[4], "sum", &v));
+    size_t length = 0;
+    sum = static_cast<const double*>(typed_data(env, v, napi_float64_array, length));
+    if (!sum) return nullptr;
+    if (length != floats) {
+      napi_throw_range_error(env, nullptr, ("a month sum holds " + std::to_string(floats) + " values").c_str());
+      return nullptr;
+    }
+  }
+  EXO_OK_OR_THROW(exo_load_averager(e, id, months.data(), values.data(), (int)n_months, years.data(), (int)n_years,
+                                    &current, sum, floats));
+  return nullptr;
+}
+
+// null, or {year, month, samples, step, years, values: Float32Array}.
+napi_value read_climatology(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int id = int_of(env, a.v[1]);
+  std::vector<float> out(snapshot_floats(env, e, id));
+  exo_month_info m{};
+  int years = 0;
+  EXO_OK_OR_THROW(exo_read_climatology(e, id, int_of(env, a.v[2]), &m, out.data(), out.size(), &years));
+  napi_value obj;
+  if (years == 0) {
+    NAPI_OK(napi_get_null(env, &obj));
+    return obj;
+  }
+  NAPI_OK(napi_create_object(env, &obj));
+  if (!set_month_info(env, obj, m)) return nullptr;
+  NAPI_OK(napi_set_named_property(env, obj, "years", number(env, years)));
+  NAPI_OK(napi_set_named_property(env, obj, "values", float_array(env, out)));
+  return obj;
+}
+
+// GCM_BALANCE.assess(years) -> GcmBalanceAssessment.
+napi_value balance_assess(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  uint32_t n = 0;
+  NAPI_OK(napi_get_array_length(env, a.v[0], &n));
+  std::vector<exo_yearly_balance> years(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value y;
+    NAPI_OK(napi_get_element(env, a.v[0], i, &y));
+    years[i] = exo_yearly_balance{(int)number_of(env, y, "year"), number_of(env, y, "toaNetRadiation"),
+                                  number_of(env, y, "surfaceNetHeat"), number_of(env, y, "surfaceTemperature")};
+  }
+  exo_balance_assessment out{};
+  EXO_OK_OR_THROW(exo_balance_assess(years.data(), (int)n, &out));
+  const char* status = out.status == EXO_BALANCE_BALANCED   ? "balanced"
+                       : out.status == EXO_BALANCE_DRIFTING ? "drifting"
+                                                            : "spinning-up";
+  napi_value obj, s;
+  NAPI_OK(napi_create_object(env, &obj));
+  NAPI_OK(napi_create_string_utf8(env, status, NAPI_AUTO_LENGTH, &s));
+  NAPI_OK(napi_set_named_property(env, obj, "status", s));
+  NAPI_OK(napi_set_named_property(env, obj, "yearsNeeded", number(env, out.years_needed)));
+  NAPI_OK(napi_set_named_property(env, obj, "toaDriftPerYear", number(env, out.toa_drift_per_year)));
+  NAPI_OK(napi_set_named_property(env, obj, "surfaceDriftPerYear", number(env, out.surface_drift_per_year)));
+  return obj;
+}
+
 napi_value init(napi_env env, napi_value exports) {
   const std::pair<const char*, napi_callback> fns[] = {
       {"configDefault", config_default}, {"create", create},
@@ -340,6 +605,11 @@ napi_value init(napi_env env, napi_value exports) {
       {"dims", dims},                    {"readField", read_field},
       {"readSnapshot", read_snapshot},   {"globalMeans", global_means},
       {"checkpoint", checkpoint},        {"restore", restore},
+      {"setAveraging", set_averaging},   {"monthCount", month_count},
+      {"readMonth", read_month},         {"readAnnual", read_annual},
+      {"yearlyBalances", yearly_balances}, {"readAccumulator", read_accumulator},
+      {"loadAverager", load_averager},   {"readClimatology", read_climatology},
+      {"balanceAssess", balance_assess},
   };
   for (const auto& f : fns) {
     napi_value fn;

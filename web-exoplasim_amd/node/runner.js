@@ -10,6 +10,13 @@
 // The engine runs on the calling thread: each slice enqueues publishEvery
 // steps, waits for them, publishes one snapshot, then yields to the event
 // loop (setImmediate), so pause/stop/checkpoint land between slices.
+//
+// Averages follow gcm.worker.ts:197 and :324-364: every published snapshot
+// goes into the member's monthly means (on the device); a slice that closes
+// a month reports the newly closed months and the yearly balances through
+// onAverages; a checkpoint carries the averager (GcmAverager: the open
+// month, the closed months, the yearly balances), and request.averager
+// restores it.
 
 const path = require("path")
 const addon = require(path.join(__dirname, "exoplasim_napi.node"))
@@ -52,8 +59,57 @@ function engineConfig(config, planet) {
 	return merged
 }
 
+// A closed month as GcmMonthlyMean (averages/index.ts:50-69: the month's
+// first day as its date, its last snapshot's step).
+function monthlyMean(layers, m) {
+	return {
+		year: m.year,
+		month: m.month,
+		snapshot: { layers, step: m.step, date: { year: m.year, month: m.month, day: 1, hour: 0, minute: 0 }, values: m.values },
+	}
+}
+
+// GcmAverager of the member: current (the open month, with its last
+// snapshot), months, years.
+function readAverager(engine, id, layers, latest) {
+	const months = []
+	const count = addon.monthCount(engine, id)
+	for (let i = 0; i < count; i++) months.push(monthlyMean(layers, addon.readMonth(engine, id, i)))
+	const open = addon.readAccumulator(engine, id)
+	const current = open && {
+		year: open.year,
+		month: open.month,
+		sum: open.sum,
+		samples: open.samples,
+		last: latest || { layers, step: open.step, date: addon.memberDate(engine, id), values: addon.readSnapshot(engine, id) },
+	}
+	return { current, months, years: addon.yearlyBalances(engine, id) }
+}
+
+// request.averager (GcmAverager) onto the device.
+function loadAverager(engine, id, averager) {
+	const months = (averager.months || []).map((m) => ({
+		year: m.year,
+		month: m.month,
+		samples: 0,
+		step: m.snapshot.step,
+		values: m.snapshot.values,
+	}))
+	const c = averager.current
+	const current = c ? { year: c.year, month: c.month, samples: c.samples, step: c.last.step, sum: c.sum } : null
+	addon.loadAverager(engine, id, months, averager.years || [], current)
+}
+
 function start(request) {
-	const state = { paused: !!request.startPaused, stopped: false, checkpoint: false, engine: null, id: -1 }
+	const state = {
+		paused: !!request.startPaused,
+		stopped: false,
+		checkpoint: false,
+		engine: null,
+		id: -1,
+		lastClosedStep: -1, // step of the newest closed month already reported
+		latest: null, // the last published snapshot
+	}
 	const status = (s, message = "") => request.onStatus && request.onStatus({ status: s, message })
 	const fail = (error) => {
 		status("failed", error && error.message ? error.message : String(error))
@@ -62,6 +118,22 @@ function start(request) {
 	const close = () => {
 		if (state.engine) addon.destroy(state.engine)
 		state.engine = null
+	}
+
+	// Months closed since the last report, oldest first, and every yearly
+	// balance (gcm.worker.ts:324-341); nothing when no month closed.
+	const reportAverages = () => {
+		const count = addon.monthCount(state.engine, state.id)
+		const fresh = []
+		for (let i = count - 1; i >= 0; i--) {
+			const info = addon.readMonth(state.engine, state.id, i, false)
+			if (info.step <= state.lastClosedStep) break
+			fresh.unshift(i)
+		}
+		if (!fresh.length) return
+		const months = fresh.map((i) => monthlyMean(request.config.layers, addon.readMonth(state.engine, state.id, i)))
+		state.lastClosedStep = months[months.length - 1].snapshot.step
+		request.onAverages && request.onAverages({ months, years: addon.yearlyBalances(state.engine, state.id) })
 	}
 
 	const slice = () => {
@@ -76,8 +148,8 @@ function start(request) {
 				state.checkpoint = false
 				status("checkpointing")
 				const restart = addon.checkpoint(state.engine, state.id)
-				request.onCheckpoint &&
-					request.onCheckpoint({ targeting: null, restart, averager: request.averager || null })
+				const averager = readAverager(state.engine, state.id, request.config.layers, state.latest)
+				request.onCheckpoint && request.onCheckpoint({ targeting: null, restart, averager })
 				status(state.paused ? "paused" : "running")
 			}
 			if (state.paused) return
@@ -85,13 +157,14 @@ function start(request) {
 			addon.step(state.engine, every)
 			addon.sync(state.engine)
 			const values = addon.readSnapshot(state.engine, state.id)
-			request.onSnapshot &&
-				request.onSnapshot({
-					layers: request.config.layers,
-					step: addon.memberStep(state.engine, state.id),
-					date: addon.memberDate(state.engine, state.id),
-					values,
-				})
+			state.latest = {
+				layers: request.config.layers,
+				step: addon.memberStep(state.engine, state.id),
+				date: addon.memberDate(state.engine, state.id),
+				values,
+			}
+			request.onSnapshot && request.onSnapshot(state.latest)
+			reportAverages()
 			setImmediate(slice)
 		} catch (error) {
 			fail(error)
@@ -106,6 +179,12 @@ function start(request) {
 			state.id = addon.addMember(state.engine, engineConfig(request.config, request.planet), precision)
 			addon.setPublishEvery(state.engine, Math.max(1, request.publishEvery | 0))
 			if (request.restart) addon.restore(state.engine, state.id, request.restart)
+			addon.setAveraging(state.engine, state.id, true)
+			if (request.averager) {
+				loadAverager(state.engine, state.id, request.averager)
+				const held = request.averager.months || []
+				if (held.length) state.lastClosedStep = held[held.length - 1].snapshot.step
+			}
 			request.onDevice && request.onDevice("AMD Instinct MI355X (HIP, gfx950)")
 			status(state.paused ? "paused" : "running")
 			setImmediate(slice)
@@ -136,4 +215,4 @@ function start(request) {
 	}
 }
 
-module.exports = { start, engineConfig, addon }
+module.exports = { start, engineConfig, addon, readAverager, loadAverager }
