@@ -332,6 +332,29 @@ int exo_read_climatology(exo_engine *e, int member, int month,
 int exo_balance_assess(const exo_yearly_balance *years, int count,
                        exo_balance_assessment *out);
 
+/* ---- Native PlaSim restart files ------------------------------------
+   plasim_restart / plasim_status: Fortran sequential records (uint32
+   byte count, bytes, the count again), each field a 16-character name
+   record followed by its FP32 or FP64 values. Host only. */
+
+/* NATIVE_RESTART.patch (src/model/climate/gcm/checkpoint/native/
+   index.ts:3-46): copies the file's `bytes` into `out` (which may be
+   `restart` itself) and overwrites every record of each named field with
+   values[i] (counts[i] doubles, stored at the record's precision).
+   EXO_E_FORMAT with the reference's message ("Truncated native restart
+   record", "Invalid native restart record", "Invalid native restart field
+   name", "Native restart shape mismatch for <name>", "Native restart
+   fields missing: <names>"); `out` is then unspecified. */
+int exo_native_patch(const void *restart, size_t bytes, int n_fields,
+                     const char *const *names, const double *const *values,
+                     const size_t *counts, void *out);
+/* The first record of field `name`: byte offset of its values in the file
+   and their byte count (4 or 8 per value). EXO_E_FORMAT on a malformed
+   file, EXO_E_ARG when the field is absent. No counterpart in the
+   reference, which only patches; checkers read fields through it. */
+int exo_native_find(const void *restart, size_t bytes, const char *name,
+                    size_t *offset, size_t *data_bytes);
+
 #ifdef __cplusplus
 }
 #endif

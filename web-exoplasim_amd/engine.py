@@ -163,6 +163,10 @@ def library():
         "exo_load_averager": (I, [P, I, ctypes.POINTER(ExoMonthInfo), F, I, ctypes.POINTER(ExoYearlyBalance), I,
                                   ctypes.POINTER(ExoMonthInfo), D, S]),
         "exo_read_climatology": (I, [P, I, I, ctypes.POINTER(ExoMonthInfo), F, S, ctypes.POINTER(I)]),
+        "exo_native_patch": (I, [ctypes.c_char_p, S, I, ctypes.POINTER(ctypes.c_char_p),
+                                 ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(S),
+                                 ctypes.c_void_p]),
+        "exo_native_find": (I, [ctypes.c_char_p, S, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -188,6 +192,33 @@ def balance_assess(years):
     _check(lib.exo_balance_assess(arr, len(years), ctypes.byref(out)))
     return {"status": BALANCE_STATUS[out.status], "years_needed": out.years_needed,
             "toa_drift_per_year": out.toa_drift_per_year, "surface_drift_per_year": out.surface_drift_per_year}
+
+
+def native_patch(restart, fields):
+    """NATIVE_RESTART.patch: a copy of the native restart file `restart`
+    (bytes) with each named field's records overwritten by its values
+    ({name: sequence of floats}). Host only."""
+    lib = library()
+    data = bytes(restart)
+    names = list(fields)
+    arrays = [np.ascontiguousarray(np.asarray(fields[n], dtype=np.float64).reshape(-1)) for n in names]
+    c_names = (ctypes.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+    c_values = (ctypes.POINTER(ctypes.c_double) * max(1, len(names)))(
+        *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrays])
+    c_counts = (ctypes.c_size_t * max(1, len(names)))(*[a.size for a in arrays])
+    out = ctypes.create_string_buffer(len(data))
+    _check(lib.exo_native_patch(data, len(data), len(names), c_names, c_values, c_counts, out))
+    return out.raw
+
+
+def native_record(restart, name):
+    """The bytes of field `name`'s values in a native restart file (FP32 or
+    FP64 as the model that wrote it was built). Host only."""
+    lib = library()
+    data = bytes(restart)
+    off, n = ctypes.c_size_t(), ctypes.c_size_t()
+    _check(lib.exo_native_find(data, len(data), name.encode(), ctypes.byref(off), ctypes.byref(n)))
+    return data[off.value:off.value + n.value]
 
 
 def exported_symbols():
