@@ -274,8 +274,9 @@ class Stepper {
   void inv(const InvJobs& jobs) {
     // the Legendre half per operand form: plain FP32 over member groups,
     // split FP16 per member
+    // one workgroup per wavenumber pair (m, T - m) and job
     const GroupPlan pg = group_plan([](const DevMember& d) { return d.precision == 0; },
-                                    [&](const DevMember& d, int) { return (d.ntru + 1) * jobs.n; });
+                                    [&](const DevMember& d, int) { return (d.ntru + 2) / 2 * jobs.n; });
     if (pg.first[pg.n] > 0) launch(e_, k_leg_inv, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
     const LaunchPlan pl = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
     if (pl.first[pl.n] > 0) launch(e_, k_leg_inv_split, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);

@@ -109,15 +109,16 @@ __device__ __forceinline__ Col gcol(int g, int nf, int P, int field, int p, int 
 }
 
 // inverse: plain (S); dv2uv fields 0 = D, 1 = zeta, block 0 = qu, 1 = qv.
-// Column c of the chunk belongs to member c / NCm.
-__device__ __forceinline__ Col inv_col(bool winds, int blk, int c, int NC, int NCm, int P) {
+// Column c of the chunk belongs to member c / NCm; staged rows hold
+// row_floats floats.
+__device__ __forceinline__ Col inv_col(bool winds, int blk, int c, int NC, int NCm, int P, int row_floats) {
   if (c >= NC) return Col{0, 0.0f};
   const int g = c / NCm;
   c -= g * NCm;
-  if (!winds) return gcol(g, 1, P, 0, c >> 1, c & 1, 1.0f, kRawK);
+  if (!winds) return gcol(g, 1, P, 0, c >> 1, c & 1, 1.0f, row_floats);
   const int p = c >> 2, q = c & 3;
-  if (blk == 0) return gcol(g, 2, P, q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? -1.0f : 1.0f, kRawK);
-  return gcol(g, 2, P, (q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? -1.0f : 1.0f, kRawK);
+  if (blk == 0) return gcol(g, 2, P, q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? -1.0f : 1.0f, row_floats);
+  return gcol(g, 2, P, (q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? -1.0f : 1.0f, row_floats);
 }
 
 // forward: fields 0 = u (plain: the input), 1 = v, 2 = ke (qtend: s)
@@ -263,21 +264,70 @@ static_assert(kMaxLat / 2 * 64 % kGroupThreads == 0 && kFwdRows * 256 % kGroupTh
 static_assert(2 * kMaxK <= 128 && 2 * kMaxLat <= 256 && kMaxK <= 64, "batch columns");
 
 // ------------------------------------------------------------- inverse
+// A workgroup takes the wavenumber pair (m, T - m) of one job and member
+// group: K(m) + K(T - m) = T + 2 modes whatever m, so every workgroup does
+// the same work (m = T / 2 alone when T is even), and one staging round
+// serves both. The pair's table columns and inputs sit side by side, each
+// wavenumber's padded to a multiple of 4 with zeros: columns [0, Ra) for
+// m, [Ra, Ra + Rb) for T - m.
+
+constexpr int kPairK = 72;                 // Ra + Rb <= 72 at T63
+constexpr int kPairRawK = 2 * kPairK + 4;  // input row: 4 x odd floats
+constexpr int kInvGroupRows = kLegGroup * 2 * kMaxLev;  // members x fields x levels
+// row strides of 2 mod 4 floats: the 16 rows x 4 columns one MFMA step
+// reads fall in distinct banks
+__host__ __device__ constexpr int pair_stride(int r) { return r + 2; }
+constexpr int kPairAs = kMaxLat * pair_stride(kPairK);
+// Staging of a rows x C block with a fixed column per thread: thread t
+// takes column t % C and rows t / C + S j (S = 512 / C rows per pass), so
+// its addresses step by a constant; all N loads are issued before any
+// store. Table 96 x 72: 7 rows per pass, 14 each; inputs 40 x 144: 3 rows
+// per pass, 14 each.
+constexpr int kPairTabS = kGroupThreads / kPairK;
+constexpr int kPairTabN = (kMaxLat + kPairTabS - 1) / kPairTabS;
+constexpr int kPairRawS = kGroupThreads / (2 * kPairK);
+constexpr int kPairRawN = (kInvRows + kPairRawS - 1) / kPairRawS;
+
+template <int N, int C, typename Load>
+__device__ __forceinline__ void load_cols(int rows, Load load, float* v) {
+  constexpr int S = kGroupThreads / C;
+  const int c = threadIdx.x % C, r0 = threadIdx.x / C;
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int r = r0 + S * j;
+    v[j] = (r0 < S && r < rows) ? load(r, c) : 0.0f;
+  }
+}
+
+template <int N, int C, typename Store>
+__device__ __forceinline__ void store_cols(int rows, Store store, const float* v) {
+  constexpr int S = kGroupThreads / C;
+  const int c = threadIdx.x % C, r0 = threadIdx.x / C;
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int r = r0 + S * j;
+    if (r0 < S && r < rows) store(r, c, v[j]);
+  }
+}
 
 template <int P>
-__device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int m, float* As,
-                        float* raw, const float** rowp, float* rowsub) {
+__device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
+                        float* As, float* raw, const float** rowp, float* rowsub) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
-  const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
+  const int nlat = M.nlat, ntp1 = M.ntru + 1;
   const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
+  // the pair: m = pair (K = ntp1 - m modes) and T - m (K = m + 1)
+  const int ma = pair, mb = ntp1 - 1 - pair, two = mb != ma;
+  const int Ka = ntp1 - ma, Kb = two ? ntp1 - mb : 0;
+  const int wa = mode_first(ma, ntp1), wb = two ? mode_first(mb, ntp1) : 0;
+  const int Ra = (Ka + 3) & ~3, Rb = (Kb + 3) & ~3, RT = Ra + Rb;
+  const int as = pair_stride(RT);
   const bool winds = job.kind == INV_DV2UV;
   const int nf = winds ? 2 : 1;
   const int NCm = winds ? 4 * P : 2 * P;  // product columns of one member
   const int chunk = max(1, min(count, kInvRows / (nf * P)));
   const int MT = nlat / 16;
-  const int R = (K + 3) & ~3;
-  const int as = a_stride(R);
   const int blocks = winds ? 2 : 1;
   const float* q0 = M.ltab + M.tabo[winds ? (int)TAB_legendre_qu : job.table];
   const float* q1 = M.ltab + M.tabo[TAB_legendre_qv];
@@ -289,79 +339,108 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp + 2 * w0;
-    rowsub[row] = (job.plavor && f == 1 && w0 <= 1 && 1 < w0 + K) ? G.k[CON_plavor] : 0.0f;
+    rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp;
+    rowsub[row] = (job.plavor && f == 1) ? G.k[CON_plavor] : 0.0f;
   }
-  // the table columns of m for the whole group (dv2uv: qu, qv)
+  // staged column col < RT: mode w of the pair's wavenumber, or padding
+  const auto mode_of = [=](int col) {
+    if (col < Ra) return col < Ka ? wa + col : -1;
+    const int k = col - Ra;
+    return k < Kb ? wb + k : -1;
+  };
+  // the members' coefficients (zeta without the planetary vorticity in
+  // Re mode (0, 1)), zeros in the padding, for the chunk from member c0
+  const auto raw_loader = [&](int c0) {
+    const int r0 = c0 * nf * P;
+    return [=](int row, int x) {
+      if (x >= 2 * RT) return 0.0f;
+      const int w = mode_of(x >> 1);
+      if (w < 0) return 0.0f;
+      const float v = rowp[r0 + row][2 * w + (x & 1)];
+      return (w == 1 && (x & 1) == 0) ? v - rowsub[r0 + row] : v;
+    };
+  };
+  const auto raw_store = [&](int row, int x, float v) { raw[row * kPairRawK + x] = v; };
+  __syncthreads();  // rowp
+  // one memory latency for the table columns of the pair (dv2uv: qu, qv)
+  // and the first chunk's inputs: every load is issued before any store
+  {
+    float va[2][kPairTabN], vr[kPairRawN];
 #pragma unroll
-  for (int blk = 0; blk < 2; blk++) {
-    if (blk < blocks) {
-      const float* q = blk ? q1 : q0;
-      float* A = As + blk * kInvAs;
-      float va[kInvTabN];
-      load_batch<kInvTabN, 6>(nlat, R, [&](int lat, int k) { return k < K ? q[lat * ncsp + w0 + k] : 0.0f; }, va);
-      store_batch<kInvTabN, 6>(nlat, R, [&](int lat, int k, float v) { A[lat * as + k] = v; }, va);
-    }
+    for (int blk = 0; blk < 2; blk++)
+      if (blk < blocks) {
+        const float* q = blk ? q1 : q0;
+        load_cols<kPairTabN, kPairK>(nlat, [&](int lat, int col) {
+          const int w = col < RT ? mode_of(col) : -1;
+          return w >= 0 ? q[lat * ncsp + w] : 0.0f;
+        }, va[blk]);
+      }
+    load_cols<kPairRawN, 2 * kPairK>(min(chunk, count) * nf * P, raw_loader(0), vr);
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
+      if (blk < blocks) {
+        float* A = As + blk * kPairAs;
+        store_cols<kPairTabN, kPairK>(nlat, [&](int lat, int col, float v) { if (col < RT) A[lat * as + col] = v; },
+                                      va[blk]);
+      }
+    store_cols<kPairRawN, 2 * kPairK>(min(chunk, count) * nf * P, raw_store, vr);
   }
   __syncthreads();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
-    // the members' coefficients of m (zeta without the planetary vorticity
-    // in Re mode (0, 1)), zeros past K
-    const int r0 = c0 * nf * P;
-    const auto raw_load = [&](int row, int x) {
-      const int k = x >> 1;
-      if (k >= K) return 0.0f;
-      const float v = rowp[r0 + row][x];
-      return (w0 + k == 1 && (x & 1) == 0) ? v - rowsub[r0 + row] : v;
-    };
-    const auto raw_store = [&](int row, int x, float v) { raw[row * kRawK + x] = v; };
-    float vr[kInvRawN];
-    load_batch<kInvRawN, 7>(n * nf * P, 2 * R, raw_load, vr);
-    store_batch<kInvRawN, 7>(n * nf * P, 2 * R, raw_store, vr);
-    __syncthreads();
-    f4 acc[kGTiles];
+    // the next chunk's inputs are in flight while this chunk's products run
+    const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
+    float vn[kPairRawN];
+    if (n1 > 0) load_cols<kPairRawN, 2 * kPairK>(n1 * nf * P, raw_loader(c1), vn);
+    for (int h = 0; h <= two; h++) {
+      const int m = h ? mb : ma, col0 = h ? Ra : 0, R = h ? Rb : Ra;
+      f4 acc[kGTiles];
 #pragma unroll
-    for (int t = 0; t < kGTiles; t++) acc[t] = f4{0, 0, 0, 0};
-    for (int blk = 0; blk < blocks; blk++) {
-      const float* A = As + blk * kInvAs;
+      for (int t = 0; t < kGTiles; t++) acc[t] = f4{0, 0, 0, 0};
+      for (int blk = 0; blk < blocks; blk++) {
+        const float* A = As + blk * kPairAs;
+#pragma unroll
+        for (int t = 0; t < kGTiles; t++) {
+          const int tile = wave + kGroupWaves * t;
+          if (tile < MT * NT) {
+            const int mt = tile / NT, nt = tile % NT;
+            const Col col = inv_col(winds, blk, nt * 16 + i, NC, NCm, P, kPairRawK);
+            const float* a = A + (mt * 16 + i) * as + col0 + kk;
+            const float* b = raw + col.base + 2 * (col0 + kk);
+            f4 c = acc[t];
+            for (int r = 0; r < R; r += 4)
+              c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], col.sign * b[2 * r], c, 0, 0, 0);
+            acc[t] = c;
+          }
+        }
+      }
 #pragma unroll
       for (int t = 0; t < kGTiles; t++) {
         const int tile = wave + kGroupWaves * t;
         if (tile < MT * NT) {
           const int mt = tile / NT, nt = tile % NT;
-          const Col col = inv_col(winds, blk, nt * 16 + i, NC, NCm, P);
-          const float* a = A + (mt * 16 + i) * as + kk;
-          const float* b = raw + col.base + 2 * kk;
-          f4 c = acc[t];
-          for (int r = 0; r < R; r += 4)
-            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], col.sign * b[2 * r], c, 0, 0, 0);
-          acc[t] = c;
-        }
-      }
-    }
+          int c = nt * 16 + i;
+          if (c < NC) {
+            const int g = c / NCm;
+            c -= g * NCm;
+            const DevMember& G = mems[group_member(plan, grp, c0 + g)];
+            float* base = G.arena + G.slot[winds && (c & 2) ? job.four1 : job.four0];
+            const int p = winds ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
-    for (int t = 0; t < kGTiles; t++) {
-      const int tile = wave + kGroupWaves * t;
-      if (tile < MT * NT) {
-        const int mt = tile / NT, nt = tile % NT;
-        int c = nt * 16 + i;
-        if (c < NC) {
-          const int g = c / NCm;
-          c -= g * NCm;
-          const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-          float* base = G.arena + G.slot[winds && (c & 2) ? job.four1 : job.four0];
-          const int p = winds ? c >> 2 : c >> 1, part = c & 1;
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int lat = mt * 16 + 4 * kk + r;
-            base[(p * nlat + lat) * nfc + 2 * m + part] = acc[t][r];
+            for (int r = 0; r < 4; r++) {
+              const int lat = mt * 16 + 4 * kk + r;
+              base[(p * nlat + lat) * nfc + 2 * m + part] = acc[t][r];
+            }
           }
         }
       }
     }
-    __syncthreads();  // the next chunk restages the inputs
+    if (n1 > 0) {
+      __syncthreads();  // every wave is done with this chunk's inputs
+      store_cols<kPairRawN, 2 * kPairK>(n1 * nf * P, raw_store, vn);
+      __syncthreads();
+    }
   }
 }
 
@@ -403,28 +482,40 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
     const DevMember& G = mems[group_member(plan, grp, g)];
     rowp[row] = G.arena + G.slot[fields[f]] + p * nlat * nfc + 2 * m;
   }
-  // the northern half of each table for the whole group, Ah[k][l] = q(l, w0 + k)
+  // the members' Fourier coefficients m of every row and level, for the
+  // chunk from group member c0
+  const auto raw_loader = [&](int c0) {
+    const int r0 = c0 * nf * P;
+    return [=](int row, int x) { return rowp[r0 + row][(x >> 1) * nfc + (x & 1)]; };
+  };
+  const auto raw_store = [&](int row, int x, float v) { raw[row * kRawLat + x] = v; };
+  __syncthreads();  // rowp
+  // one memory latency for the northern half of each table,
+  // Ah[k][l] = q(l, w0 + k), and the first chunk's inputs
+  {
+    float va[3][kFwdTabN], vr[kFwdRawN];
 #pragma unroll
-  for (int blk = 0; blk < 3; blk++) {
-    if (blk < blocks) {
-      const float* q = M.ltab + M.tabo[tables[blk]];
-      float* A = Ah + blk * kFwdAs;
-      float va[kFwdTabN];
-      load_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; }, va);
-      store_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k, float v) { A[k * ash + l] = v; }, va);
-    }
+    for (int blk = 0; blk < 3; blk++)
+      if (blk < blocks) {
+        const float* q = M.ltab + M.tabo[tables[blk]];
+        load_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; },
+                                va[blk]);
+      }
+    load_batch<kFwdRawN, 8>(min(chunk, count) * nf * P, 2 * nlat, raw_loader(0), vr);
+#pragma unroll
+    for (int blk = 0; blk < 3; blk++)
+      if (blk < blocks) {
+        float* A = Ah + blk * kFwdAs;
+        store_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k, float v) { A[k * ash + l] = v; }, va[blk]);
+      }
+    store_batch<kFwdRawN, 8>(min(chunk, count) * nf * P, 2 * nlat, raw_store, vr);
   }
   __syncthreads();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
-    // the members' Fourier coefficients m of every row and level
-    const int r0 = c0 * nf * P;
-    const auto raw_load = [&](int row, int x) { return rowp[r0 + row][(x >> 1) * nfc + (x & 1)]; };
-    float vr[kFwdRawN];
-    load_batch<kFwdRawN, 8>(n * nf * P, 2 * nlat, raw_load, vr);
-    store_batch<kFwdRawN, 8>(n * nf * P, 2 * nlat, [&](int row, int x, float v) { raw[row * kRawLat + x] = v; }, vr);
-    __syncthreads();
+    // the next chunk (forward rows use too many registers to prefetch)
+    const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
     f4 acc[kGTiles];
 #pragma unroll
     for (int t = 0; t < kGTiles; t++) acc[t] = f4{0, 0, 0, 0};
@@ -470,7 +561,13 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
         }
       }
     }
-    __syncthreads();  // the next chunk restages the inputs
+    if (n1 > 0) {
+      __syncthreads();  // every wave is done with this chunk's inputs
+      float vn[kFwdRawN];
+      load_batch<kFwdRawN, 8>(n1 * nf * P, 2 * nlat, raw_loader(c1), vn);
+      store_batch<kFwdRawN, 8>(n1 * nf * P, 2 * nlat, raw_store, vn);
+      __syncthreads();
+    }
   }
 }
 
@@ -705,18 +802,18 @@ __device__ void leg_fwd_split(const DevMember& M, const LegJob& job, int m, Spli
 
 extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_leg_inv(const DevMember* __restrict__ mems,
                                                                        GroupPlan plan, InvJobs jobs) {
-  __shared__ float As[2 * kInvAs];
-  __shared__ float raw[kInvRows * kRawK];
-  __shared__ const float* rowp[kLegGroupRows];
-  __shared__ float rowsub[kLegGroupRows];
+  __shared__ float As[2 * kPairAs];
+  __shared__ float raw[kInvRows * kPairRawK];
+  __shared__ const float* rowp[kInvGroupRows];
+  __shared__ float rowsub[kInvGroupRows];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
-  // job-major: neighbouring wavenumbers of one job, whose Fourier rows
-  // share cache lines, go to one XCD (xcd_block); m = 0 (longest) first
-  const int ntp1 = M.ntru + 1;
-  const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
-#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile % ntp1, As, raw, rowp, rowsub)
+  // job-major: neighbouring wavenumber pairs of one job, whose Fourier
+  // rows share cache lines, go to one XCD (xcd_block)
+  const int pairs = (M.ntru + 2) / 2;
+  const InvJob job = jobs.j[job_of(jobs, tile / pairs)];
+#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, rowsub)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
