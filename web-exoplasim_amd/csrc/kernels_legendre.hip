@@ -220,7 +220,10 @@ __device__ __forceinline__ f4 mfma_split(const h8& ahi, const h8& alo, const h8&
 
 constexpr int kGroupThreads = 512;
 constexpr int kGroupWaves = kGroupThreads / 64;
-constexpr int kGTiles = 4;     // output tiles per wave and chunk (<= 32 per chunk)
+constexpr int kGTiles = 4;     // inverse output tiles per wave and chunk (<= 32 per chunk)
+// forward: at most 4 x 4 tiles per chunk (K <= 64 rows; the 30 staged input
+// rows give at most 60 product columns)
+constexpr int kFwdGTiles = 2;
 constexpr int kInvRows = 40;   // staged input rows: inverse
 constexpr int kFwdRows = 30;   // staged input rows: forward
 // input rows of a whole group: members x fields x levels
@@ -446,6 +449,11 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
 
 // ------------------------------------------------------------- forward
 
+// forward staging with a fixed column per thread: table half x 64 (8 rows
+// per pass), inputs rows x 2 NLAT (<= 192: 2 rows per pass)
+constexpr int kFwdColTabN = (kMaxLat / 2 + kGroupThreads / 64 - 1) / (kGroupThreads / 64);
+constexpr int kFwdColRawN = (kFwdRows + kGroupThreads / (2 * kMaxLat) - 1) / (kGroupThreads / (2 * kMaxLat));
+
 template <int P>
 __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, const LegJob& job, int m, float* Ah,
                         float* raw, const float** rowp) {
@@ -486,43 +494,48 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   // chunk from group member c0
   const auto raw_loader = [&](int c0) {
     const int r0 = c0 * nf * P;
-    return [=](int row, int x) { return rowp[r0 + row][(x >> 1) * nfc + (x & 1)]; };
+    return [=](int row, int x) { return x < 2 * nlat ? rowp[r0 + row][(x >> 1) * nfc + (x & 1)] : 0.0f; };
   };
-  const auto raw_store = [&](int row, int x, float v) { raw[row * kRawLat + x] = v; };
+  const auto raw_store = [&](int row, int x, float v) {
+    if (x < 2 * nlat) raw[row * kRawLat + x] = v;
+  };
   __syncthreads();  // rowp
   // one memory latency for the northern half of each table,
   // Ah[k][l] = q(l, w0 + k), and the first chunk's inputs
   {
-    float va[3][kFwdTabN], vr[kFwdRawN];
+    float va[3][kFwdColTabN], vr[kFwdColRawN];
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         const float* q = M.ltab + M.tabo[tables[blk]];
-        load_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; },
-                                va[blk]);
+        load_cols<kFwdColTabN, 64>(half, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; },
+                                   va[blk]);
       }
-    load_batch<kFwdRawN, 8>(min(chunk, count) * nf * P, 2 * nlat, raw_loader(0), vr);
+    load_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_loader(0), vr);
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         float* A = Ah + blk * kFwdAs;
-        store_batch<kFwdTabN, 6>(half, MT * 16, [&](int l, int k, float v) { A[k * ash + l] = v; }, va[blk]);
+        store_cols<kFwdColTabN, 64>(half, [&](int l, int k, float v) { if (k < MT * 16) A[k * ash + l] = v; },
+                                    va[blk]);
       }
-    store_batch<kFwdRawN, 8>(min(chunk, count) * nf * P, 2 * nlat, raw_store, vr);
+    store_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_store, vr);
   }
   __syncthreads();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
-    // the next chunk (forward rows use too many registers to prefetch)
+    // the next chunk's inputs are in flight while this chunk's products run
     const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
-    f4 acc[kGTiles];
+    float vn[kFwdColRawN];
+    if (n1 > 0) load_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_loader(c1), vn);
+    f4 acc[kFwdGTiles];
 #pragma unroll
-    for (int t = 0; t < kGTiles; t++) acc[t] = f4{0, 0, 0, 0};
+    for (int t = 0; t < kFwdGTiles; t++) acc[t] = f4{0, 0, 0, 0};
     for (int blk = 0; blk < blocks; blk++) {
       const float* A = Ah + blk * kFwdAs;
 #pragma unroll
-      for (int t = 0; t < kGTiles; t++) {
+      for (int t = 0; t < kFwdGTiles; t++) {
         const int tile = wave + kGroupWaves * t;
         if (tile < MT * NT) {
           const int mt = tile / NT, nt = tile % NT;
@@ -542,7 +555,7 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
       }
     }
 #pragma unroll
-    for (int t = 0; t < kGTiles; t++) {
+    for (int t = 0; t < kFwdGTiles; t++) {
       const int tile = wave + kGroupWaves * t;
       if (tile < MT * NT) {
         const int mt = tile / NT, nt = tile % NT;
@@ -563,9 +576,7 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
     }
     if (n1 > 0) {
       __syncthreads();  // every wave is done with this chunk's inputs
-      float vn[kFwdRawN];
-      load_batch<kFwdRawN, 8>(n1 * nf * P, 2 * nlat, raw_loader(c1), vn);
-      store_batch<kFwdRawN, 8>(n1 * nf * P, 2 * nlat, raw_store, vn);
+      store_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_store, vn);
       __syncthreads();
     }
   }
