@@ -69,6 +69,11 @@ typedef struct exo_config {
   double sea_surface_k;        /* SST climatology, K */
   uint64_t terrain_seed;
   uint64_t noise_seed;
+  /* GcmConfig.stormclim (types.ts:30): 1 runs hurricanemod's storm
+   * diagnostics every step (hurricane_namelist NSTORMDIAG, namelist/
+   * index.ts:175; physics/storm/index.ts:22-45) into the snapshot's
+   * storm fields; 0 (default) leaves them at zero. */
+  int storm_diagnostics;
 } exo_config;
 
 /* Fills *cfg with the defaults of a T21 L10 Earth-like synthetic planet. */

@@ -77,7 +77,8 @@ void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp
   X(yls) X(yfldo) X(yclsst2) X(yclsst) X(sea_dts) X(sea_dqs) X(sea_dsst)    \
   X(sea_cheata) X(sea_cprsa) X(sea_naccua) X(localprecip)                   \
   X(localavgprecip) X(localweathering) X(localavgweather) X(localavgtemps)  \
-  X(co2) X(carbon_step)
+  X(co2) X(carbon_step) X(capen) X(lnb) X(mpoti) X(chim) X(laav) X(venti)    \
+  X(vrmpi) X(gpi) X(shear)
 
 #define TABLES(X)                                                            \
   X(fourier_forward) X(fourier_inverse) X(legendre_qc) X(legendre_qe)        \
@@ -113,7 +114,7 @@ void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp
   X(charnock) X(dz0sea) X(dz0ice) X(drhssea) X(drhsice) X(albsea) X(albice)  \
   X(dicealbmx1) X(dicealbmx2) X(dicealbmn1) X(dicealbmn2) X(ncarbon)         \
   X(nsupply) X(timeweight) X(co2earth) X(cbeta) X(kact) X(pearth) X(wmax)   \
-  X(vearth) X(plarad) X(rad_earthsq)
+  X(vearth) X(plarad) X(rad_earthsq) X(nstormdiag) X(ckcd) X(vithresh)
 
 enum {
 #define E(n) S_##n,

@@ -3,7 +3,7 @@
  * ONLY (see exo_oracle.c). Each routine restates one reference WGSL kernel
  * (file:line given) in real precision, in the reference's dispatch order
  * (stepper/index.ts:148-161): fixer, fluxstep, radstep, rainstep,
- * surfstep, carbonstep. The reference's f32 "precise_*" helpers
+ * surfstep, carbonstep, hurricanestep. The reference's f32 "precise_*" helpers
  * (shared/gpu/math/wgsl/precise.wgsl) are libm's log/exp/pow here.
  *
  * Pinned by physics on crafted columns (tests/test_oracle_physics_pins.py):
@@ -13,11 +13,16 @@
  * must obey (Kirchhoff, energy and water conservation, backward-Euler
  * diffusion, textbook latent heats, Tetens saturation, random cloud
  * overlap, Green's ozone profile, Charnock's roughness, the log law,
- * Slingo's cloud cover, the Walker-Hays-Kasting weathering law).
+ * Slingo's cloud cover, the Walker-Hays-Kasting weathering law), and
+ * hurricanemod's storm diagnostics (tests/test_oracle_storm_pins.py: the
+ * pressure-level choice, the kinematic indices, Bolton's saturation, the
+ * dry-adiabatic CAPE in closed form, tropical CAPE and potential
+ * intensity, the trigonometric root of the ventilation cubic).
  *
  * Parity unpinned: the glacier and wet-soil albedo curves, the ice
- * surface temperature's implicit solution, the weathering supply limit
- * and the cloud liquid-water profile. No closed form, library routine or
+ * surface temperature's implicit solution, the weathering supply limit,
+ * the cloud liquid-water profile and the outflow-temperature ratio in the
+ * storm potential intensity. No closed form, library routine or
  * reference golden vector checks those details; they are held only to the
  * GPU kernels written from the same reading of the reference WGSL, and by
  * the physical range of day-scale runs.
@@ -1654,6 +1659,186 @@ static void seastep(const M* v) {
   }
 }
 
+
+/* ------------------------------------------------------------- storms */
+
+/* physics/storm/wgsl/storm.wgsl:1-208 (hurricanemod.f90 at ExoPlaSim
+ * 5f8dff4): CAPE of the lowest-level parcel, the maximum potential
+ * intensity by fixed-point iteration on the central pressure, the entropy
+ * deficit, ventilation and genesis potential indices. */
+#define ST_CPD ((real)1005.7)
+#define ST_CL ((real)2500.0)
+#define ST_RVP ((real)461.5)
+#define ST_ALV0 ((real)2.501e6)
+#define ST_CPVMCL ((real)-630.0)
+
+typedef struct {
+  real energy, temperature, pressure;
+} st_cape;
+
+/* storm.wgsl:26-28: hPa over water of a Celsius temperature */
+static real st_es(real tc) { return (real)6.112 * exp((real)17.67 * tc / ((real)243.5 + tc)); }
+/* storm.wgsl:31-38 */
+static real st_vapor(real r, real p, real eps) { return r * p / (eps + r); }
+static real st_mixing(real e, real p, real eps) { return eps * e / (p - e); }
+/* storm.wgsl:41-43 */
+static real st_density(real t, real rt, real r, real eps) { return t * (1 + r / eps) / (1 + rt); }
+/* storm.wgsl:46-54 */
+static real st_entropy(real t, real r, real p, real RD, real eps) {
+  const real e = st_vapor(r, p, eps);
+  const real rh = MIN(e / st_es(t - (real)273.15), 1);
+  const real lv = ST_ALV0 + ST_CPVMCL * (t - (real)273.15);
+  const real dry = (ST_CPD + r * ST_CL) * log(t) - RD * log(p - e);
+  if (r == 0) return dry;
+  return dry + lv * r / t - r * ST_RVP * log(rh);
+}
+
+/* storm.wgsl:56-131 */
+static st_cape st_cape_of(int L, const real* t, const real* r, const real* p, real tp, real rp, real pp, real RD) {
+  const real eps = RD / ST_RVP;
+  st_cape out = {0, -1, -1};
+  if (rp < (real)1e-6 || tp < 200 || pp <= 0) return out;
+  const real rh = MIN(st_vapor(rp, pp, eps) / st_es(tp - (real)273.15), 1);
+  const real parcel_entropy = st_entropy(tp, rp, pp, RD, eps);
+  const real plcl = pp * pow(rh, tp / ((real)1669.0 - (real)122.0 * rh - tp));
+  real buoyancy[16];
+  for (int l = L - 1; l >= 0; l--) { /* the topmost level is the last one tried */
+    real tg, rg = rp;
+    if (p[l] >= plcl) {
+      tg = tp * pow(p[l] / pp, RD / ST_CPD);
+    } else {
+      real next = t[l];
+      unsigned count = 0;
+      tg = 0;
+      for (;;) {
+        if (fabs(next - tg) <= (real)0.001) break;
+        tg = next;
+        const real tc = tg - (real)273.15;
+        const real enew = st_es(tc);
+        rg = st_mixing(enew, p[l], eps);
+        count++;
+        const real lv = ST_ALV0 + ST_CPVMCL * tc;
+        const real sl = (ST_CPD + rp * ST_CL + lv * lv * rg / (ST_RVP * tg * tg)) / tg;
+        const real em = st_vapor(rg, p[l], eps);
+        if (p[l] - em <= 0 || tg <= 0) {
+          out.energy = 0, out.temperature = t[l], out.pressure = -1;
+          return out;
+        }
+        const real sg = (ST_CPD + rp * ST_CL) * log(tg) - RD * log(p[l] - em) + lv * rg / tg;
+        next = tg + (count < 3 ? (real)0.3 : (real)1.0) * (parcel_entropy - sg) / sl;
+        if (count > 500 || enew > p[l] - 1 || !(next > 0)) {
+          out.energy = 0, out.temperature = t[L - 1], out.pressure = p[L - 1];
+          return out;
+        }
+      }
+    }
+    buoyancy[l] = st_density(tg, rp, rg, eps) - st_density(t[l], r[l], r[l], eps);
+  }
+  int inb = L - 1;
+  for (int l = 0; l < L; l++)
+    if (buoyancy[l] > 0 && l < inb) inb = l;
+  if (inb == L - 1) {
+    out.energy = 0, out.temperature = t[L - 1], out.pressure = 0;
+    return out;
+  }
+  real area = 0;
+  for (int j = L - 2; j >= inb; j--)
+    area += RD * (buoyancy[j] + buoyancy[j + 1]) * (p[j + 1] - p[j]) / (p[j] + p[j + 1]);
+  area += RD * (pp - p[L - 1]) / (pp + p[L - 1]) * buoyancy[L - 1];
+  real tob = t[inb], lnb = p[inb];
+  if (inb > 0) {
+    lnb = (p[inb - 1] * buoyancy[inb] - p[inb] * buoyancy[inb - 1]) / (buoyancy[inb] - buoyancy[inb - 1]);
+    area += RD * buoyancy[inb] * (p[inb] - lnb) / (p[inb] + lnb);
+    tob = (t[inb] * (lnb - p[inb - 1]) + t[inb - 1] * (p[inb] - lnb)) / (p[inb] - p[inb - 1]);
+  }
+  out.energy = MAX(area, 0), out.temperature = tob, out.pressure = lnb;
+  return out;
+}
+
+/* storm.wgsl:133-138 */
+static real st_reduced(real vi, real vithresh) {
+  if (vi == 0 || fabs(vi) >= vithresh) return 0;
+  return 2 / sqrt((real)3) * cos(acos(-vi / vithresh) / 3);
+}
+
+/* storm.wgsl:140-208 */
+static void storm(const M* v) {
+  const int NH = v->NHOR, L = v->NLEV;
+  const real RD = K(gascon), OMEGA = K(ww), CKCD = K(ckcd), eps = RD / ST_RVP;
+  const real* sigma = T(sigma);
+  for (int h = 0; h < NH; h++) {
+    real t[16], r[16], p[16];
+    const real msl = F(dp)[h] * (real)0.01;
+    const real sst = LV(dt, L, h);
+    int i600 = 0, i850 = 0, i200 = 0, cold = 0;
+    for (int l = 0; l < L; l++) {
+      t[l] = LV(dt, l, h);
+      r[l] = LV(dq, l, h);
+      p[l] = msl * sigma[l];
+      cold = cold || t[l] <= 100;
+      if (fabs(sigma[l] - (real)0.6) < fabs(sigma[i600] - (real)0.6)) i600 = l;
+      if (fabs(sigma[l] - (real)0.85) < fabs(sigma[i850] - (real)0.85)) i850 = l;
+      if (fabs(sigma[l] - (real)0.2) < fabs(sigma[i200] - (real)0.2)) i200 = l;
+    }
+    const int bottom = L - 1;
+    const real u850 = LV(du, i850, h), v850 = LV(dv, i850, h), u200 = LV(du, i200, h), v200 = LV(dv, i200, h);
+    const real lowwind = sqrt(u850 * u850 + v850 * v850), highwind = sqrt(u200 * u200 + v200 * v200);
+    const real shear = fabs(highwind - lowwind);
+    const real vort = LV(gz, i850, h) * OMEGA;
+    real deficit = 1;
+    if (r[i600] > 0 && r[bottom] > 0) {
+      const real smr = st_mixing(st_es(t[i600] - (real)273.15), p[i600], eps);
+      const real sms = st_entropy(t[i600], smr, p[i600], RD, eps);
+      const real sme = st_entropy(t[i600], MIN(r[i600], smr), p[i600], RD, eps);
+      const real smb = st_entropy(sst, LV(dq, L, h), msl, RD, eps);
+      const real smo = st_entropy(sst, st_mixing(st_es(sst - (real)273.15), msl, eps), msl, RD, eps);
+      if (smo != smb) deficit = (sms - sme) / (smo - smb);
+    }
+    st_cape environmental = {-1, -1, -1};
+    real vmax = -1;
+    if (sst > (real)278.15 && !cold && msl > 0) {
+      environmental = st_cape_of(L, t, r, p, t[bottom], r[bottom], p[bottom], RD);
+      real pm = 970, old = pm, next = 0, capem = 0, capems = 0, ratio = 1;
+      int failed = 0;
+      unsigned count = 0;
+      for (;;) {
+        if (fabs(next - old) <= (real)0.5) break;
+        const real pp = MIN(pm, 1000);
+        const real rp = eps * r[bottom] * msl / (pp * (eps + r[bottom]) - r[bottom] * msl);
+        capem = st_cape_of(L, t, r, p, t[bottom], rp, pp, RD).energy;
+        const real rs = st_mixing(st_es(sst - (real)273.15), pp, eps);
+        const st_cape saturated = st_cape_of(L, t, r, p, sst, rs, pp, RD);
+        capems = saturated.energy;
+        ratio = sst / saturated.temperature;
+        const real tvav = (real)0.5 * (st_density(t[bottom], r[bottom], r[bottom], eps) + st_density(sst, rs, rs, eps));
+        const real cat = MAX((capem - environmental.energy) + (real)0.5 * CKCD * ratio * (capems - capem), 0);
+        next = msl * exp(-cat / (RD * tvav));
+        old = pm;
+        pm = next;
+        count++;
+        if (count > 200 || pm < 400 || !(pm > 0)) {
+          failed = 1;
+          break;
+        }
+      }
+      if (!failed) vmax = sqrt(MAX(0, CKCD * ratio * MAX(0, capems - capem)));
+    }
+    real vi = 1;
+    if (vmax > 0) vi = shear * deficit / vmax;
+    const real rh = st_vapor(r[i600], p[i600], eps) / st_es(t[i600] - (real)273.15);
+    const real v70 = vmax / 70, vent = 1 + (real)0.1 * shear;
+    F(capen)[h] = environmental.energy;
+    F(lnb)[h] = environmental.pressure;
+    F(mpoti)[h] = vmax;
+    F(chim)[h] = deficit;
+    F(laav)[h] = vort;
+    F(venti)[h] = vi;
+    F(vrmpi)[h] = st_reduced(vi, K(vithresh));
+    F(gpi)[h] = pow(fabs(vort * (real)1e5), (real)1.5) * pow(rh * 2, 3) * v70 * v70 * v70 / (vent * vent);
+    F(shear)[h] = shear;
+  }
+}
+
 /* physics/carbon/wgsl/carbonstep.wgsl:22-50 */
 static void carbonstep(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
@@ -1732,6 +1917,8 @@ int oracle_routine(const oracle_member* m, const char* name) {
     sea_ice(v);
   } else if (IS("carbon")) {
     carbonstep(v);
+  } else if (IS("storm")) {
+    storm(v);
   } else if (IS("seastep")) {
     seastep(v);
   } else {
@@ -1771,4 +1958,5 @@ void oracle_physics(const oracle_member* m) {
   }
   seastep(v);
   carbonstep(v);
+  if (K(nstormdiag) > 0) storm(v);
 }

@@ -79,6 +79,8 @@ enum ExoTable {
   /* carbonmod */                                                              \
   X(ncarbon) X(nsupply) X(timeweight) X(co2earth) X(cbeta) X(kact)             \
   X(pearth) X(wmax) X(vearth) X(plarad) X(rad_earthsq)                         \
+  /* hurricanemod */                                                           \
+  X(nstormdiag) X(ckcd) X(vithresh)                                            \
   /* snapshot */                                                               \
   X(jet_level) X(sigma_bottom)
 

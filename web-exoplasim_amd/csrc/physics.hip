@@ -23,6 +23,8 @@
 //   k_ice_sums       getiflx's two global sums
 //   k_physics_d      ice finish + slab ocean + sea surface, carbonstep,
 //                    and the physics frame out (physics-out.wgsl)
+//   k_storm          hurricanestep's storm diagnostics, members with
+//                    them on (storm/wgsl/storm.wgsl)
 //
 // Each routine's statement order follows the reference WGSL cited at it.
 #include "engine.h"
@@ -311,6 +313,227 @@ __device__ __forceinline__ void vdiff(const DevMember& M, int h) {
   diffuse_potential<NL>(M, ztn, kh, zskap);
 #pragma unroll
   for (int l = 0; l < NL; l++) LV(dtdt, l, h) = LV(dtdt, l, h) + (ztn[l] - zt[l]) / DELTSEC2;
+}
+
+// ------------------------------------------------------- storm diagnostics
+// hurricanemod's storm diagnostics (physics/storm/wgsl/storm.wgsl:1-208,
+// hurricanemod.f90 at ExoPlaSim 5f8dff4 with its humidity convention):
+// CAPE of the lowest-level parcel, Emanuel's maximum potential intensity
+// (a fixed-point iteration on the storm's central pressure), the entropy
+// deficit, the ventilation index and the genesis potential index. Every
+// level loop is unrolled so the sounding and the buoyancy stay in
+// registers; indices found at run time pick their level by comparison.
+
+namespace storm {
+
+constexpr float CPD = 1005.7f, CL = 2500.0f, RVP = 461.5f, ALV0 = 2.501e6f, CPVMCL = -630.0f;
+
+struct Cape {
+  float energy, temperature, pressure;
+};
+
+// saturation vapour pressure over water, hPa, of a Celsius temperature
+__device__ __forceinline__ float es(float tc) { return 6.112f * expf(17.67f * tc / (243.5f + tc)); }
+// vapour pressure of mixing ratio r at pressure p; eps = RD / RVP
+__device__ __forceinline__ float vapor(float r, float p, float eps) { return r * p / (eps + r); }
+// mixing ratio of vapour pressure e at pressure p
+__device__ __forceinline__ float mixing(float e, float p, float eps) { return eps * e / (p - e); }
+// density temperature of temperature t, total water rt, vapour r
+__device__ __forceinline__ float density(float t, float rt, float r, float eps) {
+  return t * (1.0f + r / eps) / (1.0f + rt);
+}
+__device__ __forceinline__ float entropy(float t, float r, float p, float RD, float eps) {
+  const float e = vapor(r, p, eps);
+  const float rh = fminf(e / es(t - 273.15f), 1.0f);
+  const float lv = ALV0 + CPVMCL * (t - 273.15f);
+  const float dry = (CPD + r * CL) * logf(t) - RD * logf(p - e);
+  if (r == 0.0f) return dry;
+  return dry + lv * r / t - r * RVP * logf(rh);
+}
+
+// storm.wgsl:56-131: CAPE of parcel (tp, rp, pp) in the sounding (t, r, p)
+template <int NL>
+__device__ __forceinline__ Cape cape(const float* t, const float* r, const float* p, float tp, float rp, float pp,
+                                     float RD) {
+  const float eps = RD / RVP;
+  if (rp < 1e-6f || tp < 200.0f || pp <= 0.0f) return Cape{0.0f, -1.0f, -1.0f};
+  const float rh = fminf(vapor(rp, pp, eps) / es(tp - 273.15f), 1.0f);
+  const float parcel_entropy = entropy(tp, rp, pp, RD, eps);
+  const float plcl = pp * powf(rh, tp / (1669.0f - 122.0f * rh - tp));
+  float buoyancy[NL];
+  // the topmost level first (storm.wgsl:67-68); a level whose saturated
+  // ascent fails ends the sounding (status 1: at that level's
+  // temperature, 2: at the bottom level's temperature and pressure)
+  int status = 0;
+  float fail_t = 0.0f;
+#pragma unroll
+  for (int l = NL - 1; l >= 0; l--) {
+    buoyancy[l] = 0.0f;
+    if (status == 0) {
+      float tg, rg = rp;
+      if (p[l] >= plcl) {
+        tg = tp * powf(p[l] / pp, RD / CPD);
+      } else {
+        float next = t[l];
+        tg = 0.0f;
+        unsigned count = 0;
+        for (;;) {
+          if (fabsf(next - tg) <= 0.001f) break;
+          tg = next;
+          const float tc = tg - 273.15f;
+          const float enew = es(tc);
+          rg = mixing(enew, p[l], eps);
+          count++;
+          const float lv = ALV0 + CPVMCL * tc;
+          const float sl = (CPD + rp * CL + lv * lv * rg / (RVP * tg * tg)) / tg;
+          const float em = vapor(rg, p[l], eps);
+          if (p[l] - em <= 0.0f || tg <= 0.0f) {
+            status = 1;
+            fail_t = t[l];
+            break;
+          }
+          const float sg = (CPD + rp * CL) * logf(tg) - RD * logf(p[l] - em) + lv * rg / tg;
+          next = tg + (count < 3u ? 0.3f : 1.0f) * (parcel_entropy - sg) / sl;
+          if (count > 500u || enew > p[l] - 1.0f || !(next > 0.0f)) {
+            status = 2;
+            break;
+          }
+        }
+      }
+      buoyancy[l] = density(tg, rp, rg, eps) - density(t[l], r[l], r[l], eps);
+    }
+  }
+  if (status == 1) return Cape{0.0f, fail_t, -1.0f};
+  if (status == 2) return Cape{0.0f, t[NL - 1], p[NL - 1]};
+  // inb: the first (topmost) positively buoyant level, with its values and
+  // those of the level above, picked while searching
+  int inb = NL - 1;
+  float bi = buoyancy[NL - 1], ti = t[NL - 1], pi = p[NL - 1], bm = 0.0f, tm = 0.0f, pm = 0.0f;
+#pragma unroll
+  for (int l = NL - 1; l >= 0; l--)
+    if (buoyancy[l] > 0.0f) {
+      inb = l;
+      bi = buoyancy[l];
+      ti = t[l];
+      pi = p[l];
+      bm = l > 0 ? buoyancy[l > 0 ? l - 1 : 0] : 0.0f;
+      tm = l > 0 ? t[l > 0 ? l - 1 : 0] : 0.0f;
+      pm = l > 0 ? p[l > 0 ? l - 1 : 0] : 0.0f;
+    }
+  if (inb == NL - 1) return Cape{0.0f, t[NL - 1], 0.0f};
+  float area = 0.0f;
+#pragma unroll
+  for (int j = NL - 2; j >= 0; j--)
+    if (j >= inb) area += RD * (buoyancy[j] + buoyancy[j + 1]) * (p[j + 1] - p[j]) / (p[j] + p[j + 1]);
+  area += RD * (pp - p[NL - 1]) / (pp + p[NL - 1]) * buoyancy[NL - 1];
+  float tob = ti, lnb = pi;
+  if (inb > 0) {
+    lnb = (pm * bi - pi * bm) / (bi - bm);
+    area += RD * bi * (pi - lnb) / (pi + lnb);
+    tob = (ti * (lnb - pm) + tm * (pi - lnb)) / (pi - pm);
+  }
+  return Cape{fmaxf(area, 0.0f), tob, lnb};
+}
+
+// storm.wgsl:133-138: the real part of upstream's principal complex cube
+// roots; zero at vi = 0 and |vi| >= vithresh
+__device__ __forceinline__ float reduced(float vi, float vithresh) {
+  if (vi == 0.0f || fabsf(vi) >= vithresh) return 0.0f;
+  return 2.0f / sqrtf(3.0f) * cosf(acosf(-vi / vithresh) / 3.0f);
+}
+
+}  // namespace storm
+
+template <int NL>
+__device__ __forceinline__ void storm_diagnostics(const DevMember& M, int h) {
+  // storm.wgsl:140-208
+  using namespace storm;
+  const int NH = M.nhor;
+  const float RD = K(gascon), OMEGA = K(ww), CKCD = K(ckcd), eps = RD / RVP;
+  const float* sigma = T(sigma);
+  float t[NL], r[NL], p[NL];
+  const float msl = F(dp)[h] * 0.01f;
+  const float sst = LV(dt, NL, h);
+  int i600 = 0, i850 = 0, i200 = 0;
+  bool cold = false;
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    t[l] = LV(dt, l, h);
+    r[l] = LV(dq, l, h);
+    p[l] = msl * sigma[l];
+    cold = cold || t[l] <= 100.0f;
+    if (fabsf(sigma[l] - 0.6f) < fabsf(sigma[i600] - 0.6f)) i600 = l;
+    if (fabsf(sigma[l] - 0.85f) < fabsf(sigma[i850] - 0.85f)) i850 = l;
+    if (fabsf(sigma[l] - 0.2f) < fabsf(sigma[i200] - 0.2f)) i200 = l;
+  }
+  const int bottom = NL - 1;
+  const float u850 = LV(du, i850, h), v850 = LV(dv, i850, h), u200 = LV(du, i200, h), v200 = LV(dv, i200, h);
+  const float lowwind = sqrtf(u850 * u850 + v850 * v850), highwind = sqrtf(u200 * u200 + v200 * v200);
+  const float shear = fabsf(highwind - lowwind);
+  const float vort = LV(gz, i850, h) * OMEGA;
+  // the level nearest sigma 0.6 is the member's (sigma is a table)
+  const float t600 = LV(dt, i600, h), r600 = LV(dq, i600, h), p600 = msl * sigma[i600];
+  float deficit = 1.0f;
+  if (r600 > 0.0f && r[bottom] > 0.0f) {
+    const float smr = mixing(es(t600 - 273.15f), p600, eps);
+    const float sms = entropy(t600, smr, p600, RD, eps);
+    const float sme = entropy(t600, fminf(r600, smr), p600, RD, eps);
+    const float smb = entropy(sst, LV(dq, NL, h), msl, RD, eps);
+    const float smo = entropy(sst, mixing(es(sst - 273.15f), msl, eps), msl, RD, eps);
+    if (smo != smb) deficit = (sms - sme) / (smo - smb);
+  }
+  Cape environmental{-1.0f, -1.0f, -1.0f};
+  float vmax = -1.0f;
+  if (sst > 278.15f && !cold && msl > 0.0f) {
+    environmental = cape<NL>(t, r, p, t[bottom], r[bottom], p[bottom], RD);
+    float pm = 970.0f, old = pm, next = 0.0f, capem = 0.0f, capems = 0.0f, ratio = 1.0f;
+    bool failed = false;
+    unsigned count = 0;
+    for (;;) {
+      if (fabsf(next - old) <= 0.5f) break;
+      const float pp = fminf(pm, 1000.0f);
+      const float rp = eps * r[bottom] * msl / (pp * (eps + r[bottom]) - r[bottom] * msl);
+      const float rs = mixing(es(sst - 273.15f), pp, eps);
+      // the two parcels of one iteration through one inlined CAPE
+      float saturated_t = 0.0f;
+#pragma unroll 1
+      for (int k = 0; k < 2; k++) {
+        const Cape c = cape<NL>(t, r, p, k ? sst : t[bottom], k ? rs : rp, pp, RD);
+        if (k == 0) {
+          capem = c.energy;
+        } else {
+          capems = c.energy;
+          saturated_t = c.temperature;
+        }
+      }
+      ratio = sst / saturated_t;
+      const float tvav = 0.5f * (density(t[bottom], r[bottom], r[bottom], eps) + density(sst, rs, rs, eps));
+      const float cat = fmaxf((capem - environmental.energy) + 0.5f * CKCD * ratio * (capems - capem), 0.0f);
+      next = msl * expf(-cat / (RD * tvav));
+      old = pm;
+      pm = next;
+      count++;
+      if (count > 200u || pm < 400.0f || !(pm > 0.0f)) {
+        failed = true;
+        break;
+      }
+    }
+    if (!failed) vmax = sqrtf(fmaxf(0.0f, CKCD * ratio * fmaxf(0.0f, capems - capem)));
+  }
+  float vi = 1.0f;
+  if (vmax > 0.0f) vi = shear * deficit / vmax;
+  const float rh = vapor(r600, p600, eps) / es(t600 - 273.15f);
+  const float v70 = vmax / 70.0f, vent = 1.0f + 0.1f * shear;
+  const float gpi = powf(fabsf(vort * 1e5f), 1.5f) * powf(rh * 2.0f, 3.0f) * v70 * v70 * v70 / (vent * vent);
+  F(capen)[h] = environmental.energy;
+  F(lnb)[h] = environmental.pressure;
+  F(mpoti)[h] = vmax;
+  F(chim)[h] = deficit;
+  F(laav)[h] = vort;
+  F(venti)[h] = vi;
+  F(vrmpi)[h] = reduced(vi, K(vithresh));
+  F(gpi)[h] = gpi;
+  F(shear)[h] = shear;
 }
 
 // ----------------------------------------------------------- radiation
@@ -1840,6 +2063,7 @@ EXO_COLUMN_KERNEL(k_clouds, clouds)
 EXO_COLUMN_KERNEL(k_soil, soil_step)
 EXO_COLUMN_KERNEL(k_physics_c, physics_c)
 EXO_COLUMN_KERNEL(k_physics_d, physics_d)
+EXO_COLUMN_KERNEL(k_storm, storm_diagnostics)
 
 extern "C" __global__ __launch_bounds__(64) void k_mkradv(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int tile;
@@ -1931,6 +2155,12 @@ void enqueue_physics(Engine& e, const std::vector<int>& active) {
   s.column(k_physics_c);
   launch(e, k_ice_sums, dim3(one.n), dim3(kReduceThreads), e.d_desc, one);
   s.column(k_physics_d);
+  // hurricanestep (stepper/index.ts:137-158): members with storm
+  // diagnostics on (physics/storm/index.ts:27)
+  const LaunchPlan storms = s.plan([](const DevMember& d) {
+    return d.k[CON_nstormdiag] > 0.0f ? (d.nhor + 63) / 64 : 0;
+  });
+  if (storms.first[storms.n] > 0) launch(e, k_storm, dim3(storms.first[storms.n]), dim3(64), e.d_desc, storms);
 }
 
 }  // namespace exo

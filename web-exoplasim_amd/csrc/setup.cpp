@@ -1527,6 +1527,10 @@ void build_member(const exo_config& cfg, Member& m) {
     k[CON_vearth] = carbon.vearth;
     k[CON_plarad] = p.plarad;
     k[CON_rad_earthsq] = 6371220.0 * 6371220.0;
+    // hurricanemod (setup/index.ts:107-114: hurricane_namelist defaults)
+    k[CON_nstormdiag] = cfg.storm_diagnostics != 0 ? 1 : 0;
+    k[CON_ckcd] = 0.9;
+    k[CON_vithresh] = 0.145;
     // snapshot (snapshot/index.ts:26-54): level nearest sigma 0.25
     int jet = 0;
     for (int l = 1; l < L; l++)

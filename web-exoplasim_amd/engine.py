@@ -55,6 +55,7 @@ class ExoConfig(ctypes.Structure):
         ("sea_surface_k", ctypes.c_double),
         ("terrain_seed", ctypes.c_uint64),
         ("noise_seed", ctypes.c_uint64),
+        ("storm_diagnostics", ctypes.c_int),
     ]
 
 

@@ -125,7 +125,7 @@ const Field kConfigFields[] = {
     EXO_FIELD('d', year_days), EXO_FIELD('d', ocean_depth_m), EXO_FIELD('i', ozone),
     EXO_FIELD('d', land_fraction), EXO_FIELD('d', terrain_max_m), EXO_FIELD('d', initial_temperature),
     EXO_FIELD('d', initial_humidity), EXO_FIELD('d', noise_kelvin), EXO_FIELD('d', sea_surface_k),
-    EXO_FIELD('u', terrain_seed), EXO_FIELD('u', noise_seed),
+    EXO_FIELD('u', terrain_seed), EXO_FIELD('u', noise_seed), EXO_FIELD('i', storm_diagnostics),
 };
 #undef EXO_FIELD
 

@@ -41,6 +41,7 @@ function engineConfig(config, planet) {
 		ozone: atmosphere.ozone === undefined ? undefined : atmosphere.ozone ? 1 : 0,
 		pressure_bar: atmosphere.pressureBar === null ? undefined : atmosphere.pressureBar,
 		gas_constant: atmosphere.gasConstant === null ? undefined : atmosphere.gasConstant,
+		storm_diagnostics: config.stormclim === undefined ? undefined : config.stormclim ? 1 : 0,
 	}
 	const p = planet || {}
 	const extra = {
