@@ -189,6 +189,15 @@ int exo_checkpoint_size(exo_engine *e, int member, size_t *bytes);
 int exo_checkpoint_save(exo_engine *e, int member, void *out, size_t bytes);
 int exo_checkpoint_restore(exo_engine *e, int member, const void *bytes,
                            size_t size);
+/* A restart under a changed configuration: exo_checkpoint_restore, then
+ * the fields that GCM_CONTINUATION.fields (continuation/index.ts:60-85;
+ * gcm.worker.ts:117-124) resets take the member's own (new) initial
+ * values: CO2 (co2, dqco2) when co2_ppm differs from *previous, the mixed
+ * layer (ymld, xmld, dmld) when ocean_depth_m does. previous = NULL is a
+ * plain restore. The reference's other resets (land albedo, star
+ * temperature, soil water capacity) have no exo_config counterpart. */
+int exo_checkpoint_continue(exo_engine *e, int member, const void *bytes,
+                            size_t size, const exo_config *previous);
 
 /*
  * Serving loop (continuous batching). Runs arrive over time

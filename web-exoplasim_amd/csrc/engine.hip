@@ -984,4 +984,25 @@ int exo_checkpoint_restore(exo_engine* h, int id, const void* bytes, size_t size
   return EXO_OK;
 }
 
+int exo_checkpoint_continue(exo_engine* h, int id, const void* bytes, size_t size, const exo_config* previous) {
+  const int rc = exo_checkpoint_restore(h, id, bytes, size);
+  if (rc != EXO_OK || !previous) return rc;
+  MemberDev* m = member_of(h, id);
+  // GCM_CONTINUATION.fields (continuation/index.ts:60-85): the fields a
+  // changed setting invalidates take the new configuration's initial
+  // values (gcm.worker.ts:117-124)
+  const exo_config& now = m->host.cfg;
+  std::vector<int> slots;
+  if (now.co2_ppm != previous->co2_ppm) slots.insert(slots.end(), {SLOT_co2, SLOT_dqco2});
+  if (now.ocean_depth_m != previous->ocean_depth_m) slots.insert(slots.end(), {SLOT_ymld, SLOT_xmld, SLOT_dmld});
+  std::vector<float> values;
+  for (const int s : slots) {
+    const size_t off = m->host.layout.offset[s], n = m->host.layout.floats[s];
+    values.assign(m->host.initial.begin() + off, m->host.initial.begin() + off + n);
+    const int w = exo_write_field(h, id, exo_slot_name(s), values.data(), n);
+    if (w != EXO_OK) return w;
+  }
+  return EXO_OK;
+}
+
 }  // extern "C"

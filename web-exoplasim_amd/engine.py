@@ -154,6 +154,7 @@ def library():
         "exo_checkpoint_size": (I, [P, I, ctypes.POINTER(S)]),
         "exo_checkpoint_save": (I, [P, I, ctypes.c_void_p, S]),
         "exo_checkpoint_restore": (I, [P, I, ctypes.c_void_p, S]),
+        "exo_checkpoint_continue": (I, [P, I, ctypes.c_void_p, S, ctypes.POINTER(ExoConfig)]),
         "exo_set_averaging": (I, [P, I, I]),
         "exo_month_count": (I, [P, I, ctypes.POINTER(I)]),
         "exo_read_month": (I, [P, I, I, ctypes.POINTER(ExoMonthInfo), F, S]),
@@ -539,9 +540,15 @@ class Engine:
         _check(self.lib.exo_checkpoint_save(self.h, member, buf, n.value))
         return buf.raw
 
-    def restore(self, member, data):
+    def restore(self, member, data, previous=None):
+        """Restores a checkpoint; with `previous` (the config it was saved
+        under) the fields a changed setting invalidates take this member's
+        initial values (GCM_CONTINUATION.fields)."""
         buf = ctypes.create_string_buffer(bytes(data), len(data))
-        _check(self.lib.exo_checkpoint_restore(self.h, member, buf, len(data)))
+        if previous is None:
+            _check(self.lib.exo_checkpoint_restore(self.h, member, buf, len(data)))
+        else:
+            _check(self.lib.exo_checkpoint_continue(self.h, member, buf, len(data), ctypes.byref(previous)))
 
     def read_arena(self, member):
         """Every slot, in arena order, as one float32 array."""

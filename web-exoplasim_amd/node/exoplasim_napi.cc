@@ -363,6 +363,14 @@ napi_value restore(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "restart must be an ArrayBuffer");
     return nullptr;
   }
+  // restore(engine, id, restart, previousConfig): with the configuration
+  // the restart was saved under, GCM_CONTINUATION.fields' resets apply
+  if (a.n > 3 && !is_nullish(env, a.v[3])) {
+    exo_config previous;
+    if (!config_from(env, a.v[3], previous)) return nullptr;
+    EXO_OK_OR_THROW(exo_checkpoint_continue(e, int_of(env, a.v[1]), p, bytes, &previous));
+    return nullptr;
+  }
   EXO_OK_OR_THROW(exo_checkpoint_restore(e, int_of(env, a.v[1]), p, bytes));
   return nullptr;
 }

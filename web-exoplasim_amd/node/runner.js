@@ -179,7 +179,15 @@ function start(request) {
 			const precision = request.precision === "split" ? addon.PRECISION_SPLIT : addon.PRECISION_FP32
 			state.id = addon.addMember(state.engine, engineConfig(request.config, request.planet), precision)
 			addon.setPublishEvery(state.engine, Math.max(1, request.publishEvery | 0))
-			if (request.restart) addon.restore(state.engine, state.id, request.restart)
+			// a continuation under an edited config resets what the edit
+			// invalidates (gcm.worker.ts:110-124, continuation/index.ts:60-85)
+			if (request.restart)
+				addon.restore(
+					state.engine,
+					state.id,
+					request.restart,
+					request.previousConfig ? engineConfig(request.previousConfig, request.planet) : null,
+				)
 			addon.setAveraging(state.engine, state.id, true)
 			if (request.averager) {
 				loadAverager(state.engine, state.id, request.averager)
