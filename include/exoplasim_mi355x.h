@@ -346,6 +346,29 @@ int exo_read_climatology(exo_engine *e, int member, int month,
 int exo_balance_assess(const exo_yearly_balance *years, int count,
                        exo_balance_assessment *out);
 
+/* ---- Boundary data ---------------------------------------------------
+   The input side of the step: a terrain raster onto a Gaussian grid and
+   ExoPlaSim's .sra surface files. */
+
+/* GCM_BOUNDARY.fromTerrain (src/model/climate/gcm/boundary/index.ts:
+   12-41; GCM_TERRAIN_REGRID.cell, terrain/regrid/index.ts:6-46): each cell
+   of the nlat x 2 nlat Gaussian grid (nlat 32, 64 or 96; rows north
+   first, longitudes from 0 east) gets the area-weighted land fraction
+   (clamped to [0, 1]) and g x the area-weighted mean of elevation x land
+   of the width x height raster (rows from 90N, columns from 180W) under
+   it, in FP64 on the engine's device. The raster is validated as
+   GCM_TERRAIN.validate does (terrain/index.ts:11-44), with its messages. */
+int exo_boundary_from_terrain(exo_engine *e, const float *elevation_m,
+                              const float *land, int width, int height,
+                              int nlat, double gravity_ms2,
+                              double *land_fraction, double *geopotential);
+/* toSra (boundary/index.ts:43-57): the .sra text of one field (code 172
+   land-sea mask, 129 orography) of an nlat x 2 nlat grid: a header of
+   eight integers, then eight values a line as JavaScript's toFixed(3).
+   *length = the text's bytes; out = NULL asks for the length only. */
+int exo_boundary_sra(int code, int nlat, const double *values, char *out,
+                     size_t capacity, size_t *length);
+
 /* ---- Native PlaSim restart files ------------------------------------
    plasim_restart / plasim_status: Fortran sequential records (uint32
    byte count, bytes, the count again), each field a 16-character name

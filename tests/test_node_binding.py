@@ -143,3 +143,25 @@ function second(saved) {
     else:
         assert second["months"][:-1] == first["months"][-len(second["months"]) + 1:]
         assert second["months"][-1][:2] == [cur["year"], cur["month"]]
+
+
+def test_boundary_sra_through_node(exo):
+    """The .sra values are formatted as Node's own Number.toFixed(3) does,
+    over random values and exact binary ties (k / 2000 with k odd and a
+    power-of-two denominator, e.g. 0.0625)."""
+    out = _node("""
+const {addon} = require('./runner.js')
+const v = new Float64Array(2 * 32 * 32)
+let seed = 7
+const rand = () => (seed = (seed * 1103515245 + 12345) % 2147483648) / 2147483648
+for (let i = 0; i < v.length; i++) v[i] = (rand() - 0.5) * Math.pow(10, Math.floor(rand() * 8) - 3)
+const ties = [0.0625, -0.0625, 0.1875, 2.0625, -1024.0625, 1.0005, -0, 0]
+ties.forEach((x, i) => { v[i] = x })
+const text = addon.boundarySra(129, 32, v)
+const mine = text.split('\\n').slice(1).join(' ').trim().split(/ +/)
+const js = Array.from(v, (x) => x.toFixed(3))
+const bad = js.map((s, i) => s === mine[i] ? -1 : i).filter((i) => i >= 0)
+console.log(JSON.stringify({bad: bad.slice(0, 5), n: js.length, first: mine.slice(0, 8)}))
+""")
+    assert out["n"] == 2048 and out["bad"] == [], out
+    assert out["first"] == ["0.063", "-0.063", "0.188", "2.063", "-1024.063", "1.000", "0.000", "0.000"]

@@ -7,4 +7,5 @@ engine interface (runner.py).
 """
 from .engine import (EXO_PRECISION_SPLIT, EXO_PRECISION_FP32, Engine,  # noqa: F401
                      ExoConfig, ExoError, constant_names, default_config,
-                     balance_assess, library, native_patch, native_record, slot_names)
+                     balance_assess, boundary_files, boundary_sra, library, native_patch,
+                     native_record, slot_names)

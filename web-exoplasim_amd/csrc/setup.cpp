@@ -802,6 +802,8 @@ void config_default(exo_config* c) {
   c->noise_seed = 14963991;
 }
 
+std::vector<double> gaussian_degrees(int nlat) { return gaussian_grid(nlat).degrees; }
+
 void member_date(const Member& m, int64_t step, int out[5]) {
   const Date d = date_of(step, m.cal.steps_per_day, m.cal.days_per_year, m.cal.step_minutes);
   out[0] = d.year;

@@ -69,6 +69,9 @@ void build_member(const exo_config& cfg, Member& m);
 // Calendar date {year, month, day, hour, minute} of main-loop step `step`
 // (calendar/date/index.ts:46-82, 360-day calendar).
 void member_date(const Member& m, int64_t step, int out[5]);
+// Gaussian latitudes in degrees, north first (GCM_GRID.forLatitudes'
+// latitudesDeg, grid/index.ts:32-82).
+std::vector<double> gaussian_degrees(int nlat);
 Layout make_layout(const ExoDims& d, int nsnap);
 TableLayout make_table_layout(const ExoDims& d);
 // STEPPER.values for model step `step` into 16 doubles: radiation_step[5],

@@ -169,6 +169,8 @@ def library():
                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(S),
                                  ctypes.c_void_p]),
         "exo_native_find": (I, [ctypes.c_char_p, S, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
+        "exo_boundary_from_terrain": (I, [P, F, F, I, I, I, ctypes.c_double, D, D]),
+        "exo_boundary_sra": (I, [I, I, D, ctypes.c_char_p, S, ctypes.POINTER(S)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -211,6 +213,31 @@ def native_patch(restart, fields):
     out = ctypes.create_string_buffer(len(data))
     _check(lib.exo_native_patch(data, len(data), len(names), c_names, c_values, c_counts, out))
     return out.raw
+
+
+LAND_SEA_CODE, OROGRAPHY_CODE = 172, 129
+
+
+def boundary_sra(code, nlat, values):
+    """toSra: the .sra text of one field of an nlat x 2 nlat grid."""
+    lib = library()
+    v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+    if v.size != 2 * nlat * nlat:
+        raise ExoError(-1, f"an N{nlat} field has {2 * nlat * nlat} values")
+    n = ctypes.c_size_t()
+    ptr = v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    _check(lib.exo_boundary_sra(code, nlat, ptr, None, 0, ctypes.byref(n)))
+    out = ctypes.create_string_buffer(n.value)
+    _check(lib.exo_boundary_sra(code, nlat, ptr, out, n.value, ctypes.byref(n)))
+    return out.raw[:n.value].decode()
+
+
+def boundary_files(nlat, land_fraction, geopotential):
+    """GCM_BOUNDARY.files: {file name: .sra text} of the land-sea mask and
+    the orography."""
+    prefix = f"N{nlat:03d}_surf_0"
+    return {f"{prefix}{LAND_SEA_CODE}.sra": boundary_sra(LAND_SEA_CODE, nlat, land_fraction),
+            f"{prefix}{OROGRAPHY_CODE}.sra": boundary_sra(OROGRAPHY_CODE, nlat, geopotential)}
 
 
 def native_record(restart, name):
@@ -527,6 +554,23 @@ class Engine:
             return None, None, 0
         nhor = self.dims(member)[1] * self.dims(member)[2]
         return _month_info(info), out.reshape(-1, nhor), years.value
+
+    def boundary_from_terrain(self, elevation_m, land, nlat, gravity_ms2=9.80665):
+        """GCM_BOUNDARY.fromTerrain on this engine's device: (land fraction,
+        geopotential), each (nlat, 2 nlat), of a (height, width) raster."""
+        elev = np.ascontiguousarray(np.asarray(elevation_m, dtype=np.float32))
+        lnd = np.ascontiguousarray(np.asarray(land, dtype=np.float32))
+        if elev.ndim != 2 or lnd.shape != elev.shape:
+            raise ExoError(-1, "Terrain arrays do not match the image dimensions")
+        height, width = elev.shape
+        fraction = np.empty(2 * nlat * nlat)
+        geo = np.empty(2 * nlat * nlat)
+        _check(self.lib.exo_boundary_from_terrain(
+            self.h, elev.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+            lnd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), width, height, nlat, gravity_ms2,
+            fraction.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            geo.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return fraction.reshape(nlat, 2 * nlat), geo.reshape(nlat, 2 * nlat)
 
     def global_means(self, member):
         t, f = ctypes.c_double(), ctypes.c_double()

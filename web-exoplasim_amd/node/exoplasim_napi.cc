@@ -603,6 +603,62 @@ napi_value balance_assess(napi_env env, napi_callback_info info) {
   return obj;
 }
 
+// ---- boundary data (GCM_BOUNDARY) ----------------------------------------
+
+// boundaryFromTerrain(engine, {width, height, elevationM: Float32Array,
+// land: Float32Array}, latitudes, gravityMs2) -> {landFraction,
+// geopotential} (Float64Array each): GCM_BOUNDARY.fromTerrain.
+napi_value boundary_from_terrain(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 4)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  const int width = (int)number_of(env, a.v[1], "width"), height = (int)number_of(env, a.v[1], "height");
+  napi_value ev, lv;
+  NAPI_OK(napi_get_named_property(env, a.v[1], "elevationM", &ev));
+  NAPI_OK(napi_get_named_property(env, a.v[1], "land", &lv));
+  size_t ne = 0, nl = 0;
+  const float* elev = static_cast<const float*>(typed_data(env, ev, napi_float32_array, ne));
+  if (!elev) return nullptr;
+  const float* land = static_cast<const float*>(typed_data(env, lv, napi_float32_array, nl));
+  if (!land) return nullptr;
+  if (ne != (size_t)width * height || nl != ne) {
+    napi_throw_error(env, nullptr, "Terrain arrays do not match the image dimensions");
+    return nullptr;
+  }
+  const int nlat = int_of(env, a.v[2]);
+  double gravity = 0;
+  napi_get_value_double(env, a.v[3], &gravity);
+  std::vector<double> fraction(2 * (size_t)nlat * nlat), geo(fraction.size());
+  EXO_OK_OR_THROW(exo_boundary_from_terrain(e, elev, land, width, height, nlat, gravity, fraction.data(), geo.data()));
+  napi_value obj;
+  NAPI_OK(napi_create_object(env, &obj));
+  NAPI_OK(napi_set_named_property(env, obj, "landFraction", double_array(env, fraction)));
+  NAPI_OK(napi_set_named_property(env, obj, "geopotential", double_array(env, geo)));
+  return obj;
+}
+
+// boundarySra(code, latitudes, values: Float64Array) -> string (toSra).
+napi_value boundary_sra(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  size_t n = 0;
+  const double* values = static_cast<const double*>(typed_data(env, a.v[2], napi_float64_array, n));
+  if (!values) return nullptr;
+  const int nlat = int_of(env, a.v[1]);
+  if (n != 2 * (size_t)nlat * nlat) {
+    napi_throw_range_error(env, nullptr, "the field does not match the grid");
+    return nullptr;
+  }
+  size_t len = 0;
+  EXO_OK_OR_THROW(exo_boundary_sra(int_of(env, a.v[0]), nlat, values, nullptr, 0, &len));
+  std::string text(len, '\0');
+  EXO_OK_OR_THROW(exo_boundary_sra(int_of(env, a.v[0]), nlat, values, text.data(), len, &len));
+  napi_value out;
+  NAPI_OK(napi_create_string_utf8(env, text.data(), len, &out));
+  return out;
+}
+
 napi_value init(napi_env env, napi_value exports) {
   const std::pair<const char*, napi_callback> fns[] = {
       {"configDefault", config_default}, {"create", create},
@@ -618,6 +674,7 @@ napi_value init(napi_env env, napi_value exports) {
       {"yearlyBalances", yearly_balances}, {"readAccumulator", read_accumulator},
       {"loadAverager", load_averager},   {"readClimatology", read_climatology},
       {"balanceAssess", balance_assess},
+      {"boundaryFromTerrain", boundary_from_terrain}, {"boundarySra", boundary_sra},
   };
   for (const auto& f : fns) {
     napi_value fn;

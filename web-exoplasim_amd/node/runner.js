@@ -224,4 +224,16 @@ function start(request) {
 	}
 }
 
-module.exports = { start, engineConfig, addon, readAverager, loadAverager }
+// GCM_BOUNDARY.files(GCM_BOUNDARY.fromTerrain(...)) (boundary/index.ts:
+// 12-79, preparation.worker.ts:48): the land-sea mask and orography .sra
+// files of a terrain raster on the Gaussian grid of `latitudes`.
+function boundaryFiles(engine, terrain, latitudes, gravityMs2) {
+	const { landFraction, geopotential } = addon.boundaryFromTerrain(engine, terrain, latitudes, gravityMs2)
+	const prefix = `N${String(latitudes).padStart(3, "0")}_surf_0`
+	return {
+		[`${prefix}172.sra`]: addon.boundarySra(172, latitudes, landFraction),
+		[`${prefix}129.sra`]: addon.boundarySra(129, latitudes, geopotential),
+	}
+}
+
+module.exports = { start, engineConfig, addon, readAverager, loadAverager, boundaryFiles }
