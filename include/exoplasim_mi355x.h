@@ -105,6 +105,16 @@ void exo_destroy(exo_engine *e);
 int exo_add_member(exo_engine *e, const exo_config *cfg, int precision,
                    int *member);
 
+/* exo_add_member with the planet's land and orography given instead of the
+ * seeded terrain: land_fraction and geopotential (m^2/s^2) on the member's
+ * nlat x 2 nlat grid (exo_boundary_from_terrain's output, or the values
+ * of .sra files 172 and 129 read by exo_boundary_read_sra). As landini
+ * (setup/surface/index.ts:293-297), land is where the fraction exceeds
+ * 0.5. */
+int exo_add_member_boundary(exo_engine *e, const exo_config *cfg,
+                            int precision, const double *land_fraction,
+                            const double *geopotential, int *member);
+
 /* Removes a member and frees its device memory. */
 int exo_remove_member(exo_engine *e, int member);
 
@@ -368,6 +378,14 @@ int exo_boundary_from_terrain(exo_engine *e, const float *elevation_m,
    *length = the text's bytes; out = NULL asks for the length only. */
 int exo_boundary_sra(int code, int nlat, const double *values, char *out,
                      size_t capacity, size_t *length);
+
+/* The values of one-field .sra text (the surface setup's read with one
+   month, setup/surface/index.ts:76-107): whitespace-separated numbers, an
+   eight-integer header, then the nlat x 2 nlat values into out. A text
+   without a whole field fails with "N032_surf_0172.sra has 0 fields,
+   expected 1" (the file name of `code`). */
+int exo_boundary_read_sra(int code, int nlat, const char *text, size_t length,
+                          double *out);
 
 /* ---- Native PlaSim restart files ------------------------------------
    plasim_restart / plasim_status: Fortran sequential records (uint32

@@ -96,3 +96,35 @@ def test_oracle_regrid_closed_forms():
         cells = np.sum(_cell_areas(nlat) * frac)
         assert cells == pytest.approx(raster, rel=1e-12)
         assert math.isclose(np.sum(_cell_areas(nlat)), 2.0, rel_tol=1e-12)
+
+
+def test_sra_round_trip_and_boundary_member(exo):
+    """The .sra reader returns what the writer wrote (to its 3 decimals),
+    rejects a text without a whole field with the setup's message, and a
+    host member built from a land fraction and orography takes its land
+    mask (fraction > 0.5) and surface geopotential from them."""
+    rng = np.random.default_rng(2)
+    nlat = 32
+    frac = rng.uniform(0, 1, (nlat, 2 * nlat))
+    geo = np.where(frac > 0.5, rng.uniform(0, 3e4, frac.shape), 0.0)
+    files = exo.boundary_files(nlat, frac, geo)
+    back_f = exo.boundary_read_sra(172, nlat, files["N032_surf_0172.sra"])
+    back_g = exo.boundary_read_sra(129, nlat, files["N032_surf_0129.sra"])
+    np.testing.assert_allclose(back_f, frac, atol=5e-4 + 1e-12)
+    np.testing.assert_allclose(back_g, geo, atol=5e-4 + 1e-9)
+    np.testing.assert_array_equal(back_f, [[float(ob.to_fixed3(v)) for v in row] for row in frac])
+    with pytest.raises(exo.ExoError, match="N032_surf_0172.sra has 0 fields, expected 1"):
+        exo.boundary_read_sra(172, nlat, files["N032_surf_0172.sra"][:500])
+    host = exo.Engine(-1, 2)
+    try:
+        cfg = exo.default_config(truncation=21, levels=5)
+        m = host.add_member(cfg, land_fraction=back_f, geopotential=back_g)
+        initial = host.read_table(m, "initial")
+        off, n = {name: (o, k) for name, o, k in host.layout(m)}["dls"]
+        np.testing.assert_array_equal(initial[off:off + n], (back_f > 0.5).reshape(-1).astype(float))
+        seeded = host.add_member(cfg)
+        assert not np.array_equal(host.read_table(seeded, "initial")[off:off + n], initial[off:off + n])
+        with pytest.raises(exo.ExoError):
+            host.add_member(cfg, land_fraction=back_f[:10], geopotential=back_g[:10])
+    finally:
+        host.close()

@@ -7,5 +7,5 @@ engine interface (runner.py).
 """
 from .engine import (EXO_PRECISION_SPLIT, EXO_PRECISION_FP32, Engine,  # noqa: F401
                      ExoConfig, ExoError, constant_names, default_config,
-                     balance_assess, boundary_files, boundary_sra, library, native_patch,
+                     balance_assess, boundary_files, boundary_read_sra, boundary_sra, library, native_patch,
                      native_record, slot_names)

@@ -16,7 +16,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <cctype>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -174,5 +177,32 @@ extern "C" int exo_boundary_sra(int code, int nlat, const double* values, char* 
   if (!out) return EXO_OK;
   if (capacity < text.size()) return exo::set_error(EXO_E_ARG, "the .sra text needs " + std::to_string(text.size()) + " bytes");
   std::memcpy(out, text.data(), text.size());
+  return EXO_OK;
+}
+
+extern "C" int exo_boundary_read_sra(int code, int nlat, const char* text, size_t length, double* out) {
+  if (!text || !out || nlat < 1) return exo::set_error(EXO_E_ARG, "missing argument");
+  const size_t points = 2 * (size_t)nlat * nlat, record = 8 + points;
+  char name[64];
+  std::snprintf(name, sizeof(name), "N%03d_surf_%04d.sra", nlat, code);
+  // text.trim().split(/\s+/).map(Number): a token that is not wholly a
+  // number is NaN
+  std::vector<double> tokens;
+  size_t i = 0;
+  std::string token;
+  while (i < length) {
+    while (i < length && std::isspace((unsigned char)text[i])) i++;
+    const size_t start = i;
+    while (i < length && !std::isspace((unsigned char)text[i])) i++;
+    if (i == start) break;
+    token.assign(text + start, i - start);
+    char* end = nullptr;
+    const double v = std::strtod(token.c_str(), &end);
+    tokens.push_back(end && *end == '\0' ? v : NAN);
+    if (tokens.size() >= record) break;  // one field is all a one-month read takes
+  }
+  if (tokens.size() < record)
+    return exo::set_error(EXO_E_FORMAT, std::string(name) + " has 0 fields, expected 1");
+  std::copy(tokens.begin() + 8, tokens.begin() + record, out);
   return EXO_OK;
 }

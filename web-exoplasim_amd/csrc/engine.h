@@ -343,7 +343,9 @@ void enqueue_step(Engine& e, const std::vector<int>& active);
 // member_bytes is the device memory admitting it takes, its truncation's
 // shared tables included when no member holds them yet; drop_member frees
 // it after the queued work.
-std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision);
+std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision,
+                                          const double* land_fraction = nullptr,
+                                          const double* geopotential = nullptr);
 // Sets the calling thread's exo_last_error() text; returns `code`.
 int set_error(int code, const std::string& message);
 size_t member_bytes(const Engine& e, const MemberDev& md);

@@ -156,11 +156,12 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
 
 namespace exo {
 
-std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision) {
+std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision, const double* land_fraction,
+                                          const double* geopotential) {
   if (precision != EXO_PRECISION_FP32 && precision != EXO_PRECISION_SPLIT)
     throw std::runtime_error("unknown precision");
   auto md = std::make_unique<MemberDev>();
-  build_member(cfg, md->host);
+  build_member(cfg, md->host, land_fraction, geopotential);
   md->precision = precision;
   md->step = md->host.start_step;
   return md;
@@ -375,10 +376,15 @@ void exo_destroy(exo_engine* h) {
 }
 
 int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* member) {
-  if (!h || !cfg || !member) return fail(EXO_E_ARG, "null argument");
+  return exo_add_member_boundary(h, cfg, precision, nullptr, nullptr, member);
+}
+
+int exo_add_member_boundary(exo_engine* h, const exo_config* cfg, int precision, const double* land_fraction,
+                            const double* geopotential, int* member) {
+  if (!h || !cfg || !member || (!land_fraction) != (!geopotential)) return fail(EXO_E_ARG, "null argument");
   std::unique_ptr<MemberDev> md;
   try {
-    md = exo::prepare_member(*cfg, precision);
+    md = exo::prepare_member(*cfg, precision, land_fraction, geopotential);
   } catch (const std::exception& ex) {
     return fail(EXO_E_ARG, ex.what());
   }

@@ -813,7 +813,7 @@ void member_date(const Member& m, int64_t step, int out[5]) {
   out[4] = d.minute;
 }
 
-void build_member(const exo_config& cfg, Member& m) {
+void build_member(const exo_config& cfg, Member& m, const double* land_fraction, const double* geopotential) {
   if (cfg.levels != 5 && cfg.levels != 10) throw std::runtime_error("Choose 5 or 10 atmospheric layers");
   if (!(cfg.timestep_minutes > 0)) throw std::runtime_error("timestep must be positive");
   if (!(cfg.land_fraction >= 0 && cfg.land_fraction <= 1)) throw std::runtime_error("land fraction must be in [0, 1]");
@@ -1052,7 +1052,13 @@ void build_member(const exo_config& cfg, Member& m) {
     spectral_to_grid(m, spec.data(), height.data());
   }
   std::vector<double> dls(NHOR, 0.0), orography(NHOR, 0.0);
-  {
+  if (land_fraction && geopotential) {
+    // boundary files 172 and 129 (setup/surface/index.ts:293-297)
+    for (int h = 0; h < NHOR; h++) {
+      dls[h] = land_fraction[h] > 0.5 ? 1 : 0;
+      orography[h] = geopotential[h];
+    }
+  } else {
     std::vector<double> sorted = height;
     std::sort(sorted.begin(), sorted.end());
     const int nland = (int)std::lround(cfg.land_fraction * NHOR);

@@ -65,7 +65,12 @@ struct Member {
 void config_default(exo_config* cfg);
 // Builds everything the device needs for one member; throws
 // std::runtime_error on an invalid configuration.
-void build_member(const exo_config& cfg, Member& m);
+// With land_fraction and geopotential (NHOR each, the grid's cells) the
+// planet's land mask and orography come from them (landini, setup/
+// surface/index.ts:293-297: land where the fraction exceeds 0.5) instead
+// of the seeded terrain.
+void build_member(const exo_config& cfg, Member& m, const double* land_fraction = nullptr,
+                  const double* geopotential = nullptr);
 // Calendar date {year, month, day, hour, minute} of main-loop step `step`
 // (calendar/date/index.ts:46-82, 360-day calendar).
 void member_date(const Member& m, int64_t step, int out[5]);

@@ -171,6 +171,8 @@ def library():
         "exo_native_find": (I, [ctypes.c_char_p, S, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
         "exo_boundary_from_terrain": (I, [P, F, F, I, I, I, ctypes.c_double, D, D]),
         "exo_boundary_sra": (I, [I, I, D, ctypes.c_char_p, S, ctypes.POINTER(S)]),
+        "exo_boundary_read_sra": (I, [I, I, ctypes.c_char_p, S, D]),
+        "exo_add_member_boundary": (I, [P, ctypes.POINTER(ExoConfig), I, D, D, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -230,6 +232,15 @@ def boundary_sra(code, nlat, values):
     out = ctypes.create_string_buffer(n.value)
     _check(lib.exo_boundary_sra(code, nlat, ptr, out, n.value, ctypes.byref(n)))
     return out.raw[:n.value].decode()
+
+
+def boundary_read_sra(code, nlat, text):
+    """The nlat x 2 nlat values of a one-field .sra text."""
+    lib = library()
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    out = np.empty(2 * nlat * nlat)
+    _check(lib.exo_boundary_read_sra(code, nlat, raw, len(raw), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out.reshape(nlat, 2 * nlat)
 
 
 def boundary_files(nlat, land_fraction, geopotential):
@@ -329,10 +340,24 @@ class Engine:
         self.close()
 
     # members ------------------------------------------------------------
-    def add_member(self, cfg=None, precision=EXO_PRECISION_FP32):
+    def add_member(self, cfg=None, precision=EXO_PRECISION_FP32, land_fraction=None, geopotential=None):
+        """Adds a planet; with land_fraction and geopotential (its grid's
+        cells) its land and orography come from them instead of the seeded
+        terrain."""
         cfg = cfg or default_config()
         mid = ctypes.c_int()
-        _check(self.lib.exo_add_member(self.h, ctypes.byref(cfg), precision, ctypes.byref(mid)))
+        if land_fraction is None and geopotential is None:
+            _check(self.lib.exo_add_member(self.h, ctypes.byref(cfg), precision, ctypes.byref(mid)))
+            return mid.value
+        lf = np.ascontiguousarray(np.asarray(land_fraction, dtype=np.float64).reshape(-1))
+        geo = np.ascontiguousarray(np.asarray(geopotential, dtype=np.float64).reshape(-1))
+        nlat = {21: 32, 42: 64, 63: 96}.get(cfg.truncation, 0)
+        if lf.size != 2 * nlat * nlat or geo.size != lf.size:
+            raise ExoError(-1, f"T{cfg.truncation} boundary fields need {2 * nlat * nlat} values each")
+        _check(self.lib.exo_add_member_boundary(self.h, ctypes.byref(cfg), precision,
+                                                lf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                geo.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                ctypes.byref(mid)))
         return mid.value
 
     def remove_member(self, member):
