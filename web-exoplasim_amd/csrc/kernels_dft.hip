@@ -11,11 +11,14 @@
 // split by the parity of m into two half-width ones: E[x] = sum over even
 // m, O[x] = sum over odd m (x < N/2); g[x] = E + O, g[x + N/2] = E - O:
 // the same terms as the full product, in another order, for half its
-// multiply-adds. (The forward product, split the same way, needs twice
-// the staged rows and ran slower; it stays whole.) Rows of one job are
-// contiguous in the arena (level-major
-// planes of latitude rows); the tables are the right operand, shared by
-// every row, job and member of a truncation.
+// multiply-adds. The forward product splits the same way: half-row sums
+// give the even wavenumbers, differences the odd ones. The tables are the
+// right operand, shared by every row, job and member of a truncation.
+//
+// Fourier slots are coefficient-major (coefficient k of row r at k R + r,
+// R the slot's rows): the inverse stages a chunk with lanes over its 128
+// rows, and the forward writes its results through LDS one coefficient of
+// 64 consecutive rows at a time, so both move contiguous runs.
 //
 // A workgroup (8 waves) takes 128 rows. Rows run over the jobs of a launch
 // and the members of a group (members of one truncation and level count,
@@ -64,6 +67,29 @@ __device__ __forceinline__ void store_regs(int rows, int cols, Store store, cons
     if (r < rows && c < cols) store(r, c, v[j]);
   }
 }
+
+// A chunk staged with lanes over its 128 rows: element e = threadIdx.x +
+// kThreads j is row e % 128, column e / 128, so a wave reads one
+// coefficient of 64 consecutive rows, contiguous in a coefficient-major
+// Fourier slot.
+template <int N, typename Load>
+__device__ __forceinline__ void load_rows(Load load, float* v) {
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int e = threadIdx.x + kThreads * j;
+    v[j] = load(e & (kRows - 1), e / kRows);
+  }
+}
+
+template <int N, typename Store>
+__device__ __forceinline__ void store_rows(Store store, const float* v) {
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int e = threadIdx.x + kThreads * j;
+    store(e & (kRows - 1), e / kRows, v[j]);
+  }
+}
+static_assert(kRows * kChunk == kAN * kThreads && (kRows & (kRows - 1)) == 0, "row staging");
 
 // Coefficient index k of the j-th coefficient of the wavenumbers of one
 // parity (0: m = 0, 2, ..; 1: m = 1, 3, ..).
@@ -121,10 +147,11 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember
   __shared__ const float* src[kRows];
   __shared__ float* dst[kRows];
   __shared__ int derived[kRows];
+  __shared__ int stride[kRows];  // rows of the row's Fourier slot: coefficient k at src + k * stride
   int block;
   const int grp = group_find(plan, blockIdx.x, &block);
   const DevMember& M = mems[plan.list[plan.start[grp]]];
-  const int nlat = M.nlat, nlon = M.nlon, nfc = M.nfc, ntp1 = M.ntru + 1;
+  const int nlat = M.nlat, nlon = M.nlon, ntp1 = M.ntru + 1;
   const auto rows_of = [&](const InvJob& x) { return inv_rows(M, x); };
   int per_member = 0;
   for (int j = 0; j < jobs.n; j++) per_member += rows_of(jobs.j[j]);
@@ -139,20 +166,22 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember
     const int row = row0 + threadIdx.x;
     const float* s = nullptr;
     float* d = nullptr;
-    int dl = 0;
+    int dl = 0, st = 0;
     if (row < total) {
       int g, r;
       const InvJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
       const DevMember& G = mems[plan.list[plan.start[grp] + g]];
       const int plane_rows = (job.per_level ? M.nlev : 1) * nlat;  // rows of one output
       const int o = r / plane_rows, pr = r - o * plane_rows;
-      s = G.arena + G.slot[(job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0] + pr * nfc;
+      s = G.arena + G.slot[(job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0] + pr;
       d = G.arena + G.slot[o == 0 ? job.dst0 : job.dst1] + pr * nlon;
       dl = job.kind == INV_DLAMBDA && o == 1;
+      st = plane_rows;
     }
     src[threadIdx.x] = s;
     dst[threadIdx.x] = d;
     derived[threadIdx.x] = dl;
+    stride[threadIdx.x] = st;
   }
   __syncthreads();
   // chunk c: parity c >= CE, coefficients j0 .. j0 + 32 of that parity
@@ -162,12 +191,12 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember
       const int j = j0 + x;
       const float* F = src[r];
       if (!F || j >= n) return 0.0f;
-      const int k = parity_k(parity, j);
+      const int k = parity_k(parity, j), R = stride[r];
       if (derived[r]) {
         const float m = (float)(k >> 1);
-        return (k & 1) ? m * F[k - 1] : -m * F[k + 1];
+        return (k & 1) ? m * F[(k - 1) * R] : -m * F[(k + 1) * R];
       }
-      return F[k];
+      return F[k * R];
     };
   };
   const auto b_load = [&](int c) {
@@ -178,14 +207,14 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember
 #pragma unroll
   for (int nt = 0; nt < kInvTiles; nt++) even[nt] = odd[nt] = f4{0, 0, 0, 0};
   float va[kAN], vb[kBN];
-  load_regs<kAN, 5>(kRows, kChunk, a_load(0), va);
+  load_rows<kAN>(a_load(0), va);
   load_regs<kBN, 7>(kChunk, half, b_load(0), vb);
   for (int c = 0; c < chunks; c++) {
-    store_regs<kAN, 5>(kRows, kChunk, [&](int r, int x, float v) { As[r * a_stride(kChunk) + x] = v; }, va);
+    store_rows<kAN>([&](int r, int x, float v) { As[r * a_stride(kChunk) + x] = v; }, va);
     store_regs<kBN, 7>(kChunk, half, [&](int y, int x, float v) { Bs[y * bs + x] = v; }, vb);
     __syncthreads();
     if (c + 1 < chunks) {
-      load_regs<kAN, 5>(kRows, kChunk, a_load(c + 1), va);
+      load_rows<kAN>(a_load(c + 1), va);
       load_regs<kBN, 7>(kChunk, half, b_load(c + 1), vb);
     }
     mfma_rows<kInvTiles>(As, Bs, bs, NT, c < CE ? even : odd);
@@ -216,6 +245,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_fwd(const DevMember
   __shared__ float Bs[kFwdBs];
   __shared__ const float* src[kRows];
   __shared__ float* dst[kRows];
+  __shared__ int stride[kRows];  // rows of the row's Fourier slot
   int block;
   const int grp = group_find(plan, blockIdx.x, &block);
   const DevMember& M = mems[plan.list[plan.start[grp]]];
@@ -234,15 +264,18 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_fwd(const DevMember
     const int row = row0 + threadIdx.x;
     const float* s = nullptr;
     float* d = nullptr;
+    int st = 0;
     if (row < total) {
       int g, r;
       const FwdJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
       const DevMember& G = mems[plan.list[plan.start[grp] + g]];
       s = G.arena + G.slot[job.src] + r * nlon;
-      d = G.arena + G.slot[job.dst] + r * nfc;
+      d = G.arena + G.slot[job.dst] + r;
+      st = rows_of(job);
     }
     src[threadIdx.x] = s;
     dst[threadIdx.x] = d;
+    stride[threadIdx.x] = st;
   }
   __syncthreads();
   const auto g_load = [&](int x0, int shift) {
@@ -284,17 +317,30 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_fwd(const DevMember
     mfma_rows<kFwdTiles / 2>(As + kAs, Bs + 64, bs, NTO, odd);
     __syncthreads();
   }
+  // Out through LDS, one parity at a time: the accumulators go to T[j][row]
+  // (rows padded to 129 floats), then each wave writes one coefficient of
+  // 64 consecutive rows, contiguous in the coefficient-major Fourier slot.
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float* Tt = As;
+  static_assert(64 * (kRows + 1) <= 2 * kAs, "transpose buffer");
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    float* F = dst[16 * wave + 4 * (lane >> 4) + r];
-    if (F) {
+  for (int parity = 0; parity < 2; parity++) {
+    const int K = parity ? KO : KE, NT = parity ? NTO : NTE;
 #pragma unroll
-      for (int nt = 0; nt < kFwdTiles / 2; nt++) {
+    for (int nt = 0; nt < kFwdTiles / 2; nt++) {
+      if (nt < NT) {
         const int j = 16 * nt + (lane & 15);
-        if (nt < NTE && j < KE) F[parity_k(0, j)] = even[nt][r];
-        if (nt < NTO && j < KO) F[parity_k(1, j)] = odd[nt][r];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          Tt[j * (kRows + 1) + 16 * wave + 4 * (lane >> 4) + r] = parity ? odd[nt][r] : even[nt][r];
       }
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < K * kRows; e += kThreads) {
+      const int j = e / kRows, row = e & (kRows - 1);
+      float* F = dst[row];
+      if (F) F[parity_k(parity, j) * stride[row]] = Tt[j * (kRows + 1) + row];
+    }
+    __syncthreads();  // T is rewritten by the next parity
   }
 }

@@ -50,14 +50,17 @@ __device__ __forceinline__ int job_of(const Jobs& jobs, int index) {
 // 2 mod 32 spreads them over 32 banks).
 __host__ __device__ constexpr int a_stride(int r) { return (r + 29) / 32 * 32 + 2; }
 
-// Staged inputs: inverse raw[row * kRawK + 2k + part] (k < K padded with
-// zeros to a multiple of 4); forward raw[row * kRawLat + 2 lat + part],
-// rows over (member, field, level).
+// Staged inputs: inverse raw[row * kPairRawK + 2k + part] (below); forward
+// raw[row * kRawLat + 2 lat + part], rows over (member, field, level).
 // Row strides are 4 mod 32: the 16 product columns one MFMA step reads
 // (8 rows x 2 parts) and its 2 row offsets then fall in 32 distinct banks.
-constexpr int kRawK = 2 * kMaxK + 4;
+//
+// Fourier coefficients live coefficient-major: coefficient k = 2m + part of
+// row (plane p, latitude lat) of a slot with R = planes x NLAT rows is
+// element k R + p NLAT + lat, so one wavenumber's coefficients of every
+// row and latitude are one contiguous block, which the per-wavenumber
+// workgroups read and write whole.
 constexpr int kRawLat = 2 * kMaxLat + 4;
-constexpr int kInvAs = kMaxLat * a_stride(kMaxK);
 constexpr int kFwdAs = kMaxK * a_stride(kMaxLat / 2);
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
@@ -234,38 +237,6 @@ __device__ __forceinline__ int group_member(const GroupPlan& plan, int g, int i)
   return plan.list[plan.start[g] + i];
 }
 
-// Batched staging: element e = threadIdx.x + kGroupThreads j (j < N) is row
-// e >> LOGC, column e & (2^LOGC - 1) of a rows x cols block. load_batch
-// issues all N loads before anything waits on them; store_batch then
-// writes them to LDS, so a staged block costs one memory latency.
-template <int N, int LOGC, typename Load>
-__device__ __forceinline__ void load_batch(int rows, int cols, Load load, float* v) {
-#pragma unroll
-  for (int j = 0; j < N; j++) {
-    const int e = threadIdx.x + kGroupThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
-    v[j] = (r < rows && c < cols) ? load(r, c) : 0.0f;
-  }
-}
-
-template <int N, int LOGC, typename Store>
-__device__ __forceinline__ void store_batch(int rows, int cols, Store store, const float* v) {
-#pragma unroll
-  for (int j = 0; j < N; j++) {
-    const int e = threadIdx.x + kGroupThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
-    if (r < rows && c < cols) store(r, c, v[j]);
-  }
-}
-
-// batch sizes: inverse table nlat x 64 (<= 96 x 64), inputs rows x 128;
-// forward table half x 64 (<= 48 x 64), inputs rows x 256 (2 nlat <= 192)
-constexpr int kInvTabN = kMaxLat * 64 / kGroupThreads;
-constexpr int kInvRawN = kInvRows * 128 / kGroupThreads;
-constexpr int kFwdTabN = kMaxLat / 2 * 64 / kGroupThreads;
-constexpr int kFwdRawN = kFwdRows * 256 / kGroupThreads;
-static_assert(kMaxLat * 64 % kGroupThreads == 0 && kInvRows * 128 % kGroupThreads == 0, "batch sizes");
-static_assert(kMaxLat / 2 * 64 % kGroupThreads == 0 && kFwdRows * 256 % kGroupThreads == 0, "batch sizes");
-static_assert(2 * kMaxK <= 128 && 2 * kMaxLat <= 256 && kMaxK <= 64, "batch columns");
-
 // ------------------------------------------------------------- inverse
 // A workgroup takes the wavenumber pair (m, T - m) of one job and member
 // group: K(m) + K(T - m) = T + 2 modes whatever m, so every workgroup does
@@ -319,7 +290,7 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1;
-  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
+  const int ncsp = M.ncsp, nrsp = M.nrsp;
   // the pair: m = pair (K = ntp1 - m modes) and T - m (K = m + 1)
   const int ma = pair, mb = ntp1 - 1 - pair, two = mb != ma;
   const int Ka = ntp1 - ma, Kb = two ? ntp1 - mb : 0;
@@ -433,7 +404,7 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
 #pragma unroll
             for (int r = 0; r < 4; r++) {
               const int lat = mt * 16 + 4 * kk + r;
-              base[(p * nlat + lat) * nfc + 2 * m + part] = acc[t][r];
+              base[(2 * m + part) * (P * nlat) + p * nlat + lat] = acc[t][r];
             }
           }
         }
@@ -460,7 +431,7 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, half = nlat / 2, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
-  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
+  const int ncsp = M.ncsp, nrsp = M.nrsp;
   const bool uv = job.kind == LEG_UV2DV;
   const int NCm = uv ? 4 * P : 2 * P;
   const int MT = (K + 15) / 16;
@@ -488,16 +459,22 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = G.arena + G.slot[fields[f]] + p * nlat * nfc + 2 * m;
+    rowp[row] = G.arena + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
   }
   // the members' Fourier coefficients m of every row and level, for the
   // chunk from group member c0
   const auto raw_loader = [&](int c0) {
     const int r0 = c0 * nf * P;
-    return [=](int row, int x) { return x < 2 * nlat ? rowp[r0 + row][(x >> 1) * nfc + (x & 1)] : 0.0f; };
+    // column c: part c / nlat, latitude c % nlat, so a wave reads
+    // consecutive latitudes of one coefficient
+    return [=](int row, int c) {
+      const int part = c >= nlat, lat = c - part * nlat;
+      return c < 2 * nlat ? rowp[r0 + row][part * (P * nlat) + lat] : 0.0f;
+    };
   };
-  const auto raw_store = [&](int row, int x, float v) {
-    if (x < 2 * nlat) raw[row * kRawLat + x] = v;
+  const auto raw_store = [&](int row, int c, float v) {
+    const int part = c >= nlat, lat = c - part * nlat;
+    if (c < 2 * nlat) raw[row * kRawLat + 2 * lat + part] = v;
   };
   __syncthreads();  // rowp
   // one memory latency for the northern half of each table,
@@ -640,7 +617,7 @@ struct SplitFwdLds {
 template <int P>
 __device__ void leg_inv_split(const DevMember& M, const InvJob& job, int m, SplitInvLds& L) {
   const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
-  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
+  const int ncsp = M.ncsp, nrsp = M.nrsp;
   const bool winds = job.kind == INV_DV2UV;
   const int NC = winds ? 4 * P : 2 * P;
   const int NT = (NC + 15) / 16, MT = nlat / 16;
@@ -702,7 +679,7 @@ __device__ void leg_inv_split(const DevMember& M, const InvJob& job, int m, Spli
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int lat = mt * 16 + 4 * g + r;
-          base[(p * nlat + lat) * nfc + 2 * m + part] = acc[t][r];
+          base[(2 * m + part) * (P * nlat) + p * nlat + lat] = acc[t][r];
         }
       }
     }
@@ -712,7 +689,7 @@ __device__ void leg_inv_split(const DevMember& M, const InvJob& job, int m, Spli
 template <int P>
 __device__ void leg_fwd_split(const DevMember& M, const LegJob& job, int m, SplitFwdLds& L) {
   const int nlat = M.nlat, half = nlat / 2, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
-  const int ncsp = M.ncsp, nrsp = M.nrsp, nfc = M.nfc;
+  const int ncsp = M.ncsp, nrsp = M.nrsp;
   const bool uv = job.kind == LEG_UV2DV;
   const int NC = uv ? 4 * P : 2 * P;
   const int NT = (NC + 15) / 16, MT = (K + 15) / 16;
@@ -734,7 +711,7 @@ __device__ void leg_fwd_split(const DevMember& M, const LegJob& job, int m, Spli
       nf * P, 2 * nlat, kSFwd,
       [&](int fp, int x) {
         const int f = fp / P, p = fp % P;
-        return M.arena[M.slot[fields[f]] + (p * nlat + (x >> 1)) * nfc + 2 * m + (x & 1)];
+        return M.arena[M.slot[fields[f]] + (2 * m + (x & 1)) * (P * nlat) + p * nlat + (x >> 1)];
       },
       L.bhi, L.blo, L.bscale);
   f4 acc[kMaxTiles];
