@@ -24,7 +24,7 @@ extern "C" __global__ void k_dft_fwd(const DevMember*, GroupPlan, FwdJobs);
 extern "C" __global__ void k_leg_fwd(const DevMember*, GroupPlan, LegJobs);
 extern "C" __global__ void k_leg_inv_split(const DevMember*, LaunchPlan, InvJobs);
 extern "C" __global__ void k_leg_fwd_split(const DevMember*, LaunchPlan, LegJobs);
-extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan, int);
+extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_mean_dekin(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
 extern "C" __global__ void k_calcgp(const DevMember*, LaunchPlan);
@@ -267,9 +267,9 @@ class Stepper {
     const LaunchPlan p = plan([](const DevMember& d) { return d.nlev; });
     launch(e_, k, dim3(p.first[p.n]), dim3(threads), e_.d_desc, p);
   }
-  void mkdheat(int stage) {
+  void mkdheat() {
     const LaunchPlan p = plan(tiles_levels);
-    launch(e_, k_mkdheat, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, stage);
+    launch(e_, k_mkdheat, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
   }
   void inv(const InvJobs& jobs) {
     // the Legendre half per operand form: plain FP32 over member groups,

@@ -135,20 +135,23 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
   }));
   // spectrald (dynamics/index.ts:239-296)
   s.coeff(k_spectrald_diffusion);
+  // mkdheat's three stages (dynamics/index.ts:249-289) in one pass each
+  // way: the winds and both increments' winds in one inverse (the first
+  // increments' in the frame's wind rows gu/gv and fgu/fgv, free by now),
+  // stages 1 and 2 in one mkdheat, then stage 3: zdekin -> zsdef -> mean
+  // mode -> zdekin -> heating (only spectral mode (0, 0) survives the
+  // mean-mode cut, k_mean_dekin). Stage 1's heating leaves through the
+  // frame's gtn and fgtn rows, stage 3's through zdtdt and fzdtdt, into
+  // one forward transform.
   s.inv(jobs_of<InvJobs, InvJob>({
       inv_winds(SLOT_sdp, SLOT_szp, SLOT_fzu, SLOT_fzv, SLOT_zu, SLOT_zv, 1),
-      inv_winds(SLOT_zdsdp1, SLOT_zdszp1, SLOT_fzdu, SLOT_fzdv, SLOT_zdu, SLOT_zdv, 0),
+      inv_winds(SLOT_zdsdp1, SLOT_zdszp1, SLOT_fgu, SLOT_fgv, SLOT_gu, SLOT_gv, 0),
+      inv_winds(SLOT_zdsdp2, SLOT_zdszp2, SLOT_fzdu, SLOT_fzdv, SLOT_zdu, SLOT_zdv, 0),
   }));
-  s.mkdheat(1);
-  s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
-  s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt1, 1)}));
-  s.inv(jobs_of<InvJobs, InvJob>({inv_winds(SLOT_zdsdp2, SLOT_zdszp2, SLOT_fzdu, SLOT_fzdv, SLOT_zdu, SLOT_zdv, 0)}));
-  s.mkdheat(2);
-  // zdekin -> zsdef -> mean mode -> zdekin -> heating, in one pass: only
-  // spectral mode (0, 0) survives the mean-mode cut (k_mean_dekin)
+  s.mkdheat();
   s.member_levels(k_mean_dekin, 1024);
-  s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
-  s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fzdtdt, SLOT_zstt2, 1)}));
+  s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_gtn, SLOT_fgtn, 1), fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
+  s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fgtn, SLOT_zstt1, 1), leg_plain(SLOT_fzdtdt, SLOT_zstt2, 1)}));
   s.coeff(k_spectrald_update);
 }
 

@@ -383,9 +383,14 @@ extern "C" __global__ __launch_bounds__(64) void k_physics_in(const DevMember* _
   EXO_LEVEL_DISPATCH(physics_in_column, M, h);
 }
 
-// dynamics/wgsl/mkdheat.wgsl:17-44, per level and point.
-extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __restrict__ mems, LaunchPlan plan,
-                                                            int stage) {
+// dynamics/wgsl/mkdheat.wgsl:17-44, stages 1 and 2 in one pass per level
+// and point. The reference runs them one after the other because both
+// use the increment winds' rows (zdu, zdv); here the first increments'
+// winds come in the frame's wind rows (gu, gv), free by then, and stage
+// 1's heating goes out through the frame's gtn row, so both stages read
+// the same winds (zu, zv) together: stage 1 heats from the first
+// increments, stage 2 takes the kinetic-energy defect of the second.
+extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
   const int i = tile * 64 + threadIdx.x;
@@ -396,22 +401,20 @@ extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __re
   const float ACPD = M.k[CON_acpd], ADV = M.k[CON_adv], DELTSEC2 = M.k[CON_deltsec2];
   const float heat = 1.0f + ADV * A[M.slot[SLOT_dq] + i];
   const float dsig = M.tab[M.tabo[TAB_dsigma] + l];
-  if (stage == 3) {
-    const float dp = A[M.slot[SLOT_dp] + h];
-    const float dtdt = -A[M.slot[SLOT_zdekin] + i] * 0.5f / ACPD / heat / dp * GA / dsig;
-    A[M.slot[SLOT_zdtdt] + i] = dtdt / CT / WW;
-    return;
-  }
   const float scale = CV * sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
   const float u = A[M.slot[SLOT_zu] + i] * scale;
   const float v = A[M.slot[SLOT_zv] + i] * scale;
-  const float du = A[M.slot[SLOT_zdu] + i] * scale;
-  const float dv = A[M.slot[SLOT_zdv] + i] * scale;
-  const float change = du * (2.0f * u + du) + dv * (2.0f * v + dv);
-  if (stage == 1) {
+  {
+    const float du = A[M.slot[SLOT_gu] + i] * scale;
+    const float dv = A[M.slot[SLOT_gv] + i] * scale;
+    const float change = du * (2.0f * u + du) + dv * (2.0f * v + dv);
     const float dtdt = -change / DELTSEC2 * 0.5f / ACPD / heat;
-    A[M.slot[SLOT_zdtdt] + i] = dtdt / CT / WW;
-  } else {
+    A[M.slot[SLOT_gtn] + i] = dtdt / CT / WW;
+  }
+  {
+    const float du = A[M.slot[SLOT_zdu] + i] * scale;
+    const float dv = A[M.slot[SLOT_zdv] + i] * scale;
+    const float change = du * (2.0f * u + du) + dv * (2.0f * v + dv);
     A[M.slot[SLOT_zdekin] + i] = change / DELTSEC2 * A[M.slot[SLOT_dp] + h] / GA * dsig;
   }
 }

@@ -66,7 +66,7 @@ def work_per_step(kernel, cfg):
     raise KeyError(f"no algorithmic work model for {kernel}")
 
 
-LAUNCHES_PER_STEP = {"k_leg_inv": 4, "k_dft_inv": 4, "k_leg_fwd": 4, "k_dft_fwd": 4, "k_surface_fluxes": 1,
+LAUNCHES_PER_STEP = {"k_leg_inv": 3, "k_dft_inv": 3, "k_leg_fwd": 3, "k_dft_fwd": 3, "k_surface_fluxes": 1,
                      "k_shortwave": 1, "k_longwave": 1, "k_convection": 1}
 
 
