@@ -171,3 +171,42 @@ def test_loaded_history_climatology(exo):
             eng.load_averager(m, {"current": None, "months": months + months[:1], "years": []})
     finally:
         eng.close()
+
+
+def test_averaging_members_in_a_mixed_batch(exo):
+    """Averaging is per member: in a batch of a T21 L5 planet that averages
+    and a T42 L10 planet that does not, the first's months equal the
+    oracle's fed with its own snapshots and the second holds none; the
+    second starts its months when averaging is switched on; removing an
+    averaging member frees it while the other goes on."""
+    every = 24 * 60 // 45
+    eng = exo.Engine(0, 2)
+    try:
+        a = eng.add_member(exo.default_config(truncation=21, levels=5, timestep_minutes=45))
+        b = eng.add_member(exo.default_config(truncation=42, levels=10, timestep_minutes=45))
+        eng.set_publish_every(every)
+        eng.set_averaging(a)
+        nlat = eng.dims(a)[1]
+        ref = oracle_avg.MonthlyAverager(np.polynomial.legendre.leggauss(nlat)[1])
+        while eng.member_step(a) % every != 1:
+            eng.step(1)
+        eng.sync()
+        date = eng.member_date(a)
+        ref.add(eng.read_snapshot(a), date[0], date[1], eng.member_step(a) - 1)
+        _run_days(eng, a, 35, every, ref)
+        assert eng.month_count(b) == 0 and eng.month_count(a) == len(ref.months) >= 1
+        for i, (info, mean) in enumerate(ref.months):
+            got_info, got = eng.read_month(a, i)
+            assert got_info == info
+            np.testing.assert_array_equal(got.reshape(-1), mean.reshape(-1))
+        eng.set_averaging(b)
+        _run_days(eng, a, 40, every)
+        assert eng.month_count(b) >= 1
+        info, _ = eng.read_accumulator(b)
+        assert info is not None and info["samples"] >= 1
+        eng.remove_member(a)
+        eng.step(every)
+        eng.sync()
+        assert np.all(np.isfinite(eng.read_snapshot(b)))
+    finally:
+        eng.close()
