@@ -133,3 +133,41 @@ def test_batch_members_match_alone(exo):
             alone.sync()
             a = alone.read_arena(i)
         assert np.array_equal(a, t)
+
+
+def test_ensemble_planets_first_step(exo):
+    """Planets drawn as the benchmark ensemble draws them (rotation period
+    0.5-100 days, insolation 0.6-1.4 x 1367 W/m^2, 0.3-3 bar, land 0-60 %),
+    two at each of T21 L5, T42 L10 and T63 L10 plus a T42 L5 and a T63 L5,
+    stepped together in one batch: every member's first step is within
+    1e-5 of the FP64 oracle started from its own set-up."""
+    import importlib
+    wl = importlib.import_module(exo.__name__ + ".workloads")
+    rng = np.random.default_rng(2026)
+    shapes = [(21, 5), (21, 5), (42, 10), (42, 10), (63, 10), (63, 10), (42, 5), (63, 5)]
+    cfgs = [wl._planet(exo, rng, t, l, 45) for t, l in shapes]
+    gpu = exo.Engine(0, len(cfgs))
+    host = exo.Engine(-1, len(cfgs))
+    try:
+        gids = [gpu.add_member(c) for c in cfgs]
+        hids = [host.add_member(c) for c in cfgs]
+        gpu.step(1)
+        gpu.sync()
+        for (t, l), c, g, h in zip(shapes, cfgs, gids, hids):
+            oracle = OracleRun(host, h)
+            oracle.step(1, physics=True)
+            k = host.constants(h, 0)
+            for name in SPECTRAL:
+                a = gpu.read_field(g, name).astype(np.float64)
+                b = oracle.field(name)
+                assert np.all(np.isfinite(a)), name
+                if name == "st":
+                    rel = np.sqrt(np.mean((a - b) ** 2)) * k["ct"] / np.sqrt(2.0) / 250.0
+                else:
+                    rel = _rms(a, b)
+                assert rel < 1e-5, (t, l, c.rotation_period_days, c.pressure_bar, name, rel)
+            print(f"T{t} L{l} rot {c.rotation_period_days:6.2f} d, {c.pressure_bar:4.2f} bar, "
+                  f"land {c.land_fraction:.2f}: step 1 within 1e-5")
+    finally:
+        gpu.close()
+        host.close()
