@@ -149,7 +149,9 @@ int exo_slot(exo_engine *e, int member, const char *name, size_t *offset,
 const char *exo_slot_name(int index);
 
 /* Field readback / write (LAYOUT.read / LAYOUT.write, layout/index.ts:
- * 66-108). `floats` must equal the slot's length. */
+ * 66-108). `floats` must equal the slot's length. Fourier slots read and
+ * write as the reference's rows (row r's 2(T+1) coefficients together),
+ * whatever their arrangement on the device; so does a checkpoint. */
 int exo_read_field(exo_engine *e, int member, const char *name, float *out,
                    size_t floats);
 int exo_write_field(exo_engine *e, int member, const char *name,

@@ -665,6 +665,43 @@ int exo_slot(exo_engine* h, int id, const char* name, size_t* offset, size_t* fl
   return EXO_OK;
 }
 
+namespace {
+// The arena keeps a Fourier slot coefficient-major (coefficient k of row r
+// at k R + r, R = planes x NLAT rows) so that one wavenumber of every row
+// is contiguous for the transforms. Field reads, writes and checkpoints
+// show the reference's row-major rows instead: row r's 2(T+1) coefficients
+// together at r 2(T+1) + k (layout/index.ts:14-20 plane sizes, rows north
+// to south per plane).
+void fourier_to_rows(const float* arena, float* rows, size_t floats, size_t nfc) {
+  const size_t R = floats / nfc;
+  for (size_t r = 0; r < R; r++)
+    for (size_t k = 0; k < nfc; k++) rows[r * nfc + k] = arena[k * R + r];
+}
+
+void rows_to_fourier(const float* rows, float* arena, size_t floats, size_t nfc) {
+  const size_t R = floats / nfc;
+  for (size_t k = 0; k < nfc; k++)
+    for (size_t r = 0; r < R; r++) arena[k * R + r] = rows[r * nfc + k];
+}
+
+// Rearranges every Fourier slot of a whole arena image in place; to_rows
+// picks the direction.
+void arena_fourier_rows(const MemberDev* m, float* image, bool to_rows) {
+  const exo::Layout& lay = m->host.layout;
+  const size_t nfc = (size_t)exo_nfc(m->host.dims.ntru);
+  std::vector<float> tmp;
+  for (int s = 0; s < SLOT_COUNT; s++) {
+    if (!lay.fourier[s]) continue;
+    float* slot = image + lay.offset[s];
+    tmp.assign(slot, slot + lay.floats[s]);
+    if (to_rows)
+      fourier_to_rows(tmp.data(), slot, lay.floats[s], nfc);
+    else
+      rows_to_fourier(tmp.data(), slot, lay.floats[s], nfc);
+  }
+}
+}  // namespace
+
 int exo_read_field(exo_engine* h, int id, const char* name, float* out, size_t floats) {
   MemberDev* m = member_of(h, id);
   if (!m || !out) return fail(EXO_E_ARG, "no member " + std::to_string(id));
@@ -676,10 +713,13 @@ int exo_read_field(exo_engine* h, int id, const char* name, float* out, size_t f
   Engine& e = h->e;
   if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
   hipSetDevice(e.device);
-  hipError_t err = hipMemcpyAsync(out, m->d_arena + m->host.layout.offset[s], floats * sizeof(float),
-                                  hipMemcpyDeviceToHost, e.stream);
+  const bool fourier = m->host.layout.fourier[s];
+  std::vector<float> staged(fourier ? floats : 0);
+  hipError_t err = hipMemcpyAsync(fourier ? staged.data() : out, m->d_arena + m->host.layout.offset[s],
+                                  floats * sizeof(float), hipMemcpyDeviceToHost, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  if (fourier) fourier_to_rows(staged.data(), out, floats, (size_t)exo_nfc(m->host.dims.ntru));
   return EXO_OK;
 }
 
@@ -694,8 +734,11 @@ int exo_write_field(exo_engine* h, int id, const char* name, const float* values
   Engine& e = h->e;
   if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
   hipSetDevice(e.device);
-  hipError_t err = hipMemcpyAsync(m->d_arena + m->host.layout.offset[s], values, floats * sizeof(float),
-                                  hipMemcpyHostToDevice, e.stream);
+  const bool fourier = m->host.layout.fourier[s];
+  std::vector<float> staged(fourier ? floats : 0);
+  if (fourier) rows_to_fourier(values, staged.data(), floats, (size_t)exo_nfc(m->host.dims.ntru));
+  hipError_t err = hipMemcpyAsync(m->d_arena + m->host.layout.offset[s], fourier ? staged.data() : values,
+                                  floats * sizeof(float), hipMemcpyHostToDevice, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
   return EXO_OK;
@@ -889,9 +932,12 @@ int exo_checkpoint_save(exo_engine* h, int id, void* out, size_t bytes) {
   const size_t start = (ml + 4 + json.size() + 3) / 4 * 4;
   Engine& e = h->e;
   hipSetDevice(e.device);
-  hipError_t err = hipMemcpyAsync(p + start, m->d_arena, m->host.layout.total * 4, hipMemcpyDeviceToHost, e.stream);
+  std::vector<float> image(m->host.layout.total);
+  hipError_t err = hipMemcpyAsync(image.data(), m->d_arena, image.size() * 4, hipMemcpyDeviceToHost, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  arena_fourier_rows(m, image.data(), true);
+  std::memcpy(p + start, image.data(), image.size() * 4);
   return EXO_OK;
 }
 
@@ -983,7 +1029,10 @@ int exo_checkpoint_restore(exo_engine* h, int id, const void* bytes, size_t size
   Engine& e = h->e;
   if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
   hipSetDevice(e.device);
-  hipError_t err = hipMemcpyAsync(m->d_arena, p + start, m->host.layout.total * 4, hipMemcpyHostToDevice, e.stream);
+  std::vector<float> image(m->host.layout.total);
+  std::memcpy(image.data(), p + start, image.size() * 4);
+  arena_fourier_rows(m, image.data(), false);
+  hipError_t err = hipMemcpyAsync(m->d_arena, image.data(), image.size() * 4, hipMemcpyHostToDevice, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
   // A restart skips the start-up steps (gcm.worker.ts:110-130).

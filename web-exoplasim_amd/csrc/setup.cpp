@@ -41,6 +41,7 @@ Layout make_layout(const ExoDims& d, int nsnap) {
   lay.offset.resize(SLOT_COUNT);
   lay.floats.resize(SLOT_COUNT);
   lay.plane.resize(SLOT_COUNT);
+  lay.fourier.resize(SLOT_COUNT);
   size_t at = 0;
 #define EXO_LAYOUT_SLOT(name, shape, count)          \
   {                                                  \
@@ -49,6 +50,7 @@ Layout make_layout(const ExoDims& d, int nsnap) {
     lay.offset[SLOT_##name] = at;                    \
     lay.floats[SLOT_##name] = n;                     \
     lay.plane[SLOT_##name] = plane;                  \
+    lay.fourier[SLOT_##name] = #shape[0] == 'F';     \
     at += n;                                         \
   }
   EXO_FIELDS(EXO_LAYOUT_SLOT)

@@ -15,6 +15,7 @@ namespace exo {
 
 struct Layout {
   std::vector<size_t> offset, floats, plane;  // per ExoSlot
+  std::vector<char> fourier;                  // per ExoSlot: shape F
   size_t total = 0;
 };
 
