@@ -151,7 +151,11 @@ const char *exo_slot_name(int index);
 /* Field readback / write (LAYOUT.read / LAYOUT.write, layout/index.ts:
  * 66-108). `floats` must equal the slot's length. Fourier slots read and
  * write as the reference's rows (row r's 2(T+1) coefficients together),
- * whatever their arrangement on the device; so does a checkpoint. */
+ * whatever their arrangement on the device; so does a checkpoint.
+ * Rows that are scratch inside a step can hold other intermediates at a
+ * step's end than the reference's: the dissipation tail borrows the
+ * physics frame's gu, gv, fgu, fgv, gtn and fgtn, and fzdekin is not
+ * formed. No step reads them before writing them. */
 int exo_read_field(exo_engine *e, int member, const char *name, float *out,
                    size_t floats);
 int exo_write_field(exo_engine *e, int member, const char *name,
