@@ -175,7 +175,6 @@ inline void launch(Engine& e, void (*k)(Params...), dim3 grid, dim3 block, Args.
 }
 
 inline int tiles_column(const DevMember& d) { return (d.nhor + 63) / 64; }
-inline int tiles_coeff(const DevMember& d) { return (d.nrsp + 63) / 64; }
 inline int tiles_levels(const DevMember& d) { return (d.nlev * d.nhor + 63) / 64; }
 
 template <typename Jobs>
@@ -254,9 +253,18 @@ class Stepper {
     const LaunchPlan p = plan(tiles_column);
     launch(e_, k, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
   }
-  void coeff(void (*k)(const DevMember*, LaunchPlan)) {
-    const LaunchPlan p = plan(tiles_coeff);
-    launch(e_, k, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
+  // one thread per spectral coefficient and level, 256 a workgroup
+  void coeff_levels(void (*k)(const DevMember*, LaunchPlan)) {
+    const LaunchPlan p = plan([](const DevMember& d) { return (d.nlev * d.nrsp + 255) / 256; });
+    launch(e_, k, dim3(p.first[p.n]), dim3(256), e_.d_desc, p);
+  }
+  // k_spectrala: 10 waves a workgroup over 640 / nlev coefficients
+  void spectrala() {
+    const LaunchPlan p = plan([](const DevMember& d) {
+      const int per = 64 * (10 / d.nlev);
+      return (d.nrsp + per - 1) / per;
+    });
+    launch(e_, k_spectrala, dim3(p.first[p.n]), dim3(640), e_.d_desc, p);
   }
   void levels(void (*k)(const DevMember*, LaunchPlan)) {
     const LaunchPlan p = plan(tiles_levels);

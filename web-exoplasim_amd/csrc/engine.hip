@@ -108,7 +108,7 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
       LegJob{LEG_QTEND, SLOT_fguq, SLOT_fgvq, -1, SLOT_fgqn, SLOT_sqt, -1, 1},
   }));
   // spectrala
-  s.coeff(k_spectrala);
+  s.spectrala();
   // gridpointd's frame in (dynamics/index.ts:299-320)
   s.inv(jobs_of<InvJobs, InvJob>({
       inv_winds(SLOT_sd, SLOT_sz, SLOT_fgu, SLOT_fgv, SLOT_gu, SLOT_gv, 1),
@@ -134,7 +134,7 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
       LegJob{LEG_UV2DV, SLOT_fgudt, SLOT_fgvdt, -1, -1, SLOT_sdt, SLOT_szt, 1},
   }));
   // spectrald (dynamics/index.ts:239-296)
-  s.coeff(k_spectrald_diffusion);
+  s.coeff_levels(k_spectrald_diffusion);
   // mkdheat's three stages (dynamics/index.ts:249-289) in one pass each
   // way: the winds and both increments' winds in one inverse (the first
   // increments' in the frame's wind rows gu/gv and fgu/fgv, free by now),
@@ -152,7 +152,7 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
   s.member_levels(k_mean_dekin, 1024);
   s.fwd(jobs_of<FwdJobs, FwdJob>({fwd_job(SLOT_gtn, SLOT_fgtn, 1), fwd_job(SLOT_zdtdt, SLOT_fzdtdt, 1)}));
   s.leg(jobs_of<LegJobs, LegJob>({leg_plain(SLOT_fgtn, SLOT_zstt1, 1), leg_plain(SLOT_fzdtdt, SLOT_zstt2, 1)}));
-  s.coeff(k_spectrald_update);
+  s.coeff_levels(k_spectrald_update);
 }
 
 }  // namespace exo

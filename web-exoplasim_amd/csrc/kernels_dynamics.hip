@@ -121,112 +121,115 @@ __device__ void calcgp_column(const DevMember& M, int h) {
   A[M.slot[SLOT_dp0] + h] = PSURF * ps;
 }
 
+// dynamics/wgsl/spectrala.wgsl:15-109, one thread per coefficient and
+// level. The semi-implicit solve couples a coefficient's levels three
+// times (the G products, the bm1 solve, the tau products); the level
+// vectors pass through LDS between them. A workgroup of 10 waves takes 64
+// coefficients of an L10 member or 128 of an L5 one (wave w: level w % NL
+// of coefficient tile w / NL). Every sum runs over the levels in the same
+// order as a one-thread-per-coefficient pass.
+constexpr int kSpaWaves = 10;
+
 template <int NL>
-__device__ void spectrala_coeff(const DevMember& M, int s) {
-  // dynamics/wgsl/spectrala.wgsl:15-109
+__device__ void spectrala_levels(const DevMember& M, int tile) {
+  constexpr int kSub = kSpaWaves / NL;
+  static_assert(kSub * NL == kSpaWaves, "levels must divide the workgroup's waves");
+  __shared__ float sh_stt[kSpaWaves][64], sh_atm[kSpaWaves][64], sh_zz[kSpaWaves][64], sh_sdt[kSpaWaves][64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int l = wave % NL, row0 = wave - l;
+  const int s = (tile * kSub + wave / NL) * 64 + lane;
   const int NRSP = M.nrsp;
+  const bool live = s < NRSP;
+  const int sc = live ? s : 0;
   float* A = M.arena;
   const float* T = M.tab;
-  auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
+  auto lev = [&](int slot) -> float& { return A[M.slot[slot] + l * NRSP + sc]; };
   auto tab = [&](int t, int i) { return T[M.tabo[t] + i]; };
   const float DELT = M.k[CON_delt], DELT2 = M.k[CON_delt2];
   const float PNU = M.k[CON_pnu], PNU21 = M.k[CON_pnu21];
   const bool FILTER = M.k[CON_filter] != 0.0f;
-  const float apm = A[M.slot[SLOT_spm] + s];
-  float adm[NL], azm[NL], atm[NL], aqm[NL], adt[NL], stt[NL];
+  const float adm = lev(SLOT_sdm), azm = lev(SLOT_szm), atm = lev(SLOT_stm), aqm = lev(SLOT_sqm);
+  const float adt = lev(SLOT_sdt), stt0 = lev(SLOT_stt);
+  const float apm = A[M.slot[SLOT_spm] + sc];
+  const float spt0 = A[M.slot[SLOT_spt] + sc];
+  const float spp = A[M.slot[SLOT_spp] + sc];
+  const float sop = tab(TAB_sop, sc);
+  const int jn = (int)tab(TAB_nindex, sc);
+  sh_stt[wave][lane] = stt0;
+  sh_atm[wave][lane] = atm;
+  __syncthreads();
+  float gt = 0, gm = 0;
 #pragma unroll
-  for (int l = 0; l < NL; l++) {
-    adm[l] = lev(SLOT_sdm, l);
-    azm[l] = lev(SLOT_szm, l);
-    atm[l] = lev(SLOT_stm, l);
-    aqm[l] = lev(SLOT_sqm, l);
-    adt[l] = lev(SLOT_sdt, l);
-    stt[l] = lev(SLOT_stt, l);
+  for (int k = 0; k < NL; k++) {
+    gt = gt + tab(TAB_g, l * NL + k) * sh_stt[row0 + k][lane];
+    gm = gm + tab(TAB_g, l * NL + k) * sh_atm[row0 + k][lane];
   }
-  float zgt[NL], zgm[NL];
+  // this level's term of the bm1 solve
+  const float zq = 1.0f / (float)(jn * (jn + 1));
+  const float z0 = tab(TAB_t0, l);
+  const float zt = gt - z0 * spt0;
+  const float zm = gm + z0 * apm;
+  const float za = adt * zq + sop;
+  const float zb = adm * zq;
+  sh_zz[wave][lane] = zb + DELT * (zm + za + DELT * zt);
+  __syncthreads();
+  float zsum = 0;
+  if (jn > 0) {
+    const float* bm1 = T + M.tabo[TAB_bm1] + ((jn - 1) * NL + l) * NL;
 #pragma unroll
-  for (int l = 0; l < NL; l++) {
-    float gt = 0, gm = 0;
-    for (int k = 0; k < NL; k++) {
-      gt = gt + tab(TAB_g, l * NL + k) * stt[k];
-      gm = gm + tab(TAB_g, l * NL + k) * atm[k];
-    }
-    zgt[l] = gt;
-    zgm[l] = gm;
+    for (int l2 = 0; l2 < NL; l2++) zsum = zsum + sh_zz[row0 + l2][lane] * bm1[l2];
   }
-  float spt = A[M.slot[SLOT_spt] + s];
-  const float spm = apm;
-  const float sop = tab(TAB_sop, s);
-  const int jn = (int)tab(TAB_nindex, s);
-  float sdt[NL];
+  const float sdt = s < 2 ? 0.0f : zsum;
+  sh_sdt[wave][lane] = sdt;
+  __syncthreads();
+  float tau = 0;
 #pragma unroll
-  for (int l = 0; l < NL; l++) {
-    float zsum = 0;
-    if (jn > 0) {
-      const float zq = 1.0f / (float)(jn * (jn + 1));
-      const float* bm1 = T + M.tabo[TAB_bm1] + ((jn - 1) * NL + l) * NL;
-      for (int l2 = 0; l2 < NL; l2++) {
-        const float z0 = tab(TAB_t0, l2);
-        const float zt = zgt[l2] - z0 * spt;
-        const float zm = zgm[l2] + z0 * spm;
-        const float za = adt[l2] * zq + sop;
-        const float zb = adm[l2] * zq;
-        const float zz = zb + DELT * (zm + za + DELT * zt);
-        zsum = zsum + zz * bm1[l2];
-      }
-    }
-    sdt[l] = s < 2 ? 0.0f : zsum;
+  for (int k = 0; k < NL; k++) tau = tau + tab(TAB_tau, l * NL + k) * sh_sdt[row0 + k][lane];
+  const float stt = stt0 - tau;
+  if (!live) return;
+  if (l == 0) {
+    float spt = spt0;
+#pragma unroll
+    for (int k = 0; k < NL; k++) spt = spt + tab(TAB_dsigma, k) * sh_sdt[row0 + k][lane];
+    if (FILTER) A[M.slot[SLOT_spm] + s] = PNU21 * spp + PNU * apm;
+    const float sppNew = s < 2 ? 0.0f : apm - DELT2 * spt;
+    A[M.slot[SLOT_spp] + s] = sppNew;
+    A[M.slot[SLOT_sp] + s] = sppNew;
+    A[M.slot[SLOT_spt] + s] = spt;
   }
-#pragma unroll
-  for (int l = 0; l < NL; l++) spt = spt + tab(TAB_dsigma, l) * sdt[l];
-#pragma unroll
-  for (int l = 0; l < NL; l++) {
-    float tau = 0;
-    for (int k = 0; k < NL; k++) tau = tau + tab(TAB_tau, l * NL + k) * sdt[k];
-    stt[l] = stt[l] - tau;
+  if (FILTER) {
+    lev(SLOT_sdm) = PNU21 * lev(SLOT_sdp) + PNU * adm;
+    lev(SLOT_szm) = PNU21 * lev(SLOT_szp) + PNU * azm;
+    lev(SLOT_stm) = PNU21 * lev(SLOT_stp) + PNU * atm;
+    lev(SLOT_sqm) = PNU21 * lev(SLOT_sqp) + PNU * aqm;
   }
-  const float spp = A[M.slot[SLOT_spp] + s];
-  if (FILTER) A[M.slot[SLOT_spm] + s] = PNU21 * spp + PNU * apm;
-  const float sppNew = s < 2 ? 0.0f : apm - DELT2 * spt;
-  A[M.slot[SLOT_spp] + s] = sppNew;
-  A[M.slot[SLOT_sp] + s] = sppNew;
-  A[M.slot[SLOT_spt] + s] = spt;
-#pragma unroll
-  for (int l = 0; l < NL; l++) {
-    if (FILTER) {
-      lev(SLOT_sdm, l) = PNU21 * lev(SLOT_sdp, l) + PNU * adm[l];
-      lev(SLOT_szm, l) = PNU21 * lev(SLOT_szp, l) + PNU * azm[l];
-      lev(SLOT_stm, l) = PNU21 * lev(SLOT_stp, l) + PNU * atm[l];
-      lev(SLOT_sqm, l) = PNU21 * lev(SLOT_sqp, l) + PNU * aqm[l];
-    }
-    const float sdp = 2.0f * sdt[l] - adm[l];
-    const float stp = DELT2 * stt[l] + atm[l];
-    const float szp = DELT2 * lev(SLOT_szt, l) + azm[l];
-    const float sqp = DELT2 * lev(SLOT_sqt, l) + aqm[l];
-    lev(SLOT_sdt, l) = sdt[l];
-    lev(SLOT_stt, l) = stt[l];
-    lev(SLOT_sdp, l) = sdp;
-    lev(SLOT_stp, l) = stp;
-    lev(SLOT_szp, l) = szp;
-    lev(SLOT_sqp, l) = sqp;
-    lev(SLOT_sd, l) = sdp;
-    lev(SLOT_st, l) = stp;
-    lev(SLOT_sz, l) = szp;
-    lev(SLOT_sq, l) = sqp;
-  }
+  const float sdp = 2.0f * sdt - adm;
+  const float stp = DELT2 * stt + atm;
+  const float szp = DELT2 * lev(SLOT_szt) + azm;
+  const float sqp = DELT2 * lev(SLOT_sqt) + aqm;
+  lev(SLOT_sdt) = sdt;
+  lev(SLOT_stt) = stt;
+  lev(SLOT_sdp) = sdp;
+  lev(SLOT_stp) = stp;
+  lev(SLOT_szp) = szp;
+  lev(SLOT_sqp) = sqp;
+  lev(SLOT_sd) = sdp;
+  lev(SLOT_st) = stp;
+  lev(SLOT_sz) = szp;
+  lev(SLOT_sq) = sqp;
 }
 
-template <int NL>
-__device__ void diffusion_coeff(const DevMember& M, int s) {
+// One thread per coefficient and level: the diffusion and the update
+// touch no other level.
+__device__ void diffusion_coeff(const DevMember& M, int s, int l) {
   // dynamics/wgsl/spectrald-diffusion.wgsl:12-60
   const int NRSP = M.nrsp;
   float* A = M.arena;
   const float* T = M.tab;
   auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
   const float DELT2 = M.k[CON_delt2], PLAVOR = M.k[CON_plavor];
-  if (s < 2) A[M.slot[SLOT_spp] + s] = 0.0f;
-#pragma unroll
-  for (int l = 0; l < NL; l++) {
+  if (s < 2 && l == 0) A[M.slot[SLOT_spp] + s] = 0.0f;
+  {
     const float szp = lev(SLOT_szp, l) + DELT2 * lev(SLOT_szt, l);
     const float stp = lev(SLOT_stp, l) + DELT2 * lev(SLOT_stt, l);
     const float sdp = lev(SLOT_sdp, l) + DELT2 * lev(SLOT_sdt, l);
@@ -266,19 +269,19 @@ __device__ void diffusion_coeff(const DevMember& M, int s) {
   }
 }
 
-template <int NL>
-__device__ void update_coeff(const DevMember& M, int s) {
+__device__ void update_coeff(const DevMember& M, int s, int l) {
   // dynamics/wgsl/spectrald-update.wgsl:12-45
   const int NRSP = M.nrsp;
   float* A = M.arena;
   auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
   const float DELT2 = M.k[CON_delt2], PNU = M.k[CON_pnu];
   const bool FILTER = M.k[CON_filter] != 0.0f;
-  const float spp = A[M.slot[SLOT_spp] + s];
-  if (FILTER) A[M.slot[SLOT_spm] + s] = A[M.slot[SLOT_spm] + s] + PNU * spp;
-  A[M.slot[SLOT_sp] + s] = spp;
-#pragma unroll
-  for (int l = 0; l < NL; l++) {
+  if (l == 0) {
+    const float spp = A[M.slot[SLOT_spp] + s];
+    if (FILTER) A[M.slot[SLOT_spm] + s] = A[M.slot[SLOT_spm] + s] + PNU * spp;
+    A[M.slot[SLOT_sp] + s] = spp;
+  }
+  {
     const float stt = lev(SLOT_stt, l) + lev(SLOT_zstt1, l) + lev(SLOT_zstt2, l);
     lev(SLOT_stt, l) = stt;
     const float szp = lev(SLOT_szp, l) + DELT2 * lev(SLOT_szt, l);
@@ -349,30 +352,30 @@ extern "C" __global__ __launch_bounds__(64) void k_calcgp(const DevMember* __res
   EXO_LEVEL_DISPATCH(calcgp_column, M, h);
 }
 
-extern "C" __global__ __launch_bounds__(64) void k_spectrala(const DevMember* __restrict__ mems, LaunchPlan plan) {
+extern "C" __global__ __launch_bounds__(640) void k_spectrala(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int s = tile * 64 + threadIdx.x;
-  if (s >= M.nrsp) return;
-  EXO_LEVEL_DISPATCH(spectrala_coeff, M, s);
+  EXO_LEVEL_DISPATCH(spectrala_levels, M, tile);
 }
 
-extern "C" __global__ __launch_bounds__(64) void k_spectrald_diffusion(const DevMember* __restrict__ mems,
-                                                                        LaunchPlan plan) {
+extern "C" __global__ __launch_bounds__(256) void k_spectrald_diffusion(const DevMember* __restrict__ mems,
+                                                                         LaunchPlan plan) {
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int s = tile * 64 + threadIdx.x;
-  if (s >= M.nrsp) return;
-  EXO_LEVEL_DISPATCH(diffusion_coeff, M, s);
+  const int i = tile * 256 + threadIdx.x;
+  if (i >= M.nlev * M.nrsp) return;
+  const int l = i / M.nrsp;
+  diffusion_coeff(M, i - l * M.nrsp, l);
 }
 
-extern "C" __global__ __launch_bounds__(64) void k_spectrald_update(const DevMember* __restrict__ mems,
-                                                                     LaunchPlan plan) {
+extern "C" __global__ __launch_bounds__(256) void k_spectrald_update(const DevMember* __restrict__ mems,
+                                                                      LaunchPlan plan) {
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int s = tile * 64 + threadIdx.x;
-  if (s >= M.nrsp) return;
-  EXO_LEVEL_DISPATCH(update_coeff, M, s);
+  const int i = tile * 256 + threadIdx.x;
+  if (i >= M.nlev * M.nrsp) return;
+  const int l = i / M.nrsp;
+  update_coeff(M, i - l * M.nrsp, l);
 }
 
 extern "C" __global__ __launch_bounds__(64) void k_physics_in(const DevMember* __restrict__ mems, LaunchPlan plan) {
