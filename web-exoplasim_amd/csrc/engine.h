@@ -275,6 +275,11 @@ class Stepper {
     const LaunchPlan p = plan([](const DevMember& d) { return d.nlev; });
     launch(e_, k, dim3(p.first[p.n]), dim3(threads), e_.d_desc, p);
   }
+  // one thread per point and plane (nlev + extra planes), 256 a workgroup
+  void point_levels(void (*k)(const DevMember*, LaunchPlan), int extra = 0) {
+    const LaunchPlan p = plan([extra](const DevMember& d) { return ((d.nlev + extra) * d.nhor + 255) / 256; });
+    launch(e_, k, dim3(p.first[p.n]), dim3(256), e_.d_desc, p);
+  }
   void mkdheat() {
     const LaunchPlan p = plan(tiles_levels);
     launch(e_, k_mkdheat, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);

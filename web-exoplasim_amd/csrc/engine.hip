@@ -118,7 +118,7 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
       // rainstep's mkdqtgp (physics/precipitation/index.ts:103-113)
       inv_plain(TAB_legendre_qi, SLOT_sqt, SLOT_fdqt, SLOT_dqt, 1),
   }));
-  s.column(k_physics_in);
+  s.point_levels(k_physics_in, 1);
   // parameterizations and the frame out's tendency conversion
   enqueue_physics(e, active);
   // gridpointd's frame out (dynamics/index.ts:321-344)

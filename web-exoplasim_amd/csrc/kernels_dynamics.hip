@@ -305,33 +305,30 @@ __device__ void update_coeff(const DevMember& M, int s, int l) {
   }
 }
 
-template <int NL>
-__device__ void physics_in_column(const DevMember& M, int h) {
-  // dynamics/wgsl/physics-in.wgsl:8-34
-  const int NHOR = M.nhor;
+// dynamics/wgsl/physics-in.wgsl:8-34 at one point and level (levels
+// 0..NL: the tendency rows have NL + 1 planes); each thread takes dp from
+// gp itself, the level-0 thread stores it.
+__device__ void physics_in_point(const DevMember& M, int h, int l) {
+  const int NHOR = M.nhor, NL = M.nlev;
   float* A = M.arena;
   const float PSURF = M.k[CON_psurf], CV = M.k[CON_cv], CT = M.k[CON_ct];
+  const int at = l * NHOR + h;
+  A[M.slot[SLOT_dudt] + at] = 0.0f;
+  A[M.slot[SLOT_dvdt] + at] = 0.0f;
+  A[M.slot[SLOT_dtdt] + at] = 0.0f;
+  A[M.slot[SLOT_dqdt] + at] = 0.0f;
+  if (l >= NL) return;
   const float dp = PSURF * expf(A[M.slot[SLOT_gp] + h]);
-  A[M.slot[SLOT_dp] + h] = dp;
+  if (l == 0) A[M.slot[SLOT_dp] + h] = dp;
   const float scale = sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
-#pragma unroll
-  for (int l = 0; l < NL + 1; l++) {
-    const int at = l * NHOR + h;
-    A[M.slot[SLOT_dudt] + at] = 0.0f;
-    A[M.slot[SLOT_dvdt] + at] = 0.0f;
-    A[M.slot[SLOT_dtdt] + at] = 0.0f;
-    A[M.slot[SLOT_dqdt] + at] = 0.0f;
-    if (l < NL) {
-      A[M.slot[SLOT_gudt] + at] = 0.0f;
-      A[M.slot[SLOT_gvdt] + at] = 0.0f;
-      A[M.slot[SLOT_gtdt] + at] = 0.0f;
-      A[M.slot[SLOT_gqdt] + at] = 0.0f;
-      A[M.slot[SLOT_du] + at] = CV * A[M.slot[SLOT_gu] + at] * scale;
-      A[M.slot[SLOT_dv] + at] = CV * A[M.slot[SLOT_gv] + at] * scale;
-      A[M.slot[SLOT_dt] + at] = CT * (A[M.slot[SLOT_gt] + at] + M.tab[M.tabo[TAB_t0] + l]);
-      A[M.slot[SLOT_dq] + at] = A[M.slot[SLOT_gq] + at] * PSURF / dp;
-    }
-  }
+  A[M.slot[SLOT_gudt] + at] = 0.0f;
+  A[M.slot[SLOT_gvdt] + at] = 0.0f;
+  A[M.slot[SLOT_gtdt] + at] = 0.0f;
+  A[M.slot[SLOT_gqdt] + at] = 0.0f;
+  A[M.slot[SLOT_du] + at] = CV * A[M.slot[SLOT_gu] + at] * scale;
+  A[M.slot[SLOT_dv] + at] = CV * A[M.slot[SLOT_gv] + at] * scale;
+  A[M.slot[SLOT_dt] + at] = CT * (A[M.slot[SLOT_gt] + at] + M.tab[M.tabo[TAB_t0] + l]);
+  A[M.slot[SLOT_dq] + at] = A[M.slot[SLOT_gq] + at] * PSURF / dp;
 }
 
 }  // namespace
@@ -378,12 +375,13 @@ extern "C" __global__ __launch_bounds__(256) void k_spectrald_update(const DevMe
   update_coeff(M, i - l * M.nrsp, l);
 }
 
-extern "C" __global__ __launch_bounds__(64) void k_physics_in(const DevMember* __restrict__ mems, LaunchPlan plan) {
+extern "C" __global__ __launch_bounds__(256) void k_physics_in(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  const int h = tile * 64 + threadIdx.x;
-  if (h >= M.nhor) return;
-  EXO_LEVEL_DISPATCH(physics_in_column, M, h);
+  const int i = tile * 256 + threadIdx.x;
+  if (i >= (M.nlev + 1) * M.nhor) return;
+  const int l = i / M.nhor;
+  physics_in_point(M, i - l * M.nhor, l);
 }
 
 // dynamics/wgsl/mkdheat.wgsl:17-44, stages 1 and 2 in one pass per level
