@@ -1,11 +1,19 @@
 // Dynamical core of the ExoPlaSim step on gfx950: gridpointa's column
 // work, the semi-implicit spectrala, spectrald's implicit diffusion and
 // time filter, mkdheat's grid-point stages, and gridpointd's physics frame.
-// One lane per grid column or per real spectral coefficient; the level
-// count is a template parameter so the column lives in registers.
+// spectrala, whose levels couple, runs one thread per coefficient and level
+// and passes level vectors through LDS; calcgp and mkdheat's stages one
+// thread per column or point; the rest one thread per point or coefficient
+// and level. The level count is a template parameter.
 #include "device.h"
 
 namespace {
+
+// Waves of a level-coupled workgroup: 64 coefficients of an L10 member,
+// 128 of an L5 one (wave w: level w % NL of tile w / NL).
+constexpr int kLevelWaves = 10;
+
+typedef float LevelRows[kLevelWaves][64];
 
 template <int NL>
 __device__ void calcgp_column(const DevMember& M, int h) {
@@ -124,17 +132,14 @@ __device__ void calcgp_column(const DevMember& M, int h) {
 // dynamics/wgsl/spectrala.wgsl:15-109, one thread per coefficient and
 // level. The semi-implicit solve couples a coefficient's levels three
 // times (the G products, the bm1 solve, the tau products); the level
-// vectors pass through LDS between them. A workgroup of 10 waves takes 64
-// coefficients of an L10 member or 128 of an L5 one (wave w: level w % NL
-// of coefficient tile w / NL). Every sum runs over the levels in the same
-// order as a one-thread-per-coefficient pass.
-constexpr int kSpaWaves = 10;
+// vectors pass through LDS between them. Every sum runs over the levels in
+// the same order as a one-thread-per-coefficient pass.
 
 template <int NL>
-__device__ void spectrala_levels(const DevMember& M, int tile) {
-  constexpr int kSub = kSpaWaves / NL;
-  static_assert(kSub * NL == kSpaWaves, "levels must divide the workgroup's waves");
-  __shared__ float sh_stt[kSpaWaves][64], sh_atm[kSpaWaves][64], sh_zz[kSpaWaves][64], sh_sdt[kSpaWaves][64];
+__device__ void spectrala_levels(const DevMember& M, int tile, LevelRows* lds) {
+  constexpr int kSub = kLevelWaves / NL;
+  static_assert(kSub * NL == kLevelWaves, "levels must divide the workgroup's waves");
+  LevelRows &sh_stt = lds[0], &sh_atm = lds[1], &sh_zz = lds[2], &sh_sdt = lds[3];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int l = wave % NL, row0 = wave - l;
   const int s = (tile * kSub + wave / NL) * 64 + lane;
@@ -352,7 +357,11 @@ extern "C" __global__ __launch_bounds__(64) void k_calcgp(const DevMember* __res
 extern "C" __global__ __launch_bounds__(640) void k_spectrala(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
-  EXO_LEVEL_DISPATCH(spectrala_levels, M, tile);
+  __shared__ LevelRows lds[4];
+  if (M.nlev == 10)
+    spectrala_levels<10>(M, tile, lds);
+  else
+    spectrala_levels<5>(M, tile, lds);
 }
 
 extern "C" __global__ __launch_bounds__(256) void k_spectrald_diffusion(const DevMember* __restrict__ mems,
