@@ -38,22 +38,33 @@ def _package():
     return entry._package()
 
 
-def run_gpu(exo, cfgs, steps, warmup, probe, precision=0):
-    """Steps every config together; returns (wall s, device ms, probe)."""
+def probe_steps(steps):
+    """Steps of the probe window that follows the timed region."""
+    return min(steps, 96)
+
+
+def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True):
+    """Steps every config together; returns (wall s, device ms, probe).
+    The timed steps replay the captured step graph (graphs=True) or launch
+    every kernel (False); the probe then times single launches of its
+    kernel over a further window of steps, after the timed region."""
     eng = exo.Engine(0, len(cfgs))
     try:
+        eng.set_step_graphs(graphs)
         ids = [eng.add_member(c, precision=precision) for c in cfgs]
         eng.step(warmup)
         eng.sync()
-        if probe:
-            eng.probe_kernel(probe, 65536)
         t0 = time.perf_counter()
         eng.step(steps)
         eng.sync()
         wall = time.perf_counter() - t0
         dev_ms = eng.last_step_ms()
-        probed = eng.probe_result() if probe else None
+        probed = None
         if probe:
+            eng.probe_kernel(probe, 65536)
+            eng.step(probe_steps(steps))
+            eng.sync()
+            probed = eng.probe_result()
             eng.probe_kernel(None)
         for i in ids:
             ts, _ = eng.global_means(i)
@@ -223,6 +234,10 @@ def main():
         line["plain_single"] = {"value": plain_value, "unit": "simulated days/s",
                                 "ms_per_step": pw * 1000.0 / 480,
                                 "config": "plain FP32 single planet T63 L10, 30-min step"}
+        ew, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None, graphs=False)
+        line["plain_single_eager"] = {"value": (480 / 48) / ew, "unit": "simulated days/s",
+                                      "ms_per_step": 1e3 * ew / 480,
+                                      "config": "as plain_single, every kernel launched (no step graph)"}
         line["batched_vs_plain"] = value / plain_value
         if "batched_split_fp16" in line:
             line["batched_split_vs_plain"] = line["batched_split_fp16"]["value"] / plain_value
@@ -230,7 +245,8 @@ def main():
         line["serving"] = run_serving(exo, args.serve_seconds)
     if probed:
         total_ms, launches, missed = probed
-        line["roofline"] = roofline.figure(exo, probe, cfgs, total_ms, launches, missed, args.steps)
+        line["roofline"] = roofline.figure(exo, probe, cfgs, total_ms, launches, missed,
+                                           probe_steps(args.steps))
     else:
         line["roofline"] = None
     line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(exo, cfgs, args.cpu_seconds)

@@ -130,6 +130,14 @@ int exo_step(exo_engine *e, int steps);
  * GcmRunRequest.publishEvery (runner/types.ts:47-67). */
 int exo_set_publish_every(exo_engine *e, int every);
 
+/* Replay each step of exo_step from a captured HIP graph (enable != 0,
+ * the default) or launch its kernels one by one (0). The graph is captured
+ * when two consecutive steps of a call have the same active members and
+ * start-up phases, and is kept while they last; results are bit-identical
+ * either way. The reference submits one WebGPU command buffer per batch of
+ * steps (stepper/index.ts:175-276); this is that batching's launch side. */
+int exo_set_step_graphs(exo_engine *e, int enable);
+
 /* Blocks until queued steps have finished. Stands in for mapAsync. */
 int exo_sync(exo_engine *e);
 

@@ -27,6 +27,8 @@ extern "C" __global__ void k_leg_fwd_split(const DevMember*, LaunchPlan, LegJobs
 extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_mean_dekin(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
+extern "C" __global__ void k_load_step_at(const DevMember*, LaunchPlan, const float*, const int*, int);
+extern "C" __global__ void k_step_index(int*, int);
 extern "C" __global__ void k_calcgp(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_spectrala(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_spectrald_diffusion(const DevMember*, LaunchPlan);
@@ -159,6 +161,17 @@ struct Engine {
   bool timed = false;
   int publish_every = 0;
   KernelProbe probe;
+  // Step graph: one captured step (calendar values, dynamics, physics,
+  // snapshot pack) replayed while its key holds. The key is the active
+  // descriptors, their loaded phases, the calendar buffer and `epoch`,
+  // which admit_member and drop_member bump. The captured k_load_step_at
+  // reads its calendar row at *d_step_index, which the graph's last node
+  // advances.
+  bool graphs = true;
+  uint64_t epoch = 0;
+  std::vector<int64_t> graph_key;
+  hipGraphExec_t graph = nullptr;
+  int* d_step_index = nullptr;
 };
 
 // Every launch of the step goes through here, so the probe can bracket
@@ -322,6 +335,11 @@ class Stepper {
   void load_step(const float* d_params) {
     const LaunchPlan p = plan([](const DevMember&) { return 1; });
     launch(e_, k_load_step, dim3(p.n), dim3(64), e_.d_desc, p, d_params);
+  }
+  // The calendar row at base + *index * per_step (the step graph's form).
+  void load_step_at(const float* base, const int* index, int per_step) {
+    const LaunchPlan p = plan([](const DevMember&) { return 1; });
+    launch(e_, k_load_step_at, dim3(p.n), dim3(64), e_.d_desc, p, base, index, per_step);
   }
 
  private:

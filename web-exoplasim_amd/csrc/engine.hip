@@ -53,13 +53,10 @@ void release_shared_tables(exo::Engine& e, int key) {
 
 // Writes each member's 16 per-step values into its arena's table slots
 // (the params-to-state copies of STEPPER.batch, stepper/index.ts:242-255).
-extern "C" __global__ __launch_bounds__(64) void k_load_step(const DevMember* __restrict__ mems, LaunchPlan plan,
-                                                              const float* __restrict__ params) {
-  const int i = blockIdx.x;  // one block per planned member
-  if (i >= plan.n || threadIdx.x >= EXO_STEP_VALUES) return;
-  const DevMember& M = mems[plan.member[i]];
-  const float v = params[i * EXO_STEP_VALUES + threadIdx.x];
-  const int t = threadIdx.x;
+namespace {
+
+// Value t of a member's calendar row into its step slots.
+__device__ __forceinline__ void store_step_value(const DevMember& M, int t, float v) {
   int slot, at;
   if (t < 5) {
     slot = SLOT_radiation_step;
@@ -75,6 +72,30 @@ extern "C" __global__ __launch_bounds__(64) void k_load_step(const DevMember* __
     at = t - 13;
   }
   M.arena[M.slot[slot] + at] = v;
+}
+
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(64) void k_load_step(const DevMember* __restrict__ mems, LaunchPlan plan,
+                                                              const float* __restrict__ params) {
+  const int i = blockIdx.x;  // one block per planned member
+  if (i >= plan.n || threadIdx.x >= EXO_STEP_VALUES) return;
+  store_step_value(mems[plan.member[i]], threadIdx.x, params[i * EXO_STEP_VALUES + threadIdx.x]);
+}
+
+// The step graph's form: the calendar rows of step *index of the call.
+extern "C" __global__ __launch_bounds__(64) void k_load_step_at(const DevMember* __restrict__ mems, LaunchPlan plan,
+                                                                 const float* __restrict__ base,
+                                                                 const int* __restrict__ index, int per_step) {
+  const int i = blockIdx.x;
+  if (i >= plan.n || threadIdx.x >= EXO_STEP_VALUES) return;
+  const float* params = base + (size_t)(*index) * per_step;
+  store_step_value(mems[plan.member[i]], threadIdx.x, params[i * EXO_STEP_VALUES + threadIdx.x]);
+}
+
+// Sets the step graph's calendar index (value >= 0) or advances it (-1).
+extern "C" __global__ void k_step_index(int* index, int value) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *index = value >= 0 ? value : *index + 1;
 }
 
 namespace exo {
@@ -279,6 +300,7 @@ int admit_member(Engine& e, std::unique_ptr<MemberDev> md) {
     e.members.emplace_back();
   }
   e.members[id] = std::move(md);
+  e.epoch++;
   return id;
 }
 
@@ -294,6 +316,7 @@ void drop_member(Engine& e, int id) {
     release_shared_tables(e, m->shared_key);
   }
   e.members[id].reset();
+  e.epoch++;
 }
 
 }  // namespace exo
@@ -342,6 +365,7 @@ int exo_create(int device, int max_members, exo_engine** out) {
     HIP_OR_THROW(hipEventCreateWithFlags(&e.params_done, hipEventDisableTiming));
     HIP_OR_THROW(hipEventCreate(&e.t0));
     HIP_OR_THROW(hipEventCreate(&e.t1));
+    HIP_OR_THROW(hipMalloc(&e.d_step_index, sizeof(int)));
   } catch (const std::exception& ex) {
     return fail(EXO_E_DEVICE, ex.what());
   }
@@ -366,6 +390,8 @@ void exo_destroy(exo_engine* h) {
       hipFree(m->d_bm1);
     }
   for (auto& entry : e.transform_tables) hipFree(entry.second.d);
+  if (e.graph) hipGraphExecDestroy(e.graph);
+  hipFree(e.d_step_index);
   hipFree(e.d_desc);
   hipFree(e.d_params);
   if (e.h_params) hipHostFree(e.h_params);
@@ -466,6 +492,20 @@ int exo_step(exo_engine* h, int steps) {
     HIP_OR_THROW(hipMemcpyAsync(e.d_params, e.h_params, need * sizeof(float), hipMemcpyHostToDevice, e.stream));
     HIP_OR_THROW(hipEventRecord(e.params_done, e.stream));
     HIP_OR_THROW(hipEventRecord(e.t0, e.stream));
+    // A step's launches depend only on the active members and their loaded
+    // phases (the storm plan reads the phase constants), so steps with one
+    // key replay one graph. The kernel probe brackets single launches and
+    // needs the launches themselves.
+    const bool graphs = e.graphs && !e.probe.kernel;
+    const auto key_at = [&](int k) {
+      std::vector<int64_t> key{(int64_t)e.epoch, (int64_t)(intptr_t)e.d_params, (int64_t)per_step};
+      for (size_t i = 0; i < act.size(); i++) {
+        key.push_back(desc[i]);
+        key.push_back(phase_at[k][i]);
+      }
+      return key;
+    };
+    int index_at = -1;  // the calendar index the device holds, -1 unknown
     for (int k = 0; k < steps; k++) {
       for (size_t i = 0; i < act.size(); i++) {
         MemberDev* m = act[i];
@@ -481,14 +521,49 @@ int exo_step(exo_engine* h, int steps) {
           m->loaded_phase = p;
         }
       }
-      exo::Stepper(e, desc).load_step(e.d_params + k * per_step);
-      exo::enqueue_step(e, desc);
       // gcm_publish_step's pack runs after every main-loop step
       // (stepper/index.ts:256-260); the start-up steps skip it.
       std::vector<int> main_desc;
       for (size_t i = 0; i < act.size(); i++)
         if (phase_at[k][i] >= act[i]->host.startup_steps) main_desc.push_back(desc[i]);
-      if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
+      bool replay = false;
+      if (graphs) {
+        const std::vector<int64_t> key = key_at(k);
+        replay = e.graph && key == e.graph_key;
+        // capture when the key is new and the next step shares it
+        if (!replay && k + 1 < steps && key_at(k + 1) == key) {
+          if (e.graph) HIP_OR_THROW(hipGraphExecDestroy(e.graph));
+          e.graph = nullptr;
+          e.graph_key.clear();
+          HIP_OR_THROW(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+          hipGraph_t g = nullptr;
+          try {
+            exo::Stepper(e, desc).load_step_at(e.d_params, e.d_step_index, (int)per_step);
+            exo::enqueue_step(e, desc);
+            if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
+            hipLaunchKernelGGL(k_step_index, dim3(1), dim3(64), 0, e.stream, e.d_step_index, -1);
+          } catch (...) {
+            hipStreamEndCapture(e.stream, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+          }
+          HIP_OR_THROW(hipStreamEndCapture(e.stream, &g));
+          const hipError_t err = hipGraphInstantiateWithFlags(&e.graph, g, 0);
+          hipGraphDestroy(g);
+          HIP_OR_THROW(err);
+          e.graph_key = key;
+          replay = true;
+        }
+      }
+      if (replay) {
+        if (index_at != k) hipLaunchKernelGGL(k_step_index, dim3(1), dim3(64), 0, e.stream, e.d_step_index, k);
+        HIP_OR_THROW(hipGraphLaunch(e.graph, e.stream));
+        index_at = k + 1;
+      } else {
+        exo::Stepper(e, desc).load_step(e.d_params + k * per_step);
+        exo::enqueue_step(e, desc);
+        if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
+      }
       if (!avg_at[k].empty()) exo::enqueue_averages(e, avg_at[k], avg_step[k]);
     }
     HIP_OR_THROW(hipEventRecord(e.t1, e.stream));
@@ -839,6 +914,12 @@ int exo_step_values(exo_engine* h, int id, int64_t step, double* out16) {
   return EXO_OK;
 }
 
+
+int exo_set_step_graphs(exo_engine* h, int enable) {
+  if (!h) return fail(EXO_E_ARG, "null engine");
+  h->e.graphs = enable != 0;
+  return EXO_OK;
+}
 
 int exo_set_publish_every(exo_engine* h, int every) {
   if (!h || every < 0) return fail(EXO_E_ARG, "publish interval must be >= 0");

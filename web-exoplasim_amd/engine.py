@@ -149,6 +149,7 @@ def library():
                           ctypes.POINTER(ExoResult), D]),
         "exo_probe_result": (I, [P, D, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "exo_set_publish_every": (I, [P, I]),
+        "exo_set_step_graphs": (I, [P, I]),
         "exo_read_snapshot": (I, [P, I, F, S]),
         "exo_global_means": (I, [P, I, D, D]),
         "exo_checkpoint_size": (I, [P, I, ctypes.POINTER(S)]),
@@ -470,6 +471,11 @@ class Engine:
         _check(self.lib.exo_step_values(self.h, member, step,
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
+
+    def set_step_graphs(self, enable):
+        """Replay steps from a captured HIP graph (True, the default) or
+        launch their kernels one by one (False)."""
+        _check(self.lib.exo_set_step_graphs(self.h, 1 if enable else 0))
 
     def set_publish_every(self, every):
         _check(self.lib.exo_set_publish_every(self.h, every))
