@@ -43,7 +43,7 @@ def probe_steps(steps):
     return min(steps, 96)
 
 
-def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True):
+def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True, tune=False):
     """Steps every config together; returns (wall s, device ms, probe).
     The timed steps replay the captured step graph (graphs=True) or launch
     every kernel (False); the probe then times single launches of its
@@ -54,6 +54,15 @@ def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True):
         ids = [eng.add_member(c, precision=precision) for c in cfgs]
         eng.step(warmup)
         eng.sync()
+        tuned = None
+        if tune:
+            # transform group sizes for this batch, picked by timing; the
+            # state is left as it was
+            t_tune = time.perf_counter()
+            ms = eng.autotune(trial_steps=8)
+            classes = sorted({(c.truncation, c.levels) for c in cfgs})
+            tuned = {"groups": {f"T{t} L{l}": eng.transform_group(t, l) for t, l in classes},
+                     "best_trial_ms_per_step": ms, "seconds": time.perf_counter() - t_tune}
         t0 = time.perf_counter()
         eng.step(steps)
         eng.sync()
@@ -70,6 +79,7 @@ def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True):
             ts, _ = eng.global_means(i)
             if not (100.0 < ts < 500.0):
                 raise RuntimeError(f"member {i} left the physical range: mean Ts {ts}")
+        run_gpu.tuned = tuned
         return wall, dev_ms, probed
     finally:
         eng.close()
@@ -160,6 +170,8 @@ def main():
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default transform group size (4 members) instead of tuning it")
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (the known-size kernel that "
@@ -201,7 +213,8 @@ def main():
     probe = roofline.DOMINANT[args.workload] if args.probe == "auto" else (args.probe or None)
     if probe and precision == exo.EXO_PRECISION_SPLIT and probe in ("k_leg_inv", "k_leg_fwd"):
         probe += "_split"
-    wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probe, precision)
+    wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probe, precision,
+                                   tune=not args.no_autotune)
     days = sum(args.steps / workloads.steps_per_day(c.timestep_minutes) for c in cfgs)
     value = days / wall
     line = {
@@ -219,10 +232,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic seeded planets (SH terrain, isothermal 250 K start)",
         "config": workload,
+        "autotune": run_gpu.tuned,
     }
     if args.workload == "batched" and not args.no_plain:
         other = exo.EXO_PRECISION_FP32 if precision == exo.EXO_PRECISION_SPLIT else exo.EXO_PRECISION_SPLIT
-        ow, _, _ = run_gpu(exo, cfgs, args.steps, args.warmup, None, other)
+        ow, _, _ = run_gpu(exo, cfgs, args.steps, args.warmup, None, other, tune=not args.no_autotune)
         name = "batched_plain_fp32" if other == exo.EXO_PRECISION_FP32 else "batched_split_fp16"
         line[name] = {"value": days / ow, "unit": "simulated days/s", "ms_per_step": ow * 1000.0 / args.steps,
                       "form": ("plain FP32" if other == exo.EXO_PRECISION_FP32

@@ -130,6 +130,25 @@ int exo_step(exo_engine *e, int steps);
  * GcmRunRequest.publishEvery (runner/types.ts:47-67). */
 int exo_set_publish_every(exo_engine *e, int every);
 
+/* Tile shape of the grouped transforms: how many members of one
+ * (truncation, levels) class share a workgroup of the Legendre and zonal
+ * transforms, 1..4 (default 4). Members compute bit-identically for every
+ * size. The reference compiles one pipeline per planet and has no such
+ * choice (kernel/index.ts:17-122); these calls and exo_autotune serve the
+ * batched engine. */
+int exo_set_transform_group(exo_engine *e, int truncation, int levels,
+                            int members);
+int exo_transform_group(exo_engine *e, int truncation, int levels,
+                        int *members);
+
+/* Picks each class's transform group size for the active batch by timing
+ * `trial_steps` steps per candidate (coordinate descent over the classes
+ * that hold two or more members), then restores every member's state,
+ * step counters and phase as they were: the next exo_step continues
+ * exactly as without the call. Nothing is published during the trials.
+ * tuned_ms (may be null) receives the best trial's device ms per step. */
+int exo_autotune(exo_engine *e, int trial_steps, double *tuned_ms);
+
 /* Replay each step of exo_step from a captured HIP graph (enable != 0,
  * the default) or launch its kernels one by one (0). The graph is captured
  * when two consecutive steps of a call have the same active members and

@@ -169,6 +169,15 @@ struct Engine {
   // advances.
   bool graphs = true;
   uint64_t epoch = 0;
+  // Members per group of the grouped transforms, per (truncation, levels);
+  // kLegGroup where unset. exo_autotune picks them by timing; any value in
+  // 1..kLegGroup gives bit-identical members, since every member's rows
+  // are reduced on their own.
+  std::map<std::pair<int, int>, int> group_cap;
+  int group_cap_of(const DevMember& d) const {
+    const auto it = group_cap.find({d.ntru, d.nlev});
+    return it == group_cap.end() ? kLegGroup : it->second;
+  }
   std::vector<int64_t> graph_key;
   hipGraphExec_t graph = nullptr;
   int* d_step_index = nullptr;
@@ -228,7 +237,8 @@ class Stepper {
       const DevMember& d = e_.h_desc[active_[a]];
       if (taken[a] || !include(d)) continue;
       std::vector<int> g;
-      for (size_t b = a; b < active_.size() && (int)g.size() < kLegGroup; b++) {
+      const int cap = e_.group_cap_of(d);
+      for (size_t b = a; b < active_.size() && (int)g.size() < cap; b++) {
         const DevMember& o = e_.h_desc[active_[b]];
         if (taken[b] || !include(o) || o.ltab != d.ltab || o.nlev != d.nlev) continue;
         taken[b] = 1;

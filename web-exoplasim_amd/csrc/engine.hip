@@ -915,6 +915,107 @@ int exo_step_values(exo_engine* h, int id, int64_t step, double* out16) {
 }
 
 
+int exo_set_transform_group(exo_engine* h, int truncation, int levels, int members) {
+  if (!h) return fail(EXO_E_ARG, "null engine");
+  if (members < 1 || members > kLegGroup)
+    return fail(EXO_E_ARG, "transform group must hold 1 to " + std::to_string(kLegGroup) + " members");
+  h->e.group_cap[{truncation, levels}] = members;
+  h->e.epoch++;
+  return EXO_OK;
+}
+
+int exo_transform_group(exo_engine* h, int truncation, int levels, int* members) {
+  if (!h || !members) return fail(EXO_E_ARG, "null argument");
+  DevMember d{};
+  d.ntru = truncation;
+  d.nlev = levels;
+  *members = h->e.group_cap_of(d);
+  return EXO_OK;
+}
+
+int exo_autotune(exo_engine* h, int trial_steps, double* tuned_ms) {
+  if (!h || trial_steps < 1) return fail(EXO_E_ARG, "invalid trial step count");
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "host-only engine cannot step");
+  struct Saved {
+    MemberDev* m;
+    int phase;
+    int64_t step, carbon;
+    float* arena;
+  };
+  std::vector<Saved> saved;
+  std::map<std::pair<int, int>, int> classes;  // (truncation, levels) -> active members
+  for (auto& m : e.members)
+    if (m && m->active) {
+      saved.push_back({m.get(), m->phase, m->step, m->carbon_count, nullptr});
+      const DevMember& d = e.h_desc[m->desc];
+      classes[{d.ntru, d.nlev}]++;
+    }
+  const int publish = e.publish_every;
+  int rc = EXO_OK;
+  double best_all = 0.0;
+  try {
+    HIP_OR_THROW(hipSetDevice(e.device));
+    HIP_OR_THROW(hipStreamSynchronize(e.stream));
+    for (Saved& s : saved) {
+      const size_t bytes = s.m->host.layout.total * sizeof(float);
+      HIP_OR_THROW(hipMalloc(&s.arena, bytes));
+      HIP_OR_THROW(hipMemcpyAsync(s.arena, s.m->d_arena, bytes, hipMemcpyDeviceToDevice, e.stream));
+    }
+    // Trials publish nothing, so no snapshot or average is touched.
+    e.publish_every = 0;
+    const auto trial = [&]() {
+      // the first call captures the step graph; the second is timed
+      if (exo_step(h, 2) != EXO_OK || exo_step(h, trial_steps) != EXO_OK || exo_sync(h) != EXO_OK)
+        throw std::runtime_error(exo_last_error());
+      double ms = 0;
+      if (exo_last_step_ms(h, &ms) != EXO_OK) throw std::runtime_error(exo_last_error());
+      return ms;
+    };
+    best_all = trial();
+    // Coordinate descent: each class with more than one member in turn,
+    // the others at their current sizes.
+    for (const auto& c : classes) {
+      if (c.second < 2) continue;
+      DevMember d{};
+      d.ntru = c.first.first;
+      d.nlev = c.first.second;
+      int best = e.group_cap_of(d);
+      for (int cap = 1; cap <= kLegGroup; cap++) {
+        if (cap == best) continue;
+        e.group_cap[c.first] = cap;
+        e.epoch++;
+        const double ms = trial();
+        if (ms < best_all) {
+          best_all = ms;
+          best = cap;
+        }
+      }
+      e.group_cap[c.first] = best;
+      e.epoch++;
+    }
+  } catch (const std::exception& ex) {
+    rc = fail(EXO_E_DEVICE, std::string("autotune: ") + ex.what());
+  }
+  // Restore every member's state and counters as they were.
+  e.publish_every = publish;
+  hipStreamSynchronize(e.stream);
+  for (Saved& s : saved) {
+    if (s.arena) {
+      const hipError_t err = hipMemcpy(s.m->d_arena, s.arena, s.m->host.layout.total * sizeof(float),
+                                       hipMemcpyDeviceToDevice);
+      if (err != hipSuccess && rc == EXO_OK) rc = fail(EXO_E_DEVICE, hipGetErrorString(err));
+      hipFree(s.arena);
+    }
+    s.m->phase = s.phase;
+    s.m->step = s.step;
+    s.m->carbon_count = s.carbon;
+  }
+  e.timed = false;
+  if (rc == EXO_OK && tuned_ms) *tuned_ms = best_all / trial_steps;
+  return rc;
+}
+
 int exo_set_step_graphs(exo_engine* h, int enable) {
   if (!h) return fail(EXO_E_ARG, "null engine");
   h->e.graphs = enable != 0;

@@ -150,6 +150,9 @@ def library():
         "exo_probe_result": (I, [P, D, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "exo_set_publish_every": (I, [P, I]),
         "exo_set_step_graphs": (I, [P, I]),
+        "exo_set_transform_group": (I, [P, I, I, I]),
+        "exo_transform_group": (I, [P, I, I, ctypes.POINTER(ctypes.c_int)]),
+        "exo_autotune": (I, [P, I, ctypes.POINTER(ctypes.c_double)]),
         "exo_read_snapshot": (I, [P, I, F, S]),
         "exo_global_means": (I, [P, I, D, D]),
         "exo_checkpoint_size": (I, [P, I, ctypes.POINTER(S)]),
@@ -471,6 +474,23 @@ class Engine:
         _check(self.lib.exo_step_values(self.h, member, step,
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
+
+    def set_transform_group(self, truncation, levels, members):
+        """Members of one (truncation, levels) class per transform workgroup."""
+        _check(self.lib.exo_set_transform_group(self.h, truncation, levels, members))
+
+    def transform_group(self, truncation, levels):
+        n = ctypes.c_int()
+        _check(self.lib.exo_transform_group(self.h, truncation, levels, ctypes.byref(n)))
+        return n.value
+
+    def autotune(self, trial_steps=8):
+        """Times the transform group sizes on the active batch and keeps
+        the fastest per class; the state is left as it was. Returns the
+        best trial's device ms per step."""
+        ms = ctypes.c_double()
+        _check(self.lib.exo_autotune(self.h, trial_steps, ctypes.byref(ms)))
+        return ms.value
 
     def set_step_graphs(self, enable):
         """Replay steps from a captured HIP graph (True, the default) or
