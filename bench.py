@@ -62,6 +62,7 @@ def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True, tune=Fals
             ms = eng.autotune(trial_steps=8)
             classes = sorted({(c.truncation, c.levels) for c in cfgs})
             tuned = {"groups": {f"T{t} L{l}": eng.transform_group(t, l) for t, l in classes},
+                     "step_graphs": eng.step_graphs(),
                      "best_trial_ms_per_step": ms, "seconds": time.perf_counter() - t_tune}
         t0 = time.perf_counter()
         eng.step(steps)
@@ -243,11 +244,13 @@ def main():
                                else "split-operand Legendre (FP16 hi/lo, FP32 accumulate)")}
     if args.workload == "batched" and not args.no_plain:
         plain_cfg = workloads.plain(exo, 63, 10, 30)
-        pw, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None)
+        pw, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None, tune=not args.no_autotune)
+        plain_tuned = run_gpu.tuned
         plain_value = (480 / 48) / pw
         line["plain_single"] = {"value": plain_value, "unit": "simulated days/s",
                                 "ms_per_step": pw * 1000.0 / 480,
-                                "config": "plain FP32 single planet T63 L10, 30-min step"}
+                                "config": "plain FP32 single planet T63 L10, 30-min step",
+                                "autotune": plain_tuned}
         ew, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None, graphs=False)
         line["plain_single_eager"] = {"value": (480 / 48) / ew, "unit": "simulated days/s",
                                       "ms_per_step": 1e3 * ew / 480,

@@ -146,6 +146,8 @@ int exo_transform_group(exo_engine *e, int truncation, int levels,
  * that hold two or more members), then restores every member's state,
  * step counters and phase as they were: the next exo_step continues
  * exactly as without the call. Nothing is published during the trials.
+ * It then times the other launch mode (see exo_set_step_graphs) and keeps
+ * the faster; a later exo_set_step_graphs overrides that choice.
  * tuned_ms (may be null) receives the best trial's device ms per step. */
 int exo_autotune(exo_engine *e, int trial_steps, double *tuned_ms);
 
@@ -156,6 +158,7 @@ int exo_autotune(exo_engine *e, int trial_steps, double *tuned_ms);
  * either way. The reference submits one WebGPU command buffer per batch of
  * steps (stepper/index.ts:175-276); this is that batching's launch side. */
 int exo_set_step_graphs(exo_engine *e, int enable);
+int exo_step_graphs(exo_engine *e, int *enabled);
 
 /* Blocks until queued steps have finished. Stands in for mapAsync. */
 int exo_sync(exo_engine *e);

@@ -27,6 +27,7 @@ def _run(exo, cfgs, steps, tune_at=None, groups=None):
             ms = e.autotune(trial_steps=4)
             picked = {k: e.transform_group(*k) for k in ((21, 5), (42, 10))}
             assert all(1 <= n <= 4 for n in picked.values())
+            assert e.step_graphs() in (True, False)
             steps -= tune_at
         e.step(steps)
         e.sync()

@@ -994,6 +994,15 @@ int exo_autotune(exo_engine* h, int trial_steps, double* tuned_ms) {
       e.group_cap[c.first] = best;
       e.epoch++;
     }
+    // Launch mode: graph replays or single launches (bit-identical), the
+    // faster of the two at the tuned sizes.
+    const bool graphs = e.graphs;
+    e.graphs = !graphs;
+    const double ms = trial();
+    if (ms < best_all)
+      best_all = ms;
+    else
+      e.graphs = graphs;
   } catch (const std::exception& ex) {
     rc = fail(EXO_E_DEVICE, std::string("autotune: ") + ex.what());
   }
@@ -1014,6 +1023,12 @@ int exo_autotune(exo_engine* h, int trial_steps, double* tuned_ms) {
   e.timed = false;
   if (rc == EXO_OK && tuned_ms) *tuned_ms = best_all / trial_steps;
   return rc;
+}
+
+int exo_step_graphs(exo_engine* h, int* enabled) {
+  if (!h || !enabled) return fail(EXO_E_ARG, "null argument");
+  *enabled = h->e.graphs ? 1 : 0;
+  return EXO_OK;
 }
 
 int exo_set_step_graphs(exo_engine* h, int enable) {

@@ -150,6 +150,7 @@ def library():
         "exo_probe_result": (I, [P, D, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "exo_set_publish_every": (I, [P, I]),
         "exo_set_step_graphs": (I, [P, I]),
+        "exo_step_graphs": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "exo_set_transform_group": (I, [P, I, I, I]),
         "exo_transform_group": (I, [P, I, I, ctypes.POINTER(ctypes.c_int)]),
         "exo_autotune": (I, [P, I, ctypes.POINTER(ctypes.c_double)]),
@@ -491,6 +492,12 @@ class Engine:
         ms = ctypes.c_double()
         _check(self.lib.exo_autotune(self.h, trial_steps, ctypes.byref(ms)))
         return ms.value
+
+    def step_graphs(self):
+        """Whether steps replay the captured step graph."""
+        n = ctypes.c_int()
+        _check(self.lib.exo_step_graphs(self.h, ctypes.byref(n)))
+        return bool(n.value)
 
     def set_step_graphs(self, enable):
         """Replay steps from a captured HIP graph (True, the default) or
