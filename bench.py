@@ -130,8 +130,16 @@ def run_serving(exo, seconds, rate=4.0, seed=5):
     reqs = workloads.serving_requests(exo, seed=seed, rate=rate, seconds=seconds)
     spd = workloads.steps_per_day(45)
     runs = [(t, cfg, max(1, int(round(days * spd)))) for t, cfg, days in reqs]
+    mark = [0]
+
+    def progress(done, total, elapsed):
+        # one line per hundred runs ended
+        if done // 100 > mark[0] or done == total:
+            mark[0] = done // 100
+            print(f"serving: {done}/{total} runs ended at {elapsed:.1f} s", file=sys.stderr, flush=True)
+
     with exo.Engine(0, 64) as eng:
-        results, wall = eng.serve(runs, max_chunk_steps=32, prepare_threads=8)
+        results, wall = eng.serve(runs, max_chunk_steps=32, prepare_threads=8, progress=progress)
     ok = [r for r in results if r["status"] == 0]
     if len(ok) != len(results):
         raise RuntimeError(f"{len(results) - len(ok)} served runs failed")

@@ -276,10 +276,17 @@ typedef struct {
   int reserved;
 } exo_result;
 
+/* Progress report of the serving loop: called on the calling thread after
+   each batched advance that ended at least one run, with the runs ended so
+   far (finished or failed) out of `count`. */
+typedef void (*exo_serve_progress)(void *user, int finished, int count, double elapsed_s);
+
 typedef struct {
   size_t hbm_budget_bytes;
   int max_chunk_steps; /* 0: 32 */
   int prepare_threads; /* 0: 4 */
+  exo_serve_progress progress; /* may be NULL */
+  void *progress_user;
 } exo_serve_options;
 
 int exo_serve(exo_engine *e, const exo_request *requests, int count,

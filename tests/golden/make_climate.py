@@ -4,7 +4,7 @@ seeded runs, from the engine's own FP64 set-up of the same configuration.
 
     python tests/golden/make_climate.py CASE [--fp32]
 
-CASE is one of CASES below. Each run writes tests/golden/climate_<CASE>
+CASE is one of CASES or ENSEMBLE_MEMBERS below. Each run writes tests/golden/climate_<CASE>
 [_fp32].json: {"case", "config", "steps", "ts": [...], "toa": [...]}, one
 entry per step. The means are area-weighted with the Gaussian weights, as
 the engine's exo_global_means computes them (surface level of dt; level 0
@@ -32,6 +32,19 @@ CASES = {
     # precision form: T63 L10, 30 min, 30 days
     "t63_day30": (dict(truncation=63, levels=10, timestep_minutes=30), 1440),
 }
+# The batched ensemble (64 planets drawn from seed 64: 32 x T21 L5,
+# 24 x T42 L10, 8 x T63 L10, 45-minute steps): the first member of each
+# class, 3 start-up steps and 349 main-loop steps (just under 11 days).
+ENSEMBLE_MEMBERS = {"c4_member0": 0, "c4_member32": 32, "c4_member56": 56}
+ENSEMBLE_STEPS = 352
+
+
+def ensemble_case(exo, case):
+    """(config overrides, steps) of an ensemble member case."""
+    import importlib
+    workloads = importlib.import_module(exo.__name__ + ".workloads")
+    cfg = workloads.ensemble(exo, seed=64)[ENSEMBLE_MEMBERS[case]]
+    return {name: getattr(cfg, name) for name, _ in cfg._fields_}, ENSEMBLE_STEPS
 
 
 def global_means(run, weights, nlev, nhor):
@@ -47,11 +60,11 @@ def main():
     import __graft_entry__ as entry
     from oracle_harness import OracleRun
     ap = argparse.ArgumentParser()
-    ap.add_argument("case", choices=sorted(CASES))
+    ap.add_argument("case", choices=sorted(CASES) + sorted(ENSEMBLE_MEMBERS))
     ap.add_argument("--fp32", action="store_true")
     args = ap.parse_args()
     exo = entry._package()
-    overrides, steps = CASES[args.case]
+    overrides, steps = CASES[args.case] if args.case in CASES else ensemble_case(exo, args.case)
     cfg = exo.default_config(**overrides)
     host = exo.Engine(-1, 1)
     m = host.add_member(cfg)

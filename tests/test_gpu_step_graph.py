@@ -63,3 +63,29 @@ def test_graph_replays_match_single_launches(exo):
     finally:
         for e in engines:
             e.close()
+
+
+def test_recapture_while_replays_are_queued(exo):
+    """A member joins while earlier replays may still be queued (no sync in
+    between): the executable they use is replaced only after they finish,
+    and the state equals single launches bit for bit."""
+    cfgs = _planets(exo)
+    engines = [exo.Engine(0, 4), exo.Engine(0, 4)]
+    try:
+        engines[0].set_step_graphs(False)
+        engines[1].set_step_graphs(True)
+        ids = [[e.add_member(c) for c in cfgs[:2]] for e in engines]
+        for e in engines:
+            e.step(12)  # start-up, then replays of one captured step
+        joined = [e.add_member(cfgs[2]) for e in engines]
+        for e in engines:
+            e.step(6)  # the batch changed: captured again
+        for e in engines:
+            e.sync()
+        assert ids[0] == ids[1] and joined[0] == joined[1]
+        for m in ids[0] + [joined[0]]:
+            a, b = (e.read_arena(m) for e in engines)
+            assert np.array_equal(a, b), f"member {m} arena differs"
+    finally:
+        for e in engines:
+            e.close()

@@ -446,6 +446,17 @@ int exo_step(exo_engine* h, int steps) {
       desc.push_back(m->desc);
     }
   if (act.empty() || steps == 0) return EXO_OK;
+  std::vector<std::vector<int>> phase_at;
+  std::vector<std::vector<int64_t>> stepno_at, carbon_at;
+  int done = 0;  // steps wholly enqueued
+  const auto commit = [&](int k) {
+    if (phase_at.empty()) return;
+    for (size_t i = 0; i < act.size(); i++) {
+      act[i]->phase = phase_at[k][i];
+      act[i]->step = stepno_at[k][i];
+      act[i]->carbon_count = carbon_at[k][i];
+    }
+  };
   try {
     HIP_OR_THROW(hipSetDevice(e.device));
     const size_t per_step = act.size() * EXO_STEP_VALUES;
@@ -466,7 +477,15 @@ int exo_step(exo_engine* h, int steps) {
       stepno[i] = act[i]->step;
       carbon[i] = act[i]->carbon_count;
     }
-    std::vector<std::vector<int>> phase_at(steps, std::vector<int>(act.size()));
+    // Each member's phase and counters before step k (k = steps: after the
+    // call), so that a failure part way through leaves the counters at the
+    // steps that were enqueued.
+    phase_at.assign(steps + 1, std::vector<int>(act.size()));
+    stepno_at.assign(steps + 1, std::vector<int64_t>(act.size()));
+    carbon_at.assign(steps + 1, std::vector<int64_t>(act.size()));
+    phase_at[0] = phase;
+    stepno_at[0] = stepno;
+    carbon_at[0] = carbon;
     // per step: the averaging members that publish at it, with their step
     std::vector<std::vector<MemberDev*>> avg_at(steps);
     std::vector<std::vector<int64_t>> avg_step(steps);
@@ -482,13 +501,20 @@ int exo_step(exo_engine* h, int steps) {
         }
         exo::step_values(m, stepno[i], carbon[i], publish, v);
         for (int c = 0; c < EXO_STEP_VALUES; c++) e.h_params[k * per_step + i * EXO_STEP_VALUES + c] = (float)v[c];
-        carbon[i] = v[12] != 0 ? 0 : carbon[i] + 1;
         phase_at[k][i] = phase[i];
+        stepno_at[k][i] = stepno[i];
+        carbon_at[k][i] = carbon[i];
+        carbon[i] = v[12] != 0 ? 0 : carbon[i] + 1;
         if (startup)
           phase[i]++;
         else
           stepno[i]++;
       }
+    for (size_t i = 0; i < act.size(); i++) {
+      phase_at[steps][i] = phase[i];
+      stepno_at[steps][i] = stepno[i];
+      carbon_at[steps][i] = carbon[i];
+    }
     HIP_OR_THROW(hipMemcpyAsync(e.d_params, e.h_params, need * sizeof(float), hipMemcpyHostToDevice, e.stream));
     HIP_OR_THROW(hipEventRecord(e.params_done, e.stream));
     HIP_OR_THROW(hipEventRecord(e.t0, e.stream));
@@ -532,7 +558,11 @@ int exo_step(exo_engine* h, int steps) {
         replay = e.graph && key == e.graph_key;
         // capture when the key is new and the next step shares it
         if (!replay && k + 1 < steps && key_at(k + 1) == key) {
-          if (e.graph) HIP_OR_THROW(hipGraphExecDestroy(e.graph));
+          if (e.graph) {
+            // earlier replays of this executable may still be queued
+            HIP_OR_THROW(hipStreamSynchronize(e.stream));
+            HIP_OR_THROW(hipGraphExecDestroy(e.graph));
+          }
           e.graph = nullptr;
           e.graph_key.clear();
           HIP_OR_THROW(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
@@ -565,16 +595,14 @@ int exo_step(exo_engine* h, int steps) {
         if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
       }
       if (!avg_at[k].empty()) exo::enqueue_averages(e, avg_at[k], avg_step[k]);
+      done = k + 1;
     }
     HIP_OR_THROW(hipEventRecord(e.t1, e.stream));
     HIP_OR_THROW(hipGetLastError());
     e.timed = true;
-    for (size_t i = 0; i < act.size(); i++) {
-      act[i]->phase = phase[i];
-      act[i]->step = stepno[i];
-      act[i]->carbon_count = carbon[i];
-    }
+    commit(steps);
   } catch (const std::exception& ex) {
+    commit(done);
     return fail(EXO_E_DEVICE, ex.what());
   }
   return EXO_OK;
@@ -964,10 +992,17 @@ int exo_autotune(exo_engine* h, int trial_steps, double* tuned_ms) {
     }
     // Trials publish nothing, so no snapshot or average is touched.
     e.publish_every = 0;
+    // The first untimed call also runs every member out of its start-up
+    // steps, so that every timed call steps the main loop alone.
+    int lead = 0;
+    for (const Saved& s : saved) lead = std::max(lead, s.m->host.startup_steps - s.phase);
     const auto trial = [&]() {
-      // the first call captures the step graph; the second is timed
-      if (exo_step(h, 2) != EXO_OK || exo_step(h, trial_steps) != EXO_OK || exo_sync(h) != EXO_OK)
+      // an untimed call of the timed length sizes the calendar buffer and
+      // captures the step graph; the second call, identical, is timed
+      if (exo_step(h, trial_steps + lead) != EXO_OK || exo_step(h, trial_steps) != EXO_OK ||
+          exo_sync(h) != EXO_OK)
         throw std::runtime_error(exo_last_error());
+      lead = 0;
       double ms = 0;
       if (exo_last_step_ms(h, &ms) != EXO_OK) throw std::runtime_error(exo_last_error());
       return ms;

@@ -178,6 +178,7 @@ extern "C" int exo_serve(exo_engine* h, const exo_request* requests, int count, 
       status = rc;
       break;
     }
+    const int finished_before = finished;
     for (auto it = resident.begin(); it != resident.end();) {
       it->remaining -= chunk;
       if (it->remaining > 0) {
@@ -193,6 +194,8 @@ extern "C" int exo_serve(exo_engine* h, const exo_request* requests, int count, 
       finished++;
       it = resident.erase(it);
     }
+    if (opt.progress && finished > finished_before)
+      opt.progress(opt.progress_user, finished, count, seconds_since(t0));
   }
   {
     std::lock_guard<std::mutex> lock(mu);

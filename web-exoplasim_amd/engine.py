@@ -74,9 +74,13 @@ class ExoResult(ctypes.Structure):
                 ("steps", ctypes.c_int64), ("status", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
+SERVE_PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double)
+
+
 class ExoServeOptions(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_chunk_steps", ctypes.c_int),
-                ("prepare_threads", ctypes.c_int)]
+                ("prepare_threads", ctypes.c_int), ("progress", SERVE_PROGRESS),
+                ("progress_user", ctypes.c_void_p)]
 
 
 def _month_info(info):
@@ -390,9 +394,11 @@ class Engine:
         _check(self.lib.exo_last_step_ms(self.h, ctypes.byref(v)))
         return v.value
 
-    def serve(self, requests, hbm_budget_bytes=0, max_chunk_steps=32, prepare_threads=4):
+    def serve(self, requests, hbm_budget_bytes=0, max_chunk_steps=32, prepare_threads=4, progress=None):
         """Runs the serving loop over [(arrival_s, config, steps)] (FP32
-        form); returns (results as dicts in request order, wall seconds)."""
+        form); returns (results as dicts in request order, wall seconds).
+        progress(finished, count, elapsed_s), when given, is called after
+        each batched advance that ended runs."""
         n = len(requests)
         req = (ExoRequest * max(n, 1))()
         for i, (arrival, cfg, steps) in enumerate(requests):
@@ -401,7 +407,8 @@ class Engine:
             req[i].arrival_s = float(arrival)
             req[i].steps = int(steps)
         res = (ExoResult * max(n, 1))()
-        opt = ExoServeOptions(hbm_budget_bytes, max_chunk_steps, prepare_threads)
+        cb = SERVE_PROGRESS(lambda _u, done, total, t: progress(done, total, t)) if progress else SERVE_PROGRESS()
+        opt = ExoServeOptions(hbm_budget_bytes, max_chunk_steps, prepare_threads, cb, None)
         wall = ctypes.c_double()
         _check(self.lib.exo_serve(self.h, req, n, ctypes.byref(opt), res, ctypes.byref(wall)))
         out = [{name: getattr(res[i], name) for name, _ in ExoResult._fields_ if name != "reserved"}
