@@ -40,6 +40,10 @@ void oracle_step(const oracle_member* m, int with_physics);
 void oracle_physics(const oracle_member* m);
 /* One parameterization by name (see exo_oracle_physics.c); -1 if unknown. */
 int oracle_routine(const oracle_member* m, const char* name);
+/* The radiation's absorber fits at n paths, and the storm's potential
+ * intensity parts of one column (see exo_oracle_physics.c). */
+int oracle_radiation_fit(const oracle_member* m, int which, const real* u, int n, real* out);
+int oracle_storm_potential(const oracle_member* m, int h, real* out);
 /* Set-up transforms over `planes` planes (qi or qj when dmu). */
 void oracle_transform_inverse(const oracle_member* m, const real* sp, real* grid, int planes, int dmu);
 void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp, int planes);

@@ -19,13 +19,22 @@
  * dry-adiabatic CAPE in closed form, tropical CAPE and potential
  * intensity, the trigonometric root of the ventilation cubic).
  *
- * Parity unpinned: the glacier and wet-soil albedo curves, the ice
- * surface temperature's implicit solution, the weathering supply limit,
- * the cloud liquid-water profile and the outflow-temperature ratio in the
- * storm potential intensity. No closed form, library routine or
- * reference golden vector checks those details; they are held only to the
- * GPU kernels written from the same reading of the reference WGSL, and by
- * the physical range of day-scale runs.
+ * The details below are pinned on their own (tests/test_oracle_fit_pins.py):
+ * lwr's absorptivity fits (zero at zero path, continuous across the branch
+ * points the reference's offsets are derived from, monotone, bounded),
+ * swr's ozone and vapour absorptions (zero, rising, saturating), the ice
+ * surface temperature (the implicit step closes the top layer's energy
+ * balance), the glacier and wet-soil albedo curves (endpoints, ramp, the
+ * inverted retention curve), the cloud liquid water (Kiehl et al.'s CCM3
+ * profile at hypsometric heights), the weathering supply limit
+ * (asymptotes, inverse) and the storm's outflow ratio (T_s over the
+ * environment's temperature at the level of neutral buoyancy).
+ *
+ * Parity unpinned: two coefficients no property fixes, the slope of the
+ * CO2 absorptivity's logarithmic branch (0.0546 per decade of path above
+ * 1; its value at the branch point is pinned) and the exponent of the
+ * ozone absorption's ultraviolet term (0.805). They are held only to the
+ * GPU kernels written from the same reading of the reference WGSL.
  */
 #define EXO_ORACLE_INTERNAL
 #include <tgmath.h>
@@ -527,6 +536,36 @@ static void swr(const M* v) {
   }
 }
 
+/* lwr's band absorptivities of an absorber path u (lwr.wgsl:36-179): each
+ * is a power law below its branch point and a logarithm above it, with the
+ * offsets absorptivityOffsets() computes (physics/radiation/index.ts:
+ * 108-125). */
+static real lwr_ah2o(const M* v, real u) {
+  real a;
+  if (u <= 0.01)
+    a = 0.846 * pow(u + 3.59e-5, 0.243) - K(zh2o0a);
+  else
+    a = 0.24 * log10(u + 0.01) + K(zah2oc);
+  if (K(th2oc) > 0) a = MIN(a + (1.0 - exp(-K(th2oc) * u)), 1.0);
+  return a;
+}
+
+static real lwr_aco2(const M* v, real u) {
+  if (u <= 1.0) return 0.0676 * pow(u + 0.01022, 0.421) - K(zco20);
+  return 0.0546 * log10(u) + K(zaco2c);
+}
+
+/* transmission of the water vapour overlapping the CO2 band */
+static real lwr_th2o(const M* v, real u) {
+  if (u <= 2.0) return 1.0 - (0.832 * pow(u + 0.0286, 0.26) - K(zh2o0));
+  return MAX(0.0, K(zth2oc) - 0.1196 * log(u - 0.6931));
+}
+
+static real lwr_ao3(const M* v, real u) {
+  if (u <= 0.01) return 0.209 * pow(u + 7.0e-5, 0.436) - K(zao30);
+  return 0.0212 * log10(u) + K(zao3c);
+}
+
 /* physics/radiation/wgsl/lwr.wgsl:36-179 */
 static void lwr(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
@@ -567,27 +606,8 @@ static void lwr(const M* v) {
         sumwv = sumwv + q[k];
         sumo3 = sumo3 + qo3[k];
         sumco2 = sumco2 + qco2[k];
-        real ah2o;
-        if (sumwv <= 0.01)
-          ah2o = 0.846 * pow(sumwv + 3.59e-5, 0.243) - K(zh2o0a);
-        else
-          ah2o = 0.24 * log10(sumwv + 0.01) + K(zah2oc);
-        if (K(th2oc) > 0) ah2o = MIN(ah2o + (1.0 - exp(-K(th2oc) * sumwv)), 1.0);
-        real aco2;
-        if (sumco2 <= 1.0)
-          aco2 = 0.0676 * pow(sumco2 + 0.01022, 0.421) - K(zco20);
-        else
-          aco2 = 0.0546 * log10(sumco2) + K(zaco2c);
-        real th2o;
-        if (sumwv <= 2.0)
-          th2o = 1.0 - (0.832 * pow(sumwv + 0.0286, 0.26) - K(zh2o0));
-        else
-          th2o = MAX(0.0, K(zth2oc) - 0.1196 * log(sumwv - 0.6931));
-        real ao3;
-        if (sumo3 <= 0.01)
-          ao3 = 0.209 * pow(sumo3 + 7.0e-5, 0.436) - K(zao30);
-        else
-          ao3 = 0.0212 * log10(sumo3) + K(zao3c);
+        const real ah2o = lwr_ah2o(v, sumwv), aco2 = lwr_aco2(v, sumco2);
+        const real th2o = lwr_th2o(v, sumwv), ao3 = lwr_ao3(v, sumo3);
         const real cs = MIN(1.0 - ZERO, MAX(ZERO, 1.0 - ah2o - ao3 - aco2 * th2o));
         taucs[k] = cs;
         taucc = taucc * taucc0[k];
@@ -1755,6 +1775,47 @@ static st_cape st_cape_of(int L, const real* t, const real* r, const real* p, re
   return out;
 }
 
+/* The maximum potential intensity's iteration (storm.wgsl:140-208): the
+ * central pressure pm is iterated until it moves by at most 0.5 hPa; -1
+ * when it fails. parts (may be NULL) receives the last iteration's outflow
+ * ratio sst / T_out, T_out (the environment's temperature at the saturated
+ * parcel's level of neutral buoyancy), that level's pressure, and the CAPE
+ * of the saturated and of the ambient parcel at pm. */
+static real st_potential(int L, const real* t, const real* r, const real* p, real sst, real msl, real RD,
+                         real CKCD, real environmental, real* parts) {
+  const real eps = RD / ST_RVP;
+  const int bottom = L - 1;
+  real pm = 970, old = pm, next = 0, capem = 0, capems = 0, ratio = 1, tout = -1, pout = -1;
+  unsigned count = 0;
+  for (;;) {
+    if (fabs(next - old) <= (real)0.5) break;
+    const real pp = MIN(pm, 1000);
+    const real rp = eps * r[bottom] * msl / (pp * (eps + r[bottom]) - r[bottom] * msl);
+    capem = st_cape_of(L, t, r, p, t[bottom], rp, pp, RD).energy;
+    const real rs = st_mixing(st_es(sst - (real)273.15), pp, eps);
+    const st_cape saturated = st_cape_of(L, t, r, p, sst, rs, pp, RD);
+    capems = saturated.energy;
+    tout = saturated.temperature;
+    pout = saturated.pressure;
+    ratio = sst / saturated.temperature;
+    const real tvav = (real)0.5 * (st_density(t[bottom], r[bottom], r[bottom], eps) + st_density(sst, rs, rs, eps));
+    const real cat = MAX((capem - environmental) + (real)0.5 * CKCD * ratio * (capems - capem), 0);
+    next = msl * exp(-cat / (RD * tvav));
+    old = pm;
+    pm = next;
+    count++;
+    if (count > 200 || pm < 400 || !(pm > 0)) return -1;
+  }
+  if (parts) {
+    parts[0] = ratio;
+    parts[1] = tout;
+    parts[2] = pout;
+    parts[3] = capems;
+    parts[4] = capem;
+  }
+  return sqrt(MAX(0, CKCD * ratio * MAX(0, capems - capem)));
+}
+
 /* storm.wgsl:133-138 */
 static real st_reduced(real vi, real vithresh) {
   if (vi == 0 || fabs(vi) >= vithresh) return 0;
@@ -1798,30 +1859,7 @@ static void storm(const M* v) {
     real vmax = -1;
     if (sst > (real)278.15 && !cold && msl > 0) {
       environmental = st_cape_of(L, t, r, p, t[bottom], r[bottom], p[bottom], RD);
-      real pm = 970, old = pm, next = 0, capem = 0, capems = 0, ratio = 1;
-      int failed = 0;
-      unsigned count = 0;
-      for (;;) {
-        if (fabs(next - old) <= (real)0.5) break;
-        const real pp = MIN(pm, 1000);
-        const real rp = eps * r[bottom] * msl / (pp * (eps + r[bottom]) - r[bottom] * msl);
-        capem = st_cape_of(L, t, r, p, t[bottom], rp, pp, RD).energy;
-        const real rs = st_mixing(st_es(sst - (real)273.15), pp, eps);
-        const st_cape saturated = st_cape_of(L, t, r, p, sst, rs, pp, RD);
-        capems = saturated.energy;
-        ratio = sst / saturated.temperature;
-        const real tvav = (real)0.5 * (st_density(t[bottom], r[bottom], r[bottom], eps) + st_density(sst, rs, rs, eps));
-        const real cat = MAX((capem - environmental.energy) + (real)0.5 * CKCD * ratio * (capems - capem), 0);
-        next = msl * exp(-cat / (RD * tvav));
-        old = pm;
-        pm = next;
-        count++;
-        if (count > 200 || pm < 400 || !(pm > 0)) {
-          failed = 1;
-          break;
-        }
-      }
-      if (!failed) vmax = sqrt(MAX(0, CKCD * ratio * MAX(0, capems - capem)));
+      vmax = st_potential(L, t, r, p, sst, msl, RD, CKCD, environmental.energy, NULL);
     }
     real vi = 1;
     if (vmax > 0) vi = shear * deficit / vmax;
@@ -1874,6 +1912,52 @@ static void carbonstep(const M* v) {
 /* One parameterization by name, for the invariant checks of the oracle's
  * routines; -1 for an unknown name. "fixer" runs the column, row, global
  * and apply passes; "rain" runs mklsp, mkrain and mkclouds per column. */
+/* The fits of lwr (which: 0 H2O absorptivity, 1 CO2 absorptivity, 2 H2O
+ * transmission in the CO2 band, 3 O3 absorptivity) or of swr (4 ozone
+ * absorption, 5 water vapour absorption) at n absorber paths; -1 for an
+ * unknown fit. For the pins of the fits alone. */
+int oracle_radiation_fit(const oracle_member* m, int which, const real* u, int n, real* out) {
+  M vv = view(m);
+  const M* v = &vv;
+  for (int i = 0; i < n; i++) {
+    switch (which) {
+      case 0: out[i] = lwr_ah2o(v, u[i]); break;
+      case 1: out[i] = lwr_aco2(v, u[i]); break;
+      case 2: out[i] = lwr_th2o(v, u[i]); break;
+      case 3: out[i] = lwr_ao3(v, u[i]); break;
+      case 4: out[i] = ozone_absorption(v, u[i]); break;
+      case 5: out[i] = vapour_absorption(v, u[i]); break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+
+/* storm's potential-intensity iteration for column h of the arena (the
+ * column storm() builds); out[0..4] as st_potential's parts, out[5] the
+ * intensity. -1 when the column is skipped (cool water, cold levels) or
+ * the iteration fails. */
+int oracle_storm_potential(const oracle_member* m, int h, real* out) {
+  M vv = view(m);
+  const M* v = &vv;
+  const int L = v->NLEV, NH = v->NHOR;
+  const real RD = K(gascon);
+  real t[16], r[16], p[16];
+  const real msl = F(dp)[h] * (real)0.01, sst = LV(dt, L, h);
+  int cold = 0;
+  for (int l = 0; l < L; l++) {
+    t[l] = LV(dt, l, h);
+    r[l] = LV(dq, l, h);
+    p[l] = msl * T(sigma)[l];
+    cold = cold || t[l] <= 100;
+  }
+  (void)NH;
+  if (!(sst > (real)278.15 && !cold && msl > 0)) return -1;
+  const st_cape env = st_cape_of(L, t, r, p, t[L - 1], r[L - 1], p[L - 1], RD);
+  out[5] = st_potential(L, t, r, p, sst, msl, RD, K(ckcd), env.energy, out);
+  return out[5] < 0 ? -1 : 0;
+}
+
 int oracle_routine(const oracle_member* m, const char* name) {
   M vv = view(m);
   const M* v = &vv;

@@ -50,6 +50,10 @@ def oracle_library(precision="fp64"):
     P = ctypes.POINTER(ct)
     lib.oracle_transform_inverse.argtypes = [ctypes.POINTER(Mem), P, P, ctypes.c_int, ctypes.c_int]
     lib.oracle_transform_forward.argtypes = [ctypes.POINTER(Mem), P, P, ctypes.c_int]
+    lib.oracle_radiation_fit.argtypes = [ctypes.POINTER(Mem), ctypes.c_int, P, ctypes.c_int, P]
+    lib.oracle_radiation_fit.restype = ctypes.c_int
+    lib.oracle_storm_potential.argtypes = [ctypes.POINTER(Mem), ctypes.c_int, P]
+    lib.oracle_storm_potential.restype = ctypes.c_int
     _LIBS[precision] = lib
     return lib
 
@@ -159,6 +163,30 @@ class OracleRun:
         m = self._struct(consts)
         if self.lib.oracle_routine(ctypes.byref(m), name.encode()) != 0:
             raise KeyError(name)
+
+    FITS = {"lwr_h2o": 0, "lwr_co2": 1, "lwr_h2o_transmission": 2, "lwr_o3": 3, "swr_o3": 4, "swr_h2o": 5}
+
+    def radiation_fit(self, name, paths):
+        """One of the radiation's absorber fits at the given paths."""
+        m = self._struct(self.consts[-1])
+        u = np.ascontiguousarray(paths, dtype=self.dtype)
+        out = np.zeros_like(u)
+        if self.lib.oracle_radiation_fit(ctypes.byref(m), self.FITS[name], _ptr(u), len(u), _ptr(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def storm_potential(self, column):
+        """(outflow ratio, outflow temperature, outflow pressure, saturated
+        CAPE, ambient CAPE, intensity) of the storm's potential-intensity
+        iteration on one column, or None when it is skipped or fails."""
+        m = self._struct(self.consts[-1])
+        out = np.zeros(6, dtype=self.dtype)
+        if self.lib.oracle_storm_potential(ctypes.byref(m), int(column), _ptr(out)) != 0:
+            return None
+        return out
+
+    def set_constant(self, name, value, phase=None):
+        self.consts[-1 if phase is None else phase][self.const_names.index(name)] = value
 
     def constant(self, name, phase=None):
         return float(self.consts[-1 if phase is None else phase][self.const_names.index(name)])

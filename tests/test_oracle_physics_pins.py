@@ -51,9 +51,10 @@ oracle fails at least one test each.
   d / (d + 1 cm), saturated surface air.
 * carbon: the Walker-Hays-Kasting weathering law against the present Earth.
 
-Not pinned here: the glacier and wet-soil albedo curves, the ice surface
-temperature's implicit solution, the weathering supply limit (off by
-default) and the cloud liquid-water profile.
+The glacier and wet-soil albedo curves, the ice surface temperature's
+implicit solution, the weathering supply limit, the cloud liquid-water
+profile, the radiation's absorber fits and the storm's outflow ratio are
+pinned in tests/test_oracle_fit_pins.py.
 Each test states the tolerance it needs: FP64 arithmetic, so conservation
 holds to rounding.
 """
