@@ -218,6 +218,12 @@ int exo_step_values(exo_engine *e, int member, int64_t step, double *out16);
  */
 int exo_read_snapshot(exo_engine *e, int member, float *out, size_t floats);
 
+/* Id of snapshot plane `index` for a member of `levels` levels, as
+ * GCM_FIELDS orders them (fields/index.ts:81-128, fieldCount :416-418);
+ * NULL past the last plane. The string stays valid until the calling
+ * thread's next call. */
+const char *exo_snapshot_field_name(int levels, int index);
+
 /* Area-weighted global means (GCM_GRID.globalMean, grid/index.ts:99-111)
  * of surface temperature (K) and TOA net flux (W/m^2). */
 int exo_global_means(exo_engine *e, int member, double *tsurf,

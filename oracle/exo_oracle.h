@@ -44,6 +44,10 @@ int oracle_routine(const oracle_member* m, const char* name);
  * intensity parts of one column (see exo_oracle_physics.c). */
 int oracle_radiation_fit(const oracle_member* m, int which, const real* u, int n, real* out);
 int oracle_storm_potential(const oracle_member* m, int h, real* out);
+/* The snapshot pack of one main-loop step (snapshot/wgsl/pack.wgsl:14-107):
+ * surface-temperature extrema tracking and, when snapshot_step[2] is set,
+ * the published planes. */
+void oracle_pack(const oracle_member* m);
 /* Set-up transforms over `planes` planes (qi or qj when dmu). */
 void oracle_transform_inverse(const oracle_member* m, const real* sp, real* grid, int planes, int dmu);
 void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp, int planes);
@@ -82,7 +86,9 @@ void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp
   X(sea_cheata) X(sea_cprsa) X(sea_naccua) X(localprecip)                   \
   X(localavgprecip) X(localweathering) X(localavgweather) X(localavgtemps)  \
   X(co2) X(carbon_step) X(capen) X(lnb) X(mpoti) X(chim) X(laav) X(venti)    \
-  X(vrmpi) X(gpi) X(shear)
+  X(vrmpi) X(gpi) X(shear) X(dpersist) X(dcveg) X(dcsoil) X(dlai) X(dvsoil) \
+  X(dagg) X(dgrow) X(dsc) X(dmr) X(dnpp) X(snapshot) X(snapshot_tracking)    \
+  X(snapshot_step)
 
 #define TABLES(X)                                                            \
   X(fourier_forward) X(fourier_inverse) X(legendre_qc) X(legendre_qe)        \

@@ -1903,6 +1903,121 @@ static void carbonstep(const M* v) {
   }
 }
 
+/* ------------------------------------------------------------- snapshot */
+
+/* snapshot/wgsl/pack.wgsl:14-107. Plane k of the snapshot is published
+ * field k of GCM_FIELDS ORDER (fields/index.ts:81-128); plane 26 (sea-level
+ * pressure) and the unset layer planes stay as they are. The tracking rows
+ * hold the running minimum and maximum of the surface temperature (0, 1),
+ * the last closed window's minimum, maximum and range (2-4), the sum and
+ * count of closed ranges since the last publish (5, 6), whether a window
+ * is open (7) and whether one has closed (8). */
+void oracle_pack(const oracle_member* m) {
+  M vv = view(m);
+  const M* v = &vv;
+  const int NH = v->NHOR, L = v->NLEV;
+  /* gcm_publish.f90's jet level: the full level nearest sigma 0.25
+   * (snapshot/index.ts:34-38) */
+  int jet = 0;
+  for (int l = 1; l < L; l++)
+    if (fabs(T(sigma)[l] - 0.25) < fabs(T(sigma)[jet] - 0.25)) jet = l;
+  const real* ss = F(snapshot_step);
+  const unsigned step = (unsigned)ss[0], window = (unsigned)ss[1];
+  real* snap = F(snapshot);
+  real* tr = F(snapshot_tracking);
+#define PUT(k, value) (snap[(long)(k) * NH + h] = (value))
+#define TR(k) tr[(long)(k) * NH + h]
+  for (int h = 0; h < NH; h++) {
+    const real ts = LV(dt, L, h);
+    if (TR(7) == 0) {
+      TR(0) = ts;
+      TR(1) = ts;
+      TR(7) = 1;
+    }
+    TR(0) = MIN(TR(0), ts);
+    TR(1) = MAX(TR(1), ts);
+    if (step > 0 && window > 0 && step % window == 0) {
+      const real range = TR(1) - TR(0);
+      TR(2) = TR(0);
+      TR(3) = TR(1);
+      TR(4) = range;
+      TR(5) = TR(5) + range;
+      TR(6) = TR(6) + 1;
+      TR(7) = 0;
+      TR(8) = 1;
+    }
+    if (ss[2] == 0) continue;
+    const real sw_top = LV(dswfl, 0, h), lw_top = LV(dlwfl, 0, h);
+    const real sw_sfc = LV(dswfl, L, h), lw_sfc = LV(dlwfl, L, h);
+    PUT(0, ts);                                   /* surfaceTemperature */
+    PUT(1, F(dtsa)[h]);                           /* airTemperature */
+    PUT(2, LV(du, L - 1, h));                     /* windU, lowest level */
+    PUT(3, LV(dv, L - 1, h));
+    PUT(4, LV(dcc, L, h));                        /* total cloud cover */
+    PUT(5, F(dprc)[h] + F(dprl)[h]);
+    PUT(6, F(dp)[h]);
+    PUT(7, F(dqvi)[h]);
+    PUT(8, F(dicec)[h]);
+    PUT(9, F(dsnow)[h]);
+    PUT(10, F(dwatc)[h]);
+    PUT(11, F(dalb)[h]);
+    PUT(12, LV(du, jet, h));
+    PUT(13, LV(dv, jet, h));
+    PUT(14, -lw_top);                             /* outgoing longwave */
+    PUT(15, F(dls)[h]);
+    PUT(16, sw_top + lw_top);                     /* TOA net */
+    /* hfns = snow melt x L_f rho_water + rss + rls + hfss + hfls */
+    PUT(17, F(dsmelt)[h] * (real)-333700000.0 + sw_sfc + lw_sfc + F(dshfl)[h] + F(dlhfl)[h]);
+    PUT(18, -F(devap)[h]);
+    PUT(19, F(drunoff)[h]);
+    PUT(20, F(dforest)[h]);
+    PUT(21, F(dlai)[h]);
+    PUT(22, F(dprs)[h]);
+    PUT(23, F(dsmelt)[h]);
+    PUT(24, F(diced)[h]);
+    PUT(25, F(dglac)[h]);
+    PUT(27, sw_sfc);
+    PUT(28, lw_sfc);
+    PUT(29, F(dshfl)[h]);
+    PUT(30, F(dlhfl)[h]);
+    PUT(31, F(dprc)[h]);
+    PUT(32, F(dprl)[h]);
+    PUT(33, sw_top);
+    PUT(34, lw_top);
+    const int closed = TR(8) != 0;
+    PUT(35, closed ? TR(2) : TR(0));
+    PUT(36, closed ? TR(3) : TR(1));
+    PUT(37, closed ? TR(4) : TR(1) - TR(0));
+    PUT(38, TR(5));
+    PUT(39, TR(6));
+    PUT(40, F(dnpp)[h]);
+    PUT(41, F(gmu0)[h]);
+    TR(5) = 0;
+    TR(6) = 0;
+    PUT(42, F(capen)[h]);
+    PUT(43, F(lnb)[h]);
+    PUT(44, F(mpoti)[h]);
+    PUT(45, F(chim)[h]);
+    PUT(46, F(laav)[h]);
+    PUT(47, F(venti)[h]);
+    PUT(48, F(vrmpi)[h]);
+    PUT(49, F(gpi)[h]);
+    PUT(50, F(shear)[h]);
+    for (int l = 0; l < L; l++) {
+      const int k = 51 + 5 * l;
+      PUT(k, LV(dt, l, h));
+      PUT(k + 1, LV(du, l, h));
+      PUT(k + 2, LV(dv, l, h));
+      PUT(k + 4, LV(dcc, l, h));
+    }
+    /* relative humidity of the lowest level only */
+    const real qs = sat_divided(v, LV(dt, L - 1, h), F(dp)[h] * T(sigma)[L - 1]);
+    PUT(51 + 5 * (L - 1) + 3, MAX(0.0, LV(dq, L - 1, h)) / MAX((real)1.17549435e-38, qs));
+  }
+#undef PUT
+#undef TR
+}
+
 /* ------------------------------------------------------------- sequence */
 
 /* gridpointd's parameterizations in the reference's dispatch order

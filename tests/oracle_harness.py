@@ -54,6 +54,7 @@ def oracle_library(precision="fp64"):
     lib.oracle_radiation_fit.restype = ctypes.c_int
     lib.oracle_storm_potential.argtypes = [ctypes.POINTER(Mem), ctypes.c_int, P]
     lib.oracle_storm_potential.restype = ctypes.c_int
+    lib.oracle_pack.argtypes = [ctypes.POINTER(Mem)]
     _LIBS[precision] = lib
     return lib
 
@@ -184,6 +185,12 @@ class OracleRun:
         if self.lib.oracle_storm_potential(ctypes.byref(m), int(column), _ptr(out)) != 0:
             return None
         return out
+
+    def pack(self):
+        """The snapshot pack of one main-loop step on the current arena
+        (tracking, and the planes when its snapshot_step says publish)."""
+        m = self._struct(self.consts[-1])
+        self.lib.oracle_pack(ctypes.byref(m))
 
     def set_constant(self, name, value, phase=None):
         self.consts[-1 if phase is None else phase][self.const_names.index(name)] = value

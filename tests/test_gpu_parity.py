@@ -73,6 +73,66 @@ def test_first_step_within_1e5(exo, trunc, levels, minutes, land):
 
 
 @pytest.mark.parametrize("trunc,levels,minutes,land", CASES)
+def test_every_coefficient_within_1e5_per_level(exo, trunc, levels, minutes, land):
+    """Max-norm per level and field, after the first step (a start-up
+    step) and after the first main-loop step: |GPU - FP64|_inf <= 1e-5 x
+    |FP64|_inf over each level's coefficients. Temperature's spectral field
+    is the deviation from the reference profile, so each of its
+    coefficients is measured in kelvin against the 250 K state it perturbs
+    (|dst|_inf ct / sqrt 2 <= 1e-5 x 250 K).
+
+    Divergence starts from rest and is a small residual of large,
+    cancelling terms: sequential FP32 arithmetic itself (the oracle's FP32
+    emulation) misses 1e-5 on its top levels, by up to 2.4e-5. Where the
+    emulation's own max-norm error exceeds 1e-5, the GPU is held to that
+    error instead (no worse than sequential FP32)."""
+    cfg = exo.default_config(truncation=trunc, levels=levels, timestep_minutes=minutes, land_fraction=land)
+    gpu = exo.Engine(0, 1)
+    host = exo.Engine(-1, 1)
+    try:
+        g = gpu.add_member(cfg)
+        h = host.add_member(cfg)
+        o64 = OracleRun(host, h)
+        o32 = OracleRun(host, h, precision="fp32")
+        _, startup = host.phases(h)
+        k = host.constants(h, 0)
+        done = 0
+        worst = {}
+        bad = []
+        for after in (1, startup + 1):
+            gpu.step(after - done)
+            gpu.sync()
+            o64.step(after - done, physics=True)
+            o32.step(after - done, physics=True)
+            done = after
+            for name in SPECTRAL:
+                a = gpu.read_field(g, name).astype(np.float64)
+                c = o32.field(name).astype(np.float64)
+                b = o64.field(name)
+                assert np.all(np.isfinite(a)), name
+                planes = a.size // ((trunc + 1) * (trunc + 2))
+                a, b, c = (x.reshape(planes, -1) for x in (a, b, c))
+                for l in range(planes):
+                    if name == "st":
+                        scale = np.sqrt(2.0) * 250.0 / k["ct"]
+                    else:
+                        scale = max(np.max(np.abs(b[l])), 1e-300)
+                    rel = np.max(np.abs(a[l] - b[l])) / scale
+                    emu = np.max(np.abs(c[l] - b[l])) / scale
+                    key = (after, name)
+                    worst[key] = max(worst.get(key, (0.0, 0.0)), (rel, emu))
+                    if not rel <= max(1e-5, emu):
+                        bad.append((after, name, l, rel, emu))
+        for (after, name), (rel, emu) in sorted(worst.items()):
+            print(f"T{trunc} L{levels} after step {after} {name}: worst level max-norm GPU {rel:.2e} "
+                  f"(FP32 emulation {emu:.2e})")
+        assert not bad, bad
+    finally:
+        gpu.close()
+        host.close()
+
+
+@pytest.mark.parametrize("trunc,levels,minutes,land", CASES)
 def test_steps_within_fp32_envelope(exo, trunc, levels, minutes, land):
     """Per field, the RMS over steps 1..N of the GPU's relative error must
     stay within twice the same RMS of the FP32 emulation's (single-step

@@ -8,6 +8,8 @@
 //  k_global_means  area-weighted global means of surface temperature and
 //                  TOA net flux (GCM_GRID.globalMean, grid/index.ts:99-111),
 //                  accumulated in FP64 by one workgroup per member.
+#include <string>
+
 #include "engine.h"
 
 namespace {
@@ -158,4 +160,30 @@ extern "C" __global__ __launch_bounds__(256) void k_global_means(const DevMember
     out[2 * blockIdx.x] = s0[0] / s2[0];
     out[2 * blockIdx.x + 1] = s1[0] / s2[0];
   }
+}
+
+// The published fields' ids in plane order (GCM_FIELDS ORDER, fields/
+// index.ts:81-128): 42 surface fields, the 9 storm fields, then five per
+// level from the top.
+extern "C" const char* exo_snapshot_field_name(int levels, int index) {
+  static const char* const surface[] = {
+      "surfaceTemperature", "airTemperature", "windU", "windV", "cloudCover", "precipitation",
+      "surfacePressure", "waterVapor", "seaIce", "snowDepth", "soilWater", "albedo", "jetU", "jetV",
+      "outgoingLongwave", "landFraction", "toaNetRadiation", "surfaceNetHeat", "evaporation", "runoff",
+      "forestCover", "leafAreaIndex", "snowfall", "snowmelt", "seaIceThickness", "glacierCover",
+      "seaLevelPressure", "surfaceShortwave", "surfaceLongwave", "sensibleHeatFlux", "latentHeatFlux",
+      "convectivePrecipitation", "largeScalePrecipitation", "toaShortwave", "toaLongwave",
+      "surfaceTemperatureMin", "surfaceTemperatureMax", "diurnalTemperatureRange", "dailyTemperatureRangeSum",
+      "dailyTemperatureRangeDays", "netPrimaryProduction", "solarZenithCosine", "cape",
+      "neutralBuoyancyPressure", "potentialIntensity", "entropyDeficit", "absoluteVorticity",
+      "ventilationIndex", "ventilationReducedIntensity", "genesisPotentialIndex", "windShear"};
+  static const char* const layer[] = {"layerTemperature", "layerWindU", "layerWindV", "layerHumidity",
+                                      "layerCloud"};
+  static thread_local std::string name;
+  const int n_surface = (int)(sizeof(surface) / sizeof(surface[0]));
+  if ((levels != 5 && levels != 10) || index < 0 || index >= n_surface + 5 * levels) return nullptr;
+  if (index < n_surface) return surface[index];
+  const int k = index - n_surface;
+  name = std::string(layer[k % 5]) + std::to_string(k / 5 + 1);
+  return name.c_str();
 }

@@ -138,6 +138,7 @@ def library():
         "exo_arena_floats": (I, [P, I, ctypes.POINTER(S)]),
         "exo_slot": (I, [P, I, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
         "exo_slot_name": (ctypes.c_char_p, [I]),
+        "exo_snapshot_field_name": (ctypes.c_char_p, [I, I]),
         "exo_read_field": (I, [P, I, ctypes.c_char_p, F, S]),
         "exo_write_field": (I, [P, I, ctypes.c_char_p, F, S]),
         "exo_read_table": (I, [P, I, ctypes.c_char_p, D, S]),
@@ -307,6 +308,17 @@ def slot_names():
             return out
         out.append(n.decode())
         i += 1
+
+
+def snapshot_field_names(levels):
+    """Ids of the snapshot planes of a member of `levels` levels."""
+    lib = library()
+    out = []
+    while True:
+        n = lib.exo_snapshot_field_name(levels, len(out))
+        if n is None:
+            return out
+        out.append(n.decode())
 
 
 def constant_names():
