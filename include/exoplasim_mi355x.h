@@ -126,6 +126,20 @@ int exo_remove_member(exo_engine *e, int member);
  */
 int exo_step(exo_engine *e, int steps);
 
+/* exo_step for callers that cannot allocate (code that must not call
+ * hipMalloc: tight serving loops, harnesses that forbid allocation): no
+ * counterpart in the reference. It enqueues the same kernels with the same
+ * results, launched one by one (no step graph, whose instantiation
+ * allocates), and keeps the steps' calendar rows in a static device array
+ * of EXO_NOALLOC_ROWS member-steps (1 MiB per device, held until the
+ * process ends) staged through a pinned block exo_create allocated; it
+ * calls no hipMalloc, hipFree or other allocating routine. active members
+ * x steps above EXO_NOALLOC_ROWS fails with EXO_E_ARG before any work.
+ * Every engine on one device shares the array: calls that overlap on one
+ * device from different host threads race. */
+#define EXO_NOALLOC_ROWS 16384
+int exo_step_noalloc(exo_engine *e, int steps);
+
 /* Publish a snapshot every `every` main-loop steps (0: never), as
  * GcmRunRequest.publishEvery (runner/types.ts:47-67). */
 int exo_set_publish_every(exo_engine *e, int every);

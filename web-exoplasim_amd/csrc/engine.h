@@ -181,6 +181,13 @@ struct Engine {
   std::vector<int64_t> graph_key;
   hipGraphExec_t graph = nullptr;
   int* d_step_index = nullptr;
+  // exo_step_noalloc's calendar rows: the device address of the file-scope
+  // scratch array on this engine's device (hipGetSymbolAddress at create),
+  // a pinned host staging block of the same size allocated at create, and
+  // the event after the last upload from it.
+  float* d_scratch = nullptr;
+  float* h_scratch = nullptr;
+  hipEvent_t scratch_done = nullptr;
 };
 
 // Every launch of the step goes through here, so the probe can bracket

@@ -37,6 +37,12 @@ int set_error(int code, const std::string& message) { return fail(code, message)
 
 extern "C" __global__ void k_global_means(const DevMember*, LaunchPlan, double*);
 
+// Calendar rows of exo_step_noalloc (EXO_NOALLOC_ROWS member-steps of
+// EXO_STEP_VALUES floats, 1 MiB). Host code never passes this name to a
+// kernel or a copy: it uses the address hipGetSymbolAddress gives for the
+// engine's device (Engine::d_scratch). Every engine on a device shares it.
+static __device__ __align__(256) float scratch[EXO_NOALLOC_ROWS * EXO_STEP_VALUES];
+
 namespace {
 
 // Drops a member's reference to its truncation's transform tables.
@@ -366,7 +372,15 @@ int exo_create(int device, int max_members, exo_engine** out) {
     HIP_OR_THROW(hipEventCreate(&e.t0));
     HIP_OR_THROW(hipEventCreate(&e.t1));
     HIP_OR_THROW(hipMalloc(&e.d_step_index, sizeof(int)));
+    // exo_step_noalloc's buffers, set up once here so that it allocates
+    // nothing: the device address of the scratch array (with this device
+    // current) and a pinned staging block of the same size
+    HIP_OR_THROW(hipGetSymbolAddress(reinterpret_cast<void**>(&e.d_scratch), HIP_SYMBOL(scratch)));
+    HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&e.h_scratch), sizeof(scratch), hipHostMallocDefault));
+    HIP_OR_THROW(hipEventCreateWithFlags(&e.scratch_done, hipEventDisableTiming));
   } catch (const std::exception& ex) {
+    if (e.h_scratch) hipHostFree(e.h_scratch);
+    if (e.scratch_done) hipEventDestroy(e.scratch_done);
     return fail(EXO_E_DEVICE, ex.what());
   }
   *out = h.release();
@@ -395,6 +409,8 @@ void exo_destroy(exo_engine* h) {
   hipFree(e.d_desc);
   hipFree(e.d_params);
   if (e.h_params) hipHostFree(e.h_params);
+  if (e.h_scratch) hipHostFree(e.h_scratch);
+  if (e.scratch_done) hipEventDestroy(e.scratch_done);
   if (e.params_done) hipEventDestroy(e.params_done);
   if (e.t0) hipEventDestroy(e.t0);
   if (e.t1) hipEventDestroy(e.t1);
@@ -434,7 +450,10 @@ int exo_remove_member(exo_engine* h, int id) {
   return EXO_OK;
 }
 
-int exo_step(exo_engine* h, int steps) {
+// exo_step (fixed = false: calendar rows in the engine's growing buffer,
+// step-graph replays allowed) and exo_step_noalloc (fixed = true: rows in
+// the static scratch array, single launches, no allocation).
+static int step_members(exo_engine* h, int steps, bool fixed) {
   if (!h || steps < 0) return fail(EXO_E_ARG, "invalid step count");
   Engine& e = h->e;
   if (e.device < 0) return fail(EXO_E_STATE, "host-only engine cannot step");
@@ -461,14 +480,25 @@ int exo_step(exo_engine* h, int steps) {
     HIP_OR_THROW(hipSetDevice(e.device));
     const size_t per_step = act.size() * EXO_STEP_VALUES;
     const size_t need = per_step * (size_t)steps;
-    if (e.params_cap > 0) HIP_OR_THROW(hipEventSynchronize(e.params_done));
-    if (need > e.params_cap) {
-      if (e.h_params) HIP_OR_THROW(hipHostFree(e.h_params));
-      if (e.d_params) HIP_OR_THROW(hipFree(e.d_params));
-      HIP_OR_THROW(hipHostMalloc(&e.h_params, need * sizeof(float), hipHostMallocDefault));
-      HIP_OR_THROW(hipMalloc(&e.d_params, need * sizeof(float)));
-      e.params_cap = need;
+    if (fixed) {
+      if (act.size() * (size_t)steps > EXO_NOALLOC_ROWS)
+        return fail(EXO_E_ARG, std::to_string(act.size()) + " members x " + std::to_string(steps) +
+                                   " steps exceed the " + std::to_string(EXO_NOALLOC_ROWS) +
+                                   " member-steps of exo_step_noalloc's calendar buffer");
+      HIP_OR_THROW(hipEventSynchronize(e.scratch_done));
+    } else {
+      if (e.params_cap > 0) HIP_OR_THROW(hipEventSynchronize(e.params_done));
+      if (need > e.params_cap) {
+        if (e.h_params) HIP_OR_THROW(hipHostFree(e.h_params));
+        if (e.d_params) HIP_OR_THROW(hipFree(e.d_params));
+        HIP_OR_THROW(hipHostMalloc(&e.h_params, need * sizeof(float), hipHostMallocDefault));
+        HIP_OR_THROW(hipMalloc(&e.d_params, need * sizeof(float)));
+        e.params_cap = need;
+      }
     }
+    float* const h_rows = fixed ? e.h_scratch : e.h_params;
+    float* const d_rows = fixed ? e.d_scratch : e.d_params;
+    const hipEvent_t rows_done = fixed ? e.scratch_done : e.params_done;
     // Calendar values for every step of the call, then one upload.
     std::vector<int> phase(act.size());
     std::vector<int64_t> stepno(act.size()), carbon(act.size());
@@ -500,7 +530,7 @@ int exo_step(exo_engine* h, int steps) {
           avg_step[k].push_back(stepno[i]);
         }
         exo::step_values(m, stepno[i], carbon[i], publish, v);
-        for (int c = 0; c < EXO_STEP_VALUES; c++) e.h_params[k * per_step + i * EXO_STEP_VALUES + c] = (float)v[c];
+        for (int c = 0; c < EXO_STEP_VALUES; c++) h_rows[k * per_step + i * EXO_STEP_VALUES + c] = (float)v[c];
         phase_at[k][i] = phase[i];
         stepno_at[k][i] = stepno[i];
         carbon_at[k][i] = carbon[i];
@@ -515,14 +545,14 @@ int exo_step(exo_engine* h, int steps) {
       stepno_at[steps][i] = stepno[i];
       carbon_at[steps][i] = carbon[i];
     }
-    HIP_OR_THROW(hipMemcpyAsync(e.d_params, e.h_params, need * sizeof(float), hipMemcpyHostToDevice, e.stream));
-    HIP_OR_THROW(hipEventRecord(e.params_done, e.stream));
+    HIP_OR_THROW(hipMemcpyAsync(d_rows, h_rows, need * sizeof(float), hipMemcpyHostToDevice, e.stream));
+    HIP_OR_THROW(hipEventRecord(rows_done, e.stream));
     HIP_OR_THROW(hipEventRecord(e.t0, e.stream));
     // A step's launches depend only on the active members and their loaded
     // phases (the storm plan reads the phase constants), so steps with one
     // key replay one graph. The kernel probe brackets single launches and
     // needs the launches themselves.
-    const bool graphs = e.graphs && !e.probe.kernel;
+    const bool graphs = !fixed && e.graphs && !e.probe.kernel;
     const auto key_at = [&](int k) {
       std::vector<int64_t> key{(int64_t)e.epoch, (int64_t)(intptr_t)e.d_params, (int64_t)per_step};
       for (size_t i = 0; i < act.size(); i++) {
@@ -590,7 +620,7 @@ int exo_step(exo_engine* h, int steps) {
         HIP_OR_THROW(hipGraphLaunch(e.graph, e.stream));
         index_at = k + 1;
       } else {
-        exo::Stepper(e, desc).load_step(e.d_params + k * per_step);
+        exo::Stepper(e, desc).load_step(d_rows + k * per_step);
         exo::enqueue_step(e, desc);
         if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
       }
@@ -607,6 +637,10 @@ int exo_step(exo_engine* h, int steps) {
   }
   return EXO_OK;
 }
+
+int exo_step(exo_engine* h, int steps) { return step_members(h, steps, false); }
+
+int exo_step_noalloc(exo_engine* h, int steps) { return step_members(h, steps, true); }
 
 int exo_sync(exo_engine* h) {
   if (!h) return fail(EXO_E_ARG, "null engine");

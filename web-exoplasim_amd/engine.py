@@ -132,6 +132,7 @@ def library():
         "exo_add_member": (I, [P, ctypes.POINTER(ExoConfig), I, ctypes.POINTER(I)]),
         "exo_remove_member": (I, [P, I]),
         "exo_step": (I, [P, I]),
+        "exo_step_noalloc": (I, [P, I]),
         "exo_sync": (I, [P]),
         "exo_member_step": (I, [P, I, ctypes.POINTER(ctypes.c_int64)]),
         "exo_member_date": (I, [P, I, ctypes.POINTER(I)]),
@@ -397,6 +398,11 @@ class Engine:
     # stepping -----------------------------------------------------------
     def step(self, steps=1):
         _check(self.lib.exo_step(self.h, steps))
+
+    def step_noalloc(self, steps=1):
+        """exo_step without any allocation (calendar rows in the library's
+        static device array; single launches)."""
+        _check(self.lib.exo_step_noalloc(self.h, steps))
 
     def sync(self):
         _check(self.lib.exo_sync(self.h))
