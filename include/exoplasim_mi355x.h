@@ -66,7 +66,9 @@ typedef struct exo_config {
   double initial_temperature;  /* isothermal atmosphere, K */
   double initial_humidity;     /* kg/kg */
   double noise_kelvin;         /* std of grid-point temperature noise, K */
-  double sea_surface_k;        /* SST climatology, K */
+  double sea_surface_k;        /* uniform SST climatology (code 169), K; NaN: none, and the
+                                  ocean starts as the reference's does without one, at the
+                                  freezing point under sea ice */
   uint64_t terrain_seed;
   uint64_t noise_seed;
   /* GcmConfig.stormclim (types.ts:30): 1 runs hurricanemod's storm
@@ -74,9 +76,38 @@ typedef struct exo_config {
    * index.ts:175; physics/storm/index.ts:22-45) into the snapshot's
    * storm fields; 0 (default) leaves them at zero. */
   int storm_diagnostics;
+  /* The rest of GcmConfig (types.ts:28-62), as GCM_NAMELIST.edits writes
+   * them into the namelists (namelist/index.ts:40-182). A real the
+   * reference allows to be null takes NaN for null. */
+  int vegetation;                  /* vegetation 0, 1 or 2: NVEG */
+  double initial_growth;           /* initialGrowth: RINIDAGG */
+  int glaciers;                    /* glaciers.enabled: NGLACIER */
+  double glacier_min_depth_m;      /* glaciers.minDepthM: GLACELIM */
+  double glacier_initial_height_m; /* glaciers.initialHeightM: ICESHEETH (< 0: none) */
+  double star_temperature_k;       /* starTemperatureK: > 0 sets NSTARTEMP 1, STARBBTEMP */
+  int synchronous;                 /* synchronous: NFIXED */
+  double substellar_longitude_deg; /* substellarLongitudeDeg: FIXEDLON */
+  int fixed_orbit;                 /* fixedOrbit: NFIXORB (0, Berger's orbit, is not built) */
+  int keplerian;                   /* keplerian: NGENKEPLERIAN */
+  double periapse_longitude_deg;   /* periapseLongitudeDeg: MVELP (NaN: null) */
+  double initial_mean_anomaly_deg; /* initialMeanAnomalyDeg: MEANANOM0 (NaN: null) */
+  double soil_depth_scale;         /* soilDepthScale: DSOILZ scale */
+  double soil_water_capacity_m;    /* soilWaterCapacityM: WSMAX (NaN: null) */
+  double soil_heat_capacity;       /* soilHeatCapacity: SOILCAP (NaN: null) */
+  double land_albedo;              /* landAlbedo: ALBLAND, DGROUNDALB (NaN: null) */
+  double ocean_albedo;             /* oceanAlbedo: ALBSEA, DOCEANALB (NaN: null) */
+  double snow_ice_albedo;          /* snowIceAlbedo: ALBICE, DICEALBMN/MX, DSNOWALBMN, DGLACALBMN,
+                                      DSNOWALB (NaN: null) */
+  int wet_soil;                    /* wetSoil: NWETSOIL */
+  int filter;                      /* physicsFilter's NFILTER: 0 none, 1 cesaro, 2 exp, 3 lh
+                                      (NGPTFILTER and NSPVFILTER stay 1, as in the base
+                                      namelist) */
 } exo_config;
 
-/* Fills *cfg with the defaults of a T21 L10 Earth-like synthetic planet. */
+/* Fills *cfg with the defaults of a T21 L10 Earth-like synthetic planet:
+ * 1 bar N2 with 360 ppm CO2, 1367 W/m^2 from a 5780 K star, obliquity
+ * 23.4 deg, a circular orbit, and every other GcmConfig field at
+ * GCM.DEFAULT_CONFIG's value (index.ts:4-44). */
 void exo_config_default(exo_config *cfg);
 
 /* Last error message of the calling thread ("" if none). */
@@ -114,6 +145,41 @@ int exo_add_member(exo_engine *e, const exo_config *cfg, int precision,
 int exo_add_member_boundary(exo_engine *e, const exo_config *cfg,
                             int precision, const double *land_fraction,
                             const double *geopotential, int *member);
+
+/* One surface field of a planet's .sra files (surface_ini's codes,
+ * setup/surface/index.ts:49-70: 172 land-sea mask, 129 orography
+ * (geopotential, m^2/s^2), 169 surface temperature, 173 roughness, 1730
+ * orographic roughness, 174/175/176 albedos, 209 soil temperature, 210 ice
+ * cover, 211 ice thickness, 212 forest, 229 field capacity, 232 glacier,
+ * 1604-1607 vegetation): `months` records (1, 12 or 14) of nlat x 2 nlat
+ * values each, north first. */
+typedef struct {
+  int code;
+  int months;
+  const double *values;
+} exo_surface_field;
+
+/* exo_add_member with the planet's surface from .sra fields, as the
+ * reference's surface set-up reads them (SURFACE_SETUP.initial and its
+ * read, setup/surface/index.ts:76-130, :267-643): each code absent takes
+ * the reference's fallback; a 12-month climatology gains December before
+ * and January after, a single record fills all 14 months, any other count
+ * fails with the reference's "N032_surf_0169.sra has 3 fields, expected
+ * 14". Without both 172 and 129 the seeded terrain of *cfg gives the land
+ * and orography; cfg->sea_surface_k stands for a missing 169. */
+int exo_add_member_surface(exo_engine *e, const exo_config *cfg, int precision,
+                           const exo_surface_field *fields, int count, int *member);
+
+/* GCM_ATMOSPHERE.resolve (atmosphere/index.ts:84-160): surface pressure
+ * (bar), gas constant (J/(kg K)) and CO2 (ppm) from partial pressures
+ * (bar; NaN: the reference's null) of H2, He, N2, O2, Ar, Ne, Kr, CO2 and
+ * H2O, in that order, in the mode "partialPressures" (pressure_bar and
+ * gas_constant NaN) or "pressureAndGasConstant" (either may be NaN for
+ * the native value). EXO_E_ARG with the reference's message when the
+ * atmosphere is invalid. */
+int exo_atmosphere_resolve(const double *partial_pressures_bar, int pressure_and_gas_constant,
+                           double pressure_bar, double gas_constant, double *out_pressure_bar,
+                           double *out_gas_constant, double *out_co2_ppm);
 
 /* Removes a member and frees its device memory. */
 int exo_remove_member(exo_engine *e, int member);

@@ -76,7 +76,8 @@ void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp
   X(dsnow) X(dsmelt) X(dsndch) X(dtsoil) X(dtd2) X(dtd3) X(dtd4) X(dtd5)    \
   X(diced) X(dmld) X(surface_step) X(land_dts) X(land_dtsm) X(land_dqs)     \
   X(driver) X(duroff) X(dvroff) X(darea) X(dsnowt) X(dsnowz) X(dwater)      \
-  X(dalbclim) X(dalbclim1) X(dalbclim2) X(dsoilt) X(dalbcl) X(dalbcl1)      \
+  X(dz0climo) X(dalbclim) X(dalbclim1) X(dalbclim2) X(dsoilt) X(dalbcl)     \
+  X(dalbcl1)                                                                 \
   X(dalbcl2) X(xts) X(xsst) X(xmld) X(xls) X(xiced) X(xicec) X(xsnow)       \
   X(xqmelt) X(xicecc) X(xcflux) X(xcfluxf) X(xoheat) X(xoflux) X(xgw)       \
   X(xclsst2) X(xclicec2) X(xcliced2) X(xclssto) X(xcfluxr) X(ximelt)        \
@@ -118,7 +119,8 @@ void oracle_transform_forward(const oracle_member* m, const real* grid, real* sp
   X(albsmin2) X(albsmaxf2) X(albsminf2) X(albgmax) X(albgmin) X(albgmax1)    \
   X(albgmin1) X(albgmax2) X(albgmin2) X(drhsfull) X(nwetsoil) X(wsmax)       \
   X(albland) X(alblandmax) X(dgroundalb1) X(dgroundalb2) X(doceanalb1)       \
-  X(doceanalb2) X(nveg) X(nglacier) X(glacelim) X(xdt) X(xmind) X(xmaxd)     \
+  X(doceanalb2) X(nveg) X(nglacier) X(glacelim) X(rlue) X(co2conv)           \
+  X(tau_veg) X(tau_soil) X(ncveg) X(cveg_e) X(cveg_g) X(xdt) X(xmind) X(xmaxd) \
   X(cicemin) X(thicec) X(tfreeze) X(crhoi) X(crhos) X(crhosn) X(cpi)         \
   X(ckapi) X(ckapsn) X(clfi) X(clfsn) X(ice_coupled) X(dtmix) X(cps)         \
   X(charnock) X(dz0sea) X(dz0ice) X(drhssea) X(drhsice) X(albsea) X(albice)  \

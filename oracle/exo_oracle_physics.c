@@ -1360,8 +1360,81 @@ static real glacier_albedo(const M* v, real mx, real mn, real ts) {
   return MAX(mn, MIN(mx, mx - (mx - mn) * (ts - 263.16) / (K(tmelt) - 263.16)));
 }
 
-/* physics/land/wgsl/surface.wgsl:67-142 (vegstep and glacierstep are
- * inert with nveg = 0 and nglacier = 0) */
+/* simba's vegstep, one land column below 90 % glacier
+ * (physics/vegetation/wgsl/vegstep.wgsl:31-99). */
+static void vegstep(const M* v, int h) {
+  const int NH = v->NHOR, L = v->NLEV;
+  if (!(F(dls)[h] > 0 && F(dglac)[h] < 0.90)) return;
+  const real VWS_CRIT = 0.25, Q10 = 2, ZLAIMAX = 9, ZLAIMIN = 0.1, VALB_MIN = 0.12, VALB_MAX = 0.3;
+  const real VSALB_MIN = 0.2, VWMAX_MAX = 0.5, VWMAX_MIN = 0.05, VZ0_MIN = 0.05, CVEG_A = 2.9;
+  const real CVEG_C = 1.570796326794897, CVEG_F = 3, CVEG_K = 0.5, CVEG_L = 0.25, CT_CRIT = 5;
+  const real CO2_REF = 360, CO2_COMP = 0, CO2_SENS = 0.3;
+  const real co2 = F(co2)[0];
+  real zbeta = 0;
+  if (!(co2 < 1.0e-8)) zbeta = MAX(0.0, 1.0 + CO2_SENS * log((co2 - CO2_COMP) / (CO2_REF - CO2_COMP)));
+  const real ts = LV(dt, L, h), dp = F(dp)[h], dwatc = F(dwatc)[h], dsnow = F(dsnow)[h];
+  real zwmax = F(dwmax)[h];
+  const real zglacfree = 1.0 - F(dglac)[h];
+  const real zsvp = K(ra1) * exp(K(ra2) * (ts - K(tmelt)) / (ts - K(ra4)));
+  const real zvpd = MAX(0.01, zsvp - dp * LV(dq, L - 1, h) / 0.622);
+  real dcveg = F(dcveg)[h], dcsoil = F(dcsoil)[h];
+  const real zlaim = ZLAIMIN + 0.63661977236758 * ZLAIMAX * atan(CVEG_L * 2.0 * F(dagg)[h] * dcveg);
+  const real zvegm = 1.0 - exp(-CVEG_K * zlaim);
+  real zveg = 0;
+  if (zwmax > 0) zveg = MIN(MIN(1.0, zvegm), MAX(0.0, dwatc / (zwmax * VWS_CRIT)));
+  const real zft = MIN(1.0, MAX(0.0, (ts - K(tmelt)) / CT_CRIT));
+  const real zgppl = K(rlue) * zbeta * zft * zveg * LV(dfd, L, h);
+  real zgppw = 0;
+  if (!(co2 < 1.0e-8)) zgppw = K(co2conv) * dp * zveg * (-F(devap)[h]) / zvpd * (0.3 * co2);
+  zgppw = MIN(1.0, MAX(0.0, zgppw));
+  const real zgpp = MIN(zgppl, zgppw);
+  const real znpp = F(dgrow)[h] * 0.5 * zgpp * zglacfree;
+  F(dnpp)[h] = znpp;
+  const real zlitter = dcveg / K(tau_veg);
+  const real zres = pow(Q10, (ts - K(tmelt) - 10.0) / 10.0) * dcsoil / K(tau_soil);
+  dcveg = dcveg + (znpp - zlitter) * K(deltsec) * K(ncveg);
+  dcsoil = dcsoil + (zlitter - zres) * K(deltsec) * K(ncveg);
+  const real zforest = MIN(1.0, MAX(0.0, (atan(dcveg - CVEG_A) - K(cveg_e)) / (CVEG_C - K(cveg_e))));
+  const real dvsoil = MIN(1.0, MAX(0.0, (atan(dcveg - CVEG_F) - K(cveg_g)) / (CVEG_C - K(cveg_g))));
+  F(dcveg)[h] = dcveg;
+  F(dcsoil)[h] = dcsoil;
+  F(dvsoil)[h] = dvsoil;
+  F(dlai)[h] = -log(1.0 - zveg) / CVEG_K;
+  const real zvz0 = F(dmr)[h] * zforest + VZ0_MIN * (1.0 - zforest);
+  real zvalb = VALB_MIN * zveg + VALB_MAX * (1.0 - zveg);
+  zwmax = VWMAX_MAX * dvsoil + VWMAX_MIN * (1.0 - dvsoil);
+  real zvrhs = F(dsc)[h] * MIN(1.0, MAX(0.0, dwatc / (zwmax * VWS_CRIT)));
+  if (dsnow > 0) {
+    real zalbsn = (K(albsmax) - K(albsmin)) * (ts - 263.16) / (K(tmelt) - 263.16);
+    zalbsn = MAX(K(albsmin), MIN(K(albsmax), K(albsmax) - zalbsn));
+    zvalb = zvalb + (zalbsn - zvalb) * dsnow / (dsnow + 0.01);
+    zvalb = VSALB_MIN * zforest + zvalb * (1.0 - zforest);
+    zvrhs = 1.0;
+  }
+  if (K(nveg) == 2) {
+    const real z0clim = F(dz0climo)[h];
+    F(dz0)[h] = sqrt(zvz0 * zvz0 + z0clim * z0clim);
+    F(dwmax)[h] = zwmax;
+    F(drhs)[h] = zvrhs;
+    F(dalb)[h] = zvalb;
+    F(dforest)[h] = zforest;
+  }
+}
+
+/* glacierstep and glacierstop's yearly promotion
+ * (physics/glacier/wgsl/glacierstep.wgsl:1-25). */
+static void glacierstep(const M* v, int h) {
+  if (K(nglacier) == 0) return;
+  const int land = F(dls)[h] > 0.5;
+  if (land && F(dsnowz)[h] < K(glacelim)) F(dpersist)[h] = 0;
+  if (land && F(dsnowz)[h] < 0.1) F(dglac)[h] = 0;
+  if (F(surface_step)[6] > 0.5) {
+    if (F(dpersist)[h] > 0.5) F(dglac)[h] = 1;
+    F(dpersist)[h] = F(dls)[h] >= 0.5 ? 1 : 0;
+  }
+}
+
+/* physics/land/wgsl/surface.wgsl:67-142 */
 static void land_surface(const M* v) {
   const int NH = v->NHOR, L = v->NLEV;
   for (int h = 0; h < NH; h++) {
@@ -1420,12 +1493,14 @@ static void land_surface(const M* v) {
       LV(dt, L, h) = dts;
       LV(dq, L, h) = dqs;
     }
+    if (K(nveg) > 0) vegstep(v, h);
     if (F(dglac)[h] > 0.5 && dls > 0) {
       F(dalb)[h] = glacier_albedo(v, K(albgmax), K(albgmin), dts);
       F(dsalb)[h] = glacier_albedo(v, K(albgmax1), K(albgmin1), dts);
       F(dsalb)[NH + h] = glacier_albedo(v, K(albgmax2), K(albgmin2), dts);
       F(drhs)[h] = 1.0;
     }
+    glacierstep(v, h);
   }
 }
 

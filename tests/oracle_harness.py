@@ -8,8 +8,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIBS = {"fp64": os.path.join(ROOT, "oracle", "liboracle.so"),
-               "fp32": os.path.join(ROOT, "oracle", "liboracle32.so")}
-CTYPES = {"fp64": (ctypes.c_double, np.float64), "fp32": (ctypes.c_float, np.float32)}
+               "fp32": os.path.join(ROOT, "oracle", "liboracle32.so"),
+               "fp32fma": os.path.join(ROOT, "oracle", "liboracle32fma.so")}
+CTYPES = {"fp64": (ctypes.c_double, np.float64), "fp32": (ctypes.c_float, np.float32),
+          "fp32fma": (ctypes.c_float, np.float32)}
 
 
 def _member_struct(ct):

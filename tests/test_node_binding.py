@@ -50,18 +50,130 @@ console.log(JSON.stringify(r))
     assert len(out["errors"]) == 4
 
 
+# GCM.DEFAULT_CONFIG (src/model/climate/gcm/index.ts:4-44) with
+# GCM_ATMOSPHERE.DEFAULT_CONFIG (atmosphere/index.ts:66-81), as JSON.
+DEFAULT_CONFIG = {
+    "targetTemperatureK": None, "stormclim": False,
+    "atmosphere": {"mode": "partialPressures",
+                   "partialPressures": {g: None for g in ("H2", "He", "N2", "O2", "Ar", "Ne", "Kr", "CO2", "H2O")},
+                   "pressureBar": None, "gasConstant": None, "ozone": True},
+    "layers": 10, "periapseLongitudeDeg": None, "keplerian": False, "initialMeanAnomalyDeg": None,
+    "soilWaterCapacityM": None, "soilHeatCapacity": None, "soilDepthScale": 1, "landAlbedo": None,
+    "oceanAlbedo": None, "snowIceAlbedo": None, "radiusEarth": 1, "gravityMs2": 9.80665, "synchronous": False,
+    "substellarLongitudeDeg": 180, "oceanDepthM": 50, "fluxWm2": 1367, "starTemperatureK": 5780, "yearDays": 360,
+    "eccentricity": 0.016715, "rotationPeriodDays": 1, "obliquityDeg": 23.5, "fixedOrbit": True, "wetSoil": False,
+    "vegetation": 2, "initialGrowth": 0.5, "glaciers": {"enabled": True, "minDepthM": 2, "initialHeightM": -1},
+    "physicsFilter": ["gp", "exp", "sp"], "timestepMinutes": 45, "truncation": 21,
+    "earthSurfaceClimatology": False,
+}
+
+
 def test_engine_config_mapping():
-    out = _node("""
-const {engineConfig} = require('./runner.js')
-const c = engineConfig({truncation: 63, layers: 5, timestepMinutes: 30, radiusEarth: 1.5, gravityMs2: 9.0,
-  rotationPeriodDays: 2, fluxWm2: 1000, obliquityDeg: 10, eccentricity: 0.1, yearDays: 200, oceanDepthM: 20,
-  atmosphere: {ozone: false, pressureBar: 2, gasConstant: null}}, {landFraction: 0.2, terrainSeed: 9})
-console.log(JSON.stringify(c))
+    """The reference's default GcmConfig maps onto every exo_config field
+    it implies, the atmosphere resolved as GCM_ATMOSPHERE.resolve does
+    (1.011 bar of Earth air, R = 8314.46 / 28.970253, 300 ppm CO2)."""
+    out = _node(f"""
+const {{engineConfig}} = require('./runner.js')
+const c = engineConfig({json.dumps(DEFAULT_CONFIG)})
+for (const k of Object.keys(c)) if (Number.isNaN(c[k])) c[k] = 'NaN'
+const edited = engineConfig(Object.assign({json.dumps(DEFAULT_CONFIG)}, {{
+  truncation: 63, layers: 5, physicsFilter: ['gp', 'lh', 'sp'], landAlbedo: 0.3, vegetation: 0,
+  atmosphere: {{mode: 'partialPressures', partialPressures: {{N2: 0.79, O2: 0.21, CO2: 0.0004}}, ozone: false}}}}),
+  {{landFraction: 0.2, terrainSeed: 9}})
+console.log(JSON.stringify({{c, edited}}))
 """)
-    assert out == {"truncation": 63, "levels": 5, "timestep_minutes": 30, "radius_earth": 1.5, "gravity": 9.0,
-                   "rotation_period_days": 2, "flux_wm2": 1000, "obliquity_deg": 10, "eccentricity": 0.1,
-                   "year_days": 200, "ocean_depth_m": 20, "ozone": 0, "pressure_bar": 2, "land_fraction": 0.2,
-                   "terrain_seed": 9}
+    c = out["c"]
+    assert c["pressure_bar"] == 1.011 and c["co2_ppm"] == 300
+    assert abs(c["gas_constant"] - 8314.46261815324 / 28.970253) < 1e-9
+    assert (c["vegetation"], c["glaciers"], c["glacier_min_depth_m"], c["glacier_initial_height_m"]) == (2, 1, 2, -1)
+    assert (c["star_temperature_k"], c["filter"], c["fixed_orbit"], c["synchronous"]) == (5780, 2, 1, 0)
+    assert (c["eccentricity"], c["obliquity_deg"], c["substellar_longitude_deg"]) == (0.016715, 23.5, 180)
+    for k in ("periapse_longitude_deg", "initial_mean_anomaly_deg", "soil_water_capacity_m", "land_albedo",
+              "ocean_albedo", "snow_ice_albedo", "soil_heat_capacity"):
+        assert c[k] == "NaN", k
+    e = out["edited"]
+    assert (e["truncation"], e["levels"], e["filter"], e["land_albedo"], e["vegetation"], e["ozone"]) == \
+        (63, 5, 3, 0.3, 0, 0)
+    assert abs(e["pressure_bar"] - 1.0004) < 1e-12 and abs(e["co2_ppm"] - 0.0004 / 1.0004 * 1e6) < 1e-6
+    assert (e["land_fraction"], e["terrain_seed"]) == (0.2, 9)
+
+
+def test_engine_config_refuses_what_it_cannot_honour():
+    out = _node(f"""
+const {{engineConfig, start}} = require('./runner.js')
+const base = {json.dumps(DEFAULT_CONFIG)}
+const errors = []
+for (const edit of [{{targetTemperatureK: 288}}, {{fixedOrbit: false}}, {{bogus: 1}}, {{physicsFilter: ['gauss']}},
+                    {{atmosphere: {{mode: 'x', partialPressures: {{}}}}}},
+                    {{atmosphere: {{mode: 'partialPressures', partialPressures: {{N2: -1}}}}}}]) {{
+  try {{ engineConfig(Object.assign({{}}, base, edit)); errors.push(null) }} catch (x) {{ errors.push(x.message) }}
+}}
+const resolution = {{layers: 10, truncation: 42, latitudes: 64, cpu: null, gpu: {{inputs: 't42/inputs.json'}}}}
+for (const request of [{{config: base, resolution}}, {{config: Object.assign({{}}, base, {{truncation: 42}}), resolution, engine: 'cpu'}}]) {{
+  try {{ start(request); errors.push(null) }} catch (x) {{ errors.push(x.message) }}
+}}
+console.log(JSON.stringify(errors))
+""")
+    assert out == [
+        "The targeting runner (targetTemperatureK) is not built for the MI355X engine",
+        "A free orbit (fixedOrbit false, Berger's series) is not built for the MI355X engine",
+        "Unknown GcmConfig field bogus",
+        "Unknown physics filter gauss",
+        "Choose an atmosphere input mode.",
+        "Nitrogen (N₂) partial pressure must be finite and nonnegative.",
+        "Model assets do not match the configured resolution and layers",
+        "T42 is not built for the CPU engine",
+    ]
+
+
+@pytest.mark.gpu
+def test_gcm_run_from_the_reference_defaults_and_a_terrain():
+    """GcmRun started as the page starts it: GCM.DEFAULT_CONFIG, a terrain
+    raster (the land mask and orography as GCM_BOUNDARY.files makes them)
+    and no Earth climatology. It runs the reference's atmosphere (1.011
+    bar of air: the surface-pressure plane's mean over sea stays near
+    101100 Pa) with vegetation and glaciers on, and its ocean starts at the
+    freezing point under ice, as the reference's does without an SST
+    file."""
+    out = _node(f"""
+const {{start}} = require('./runner.js')
+const w = 360, h = 180
+const elevationM = new Float32Array(w * h), land = new Float32Array(w * h)
+for (let y = 0; y < h; y++) for (let x = 0; x < w; x++) {{
+  const lat = 90 - (y + 0.5), lon = -180 + (x + 0.5)
+  const inside = Math.abs(lat) < 40 && lon > -30 && lon < 60
+  land[y * w + x] = inside ? 1 : 0
+  elevationM[y * w + x] = inside ? 800 * Math.cos(lat * Math.PI / 90) : 0
+}}
+const snaps = []
+const resolution = {{layers: 10, truncation: 21, latitudes: 32, cpu: null, gpu: {{inputs: 't21/inputs.json'}}}}
+const run = start({{config: {json.dumps(DEFAULT_CONFIG)}, resolution, engine: 'gpu',
+  terrain: {{name: 'box', width: w, height: h, elevationM, land}}, publishEvery: 8, restart: null,
+  startPaused: false,
+  onStatus: (u) => {{ if (u.status === 'failed') console.log(JSON.stringify({{failed: u.message}})) }},
+  onDevice: () => {{}},
+  onSnapshot: (s) => {{
+    const nh = 2048, v = s.values
+    const mask = v.subarray(15 * nh, 16 * nh), ps = v.subarray(6 * nh, 7 * nh), ice = v.subarray(24 * nh, 25 * nh)
+    let sea = 0, psum = 0, iced = 0
+    for (let i = 0; i < nh; i++) if (mask[i] < 0.5) {{ sea++; psum += ps[i]; iced += ice[i] }}
+    snaps.push({{finite: v.every(Number.isFinite), sea, ps: psum / sea, ice: iced / sea,
+                 land: nh - sea, ts: v[0]}})
+    if (snaps.length === 3) run.stop()
+  }},
+  onCheckpoint: () => {{}}}})
+setTimeout(() => console.log(JSON.stringify({{snaps}})), 60000).unref()
+process.on('beforeExit', () => console.log(JSON.stringify({{snaps}})))
+""", timeout=300)
+    assert "failed" not in out, out
+    snaps = out["snaps"]
+    assert len(snaps) == 3 and all(s["finite"] for s in snaps)
+    assert 0 < snaps[0]["land"] < 2048
+    assert all(abs(s["ps"] - 101100.0) < 1500.0 for s in snaps), snaps
+    # the whole ocean starts under sea ice about a metre thick (the
+    # snapshot's seaIceThickness); its cover follows the ice model's
+    # concentration, which the reference's set-up leaves at 0
+    assert snaps[0]["ice"] > 0.5, snaps
 
 
 @pytest.mark.gpu

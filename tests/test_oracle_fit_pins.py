@@ -52,7 +52,7 @@ from oracle_harness import OracleRun
 @pytest.fixture(scope="module")
 def member(exo):
     host = exo.Engine(-1, 1)
-    cfg = exo.default_config(truncation=21, levels=10)
+    cfg = exo.default_config(truncation=21, levels=10, star_temperature_k=0.0, vegetation=0, glaciers=0)
     cfg.storm_diagnostics = 1
     m = host.add_member(cfg)
     yield host, m

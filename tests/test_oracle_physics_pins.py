@@ -69,7 +69,11 @@ STEFAN_BOLTZMANN = 5.670374419e-8  # W m^-2 K^-4, CODATA 2018
 @pytest.fixture(scope="module")
 def column(exo):
     host = exo.Engine(-1, 1)
-    m = host.add_member(exo.default_config(truncation=21, levels=10))
+    # grey radiation and no vegetation or glacier model: each pin below
+    # isolates one parameterization (tests/test_oracle_surface_pins.py pins
+    # the star spectrum, vegetation and glaciers)
+    m = host.add_member(exo.default_config(truncation=21, levels=10, star_temperature_k=0.0, vegetation=0,
+                                           glaciers=0))
     _, nlat, nlon, nlev = host.dims(m)
     yield host, m, nlat, nlon, nlev
     host.close()

@@ -392,8 +392,7 @@ void enqueue_step(Engine& e, const std::vector<int>& active);
 // shared tables included when no member holds them yet; drop_member frees
 // it after the queued work.
 std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision,
-                                          const double* land_fraction = nullptr,
-                                          const double* geopotential = nullptr);
+                                          const exo_surface_field* fields = nullptr, int count = 0);
 // Sets the calling thread's exo_last_error() text; returns `code`.
 int set_error(int code, const std::string& message);
 size_t member_bytes(const Engine& e, const MemberDev& md);

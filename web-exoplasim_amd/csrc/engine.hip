@@ -186,12 +186,12 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
 
 namespace exo {
 
-std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision, const double* land_fraction,
-                                          const double* geopotential) {
+std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision, const exo_surface_field* fields,
+                                          int count) {
   if (precision != EXO_PRECISION_FP32 && precision != EXO_PRECISION_SPLIT)
     throw std::runtime_error("unknown precision");
   auto md = std::make_unique<MemberDev>();
-  build_member(cfg, md->host, land_fraction, geopotential);
+  build_member(cfg, md->host, fields, count);
   md->precision = precision;
   md->step = md->host.start_step;
   return md;
@@ -421,15 +421,24 @@ void exo_destroy(exo_engine* h) {
 }
 
 int exo_add_member(exo_engine* h, const exo_config* cfg, int precision, int* member) {
-  return exo_add_member_boundary(h, cfg, precision, nullptr, nullptr, member);
+  return exo_add_member_surface(h, cfg, precision, nullptr, 0, member);
 }
 
 int exo_add_member_boundary(exo_engine* h, const exo_config* cfg, int precision, const double* land_fraction,
                             const double* geopotential, int* member) {
-  if (!h || !cfg || !member || (!land_fraction) != (!geopotential)) return fail(EXO_E_ARG, "null argument");
+  if (!land_fraction || !geopotential) return fail(EXO_E_ARG, "null argument");
+  const exo_surface_field fields[2] = {{172, 1, land_fraction}, {129, 1, geopotential}};
+  return exo_add_member_surface(h, cfg, precision, fields, 2, member);
+}
+
+int exo_add_member_surface(exo_engine* h, const exo_config* cfg, int precision, const exo_surface_field* fields,
+                           int count, int* member) {
+  if (!h || !cfg || !member || count < 0 || (count > 0 && !fields)) return fail(EXO_E_ARG, "null argument");
+  for (int i = 0; i < count; i++)
+    if (!fields[i].values) return fail(EXO_E_ARG, "null argument");
   std::unique_ptr<MemberDev> md;
   try {
-    md = exo::prepare_member(*cfg, precision, land_fraction, geopotential);
+    md = exo::prepare_member(*cfg, precision, fields, count);
   } catch (const std::exception& ex) {
     return fail(EXO_E_ARG, ex.what());
   }

@@ -66,7 +66,8 @@ enum ExoTable {
   X(albgmin2) X(drhsfull) X(nwetsoil) X(wsmax) X(albland) X(alblandmax)        \
   X(dgroundalb1) X(dgroundalb2) X(doceanalb1) X(doceanalb2)                    \
   /* vegetation and glacier */                                                 \
-  X(nveg) X(nglacier) X(glacelim)                                              \
+  X(nveg) X(nglacier) X(glacelim) X(rlue) X(co2conv) X(tau_veg) X(tau_soil)   \
+  X(ncveg) X(cveg_e) X(cveg_g)                                                 \
   /* icemod */                                                                 \
   X(xdt) X(xmind) X(xmaxd) X(cicemin) X(thicec) X(tfreeze) X(crhoi)            \
   X(crhos) X(crhosn) X(cpi) X(ckapi) X(ckapsn) X(clfi) X(clfsn)                \

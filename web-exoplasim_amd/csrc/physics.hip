@@ -1583,10 +1583,93 @@ __device__ __forceinline__ float glacier_albedo(const DevMember& M, float mx, fl
   return fmaxf(mn, fminf(mx, mx - (mx - mn) * (ts - 263.16f) / (K(tmelt) - 263.16f)));
 }
 
+// simba's vegstep for one land column below 90 % glacier (physics/
+// vegetation/wgsl/vegstep.wgsl:31-99): light- and water-limited primary
+// production, vegetation and soil carbon, forest and soil cover; with
+// nveg = 2 also roughness, soil water capacity, surface humidity factor,
+// albedo and forest cover.
+template <int NL>
+__device__ __forceinline__ void vegstep(const DevMember& M, int h) {
+  if (!(F(dls)[h] > 0.0f && F(dglac)[h] < 0.90f)) return;
+  const int NH = M.nhor;
+  // simba's fixed parameters (setup/index.ts:263-282)
+  constexpr float VWS_CRIT = 0.25f, Q10 = 2.0f, ZLAIMAX = 9.0f, ZLAIMIN = 0.1f, VALB_MIN = 0.12f;
+  constexpr float VALB_MAX = 0.3f, VSALB_MIN = 0.2f, VWMAX_MAX = 0.5f, VWMAX_MIN = 0.05f, VZ0_MIN = 0.05f;
+  constexpr float CVEG_A = 2.9f, CVEG_C = 1.570796326794897f, CVEG_F = 3.0f, CVEG_K = 0.5f, CVEG_L = 0.25f;
+  constexpr float CT_CRIT = 5.0f, CO2_REF = 360.0f, CO2_COMP = 0.0f, CO2_SENS = 0.3f;
+  const float co2 = F(co2)[0];
+  float zbeta = 0.0f;
+  if (!(co2 < 1.0e-8f)) zbeta = fmaxf(0.0f, 1.0f + CO2_SENS * logf((co2 - CO2_COMP) / (CO2_REF - CO2_COMP)));
+  const float ts = LV(dt, NL, h);
+  const float dp = F(dp)[h];
+  const float dwatc = F(dwatc)[h];
+  const float dsnow = F(dsnow)[h];
+  float zwmax = F(dwmax)[h];
+  const float zglacfree = 1.0f - F(dglac)[h];
+  const float zsvp = K(ra1) * expf(K(ra2) * (ts - K(tmelt)) / (ts - K(ra4)));
+  const float zvpd = fmaxf(0.01f, zsvp - dp * LV(dq, NL - 1, h) / 0.622f);
+  float dcveg = F(dcveg)[h];
+  float dcsoil = F(dcsoil)[h];
+  const float zlaim = ZLAIMIN + 0.63661977236758f * ZLAIMAX * atanf(CVEG_L * 2.0f * F(dagg)[h] * dcveg);
+  const float zvegm = 1.0f - expf(-CVEG_K * zlaim);
+  float zveg = 0.0f;
+  if (zwmax > 0.0f) zveg = fminf(fminf(1.0f, zvegm), fmaxf(0.0f, dwatc / (zwmax * VWS_CRIT)));
+  const float zft = fminf(1.0f, fmaxf(0.0f, (ts - K(tmelt)) / CT_CRIT));
+  const float zgppl = K(rlue) * zbeta * zft * zveg * LV(dfd, NL, h);
+  float zgppw = 0.0f;
+  if (!(co2 < 1.0e-8f)) zgppw = K(co2conv) * dp * zveg * (-F(devap)[h]) / zvpd * (0.3f * co2);
+  zgppw = fminf(1.0f, fmaxf(0.0f, zgppw));
+  const float zgpp = fminf(zgppl, zgppw);
+  const float znpp = F(dgrow)[h] * 0.5f * zgpp * zglacfree;
+  F(dnpp)[h] = znpp;
+  const float zlitter = dcveg / K(tau_veg);
+  const float zres = powf(Q10, (ts - K(tmelt) - 10.0f) / 10.0f) * dcsoil / K(tau_soil);
+  dcveg = dcveg + (znpp - zlitter) * K(deltsec) * K(ncveg);
+  dcsoil = dcsoil + (zlitter - zres) * K(deltsec) * K(ncveg);
+  const float zforest = fminf(1.0f, fmaxf(0.0f, (atanf(dcveg - CVEG_A) - K(cveg_e)) / (CVEG_C - K(cveg_e))));
+  const float dvsoil = fminf(1.0f, fmaxf(0.0f, (atanf(dcveg - CVEG_F) - K(cveg_g)) / (CVEG_C - K(cveg_g))));
+  F(dcveg)[h] = dcveg;
+  F(dcsoil)[h] = dcsoil;
+  F(dvsoil)[h] = dvsoil;
+  F(dlai)[h] = -logf(1.0f - zveg) / CVEG_K;
+  const float zvz0 = F(dmr)[h] * zforest + VZ0_MIN * (1.0f - zforest);
+  float zvalb = VALB_MIN * zveg + VALB_MAX * (1.0f - zveg);
+  zwmax = VWMAX_MAX * dvsoil + VWMAX_MIN * (1.0f - dvsoil);
+  float zvrhs = F(dsc)[h] * fminf(1.0f, fmaxf(0.0f, dwatc / (zwmax * VWS_CRIT)));
+  if (dsnow > 0.0f) {
+    float zalbsn = (K(albsmax) - K(albsmin)) * (ts - 263.16f) / (K(tmelt) - 263.16f);
+    zalbsn = fmaxf(K(albsmin), fminf(K(albsmax), K(albsmax) - zalbsn));
+    zvalb = zvalb + (zalbsn - zvalb) * dsnow / (dsnow + 0.01f);
+    zvalb = VSALB_MIN * zforest + zvalb * (1.0f - zforest);
+    zvrhs = 1.0f;
+  }
+  if (K(nveg) == 2.0f) {
+    const float z0clim = F(dz0climo)[h];
+    F(dz0)[h] = sqrtf(zvz0 * zvz0 + z0clim * z0clim);
+    F(dwmax)[h] = zwmax;
+    F(drhs)[h] = zvrhs;
+    F(dalb)[h] = zvalb;
+    F(dforest)[h] = zforest;
+  }
+}
+
+// glaciermod's glacierstep with glacierstop's promotion at each calendar
+// year's end (physics/glacier/wgsl/glacierstep.wgsl:1-25): land that kept
+// glacelim of snow all year becomes glacier.
+__device__ __forceinline__ void glacierstep(const DevMember& M, int h) {
+  if (K(nglacier) == 0.0f) return;
+  const bool land = F(dls)[h] > 0.5f;
+  if (land && F(dsnowz)[h] < K(glacelim)) F(dpersist)[h] = 0.0f;
+  if (land && F(dsnowz)[h] < 0.1f) F(dglac)[h] = 0.0f;
+  if (F(surface_step)[6] > 0.5f) {
+    if (F(dpersist)[h] > 0.5f) F(dglac)[h] = 1.0f;
+    F(dpersist)[h] = F(dls)[h] >= 0.5f ? 1.0f : 0.0f;
+  }
+}
+
 template <int NL>
 __device__ __forceinline__ void land_surface(const DevMember& M, int h) {
-  // physics/land/wgsl/surface.wgsl:67-142 (vegstep and glacierstep are
-  // inert here: nveg = 0, nglacier = 0)
+  // physics/land/wgsl/surface.wgsl:67-142
   const int NH = M.nhor;
   const float dls = F(dls)[h];
   if (dls > 0) F(driver)[h] = F(driver)[h] + F(drunoff)[h] * K(deltsec);
@@ -1643,12 +1726,14 @@ __device__ __forceinline__ void land_surface(const DevMember& M, int h) {
     LV(dt, NL, h) = dts;
     LV(dq, NL, h) = dqs;
   }
+  if (K(nveg) > 0.0f) vegstep<NL>(M, h);
   if (F(dglac)[h] > 0.5f && dls > 0) {
     F(dalb)[h] = glacier_albedo(M, K(albgmax), K(albgmin), dts);
     F(dsalb)[h] = glacier_albedo(M, K(albgmax1), K(albgmin1), dts);
     F(dsalb)[NH + h] = glacier_albedo(M, K(albgmax2), K(albgmin2), dts);
     F(drhs)[h] = 1.0f;
   }
+  glacierstep(M, h);
 }
 
 // --------------------------------------------------- sea ice and ocean

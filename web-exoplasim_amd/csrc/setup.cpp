@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <stdexcept>
 
@@ -130,11 +132,21 @@ Pumamod pumamod_read(const exo_config& c) {
   p.gsol0 = c.flux_wm2;
   p.eccen = c.eccentricity;
   p.obliq = c.obliquity_deg;
-  p.nfixorb = 1;
-  p.ngenkeplerian = 0;
+  p.nfixorb = c.fixed_orbit ? 1 : 0;
+  p.ngenkeplerian = c.keplerian ? 1 : 0;
   p.gascon = c.gas_constant;
   p.sidereal_year = c.year_days * 86400.0;
   p.mvelp = 102.7;
+  // the orbital-angle edits (namelist/index.ts:57-75)
+  for (double a : {c.periapse_longitude_deg, c.initial_mean_anomaly_deg})
+    if (!std::isnan(a) && (!std::isfinite(a) || a < -360 || a > 360))
+      throw std::runtime_error("Orbital angles must be finite and between -360 and 360 degrees");
+  if (!std::isnan(c.periapse_longitude_deg)) {
+    if (c.periapse_longitude_deg < 0) throw std::runtime_error("Periapse longitude must be between 0 and 360 degrees");
+    p.mvelp = c.periapse_longitude_deg;
+  }
+  if (!std::isnan(c.initial_mean_anomaly_deg)) p.meananom0 = c.initial_mean_anomaly_deg;
+  if (!c.fixed_orbit) throw std::runtime_error("fixedOrbit false (Berger's orbital series) is not built");
   int ndpy = (int)std::lround(c.year_days);
   if (c.rotation_period_days != 1) {
     p.rotspd = 1.0 / c.rotation_period_days;
@@ -165,16 +177,18 @@ Pumamod pumamod_read(const exo_config& c) {
   }
   p.kick = 1;
   p.nqspec = 1;
-  p.nveg = 0;
+  if (c.vegetation < 0 || c.vegetation > 2) throw std::runtime_error("vegetation must be 0, 1 or 2");
+  p.nveg = c.vegetation;
   p.neqsig = 0;
-  p.nfilter = 2;      // physicsFilter "exp"
-  p.ngptfilter = 1;   // "gp"
-  p.nspvfilter = 1;   // "sp"
+  if (c.filter < 0 || c.filter > 3) throw std::runtime_error("physicsFilter: NFILTER must be 0 to 3");
+  p.nfilter = c.filter;  // the last of none / cesaro / exp / lh in physicsFilter
+  p.ngptfilter = 1;      // the base namelist's, which "gp" only repeats
+  p.nspvfilter = 1;      // likewise "sp"
   p.filterkappa = 8;
   p.nfilterexp = 8;
   p.landhoskn0 = 15;
   p.psurf = c.pressure_bar * 1e5;
-  p.fixedlon = 180;
+  p.fixedlon = c.substellar_longitude_deg;
   p.n_days_per_year = ndpy;
   p.n_start_year = ndpy == 365 ? 1 : 0;
   p.n_start_month = 1;
@@ -777,6 +791,110 @@ std::vector<double> monthly(const std::vector<double>& table, int64_t step, cons
 
 }  // namespace
 
+// ------------------------------------------------------ star spectrum
+
+namespace {
+
+#include "specblock.inc"
+
+// solarini's split of a blackbody star (SPECTRUM.split, physics/radiation/
+// spectrum/index.ts:58-168): the energy shares of the bands below and above
+// 0.75 um, the Rayleigh coefficient relative to the Sun's, and each
+// surface's spectral albedo averaged over the star in each band. The sums
+// run in the reference's order: trapezoids over 1024 log-spaced points per
+// band, the gap between the bands added to the first.
+struct Spectrum {
+  double zsolar1 = 0, zsolar2 = 0, rcoeff = 1;
+  double dsnowalb[2], dsnowalbmn[2], dsnowalbmx[2], dicealbmn[2], dicealbmx[2], dglacalbmn[2];
+  double dgroundalb[2], doceanalb[2];
+};
+
+Spectrum spectrum_split(double star_k, double min_wavelength_m, bool simple_albedo) {
+  const double hc_over_k = 0.0143877735383, sun_k = 5772, default_zsolar1 = 0.517;
+  const int n = 1024, band_two = 41;
+  const double cg = hc_over_k / sun_k, cs = hc_over_k / star_k;
+  const double lw0 = std::log10(min_wavelength_m), hinge = std::log10(7.5e-7);
+  const double dl1 = (hinge - lw0) / n, dl2 = (-4 - hinge) / n;
+  const auto pow5 = [](double x) {
+    const double x2 = x * x;
+    return x2 * x2 * x;
+  };
+  const auto pow4 = [](double x) {
+    const double x2 = x * x;
+    return x2 * x2;
+  };
+  // Planck's law up to a constant: 1 / scaled / (exp(c / lambda) - 1)
+  const auto planck = [](double lambda, double scaled, double c) { return ((1 / scaled) * 1) / (std::exp(c / lambda) - 1); };
+  std::vector<double> w1(n), w2(n), b1(n), b2(n), g1(n), g2(n);
+  for (int k = 0; k < n; k++) {
+    w1[k] = std::pow(10.0, lw0 + k * dl1);
+    w2[k] = std::pow(10.0, hinge + k * dl2);
+    const double s1 = pow5(1e6 * w1[k]), s2 = pow5(1e6 * w2[k]);
+    g1[k] = planck(w1[k], s1, cg);
+    g2[k] = planck(w2[k], s2, cg);
+    b1[k] = planck(w1[k], s1, cs);
+    b2[k] = planck(w2[k], s2, cs);
+  }
+  const auto trap = [](const std::vector<double>& v, const std::vector<double>& at, int k) {
+    return 0.5 * (v[k] + v[k + 1]) * (at[k + 1] - at[k]);
+  };
+  // the lambda^-4 (Rayleigh cross-section) weighted trapezoid
+  const auto cross = [&](const std::vector<double>& v, const std::vector<double>& at, int k) {
+    return 0.5 * (v[k] / pow4(at[k] * 1e6) + v[k + 1] / pow4(at[k + 1] * 1e6)) * (at[k + 1] - at[k]);
+  };
+  double z1 = 0, z2 = 0, zg1 = 0, zg2 = 0, zc1 = 0, zc2 = 0, zgc1 = 0, zgc2 = 0;
+  for (int k = 0; k < n - 1; k++) {
+    z1 = z1 + trap(b1, w1, k);
+    z2 = z2 + trap(b2, w2, k);
+    zg1 = zg1 + trap(g1, w1, k);
+    zg2 = zg2 + trap(g2, w2, k);
+    zc1 = zc1 + cross(b1, w1, k);
+    zc2 = zc2 + cross(b2, w2, k);
+    zgc1 = zgc1 + cross(g1, w1, k);
+    zgc2 = zgc2 + cross(g2, w2, k);
+  }
+  const int last = n - 1;
+  const double gap = w2[0] - w1[last];
+  z1 = z1 + 0.5 * (b1[last] + b2[0]) * gap;
+  zc1 = zc1 + 0.5 * (b1[last] / pow4(w1[last] * 1e6) + b2[0] / pow4(w2[0] * 1e6)) * gap;
+  zg1 = zg1 + 0.5 * (g1[last] + g2[0]) * gap;
+  zgc1 = zgc1 + 0.5 * (g1[last] / pow4(w1[last] * 1e6) + g2[0] / pow4(w2[0] * 1e6)) * gap;
+  Spectrum out;
+  const double zchi = (zgc1 + zgc2) / (zg1 + zg2);
+  out.rcoeff = ((zc1 + zc2) * default_zsolar1) / z1 / zchi;
+  const double zdenom1 = 0.01 / z1, zdenom2 = 0.01 / z2;
+  out.zsolar1 = z1 / (z1 + z2);
+  out.zsolar2 = 1 - out.zsolar1;
+  std::vector<double> b3(kSpecPoints);
+  for (int k = 0; k < kSpecPoints; k++)
+    b3[k] = planck(kSpec_wavelengths[k], pow5(kSpec_wavelengths[k]), 1e6 * cs);
+  const auto band = [&](const double* albedo, double* dst) {
+    double a1 = 0, a2 = 0;
+    for (int k = 0; k < band_two; k++)
+      a1 = a1 + 0.5 * (b3[k] * albedo[k] + b3[k + 1] * albedo[k + 1]) * 1e-6 *
+                    (kSpec_wavelengths[k + 1] - kSpec_wavelengths[k]);
+    for (int k = band_two; k < kSpecPoints - 1; k++)
+      a2 = a2 + 0.5 * (b3[k] * albedo[k] + b3[k + 1] * albedo[k + 1]) * 1e-6 *
+                    (kSpec_wavelengths[k + 1] - kSpec_wavelengths[k]);
+    a1 = zdenom1 * a1;
+    a2 = zdenom2 * a2;
+    const double broadband = out.zsolar1 * a1 + out.zsolar2 * a2;
+    dst[0] = simple_albedo ? broadband : a1;
+    dst[1] = simple_albedo ? broadband : a2;
+  };
+  band(kSpec_iceblend, out.dsnowalb);
+  band(kSpec_iceblendmin, out.dsnowalbmn);
+  band(kSpec_iceblendmax, out.dsnowalbmx);
+  band(kSpec_seaicemin, out.dicealbmn);
+  band(kSpec_seaicemax, out.dicealbmx);
+  band(kSpec_glacalbmin, out.dglacalbmn);
+  band(kSpec_groundblend, out.dgroundalb);
+  band(kSpec_oceanblend, out.doceanalb);
+  return out;
+}
+
+}  // namespace
+
 void config_default(exo_config* c) {
   std::memset(c, 0, sizeof(*c));
   c->truncation = 21;
@@ -802,6 +920,27 @@ void config_default(exo_config* c) {
   c->sea_surface_k = 280;
   c->terrain_seed = 20240521;
   c->noise_seed = 14963991;
+  // GCM.DEFAULT_CONFIG (index.ts:4-44) for the rest
+  c->vegetation = 2;
+  c->initial_growth = 0.5;
+  c->glaciers = 1;
+  c->glacier_min_depth_m = 2;
+  c->glacier_initial_height_m = -1;
+  c->star_temperature_k = 5780;
+  c->synchronous = 0;
+  c->substellar_longitude_deg = 180;
+  c->fixed_orbit = 1;
+  c->keplerian = 0;
+  c->periapse_longitude_deg = NAN;
+  c->initial_mean_anomaly_deg = NAN;
+  c->soil_depth_scale = 1;
+  c->soil_water_capacity_m = NAN;
+  c->soil_heat_capacity = NAN;
+  c->land_albedo = NAN;
+  c->ocean_albedo = NAN;
+  c->snow_ice_albedo = NAN;
+  c->wet_soil = 0;
+  c->filter = 2;
 }
 
 std::vector<double> gaussian_degrees(int nlat) { return gaussian_grid(nlat).degrees; }
@@ -815,7 +954,7 @@ void member_date(const Member& m, int64_t step, int out[5]) {
   out[4] = d.minute;
 }
 
-void build_member(const exo_config& cfg, Member& m, const double* land_fraction, const double* geopotential) {
+void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fields, int count) {
   if (cfg.levels != 5 && cfg.levels != 10) throw std::runtime_error("Choose 5 or 10 atmospheric layers");
   if (!(cfg.timestep_minutes > 0)) throw std::runtime_error("timestep must be positive");
   if (!(cfg.land_fraction >= 0 && cfg.land_fraction <= 1)) throw std::runtime_error("land fraction must be in [0, 1]");
@@ -877,19 +1016,49 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
   const std::vector<double>& sigmah = dyn.sigmah;
   const std::vector<double>& dsigma = dyn.dsigma;
 
-  // Module set-ups (setup/modules/index.ts) with the page switches this
-  // build fixes: nstartemp 0 (grey band split), no vegetation, no glacier
-  // growth, no storm diagnostics.
+  // Module set-ups (MODULE_SETUP, setup/modules/index.ts:13-305): each
+  // module's namelist defaults, the configuration's edits over them
+  // (namelist/index.ts:40-182) and what its *ini routine derives.
   struct {
     double gsol0, zsolar1 = 0.517, zsolar2 = 0.483, zsolars1 = 0, zsolars2 = 0, rcoeff = 1;
     double th2oc = 0.024, tswr1 = 0.077, tswr2 = 0.065, tswr3 = 0.0055, tpofmt = 1, acllwr = 0.1;
     double a0o3 = 0.25, a1o3 = 0.11, aco3 = 0.08, bo3 = 20000, co3 = 5000, toffo3 = 0.25;
-    double dawn = 0, co2 = 360;
-    int nfixed = 0, nstartemp = 0, no3 = 1;
+    double dawn = 0, co2 = 360, starbbtemp = 5772, minwavel = 316.036116751;
+    int nfixed = 0, nstartemp = 0, no3 = 1, nsimplealbedo = 1;
   } rad;
   rad.gsol0 = p.gsol0;
   rad.co2 = cfg.co2_ppm;
   rad.no3 = cfg.ozone ? 1 : 0;
+  rad.nfixed = cfg.synchronous ? 1 : 0;
+  if (cfg.star_temperature_k > 0) {
+    rad.nstartemp = 1;
+    rad.starbbtemp = cfg.star_temperature_k;
+  }
+  // radini's band albedos: the star's spectral averages (solarini), else
+  // the grey defaults (setup/modules/index.ts:66-83)
+  double dsnowalb[2] = {0.6, 0.6}, dsnowalbmn[2] = {0.4, 0.4}, dsnowalbmx[2] = {0.8, 0.8};
+  double dicealbmn[2] = {0.5, 0.5}, dicealbmx[2] = {0.7, 0.7};
+  double dglacalbmn[2] = {0.6, 0.6}, dgroundalb[2] = {0.2, 0.2};
+  double doceanalb[2] = {0.069, 0.069};
+  if (rad.nstartemp > 0) {
+    const Spectrum sp = spectrum_split(rad.starbbtemp, rad.minwavel * 1e-9, rad.nsimplealbedo > 0.5);
+    rad.zsolar1 = sp.zsolar1;
+    rad.zsolar2 = sp.zsolar2;
+    rad.zsolars1 = sp.zsolar1;
+    rad.zsolars2 = sp.zsolar2;
+    rad.rcoeff = sp.rcoeff;
+    for (int b = 0; b < 2; b++) {
+      dsnowalb[b] = sp.dsnowalb[b];
+      dsnowalbmn[b] = sp.dsnowalbmn[b];
+      dsnowalbmx[b] = sp.dsnowalbmx[b];
+      dicealbmn[b] = sp.dicealbmn[b];
+      dicealbmx[b] = sp.dicealbmx[b];
+      dglacalbmn[b] = sp.dglacalbmn[b];
+      dgroundalb[b] = sp.dgroundalb[b];
+      doceanalb[b] = sp.doceanalb[b];
+    }
+  }
+  (void)dsnowalb;
   struct {
     double rhbeta = 0, nbeta = 3, pdeep = 999999, pdeepth = 70000, rkshallow = 10, gamma;
     double clwcrit1 = -0.1, clwcrit2 = 0, rcritmod = 1, rcritslope = 0, clwfac;
@@ -898,23 +1067,50 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
   rain.clwfac = rain.clwcrit1 > rain.clwcrit2 ? 1 / (rain.clwcrit1 - rain.clwcrit2)
                 : rain.clwcrit1 < rain.clwcrit2 ? -1 : 1;
   const double zumin = 1, vdiff_lamm = 160, vdiff_b = 5, vdiff_c = 5, vdiff_d = 5;
-  // band albedos without the stellar spectrum (setup/modules/index.ts:74-83)
-  const double dsnowalbmn[2] = {0.4, 0.4}, dsnowalbmx[2] = {0.8, 0.8};
-  const double dicealbmn[2] = {0.5, 0.5}, dicealbmx[2] = {0.7, 0.7};
-  const double dglacalbmn[2] = {0.6, 0.6}, dgroundalb[2] = {0.2, 0.2};
-  const double doceanalb[2] = {0.069, 0.069};
   struct {
     double nwetsoil = 0, albland = 0.22, albsmin = 0.4, albsmax = 0.8, albsminf = 0.3, albsmaxf = 0.4;
     double albgmin = 0.6, albgmax = 0.8, alblandmax = 0.4, drhsfull = 0.4, dztop = 0.2, dsmax = 5;
     double wsmax = 0.5, rhosnow = 330, soildiff = 1.8, sicediff = 2.03, snowdiff = 0.31;
     double soilcap = 2.4e6, sicecap = 2.07e6, snowcap = 0.6897e6, rinifor = 0.5, dz0land = 2;
     double drhsland = 0.25, dzglac = -1, nwatcini = 0, dwatcini = 0;
+    double rlue = 3.4e-10, co2conv = 8.3e-4, tau_veg = 10, tau_soil = 42;
     double dsoilz[5] = {0.4, 0.8, 1.6, 3.2, 6.4};
   } land;
+  // landmod_namelist edits (namelist/index.ts:114-163)
+  for (double v : {cfg.soil_depth_scale, cfg.soil_water_capacity_m, cfg.soil_heat_capacity})
+    if (!std::isnan(v) && (!std::isfinite(v) || v <= 0))
+      throw std::runtime_error("Soil depth, water capacity and heat capacity must be positive");
+  if (std::isnan(cfg.soil_depth_scale)) throw std::runtime_error("Soil depth, water capacity and heat capacity must be positive");
+  for (double v : {cfg.land_albedo, cfg.ocean_albedo, cfg.snow_ice_albedo})
+    if (!std::isnan(v) && (!std::isfinite(v) || v < 0 || v > 1))
+      throw std::runtime_error("Albedo overrides must be between 0 and 1");
+  land.nwetsoil = cfg.wet_soil ? 1 : 0;
+  for (int i = 0; i < 5; i++) land.dsoilz[i] *= cfg.soil_depth_scale;
+  if (!std::isnan(cfg.soil_water_capacity_m)) land.wsmax = cfg.soil_water_capacity_m;
+  if (!std::isnan(cfg.soil_heat_capacity)) land.soilcap = cfg.soil_heat_capacity;
   struct {
     double albsea = 0.069, albice = 0.7, dz0sea = 1.5e-5, dz0ice = 0.001, drhssea = 1, drhsice = 1;
     double charnock = 0.018;
   } sea;
+  if (!std::isnan(cfg.land_albedo)) {
+    land.albland = cfg.land_albedo;
+    dgroundalb[0] = dgroundalb[1] = cfg.land_albedo;
+  }
+  if (!std::isnan(cfg.ocean_albedo)) {
+    sea.albsea = cfg.ocean_albedo;
+    doceanalb[0] = doceanalb[1] = cfg.ocean_albedo;
+  }
+  if (!std::isnan(cfg.snow_ice_albedo)) {
+    const double a = cfg.snow_ice_albedo;
+    sea.albice = a;
+    for (int b = 0; b < 2; b++)
+      dicealbmn[b] = dicealbmx[b] = dsnowalbmn[b] = dglacalbmn[b] = dsnowalb[b] = a;
+  }
+  // landini: snow and glacier band albedos from the band maxima and minima,
+  // vegetation time scales from years to seconds (setup/modules/index.ts:
+  // 176-195)
+  land.tau_veg = land.tau_veg * p.m_days_per_year * p.day_24hr;
+  land.tau_soil = land.tau_soil * p.m_days_per_year * p.day_24hr;
   struct {
     double nice = 1, nseaice = 1, xmind = 0.1, xmaxd = 9, thicec = 0.5, tfreeze = 271.25;
     double cicemin = 0.5;
@@ -930,7 +1126,15 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
   const double timeweight = 1 / (carbon.frequency * p.m_days_per_year);
   const double mldepth = cfg.ocean_depth_m;
   const int nspinit = 1, noromax = T;
-  const double glacelim = 2;
+  // vegmod_namelist (setup/modules/index.ts:234-247; RINIDAGG edited)
+  struct {
+    double ncveg = 1, forgrow = 1, rinidagg = 0.5, rinidsc = 1, rinidmr = 2, rinisoil = 0, riniveg = 0;
+  } veg;
+  veg.rinidagg = cfg.initial_growth;
+  // glacier_namelist (setup/modules/index.ts:293-298 defaults; edits)
+  const int nglacier = cfg.glaciers ? 1 : 0;
+  const double glacelim = cfg.glacier_min_depth_m;
+  const double icesheeth = cfg.glacier_initial_height_m;
 
   // ------------------------------------------------- physics tables
   {
@@ -1053,13 +1257,57 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
       }
     spectral_to_grid(m, spec.data(), height.data());
   }
-  std::vector<double> dls(NHOR, 0.0), orography(NHOR, 0.0);
-  if (land_fraction && geopotential) {
-    // boundary files 172 and 129 (setup/surface/index.ts:293-297)
-    for (int h = 0; h < NHOR; h++) {
-      dls[h] = land_fraction[h] > 0.5 ? 1 : 0;
-      orography[h] = geopotential[h];
+  // The .sra fields given, by code, read as get_surf_array reads them
+  // (setup/surface/index.ts:76-107): the first `months` records; 12 of 14
+  // gain December before and January after, one fills all 14.
+  std::map<int, const exo_surface_field*> given;
+  for (int i = 0; i < count; i++) {
+    if (fields[i].months < 1) throw std::runtime_error("a surface field needs at least one month");
+    if (!given.emplace(fields[i].code, &fields[i]).second)
+      throw std::runtime_error("surface field " + std::to_string(fields[i].code) + " given twice");
+  }
+  const auto read = [&](int code, int months, std::vector<double>& out) {
+    const auto it = given.find(code);
+    if (it == given.end()) return false;
+    const int found = std::min(months, it->second->months);
+    out.assign((size_t)months * NHOR, 0.0);
+    std::copy(it->second->values, it->second->values + (size_t)found * NHOR, out.begin());
+    if (found == months) return true;
+    auto at = [&](int month) { return out.begin() + (size_t)month * NHOR; };
+    if (found == 12 && months == CLIM_MONTHS) {
+      for (int mo = 12; mo >= 1; mo--) std::copy(at(mo - 1), at(mo), at(mo));
+      std::copy(at(1), at(2), at(13));
+      std::copy(at(12), at(13), at(0));
+    } else if (found == 1 && months == CLIM_MONTHS) {
+      for (int mo = 1; mo < CLIM_MONTHS; mo++) std::copy(at(0), at(1), at(mo));
+    } else {
+      char name[32];
+      std::snprintf(name, sizeof name, "N%03d_surf_%04d.sra", NLAT, code);
+      throw std::runtime_error(std::string(name) + " has " + std::to_string(found) + " fields, expected " +
+                               std::to_string(months));
     }
+    return true;
+  };
+  const auto field = [&](int code, double fallback) {
+    std::vector<double> v;
+    if (!read(code, 1, v)) v.assign(NHOR, fallback);
+    return v;
+  };
+  const auto climatology = [&](int code, double fallback) {
+    std::vector<double> v;
+    if (!read(code, CLIM_MONTHS, v)) v.assign((size_t)CLIM_MONTHS * NHOR, fallback);
+    return v;
+  };
+
+  // landini's land mask (172) and orography (129): from the fields, else
+  // the seeded terrain
+  std::vector<double> dls(NHOR, 0.0), orography(NHOR, 0.0);
+  const bool seeded = !given.count(172) && !given.count(129);
+  if (!seeded) {
+    // boundary files 172 and 129 (setup/surface/index.ts:293-297)
+    const std::vector<double> mask = field(172, 1);
+    orography = field(129, 0);
+    for (int h = 0; h < NHOR; h++) dls[h] = mask[h] > 0.5 ? 1 : 0;
   } else {
     std::vector<double> sorted = height;
     std::sort(sorted.begin(), sorted.end());
@@ -1075,15 +1323,23 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
     }
   }
   m.land_mask = dls;
+  // the land mask as icemod and oceanmod read it (fallback 0, not 1)
+  std::vector<double> xls(NHOR);
+  {
+    const std::vector<double> mask = seeded ? dls : field(172, 0);
+    for (int h = 0; h < NHOR; h++) xls[h] = mask[h] > 0.5 ? 1 : 0;
+  }
 
   // ------------------------------------------------- surface initial state
-  // SURFACE_SETUP.initial (setup/surface/index.ts:267-643) for a planet
-  // whose only boundary files are the land mask (172) and orography (129);
-  // every other code takes its fallback, except the sea-surface
-  // temperature climatology (169), which is a flat sea_surface_k.
+  // SURFACE_SETUP.initial (setup/surface/index.ts:267-643): landini,
+  // glacierini and oroini, vegini, iceini and oceanini, seaini; every code
+  // not given takes the reference's fallback. A missing surface
+  // temperature climatology (169) is the configuration's uniform
+  // sea_surface_k when that is a number.
   std::vector<double> A(m.layout.total, 0.0);
   auto slot = [&](ExoSlot s) { return &A[m.layout.offset[s]]; };
   const double tmelt = p.tmelt;
+  std::vector<double> doro = orography;
   {
     double* a_dls = slot(SLOT_dls);
     for (int h = 0; h < NHOR; h++) a_dls[h] = dls[h];
@@ -1096,14 +1352,20 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
     double* dt = slot(SLOT_dt);
     double* dq = slot(SLOT_dq);
     const size_t surf = (size_t)L * NHOR;
-    std::vector<double> dwmax(NHOR, std::max(land.wsmax, 0.0));
-    std::vector<double> dz0clim(NHOR, land.dz0land);
-    std::vector<double> dglac(NHOR, 0.0);
-    std::vector<double> dforest(NHOR, land.rinifor);
-    std::vector<double> dtclsoil(NHOR, tmelt);
-    std::vector<double> dalbcl((size_t)CLIM_MONTHS * NHOR, land.albland);
-    std::vector<double> dalbcl1((size_t)CLIM_MONTHS * NHOR, dgroundalb[0]);
-    std::vector<double> dalbcl2((size_t)CLIM_MONTHS * NHOR, dgroundalb[1]);
+    const std::vector<double> dz0clim = field(173, land.dz0land);
+    const std::vector<double> dz0climo = field(1730, 0);
+    std::vector<double> dglac = field(232, 0);
+    if (land.dzglac > 0)
+      for (int h = 0; h < NHOR; h++)
+        if (orography[h] > land.dzglac * p.ga) dglac[h] = 1;
+    for (auto& v : dglac) v = v > 0.5 ? 1 : 0;
+    std::vector<double> dforest = field(212, land.rinifor);
+    std::vector<double> dwmax = field(229, land.wsmax);
+    for (auto& v : dwmax) v = std::max(v, 0.0);
+    const std::vector<double> dtclsoil = field(209, tmelt);
+    const std::vector<double> dalbcl = climatology(174, land.albland);
+    const std::vector<double> dalbcl1 = climatology(175, dgroundalb[0]);
+    const std::vector<double> dalbcl2 = climatology(176, dgroundalb[1]);
     double* land_dts = slot(SLOT_land_dts);
     double* land_dtsm = slot(SLOT_land_dtsm);
     double* land_dqs = slot(SLOT_land_dqs);
@@ -1122,6 +1384,30 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
     std::vector<double> dalbclim = monthly(dalbcl, step + 1, cal, NHOR);
     std::vector<double> dalbclim1 = monthly(dalbcl1, step + 1, cal, NHOR);
     std::vector<double> dalbclim2 = monthly(dalbcl2, step + 1, cal, NHOR);
+    if (land.nwetsoil > 0.5) {
+      // getalb at a fresh start: the bucket is empty (bf = 0)
+      const auto wet_soil = [](double bf, double dry, double ocean) {
+        const double al = 1 / 5.2 * std::pow(std::pow(std::max(bf, 1e-3), (1 - 4.0) / 4) - 1, 1 / 4.0);
+        const double dry_w = std::exp(-std::pow(bf * 25, 6)), wet_w = std::exp(-std::pow((1 - bf) * 30, 9));
+        return dry * dry_w + al * (1 - dry_w - wet_w) + ocean * wet_w;
+      };
+      for (int h = 0; h < NHOR; h++) {
+        if (dls[h] <= 0.5) continue;
+        if (land_dts[h] > 273.15) {
+          dalbclim[h] = wet_soil(0, land.alblandmax, doceanalb[0]);
+          dalbclim1[h] = dalbclim[h];
+          dalbclim2[h] = wet_soil(0, land.alblandmax, doceanalb[1]);
+        } else {
+          dalbclim[h] = land.albland;
+          dalbclim1[h] = dgroundalb[0];
+          dalbclim2[h] = dgroundalb[1];
+        }
+      }
+    }
+    const double albgmax1 = dsnowalbmx[0], albgmax2 = dsnowalbmx[1];
+    const auto glacier_albedo = [&](double mx, double mn, double ts) {
+      return std::max(mn, std::min(mx, mx - ((mx - mn) * (ts - 263.16)) / (tmelt - 263.16)));
+    };
     double* soil[5] = {slot(SLOT_dtsoil), slot(SLOT_dtd2), slot(SLOT_dtd3), slot(SLOT_dtd4), slot(SLOT_dtd5)};
     for (int h = 0; h < NHOR; h++) {
       if (dls[h] <= 0) continue;
@@ -1135,14 +1421,44 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
       if (dwmax[h] > 0) drhs[h] = std::min(1.0, dwatc[h] / (land.drhsfull * dwmax[h]));
       dz0[h] = dz0clim[h];
       for (int l = 0; l < SOIL_LAYERS; l++) soil[l][h] = dsoilt[(size_t)l * NHOR + h];
+      if (dglac[h] > 0.5) {
+        dsnowz[h] = std::max(land.dsmax, 0.0);
+        dsnow[h] = dsnowz[h];
+        dalb[h] = glacier_albedo(land.albgmax, land.albgmin, ts);
+        dsalb[h] = glacier_albedo(albgmax1, dglacalbmn[0], ts);
+        dsalb[NHOR + h] = glacier_albedo(albgmax2, dglacalbmn[1], ts);
+        drhs[h] = 1;
+      }
       dt[surf + h] = ts;
       dq[surf + h] = land_dqs[h];
+    }
+    // glacierini and oroini: glacier ice raises the orography; rivers
+    // follow it and its transform is so (setup/surface/index.ts:412-467)
+    if (icesheeth >= 0) {
+      for (int h = 0; h < NHOR; h++) {
+        const bool covered = orography[h] != 0;
+        dglac[h] = covered ? 1 : 0;
+        dsnowz[h] = covered ? icesheeth : 0;
+      }
+    } else {
+      for (int h = 0; h < NHOR; h++)
+        if (dglac[h] > 0 && dsnowz[h] == 0) dsnowz[h] = glacelim;
+    }
+    if (nglacier == 1) {
+      const double gm = 6.67408e-11 * 5.9722e24, re = 6.371e6, rho = 850;
+      for (int h = 0; h < NHOR; h++) {
+        const double ground = orography[h] * 1;
+        const double dz = (re * re * ground) / (gm - re * ground);
+        const double snow = dsnowz[h] / (rho / 1000);
+        const double gravity = gm / ((re + dz) * (re + dz));
+        doro[h] = ground + (gravity * snow - (gravity / (dz + re)) * snow * snow);
+      }
     }
     // oroini's river directions (setup/surface/index.ts:199-262).
     std::vector<double> duroff(NHOR, 0.0), dvroff(NHOR, 0.0);
     {
       std::vector<double> ht(NHOR);
-      for (int h = 0; h < NHOR; h++) ht[h] = dls[h] < 1 ? dls[h] - 1 : std::max(orography[h], 0.0);
+      for (int h = 0; h < NHOR; h++) ht[h] = dls[h] < 1 ? dls[h] - 1 : std::max(doro[h], 0.0);
       const double hmax = *std::max_element(ht.begin(), ht.end());
       const double hmin = *std::min_element(ht.begin(), ht.end());
       if ((hmax - hmin) / p.ga > 0.01) {
@@ -1196,23 +1512,26 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
     }
     std::copy(duroff.begin(), duroff.end(), slot(SLOT_duroff));
     std::copy(dvroff.begin(), dvroff.end(), slot(SLOT_dvroff));
-    // glacierini: no glacier file, icesheeth < 0
-    for (int h = 0; h < NHOR; h++)
-      if (dglac[h] > 0 && dsnowz[h] == 0) dsnowz[h] = glacelim;
+    if (nglacier == 1)
+      for (int h = 0; h < NHOR; h++)
+        if (dsnowz[h] > 30) dglac[h] = 1;
+    // vegini (setup/surface/index.ts:471-479)
+    const std::vector<double> dagg = field(1605, veg.rinidagg), dsc = field(1606, veg.rinidsc);
+    const std::vector<double> dmr = field(1607, veg.rinidmr), dgrow = field(1604, veg.forgrow);
     for (int h = 0; h < NHOR; h++) {
       slot(SLOT_dpersist)[h] = dls[h] < 0.5 ? 0 : 1;
       slot(SLOT_dforest)[h] = dls[h] == 0 ? 0 : dforest[h];
       slot(SLOT_dglac)[h] = dglac[h];
       slot(SLOT_dwmax)[h] = dwmax[h];
       slot(SLOT_darea)[h] = grid.weights[h / NLON];
-      slot(SLOT_dz0climo)[h] = 0;
+      slot(SLOT_dz0climo)[h] = dz0climo[h];
       slot(SLOT_dalbclim)[h] = dalbclim[h];
       slot(SLOT_dalbclim1)[h] = dalbclim1[h];
       slot(SLOT_dalbclim2)[h] = dalbclim2[h];
-      slot(SLOT_dagg)[h] = 0.5;
-      slot(SLOT_dsc)[h] = 1;
-      slot(SLOT_dmr)[h] = 2;
-      slot(SLOT_dgrow)[h] = 1;
+      slot(SLOT_dagg)[h] = dagg[h];
+      slot(SLOT_dsc)[h] = dsc[h];
+      slot(SLOT_dmr)[h] = dmr[h];
+      slot(SLOT_dgrow)[h] = dgrow[h];
       slot(SLOT_xgw)[h] = grid.weights[h / NLON];
     }
     std::copy(dsoilt.begin(), dsoilt.end(), slot(SLOT_dsoilt));
@@ -1223,31 +1542,41 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
       for (int h = 0; h < NHOR; h++) slot(SLOT_dqco2)[(size_t)l * NHOR + h] = rad.co2 > 0 ? rad.co2 : 0;
     slot(SLOT_co2)[0] = rad.co2;
 
-    // iceini and oceanini (setup/surface/index.ts:536-575)
+    // iceini's read_ice_surface and iceget, oceanini's oceanget
+    // (setup/surface/index.ts:481-536)
     const double tfreeze = ice.nseaice == 0 ? 0 : ice.tfreeze;
-    std::vector<double> xls(NHOR), xclsst((size_t)CLIM_MONTHS * NHOR, cfg.sea_surface_k);
-    std::vector<double> xclicec((size_t)CLIM_MONTHS * NHOR), xcliced((size_t)CLIM_MONTHS * NHOR, 0.0);
-    for (int h = 0; h < NHOR; h++) xls[h] = dls[h] > 0.5 ? 1 : 0;
-    for (size_t i = 0; i < xclicec.size(); i++)
-      xclicec[i] = (ice.nice > 0.5 && xclsst[i] <= tfreeze) ? 1 : 0;
-    // make_ice_thickness (setup/surface/index.ts:265-292 of the reference
-    // listing) for the synthetic cover
-    {
-      static const double fac[14] = {0.912, 0.942, 1, 1.058, 1.124, 1.161, 1.175,
-                                      1.058, 0.931, 0.883, 0.88,  0.876, 0.912, 0.942};
-      for (int mo = 0; mo < CLIM_MONTHS; mo++)
-        for (int h = 0; h < NHOR; h++) {
-          const double c = xclicec[(size_t)mo * NHOR + h];
-          const bool north = (h / NLON) < NLAT / 2;
-          double d = 0;
-          if (north) {
-            if (c >= 0.9) d = 3 * fac[mo];
-            else if (c >= 0.1) d = (0.25 + (3 - 0.25) * (c - 0.1) / (0.9 - 0.1)) * fac[mo];
-            else if (c > 0) d = 0.25 * fac[mo];
-          } else if (c >= 0.25) d = 0.5;
-          else if (c > 0) d = 0.25 + c;
-          xcliced[(size_t)mo * NHOR + h] = d;
-        }
+    std::vector<double> xclsst;
+    if (!read(169, CLIM_MONTHS, xclsst))
+      xclsst.assign((size_t)CLIM_MONTHS * NHOR, std::isnan(cfg.sea_surface_k) ? -999.0 : cfg.sea_surface_k);
+    std::vector<double> xclicec = climatology(210, -999), xcliced = climatology(211, -999);
+    if (ice.nseaice == 0) {
+      std::fill(xclicec.begin(), xclicec.end(), 0.0);
+      std::fill(xcliced.begin(), xcliced.end(), 0.0);
+    }
+    const double cover_max = *std::max_element(xclicec.begin(), xclicec.end());
+    if (ice.nice >= 0) {
+      if (cover_max > 5)
+        for (auto& v : xclicec) v *= 0.01;
+      if (cover_max < 0)
+        for (size_t i = 0; i < xclicec.size(); i++) xclicec[i] = (ice.nice > 0.5 && xclsst[i] <= tfreeze) ? 1 : 0;
+      if (xcliced[0] < 0) {
+        // make_ice_thickness (CCM3; setup/surface/index.ts:265-290)
+        static const double fac[14] = {0.912, 0.942, 1, 1.058, 1.124, 1.161, 1.175,
+                                        1.058, 0.931, 0.883, 0.88,  0.876, 0.912, 0.942};
+        for (int mo = 0; mo < CLIM_MONTHS; mo++)
+          for (int h = 0; h < NHOR; h++) {
+            const double c = xclicec[(size_t)mo * NHOR + h];
+            const bool north = (h / NLON) < NLAT / 2;
+            double d = 0;
+            if (north) {
+              if (c >= 0.9) d = 3 * fac[mo];
+              else if (c >= 0.1) d = (0.25 + (3 - 0.25) * (c - 0.1) / (0.9 - 0.1)) * fac[mo];
+              else if (c > 0) d = 0.25 * fac[mo];
+            } else if (c >= 0.25) d = 0.5;
+            else if (c > 0) d = 0.25 + c;
+            xcliced[(size_t)mo * NHOR + h] = d;
+          }
+      }
     }
     for (size_t i = 0; i < xclicec.size(); i++)
       if (ice.nseaice == 0 || xls[i % NHOR] >= 1) {
@@ -1262,8 +1591,10 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
     for (int h = 0; h < NHOR; h++) xclicec2[h] = xcliced2[h] <= 0 ? 0 : std::max(xclicec2[h], 0.0);
     std::vector<double> xiced = xcliced2, xicec(NHOR);
     for (int h = 0; h < NHOR; h++) xicec[h] = xclicec2[h] >= ice.thicec ? 1 : 0;
-    std::vector<double> yclsst((size_t)CLIM_MONTHS * NHOR);
-    for (size_t i = 0; i < yclsst.size(); i++) yclsst[i] = std::max(cfg.sea_surface_k, tfreeze);
+    std::vector<double> yclsst;
+    if (!read(169, CLIM_MONTHS, yclsst))
+      yclsst.assign((size_t)CLIM_MONTHS * NHOR, std::isnan(cfg.sea_surface_k) ? 0.0 : cfg.sea_surface_k);
+    for (auto& v : yclsst) v = std::max(v, tfreeze);
     std::vector<double> yclsst2 = monthly(yclsst, step + 1, cal, NHOR);
     // seaini on sea points
     double* dicec = slot(SLOT_dicec);
@@ -1311,13 +1642,12 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
     std::copy(xclicec.begin(), xclicec.end(), slot(SLOT_xclicec));
     std::copy(xcliced.begin(), xcliced.end(), slot(SLOT_xcliced));
     std::copy(yclsst.begin(), yclsst.end(), slot(SLOT_yclsst));
-    (void)dz0clim;
   }
 
   // Spectral orography so (oroini: FOURIER.analyse + LEGENDRE.analyse with
   // skgpsp, / cv^2, truncated above noromax).
   std::vector<double> so(NRSP, 0.0);
-  grid_to_spectral(m, orography.data(), so.data(), true);
+  grid_to_spectral(m, doro.data(), so.data(), true);
   for (auto& v : so) v /= p.cv * p.cv;
   {
     int r = 0;
@@ -1494,8 +1824,18 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
     k[CON_doceanalb1] = doceanalb[0];
     k[CON_doceanalb2] = doceanalb[1];
     k[CON_nveg] = p.nveg;
-    k[CON_nglacier] = 0;
+    k[CON_nglacier] = nglacier > 0 ? 1 : 0;
     k[CON_glacelim] = glacelim;
+    // simba's vegstep (setup/index.ts:256-284): the namelist's light-use
+    // efficiency, CO2 conversion and time scales; atan(cveg_d) and
+    // atan(-cveg_f) in FP64
+    k[CON_rlue] = land.rlue;
+    k[CON_co2conv] = land.co2conv;
+    k[CON_tau_veg] = land.tau_veg;
+    k[CON_tau_soil] = land.tau_soil;
+    k[CON_ncveg] = veg.ncveg;
+    k[CON_cveg_e] = std::atan(-1.9);
+    k[CON_cveg_g] = std::atan(-3.0);
     // icemod (setup/modules/index.ts:212-232; setup/index.ts:28-40)
     k[CON_xdt] = p.solar_day / p.ntspd;
     k[CON_xmind] = ice.xmind;
@@ -1560,12 +1900,40 @@ void build_member(const exo_config& cfg, Member& m, const double* land_fraction,
 
 void step_values(const Member& m, int64_t step, int64_t carbon_count, int publish, double* out) {
   const Calendar& c = m.cal;
-  // SOLAR.position, non-Keplerian branch (calendar/solar/index.ts:25-93)
+  // SOLAR.position (calendar/solar/index.ts:25-93)
   const double calday = (double)(step % c.steps_per_year) / c.steps_per_year;
   const double minutes = std::trunc((double)(step % c.steps_per_day) * c.step_minutes);
   const double e = c.eccentricity;
   double lon, dist;
-  {
+  if (c.keplerian) {
+    // Kepler's equation by Newton's method (calendar/solar/index.ts:13-24,
+    // :44-66)
+    double anomaly_true = 0;
+    dist = 1;
+    if (!c.synchronous) {
+      double mean = calday * TWOPI + c.mean_anomaly_rad;
+      while (mean > TWOPI) mean -= TWOPI;
+      while (mean < 0) mean += TWOPI;
+      if (e > 0) {
+        double ea = e < 0.5 ? mean : PI;
+        for (int it = 0; it < 101; it++) {
+          const double prev = ea;
+          ea -= (ea - (mean + e * std::sin(ea))) / std::max(1e-15, 1 - e * std::cos(ea));
+          if (std::fabs(ea - prev) <= 1e-14) break;
+        }
+        const double tangent = std::tan(ea * 0.5);
+        const double magnitude = std::sqrt(((1 + e) / (1 - e)) * tangent * tangent);
+        anomaly_true = 2 * std::atan(tangent < 0 ? -magnitude : magnitude);
+        if (anomaly_true < 0) anomaly_true += TWOPI;
+        anomaly_true = std::fmod(anomaly_true, TWOPI);
+        const double inverse = 1 / (1 - e * std::cos(ea));
+        dist = inverse * inverse;
+      } else {
+        anomaly_true = mean;
+      }
+    }
+    lon = std::fmod(c.perihelion_lon_rad + anomaly_true, TWOPI);
+  } else {
     const double ml = c.mean_lon_equinox_rad + (calday - 80.5 / 365) * 2 * M_PI;
     const double an = ml - c.perihelion_lon_rad;
     const double s = std::sin(an);

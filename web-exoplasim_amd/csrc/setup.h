@@ -66,12 +66,11 @@ struct Member {
 void config_default(exo_config* cfg);
 // Builds everything the device needs for one member; throws
 // std::runtime_error on an invalid configuration.
-// With land_fraction and geopotential (NHOR each, the grid's cells) the
-// planet's land mask and orography come from them (landini, setup/
-// surface/index.ts:293-297: land where the fraction exceeds 0.5) instead
-// of the seeded terrain.
-void build_member(const exo_config& cfg, Member& m, const double* land_fraction = nullptr,
-                  const double* geopotential = nullptr);
+// The surface fields (exo_add_member_surface) replace the fallbacks of
+// their codes; with codes 172 and 129 the planet's land mask and
+// orography come from them (landini, setup/surface/index.ts:293-297: land
+// where the fraction exceeds 0.5) instead of the seeded terrain.
+void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fields = nullptr, int count = 0);
 // Calendar date {year, month, day, hour, minute} of main-loop step `step`
 // (calendar/date/index.ts:46-82, 360-day calendar).
 void member_date(const Member& m, int64_t step, int out[5]);

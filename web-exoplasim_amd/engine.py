@@ -56,7 +56,51 @@ class ExoConfig(ctypes.Structure):
         ("terrain_seed", ctypes.c_uint64),
         ("noise_seed", ctypes.c_uint64),
         ("storm_diagnostics", ctypes.c_int),
+        ("vegetation", ctypes.c_int),
+        ("initial_growth", ctypes.c_double),
+        ("glaciers", ctypes.c_int),
+        ("glacier_min_depth_m", ctypes.c_double),
+        ("glacier_initial_height_m", ctypes.c_double),
+        ("star_temperature_k", ctypes.c_double),
+        ("synchronous", ctypes.c_int),
+        ("substellar_longitude_deg", ctypes.c_double),
+        ("fixed_orbit", ctypes.c_int),
+        ("keplerian", ctypes.c_int),
+        ("periapse_longitude_deg", ctypes.c_double),
+        ("initial_mean_anomaly_deg", ctypes.c_double),
+        ("soil_depth_scale", ctypes.c_double),
+        ("soil_water_capacity_m", ctypes.c_double),
+        ("soil_heat_capacity", ctypes.c_double),
+        ("land_albedo", ctypes.c_double),
+        ("ocean_albedo", ctypes.c_double),
+        ("snow_ice_albedo", ctypes.c_double),
+        ("wet_soil", ctypes.c_int),
+        ("filter", ctypes.c_int),
     ]
+
+
+class ExoSurfaceField(ctypes.Structure):
+    """exo_surface_field: one .sra field of a planet's surface."""
+
+    _fields_ = [("code", ctypes.c_int), ("months", ctypes.c_int), ("values", ctypes.POINTER(ctypes.c_double))]
+
+
+GASES = ("H2", "He", "N2", "O2", "Ar", "Ne", "Kr", "CO2", "H2O")
+
+
+def atmosphere_resolve(partial_pressures=None, mode="partialPressures", pressure_bar=None, gas_constant=None):
+    """GCM_ATMOSPHERE.resolve: (pressure bar, gas constant, CO2 ppm) of a
+    gas mix given as {gas: bar or None}."""
+    lib = library()
+    pp = partial_pressures or {}
+    arr = (ctypes.c_double * 9)(*[float("nan") if pp.get(g) is None else float(pp[g]) for g in GASES])
+    out = [ctypes.c_double() for _ in range(3)]
+    nan = float("nan")
+    _check(lib.exo_atmosphere_resolve(arr, 1 if mode == "pressureAndGasConstant" else 0,
+                                      nan if pressure_bar is None else float(pressure_bar),
+                                      nan if gas_constant is None else float(gas_constant),
+                                      *[ctypes.byref(o) for o in out]))
+    return tuple(o.value for o in out)
 
 
 class ExoRequest(ctypes.Structure):
@@ -139,6 +183,9 @@ def library():
         "exo_arena_floats": (I, [P, I, ctypes.POINTER(S)]),
         "exo_slot": (I, [P, I, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
         "exo_slot_name": (ctypes.c_char_p, [I]),
+        "exo_add_member_surface": (I, [P, ctypes.POINTER(ExoConfig), I, ctypes.POINTER(ExoSurfaceField), I,
+                                       ctypes.POINTER(I)]),
+        "exo_atmosphere_resolve": (I, [D, I, ctypes.c_double, ctypes.c_double, D, D, D]),
         "exo_snapshot_field_name": (ctypes.c_char_p, [I, I]),
         "exo_read_field": (I, [P, I, ctypes.c_char_p, F, S]),
         "exo_write_field": (I, [P, I, ctypes.c_char_p, F, S]),
@@ -380,6 +427,27 @@ class Engine:
                                                 lf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                 geo.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                 ctypes.byref(mid)))
+        return mid.value
+
+    def add_member_surface(self, cfg=None, fields=None, precision=EXO_PRECISION_FP32):
+        """Adds a planet whose surface comes from .sra fields: {code:
+        array of months x the grid's cells}; codes absent take the
+        reference's fallbacks."""
+        cfg = cfg or default_config()
+        arrays, table = [], (ExoSurfaceField * max(1, len(fields or {})))()
+        nlat = {21: 32, 42: 64, 63: 96}.get(cfg.truncation, 0)
+        nhor = 2 * nlat * nlat
+        for i, (code, values) in enumerate(sorted((fields or {}).items())):
+            a = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+            if nhor == 0 or a.size % nhor:
+                raise ExoError(-1, f"field {code} does not hold whole T{cfg.truncation} grids")
+            arrays.append(a)
+            table[i].code = int(code)
+            table[i].months = a.size // nhor
+            table[i].values = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        mid = ctypes.c_int()
+        _check(self.lib.exo_add_member_surface(self.h, ctypes.byref(cfg), precision, table, len(arrays),
+                                               ctypes.byref(mid)))
         return mid.value
 
     def remove_member(self, member):

@@ -126,6 +126,14 @@ const Field kConfigFields[] = {
     EXO_FIELD('d', land_fraction), EXO_FIELD('d', terrain_max_m), EXO_FIELD('d', initial_temperature),
     EXO_FIELD('d', initial_humidity), EXO_FIELD('d', noise_kelvin), EXO_FIELD('d', sea_surface_k),
     EXO_FIELD('u', terrain_seed), EXO_FIELD('u', noise_seed), EXO_FIELD('i', storm_diagnostics),
+    EXO_FIELD('i', vegetation), EXO_FIELD('d', initial_growth), EXO_FIELD('i', glaciers),
+    EXO_FIELD('d', glacier_min_depth_m), EXO_FIELD('d', glacier_initial_height_m),
+    EXO_FIELD('d', star_temperature_k), EXO_FIELD('i', synchronous), EXO_FIELD('d', substellar_longitude_deg),
+    EXO_FIELD('i', fixed_orbit), EXO_FIELD('i', keplerian), EXO_FIELD('d', periapse_longitude_deg),
+    EXO_FIELD('d', initial_mean_anomaly_deg), EXO_FIELD('d', soil_depth_scale),
+    EXO_FIELD('d', soil_water_capacity_m), EXO_FIELD('d', soil_heat_capacity), EXO_FIELD('d', land_albedo),
+    EXO_FIELD('d', ocean_albedo), EXO_FIELD('d', snow_ice_albedo), EXO_FIELD('i', wet_soil),
+    EXO_FIELD('i', filter),
 };
 #undef EXO_FIELD
 
@@ -219,6 +227,69 @@ napi_value add_member(napi_env env, napi_callback_info info) {
   int id = -1;
   EXO_OK_OR_THROW(exo_add_member(e, &c, int_of(env, a.v[2]), &id));
   return number(env, id);
+}
+
+// Gaussian latitudes of a truncation (0 for one that is not built).
+int exo_latitudes_of(int truncation) { return truncation == 21 ? 32 : truncation == 42 ? 64 : truncation == 63 ? 96 : 0; }
+
+// addMemberSurface(engine, config, precision, [{code, months, values:
+// Float64Array}]) -> id: exo_add_member_surface (the planet's .sra fields).
+napi_value add_member_surface(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 4)) return nullptr;
+  exo_engine* e = engine_of(env, a.v[0]);
+  if (!e) return nullptr;
+  exo_config c;
+  if (!config_from(env, a.v[1], c)) return nullptr;
+  uint32_t n = 0;
+  NAPI_OK(napi_get_array_length(env, a.v[3], &n));
+  std::vector<exo_surface_field> fields(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value f, v;
+    NAPI_OK(napi_get_element(env, a.v[3], i, &f));
+    fields[i].code = (int)number_of(env, f, "code");
+    fields[i].months = (int)number_of(env, f, "months");
+    NAPI_OK(napi_get_named_property(env, f, "values", &v));
+    size_t count = 0;
+    fields[i].values = static_cast<const double*>(typed_data(env, v, napi_float64_array, count));
+    if (!fields[i].values) return nullptr;
+    if (count != (size_t)fields[i].months * 2 * exo_latitudes_of(c.truncation) * exo_latitudes_of(c.truncation)) {
+      napi_throw_range_error(env, nullptr, "a surface field does not match the grid");
+      return nullptr;
+    }
+  }
+  int id = -1;
+  EXO_OK_OR_THROW(exo_add_member_surface(e, &c, int_of(env, a.v[2]), fields.data(), (int)n, &id));
+  return number(env, id);
+}
+
+// atmosphereResolve([9 partial pressures, NaN for null], pressureAndGasConstant,
+// pressureBar, gasConstant) -> {pressureBar, gasConstant, co2Ppm}:
+// GCM_ATMOSPHERE.resolve.
+napi_value atmosphere_resolve(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 4)) return nullptr;
+  double partial[9];
+  for (uint32_t i = 0; i < 9; i++) {
+    napi_value v;
+    NAPI_OK(napi_get_element(env, a.v[0], i, &v));
+    if (napi_get_value_double(env, v, &partial[i]) != napi_ok) {
+      napi_throw_type_error(env, nullptr, "partial pressures must be numbers");
+      return nullptr;
+    }
+  }
+  bool mode = false;
+  napi_get_value_bool(env, a.v[1], &mode);
+  double bar = 0, gas = 0, out[3];
+  napi_get_value_double(env, a.v[2], &bar);
+  napi_get_value_double(env, a.v[3], &gas);
+  EXO_OK_OR_THROW(exo_atmosphere_resolve(partial, mode ? 1 : 0, bar, gas, &out[0], &out[1], &out[2]));
+  napi_value obj;
+  NAPI_OK(napi_create_object(env, &obj));
+  NAPI_OK(napi_set_named_property(env, obj, "pressureBar", number(env, out[0])));
+  NAPI_OK(napi_set_named_property(env, obj, "gasConstant", number(env, out[1])));
+  NAPI_OK(napi_set_named_property(env, obj, "co2Ppm", number(env, out[2])));
+  return obj;
 }
 
 napi_value remove_member(napi_env env, napi_callback_info info) {
@@ -675,6 +746,7 @@ napi_value init(napi_env env, napi_value exports) {
       {"loadAverager", load_averager},   {"readClimatology", read_climatology},
       {"balanceAssess", balance_assess},
       {"boundaryFromTerrain", boundary_from_terrain}, {"boundarySra", boundary_sra},
+      {"addMemberSurface", add_member_surface}, {"atmosphereResolve", atmosphere_resolve},
   };
   for (const auto& f : fns) {
     napi_value fn;

@@ -18,14 +18,47 @@
 // month, the closed months, the yearly balances), and request.averager
 // restores it.
 
+const fs = require("fs")
 const path = require("path")
 const addon = require(path.join(__dirname, "exoplasim_napi.node"))
 
-// GcmConfig (src/model/climate/gcm/types.ts:29-66) onto exo_config. The
-// synthetic-planet fields (landFraction, terrainMaxM, seeds, ...) come from
-// request.planet; the reference reads them from terrain rasters instead.
+// GcmConfig's fields (src/model/climate/gcm/types.ts:28-62).
+const CONFIG_KEYS = [
+	"targetTemperatureK", "stormclim", "atmosphere", "layers", "periapseLongitudeDeg", "keplerian",
+	"initialMeanAnomalyDeg", "soilWaterCapacityM", "soilHeatCapacity", "soilDepthScale", "landAlbedo",
+	"oceanAlbedo", "snowIceAlbedo", "radiusEarth", "gravityMs2", "synchronous", "substellarLongitudeDeg",
+	"oceanDepthM", "fluxWm2", "starTemperatureK", "yearDays", "eccentricity", "rotationPeriodDays",
+	"obliquityDeg", "fixedOrbit", "wetSoil", "vegetation", "initialGrowth", "glaciers", "physicsFilter",
+	"timestepMinutes", "truncation", "earthSurfaceClimatology",
+]
+// GCM_ATMOSPHERE.GASES order (atmosphere/index.ts:14-64).
+const GASES = ["H2", "He", "N2", "O2", "Ar", "Ne", "Kr", "CO2", "H2O"]
+// physicsFilter's NFILTER switches (namelist/index.ts:27-34); "gp" and
+// "sp" set NGPTFILTER / NSPVFILTER, which the base namelist already sets.
+const NFILTER = { none: 0, cesaro: 1, exp: 2, lh: 3 }
+
+function nullable(value) {
+	return value === null || value === undefined ? NaN : value
+}
+
+function flag(value) {
+	return value ? 1 : 0
+}
+
+// GcmConfig (types.ts:28-62) onto exo_config, as GCM_NAMELIST.edits
+// writes it into the namelists (namelist/index.ts:40-182), the
+// atmosphere through GCM_ATMOSPHERE.resolve. Every field is honoured or
+// refused: an unknown field, a targeting run and a free (Berger) orbit
+// throw. `planet` (optional) carries the synthetic-planet fields of the
+// seeded terrain (landFraction, terrainMaxM, seeds, seaSurfaceK, ...),
+// which the reference takes from terrain rasters instead.
 function engineConfig(config, planet) {
-	const atmosphere = config.atmosphere || {}
+	for (const key of Object.keys(config))
+		if (!CONFIG_KEYS.includes(key)) throw new Error(`Unknown GcmConfig field ${key}`)
+	if (config.targetTemperatureK !== null && config.targetTemperatureK !== undefined)
+		throw new Error("The targeting runner (targetTemperatureK) is not built for the MI355X engine")
+	if (config.fixedOrbit === false)
+		throw new Error("A free orbit (fixedOrbit false, Berger's series) is not built for the MI355X engine")
 	const out = {
 		truncation: config.truncation,
 		levels: config.layers,
@@ -38,10 +71,57 @@ function engineConfig(config, planet) {
 		eccentricity: config.eccentricity,
 		year_days: config.yearDays,
 		ocean_depth_m: config.oceanDepthM,
-		ozone: atmosphere.ozone === undefined ? undefined : atmosphere.ozone ? 1 : 0,
-		pressure_bar: atmosphere.pressureBar === null ? undefined : atmosphere.pressureBar,
-		gas_constant: atmosphere.gasConstant === null ? undefined : atmosphere.gasConstant,
-		storm_diagnostics: config.stormclim === undefined ? undefined : config.stormclim ? 1 : 0,
+		storm_diagnostics: config.stormclim === undefined ? undefined : flag(config.stormclim),
+		vegetation: config.vegetation,
+		initial_growth: config.initialGrowth,
+		star_temperature_k: config.starTemperatureK,
+		synchronous: config.synchronous === undefined ? undefined : flag(config.synchronous),
+		substellar_longitude_deg: config.substellarLongitudeDeg,
+		fixed_orbit: config.fixedOrbit === undefined ? undefined : flag(config.fixedOrbit),
+		keplerian: config.keplerian === undefined ? undefined : flag(config.keplerian),
+		soil_depth_scale: config.soilDepthScale,
+		wet_soil: config.wetSoil === undefined ? undefined : flag(config.wetSoil),
+	}
+	for (const [key, name] of [
+		["periapseLongitudeDeg", "periapse_longitude_deg"],
+		["initialMeanAnomalyDeg", "initial_mean_anomaly_deg"],
+		["soilWaterCapacityM", "soil_water_capacity_m"],
+		["soilHeatCapacity", "soil_heat_capacity"],
+		["landAlbedo", "land_albedo"],
+		["oceanAlbedo", "ocean_albedo"],
+		["snowIceAlbedo", "snow_ice_albedo"],
+	])
+		if (key in config) out[name] = nullable(config[key])
+	if (config.glaciers) {
+		out.glaciers = flag(config.glaciers.enabled)
+		out.glacier_min_depth_m = config.glaciers.minDepthM
+		out.glacier_initial_height_m = config.glaciers.initialHeightM
+	}
+	if (config.physicsFilter) {
+		let nfilter = 0
+		for (const filter of config.physicsFilter) {
+			if (filter === "gp" || filter === "sp") continue
+			if (!(filter in NFILTER)) throw new Error(`Unknown physics filter ${filter}`)
+			nfilter = NFILTER[filter]
+		}
+		out.filter = nfilter
+	}
+	const atmosphere = config.atmosphere
+	if (atmosphere) {
+		const mode = atmosphere.mode
+		if (mode !== "partialPressures" && mode !== "pressureAndGasConstant")
+			throw new Error("Choose an atmosphere input mode.")
+		const partial = GASES.map((gas) => nullable((atmosphere.partialPressures || {})[gas]))
+		const resolved = addon.atmosphereResolve(
+			partial,
+			mode === "pressureAndGasConstant",
+			nullable(atmosphere.pressureBar),
+			nullable(atmosphere.gasConstant),
+		)
+		out.pressure_bar = resolved.pressureBar
+		out.gas_constant = resolved.gasConstant
+		out.co2_ppm = resolved.co2Ppm
+		if (atmosphere.ozone !== undefined) out.ozone = flag(atmosphere.ozone)
 	}
 	const p = planet || {}
 	const extra = {
@@ -50,14 +130,68 @@ function engineConfig(config, planet) {
 		initial_temperature: p.initialTemperatureK,
 		initial_humidity: p.initialHumidity,
 		noise_kelvin: p.noiseKelvin,
-		sea_surface_k: p.seaSurfaceK,
-		co2_ppm: p.co2Ppm,
+		sea_surface_k: p.seaSurfaceK === null ? NaN : p.seaSurfaceK,
 		terrain_seed: p.terrainSeed,
 		noise_seed: p.noiseSeed,
 	}
+	if (p.co2Ppm !== undefined) extra.co2_ppm = p.co2Ppm
 	const merged = Object.assign(out, extra)
 	for (const key of Object.keys(merged)) if (merged[key] === undefined) delete merged[key]
 	return merged
+}
+
+// Every resolution this engine runs (the shape of GCM_RUNNER.resolutions,
+// runner/index.ts:12-59): T21, T42 and T63 at 5 and 10 levels, on the GPU
+// engine; `inputs` is the resolution's inputs.json under public/gcm.
+function resolutions() {
+	const out = []
+	for (const [truncation, latitudes] of [[21, 32], [42, 64], [63, 96]])
+		for (const layers of [5, 10])
+			out.push({ layers, truncation, latitudes, cpu: null, gpu: { inputs: `t${truncation}/inputs.json`, downloadBytes: 0 } })
+	return out
+}
+
+// GCM_INPUTS.keepSurfaceFile (runner/inputs/index.ts:5-27) for this config.
+function keepSurfaceFile(name, config) {
+	if (!config.earthSurfaceClimatology) return false
+	if (config.landAlbedo !== null && config.landAlbedo !== undefined && name.endsWith("0174.sra")) return false
+	if (config.soilWaterCapacityM !== null && config.soilWaterCapacityM !== undefined && name.endsWith("0229.sra"))
+		return false
+	const sheet = config.glaciers ? Number(config.glaciers.initialHeightM) : NaN
+	return !(sheet > 0 && ["0174.sra", "1740.sra", "0210.sra", "0232.sra"].some((s) => name.endsWith(s)))
+}
+
+// The records of one .sra file (get_surf_array, setup/surface/index.ts:
+// 76-107): an eight-integer header, then the grid's values, per month.
+function sraField(name, text, latitudes) {
+	const code = Number(name.slice(-8, -4))
+	const tokens = text.trim().split(/\s+/).map(Number)
+	const points = 2 * latitudes * latitudes
+	const record = 8 + points
+	const months = Math.floor(tokens.length / record)
+	const values = new Float64Array(months * points)
+	for (let m = 0; m < months; m++) values.set(tokens.slice(m * record + 8, (m + 1) * record), m * points)
+	return { code, months, values }
+}
+
+// The surface fields of a run (runner/webgpu/index.ts:39-80's files): the
+// Earth climatologies kept for the config, then the terrain's land mask
+// and orography (GCM_BOUNDARY.files).
+function surfaceFields(engine, request) {
+	const config = request.config
+	const latitudes = { 21: 32, 42: 64, 63: 96 }[config.truncation]
+	const files = {}
+	if (config.earthSurfaceClimatology) {
+		let inputs = request.inputs
+		if (!inputs && request.publicDir && request.resolution && request.resolution.gpu)
+			inputs = JSON.parse(fs.readFileSync(path.join(request.publicDir, request.resolution.gpu.inputs), "utf8"))
+		if (!inputs) throw new Error(`Missing t${config.truncation}/inputs.json in public/gcm`)
+		for (const [name, text] of Object.entries(inputs.files))
+			if (name.endsWith(".sra") && keepSurfaceFile(name, config)) files[name] = text
+	}
+	if (request.terrain)
+		Object.assign(files, boundaryFiles(engine, request.terrain, latitudes, config.gravityMs2 === undefined ? 9.80665 : config.gravityMs2))
+	return Object.entries(files).map(([name, text]) => sraField(name, text, latitudes))
 }
 
 // A closed month as GcmMonthlyMean (averages/index.ts:50-69: the month's
@@ -101,7 +235,15 @@ function loadAverager(engine, id, averager) {
 	addon.loadAverager(engine, id, months, averager.years || [], current)
 }
 
+// GCM_RUNNER.start's checks (runner/index.ts:61-78), then the run.
 function start(request) {
+	if (request.resolution) {
+		if (request.config.layers !== request.resolution.layers || request.config.truncation !== request.resolution.truncation)
+			throw new Error("Model assets do not match the configured resolution and layers")
+		const engine = request.engine || "gpu"
+		if (!request.resolution[engine])
+			throw new Error(`T${request.resolution.truncation} is not built for the ${engine.toUpperCase()} engine`)
+	}
 	const state = {
 		paused: !!request.startPaused,
 		stopped: false,
@@ -177,7 +319,16 @@ function start(request) {
 			status("loading")
 			state.engine = addon.create(request.device === undefined ? 0 : request.device, 1)
 			const precision = request.precision === "split" ? addon.PRECISION_SPLIT : addon.PRECISION_FP32
-			state.id = addon.addMember(state.engine, engineConfig(request.config, request.planet), precision)
+			const config = engineConfig(request.config, request.planet)
+			const fields = surfaceFields(state.engine, request)
+			// without a surface-temperature climatology or a synthetic SST the
+			// ocean starts as the reference's does (setup/surface/index.ts:
+			// 483-527)
+			if ((request.terrain || request.config.earthSurfaceClimatology) && !(request.planet && "seaSurfaceK" in request.planet))
+				config.sea_surface_k = NaN
+			state.id = fields.length
+				? addon.addMemberSurface(state.engine, config, precision, fields)
+				: addon.addMember(state.engine, config, precision)
 			addon.setPublishEvery(state.engine, Math.max(1, request.publishEvery | 0))
 			// a continuation under an edited config resets what the edit
 			// invalidates (gcm.worker.ts:110-124, continuation/index.ts:60-85)
@@ -236,4 +387,14 @@ function boundaryFiles(engine, terrain, latitudes, gravityMs2) {
 	}
 }
 
-module.exports = { start, engineConfig, addon, readAverager, loadAverager, boundaryFiles }
+module.exports = {
+	start,
+	resolutions,
+	engineConfig,
+	addon,
+	readAverager,
+	loadAverager,
+	boundaryFiles,
+	surfaceFields,
+	keepSurfaceFile,
+}
