@@ -1,101 +1,141 @@
-// Zonal DFTs of the ExoPlaSim step on gfx950 matrix cores.
+// Zonal transforms of the ExoPlaSim step on gfx950: real FFTs, one latitude
+// row per lane.
 //
-// With the spectral truncation, a latitude row's zonal transform is a
-// product with the truncated DFT matrix of its truncation (the FP32 tables
-// FOURIER.tables builds, fourier/index.ts:19-36):
-//   fc2gp (fourier/wgsl/inverse.wgsl:4-18):  g[x] = sum_k F[k] inv[k][x]
-//   gp2fc (fourier/wgsl/forward.wgsl:4-18):  F[k] = sum_x g[x] fwd[x][k]
-// and d/dlambda (dlambda.wgsl:5-17) is fc2gp of (-m Im F, m Re F); k = 2m
-// + part. Half a period on, wavenumber m's table entries change sign by
-// (-1)^m (inv[k][x + N/2] = (-1)^m inv[k][x]), so the inverse product is
-// split by the parity of m into two half-width ones: E[x] = sum over even
-// m, O[x] = sum over odd m (x < N/2); g[x] = E + O, g[x + N/2] = E - O:
-// the same terms as the full product, in another order, for half its
-// multiply-adds. The forward product splits the same way: half-row sums
-// give the even wavenumbers, differences the odd ones. The tables are the
-// right operand, shared by every row, job and member of a truncation.
+// With the spectral truncation, a row's transforms are the truncated DFTs
+// of FOURIER.tables (fourier/index.ts:19-36):
+//   fc2gp (fourier/wgsl/inverse.wgsl:4-18):  g[x] = Re F_0 + 2 Re sum_{m=1..T} F_m e^{i m lambda_x}
+//   gp2fc (fourier/wgsl/forward.wgsl:4-18):  F_m = (1/N) sum_x g[x] e^{-i m lambda_x},  m <= T
+// with lambda_x = 2 pi x / N, F_m = F[2m] + i F[2m+1] (Im F_0 ignored on the
+// way in), and d/dlambda (dlambda.wgsl:5-17) is fc2gp of i m F_m. Here both
+// run as real FFTs of length N = NLON through a complex FFT of half the
+// length, M = N / 2 (32, 64 or 96 = 3 x 32), in the lane's registers:
+//   inverse: Z[k] = (X[k] + X*[M-k]) + i w^k (X[k] - X*[M-k]), k < M (X the
+//            truncated spectrum, w = e^{2 pi i / N}); z = sum_k Z[k] W^{kn}
+//            (W = e^{2 pi i / M}); g[2n] = Re z[n], g[2n+1] = Im z[n];
+//   forward: z[n] = g[2n] + i g[2n+1]; Z = sum_n z[n] W^{-kn};
+//            X[k] = (Z[k] + Z*[M-k]) / 2 - i w^{-k} (Z[k] - Z*[M-k]) / 2.
+// The complex FFTs are radix-2 decimation in time (input in bit-reversed
+// register order), for M = 96 three 32-point FFTs joined by radix-3
+// butterflies. The twiddles W^t and w^k come from the truncation's FP64
+// set-up as an FP32 table (fft_twiddle) staged in LDS. The sums are the
+// DFT's terms in another order: a few ulp of FP32 apart from the table
+// products.
 //
-// Fourier slots are coefficient-major (coefficient k of row r at k R + r,
-// R the slot's rows): the inverse stages a chunk with lanes over its 128
-// rows, and the forward writes its results through LDS one coefficient of
-// 64 consecutive rows at a time, so both move contiguous runs.
-//
-// A workgroup (8 waves) takes 128 rows. Rows run over the jobs of a launch
-// and the members of a group (members of one truncation and level count,
-// so one table): each row has its own source and destination pointers.
-// The reduction runs in chunks of 32: while the waves run
-// v_mfma_f32_16x16x4_f32 over their 16 rows and the column tiles of chunk
-// c from LDS, the loads of chunk c + 1 are in flight into registers.
+// Rows: lane = row; a workgroup (one wave) takes 64 rows of a member group
+// (members of one truncation and level count), over the jobs of the launch,
+// each row with its own source and destination. Fourier slots are
+// coefficient-major (coefficient k of row r at k R + r, R the slot's rows),
+// so a lane-per-row coefficient access is one contiguous run per wave.
+// Grid rows are longitude-fastest rows of N floats; they pass through LDS
+// in 32-value column chunks, each wave moving 2 rows x 128 bytes per access.
 #include "transforms.h"
 
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int kThreads = 64;  // one wave: its LDS transpositions need no cross-wave barrier
+constexpr int kRows = kThreads;  // one row per lane
+constexpr int kMaxM = 96;
+constexpr int kTs = 33;  // transposition rows: 32 values + 1 pad, conflict-free
 
-constexpr int kThreads = 512;
-constexpr int kRows = 128;  // rows per workgroup
-constexpr int kChunk = 32;  // reduction chunk
-constexpr int kInvTiles = 96 / 16;  // inverse: N/2 <= 96 columns per half
-constexpr int kFwdTiles = 128 / 16;  // forward: NFC <= 128 columns
+// ------------------------------------------------------------- FFT core
 
-__host__ __device__ constexpr int a_stride(int r) { return (r + 29) / 32 * 32 + 2; }
-__host__ __device__ constexpr int b_stride(int c) { return (c + 15) / 32 * 32 + 16; }
-constexpr int kAs = kRows * a_stride(kChunk);
-constexpr int kInvBs = kChunk * b_stride(96);
-constexpr int kFwdBs = kChunk * b_stride(128);
+__host__ __device__ constexpr int bitrev(int k, int bits) {
+  int r = 0;
+  for (int i = 0; i < bits; i++) r = (r << 1) | ((k >> i) & 1);
+  return r;
+}
+__host__ __device__ constexpr int log2i(int n) { return n <= 1 ? 0 : 1 + log2i(n / 2); }
 
-// Chunk staging through registers: element e = threadIdx.x + kThreads j is
-// row e >> LOGC, column e & (2^LOGC - 1).
-constexpr int kAN = kRows * kChunk / kThreads;  // A chunk: 128 rows x 32
-constexpr int kBN = kChunk * 128 / kThreads;    // B chunk: 32 x <= 128
-static_assert(kRows * kChunk % kThreads == 0 && kChunk * 128 % kThreads == 0, "chunk staging");
+// Register index of input k of an M-point FFT: bit-reversed for a power of
+// two; for M = 96, sub-FFT r = k mod 3 at 32 r, its input k / 3 bit-reversed.
+template <int M>
+__device__ __forceinline__ constexpr int slot_of(int k) {
+  if constexpr ((M & (M - 1)) == 0)
+    return bitrev(k, log2i(M));
+  else
+    return (k % 3) * (M / 3) + bitrev(k / 3, log2i(M / 3));
+}
 
-template <int N, int LOGC, typename Load>
-__device__ __forceinline__ void load_regs(int rows, int cols, Load load, float* v) {
+// In-place S-point radix-2 DIT FFT (S a power of two) of re/im[off..off+S),
+// input bit-reversed, output natural; twiddle e^{sign 2 pi i j / S} is
+// tw[(j * (MT / S))] of the MT-point table (cos, sin interleaved).
+template <int S, int MT, int SIGN>
+__device__ __forceinline__ void fft2(float* re, float* im, int off, const float* tw) {
 #pragma unroll
-  for (int j = 0; j < N; j++) {
-    const int e = threadIdx.x + kThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
-    v[j] = (r < rows && c < cols) ? load(r, c) : 0.0f;
+  for (int size = 2; size <= S; size *= 2) {
+    const int h = size / 2;
+#pragma unroll
+    for (int start = 0; start < S; start += size) {
+#pragma unroll
+      for (int j = 0; j < h; j++) {
+        const int a = off + start + j, b = a + h;
+        float tr = re[b], ti = im[b];
+        if (j != 0) {
+          const int t = j * (MT / size);
+          const float wr = tw[2 * t], wi = SIGN * tw[2 * t + 1];
+          const float xr = tr * wr - ti * wi;
+          ti = tr * wi + ti * wr;
+          tr = xr;
+        }
+        re[b] = re[a] - tr;
+        im[b] = im[a] - ti;
+        re[a] = re[a] + tr;
+        im[a] = im[a] + ti;
+      }
+    }
   }
 }
 
-template <int N, int LOGC, typename Store>
-__device__ __forceinline__ void store_regs(int rows, int cols, Store store, const float* v) {
+// In-place M-point FFT with e^{SIGN 2 pi i / M}, input in slot_of order.
+template <int M, int SIGN>
+__device__ __forceinline__ void fft(float* re, float* im, const float* tw) {
+  if constexpr ((M & (M - 1)) == 0) {
+    fft2<M, M, SIGN>(re, im, 0, tw);
+  } else {
+    constexpr int S = M / 3;
+    fft2<S, M, SIGN>(re, im, 0, tw);
+    fft2<S, M, SIGN>(re, im, S, tw);
+    fft2<S, M, SIGN>(re, im, 2 * S, tw);
+    // z[n1 + S n2] = sum_r (W^{r n1} A_r[n1]) W3^{r n2}
+    const float h3 = SIGN * 0.866025403784438647f;  // sin(2 pi / 3)
 #pragma unroll
-  for (int j = 0; j < N; j++) {
-    const int e = threadIdx.x + kThreads * j, r = e >> LOGC, c = e & ((1 << LOGC) - 1);
-    if (r < rows && c < cols) store(r, c, v[j]);
+    for (int n1 = 0; n1 < S; n1++) {
+      const float t0r = re[n1], t0i = im[n1];
+      float t1r = re[S + n1], t1i = im[S + n1], t2r = re[2 * S + n1], t2i = im[2 * S + n1];
+      if (n1 != 0) {
+        const float w1r = tw[2 * n1], w1i = SIGN * tw[2 * n1 + 1];
+        const float w2r = tw[4 * n1], w2i = SIGN * tw[4 * n1 + 1];
+        const float a = t1r * w1r - t1i * w1i;
+        t1i = t1r * w1i + t1i * w1r;
+        t1r = a;
+        const float b = t2r * w2r - t2i * w2i;
+        t2i = t2r * w2i + t2i * w2r;
+        t2r = b;
+      }
+      const float sr = t1r + t2r, si = t1i + t2i, dr = t1r - t2r, di = t1i - t2i;
+      const float mr = t0r - 0.5f * sr, mi = t0i - 0.5f * si;
+      re[n1] = t0r + sr;
+      im[n1] = t0i + si;
+      re[S + n1] = mr - h3 * di;
+      im[S + n1] = mi + h3 * dr;
+      re[2 * S + n1] = mr + h3 * di;
+      im[2 * S + n1] = mi - h3 * dr;
+    }
   }
 }
 
-// A chunk staged with lanes over its 128 rows: element e = threadIdx.x +
-// kThreads j is row e % 128, column e / 128, so a wave reads one
-// coefficient of 64 consecutive rows, contiguous in a coefficient-major
-// Fourier slot.
-template <int N, typename Load>
-__device__ __forceinline__ void load_rows(Load load, float* v) {
-#pragma unroll
-  for (int j = 0; j < N; j++) {
-    const int e = threadIdx.x + kThreads * j;
-    v[j] = load(e & (kRows - 1), e / kRows);
-  }
-}
+// ------------------------------------------------------------- rows
 
-template <int N, typename Store>
-__device__ __forceinline__ void store_rows(Store store, const float* v) {
-#pragma unroll
-  for (int j = 0; j < N; j++) {
-    const int e = threadIdx.x + kThreads * j;
-    store(e & (kRows - 1), e / kRows, v[j]);
-  }
-}
-static_assert(kRows * kChunk == kAN * kThreads && (kRows & (kRows - 1)) == 0, "row staging");
+// Global-memory views of the arena: the row pointers pass through LDS,
+// which would otherwise leave the compiler generic (flat) accesses.
+typedef __attribute__((address_space(1))) float gfloat;
 
-// Coefficient index k of the j-th coefficient of the wavenumbers of one
-// parity (0: m = 0, 2, ..; 1: m = 1, 3, ..).
-__device__ __forceinline__ int parity_k(int parity, int j) { return 4 * (j >> 1) + 2 * parity + (j & 1); }
-// Coefficients of one parity: 2 x the number of such m <= T.
-__device__ __forceinline__ int parity_count(int parity, int ntp1) { return 2 * ((ntp1 + 1 - parity) / 2); }
+// Wavenumbers 0..T of the truncation whose grid has 2M longitudes
+// (T21: 64, T42: 128, T63: 192).
+template <int M>
+__device__ __forceinline__ constexpr int wavenumbers() {
+  return M == 32 ? 22 : M == 64 ? 43 : 64;
+}
 
 // Rows of an inverse job: outputs x planes x latitudes.
 __device__ __forceinline__ int inv_rows(const DevMember& M, const InvJob& job) {
@@ -120,227 +160,217 @@ __device__ __forceinline__ int locate(const Jobs& jobs, int per_member, int row,
   return j;
 }
 
-// acc[nt] += A[rows 16 wave .. +16][0..32) x B[0..32)[16 nt .. +16], nt < NT
-template <int MAXT>
-__device__ __forceinline__ void mfma_rows(const float* As, const float* Bs, int bs, int NT, f4* acc) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = lane & 15, kk = lane >> 4;
-  const float* a = As + (16 * wave + i) * a_stride(kChunk) + kk;
-  const float* b = Bs + kk * bs + i;
-#pragma unroll 2
-  for (int r = 0; r < kChunk; r += 4) {
-    const float av = a[r];
+// The truncation's twiddles into LDS: W_M^t (t < M), then w_N^k (k < M).
+__device__ __forceinline__ const float* twiddle_table(const DevMember& M) {
+  return M.ltab + M.tabo[TAB_fft_twiddle];
+}
+__device__ __forceinline__ void stage_twiddles(const DevMember& M, float* tw) {
+  const float* t = twiddle_table(M);
+  for (int i = threadIdx.x; i < 2 * M.nlon; i += kThreads) tw[i] = t[i];
+}
+
+// ---------------------------------------------------------- inverse
+
+// A lane past the launch's rows reads the twiddle table (stride 0) and
+// stores nothing (null destination).
+template <int M>
+__device__ __forceinline__ void inverse_row(const float* source, int stride, bool derived, float* const* dstp,
+                                            float* T, const float* tw) {
+  constexpr int N = 2 * M, T1 = wavenumbers<M>();
+  const gfloat* src = (const gfloat*)source;
+  const float* wn = tw + N;  // w^k
+  float re[M], im[M];
+  // X[m] of this row, zero past T
+  const auto x_of = [&](int m, float* xr, float* xi) {
+    if (m >= T1) {
+      *xr = 0.0f;
+      *xi = 0.0f;
+      return;
+    }
+    const float a = src[(2 * m) * stride], b = m == 0 ? 0.0f : src[(2 * m + 1) * stride];
+    *xr = derived ? -(float)m * b : a;  // i m X for d/dlambda
+    *xi = derived ? (float)m * a : b;
+    if (m == 0) *xi = 0.0f;
+  };
+  // Z[k] and Z[M - k] from X[k] and X[M - k]
 #pragma unroll
-    for (int nt = 0; nt < MAXT; nt++)
-      if (nt < NT) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b[r * bs + 16 * nt], acc[nt], 0, 0, 0);
+  for (int k = 0; k <= M / 2; k++) {
+    float ar, ai, br, bi;
+    x_of(k, &ar, &ai);
+    if (k == 0) {
+      br = 0.0f;
+      bi = 0.0f;
+    } else {
+      x_of(M - k, &br, &bi);
+    }
+    // Z[k] = (X[k] + X*[M-k]) + i w^k (X[k] - X*[M-k])
+    {
+      const float sr = ar + br, si = ai - bi, dr = ar - br, di = ai + bi;
+      const float wr = wn[2 * k], wi = wn[2 * k + 1];
+      const float pr = dr * wr - di * wi, pi = dr * wi + di * wr;
+      re[slot_of<M>(k)] = sr - pi;
+      im[slot_of<M>(k)] = si + pr;
+    }
+    if (k != 0 && k != M / 2) {
+      // Z[M-k] = (X[M-k] + X*[k]) + i w^{M-k} (X[M-k] - X*[k])
+      const int j = M - k;
+      const float sr = br + ar, si = bi - ai, dr = br - ar, di = bi + ai;
+      const float wr = wn[2 * j], wi = wn[2 * j + 1];
+      const float pr = dr * wr - di * wi, pi = dr * wi + di * wr;
+      re[slot_of<M>(j)] = sr - pi;
+      im[slot_of<M>(j)] = si + pr;
+    }
+  }
+  fft<M, 1>(re, im, tw);
+  // out through LDS in chunks of 32 values (16 complex): the wave then
+  // writes 2 rows x 32 values per access
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < N / 32; c++) {
+#pragma unroll
+    for (int n = 0; n < 16; n++) {
+      T[lane * kTs + 2 * n] = re[16 * c + n];
+      T[lane * kTs + 2 * n + 1] = im[16 * c + n];
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int i = 0; i < 32; i++) {
+      const int row = 2 * i + (lane >> 5), x = lane & 31;
+      gfloat* G = (gfloat*)dstp[row];
+      if (G) G[32 * c + x] = T[row * kTs + x];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------- forward
+
+// Every lane has a source row (a lane past the launch's rows reads the
+// twiddle table, 2N floats); only lanes with a destination store.
+template <int M>
+__device__ __forceinline__ void forward_row(const float* const* srcp, float* destination, int stride, float* T,
+                                            const float* tw) {
+  constexpr int N = 2 * M, T1 = wavenumbers<M>();
+  const float* wn = tw + N;
+  const int lane = threadIdx.x & 63;
+  float re[M], im[M];
+  // in through LDS in chunks of 32 values: z[n] = g[2n] + i g[2n+1]; each
+  // chunk's 32 loads go in two batches of 16 to bound the live registers
+#pragma unroll
+  for (int c = 0; c < N / 32; c++) {
+#pragma unroll
+    for (int h = 0; h < 32; h += 16) {
+      float v[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const gfloat* G = (const gfloat*)srcp[2 * (h + i) + (lane >> 5)];
+        v[i] = G[32 * c + (lane & 31)];
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) T[(2 * (h + i) + (lane >> 5)) * kTs + (lane & 31)] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 16; n++) {
+      re[slot_of<M>(16 * c + n)] = T[lane * kTs + 2 * n];
+      im[slot_of<M>(16 * c + n)] = T[lane * kTs + 2 * n + 1];
+    }
+    __syncthreads();
+  }
+  fft<M, -1>(re, im, tw);
+  if (!destination) return;
+  gfloat* dst = (gfloat*)destination;
+  const float scale = 1.0f / (float)N;
+  // X[k] = (Z[k] + Z*[M-k]) / 2 - i w^{-k} (Z[k] - Z*[M-k]) / 2, k <= T
+#pragma unroll
+  for (int k = 0; k < T1; k++) {
+    const int j = k == 0 ? 0 : M - k;
+    const float zr = re[k], zi = im[k], yr = re[j], yi = -im[j];  // y = Z*[M-k]
+    const float er = 0.5f * (zr + yr), ei = 0.5f * (zi + yi);
+    const float dr = 0.5f * (zr - yr), di = 0.5f * (zi - yi);
+    // -i w^{-k} d = -i (wr - i wi) d = (-wi - i wr) d
+    const float wr = wn[2 * k], wi = wn[2 * k + 1];
+    const float orr = -wi * dr + wr * di, oi = -wi * di - wr * dr;
+    dst[(2 * k) * stride] = (er + orr) * scale;
+    dst[(2 * k + 1) * stride] = (ei + oi) * scale;
   }
 }
 
 }  // namespace
 
-extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember* __restrict__ mems, GroupPlan plan,
-                                                                  InvJobs jobs) {
-  __shared__ float As[kAs];
-  __shared__ float Bs[kInvBs];
-  // per row of the block: its Fourier row (null past the group's rows),
-  // whether it is d/dlambda's, and its grid row
-  __shared__ const float* src[kRows];
+// One wave per SIMD: the 96-point rows hold 192 values a lane.
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1))) void k_dft_inv(
+    const DevMember* __restrict__ mems, GroupPlan plan, InvJobs jobs) {
+  __shared__ float tw[4 * kMaxM];
+  __shared__ float trans[64 * kTs];
   __shared__ float* dst[kRows];
-  __shared__ int derived[kRows];
-  __shared__ int stride[kRows];  // rows of the row's Fourier slot: coefficient k at src + k * stride
   int block;
   const int grp = group_find(plan, blockIdx.x, &block);
   const DevMember& M = mems[plan.list[plan.start[grp]]];
-  const int nlat = M.nlat, nlon = M.nlon, ntp1 = M.ntru + 1;
+  const int nlat = M.nlat, nlon = M.nlon;
   const auto rows_of = [&](const InvJob& x) { return inv_rows(M, x); };
   int per_member = 0;
   for (int j = 0; j < jobs.n; j++) per_member += rows_of(jobs.j[j]);
   const int total = plan.count[grp] * per_member;
-  const int row0 = block * kRows;
-  const int half = nlon / 2, NT = half / 16;
-  const int bs = b_stride(half);
-  const int KE = parity_count(0, ntp1), KO = parity_count(1, ntp1);
-  const int CE = (KE + kChunk - 1) / kChunk, chunks = CE + (KO + kChunk - 1) / kChunk;
-  const float* inv = M.ltab + M.tabo[TAB_fourier_inverse];
-  if (threadIdx.x < kRows) {
-    const int row = row0 + threadIdx.x;
-    const float* s = nullptr;
-    float* d = nullptr;
-    int dl = 0, st = 0;
-    if (row < total) {
-      int g, r;
-      const InvJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
-      const DevMember& G = mems[plan.list[plan.start[grp] + g]];
-      const int plane_rows = (job.per_level ? M.nlev : 1) * nlat;  // rows of one output
-      const int o = r / plane_rows, pr = r - o * plane_rows;
-      s = G.arena + G.slot[(job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0] + pr;
-      d = G.arena + G.slot[o == 0 ? job.dst0 : job.dst1] + pr * nlon;
-      dl = job.kind == INV_DLAMBDA && o == 1;
-      st = plane_rows;
-    }
-    src[threadIdx.x] = s;
-    dst[threadIdx.x] = d;
-    derived[threadIdx.x] = dl;
-    stride[threadIdx.x] = st;
+  const int row = block * kRows + threadIdx.x;
+  const float* src = twiddle_table(M);
+  float* d = nullptr;
+  int stride = 0;
+  bool derived = false;
+  if (row < total) {
+    int g, r;
+    const InvJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
+    const DevMember& G = mems[plan.list[plan.start[grp] + g]];
+    const int plane_rows = (job.per_level ? M.nlev : 1) * nlat;  // rows of one output
+    const int o = r / plane_rows, pr = r - o * plane_rows;
+    src = G.arena + G.slot[(job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0] + pr;
+    d = G.arena + G.slot[o == 0 ? job.dst0 : job.dst1] + pr * nlon;
+    derived = job.kind == INV_DLAMBDA && o == 1;
+    stride = plane_rows;
   }
+  dst[threadIdx.x] = d;
+  stage_twiddles(M, tw);
   __syncthreads();
-  // chunk c: parity c >= CE, coefficients j0 .. j0 + 32 of that parity
-  const auto a_load = [&](int c) {
-    const int parity = c >= CE, j0 = kChunk * (parity ? c - CE : c), n = parity ? KO : KE;
-    return [=](int r, int x) {
-      const int j = j0 + x;
-      const float* F = src[r];
-      if (!F || j >= n) return 0.0f;
-      const int k = parity_k(parity, j), R = stride[r];
-      if (derived[r]) {
-        const float m = (float)(k >> 1);
-        return (k & 1) ? m * F[(k - 1) * R] : -m * F[(k + 1) * R];
-      }
-      return F[k * R];
-    };
-  };
-  const auto b_load = [&](int c) {
-    const int parity = c >= CE, j0 = kChunk * (parity ? c - CE : c), n = parity ? KO : KE;
-    return [=](int y, int x) { return j0 + y < n ? inv[parity_k(parity, j0 + y) * nlon + x] : 0.0f; };
-  };
-  f4 even[kInvTiles], odd[kInvTiles];
-#pragma unroll
-  for (int nt = 0; nt < kInvTiles; nt++) even[nt] = odd[nt] = f4{0, 0, 0, 0};
-  float va[kAN], vb[kBN];
-  load_rows<kAN>(a_load(0), va);
-  load_regs<kBN, 7>(kChunk, half, b_load(0), vb);
-  for (int c = 0; c < chunks; c++) {
-    store_rows<kAN>([&](int r, int x, float v) { As[r * a_stride(kChunk) + x] = v; }, va);
-    store_regs<kBN, 7>(kChunk, half, [&](int y, int x, float v) { Bs[y * bs + x] = v; }, vb);
-    __syncthreads();
-    if (c + 1 < chunks) {
-      load_rows<kAN>(a_load(c + 1), va);
-      load_regs<kBN, 7>(kChunk, half, b_load(c + 1), vb);
-    }
-    mfma_rows<kInvTiles>(As, Bs, bs, NT, c < CE ? even : odd);
-    __syncthreads();
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    float* G = dst[16 * wave + 4 * (lane >> 4) + r];
-    if (G) {
-#pragma unroll
-      for (int nt = 0; nt < kInvTiles; nt++) {
-        if (nt < NT) {
-          const int x = 16 * nt + (lane & 15);
-          G[x] = even[nt][r] + odd[nt][r];
-          G[x + half] = even[nt][r] - odd[nt][r];
-        }
-      }
-    }
-  }
+  if (nlon == 64)
+    inverse_row<32>(src, stride, derived, dst, trans, tw);
+  else if (nlon == 128)
+    inverse_row<64>(src, stride, derived, dst, trans, tw);
+  else
+    inverse_row<96>(src, stride, derived, dst, trans, tw);
 }
 
-extern "C" __global__ __launch_bounds__(kThreads) void k_dft_fwd(const DevMember* __restrict__ mems, GroupPlan plan,
-                                                                  FwdJobs jobs) {
-  // A: the half-row sums (even m) and differences (odd m); B: the table's
-  // coefficients of even m in columns 0..63 and of odd m in 64..127
-  __shared__ float As[2 * kAs];
-  __shared__ float Bs[kFwdBs];
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1))) void k_dft_fwd(
+    const DevMember* __restrict__ mems, GroupPlan plan, FwdJobs jobs) {
+  __shared__ float tw[4 * kMaxM];
+  __shared__ float trans[64 * kTs];
   __shared__ const float* src[kRows];
-  __shared__ float* dst[kRows];
-  __shared__ int stride[kRows];  // rows of the row's Fourier slot
   int block;
   const int grp = group_find(plan, blockIdx.x, &block);
   const DevMember& M = mems[plan.list[plan.start[grp]]];
-  const int nlon = M.nlon, nfc = M.nfc, ntp1 = M.ntru + 1;
+  const int nlon = M.nlon;
   const auto rows_of = [&](const FwdJob& x) { return fwd_rows(M, x); };
   int per_member = 0;
   for (int j = 0; j < jobs.n; j++) per_member += rows_of(jobs.j[j]);
   const int total = plan.count[grp] * per_member;
-  const int row0 = block * kRows;
-  const int half = nlon / 2;  // a multiple of 32
-  const int KE = parity_count(0, ntp1), KO = parity_count(1, ntp1);
-  const int NTE = (KE + 15) / 16, NTO = (KO + 15) / 16;
-  constexpr int bs = b_stride(128);
-  const float* fwd = M.ltab + M.tabo[TAB_fourier_forward];
-  if (threadIdx.x < kRows) {
-    const int row = row0 + threadIdx.x;
-    const float* s = nullptr;
-    float* d = nullptr;
-    int st = 0;
-    if (row < total) {
-      int g, r;
-      const FwdJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
-      const DevMember& G = mems[plan.list[plan.start[grp] + g]];
-      s = G.arena + G.slot[job.src] + r * nlon;
-      d = G.arena + G.slot[job.dst] + r;
-      st = rows_of(job);
-    }
-    src[threadIdx.x] = s;
-    dst[threadIdx.x] = d;
-    stride[threadIdx.x] = st;
+  const int row = block * kRows + threadIdx.x;
+  const float* s = twiddle_table(M);
+  float* d = nullptr;
+  int stride = 0;
+  if (row < total) {
+    int g, r;
+    const FwdJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
+    const DevMember& G = mems[plan.list[plan.start[grp] + g]];
+    s = G.arena + G.slot[job.src] + r * nlon;
+    d = G.arena + G.slot[job.dst] + r;
+    stride = rows_of(job);
   }
+  src[threadIdx.x] = s;
+  stage_twiddles(M, tw);
   __syncthreads();
-  const auto g_load = [&](int x0, int shift) {
-    return [=](int r, int x) {
-      const float* g = src[r];
-      return g ? g[x0 + shift + x] : 0.0f;
-    };
-  };
-  const auto b_load = [&](int x0) {
-    return [=](int y, int j) {
-      const int parity = j >> 6, jj = j & 63;
-      return jj < (parity ? KO : KE) ? fwd[(x0 + y) * nfc + parity_k(parity, jj)] : 0.0f;
-    };
-  };
-  f4 even[kFwdTiles / 2], odd[kFwdTiles / 2];
-#pragma unroll
-  for (int nt = 0; nt < kFwdTiles / 2; nt++) even[nt] = odd[nt] = f4{0, 0, 0, 0};
-  float va[kAN], vc[kAN], vb[kBN];
-  load_regs<kAN, 5>(kRows, kChunk, g_load(0, 0), va);
-  load_regs<kAN, 5>(kRows, kChunk, g_load(0, half), vc);
-  load_regs<kBN, 7>(kChunk, 128, b_load(0), vb);
-  for (int x0 = 0; x0 < half; x0 += kChunk) {
-#pragma unroll
-    for (int j = 0; j < kAN; j++) {
-      const float a = va[j], c = vc[j];
-      va[j] = a + c;
-      vc[j] = a - c;
-    }
-    store_regs<kAN, 5>(kRows, kChunk, [&](int r, int x, float v) { As[r * a_stride(kChunk) + x] = v; }, va);
-    store_regs<kAN, 5>(kRows, kChunk, [&](int r, int x, float v) { As[kAs + r * a_stride(kChunk) + x] = v; }, vc);
-    store_regs<kBN, 7>(kChunk, 128, [&](int y, int j, float v) { Bs[y * bs + j] = v; }, vb);
-    __syncthreads();
-    if (x0 + kChunk < half) {
-      load_regs<kAN, 5>(kRows, kChunk, g_load(x0 + kChunk, 0), va);
-      load_regs<kAN, 5>(kRows, kChunk, g_load(x0 + kChunk, half), vc);
-      load_regs<kBN, 7>(kChunk, 128, b_load(x0 + kChunk), vb);
-    }
-    mfma_rows<kFwdTiles / 2>(As, Bs, bs, NTE, even);
-    mfma_rows<kFwdTiles / 2>(As + kAs, Bs + 64, bs, NTO, odd);
-    __syncthreads();
-  }
-  // Out through LDS, one parity at a time: the accumulators go to T[j][row]
-  // (rows padded to 129 floats), then each wave writes one coefficient of
-  // 64 consecutive rows, contiguous in the coefficient-major Fourier slot.
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float* Tt = As;
-  static_assert(64 * (kRows + 1) <= 2 * kAs, "transpose buffer");
-#pragma unroll
-  for (int parity = 0; parity < 2; parity++) {
-    const int K = parity ? KO : KE, NT = parity ? NTO : NTE;
-#pragma unroll
-    for (int nt = 0; nt < kFwdTiles / 2; nt++) {
-      if (nt < NT) {
-        const int j = 16 * nt + (lane & 15);
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          Tt[j * (kRows + 1) + 16 * wave + 4 * (lane >> 4) + r] = parity ? odd[nt][r] : even[nt][r];
-      }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < K * kRows; e += kThreads) {
-      const int j = e / kRows, row = e & (kRows - 1);
-      float* F = dst[row];
-      if (F) F[parity_k(parity, j) * stride[row]] = Tt[j * (kRows + 1) + row];
-    }
-    __syncthreads();  // T is rewritten by the next parity
-  }
+  if (nlon == 64)
+    forward_row<32>(src, d, stride, trans, tw);
+  else if (nlon == 128)
+    forward_row<64>(src, d, stride, trans, tw);
+  else
+    forward_row<96>(src, d, stride, trans, tw);
 }

@@ -5,9 +5,9 @@
 // gp2fc/fc2sp pairs do, with the Fourier coefficients in the arena's
 // Fourier slots between them:
 //   inverse: k_leg_inv (one workgroup per zonal wavenumber m and job,
-//            kernels_legendre.hip) then k_dft_inv (64 latitude rows of one
-//            job per workgroup, kernels_dft.hip);
-//   forward: k_dft_fwd (64 rows) then k_leg_fwd (per m).
+//            kernels_legendre.hip) then k_dft_inv (real FFTs, one latitude
+//            row per lane, kernels_dft.hip);
+//   forward: k_dft_fwd (real FFTs) then k_leg_fwd (per m).
 #pragma once
 #include "device.h"
 
