@@ -272,11 +272,11 @@ class Stepper {
   }
 
   // The zonal transforms run in FP32 for both forms: every member, rows(d)
-  // rows per member, 64 rows (one a lane) per one-wave workgroup.
+  // rows per member, 32 rows (two lanes a row) per one-wave workgroup.
   template <typename F>
   GroupPlan dft_plan(F rows) const {
     return group_plan([](const DevMember&) { return true; },
-                      [&](const DevMember& d, int count) { return (count * rows(d) + 63) / 64; });
+                      [&](const DevMember& d, int count) { return (count * rows(d) + 31) / 32; });
   }
 
   void column(void (*k)(const DevMember*, LaunchPlan)) {
@@ -323,7 +323,7 @@ class Stepper {
     if (pg.first[pg.n] > 0) launch(e_, k_leg_inv, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
     const LaunchPlan pl = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
     if (pl.first[pl.n] > 0) launch(e_, k_leg_inv_split, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
-    // k_dft_inv: 64 rows (member x job x output x level x latitude) of a
+    // k_dft_inv: 32 rows (member x job x output x level x latitude) of a
     // group per workgroup, over every member of both forms
     const GroupPlan pd = dft_plan([&](const DevMember& d) {
       int rows = 0;
@@ -334,7 +334,7 @@ class Stepper {
     launch(e_, k_dft_inv, dim3(pd.first[pd.n]), dim3(64), e_.d_desc, pd, jobs);
   }
   void fwd(const FwdJobs& jobs) {
-    // 64 rows (member x job x level x latitude) of a group per workgroup
+    // 32 rows (member x job x level x latitude) of a group per workgroup
     const GroupPlan p = dft_plan([&](const DevMember& d) {
       int rows = 0;
       for (int j = 0; j < jobs.n; j++) rows += job_planes(jobs.j[j].per_level, d.nlev) * d.nlat;
