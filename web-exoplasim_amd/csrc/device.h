@@ -72,6 +72,18 @@ __device__ __forceinline__ int group_find(const GroupPlan& p, int block, int* ti
   return lo;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
+// b mod 8). Within each run of 64 blocks, XCD x is given the 8 consecutive
+// tiles 8x..8x+7 instead, so neighbouring tiles, which share cache lines
+// (of the Fourier rows of neighbouring wavenumbers, or of neighbouring
+// rows' coefficient runs), meet in one L2 while heavy and light tiles stay
+// spread over all XCDs. A last partial run keeps the plain order.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int run = b >> 6;
+  if ((run + 1) * 64 > n) return b;
+  return (run << 6) + (b & 7) * 8 + ((b >> 3) & 7);
+}
+
 #define SLOTP(M, name) ((M).arena + (M).slot[SLOT_##name])
 #define TABP(M, name) ((M).tab + (M).tabo[TAB_##name])
 #define KON(M, name) ((M).k[CON_##name])

@@ -36,7 +36,8 @@
 // coefficient access is two contiguous runs of 32 rows per wave. Grid rows
 // (longitude fastest, N floats) pass through LDS: the inverse writes 128-byte
 // runs of one row per half wave, the forward reads 256-byte runs of one row
-// per wave access.
+// per wave access. Neighbouring row blocks share the cache lines at their
+// coefficient runs' ends, so xcd_block gives them to one XCD's L2.
 #include "transforms.h"
 
 namespace {
@@ -450,7 +451,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_inv(const DevMember
   __shared__ float trans[64 * kTs];
   __shared__ float* dst[kRows];
   int block;
-  const int grp = group_find(plan, blockIdx.x, &block);
+  const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &block);
   const int nlon = mems[plan.list[plan.start[grp]]].nlon;
   if (nlon == 64)
     dft_inv<32>(mems, plan, jobs, grp, block, trans, dst);
@@ -465,7 +466,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_dft_fwd(const DevMember
   __shared__ float stage[kRows * kLs];
   __shared__ const float* src[kRows];
   int block;
-  const int grp = group_find(plan, blockIdx.x, &block);
+  const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &block);
   const int nlon = mems[plan.list[plan.start[grp]]].nlon;
   if (nlon == 64)
     dft_fwd<32>(mems, plan, jobs, grp, block, stage, src);

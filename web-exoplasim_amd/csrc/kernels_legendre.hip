@@ -63,18 +63,6 @@ __host__ __device__ constexpr int a_stride(int r) { return (r + 29) / 32 * 32 + 
 constexpr int kRawLat = 2 * kMaxLat + 4;
 constexpr int kFwdAs = kMaxK * a_stride(kMaxLat / 2);
 
-// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
-// b mod 8). Within each run of 64 blocks, XCD x is given the 8 consecutive
-// tiles 8x..8x+7 instead, so the tiles of neighbouring wavenumbers, which
-// share cache lines of the Fourier rows, meet in one L2 while the heavy
-// and light tiles stay spread over all XCDs. A last partial run keeps the
-// plain order.
-__device__ __forceinline__ int xcd_block(int b, int n) {
-  const int run = b >> 6;
-  if ((run + 1) * 64 > n) return b;
-  return (run << 6) + (b & 7) * 8 + ((b >> 3) & 7);
-}
-
 // LDS staging of a rows x cols block: lane = column (in runs of 64), the W
 // waves take rows r0, r0 + W, ...; each lane issues 8 loads before it
 // stores any, so one memory latency covers 8 rows.

@@ -14,7 +14,6 @@
   X(legendre_qm, NLAT * NCSP) X(legendre_qq, NLAT * NCSP)                      \
   X(legendre_qi, NLAT * NCSP) X(legendre_qj, NLAT * NCSP)                      \
   X(legendre_qu, NLAT * NCSP) X(legendre_qv, NLAT * NCSP)                      \
-  X(fft_twiddle, 2 * NLON)                                                     \
   X(zgw, NHOR)                                                                 \
   X(flux_sigfac, 1) X(flux_lnsig, 1) X(vdiff_skap, L) X(vdiff_skaph, L)        \
   X(vdiff_mixh2, L) X(vdiff_mixm2, L) X(vdiff_rdsig, L) X(vdiff_sigmah3, L)    \

@@ -988,18 +988,6 @@ void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fie
       }
     }
   }
-  // The real FFTs' twiddles (kernels_dft.hip): e^{2 pi i t / M} for t < M
-  // = NLON / 2, then e^{2 pi i k / NLON} for k < M, (cos, sin) pairs.
-  {
-    double* tw = tab(TAB_fft_twiddle);
-    const int half = NLON / 2;
-    for (int t = 0; t < half; t++) {
-      tw[2 * t] = std::cos(2 * M_PI * t / half);
-      tw[2 * t + 1] = std::sin(2 * M_PI * t / half);
-      tw[NLON + 2 * t] = std::cos(2 * M_PI * t / NLON);
-      tw[NLON + 2 * t + 1] = std::sin(2 * M_PI * t / NLON);
-    }
-  }
   // LEGENDRE.kernelTables (legendre/index.ts:202-221)
   const Legendre leg = legini(grid, T);
   std::vector<double> g2s, s2g;
