@@ -6,9 +6,11 @@ pressure and land fraction) advanced together, one batched step per
 "step", state resident in HBM. `value` is the aggregate simulated days/s
 of the batch. Beside it, the plain FP32 single-run form (one T63 L10
 planet, 30-minute step) is timed on the same GPU for the batched-vs-plain
-ratio, the dominant kernel's launches are timed with HIP events for the
-roofline figure, and the FP64 CPU oracle is timed on a bounded sample of
-the same workload. Prints one JSON line.
+ratio, the same planets are stepped one at a time (each alone) for the
+batched-vs-one-at-a-time ratio, the dominant kernel's launches are timed
+with HIP events for the roofline figure, and the FP64 CPU oracle is timed
+on the host's cores (one planet per thread) on a bounded sample of the
+same workload. Prints one JSON line.
 
 The serving loop (continuous batching of Poisson-arriving runs, 4 runs/s,
 T21/T42/T63 weights 0.5/0.35/0.15, L5 or L10, 1-360 simulated days each,
@@ -19,7 +21,7 @@ line's "serving" object; --workload serving makes it the measured value.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
                     [--workload batched|plain|serving] [--probe KERNEL]
                     [--serve-seconds S] [--no-serving]
-                    [--no-cpu-baseline] [--no-plain]
+                    [--no-cpu-baseline] [--cpu-cores C] [--no-plain]
 """
 import argparse
 import json
@@ -86,23 +88,34 @@ def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True, tune=Fals
         eng.close()
 
 
-def cpu_baseline(exo, cfgs, seconds_per_case):
-    """FP64 oracle steps on the host, one planet of each (truncation,
-    levels) group of the workload, scaled to the workload's aggregate
-    simulated days/s on one core."""
+def cpu_cores():
+    """Host cores the CPU baseline may use: OMP_NUM_THREADS where set (the
+    box's CPU share), else the cores this process may run on, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(exo, cfgs, seconds_per_case, cores):
+    """FP64 oracle steps on the host cores: for each (truncation, levels)
+    group of the workload, `cores` planets of the group step at once, one
+    per thread (the oracle's C step runs outside the GIL), after their
+    start-up steps; the per-step time under that full load, times the
+    group's planets over the cores, gives the workload's aggregate
+    simulated days/s on `cores` cores."""
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_harness import OracleRun
     groups = {}
     for c in cfgs:
         groups.setdefault((c.truncation, c.levels, c.timestep_minutes), []).append(c)
-    host = exo.Engine(-1, len(groups))
-    cpu_s_per_day = 0.0
-    days = 0.0
-    sample = []
-    try:
-        for (trunc, levels, minutes), members in groups.items():
-            m = host.add_member(members[0])
-            run = OracleRun(host, m)
+
+    def timed_run(cfg):
+        host = exo.Engine(-1, 1)
+        try:
+            run = OracleRun(host, host.add_member(cfg))
+            run.step(run.nstartup, physics=True)  # the start-up phases, untimed
             n = 0
             t0 = time.perf_counter()
             while True:
@@ -110,17 +123,47 @@ def cpu_baseline(exo, cfgs, seconds_per_case):
                 n += 1
                 el = time.perf_counter() - t0
                 if el >= seconds_per_case or n >= 64:
-                    break
-            per_step = el / n
-            spd = 1440.0 / minutes
-            cpu_s_per_day += per_step * spd * len(members)
-            days += len(members)
-            sample.append(f"T{trunc} L{levels}: {n} steps, {per_step * 1e3:.1f} ms/step")
-    finally:
-        host.close()
-    return {"value": days / cpu_s_per_day, "unit": "simulated days/s", "cores": 1, "kind": "port",
-            "sample": "FP64 C oracle (oracle/liboracle.so), one planet per group of the workload, "
-                      "scaled by group size: " + "; ".join(sample)}
+                    return el / n, n
+        finally:
+            host.close()
+
+    cpu_s_per_day = 0.0
+    days = 0.0
+    sample = []
+    for (trunc, levels, minutes), members in groups.items():
+        picks = [members[i % len(members)] for i in range(cores)]
+        with ThreadPoolExecutor(cores) as pool:
+            res = list(pool.map(timed_run, picks))
+        per_step = sum(r[0] for r in res) / len(res)
+        spd = 1440.0 / minutes
+        cpu_s_per_day += per_step * spd * len(members) / cores
+        days += len(members)
+        sample.append(f"T{trunc} L{levels}: {cores} planets at once, {min(r[1] for r in res)}+ main-loop steps "
+                      f"each, {per_step * 1e3:.1f} ms/step per planet")
+    return {"value": days / cpu_s_per_day, "unit": "simulated days/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 C oracle (oracle/liboracle.so) on {cores} host threads, each group of the workload "
+                      "sampled with one planet per thread after the start-up steps, scaled by group size: " +
+                      "; ".join(sample)}
+
+
+def run_one_at_a_time(exo, cfgs, precision, steps=32, warmup=4):
+    """The same planets stepped one at a time on the GPU (each alone in the
+    engine, its step graph replayed): simulated days over the summed time."""
+    from web_exoplasim_amd import workloads
+    total_s, days = 0.0, 0.0
+    with exo.Engine(0, 1) as eng:
+        for c in cfgs:
+            m = eng.add_member(c, precision=precision)
+            eng.step(warmup)
+            eng.sync()
+            t0 = time.perf_counter()
+            eng.step(steps)
+            eng.sync()
+            total_s += time.perf_counter() - t0
+            days += steps / workloads.steps_per_day(c.timestep_minutes)
+            eng.remove_member(m)
+    return {"value": days / total_s, "unit": "simulated days/s", "steps_per_planet": steps,
+            "config": "the same planets, each stepped alone (after its start-up steps), times summed"}
 
 
 def run_serving(exo, seconds, rate=4.0, seed=5):
@@ -182,6 +225,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default transform group size (4 members) instead of tuning it")
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
+    ap.add_argument("--cpu-cores", type=int, default=0, help="host threads of the CPU baseline (default: the "
+                    "box's CPU share, OMP_NUM_THREADS)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (the known-size kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -266,6 +311,10 @@ def main():
         line["batched_vs_plain"] = value / plain_value
         if "batched_split_fp16" in line:
             line["batched_split_vs_plain"] = line["batched_split_fp16"]["value"] / plain_value
+    if args.workload == "batched" and not args.no_plain:
+        one = run_one_at_a_time(exo, cfgs, precision)
+        line["one_at_a_time"] = one
+        line["batched_vs_one_at_a_time"] = value / one["value"]
     if args.workload == "batched" and not args.no_serving:
         line["serving"] = run_serving(exo, args.serve_seconds)
     if probed:
@@ -274,7 +323,8 @@ def main():
                                            probe_steps(args.steps))
     else:
         line["roofline"] = None
-    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(exo, cfgs, args.cpu_seconds)
+    line["cpu_baseline"] = (None if args.no_cpu_baseline else
+                            cpu_baseline(exo, cfgs, args.cpu_seconds, args.cpu_cores or cpu_cores()))
     print(json.dumps(line))
 
 

@@ -22,8 +22,8 @@ extern "C" __global__ void k_leg_inv(const DevMember*, GroupPlan, InvJobs);
 extern "C" __global__ void k_dft_inv(const DevMember*, GroupPlan, InvJobs);
 extern "C" __global__ void k_dft_fwd(const DevMember*, GroupPlan, FwdJobs);
 extern "C" __global__ void k_leg_fwd(const DevMember*, GroupPlan, LegJobs);
-extern "C" __global__ void k_leg_inv_split(const DevMember*, LaunchPlan, InvJobs);
-extern "C" __global__ void k_leg_fwd_split(const DevMember*, LaunchPlan, LegJobs);
+extern "C" __global__ void k_leg_inv_split(const DevMember*, GroupPlan, InvJobs);
+extern "C" __global__ void k_leg_fwd_split(const DevMember*, GroupPlan, LegJobs);
 extern "C" __global__ void k_mkdheat(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_mean_dekin(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_load_step(const DevMember*, LaunchPlan, const float*);
@@ -315,14 +315,14 @@ class Stepper {
     launch(e_, k_mkdheat, dim3(p.first[p.n]), dim3(64), e_.d_desc, p);
   }
   void inv(const InvJobs& jobs) {
-    // the Legendre half per operand form: plain FP32 over member groups,
-    // split FP16 per member
-    // one workgroup per wavenumber pair (m, T - m) and job
-    const GroupPlan pg = group_plan([](const DevMember& d) { return d.precision == 0; },
-                                    [&](const DevMember& d, int) { return (d.ntru + 2) / 2 * jobs.n; });
-    if (pg.first[pg.n] > 0) launch(e_, k_leg_inv, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
-    const LaunchPlan pl = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
-    if (pl.first[pl.n] > 0) launch(e_, k_leg_inv_split, dim3(pl.first[pl.n]), dim3(256), e_.d_desc, pl, jobs);
+    // the Legendre half per operand form (plain FP32, split FP16), each over
+    // member groups: one workgroup per wavenumber pair (m, T - m) and job
+    for (int form = 0; form < 2; form++) {
+      const GroupPlan pg = group_plan([form](const DevMember& d) { return d.precision == form; },
+                                      [&](const DevMember& d, int) { return (d.ntru + 2) / 2 * jobs.n; });
+      if (pg.first[pg.n] > 0)
+        launch(e_, form ? k_leg_inv_split : k_leg_inv, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
+    }
     // k_dft_inv: 32 rows (member x job x output x level x latitude) of a
     // group per workgroup, over every member of both forms
     const GroupPlan pd = dft_plan([&](const DevMember& d) {
@@ -343,11 +343,14 @@ class Stepper {
     launch(e_, k_dft_fwd, dim3(p.first[p.n]), dim3(64), e_.d_desc, p, jobs);
   }
   void leg(const LegJobs& jobs) {
-    const GroupPlan pg = group_plan([](const DevMember& d) { return d.precision == 0; },
-                                    [&](const DevMember& d, int) { return (d.ntru + 1) * jobs.n; });
-    if (pg.first[pg.n] > 0) launch(e_, k_leg_fwd, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
-    const LaunchPlan p = plan([&](const DevMember& d) { return d.precision == 1 ? (d.ntru + 1) * jobs.n : 0; });
-    if (p.first[p.n] > 0) launch(e_, k_leg_fwd_split, dim3(p.first[p.n]), dim3(256), e_.d_desc, p, jobs);
+    // per operand form, over member groups: one workgroup per wavenumber
+    // and job
+    for (int form = 0; form < 2; form++) {
+      const GroupPlan pg = group_plan([form](const DevMember& d) { return d.precision == form; },
+                                      [&](const DevMember& d, int) { return (d.ntru + 1) * jobs.n; });
+      if (pg.first[pg.n] > 0)
+        launch(e_, form ? k_leg_fwd_split : k_leg_fwd, dim3(pg.first[pg.n]), dim3(512), e_.d_desc, pg, jobs);
+    }
   }
   void load_step(const float* d_params) {
     const LaunchPlan p = plan([](const DevMember&) { return 1; });

@@ -455,12 +455,13 @@ extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember*
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // Fourier mode 0 of each row, weighted by its latitude's qc (mode (0, 0)
   // is even about the equator: a southern row takes its northern twin's);
-  // waves take rows, lanes points
-  __shared__ float part[1024];
-  float s = 0.0f;
+  // waves take rows, lanes points. The planet's dissipated energy is summed
+  // in FP64 and rounded to FP32 once.
+  __shared__ double part[1024];
+  double s = 0.0;
   for (int lat = wave; lat < nlat; lat += 16) {
     const float w = qc[(lat < half ? lat : nlat - 1 - lat) * ncsp];
-    for (int x = lane; x < nlon; x += 64) s += w * (zdekin[lat * nlon + x] * fwd[x * nfc]);
+    for (int x = lane; x < nlon; x += 64) s += (double)(w * (zdekin[lat * nlon + x] * fwd[x * nfc]));
   }
   part[threadIdx.x] = s;
   __syncthreads();
@@ -468,7 +469,7 @@ extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember*
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  const float S = part[0];
+  const float S = (float)part[0];
   // zsdef keeps only Re mode (0, 0)
   float* zsdef = A + M.slot[SLOT_zsdef] + l * nrsp;
   for (int c = threadIdx.x; c < nrsp; c += 1024) zsdef[c] = c == 0 ? S : 0.0f;

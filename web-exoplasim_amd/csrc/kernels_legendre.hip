@@ -130,77 +130,6 @@ __device__ __forceinline__ Col fwd_col(int kind, int blk, int c, int NC, int NCm
   return gcol(g, nf, P, 0, p, q ? 0 : 1, q ? -1.0f : 1.0f, kRawLat);
 }
 
-// ---------------------------------------------- split-operand form
-// With M.precision = 1 the products run on v_mfma_f32_16x16x32_f16 with
-// every operand split into FP16 hi + lo parts, x s = hi + lo (hi the
-// nearest FP16, lo the FP16 of the remainder), as hi.hi + hi.lo + lo.hi
-// with FP32 accumulation: 22 significant bits per operand against FP32's
-// 24. The scale s is a power of two per Legendre table (set once, from
-// the whole table) and per staged input row (level and part of one field
-// at wavenumber m), chosen so the largest magnitude lands in
-// [2^12, 2^13), well inside FP16's range; the sums are unscaled exactly.
-// Operands are split once, while staged, into hi and lo arrays laid out
-// so each lane's 8 reduction values are one 16-byte LDS read.
-
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ void split1(float x, _Float16* hi, _Float16* lo) {
-  const _Float16 h = (_Float16)x;
-  *hi = h;
-  *lo = (_Float16)(x - (float)h);
-}
-
-// Power of two putting a largest magnitude in [2^12, 2^13); 1 for zero.
-__device__ __forceinline__ float pow2_scale(float maxabs) {
-  if (!(maxabs > 0.0f)) return 1.0f;
-  int e;
-  frexpf(maxabs, &e);
-  e = e < -100 ? -100 : e;
-  return ldexpf(1.0f, 13 - e);
-}
-
-// Split staging of input rows: each wave owns rows fp = wave, wave + 4, ...
-// (field and level), its lanes walk the row's `cols` values x = 2 v + part
-// (v = mode or latitude); the row's scale per part comes from a wave
-// reduction over the lanes of that part, then the values go to
-// hi/lo[(fp * 2 + part) * stride + v].
-template <typename Load>
-__device__ __forceinline__ void stage_rows_split(int rows, int cols, int stride, Load load, _Float16* hi,
-                                                 _Float16* lo, float* scale) {
-  const int lane = threadIdx.x & 63;
-  for (int fp = threadIdx.x >> 6; fp < rows; fp += 4) {
-    float v[3];
-    float mx = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const int x = lane + 64 * j;
-      v[j] = x < cols ? load(fp, x) : 0.0f;
-      mx = fmaxf(mx, fabsf(v[j]));
-    }
-    // lanes of one parity hold one part
-#pragma unroll
-    for (int o = 32; o > 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    const float s = pow2_scale(mx);
-    if (lane < 2) scale[fp * 2 + lane] = s;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const int x = lane + 64 * j;
-      if (x < cols) {
-        const int at = (fp * 2 + (x & 1)) * stride + (x >> 1);
-        split1(v[j] * s, hi + at, lo + at);
-      }
-    }
-  }
-}
-
-__device__ __forceinline__ h8 load_h8(const _Float16* p) { return *reinterpret_cast<const h8*>(p); }
-
-__device__ __forceinline__ f4 mfma_split(const h8& ahi, const h8& alo, const h8& bhi, const h8& blo, f4 c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blo, c, 0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, c, 0, 0, 0);
-}
-
 // ------------------------------------------------ member groups (FP32)
 // Members of one truncation and level count share the transform tables, so
 // one workgroup (8 waves) serves a group of them: it stages the table
@@ -547,140 +476,279 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   }
 }
 
-// ------------------------------------------------- split-operand bodies
+// ---------------------------------------------- split-operand form (groups)
+// With M.precision = 1 the products run on v_mfma_f32_16x16x16_f16 with
+// every operand split into FP16 hi + lo parts, x s = hi + lo (hi the
+// nearest FP16, lo the FP16 of the remainder), as hi.hi + hi.lo + lo.hi
+// with FP32 accumulation: 22 significant bits per operand against FP32's
+// 24. The scale s is a power of two per Legendre table (set once, from
+// the whole table) and per staged input row (level and part of one field
+// at wavenumber m), chosen so the largest magnitude lands in
+// [2^12, 2^13), well inside FP16's range; the sums are unscaled exactly.
+//
+// The workgroups are those of the FP32 form (member groups sharing the
+// tables; inverse per wavenumber pair, forward per wavenumber): the table
+// columns and each chunk's inputs are staged in FP32 as there, then each
+// staged row is split in place by one wave into groups of 4 values,
+// 4 hi halves then 4 lo halves, so a lane's MFMA operands (4 consecutive
+// reduction values, hi and lo) are one 16-byte LDS read. A wavenumber's
+// reduction runs are padded with zeros to multiples of 16, the MFMA depth.
 
-// Where a column's operand row sits in the split input rows, and its sign.
-struct SCol {
-  int row;
-  float sign;
-};
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ SCol scol(int field, int p, int part, float sign, int P) {
-  return SCol{(field * P + p) * 2 + part, sign};
+// Power of two putting a largest magnitude in [2^12, 2^13); 1 for zero.
+__device__ __forceinline__ float pow2_scale(float maxabs) {
+  if (!(maxabs > 0.0f)) return 1.0f;
+  int e;
+  frexpf(maxabs, &e);
+  e = e < -100 ? -100 : e;
+  return ldexpf(1.0f, 13 - e);
 }
 
-template <int P>
-__device__ __forceinline__ SCol inv_scol(bool winds, int blk, int c, int NC) {
-  if (c >= NC) return SCol{0, 0.0f};
-  if (!winds) return scol(0, c >> 1, c & 1, 1.0f, P);
-  const int p = c >> 2, q = c & 3;
-  if (blk == 0) return scol(q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? -1.0f : 1.0f, P);
-  return scol((q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? -1.0f : 1.0f, P);
+// x (already scaled) as hi and lo halves at value index v of a split row.
+__device__ __forceinline__ void put_split(_Float16* row, int v, float x) {
+  const _Float16 h = (_Float16)x;
+  row[(v >> 2) * 8 + (v & 3)] = h;
+  row[(v >> 2) * 8 + 4 + (v & 3)] = (_Float16)(x - (float)h);
 }
 
-template <int P>
-__device__ __forceinline__ SCol fwd_scol(int kind, int blk, int c, int NC) {
-  if (c >= NC) return SCol{0, 0.0f};
-  if (kind == LEG_PLAIN) return scol(0, c >> 1, c & 1, 1.0f, P);
-  if (kind == LEG_UV2DV) {
-    const int p = c >> 2, q = c & 3;
-    if (blk == 0) return scol((q >> 1) ? 0 : 1, p, q & 1, (q >> 1) ? 1.0f : -1.0f, P);
-    if (blk == 1) return scol(q >> 1, p, (q & 1) ? 0 : 1, (q & 1) ? 1.0f : -1.0f, P);
-    return q < 2 ? scol(2, p, q, 1.0f, P) : SCol{0, 0.0f};
-  }
-  const int p = c >> 1, q = c & 1;
-  if (blk == 0) return scol(1, p, q, 1.0f, P);
-  if (blk == 1) return scol(2, p, q, 1.0f, P);
-  return scol(0, p, q ? 0 : 1, q ? -1.0f : 1.0f, P);
+__device__ __forceinline__ f4 mfma_split(const h8& a, const h8& b, f4 c) {
+  const h4 ah = {a[0], a[1], a[2], a[3]}, al = {a[4], a[5], a[6], a[7]};
+  const h4 bh = {b[0], b[1], b[2], b[3]}, bl = {b[4], b[5], b[6], b[7]};
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, c, 0, 0, 0);
 }
+__device__ __forceinline__ h8 load_h8(const _Float16* p) { return *reinterpret_cast<const h8*>(p); }
 
-// LDS rows of the split operands: 8 halves of padding keep 16-byte
-// alignment and spread the rows over the banks.
-constexpr int kSInvA = kMaxK + 8;     // inverse A row: modes
-constexpr int kSFwd = kMaxLat + 8;    // forward A row and input row: latitudes
-constexpr int kSInvB = kMaxK + 8;     // inverse input row: modes
+// Inverse: pair columns, each wavenumber's padded to 16: Ra + Rb <= 80 at
+// T63 (K(m) + K(T - m) = T + 2). Staged FP32 rows: table [lat][col] with
+// kSplitAs floats (20 mod 32: the 8 rows of one 16-byte read pass fall in
+// distinct banks), inputs [row][2 col + part] with kSplitRaw floats, at
+// most kSplitInvRows of them (one dv2uv member at L10), so two workgroups
+// fit a CU's LDS; split in place, a table row is 2 kSplitAs halves (col
+// groups of 8), an input row 2 kSplitRaw halves, part p's groups from
+// p * 2 kSplitPairK.
+constexpr int kSplitPairK = 80;
+constexpr int kSplitAs = 84;
+constexpr int kSplitRaw = 164;
+constexpr int kSplitInvRows = 2 * kMaxLev;
+constexpr int kSplitTabN = (kMaxLat + kGroupThreads / kSplitPairK - 1) / (kGroupThreads / kSplitPairK);
+constexpr int kSplitRawN =
+    (kSplitInvRows + kGroupThreads / (2 * kSplitPairK) - 1) / (kGroupThreads / (2 * kSplitPairK));
+static_assert(2 * kSplitPairK <= kSplitRaw && kSplitPairK <= kSplitAs, "split rows");
 
-struct SplitInvLds {
-  _Float16 ahi[kMaxLat * kSInvA], alo[kMaxLat * kSInvA];
-  _Float16 bhi[2 * kMaxLev * 2 * kSInvB], blo[2 * kMaxLev * 2 * kSInvB];
-  float bscale[2 * kMaxLev * 2];
-};
-
-struct SplitFwdLds {
-  _Float16 ahi[kMaxK * kSFwd], alo[kMaxK * kSFwd];
-  _Float16 bhi[3 * kMaxLev * 2 * kSFwd], blo[3 * kMaxLev * 2 * kSFwd];
-  float bscale[3 * kMaxLev * 2];
-};
-
-template <int P>
-__device__ void leg_inv_split(const DevMember& M, const InvJob& job, int m, SplitInvLds& L) {
-  const int nlat = M.nlat, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
-  const int ncsp = M.ncsp, nrsp = M.nrsp;
-  const bool winds = job.kind == INV_DV2UV;
-  const int NC = winds ? 4 * P : 2 * P;
-  const int NT = (NC + 15) / 16, MT = nlat / 16;
-  const int R = (K + 31) & ~31;  // FP16 MFMA depth
-  const int nf = winds ? 2 : 1;
-  const float plavor = job.plavor ? M.k[CON_plavor] : 0.0f;
-  // the job's coefficients of m (zeta without the planetary vorticity in
-  // Re mode (0, 1)), zeros past K, split per row
-  stage_rows_split(
-      nf * P, 2 * R, kSInvB,
-      [&](int fp, int x) {
-        const int f = fp / P, p = fp % P, k = x >> 1;
-        if (k >= K) return 0.0f;
-        const float v = M.arena[M.slot[f ? job.src1 : job.src0] + p * nrsp + 2 * w0 + x];
-        return (winds && f == 1 && w0 + k == 1 && (x & 1) == 0) ? v - plavor : v;
-      },
-      L.bhi, L.blo, L.bscale);
-  f4 acc[kMaxTiles];
-#pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) acc[t] = f4{0, 0, 0, 0};
+// One wave per staged input row (rows r = wave, wave + 8, ...): the row's
+// `width` FP32 values x = 2 v + part (at stride 1) become, in place, its
+// per-part scale in scale[2 r + part] and split values at halves
+// part * part_halves + group layout of v (vmap(v) reorders v, if asked).
+template <typename VMap>
+__device__ __forceinline__ void split_rows(float* raw, int rows, int stride, int width, int part_halves,
+                                           float* scale, VMap vmap) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = lane & 15, g = lane >> 4;
-  const int blocks = winds ? 2 : 1;
-  for (int blk = 0; blk < blocks; blk++) {
-    const int table = winds ? (blk == 0 ? (int)TAB_legendre_qu : (int)TAB_legendre_qv) : job.table;
-    const float* q = M.ltab + M.tabo[table];
-    const float sa = M.lscale[table - TAB_legendre_qc];
-    stage(
-        nlat, R, [&](int lat, int k) { return k < K ? q[lat * ncsp + w0 + k] * sa : 0.0f; },
-        [&](int lat, int k, float v) { split1(v, &L.ahi[lat * kSInvA + k], &L.alo[lat * kSInvA + k]); });
-    __syncthreads();
+  for (int r = wave; r < rows; r += kGroupWaves) {
+    float* row = raw + r * stride;
+    float v[3];
+    float mx = 0.0f;
 #pragma unroll
-    for (int t = 0; t < kMaxTiles; t++) {
-      const int tile = wave + 4 * t;
-      if (tile < MT * NT) {
-        const int mt = tile / NT, nt = tile % NT;
-        const SCol col = inv_scol<P>(winds, blk, nt * 16 + i, NC);
-        const int arow = (mt * 16 + i) * kSInvA + 8 * g, brow = col.row * kSInvB + 8 * g;
-        f4 c = f4{0, 0, 0, 0};
-        for (int r = 0; r < R; r += 32)
-          c = mfma_split(load_h8(L.ahi + arow + r), load_h8(L.alo + arow + r), load_h8(L.bhi + brow + r),
-                         load_h8(L.blo + brow + r), c);
-        acc[t] += c * (col.sign / (sa * L.bscale[col.row]));
-      }
+    for (int j = 0; j < 3; j++) {
+      const int x = lane + 64 * j;
+      v[j] = x < width ? row[x] : 0.0f;
+      mx = fmaxf(mx, fabsf(v[j]));
     }
-    __syncthreads();
+    // lanes of one parity hold one part
+#pragma unroll
+    for (int o = 32; o > 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    const float sc = pow2_scale(mx);
+    if (lane < 2) scale[2 * r + lane] = sc;
+    _Float16* h = reinterpret_cast<_Float16*>(row);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int x = lane + 64 * j;
+      if (x < width) put_split(h + (x & 1) * part_halves, vmap(x >> 1), v[j] * sc);
+    }
   }
-  float* F0 = M.arena + M.slot[job.four0];
-  float* F1 = winds ? M.arena + M.slot[job.four1] : nullptr;
+}
+
+// One wave per table row (rows r = wave, wave + 8, ...): `width` FP32
+// values times the table scale, split in place.
+__device__ __forceinline__ void split_table(float* A, int rows, int stride, int width, float sa) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = wave; r < rows; r += kGroupWaves) {
+    float* row = A + r * stride;
+    float v[2];
 #pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) {
-    const int tile = wave + 4 * t;
-    if (tile < MT * NT) {
-      const int mt = tile / NT, nt = tile % NT;
-      const int c = nt * 16 + i;
-      if (c < NC) {
-        float* base = winds ? ((c & 2) ? F1 : F0) : F0;
-        const int p = winds ? c >> 2 : c >> 1, part = c & 1;
+    for (int j = 0; j < 2; j++) {
+      const int x = lane + 64 * j;
+      v[j] = x < width ? row[x] * sa : 0.0f;
+    }
+    _Float16* h = reinterpret_cast<_Float16*>(row);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int lat = mt * 16 + 4 * g + r;
-          base[(2 * m + part) * (P * nlat) + p * nlat + lat] = acc[t][r];
+    for (int j = 0; j < 2; j++) {
+      const int x = lane + 64 * j;
+      if (x < width) put_split(h, x, v[j]);
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
+                               float* As, float* raw, const float** rowp, float* rowsub, float* bscale) {
+  const DevMember& M = mems[group_member(plan, grp, 0)];
+  const int count = plan.count[grp];
+  const int nlat = M.nlat, ntp1 = M.ntru + 1;
+  const int ncsp = M.ncsp, nrsp = M.nrsp;
+  const int ma = pair, mb = ntp1 - 1 - pair, two = mb != ma;
+  const int Ka = ntp1 - ma, Kb = two ? ntp1 - mb : 0;
+  const int wa = mode_first(ma, ntp1), wb = two ? mode_first(mb, ntp1) : 0;
+  const int Ra = (Ka + 15) & ~15, Rb = (Kb + 15) & ~15, RT = Ra + Rb;
+  const bool winds = job.kind == INV_DV2UV;
+  const int nf = winds ? 2 : 1;
+  const int NCm = winds ? 4 * P : 2 * P;
+  const int chunk = max(1, min(count, kSplitInvRows / (nf * P)));
+  const int MT = nlat / 16;
+  const int blocks = winds ? 2 : 1;
+  const int tabs[2] = {winds ? (int)TAB_legendre_qu : job.table, (int)TAB_legendre_qv};
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, kk = lane >> 4;
+  if (threadIdx.x < count * nf * P) {
+    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
+    const DevMember& G = mems[group_member(plan, grp, g)];
+    rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp;
+    rowsub[row] = (job.plavor && f == 1) ? G.k[CON_plavor] : 0.0f;
+  }
+  const auto mode_of = [=](int col) {
+    if (col < Ra) return col < Ka ? wa + col : -1;
+    const int k = col - Ra;
+    return k < Kb ? wb + k : -1;
+  };
+  const auto raw_loader = [&](int c0) {
+    const int r0 = c0 * nf * P;
+    return [=](int row, int x) {
+      if (x >= 2 * RT) return 0.0f;
+      const int w = mode_of(x >> 1);
+      if (w < 0) return 0.0f;
+      const float v = rowp[r0 + row][2 * w + (x & 1)];
+      return (w == 1 && (x & 1) == 0) ? v - rowsub[r0 + row] : v;
+    };
+  };
+  const auto raw_store = [&](int row, int x, float v) { raw[row * kSplitRaw + x] = v; };
+  const auto split_inputs = [&](int rows) {
+    split_rows(raw, rows, kSplitRaw, 2 * RT, 2 * kSplitPairK, bscale, [](int v) { return v; });
+  };
+  __syncthreads();  // rowp
+  {
+    float va[2][kSplitTabN], vr[kSplitRawN];
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
+      if (blk < blocks) {
+        const float* q = M.ltab + M.tabo[tabs[blk]];
+        load_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col) {
+          const int w = col < RT ? mode_of(col) : -1;
+          return w >= 0 ? q[lat * ncsp + w] : 0.0f;
+        }, va[blk]);
+      }
+    load_cols<kSplitRawN, 2 * kSplitPairK>(min(chunk, count) * nf * P, raw_loader(0), vr);
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
+      if (blk < blocks) {
+        float* A = As + blk * kMaxLat * kSplitAs;
+        store_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col, float v) { A[lat * kSplitAs + col] = v; },
+                                            va[blk]);
+      }
+    store_cols<kSplitRawN, 2 * kSplitPairK>(min(chunk, count) * nf * P, raw_store, vr);
+  }
+  __syncthreads();
+  for (int blk = 0; blk < blocks; blk++)
+    split_table(As + blk * kMaxLat * kSplitAs, nlat, kSplitAs, RT, M.lscale[tabs[blk] - TAB_legendre_qc]);
+  split_inputs(min(chunk, count) * nf * P);
+  __syncthreads();
+  for (int c0 = 0; c0 < count; c0 += chunk) {
+    const int n = min(chunk, count - c0);
+    const int NC = n * NCm, NT = (NC + 15) / 16;
+    const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
+    float vn[kSplitRawN];
+    if (n1 > 0) load_cols<kSplitRawN, 2 * kSplitPairK>(n1 * nf * P, raw_loader(c1), vn);
+    const _Float16* rawh = reinterpret_cast<const _Float16*>(raw);
+    for (int h = 0; h <= two; h++) {
+      const int m = h ? mb : ma, col0 = h ? Ra : 0, R = h ? Rb : Ra;
+      f4 acc[kGTiles];
+#pragma unroll
+      for (int t = 0; t < kGTiles; t++) acc[t] = f4{0, 0, 0, 0};
+      for (int blk = 0; blk < blocks; blk++) {
+        const _Float16* A = reinterpret_cast<const _Float16*>(As + blk * kMaxLat * kSplitAs);
+        const float sa = M.lscale[tabs[blk] - TAB_legendre_qc];
+#pragma unroll
+        for (int t = 0; t < kGTiles; t++) {
+          const int tile = wave + kGroupWaves * t;
+          if (tile < MT * NT) {
+            const int mt = tile / NT, nt = tile % NT;
+            const Col col = inv_col(winds, blk, nt * 16 + i, NC, NCm, P, kSplitRaw);
+            const int brow = col.base / kSplitRaw, part = col.base - brow * kSplitRaw;
+            // 8-half groups: lane group kk's 4 values of each depth-16 step
+            const _Float16* a = A + (mt * 16 + i) * (2 * kSplitAs) + 2 * (col0 + 4 * kk);
+            const _Float16* b = rawh + brow * (2 * kSplitRaw) + part * (2 * kSplitPairK) + 2 * (col0 + 4 * kk);
+            f4 c = f4{0, 0, 0, 0};
+            for (int r = 0; r < R; r += 16) c = mfma_split(load_h8(a + 2 * r), load_h8(b + 2 * r), c);
+            const float f = col.sign == 0.0f ? 0.0f : col.sign / (sa * bscale[2 * brow + part]);
+            acc[t] += c * f;
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kGTiles; t++) {
+        const int tile = wave + kGroupWaves * t;
+        if (tile < MT * NT) {
+          const int mt = tile / NT, nt = tile % NT;
+          int c = nt * 16 + i;
+          if (c < NC) {
+            const int g = c / NCm;
+            c -= g * NCm;
+            const DevMember& G = mems[group_member(plan, grp, c0 + g)];
+            float* base = G.arena + G.slot[winds && (c & 2) ? job.four1 : job.four0];
+            const int p = winds ? c >> 2 : c >> 1, part = c & 1;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int lat = mt * 16 + 4 * kk + r;
+              base[(2 * m + part) * (P * nlat) + p * nlat + lat] = acc[t][r];
+            }
+          }
         }
       }
     }
+    if (n1 > 0) {
+      __syncthreads();  // every wave is done with this chunk's inputs
+      store_cols<kSplitRawN, 2 * kSplitPairK>(n1 * nf * P, raw_store, vn);
+      __syncthreads();
+      split_inputs(n1 * nf * P);
+      __syncthreads();
+    }
   }
 }
 
+// Forward: A[k][l] over the northern latitudes (kSplitFwdAs floats a row,
+// split in place to 2 kSplitFwdAs halves); input rows of 2 NLAT values
+// (kRawLat floats) split in place with part p's groups from p * 2 kMaxLat
+// halves, latitudes reordered north first, then the southern ones from the
+// equator out (q = half + (nlat - 1 - lat)), so a southern depth step
+// pairs the northern table values with their mirror rows:
+//   sum_l A(k, l) B(l) = sum_{l < half} A_N(k, l) (B(l) + s_k B(nlat - 1 - l)),
+// s_k = (-1)^(n - m) (negated again for qe*); the two halves run as
+// separate products, joined with s_k per output row.
+constexpr int kSplitFwdAs = 68;
+static_assert(kMaxLat / 2 <= kSplitFwdAs && 2 * kMaxLat <= kRawLat, "split forward rows");
+
 template <int P>
-__device__ void leg_fwd_split(const DevMember& M, const LegJob& job, int m, SplitFwdLds& L) {
+__device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const GroupPlan& plan, int grp, const LegJob& job, int m,
+                               float* Ah, float* raw, const float** rowp, float* bscale) {
+  const DevMember& M = mems[group_member(plan, grp, 0)];
+  const int count = plan.count[grp];
   const int nlat = M.nlat, half = nlat / 2, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
   const int ncsp = M.ncsp, nrsp = M.nrsp;
   const bool uv = job.kind == LEG_UV2DV;
-  const int NC = uv ? 4 * P : 2 * P;
-  const int NT = (NC + 15) / 16, MT = (K + 15) / 16;
+  const int NCm = uv ? 4 * P : 2 * P;
+  const int MT = (K + 15) / 16;
   int tables[3], diff[3], blocks = 0;
   if (job.kind == LEG_PLAIN) {
     tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
@@ -695,69 +763,115 @@ __device__ void leg_fwd_split(const DevMember& M, const LegJob& job, int m, Spli
   }
   const int fields[3] = {job.kind == LEG_PLAIN ? job.src : job.u, job.v, uv ? job.ke : job.src};
   const int nf = job.kind == LEG_PLAIN ? 1 : (uv && job.ke < 0) ? 2 : 3;
-  stage_rows_split(
-      nf * P, 2 * nlat, kSFwd,
-      [&](int fp, int x) {
-        const int f = fp / P, p = fp % P;
-        return M.arena[M.slot[fields[f]] + (2 * m + (x & 1)) * (P * nlat) + p * nlat + (x >> 1)];
-      },
-      L.bhi, L.blo, L.bscale);
-  f4 acc[kMaxTiles];
-#pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) acc[t] = f4{0, 0, 0, 0};
+  const int chunk = max(1, min(count, kFwdRows / (nf * P)));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = lane & 15, g = lane >> 4;
-  for (int blk = 0; blk < blocks; blk++) {
-    const float* q = M.ltab + M.tabo[tables[blk]];
-    const float sa = M.lscale[tables[blk] - TAB_legendre_qc];
-    const int dsign = diff[blk];
-    // A[k][lat] over every latitude: southern rows (-1)^(n-m) q (negated
-    // again for qe*), exact in FP16
-    stage(
-        half, MT * 16, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] * sa : 0.0f; },
-        [&](int l, int k, float v) {
-          const float s = ((k & 1) ? -1.0f : 1.0f) * (dsign ? -1.0f : 1.0f);
-          _Float16 h, lo;
-          split1(v, &h, &lo);
-          L.ahi[k * kSFwd + l] = h;
-          L.alo[k * kSFwd + l] = lo;
-          L.ahi[k * kSFwd + nlat - 1 - l] = (_Float16)s * h;
-          L.alo[k * kSFwd + nlat - 1 - l] = (_Float16)s * lo;
-        });
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kMaxTiles; t++) {
-      const int tile = wave + 4 * t;
-      if (tile < MT * NT) {
-        const int mt = tile / NT, nt = tile % NT;
-        const SCol col = fwd_scol<P>(job.kind, blk, nt * 16 + i, NC);
-        const int arow = (mt * 16 + i) * kSFwd + 8 * g, brow = col.row * kSFwd + 8 * g;
-        f4 c = f4{0, 0, 0, 0};
-        for (int r = 0; r < nlat; r += 32)
-          c = mfma_split(load_h8(L.ahi + arow + r), load_h8(L.alo + arow + r), load_h8(L.bhi + brow + r),
-                         load_h8(L.blo + brow + r), c);
-        acc[t] += c * (col.sign / (sa * L.bscale[col.row]));
-      }
-    }
-    __syncthreads();
+  const int i = lane & 15, kk = lane >> 4;
+  if (threadIdx.x < count * nf * P) {
+    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
+    const DevMember& G = mems[group_member(plan, grp, g)];
+    rowp[row] = G.arena + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
   }
-  float* O0 = M.arena + M.slot[job.dst0];
-  float* O1 = uv ? M.arena + M.slot[job.dst1] : nullptr;
+  const auto raw_loader = [&](int c0) {
+    const int r0 = c0 * nf * P;
+    return [=](int row, int c) {
+      const int part = c >= nlat, lat = c - part * nlat;
+      return c < 2 * nlat ? rowp[r0 + row][part * (P * nlat) + lat] : 0.0f;
+    };
+  };
+  const auto raw_store = [&](int row, int c, float v) {
+    const int part = c >= nlat, lat = c - part * nlat;
+    if (c < 2 * nlat) raw[row * kRawLat + 2 * lat + part] = v;
+  };
+  const auto split_inputs = [&](int rows) {
+    split_rows(raw, rows, kRawLat, 2 * nlat, 2 * kMaxLat, bscale,
+               [=](int lat) { return lat < half ? lat : half + (nlat - 1 - lat); });
+  };
+  __syncthreads();  // rowp
+  {
+    float va[3][kFwdColTabN], vr[kFwdColRawN];
 #pragma unroll
-  for (int t = 0; t < kMaxTiles; t++) {
-    const int tile = wave + 4 * t;
-    if (tile < MT * NT) {
-      const int mt = tile / NT, nt = tile % NT;
-      const int c = nt * 16 + i;
-      if (c < NC) {
-        float* base = uv ? ((c & 2) ? O1 : O0) : O0;
-        const int p = uv ? c >> 2 : c >> 1, part = c & 1;
+    for (int blk = 0; blk < 3; blk++)
+      if (blk < blocks) {
+        const float* q = M.ltab + M.tabo[tables[blk]];
+        load_cols<kFwdColTabN, 64>(half, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; },
+                                   va[blk]);
+      }
+    load_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_loader(0), vr);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int k = mt * 16 + 4 * g + r;
-          if (k < K) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
+    for (int blk = 0; blk < 3; blk++)
+      if (blk < blocks) {
+        float* A = Ah + blk * kMaxK * kSplitFwdAs;
+        store_cols<kFwdColTabN, 64>(half, [&](int l, int k, float v) { if (k < MT * 16) A[k * kSplitFwdAs + l] = v; },
+                                    va[blk]);
+      }
+    store_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_store, vr);
+  }
+  __syncthreads();
+  for (int blk = 0; blk < blocks; blk++)
+    split_table(Ah + blk * kMaxK * kSplitFwdAs, MT * 16, kSplitFwdAs, half, M.lscale[tables[blk] - TAB_legendre_qc]);
+  split_inputs(min(chunk, count) * nf * P);
+  __syncthreads();
+  for (int c0 = 0; c0 < count; c0 += chunk) {
+    const int n = min(chunk, count - c0);
+    const int NC = n * NCm, NT = (NC + 15) / 16;
+    const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
+    float vn[kFwdColRawN];
+    if (n1 > 0) load_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_loader(c1), vn);
+    const _Float16* rawh = reinterpret_cast<const _Float16*>(raw);
+    f4 acc[kFwdGTiles];
+#pragma unroll
+    for (int t = 0; t < kFwdGTiles; t++) acc[t] = f4{0, 0, 0, 0};
+    for (int blk = 0; blk < blocks; blk++) {
+      const _Float16* A = reinterpret_cast<const _Float16*>(Ah + blk * kMaxK * kSplitFwdAs);
+      const float sa = M.lscale[tables[blk] - TAB_legendre_qc];
+      // s_k of output row k = mt 16 + 4 kk + r: (-1)^r (mt 16 + 4 kk is even)
+      const float sd = diff[blk] ? -1.0f : 1.0f;
+#pragma unroll
+      for (int t = 0; t < kFwdGTiles; t++) {
+        const int tile = wave + kGroupWaves * t;
+        if (tile < MT * NT) {
+          const int mt = tile / NT, nt = tile % NT;
+          const Col col = fwd_col(job.kind, blk, nt * 16 + i, NC, NCm, nf, P);
+          const int brow = col.base / kRawLat, part = col.base - brow * kRawLat;
+          const _Float16* a = A + (mt * 16 + i) * (2 * kSplitFwdAs) + 2 * (4 * kk);
+          const _Float16* b = rawh + brow * (2 * kRawLat) + part * (2 * kMaxLat) + 2 * (4 * kk);
+          f4 cn = f4{0, 0, 0, 0}, cs = f4{0, 0, 0, 0};
+          for (int r = 0; r < half; r += 16) {
+            const h8 av = load_h8(a + 2 * r);
+            cn = mfma_split(av, load_h8(b + 2 * r), cn);
+            cs = mfma_split(av, load_h8(b + 2 * (half + r)), cs);
+          }
+          const float f = col.sign == 0.0f ? 0.0f : col.sign / (sa * bscale[2 * brow + part]);
+          acc[t] += f * (cn + sd * f4{cs[0], -cs[1], cs[2], -cs[3]});
         }
       }
+    }
+#pragma unroll
+    for (int t = 0; t < kFwdGTiles; t++) {
+      const int tile = wave + kGroupWaves * t;
+      if (tile < MT * NT) {
+        const int mt = tile / NT, nt = tile % NT;
+        int c = nt * 16 + i;
+        if (c < NC) {
+          const int g = c / NCm;
+          c -= g * NCm;
+          const DevMember& G = mems[group_member(plan, grp, c0 + g)];
+          float* base = G.arena + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
+          const int p = uv ? c >> 2 : c >> 1, part = c & 1;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int k = mt * 16 + 4 * kk + r;
+            if (k < K) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
+          }
+        }
+      }
+    }
+    if (n1 > 0) {
+      __syncthreads();  // every wave is done with this chunk's inputs
+      store_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_store, vn);
+      __syncthreads();
+      split_inputs(n1 * nf * P);
+      __syncthreads();
     }
   }
 }
@@ -794,14 +908,19 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
 #undef CALL
 }
 
-extern "C" __global__ __launch_bounds__(256) void k_leg_inv_split(const DevMember* __restrict__ mems,
-                                                                   LaunchPlan plan, InvJobs jobs) {
-  __shared__ __align__(16) SplitInvLds L;
+extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_leg_inv_split(const DevMember* __restrict__ mems,
+                                                                             GroupPlan plan, InvJobs jobs) {
+  __shared__ __align__(16) float As[2 * kMaxLat * kSplitAs];
+  __shared__ __align__(16) float raw[kSplitInvRows * kSplitRaw];
+  __shared__ const float* rowp[kInvGroupRows];
+  __shared__ float rowsub[kInvGroupRows];
+  __shared__ float bscale[2 * kSplitInvRows];
   int tile;
-  const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
-  const int ntp1 = M.ntru + 1;
-  const InvJob job = jobs.j[job_of(jobs, tile / ntp1)];
-#define CALL(PP) leg_inv_split<PP>(M, job, tile % ntp1, L)
+  const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
+  const DevMember& M = mems[group_member(plan, grp, 0)];
+  const int pairs = (M.ntru + 2) / 2;
+  const InvJob job = jobs.j[job_of(jobs, tile / pairs)];
+#define CALL(PP) leg_inv_gsplit<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, rowsub, bscale)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
@@ -823,14 +942,18 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
 #undef CALL
 }
 
-extern "C" __global__ __launch_bounds__(256) void k_leg_fwd_split(const DevMember* __restrict__ mems,
-                                                                   LaunchPlan plan, LegJobs jobs) {
-  __shared__ __align__(16) SplitFwdLds L;
+extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_leg_fwd_split(const DevMember* __restrict__ mems,
+                                                                             GroupPlan plan, LegJobs jobs) {
+  __shared__ __align__(16) float Ah[3 * kMaxK * kSplitFwdAs];
+  __shared__ __align__(16) float raw[kFwdRows * kRawLat];
+  __shared__ const float* rowp[kLegGroupRows];
+  __shared__ float bscale[2 * kFwdRows];
   int tile;
-  const DevMember& M = mems[plan_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile)];
+  const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
+  const DevMember& M = mems[group_member(plan, grp, 0)];
   const int ntp1 = M.ntru + 1;
   const LegJob job = jobs.j[job_of(jobs, tile / ntp1)];
-#define CALL(PP) leg_fwd_split<PP>(M, job, tile % ntp1, L)
+#define CALL(PP) leg_fwd_gsplit<PP>(mems, plan, grp, job, tile % ntp1, Ah, raw, rowp, bscale)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }

@@ -2088,16 +2088,17 @@ __device__ __forceinline__ void physics_d(const DevMember& M, int h) {
 
 // Deterministic sum of two grid fields of one member by one workgroup:
 // each lane sums a strided share in order, then a fixed tree
-// (reduction/wgsl/sum.wgsl:12-34).
+// (reduction/wgsl/sum.wgsl:12-34), in FP64 (the planet's mass and energy
+// budgets), rounded to FP32 once at the end.
 // The planet-wide reductions run one workgroup of kReduceThreads per member.
 constexpr int kReduceThreads = 1024;
 
 __device__ __forceinline__ void block_sum2(const DevMember& M, const float* a, const float* b, float* out) {
-  __shared__ float sa[kReduceThreads], sb[kReduceThreads];
-  float x = 0, y = 0;
+  __shared__ double sa[kReduceThreads], sb[kReduceThreads];
+  double x = 0, y = 0;
   for (int h = threadIdx.x; h < M.nhor; h += kReduceThreads) {
-    x += a[h];
-    y += b[h];
+    x += (double)a[h];
+    y += (double)b[h];
   }
   sa[threadIdx.x] = x;
   sb[threadIdx.x] = y;
@@ -2110,8 +2111,8 @@ __device__ __forceinline__ void block_sum2(const DevMember& M, const float* a, c
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = sa[0];
-    out[1] = sb[0];
+    out[0] = (float)sa[0];
+    out[1] = (float)sb[0];
   }
 }
 
@@ -2168,14 +2169,14 @@ extern "C" __global__ __launch_bounds__(kReduceThreads) void k_fixer_reduce(cons
   float* fixneg = M.arena + M.slot[SLOT_fixneg];
   float* fixpos = M.arena + M.slot[SLOT_fixpos];
   // one wave per latitude row: lanes sum strided shares, then a fixed
-  // butterfly (the same order on every run)
+  // butterfly (the same order on every run), in FP64
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int row = wave; row < M.nlat; row += kReduceThreads / 64) {
     const int first = row * NLON;
-    float zneg = 0, zpos = 0;
+    double zneg = 0, zpos = 0;
     for (int j = lane; j < NLON; j += 64) {
-      zneg += fixneg[first + j];
-      zpos += fixpos[first + j];
+      zneg += (double)fixneg[first + j];
+      zpos += (double)fixpos[first + j];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -2183,7 +2184,7 @@ extern "C" __global__ __launch_bounds__(kReduceThreads) void k_fixer_reduce(cons
       zpos += __shfl_xor(zpos, o);
     }
     if (zneg <= 0 || zpos < zneg) continue;
-    const float factor = (zpos - zneg) / zpos;
+    const float factor = (float)((zpos - zneg) / zpos);
     for (int j = lane; j < NLON; j += 64) {
       for (int l = 0; l < L; l++) fixq[l * NH + first + j] = fmaxf(fixq[l * NH + first + j] * factor, 0.0f);
       fixneg[first + j] = 0;
