@@ -45,11 +45,12 @@ def probe_steps(steps):
     return min(steps, 96)
 
 
-def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True, tune=False):
-    """Steps every config together; returns (wall s, device ms, probe).
+def run_gpu(exo, cfgs, steps, warmup, probes=(), precision=0, graphs=True, tune=False):
+    """Steps every config together; returns (wall s, device ms, probed).
     The timed steps replay the captured step graph (graphs=True) or launch
-    every kernel (False); the probe then times single launches of its
-    kernel over a further window of steps, after the timed region."""
+    every kernel (False); each kernel of `probes` then has its single
+    launches timed over a further window of steps of its own, after the
+    timed region (probed: kernel -> (total ms, launches, missed))."""
     eng = exo.Engine(0, len(cfgs))
     try:
         eng.set_step_graphs(graphs)
@@ -71,12 +72,12 @@ def run_gpu(exo, cfgs, steps, warmup, probe, precision=0, graphs=True, tune=Fals
         eng.sync()
         wall = time.perf_counter() - t0
         dev_ms = eng.last_step_ms()
-        probed = None
-        if probe:
-            eng.probe_kernel(probe, 65536)
+        probed = {}
+        for kernel in probes:
+            eng.probe_kernel(kernel, 65536)
             eng.step(probe_steps(steps))
             eng.sync()
-            probed = eng.probe_result()
+            probed[kernel] = eng.probe_result()
             eng.probe_kernel(None)
         for i in ids:
             ts, _ = eng.global_means(i)
@@ -265,9 +266,16 @@ def main():
         workload = {"workload": "plain FP32 single planet T63 L10, 30-min step",
                     "truncation": 63, "levels": 10, "timestep_minutes": 30, "members": 1}
     probe = roofline.DOMINANT[args.workload] if args.probe == "auto" else (args.probe or None)
-    if probe and precision == exo.EXO_PRECISION_SPLIT and probe in ("k_leg_inv", "k_leg_fwd"):
-        probe += "_split"
-    wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probe, precision,
+    secondary = list(roofline.SECONDARY[args.workload]) if args.probe == "auto" else []
+
+    def form_name(k):
+        split = precision == exo.EXO_PRECISION_SPLIT and k in ("k_leg_inv", "k_leg_fwd")
+        return k + "_split" if split else k
+
+    probe = form_name(probe) if probe else None
+    secondary = [form_name(k) for k in secondary]
+    probes = ([probe] if probe else []) + secondary
+    wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probes, precision,
                                    tune=not args.no_autotune)
     days = sum(args.steps / workloads.steps_per_day(c.timestep_minutes) for c in cfgs)
     value = days / wall
@@ -290,21 +298,21 @@ def main():
     }
     if args.workload == "batched" and not args.no_plain:
         other = exo.EXO_PRECISION_FP32 if precision == exo.EXO_PRECISION_SPLIT else exo.EXO_PRECISION_SPLIT
-        ow, _, _ = run_gpu(exo, cfgs, args.steps, args.warmup, None, other, tune=not args.no_autotune)
+        ow, _, _ = run_gpu(exo, cfgs, args.steps, args.warmup, (), other, tune=not args.no_autotune)
         name = "batched_plain_fp32" if other == exo.EXO_PRECISION_FP32 else "batched_split_fp16"
         line[name] = {"value": days / ow, "unit": "simulated days/s", "ms_per_step": ow * 1000.0 / args.steps,
                       "form": ("plain FP32" if other == exo.EXO_PRECISION_FP32
                                else "split-operand Legendre (FP16 hi/lo, FP32 accumulate)")}
     if args.workload == "batched" and not args.no_plain:
         plain_cfg = workloads.plain(exo, 63, 10, 30)
-        pw, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None, tune=not args.no_autotune)
+        pw, _, _ = run_gpu(exo, [plain_cfg], 480, 48, (), tune=not args.no_autotune)
         plain_tuned = run_gpu.tuned
         plain_value = (480 / 48) / pw
         line["plain_single"] = {"value": plain_value, "unit": "simulated days/s",
                                 "ms_per_step": pw * 1000.0 / 480,
                                 "config": "plain FP32 single planet T63 L10, 30-min step",
                                 "autotune": plain_tuned}
-        ew, _, _ = run_gpu(exo, [plain_cfg], 480, 48, None, graphs=False)
+        ew, _, _ = run_gpu(exo, [plain_cfg], 480, 48, (), graphs=False)
         line["plain_single_eager"] = {"value": (480 / 48) / ew, "unit": "simulated days/s",
                                       "ms_per_step": 1e3 * ew / 480,
                                       "config": "as plain_single, every kernel launched (no step graph)"}
@@ -317,12 +325,15 @@ def main():
         line["batched_vs_one_at_a_time"] = value / one["value"]
     if args.workload == "batched" and not args.no_serving:
         line["serving"] = run_serving(exo, args.serve_seconds)
-    if probed:
-        total_ms, launches, missed = probed
+    if probe:
+        total_ms, launches, missed = probed[probe]
         line["roofline"] = roofline.figure(exo, probe, cfgs, total_ms, launches, missed,
                                            probe_steps(args.steps))
     else:
         line["roofline"] = None
+    if secondary:
+        line["roofline_secondary"] = [roofline.figure(exo, k, cfgs, *probed[k], probe_steps(args.steps))
+                                      for k in secondary]
     line["cpu_baseline"] = (None if args.no_cpu_baseline else
                             cpu_baseline(exo, cfgs, args.cpu_seconds, args.cpu_cores or cpu_cores()))
     print(json.dumps(line))

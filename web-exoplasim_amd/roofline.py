@@ -9,7 +9,14 @@ one step (the job lists the step enqueues):
   levels. (The dissipation's mean-mode round trip is not a transform
   here: k_mean_dekin computes its one surviving mode directly.) FLOPs use the north/south-symmetric count with an FMA as 2:
   2 NLAT NCSP per scalar plane, 8 per dv2uv level, 10 per uv2dv+KE level,
-  8 per uv2dv level, 6 per qtend level. Bound: the FP32 vector peak.
+  8 per uv2dv level, 6 per qtend level. Bytes: every input plane read
+  and every output plane written once (a Fourier plane is NLAT NFC
+  floats, a spectral plane 2 NCSP): inverse 17L+3 spectral planes in and
+  17L+3 Fourier planes out; forward 15L+1 Fourier planes in (the forward
+  DFT's rows) and 10L+1 spectral planes out. Their intensity, about
+  6 FLOP/B on the mix, is below the FP32 MFMA ridge (157.3 TFLOP/s over
+  8 TB/s, 19.7 FLOP/B) and far below FP16's, so the bound they are
+  measured against is HBM.
 * zonal DFTs (k_dft_inv, k_dft_fwd): 17L+4 inverse and 15L+1 forward rows
   of every latitude; each row reads or writes NFC Fourier and NLON grid
   floats once: 4 (NLON + NFC) bytes. Bound: HBM.
@@ -32,9 +39,12 @@ HBM_PEAK_GBS = 8000.0
 
 # The kernel bench.py times by default: the one with the largest share of
 # device time in that workload's rocprofv3 kernel trace.
-# (ensemble: k_dft_inv 12.6%, k_leg_inv 12.4% of device time in
+# (ensemble: k_leg_fwd 10.8%, k_leg_inv 9.4%, k_surface_fluxes and
+# k_longwave 8.8%, k_dft_inv 8.2% of device time in
 # profiles/ensemble64_fp32_kernel_trace.md)
-DOMINANT = {"batched": "k_dft_inv", "plain": "k_leg_inv"}
+DOMINANT = {"batched": "k_leg_fwd", "plain": "k_leg_inv"}
+# further kernels the bench line reports beside the dominant one
+SECONDARY = {"batched": ("k_leg_inv", "k_dft_inv", "k_dft_fwd", "k_surface_fluxes"), "plain": ()}
 
 
 def _dims(cfg):
@@ -66,6 +76,18 @@ def work_per_step(kernel, cfg):
     raise KeyError(f"no algorithmic work model for {kernel}")
 
 
+def bytes_per_step(kernel, cfg):
+    """Algorithmic HBM bytes of one model step of one member for the
+    kernels whose work_per_step counts FLOPs (None for the others)."""
+    nlat, nlon, L, ncsp, nfc = _dims(cfg)
+    kernel = kernel.replace("_split", "")
+    if kernel == "k_leg_inv":
+        return 4 * (17 * L + 3) * (2 * ncsp + nlat * nfc)
+    if kernel == "k_leg_fwd":
+        return 4 * ((15 * L + 1) * nlat * nfc + (10 * L + 1) * 2 * ncsp)
+    return None
+
+
 LAUNCHES_PER_STEP = {"k_leg_inv": 3, "k_dft_inv": 3, "k_leg_fwd": 3, "k_dft_fwd": 3, "k_surface_fluxes": 1,
                      "k_shortwave": 1, "k_longwave": 1, "k_convection": 1}
 
@@ -91,18 +113,32 @@ def measured_traffic(kernel, path=TRAFFIC_PROFILE):
 def figure(exo, kernel, cfgs, total_ms, launches, missed, steps):
     """The bench line's roofline object for `kernel` over `steps` steps of
     `cfgs`, from the probed launch durations."""
-    units, per_step = None, 0
+    units, per_step, bytes_step = None, 0, 0
     for c in cfgs:
         units, w = work_per_step(kernel, c)
         per_step += w
+        b = bytes_per_step(kernel, c)
+        bytes_step = None if b is None or bytes_step is None else bytes_step + b
     expected = LAUNCHES_PER_STEP[kernel.replace("_split", "")] * steps
     if launches + missed != expected:
         raise RuntimeError(f"{kernel}: {launches + missed} launches seen, {expected} expected")
     per_launch = per_step / LAUNCHES_PER_STEP[kernel.replace("_split", "")]
     avg_s = total_ms / launches / 1e3
+    extra = {}
+    if units == "flop":
+        flop_peak = FP16_PEAK_TFLOPS if kernel.endswith("_split") else FP32_PEAK_TFLOPS
+        ridge = flop_peak * 1e12 / (HBM_PEAK_GBS * 1e9)
+        intensity = per_step / bytes_step if bytes_step else None
+        extra = {"flop_per_launch": per_launch, "achieved_tflops": per_launch / avg_s / 1e12,
+                 "mfma_peak_tflops": flop_peak, "intensity_flop_per_byte": intensity,
+                 "ridge_flop_per_byte": ridge}
+        if intensity is not None and intensity < ridge:
+            # below the ridge: HBM bounds it, measured in algorithmic bytes
+            per_launch = bytes_step / LAUNCHES_PER_STEP[kernel.replace("_split", "")]
+            units = "byte"
     if units == "flop":
         achieved = per_launch / avg_s / 1e12
-        peak = FP16_PEAK_TFLOPS if kernel.endswith("_split") else FP32_PEAK_TFLOPS
+        peak = extra["mfma_peak_tflops"]
         unit, bound = "TFLOP/s", "mfma"
     else:
         achieved = per_launch / avg_s / 1e9
@@ -110,5 +146,5 @@ def figure(exo, kernel, cfgs, total_ms, launches, missed, steps):
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
             "traffic": measured_traffic(kernel),
             "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE) + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                              "copy-calibrated, averaged over a 5-step run)", "kernel": kernel, "avg_launch_us": avg_s * 1e6, "launches": launches,
-            "algorithmic_per_launch": per_launch, "algorithmic_unit": units}
+                              "copy-calibrated, averaged over every dispatch of a 4-step run)", "kernel": kernel, "avg_launch_us": avg_s * 1e6, "launches": launches,
+            "algorithmic_per_launch": per_launch, "algorithmic_unit": units, **extra}
