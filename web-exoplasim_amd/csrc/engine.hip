@@ -1325,9 +1325,15 @@ int exo_checkpoint_continue(exo_engine* h, int id, const void* bytes, size_t siz
   // changed setting invalidates take the new configuration's initial
   // values (gcm.worker.ts:117-124)
   const exo_config& now = m->host.cfg;
+  // a nullable setting (NaN: null) is unchanged when both are null
+  const auto same = [](double a, double b) { return a == b || (std::isnan(a) && std::isnan(b)); };
   std::vector<int> slots;
   if (now.co2_ppm != previous->co2_ppm) slots.insert(slots.end(), {SLOT_co2, SLOT_dqco2});
-  if (now.ocean_depth_m != previous->ocean_depth_m) slots.insert(slots.end(), {SLOT_ymld, SLOT_xmld, SLOT_dmld});
+  if (!same(now.ocean_depth_m, previous->ocean_depth_m))
+    slots.insert(slots.end(), {SLOT_ymld, SLOT_xmld, SLOT_dmld});
+  if (!same(now.land_albedo, previous->land_albedo) || !same(now.star_temperature_k, previous->star_temperature_k))
+    slots.insert(slots.end(), {SLOT_dalbcl, SLOT_dalbcl1, SLOT_dalbcl2});
+  if (!same(now.soil_water_capacity_m, previous->soil_water_capacity_m)) slots.push_back(SLOT_dwmax);
   std::vector<float> values;
   for (const int s : slots) {
     const size_t off = m->host.layout.offset[s], n = m->host.layout.floats[s];
