@@ -102,6 +102,23 @@ typedef struct exo_config {
   int filter;                      /* physicsFilter's NFILTER: 0 none, 1 cesaro, 2 exp, 3 lh
                                       (NGPTFILTER and NSPVFILTER stay 1, as in the base
                                       namelist) */
+  /* Fresh-start state (prolog's fresh start, setup/index.ts:434-449).
+   * 0 (default): the seeded synthetic state above: an isothermal
+   * atmosphere at initial_temperature with noise_kelvin grid-point noise,
+   * initial_humidity, ln(ps) from the orography (no reference counterpart).
+   * 1: the reference's initfd (setup/state/index.ts:60-86): temperature at
+   * setzt's restoration profile (its global mean; the whole field under a
+   * tidally locked gradient), solid-body planetary vorticity, no
+   * divergence or humidity, ln(ps) from noise(KICK) drawn from gfortran's
+   * random_number after random_seed(put=SEED), or from the orography when
+   * nspinit > 0. */
+  int initial_state;
+  int kick;          /* plasim_nl KICK: 0 none, 1 every non-zonal coefficient,
+                        2 the equatorially symmetric ones, 3 mode (1,1) alone */
+  int32_t seed[8];   /* plasim_nl SEED (the base namelist's 14963991,2,...,8);
+                        seed[0] == 0 takes the words from clock_ms (initrandom) */
+  int64_t clock_ms;  /* the clock for a seed from the clock (Date.now()) */
+  int nspinit;       /* surfmod_nl NSPINIT (default 0) */
 } exo_config;
 
 /* Fills *cfg with the defaults of a T21 L10 Earth-like synthetic planet:
@@ -249,6 +266,11 @@ int exo_member_step(exo_engine *e, int member, int64_t *step);
    counter, as GcmSnapshot.date (CALENDAR date, calendar/date/index.ts:
    46-82). */
 int exo_member_date(exo_engine *e, int member, int *date5);
+/* The same date at any model step of the member's calendar (GCM_DATE.date,
+   calendar/date/index.ts:46-82, as the worker dates its progress reports,
+   gcm.worker.ts:307-315): the 360-day calendar, or the Gregorian one for a
+   365-day year. */
+int exo_member_date_at(exo_engine *e, int member, int64_t step, int *date5);
 
 /* Arena layout: total floats and one slot's offset/length by name.
  * LAYOUT.create / LAYOUT.constants (layout/index.ts:22-53). */

@@ -241,3 +241,58 @@ def test_ensemble_planets_first_step(exo):
     finally:
         gpu.close()
         host.close()
+
+
+@pytest.mark.parametrize("trunc,levels", [(21, 10), (42, 5)])
+def test_reference_fresh_start(exo, trunc, levels):
+    """The reference's fresh start (initfd: the restoration profile's mean
+    temperature, planetary vorticity, no humidity, ln(ps) noise from
+    gfortran's random numbers with the base namelist's SEED) under the
+    reference's default atmosphere (1.011 bar of air, 300 ppm CO2): the
+    first step within 1e-5 of the FP64 oracle, and 6 steps within the FP32
+    envelope."""
+    cfg = exo.default_config(truncation=trunc, levels=levels, initial_state=1, pressure_bar=1.011,
+                             gas_constant=8314.46261815324 / 28.970253, co2_ppm=300,
+                             eccentricity=0.016715, obliquity_deg=23.5)
+    gpu = exo.Engine(0, 1)
+    host = exo.Engine(-1, 1)
+    try:
+        g = gpu.add_member(cfg)
+        h = host.add_member(cfg)
+        o64 = OracleRun(host, h)
+        o32 = OracleRun(host, h, precision="fp32")
+        o32f = OracleRun(host, h, precision="fp32fma")
+        k = host.constants(h, 0)
+        names = STATE + PHYSICS
+        eg = {n: [] for n in names}
+        ec = {n: [] for n in names}
+        for step in range(1, 7):
+            gpu.step(1)
+            gpu.sync()
+            for o in (o64, o32, o32f):
+                o.step(1, physics=True)
+            for n in names:
+                a = gpu.read_field(g, n).astype(np.float64)
+                b = o64.field(n)
+                assert np.all(np.isfinite(a)), (step, n)
+                if step == 1 and n in SPECTRAL:
+                    if n == "st":
+                        rel = np.sqrt(np.mean((a - b) ** 2)) * k["ct"] / np.sqrt(2.0) / 250.0
+                    else:
+                        rel = _rms(a, b)
+                    print(f"T{trunc} L{levels} fresh start, step 1 {n}: rel {rel:.2e}")
+                    assert rel < 1e-5, (n, rel)
+                eg[n].append(_rms(a, b))
+                ec[n].append(max(_rms(o32.field(n).astype(np.float64), b),
+                                 _rms(o32f.field(n).astype(np.float64), b)))
+        bad = []
+        for n in names:
+            ga = np.sqrt(np.mean(np.square(eg[n])))
+            ca = np.sqrt(np.mean(np.square(ec[n])))
+            print(f"T{trunc} L{levels} fresh start, steps 1..6 {n}: GPU {ga:.2e} FP32-emulation {ca:.2e}")
+            if not ga <= 2 * ca + 1e-6:
+                bad.append((n, ga, ca))
+        assert not bad, bad
+    finally:
+        gpu.close()
+        host.close()

@@ -743,6 +743,14 @@ int exo_member_date(exo_engine* h, int id, int* date5) {
   return EXO_OK;
 }
 
+int exo_member_date_at(exo_engine* h, int id, int64_t step, int* date5) {
+  MemberDev* m = member_of(h, id);
+  if (!m || !date5) return fail(EXO_E_ARG, "no member " + std::to_string(id));
+  if (step < 0) return fail(EXO_E_ARG, "step must be nonnegative");
+  exo::member_date(m->host, step, date5);
+  return EXO_OK;
+}
+
 int exo_member_step(exo_engine* h, int id, int64_t* step) {
   MemberDev* m = member_of(h, id);
   if (!m || !step) return fail(EXO_E_ARG, "no member " + std::to_string(id));

@@ -28,6 +28,34 @@ const int SOIL_LAYERS = 5;
 
 int count_of(int shape_count_L, int) { return shape_count_L; }
 
+// ------------------------------------------------- Gregorian calendar
+// calendar/date/index.ts:10-29: the 365-day year is the proleptic
+// Gregorian calendar counted in days from the start of year 0.
+
+const double MEAN_GREGORIAN_YEAR = 365.2425;
+
+int64_t floor_div(int64_t a, int64_t b) { return a / b - ((a % b != 0) && ((a < 0) != (b < 0))); }
+
+bool gregorian_leap(int64_t year) { return year % 4 == 0 && (year % 100 != 0 || year % 400 == 0); }
+
+int64_t gregorian_year_start(int64_t year) {
+  return 365 * year + floor_div(year + 3, 4) - floor_div(year + 99, 100) + floor_div(year + 399, 400);
+}
+
+int gregorian_month_days(int64_t year, int month) {
+  static const int days[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+  return days[month - 1] + (month == 2 && gregorian_leap(year) ? 1 : 0);
+}
+
+// CALENDAR.startStep (calendar/date/index.ts:31-44): the first step of
+// (year, month).
+int64_t start_step_of(int64_t mtspd, int year, int month, int days_per_year, int days_per_month) {
+  if (days_per_year != 365) return mtspd * ((int64_t)year * days_per_year + (int64_t)(month - 1) * days_per_month);
+  int64_t days = gregorian_year_start(year);
+  for (int at = 1; at < month; at++) days += gregorian_month_days(year, at);
+  return mtspd * days;
+}
+
 }  // namespace
 
 Layout make_layout(const ExoDims& d, int nsnap) {
@@ -232,11 +260,8 @@ Pumamod pumamod_read(const exo_config& c) {
   p.n_steps_per_year = std::llround(p.sidereal_year / (mpstep * 60));
   p.n_days_per_month = m_days / 12;
   p.m_days_per_year = m_days;
-  // CALENDAR.startStep for a non-365-day year (calendar/date/index.ts:31-44)
-  if (ndpy != 365)
-    p.n_start_step = (int64_t)mtspd * (p.n_start_year * ndpy + (p.n_start_month - 1) * p.n_days_per_month);
-  else
-    throw std::runtime_error("365-day calendars are not built");
+  // CALENDAR.startStep (calendar/date/index.ts:31-44)
+  p.n_start_step = start_step_of(mtspd, p.n_start_year, p.n_start_month, ndpy, p.n_days_per_month);
   return p;
 }
 
@@ -678,6 +703,68 @@ struct Rng {
   }
 };
 
+// libgfortran's random_seed(put=) and random_number for real(8), as the
+// reference restates them (shared/random/gfortran/index.ts:8-51): the
+// eight 32-bit seed words are taken in reverse order, paired into four
+// little-endian 64-bit words and XORed with libgfortran's scrambling keys;
+// each number is the top 53 bits of xoshiro256**'s next word times 2^-53.
+struct GfortranRandom {
+  uint64_t s[4];
+  explicit GfortranRandom(const uint32_t seed[8]) {
+    static const uint64_t keys[4] = {0xbd0c5b6e50c2df49ull, 0xd46061cd46e1df38ull,
+                                     0xbb4f4d4ed6103544ull, 0x114a583d0756ad39ull};
+    for (int i = 0; i < 4; i++)
+      s[i] = ((uint64_t)seed[7 - 2 * i] | ((uint64_t)seed[6 - 2 * i] << 32)) ^ keys[i];
+  }
+  double next() {
+    const uint64_t result = Rng::rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = Rng::rotl(s[3], 45);
+    return (double)(result >> 11) * 0x1.0p-53;
+  }
+};
+
+// initrandom (setup/state/index.ts:39-46): the namelist's SEED, or words
+// from the clock, (clock + 37 i) truncated to 32 bits, when its first word
+// is zero.
+std::array<uint32_t, 8> initrandom_seed(const exo_config& cfg) {
+  std::array<uint32_t, 8> words{};
+  for (int i = 0; i < 8; i++)
+    words[i] = cfg.seed[0] != 0 ? (uint32_t)cfg.seed[i] : (uint32_t)(uint64_t)(cfg.clock_ms + 37 * (i + 1));
+  return words;
+}
+
+// noise (setup/state/index.ts:12-35): white noise of amplitude 1e-4 in
+// ln(ps). Kick 1 perturbs every non-zonal coefficient in storage order,
+// kick 2 the real and imaginary parts of the modes with n + m even (one
+// draw for both), kick 3 sets mode (1, 1) alone.
+void initfd_noise(double* sp, int T, int kick, const std::array<uint32_t, 8>& seed) {
+  GfortranRandom random(seed.data());
+  const double scale = 1e-4 / std::sqrt(2.0);
+  const int ntp1 = T + 1, nrsp = (T + 1) * (T + 2);
+  if (kick == 1) {
+    for (int i = 2 * ntp1; i < nrsp; i++) sp[i] += scale * (random.next() - 0.5);
+  } else if (kick == 2) {
+    int r = 2 * ntp1 - 2;
+    for (int m = 1; m <= T; m++)
+      for (int n = m; n <= T; n++) {
+        r += 2;
+        if ((n + m) % 2 != 0) continue;
+        const double value = scale * (random.next() - 0.5);
+        sp[r] += value;
+        sp[r + 1] += value;
+      }
+  } else if (kick == 3) {
+    sp[2 * ntp1 + 2] = scale;
+    sp[2 * ntp1 + 3] = scale * 0.5;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------- set-up transforms
@@ -751,11 +838,24 @@ double saturation(double t, const Pumamod& p) {
   return q / (1 - (1 / p.rdbrv - 1) * q);
 }
 
-// calendar/date/index.ts:46-121 for the 360-day calendar.
 struct Date {
   int year, month, day, hour, minute;
 };
+// calendar/date/index.ts:46-82. A 365-day year runs the proleptic
+// Gregorian calendar from day 0 of year 0; any other year length the
+// model's 360-day calendar of twelve 30-day months.
 Date date_of(int64_t step, int steps_per_day, int days_per_year, double step_minutes) {
+  if (days_per_year == 365) {
+    const int64_t days = step / steps_per_day;
+    int64_t year = (int64_t)std::trunc((double)days / MEAN_GREGORIAN_YEAR);
+    while (gregorian_year_start(year) > days) year--;
+    while (gregorian_year_start(year + 1) <= days) year++;
+    int64_t day = days - gregorian_year_start(year) + 1;
+    int month = 1;
+    while (day > gregorian_month_days(year, month)) day -= gregorian_month_days(year, month++);
+    const int64_t minutes = (step % steps_per_day) * 1440 / steps_per_day;
+    return {(int)year, month, (int)day, (int)(minutes / 60), (int)(minutes % 60)};
+  }
   const int64_t mtspd = (int64_t)days_per_year * steps_per_day / 360;
   const int64_t days = step / mtspd;
   const int64_t doy = days % 360;
@@ -771,10 +871,11 @@ struct MonthInterp {
 MonthInterp month_interp(int64_t step, const Calendar& c) {
   const Date d = date_of(step, c.steps_per_day, c.days_per_year, c.step_minutes);
   const double fday = d.day + ((d.hour * 60 + d.minute) * 60) / c.solar_day;
-  const double cur = c.days_per_month;
+  const bool gregorian = c.days_per_year == 365;
+  const double cur = gregorian ? gregorian_month_days(d.year, d.month) : c.days_per_month;
   const double median = 0.5 * (cur + 1);
   const int neighbour = fday > median ? d.month + 1 : d.month - 1;
-  const double adj = c.days_per_month;
+  const double adj = gregorian ? gregorian_month_days(d.year, (neighbour + 11) % 12 + 1) : c.days_per_month;
   return {d.month, neighbour, std::fabs(fday - median) / (median + 0.5 * (adj + 1) - 1)};
 }
 
@@ -941,6 +1042,13 @@ void config_default(exo_config* c) {
   c->snow_ice_albedo = NAN;
   c->wet_soil = 0;
   c->filter = 2;
+  // the base plasim_namelist's KICK and SEED, surfmod's NSPINIT default
+  c->initial_state = 0;
+  c->kick = 1;
+  static const int32_t seed[8] = {14963991, 2, 3, 4, 5, 6, 7, 8};
+  std::copy(seed, seed + 8, c->seed);
+  c->clock_ms = 0;
+  c->nspinit = 0;
 }
 
 std::vector<double> gaussian_degrees(int nlat) { return gaussian_grid(nlat).degrees; }
@@ -958,6 +1066,8 @@ void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fie
   if (cfg.levels != 5 && cfg.levels != 10) throw std::runtime_error("Choose 5 or 10 atmospheric layers");
   if (!(cfg.timestep_minutes > 0)) throw std::runtime_error("timestep must be positive");
   if (!(cfg.land_fraction >= 0 && cfg.land_fraction <= 1)) throw std::runtime_error("land fraction must be in [0, 1]");
+  if (cfg.initial_state != 0 && cfg.initial_state != 1) throw std::runtime_error("initial_state must be 0 or 1");
+  if (cfg.kick < 0 || cfg.kick > 3) throw std::runtime_error("KICK must be 0 to 3");
   m.cfg = cfg;
   const int T = cfg.truncation, L = cfg.levels;
   const int NLAT = latitudes_for(T), NLON = 2 * NLAT, NHOR = NLAT * NLON;
@@ -1125,7 +1235,7 @@ void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fie
   const int carbon_interval = (int)std::trunc(p.mtspd / carbon.frequency);
   const double timeweight = 1 / (carbon.frequency * p.m_days_per_year);
   const double mldepth = cfg.ocean_depth_m;
-  const int nspinit = 1, noromax = T;
+  const int noromax = T;
   // vegmod_namelist (setup/modules/index.ts:234-247; RINIDAGG edited)
   struct {
     double ncveg = 1, forgrow = 1, rinidagg = 0.5, rinidsc = 1, rinidmr = 2, rinisoil = 0, riniveg = 0;
@@ -1659,6 +1769,37 @@ void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fie
   const std::vector<double> sr = setzt(dyn, p, T);
   std::copy(sr.begin(), sr.end(), tab(TAB_srp));
 
+  if (cfg.initial_state == 1) {
+    // The reference's fresh start (setup/index.ts:434-449): initfd after
+    // setzt (setup/state/index.ts:60-86), then ln(ps) from the orography
+    // when nspinit > 0.
+    double* sp = slot(SLOT_sp);
+    if (cfg.nspinit > 0) {
+      const double scale = p.cv * p.cv / (p.gascon * p.tgr);
+      for (int i = 0; i < NRSP; i++) sp[i] = -so[i] * scale;
+    } else {
+      for (int i = 0; i < NRSP; i++) sp[i] = 0;
+      initfd_noise(sp, T, cfg.kick, initrandom_seed(cfg));
+    }
+    std::copy(sp, sp + NRSP, slot(SLOT_spm));
+    for (int l = 0; l < L; l++) {
+      double* st = slot(SLOT_st) + (size_t)l * NRSP;
+      double* sz = slot(SLOT_sz) + (size_t)l * NRSP;
+      double* sq = slot(SLOT_sq) + (size_t)l * NRSP;
+      double* sd = slot(SLOT_sd) + (size_t)l * NRSP;
+      for (int i = 0; i < NRSP; i++) st[i] = sz[i] = sq[i] = sd[i] = 0;
+      if (p.dttl != 0)
+        std::copy(sr.begin() + (size_t)l * NRSP, sr.begin() + (size_t)(l + 1) * NRSP, st);
+      else
+        st[0] = sr[(size_t)l * NRSP];
+      sz[2] = p.plavor;
+    }
+    const size_t n = (size_t)L * NRSP;
+    std::copy(slot(SLOT_st), slot(SLOT_st) + n, slot(SLOT_stm));
+    std::copy(slot(SLOT_sz), slot(SLOT_sz) + n, slot(SLOT_szm));
+    std::copy(slot(SLOT_sq), slot(SLOT_sq) + n, slot(SLOT_sqm));
+    std::copy(slot(SLOT_sd), slot(SLOT_sd) + n, slot(SLOT_sdm));
+  } else {
   // Spectral initial state: the isothermal atmosphere at rest. ln(ps)
   // follows the orography (initfd with nspinit > 0, setup/index.ts:
   // 434-449); temperature is initial_temperature plus seeded grid-point
@@ -1689,6 +1830,7 @@ void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fie
     std::copy(slot(SLOT_st), slot(SLOT_st) + n, slot(SLOT_stm));
     std::copy(slot(SLOT_sz), slot(SLOT_sz) + n, slot(SLOT_szm));
     std::copy(slot(SLOT_sq), slot(SLOT_sq) + n, slot(SLOT_sqm));
+  }
   }
   m.initial = A;
 
@@ -1895,7 +2037,6 @@ void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fie
   // The tables hold the first phase's bm1 (setup/index.ts:470-474).
   std::copy(m.bm1[0].begin(), m.bm1[0].end(), tab(TAB_bm1));
   (void)count_of;
-  (void)nspinit;
 }
 
 void step_values(const Member& m, int64_t step, int64_t carbon_count, int publish, double* out) {

@@ -24,8 +24,9 @@ struct TableLayout {
   size_t total = 0;
 };
 
-// Fixed-orbit calendar (radmod orb_params with nfixorb = 1 and the
-// simplified 360-day calendar of calendar/date/index.ts).
+// Fixed-orbit calendar (radmod orb_params with nfixorb = 1) and the date
+// rules of calendar/date/index.ts: the 360-day calendar, or the Gregorian
+// one for a 365-day year.
 struct Calendar {
   int steps_per_day = 0;        // ntspd
   int64_t steps_per_year = 0;   // n_steps_per_year
@@ -72,7 +73,8 @@ void config_default(exo_config* cfg);
 // where the fraction exceeds 0.5) instead of the seeded terrain.
 void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fields = nullptr, int count = 0);
 // Calendar date {year, month, day, hour, minute} of main-loop step `step`
-// (calendar/date/index.ts:46-82, 360-day calendar).
+// (calendar/date/index.ts:46-82: 360-day calendar, or Gregorian for a
+// 365-day year).
 void member_date(const Member& m, int64_t step, int out[5]);
 // Gaussian latitudes in degrees, north first (GCM_GRID.forLatitudes'
 // latitudesDeg, grid/index.ts:32-82).

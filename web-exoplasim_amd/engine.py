@@ -76,6 +76,11 @@ class ExoConfig(ctypes.Structure):
         ("snow_ice_albedo", ctypes.c_double),
         ("wet_soil", ctypes.c_int),
         ("filter", ctypes.c_int),
+        ("initial_state", ctypes.c_int),
+        ("kick", ctypes.c_int),
+        ("seed", ctypes.c_int32 * 8),
+        ("clock_ms", ctypes.c_int64),
+        ("nspinit", ctypes.c_int),
     ]
 
 
@@ -180,6 +185,7 @@ def library():
         "exo_sync": (I, [P]),
         "exo_member_step": (I, [P, I, ctypes.POINTER(ctypes.c_int64)]),
         "exo_member_date": (I, [P, I, ctypes.POINTER(I)]),
+        "exo_member_date_at": (I, [P, I, ctypes.c_int64, ctypes.POINTER(I)]),
         "exo_arena_floats": (I, [P, I, ctypes.POINTER(S)]),
         "exo_slot": (I, [P, I, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
         "exo_slot_name": (ctypes.c_char_p, [I]),
@@ -520,6 +526,12 @@ class Engine:
         """(year, month, day, hour, minute) of the member's step counter."""
         d = (ctypes.c_int * 5)()
         _check(self.lib.exo_member_date(self.h, member, d))
+        return tuple(d)
+
+    def member_date_at(self, member, step):
+        """(year, month, day, hour, minute) of model step `step`."""
+        d = (ctypes.c_int * 5)()
+        _check(self.lib.exo_member_date_at(self.h, member, step, d))
         return tuple(d)
 
     # layout and fields --------------------------------------------------

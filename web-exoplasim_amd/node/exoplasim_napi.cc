@@ -113,7 +113,7 @@ napi_value float_array(napi_env env, const std::vector<float>& data) {
 // exo_config fields by their C names; the JS side maps GcmConfig onto them.
 struct Field {
   const char* name;
-  char kind;  // i: int, d: double, u: uint64
+  char kind;  // i: int, d: double, u: uint64, l: int64, s: int32[8] (an array)
   size_t offset;
 };
 #define EXO_FIELD(k, n) {#n, k, offsetof(exo_config, n)}
@@ -133,7 +133,8 @@ const Field kConfigFields[] = {
     EXO_FIELD('d', initial_mean_anomaly_deg), EXO_FIELD('d', soil_depth_scale),
     EXO_FIELD('d', soil_water_capacity_m), EXO_FIELD('d', soil_heat_capacity), EXO_FIELD('d', land_albedo),
     EXO_FIELD('d', ocean_albedo), EXO_FIELD('d', snow_ice_albedo), EXO_FIELD('i', wet_soil),
-    EXO_FIELD('i', filter),
+    EXO_FIELD('i', filter), EXO_FIELD('i', initial_state), EXO_FIELD('i', kick),
+    EXO_FIELD('s', seed), EXO_FIELD('l', clock_ms), EXO_FIELD('i', nspinit),
 };
 #undef EXO_FIELD
 
@@ -144,8 +145,17 @@ napi_value config_default(napi_env env, napi_callback_info) {
   NAPI_OK(napi_create_object(env, &obj));
   for (const Field& f : kConfigFields) {
     const char* at = reinterpret_cast<const char*>(&c) + f.offset;
+    if (f.kind == 's') {
+      napi_value arr;
+      NAPI_OK(napi_create_array_with_length(env, 8, &arr));
+      for (uint32_t j = 0; j < 8; j++)
+        NAPI_OK(napi_set_element(env, arr, j, number(env, reinterpret_cast<const int32_t*>(at)[j])));
+      NAPI_OK(napi_set_named_property(env, obj, f.name, arr));
+      continue;
+    }
     double x = f.kind == 'i' ? *reinterpret_cast<const int*>(at)
                : f.kind == 'd' ? *reinterpret_cast<const double*>(at)
+               : f.kind == 'l' ? (double)*reinterpret_cast<const int64_t*>(at)
                                : (double)*reinterpret_cast<const uint64_t*>(at);
     NAPI_OK(napi_set_named_property(env, obj, f.name, number(env, x)));
   }
@@ -174,7 +184,36 @@ bool config_from(napi_env env, napi_value obj, exo_config& c) {
     }
     napi_get_property(env, obj, k, &v);
     char* at = reinterpret_cast<char*>(&c) + f->offset;
-    if (f->kind == 'u') {
+    if (f->kind == 's') {
+      bool is_array = false;
+      uint32_t count = 0;
+      if (napi_is_array(env, v, &is_array) != napi_ok || !is_array || napi_get_array_length(env, v, &count) != napi_ok ||
+          count > 8) {
+        napi_throw_type_error(env, nullptr, (std::string(name) + " must be an array of at most 8 numbers").c_str());
+        return false;
+      }
+      int32_t* words = reinterpret_cast<int32_t*>(at);
+      for (uint32_t j = 0; j < 8; j++) {
+        double d = 0;
+        if (j < count) {
+          napi_value e;
+          napi_get_element(env, v, j, &e);
+          if (napi_get_value_double(env, e, &d) != napi_ok) {
+            napi_throw_type_error(env, nullptr, (std::string(name) + " must be an array of at most 8 numbers").c_str());
+            return false;
+          }
+        }
+        // Fortran reads the list from the front and leaves the rest zero
+        words[j] = (int32_t)(uint32_t)(int64_t)d;
+      }
+    } else if (f->kind == 'l') {
+      double d = 0;
+      if (napi_get_value_double(env, v, &d) != napi_ok) {
+        napi_throw_type_error(env, nullptr, (std::string(name) + " must be a number").c_str());
+        return false;
+      }
+      *reinterpret_cast<int64_t*>(at) = (int64_t)d;
+    } else if (f->kind == 'u') {
       bool lossless = false;
       uint64_t x = 0;
       if (napi_get_value_bigint_uint64(env, v, &x, &lossless) != napi_ok) {

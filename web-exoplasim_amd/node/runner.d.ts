@@ -128,6 +128,10 @@ export interface SyntheticPlanet {
 	co2Ppm?: number
 	terrainSeed?: number | bigint
 	noiseSeed?: number | bigint
+	// true: start from the seeded synthetic state (isothermal at
+	// initialTemperatureK, initialHumidity, ln(ps) from the orography)
+	// instead of the reference's initfd
+	syntheticStart?: boolean
 }
 
 export interface GcmRunRequest {

@@ -133,6 +133,11 @@ function engineConfig(config, planet) {
 		sea_surface_k: p.seaSurfaceK === null ? NaN : p.seaSurfaceK,
 		terrain_seed: p.terrainSeed,
 		noise_seed: p.noiseSeed,
+		// A fresh start is the reference's prolog: initfd with the base
+		// namelist's KICK 1 and SEED (setup/index.ts:434-449); the seeded
+		// synthetic state (isothermal, humid, ln(ps) from the orography)
+		// only when planet.syntheticStart asks for it.
+		initial_state: p.syntheticStart ? 0 : 1,
 	}
 	if (p.co2Ppm !== undefined) extra.co2_ppm = p.co2Ppm
 	const merged = Object.assign(out, extra)
