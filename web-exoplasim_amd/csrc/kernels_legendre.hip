@@ -336,11 +336,59 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
 }
 
 // ------------------------------------------------------------- forward
+// As fc2sp folds them (forward.wgsl:24-36), the sums run over the northern
+// latitudes of the folded rows E = F(north) + F(south) and O = F(north) -
+// F(south): with the table's parity across the equator (as for the
+// inverse), mode k = n - m of a table of P takes E for even k and O for
+// odd k, a table of dP/dmu (qe) the other. The staged table rows are the
+// even modes of m (padded to a multiple of 16 rows), then its odd ones,
+// so each 16-row output tile reads E or O per block.
 
 // forward staging with a fixed column per thread: table half x 64 (8 rows
-// per pass), inputs rows x 2 NLAT (<= 192: 2 rows per pass)
+// per pass); inputs rows x (2 parts x 48 latitude pairs) (5 rows per pass),
+// each thread loading a latitude and its mirror
 constexpr int kFwdColTabN = (kMaxLat / 2 + kGroupThreads / 64 - 1) / (kGroupThreads / 64);
+// (the split form stages unfolded rows: 2 NLAT <= 192 columns, 2 rows a pass)
 constexpr int kFwdColRawN = (kFwdRows + kGroupThreads / (2 * kMaxLat) - 1) / (kGroupThreads / (2 * kMaxLat));
+constexpr int kFwdFoldS = kGroupThreads / kMaxLat;
+constexpr int kFwdFoldN = (kFwdRows + kFwdFoldS - 1) / kFwdFoldS;
+
+// The chunk's folded inputs: thread t takes latitude pair column c = t %
+// 96 (part c / half, northern latitude c % half) of rows t / 96 + 5 j.
+template <typename Rows>
+__device__ __forceinline__ void fold_load(int rows, int nlat, int plane_stride, Rows rowp, float (*v)[2]) {
+  const int half = nlat >> 1;
+  const int c = threadIdx.x % kMaxLat, r0 = threadIdx.x / kMaxLat;
+  const int part = c >= half, l = c - part * half;
+  const bool ok = r0 < kFwdFoldS && c < 2 * half;
+#pragma unroll
+  for (int j = 0; j < kFwdFoldN; j++) {
+    const int r = r0 + kFwdFoldS * j;
+    if (ok && r < rows) {
+      const float* x = rowp(r) + part * plane_stride;
+      v[j][0] = x[l];
+      v[j][1] = x[nlat - 1 - l];
+    } else {
+      v[j][0] = v[j][1] = 0.0f;
+    }
+  }
+}
+
+// E at raw[row][2 l + part], O at raw[row][kMaxLat + 2 l + part]
+__device__ __forceinline__ void fold_store(int rows, int nlat, float* raw, const float (*v)[2]) {
+  const int half = nlat >> 1;
+  const int c = threadIdx.x % kMaxLat, r0 = threadIdx.x / kMaxLat;
+  const int part = c >= half, l = c - part * half;
+  if (r0 >= kFwdFoldS || c >= 2 * half) return;
+#pragma unroll
+  for (int j = 0; j < kFwdFoldN; j++) {
+    const int r = r0 + kFwdFoldS * j;
+    if (r < rows) {
+      raw[r * kRawLat + 2 * l + part] = v[j][0] + v[j][1];
+      raw[r * kRawLat + kMaxLat + 2 * l + part] = v[j][0] - v[j][1];
+    }
+  }
+}
 
 template <int P>
 __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, const LegJob& job, int m, float* Ah,
@@ -351,20 +399,22 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   const int ncsp = M.ncsp, nrsp = M.nrsp;
   const bool uv = job.kind == LEG_UV2DV;
   const int NCm = uv ? 4 * P : 2 * P;
-  const int MT = (K + 15) / 16;
+  // even modes in rows [0, re16), odd ones in [re16, re16 + ro16)
+  const int re16 = ((K + 1) / 2 + 15) & ~15, ro16 = (K / 2 + 15) & ~15;
+  const int MTe = re16 / 16, MT = (re16 + ro16) / 16;
   const int ash = a_stride(half);
-  // blocks: table and whether its southern rows take the extra minus sign
-  int tables[3], diff[3], blocks = 0;
+  // blocks: table and whether it is a table of dP/dmu (qe)
+  int tables[3], odd[3], blocks = 0;
   if (job.kind == LEG_PLAIN) {
-    tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qc, odd[blocks++] = 0;
   } else if (uv) {
-    tables[blocks] = TAB_legendre_qe, diff[blocks++] = 1;
-    tables[blocks] = TAB_legendre_qm, diff[blocks++] = 0;
-    if (job.ke >= 0) tables[blocks] = TAB_legendre_qq, diff[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qe, odd[blocks++] = 1;
+    tables[blocks] = TAB_legendre_qm, odd[blocks++] = 0;
+    if (job.ke >= 0) tables[blocks] = TAB_legendre_qq, odd[blocks++] = 0;
   } else {
-    tables[blocks] = TAB_legendre_qe, diff[blocks++] = 1;
-    tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
-    tables[blocks] = TAB_legendre_qm, diff[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qe, odd[blocks++] = 1;
+    tables[blocks] = TAB_legendre_qc, odd[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qm, odd[blocks++] = 0;
   }
   const int fields[3] = {job.kind == LEG_PLAIN ? job.src : job.u, job.v, uv ? job.ke : job.src};
   const int nf = job.kind == LEG_PLAIN ? 1 : (uv && job.ke < 0) ? 2 : 3;
@@ -378,42 +428,34 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
     const DevMember& G = mems[group_member(plan, grp, g)];
     rowp[row] = G.arena + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
   }
-  // the members' Fourier coefficients m of every row and level, for the
-  // chunk from group member c0
-  const auto raw_loader = [&](int c0) {
-    const int r0 = c0 * nf * P;
-    // column c: part c / nlat, latitude c % nlat, so a wave reads
-    // consecutive latitudes of one coefficient
-    return [=](int row, int c) {
-      const int part = c >= nlat, lat = c - part * nlat;
-      return c < 2 * nlat ? rowp[r0 + row][part * (P * nlat) + lat] : 0.0f;
-    };
-  };
-  const auto raw_store = [&](int row, int c, float v) {
-    const int part = c >= nlat, lat = c - part * nlat;
-    if (c < 2 * nlat) raw[row * kRawLat + 2 * lat + part] = v;
+  // staged table row r: mode k of m, or -1 for padding
+  const auto row_mode = [=](int r) {
+    const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
+    return k < K ? k : -1;
   };
   __syncthreads();  // rowp
   // one memory latency for the northern half of each table,
-  // Ah[k][l] = q(l, w0 + k), and the first chunk's inputs
+  // Ah[r][l] = q(l, w0 + k(r)), and the first chunk's inputs
   {
-    float va[3][kFwdColTabN], vr[kFwdColRawN];
+    float va[3][kFwdColTabN], vr[kFwdFoldN][2];
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         const float* q = M.ltab + M.tabo[tables[blk]];
-        load_cols<kFwdColTabN, 64>(half, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; },
-                                   va[blk]);
+        load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
+          const int k = row_mode(r);
+          return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
+        }, va[blk]);
       }
-    load_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_loader(0), vr);
+    fold_load(min(chunk, count) * nf * P, nlat, P * nlat, [&](int r) { return rowp[r]; }, vr);
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         float* A = Ah + blk * kFwdAs;
-        store_cols<kFwdColTabN, 64>(half, [&](int l, int k, float v) { if (k < MT * 16) A[k * ash + l] = v; },
+        store_cols<kFwdColTabN, 64>(half, [&](int l, int r, float v) { if (r < MT * 16) A[r * ash + l] = v; },
                                     va[blk]);
       }
-    store_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_store, vr);
+    fold_store(min(chunk, count) * nf * P, nlat, raw, vr);
   }
   __syncthreads();
   for (int c0 = 0; c0 < count; c0 += chunk) {
@@ -421,8 +463,8 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
     const int NC = n * NCm, NT = (NC + 15) / 16;
     // the next chunk's inputs are in flight while this chunk's products run
     const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
-    float vn[kFwdColRawN];
-    if (n1 > 0) load_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_loader(c1), vn);
+    float vn[kFwdFoldN][2];
+    if (n1 > 0) fold_load(n1 * nf * P, nlat, P * nlat, [&](int r) { return rowp[c1 * nf * P + r]; }, vn);
     f4 acc[kFwdGTiles];
 #pragma unroll
     for (int t = 0; t < kFwdGTiles; t++) acc[t] = f4{0, 0, 0, 0};
@@ -433,17 +475,15 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
         const int tile = wave + kGroupWaves * t;
         if (tile < MT * NT) {
           const int mt = tile / NT, nt = tile % NT;
-          const int k = mt * 16 + i;
-          // southern rows: (-1)^(n-m) q, negated again for qe*
-          const float south = ((k & 1) ? -1.0f : 1.0f) * (diff[blk] ? -1.0f : 1.0f);
+          // E for the symmetric pairs (even mode of a table of P, odd mode
+          // of a table of dP/dmu), O for the others
+          const int eo = ((mt >= MTe) + odd[blk]) & 1;
           const Col col = fwd_col(job.kind, blk, nt * 16 + i, NC, NCm, nf, P);
-          const float* a = A + k * ash;
-          const float* b = raw + col.base + 2 * kk;
+          const float* a = A + (mt * 16 + i) * ash + kk;
+          const float* b = raw + col.base + eo * kMaxLat + 2 * kk;
           f4 c = acc[t];
           for (int r = 0; r < half; r += 4)
-            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r + kk], col.sign * b[2 * r], c, 0, 0, 0);
-          for (int r = half; r < nlat; r += 4)
-            c = __builtin_amdgcn_mfma_f32_16x16x4f32(south * a[nlat - 1 - r - kk], col.sign * b[2 * r], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], col.sign * b[2 * r], c, 0, 0, 0);
           acc[t] = c;
         }
       }
@@ -462,15 +502,15 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
           const int p = uv ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const int k = mt * 16 + 4 * kk + r;
-            if (k < K) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
+            const int k = row_mode(mt * 16 + 4 * kk + r);
+            if (k >= 0) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
           }
         }
       }
     }
     if (n1 > 0) {
       __syncthreads();  // every wave is done with this chunk's inputs
-      store_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_store, vn);
+      fold_store(n1 * nf * P, nlat, raw, vn);
       __syncthreads();
     }
   }
@@ -727,17 +767,46 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
   }
 }
 
-// Forward: A[k][l] over the northern latitudes (kSplitFwdAs floats a row,
-// split in place to 2 kSplitFwdAs halves); input rows of 2 NLAT values
-// (kRawLat floats) split in place with part p's groups from p * 2 kMaxLat
-// halves, latitudes reordered north first, then the southern ones from the
-// equator out (q = half + (nlat - 1 - lat)), so a southern depth step
-// pairs the northern table values with their mirror rows:
-//   sum_l A(k, l) B(l) = sum_{l < half} A_N(k, l) (B(l) + s_k B(nlat - 1 - l)),
-// s_k = (-1)^(n - m) (negated again for qe*); the two halves run as
-// separate products, joined with s_k per output row.
+// Forward: the plain form's fold, on split operands. A[r][l] over the
+// northern latitudes, rows in the plain form's order (even modes of m,
+// then odd ones, each padded to 16 rows), kSplitFwdAs floats a row split
+// in place to 2 kSplitFwdAs halves. Each staged input row holds E at
+// [0, 2 half) and O at [kMaxLat, kMaxLat + 2 half) floats, both parts
+// interleaved; each (E or O, part) run of `half` values has its own scale
+// and is split in place within its region, part p's groups from p kMaxLat
+// halves. A 16-row output tile then takes one depth-half product per
+// block against E or O, as in the plain form.
 constexpr int kSplitFwdAs = 68;
 static_assert(kMaxLat / 2 <= kSplitFwdAs && 2 * kMaxLat <= kRawLat, "split forward rows");
+
+// One wave per (row, E or O) run pair: scale[4 row + 2 eo + part] and the
+// split values of both parts, in place.
+__device__ __forceinline__ void split_fold_rows(float* raw, int rows, int half, float* scale) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int rr = wave; rr < 2 * rows; rr += kGroupWaves) {
+    const int r = rr >> 1, eo = rr & 1;
+    float* row = raw + r * kRawLat + eo * kMaxLat;
+    float v[2];
+    float mx = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int x = lane + 64 * j;
+      v[j] = x < 2 * half ? row[x] : 0.0f;
+      mx = fmaxf(mx, fabsf(v[j]));
+    }
+    // lanes of one parity hold one part
+#pragma unroll
+    for (int o = 32; o > 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    const float sc = pow2_scale(mx);
+    if (lane < 2) scale[4 * r + 2 * eo + lane] = sc;
+    _Float16* h = reinterpret_cast<_Float16*>(row);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int x = lane + 64 * j;
+      if (x < 2 * half) put_split(h + (x & 1) * kMaxLat, x >> 1, v[j] * sc);
+    }
+  }
+}
 
 template <int P>
 __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const GroupPlan& plan, int grp, const LegJob& job, int m,
@@ -748,18 +817,19 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
   const int ncsp = M.ncsp, nrsp = M.nrsp;
   const bool uv = job.kind == LEG_UV2DV;
   const int NCm = uv ? 4 * P : 2 * P;
-  const int MT = (K + 15) / 16;
-  int tables[3], diff[3], blocks = 0;
+  const int re16 = ((K + 1) / 2 + 15) & ~15, ro16 = (K / 2 + 15) & ~15;
+  const int MTe = re16 / 16, MT = (re16 + ro16) / 16;
+  int tables[3], odd[3], blocks = 0;
   if (job.kind == LEG_PLAIN) {
-    tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qc, odd[blocks++] = 0;
   } else if (uv) {
-    tables[blocks] = TAB_legendre_qe, diff[blocks++] = 1;
-    tables[blocks] = TAB_legendre_qm, diff[blocks++] = 0;
-    if (job.ke >= 0) tables[blocks] = TAB_legendre_qq, diff[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qe, odd[blocks++] = 1;
+    tables[blocks] = TAB_legendre_qm, odd[blocks++] = 0;
+    if (job.ke >= 0) tables[blocks] = TAB_legendre_qq, odd[blocks++] = 0;
   } else {
-    tables[blocks] = TAB_legendre_qe, diff[blocks++] = 1;
-    tables[blocks] = TAB_legendre_qc, diff[blocks++] = 0;
-    tables[blocks] = TAB_legendre_qm, diff[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qe, odd[blocks++] = 1;
+    tables[blocks] = TAB_legendre_qc, odd[blocks++] = 0;
+    tables[blocks] = TAB_legendre_qm, odd[blocks++] = 0;
   }
   const int fields[3] = {job.kind == LEG_PLAIN ? job.src : job.u, job.v, uv ? job.ke : job.src};
   const int nf = job.kind == LEG_PLAIN ? 1 : (uv && job.ke < 0) ? 2 : 3;
@@ -771,52 +841,43 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
     const DevMember& G = mems[group_member(plan, grp, g)];
     rowp[row] = G.arena + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
   }
-  const auto raw_loader = [&](int c0) {
-    const int r0 = c0 * nf * P;
-    return [=](int row, int c) {
-      const int part = c >= nlat, lat = c - part * nlat;
-      return c < 2 * nlat ? rowp[r0 + row][part * (P * nlat) + lat] : 0.0f;
-    };
-  };
-  const auto raw_store = [&](int row, int c, float v) {
-    const int part = c >= nlat, lat = c - part * nlat;
-    if (c < 2 * nlat) raw[row * kRawLat + 2 * lat + part] = v;
-  };
-  const auto split_inputs = [&](int rows) {
-    split_rows(raw, rows, kRawLat, 2 * nlat, 2 * kMaxLat, bscale,
-               [=](int lat) { return lat < half ? lat : half + (nlat - 1 - lat); });
+  const auto row_mode = [=](int r) {
+    const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
+    return k < K ? k : -1;
   };
   __syncthreads();  // rowp
   {
-    float va[3][kFwdColTabN], vr[kFwdColRawN];
+    float va[3][kFwdColTabN], vr[kFwdFoldN][2];
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         const float* q = M.ltab + M.tabo[tables[blk]];
-        load_cols<kFwdColTabN, 64>(half, [&](int l, int k) { return k < K ? q[l * ncsp + w0 + k] : 0.0f; },
-                                   va[blk]);
+        load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
+          const int k = row_mode(r);
+          return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
+        }, va[blk]);
       }
-    load_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_loader(0), vr);
+    fold_load(min(chunk, count) * nf * P, nlat, P * nlat, [&](int r) { return rowp[r]; }, vr);
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         float* A = Ah + blk * kMaxK * kSplitFwdAs;
-        store_cols<kFwdColTabN, 64>(half, [&](int l, int k, float v) { if (k < MT * 16) A[k * kSplitFwdAs + l] = v; },
+        store_cols<kFwdColTabN, 64>(half, [&](int l, int r, float v) { if (r < MT * 16) A[r * kSplitFwdAs + l] = v; },
                                     va[blk]);
       }
-    store_cols<kFwdColRawN, 2 * kMaxLat>(min(chunk, count) * nf * P, raw_store, vr);
+    fold_store(min(chunk, count) * nf * P, nlat, raw, vr);
   }
   __syncthreads();
   for (int blk = 0; blk < blocks; blk++)
     split_table(Ah + blk * kMaxK * kSplitFwdAs, MT * 16, kSplitFwdAs, half, M.lscale[tables[blk] - TAB_legendre_qc]);
-  split_inputs(min(chunk, count) * nf * P);
+  split_fold_rows(raw, min(chunk, count) * nf * P, half, bscale);
   __syncthreads();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
     const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
-    float vn[kFwdColRawN];
-    if (n1 > 0) load_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_loader(c1), vn);
+    float vn[kFwdFoldN][2];
+    if (n1 > 0) fold_load(n1 * nf * P, nlat, P * nlat, [&](int r) { return rowp[c1 * nf * P + r]; }, vn);
     const _Float16* rawh = reinterpret_cast<const _Float16*>(raw);
     f4 acc[kFwdGTiles];
 #pragma unroll
@@ -824,25 +885,20 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
     for (int blk = 0; blk < blocks; blk++) {
       const _Float16* A = reinterpret_cast<const _Float16*>(Ah + blk * kMaxK * kSplitFwdAs);
       const float sa = M.lscale[tables[blk] - TAB_legendre_qc];
-      // s_k of output row k = mt 16 + 4 kk + r: (-1)^r (mt 16 + 4 kk is even)
-      const float sd = diff[blk] ? -1.0f : 1.0f;
 #pragma unroll
       for (int t = 0; t < kFwdGTiles; t++) {
         const int tile = wave + kGroupWaves * t;
         if (tile < MT * NT) {
           const int mt = tile / NT, nt = tile % NT;
+          const int eo = ((mt >= MTe) + odd[blk]) & 1;
           const Col col = fwd_col(job.kind, blk, nt * 16 + i, NC, NCm, nf, P);
           const int brow = col.base / kRawLat, part = col.base - brow * kRawLat;
           const _Float16* a = A + (mt * 16 + i) * (2 * kSplitFwdAs) + 2 * (4 * kk);
-          const _Float16* b = rawh + brow * (2 * kRawLat) + part * (2 * kMaxLat) + 2 * (4 * kk);
-          f4 cn = f4{0, 0, 0, 0}, cs = f4{0, 0, 0, 0};
-          for (int r = 0; r < half; r += 16) {
-            const h8 av = load_h8(a + 2 * r);
-            cn = mfma_split(av, load_h8(b + 2 * r), cn);
-            cs = mfma_split(av, load_h8(b + 2 * (half + r)), cs);
-          }
-          const float f = col.sign == 0.0f ? 0.0f : col.sign / (sa * bscale[2 * brow + part]);
-          acc[t] += f * (cn + sd * f4{cs[0], -cs[1], cs[2], -cs[3]});
+          const _Float16* b = rawh + brow * (2 * kRawLat) + eo * (2 * kMaxLat) + part * kMaxLat + 2 * (4 * kk);
+          f4 c = f4{0, 0, 0, 0};
+          for (int r = 0; r < half; r += 16) c = mfma_split(load_h8(a + 2 * r), load_h8(b + 2 * r), c);
+          const float f = col.sign == 0.0f ? 0.0f : col.sign / (sa * bscale[4 * brow + 2 * eo + part]);
+          acc[t] += f * c;
         }
       }
     }
@@ -860,17 +916,17 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
           const int p = uv ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const int k = mt * 16 + 4 * kk + r;
-            if (k < K) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
+            const int k = row_mode(mt * 16 + 4 * kk + r);
+            if (k >= 0) base[p * nrsp + 2 * (w0 + k) + part] = acc[t][r];
           }
         }
       }
     }
     if (n1 > 0) {
       __syncthreads();  // every wave is done with this chunk's inputs
-      store_cols<kFwdColRawN, 2 * kMaxLat>(n1 * nf * P, raw_store, vn);
+      fold_store(n1 * nf * P, nlat, raw, vn);
       __syncthreads();
-      split_inputs(n1 * nf * P);
+      split_fold_rows(raw, n1 * nf * P, half, bscale);
       __syncthreads();
     }
   }
@@ -947,7 +1003,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   __shared__ __align__(16) float Ah[3 * kMaxK * kSplitFwdAs];
   __shared__ __align__(16) float raw[kFwdRows * kRawLat];
   __shared__ const float* rowp[kLegGroupRows];
-  __shared__ float bscale[2 * kFwdRows];
+  __shared__ float bscale[4 * kFwdRows];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
