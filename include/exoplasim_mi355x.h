@@ -140,6 +140,16 @@ typedef struct exo_engine exo_engine;
 int exo_create(int device, int max_members, exo_engine **out);
 void exo_destroy(exo_engine *e);
 
+/* Evaluates the column physics' transcendental routines on the device
+ * over n host values (a, and b for the powers), writing n results: the
+ * counterparts of the reference's accurate f32 math (GPU_MATH,
+ * shared/gpu/math/wgsl/precise.wgsl:1-103), for callers that check their
+ * accuracy. function: 0 log2 and 1 pow(a, b) of positive normal a on the
+ * hardware's v_log_f32/v_exp_f32 (the longwave and shortwave absorber
+ * paths, the surface layer), 2 log10 and 3 ln likewise, 4 e^x for x <= 0
+ * (exp2 of x log2 e), 5 expf, 6 logf and 7 powf of the device library. */
+int exo_math_eval(exo_engine *e, int function, const float *a, const float *b, float *out, size_t n);
+
 /*
  * Adds a member configured from *cfg (FP64 set-up on the host: pumamod,
  * Gaussian grid, Legendre and Fourier tables, initpm/initsi/makebm/setzt,

@@ -19,6 +19,7 @@
 #include "transforms.h"
 
 extern "C" __global__ void k_leg_inv(const DevMember*, GroupPlan, InvJobs);
+extern "C" __global__ void k_math_eval(int, const float*, const float*, float*, int);
 extern "C" __global__ void k_dft_inv(const DevMember*, GroupPlan, InvJobs);
 extern "C" __global__ void k_dft_fwd(const DevMember*, GroupPlan, FwdJobs);
 extern "C" __global__ void k_leg_fwd(const DevMember*, GroupPlan, LegJobs);

@@ -1142,6 +1142,32 @@ int exo_read_snapshot(exo_engine* h, int id, float* out, size_t floats) {
   return EXO_OK;
 }
 
+int exo_math_eval(exo_engine* h, int function, const float* a, const float* b, float* out, size_t n) {
+  if (!h || !a || !out || function < 0 || function > 7 || n > (size_t)1 << 28)
+    return fail(EXO_E_ARG, "exo_math_eval: function 0..7, inputs, output and n <= 2^28 required");
+  if ((function == 1 || function == 7) && !b) return fail(EXO_E_ARG, "exo_math_eval: powers need exponents");
+  Engine& e = h->e;
+  if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
+  if (n == 0) return EXO_OK;
+  try {
+    HIP_OR_THROW(hipSetDevice(e.device));
+    float* d = nullptr;
+    HIP_OR_THROW(hipMallocAsync((void**)&d, 3 * n * sizeof(float), e.stream));
+    HIP_OR_THROW(hipMemcpyAsync(d, a, n * sizeof(float), hipMemcpyHostToDevice, e.stream));
+    // without exponents the kernel reads the inputs again as its unused b
+    if (b) HIP_OR_THROW(hipMemcpyAsync(d + n, b, n * sizeof(float), hipMemcpyHostToDevice, e.stream));
+    hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e.stream, function, d,
+                       b ? d + n : d, d + 2 * n, (int)n);
+    HIP_OR_THROW(hipGetLastError());
+    HIP_OR_THROW(hipMemcpyAsync(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, e.stream));
+    HIP_OR_THROW(hipFreeAsync(d, e.stream));
+    HIP_OR_THROW(hipStreamSynchronize(e.stream));
+  } catch (const std::exception& ex) {
+    return fail(EXO_E_DEVICE, ex.what());
+  }
+  return EXO_OK;
+}
+
 int exo_global_means(exo_engine* h, int id, double* tsurf, double* toa) {
   MemberDev* m = member_of(h, id);
   if (!m || !tsurf || !toa) return fail(EXO_E_ARG, "no member " + std::to_string(id));

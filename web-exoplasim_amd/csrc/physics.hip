@@ -2151,6 +2151,29 @@ EXO_COLUMN_KERNEL(k_physics_c, physics_c)
 EXO_COLUMN_KERNEL(k_physics_d, physics_d)
 EXO_COLUMN_KERNEL(k_storm, storm_diagnostics)
 
+// The column physics' transcendental routines over arrays, for callers
+// that check their accuracy (exo_math_eval): fn 0 log2_pos, 1 pow_pos(a,
+// b), 2 log10_pos, 3 log_pos, 4 exp_neg, 5 the library's expf, 6 logf,
+// 7 powf(a, b).
+extern "C" __global__ __launch_bounds__(256) void k_math_eval(int fn, const float* __restrict__ a,
+                                                             const float* __restrict__ b, float* __restrict__ out, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i], y = b[i];
+  float r;
+  switch (fn) {
+    case 0: r = log2_pos(x); break;
+    case 1: r = pow_pos(x, y); break;
+    case 2: r = log10_pos(x); break;
+    case 3: r = log_pos(x); break;
+    case 4: r = exp_neg(x); break;
+    case 5: r = expf(x); break;
+    case 6: r = logf(x); break;
+    default: r = powf(x, y); break;
+  }
+  out[i] = r;
+}
+
 extern "C" __global__ __launch_bounds__(64) void k_mkradv(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int tile;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];

@@ -186,6 +186,7 @@ def library():
         "exo_member_step": (I, [P, I, ctypes.POINTER(ctypes.c_int64)]),
         "exo_member_date": (I, [P, I, ctypes.POINTER(I)]),
         "exo_member_date_at": (I, [P, I, ctypes.c_int64, ctypes.POINTER(I)]),
+        "exo_math_eval": (I, [P, I, F, F, F, S]),
         "exo_arena_floats": (I, [P, I, ctypes.POINTER(S)]),
         "exo_slot": (I, [P, I, ctypes.c_char_p, ctypes.POINTER(S), ctypes.POINTER(S)]),
         "exo_slot_name": (ctypes.c_char_p, [I]),
@@ -533,6 +534,19 @@ class Engine:
         d = (ctypes.c_int * 5)()
         _check(self.lib.exo_member_date_at(self.h, member, step, d))
         return tuple(d)
+
+    def math_eval(self, function, a, b=None):
+        """The device's transcendental routine `function` (exo_math_eval)
+        over float32 arrays."""
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        out = np.empty_like(a)
+        bp = None
+        if b is not None:
+            b = np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float32)
+            bp = b.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        _check(self.lib.exo_math_eval(self.h, function, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), bp,
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), a.size))
+        return out
 
     # layout and fields --------------------------------------------------
     def arena_floats(self, member):
