@@ -30,6 +30,18 @@ one step (the job lists the step enqueues):
     k_longwave (lwr, radstep): reads 8L + 4, writes 8L + 4 -> 16L + 8;
     k_convection (kuo, mkshallow, mkdca's column pass): reads 5L + 3,
       writes 8L + 2 -> 13L + 5.
+  and, counting every plane of each grid slot the routines touch (a slot
+  read and written counts twice):
+    k_rain (mkdca apply, mklsp, mkrain): 14L + 10;
+    k_clouds (mkclouds): 8L + 10;
+    k_physics_c (land surface and albedo, sea ice to getiflx): 7L + 101;
+    k_physics_d (ice finish, slab ocean, sea surface, carbon, frame out):
+      17L + 79;
+  and for the dynamics' grid-point kernels:
+    k_calcgp: reads gp, gpj, gpmt and 6 level fields, writes gp, 12 level
+      fields, gvpp and dp0 -> 18L + 6;
+    k_physics_in: reads gp and 4 level fields, zeroes 4 half-level and 4
+      level tendencies, writes 4 level fields and dp -> 16L + 7.
 """
 import os
 
@@ -38,13 +50,11 @@ FP16_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA: the split form's bound (its 3 prod
 HBM_PEAK_GBS = 8000.0
 
 # The kernel bench.py times by default: the one with the largest share of
-# device time in that workload's rocprofv3 kernel trace.
-# (ensemble: k_leg_fwd 10.8%, k_leg_inv 9.4%, k_surface_fluxes and
-# k_longwave 8.8%, k_dft_inv 8.2% of device time in
-# profiles/ensemble64_fp32_kernel_trace.md)
-DOMINANT = {"batched": "k_leg_fwd", "plain": "k_leg_inv"}
+# device time in that workload's rocprofv3 kernel trace
+# (profiles/ensemble64_fp32_kernel_trace.md).
+DOMINANT = {"batched": "k_leg_inv", "plain": "k_leg_inv"}
 # further kernels the bench line reports beside the dominant one
-SECONDARY = {"batched": ("k_leg_inv", "k_dft_inv", "k_dft_fwd", "k_surface_fluxes"), "plain": ()}
+SECONDARY = {"batched": ("k_leg_fwd", "k_longwave", "k_dft_inv", "k_surface_fluxes", "k_dft_fwd"), "plain": ()}
 
 
 def _dims(cfg):
@@ -70,7 +80,9 @@ def work_per_step(kernel, cfg):
     if kernel == "k_dft_fwd":
         return "byte", 4 * (nlon + nfc) * nlat * (15 * L + 1)
     column = {"k_surface_fluxes": 16 * L + 18, "k_shortwave": 9 * L + 14, "k_longwave": 16 * L + 8,
-              "k_convection": 13 * L + 5}
+              "k_convection": 13 * L + 5, "k_rain": 14 * L + 10, "k_clouds": 8 * L + 10,
+              "k_physics_c": 7 * L + 101, "k_physics_d": 17 * L + 79, "k_calcgp": 18 * L + 6,
+              "k_physics_in": 16 * L + 7}
     if kernel in column:
         return "byte", 4 * column[kernel] * nlat * nlon
     raise KeyError(f"no algorithmic work model for {kernel}")
@@ -89,7 +101,8 @@ def bytes_per_step(kernel, cfg):
 
 
 LAUNCHES_PER_STEP = {"k_leg_inv": 3, "k_dft_inv": 3, "k_leg_fwd": 3, "k_dft_fwd": 3, "k_surface_fluxes": 1,
-                     "k_shortwave": 1, "k_longwave": 1, "k_convection": 1}
+                     "k_shortwave": 1, "k_longwave": 1, "k_convection": 1, "k_rain": 1, "k_clouds": 1,
+                     "k_physics_c": 1, "k_physics_d": 1, "k_calcgp": 1, "k_physics_in": 1}
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
