@@ -1201,46 +1201,6 @@ __device__ __forceinline__ void mkdca(const DevMember& M, int h) {
   F(dca_limited)[h] = limited;
 }
 
-template <int NL>
-__device__ __forceinline__ void mklsp(const DevMember& M, int h) {
-  // physics/precipitation/wgsl/mklsp.wgsl:3-39
-  const int NH = M.nhor;
-  const float DELTSEC2 = K(deltsec2), MOIST = 1.0f / K(rdbrv) - 1.0f;
-  const float dp = F(dp)[h];
-#pragma unroll
-  for (int l = 0; l < NL; l++) {
-    LV(dprsll, l, h) = 0;
-    float dprll = 0;
-    const float dq = LV(dq, l, h);
-    const float zq = dq + LV(dqdt, l, h) * DELTSEC2;
-    const float zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
-    const float zlcpe = K(alv) / (K(acpd) * (1.0f + K(adv) * dq));
-    const float p = dp * T(sigma)[l];
-    float zqsat = sat_capped(M, zt, p);
-    float zcor = 1.0f / (1.0f - MOIST * zqsat);
-    zqsat = zqsat * zcor;
-    if (zqsat < 0) zqsat = 0;
-    if (zq > zqsat) {
-      const float c = K(ra2) * (K(tmelt) - K(ra4));
-      float zdqdt = (zqsat - zq) / (1.0f + zlcpe * c * zqsat * zcor / ((zt - K(ra4)) * (zt - K(ra4))));
-      float ztn = zt - zdqdt * zlcpe;
-      float zqn = zq + zdqdt;
-      zqsat = sat_capped(M, ztn, p);
-      zcor = 1.0f / (1.0f - MOIST * zqsat);
-      zqsat = zqsat * zcor;
-      zdqdt = (zqsat - zqn) / (1.0f + zlcpe * c * zqsat * zcor / ((ztn - K(ra4)) * (ztn - K(ra4))));
-      ztn = ztn - zdqdt * zlcpe;
-      zqn = zqn + zdqdt;
-      const float dqdt = (zqn - zq) / DELTSEC2;
-      const float dtdt = (ztn - zt) / DELTSEC2;
-      LV(dqdt, l, h) = LV(dqdt, l, h) + dqdt;
-      LV(dtdt, l, h) = LV(dtdt, l, h) + dtdt;
-      dprll = -fminf(dqdt, 0.0f) * T(dsigma)[l];
-    }
-    LV(dprll, l, h) = dprll;
-  }
-}
-
 __device__ __forceinline__ float evaporate(const DevMember& M, float* flux, float deficit, float zlcpe, float zqsat,
                                            float zcor, float zt, float ds) {
   // physics/precipitation/wgsl/mkrain.wgsl:8-15
@@ -1251,79 +1211,6 @@ __device__ __forceinline__ float evaporate(const DevMember& M, float* flux, floa
   return zdqdt / ds;
 }
 
-template <int NL>
-__device__ __forceinline__ void mkrain(const DevMember& M, int h) {
-  // physics/precipitation/wgsl/mkrain.wgsl:20-95
-  const int NH = M.nhor;
-  const float DELTSEC2 = K(deltsec2), MOIST = 1.0f / K(rdbrv) - 1.0f;
-  const float ALV = K(alv), ALS = K(als), ACPD = K(acpd), ADV = K(adv), TMELT = K(tmelt);
-  const float dp = F(dp)[h];
-  float zprc = 0, zprl = 0, zprsc = 0, zprsl = 0;
-#pragma unroll
-  for (int l = 0; l < NL; l++) {
-    const float ds = T(dsigma)[l];
-    const float dq = LV(dq, l, h);
-    float zdtdt = 0, zdqdtl = 0, zdqdts = 0;
-    zprc = zprc + LV(dprcl, l, h) - LV(dprscl, l, h);
-    zprl = zprl + LV(dprll, l, h) - LV(dprsll, l, h);
-    zprsc = zprsc + LV(dprscl, l, h);
-    zprsl = zprsl + LV(dprsll, l, h);
-    const int surface = l == NL - 1 ? 2 : 1;
-    for (int below = 0; below < surface; below++) {
-      const float temperature = LV(dt, l + below, h);
-      const float zlcp = (ALS - ALV) / (ACPD * (1.0f + ADV * dq));
-      if (zprsc + zprsl > 0 && temperature > TMELT) {
-        zdtdt = -(zprsc + zprsl) / ds * zlcp + zdtdt;
-        zprc = zprsc + zprc;
-        zprl = zprsl + zprl;
-        zprsc = 0;
-        zprsl = 0;
-      } else if (zprc + zprl > 0 && temperature <= TMELT) {
-        zdtdt = (zprc + zprl) / ds * zlcp + zdtdt;
-        zprsc = zprc + zprsc;
-        zprsl = zprl + zprsl;
-        zprc = 0;
-        zprl = 0;
-      }
-    }
-    const float zq = dq + LV(dqdt, l, h) * DELTSEC2;
-    const float zt = LV(dt, l, h) + LV(dtdt, l, h) * DELTSEC2;
-    float zqsat = sat_capped(M, zt, dp * T(sigma)[l]);
-    const float zcor = 1.0f / (1.0f - MOIST * zqsat);
-    zqsat = zqsat * zcor;
-    if (zqsat < 0) zqsat = 0;
-    if (zq < zqsat) {
-      const float deficit = zqsat - zq;
-      const float water = ALV / (ACPD * (1.0f + ADV * dq));
-      const float ice = ALS / (ACPD * (1.0f + ADV * dq));
-      if (zprl > 0) {
-        const float e = evaporate(M, &zprl, deficit, water, zqsat, zcor, zt, ds);
-        zdtdt = zdtdt - e * water;
-        zdqdtl = zdqdtl + e;
-      }
-      if (zprc > 0) {
-        const float e = evaporate(M, &zprc, deficit, water, zqsat, zcor, zt, ds);
-        zdtdt = zdtdt - e * water;
-        zdqdtl = zdqdtl + e;
-      }
-      if (zprsl > 0) {
-        const float e = evaporate(M, &zprsl, deficit, ice, zqsat, zcor, zt, ds);
-        zdtdt = zdtdt - e * ice;
-        zdqdts = zdqdts + e;
-      }
-      if (zprsc > 0) {
-        const float e = evaporate(M, &zprsc, deficit, ice, zqsat, zcor, zt, ds);
-        zdtdt = zdtdt - e * ice;
-        zdqdts = zdqdts + e;
-      }
-    }
-    LV(dqdt, l, h) = LV(dqdt, l, h) + zdqdtl + zdqdts;
-    LV(dtdt, l, h) = LV(dtdt, l, h) + zdtdt;
-  }
-  F(dprc)[h] = (zprc + zprsc) * dp / K(ga) / 1000.0f;
-  F(dprl)[h] = (zprl + zprsl) * dp / K(ga) / 1000.0f;
-  F(dprs)[h] = (zprsc + zprsl) * dp / K(ga) / 1000.0f;
-}
 
 template <int NL>
 __device__ __forceinline__ void mkclouds(const DevMember& M, int h) {
@@ -2056,19 +1943,126 @@ __device__ __forceinline__ void convection(const DevMember& M, int h) {
   mkdca<NL>(M, h);
 }
 
+// The rain column (physics/precipitation/wgsl/column.wgsl:12-27 up to
+// mkrain): mkdca's apply, mklsp (mklsp.wgsl:3-39) and mkrain
+// (mkrain.wgsl:20-95) in one pass down the column. Each routine treats a
+// level on its own but for mkrain's running fluxes, which run top down,
+// so the three run level by level, each level's tendencies kept in
+// registers between them: every value gets the same operations in the
+// same order as three passes over the column, and each level field is read
+// and written once.
 template <int NL>
 __device__ __forceinline__ void rain(const DevMember& M, int h) {
-  // physics/precipitation/wgsl/column.wgsl:12-27 up to mkrain
   const int NH = M.nhor;
-  if (F(dca_skip)[0] == 0.0f) {
+  const float DELTSEC2 = K(deltsec2), MOIST = 1.0f / K(rdbrv) - 1.0f;
+  const float ALV = K(alv), ALS = K(als), ACPD = K(acpd), ADV = K(adv), TMELT = K(tmelt);
+  const bool dca = F(dca_skip)[0] == 0.0f;
+  const float dp = F(dp)[h];
+  float zprc = 0, zprl = 0, zprsc = 0, zprsl = 0;
 #pragma unroll
-    for (int l = 0; l < NL; l++) {
-      LV(dtdt, l, h) = LV(dtdt, l, h) + LV(dca_dtdt, l, h);
-      LV(dqdt, l, h) = LV(dqdt, l, h) + LV(dca_dqdt, l, h);
+  for (int l = 0; l < NL; l++) {
+    const float ds = T(dsigma)[l];
+    const float dq = LV(dq, l, h), t = LV(dt, l, h);
+    float dtdt = LV(dtdt, l, h), dqdt = LV(dqdt, l, h);
+    // mkdca's apply
+    if (dca) {
+      dtdt = dtdt + LV(dca_dtdt, l, h);
+      dqdt = dqdt + LV(dca_dqdt, l, h);
     }
+    // mklsp: large-scale condensation at level l
+    float dprll = 0;
+    const float dprsll = 0;
+    {
+      const float zq = dq + dqdt * DELTSEC2;
+      const float zt = t + dtdt * DELTSEC2;
+      const float zlcpe = ALV / (ACPD * (1.0f + ADV * dq));
+      const float p = dp * T(sigma)[l];
+      float zqsat = sat_capped(M, zt, p);
+      float zcor = 1.0f / (1.0f - MOIST * zqsat);
+      zqsat = zqsat * zcor;
+      if (zqsat < 0) zqsat = 0;
+      if (zq > zqsat) {
+        const float c = K(ra2) * (TMELT - K(ra4));
+        float zdqdt = (zqsat - zq) / (1.0f + zlcpe * c * zqsat * zcor / ((zt - K(ra4)) * (zt - K(ra4))));
+        float ztn = zt - zdqdt * zlcpe;
+        float zqn = zq + zdqdt;
+        zqsat = sat_capped(M, ztn, p);
+        zcor = 1.0f / (1.0f - MOIST * zqsat);
+        zqsat = zqsat * zcor;
+        zdqdt = (zqsat - zqn) / (1.0f + zlcpe * c * zqsat * zcor / ((ztn - K(ra4)) * (ztn - K(ra4))));
+        ztn = ztn - zdqdt * zlcpe;
+        zqn = zqn + zdqdt;
+        const float lsp_dqdt = (zqn - zq) / DELTSEC2;
+        const float lsp_dtdt = (ztn - zt) / DELTSEC2;
+        dqdt = dqdt + lsp_dqdt;
+        dtdt = dtdt + lsp_dtdt;
+        dprll = -fminf(lsp_dqdt, 0.0f) * ds;
+      }
+    }
+    LV(dprsll, l, h) = dprsll;
+    LV(dprll, l, h) = dprll;
+    // mkrain: the fluxes from above melt or freeze, then evaporate
+    float zdtdt = 0, zdqdtl = 0, zdqdts = 0;
+    const float dprscl = LV(dprscl, l, h);
+    zprc = zprc + LV(dprcl, l, h) - dprscl;
+    zprl = zprl + dprll - dprsll;
+    zprsc = zprsc + dprscl;
+    zprsl = zprsl + dprsll;
+    const int surface = l == NL - 1 ? 2 : 1;
+    for (int below = 0; below < surface; below++) {
+      const float temperature = below ? LV(dt, l + 1, h) : t;
+      const float zlcp = (ALS - ALV) / (ACPD * (1.0f + ADV * dq));
+      if (zprsc + zprsl > 0 && temperature > TMELT) {
+        zdtdt = -(zprsc + zprsl) / ds * zlcp + zdtdt;
+        zprc = zprsc + zprc;
+        zprl = zprsl + zprl;
+        zprsc = 0;
+        zprsl = 0;
+      } else if (zprc + zprl > 0 && temperature <= TMELT) {
+        zdtdt = (zprc + zprl) / ds * zlcp + zdtdt;
+        zprsc = zprc + zprsc;
+        zprsl = zprl + zprsl;
+        zprc = 0;
+        zprl = 0;
+      }
+    }
+    const float zq = dq + dqdt * DELTSEC2;
+    const float zt = t + dtdt * DELTSEC2;
+    float zqsat = sat_capped(M, zt, dp * T(sigma)[l]);
+    const float zcor = 1.0f / (1.0f - MOIST * zqsat);
+    zqsat = zqsat * zcor;
+    if (zqsat < 0) zqsat = 0;
+    if (zq < zqsat) {
+      const float deficit = zqsat - zq;
+      const float water = ALV / (ACPD * (1.0f + ADV * dq));
+      const float ice = ALS / (ACPD * (1.0f + ADV * dq));
+      if (zprl > 0) {
+        const float e = evaporate(M, &zprl, deficit, water, zqsat, zcor, zt, ds);
+        zdtdt = zdtdt - e * water;
+        zdqdtl = zdqdtl + e;
+      }
+      if (zprc > 0) {
+        const float e = evaporate(M, &zprc, deficit, water, zqsat, zcor, zt, ds);
+        zdtdt = zdtdt - e * water;
+        zdqdtl = zdqdtl + e;
+      }
+      if (zprsl > 0) {
+        const float e = evaporate(M, &zprsl, deficit, ice, zqsat, zcor, zt, ds);
+        zdtdt = zdtdt - e * ice;
+        zdqdts = zdqdts + e;
+      }
+      if (zprsc > 0) {
+        const float e = evaporate(M, &zprsc, deficit, ice, zqsat, zcor, zt, ds);
+        zdtdt = zdtdt - e * ice;
+        zdqdts = zdqdts + e;
+      }
+    }
+    LV(dqdt, l, h) = dqdt + zdqdtl + zdqdts;
+    LV(dtdt, l, h) = dtdt + zdtdt;
   }
-  mklsp<NL>(M, h);
-  mkrain<NL>(M, h);
+  F(dprc)[h] = (zprc + zprsc) * dp / K(ga) / 1000.0f;
+  F(dprl)[h] = (zprl + zprsl) * dp / K(ga) / 1000.0f;
+  F(dprs)[h] = (zprsc + zprsl) * dp / K(ga) / 1000.0f;
 }
 
 template <int NL>
