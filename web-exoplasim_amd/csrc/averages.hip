@@ -182,7 +182,7 @@ void enqueue_averages(Engine& e, const std::vector<MemberDev*>& members, const s
     int date[5];
     member_date(m->host, steps[i], date);
     const int n = a.n++;
-    a.snap[n] = m->d_arena + m->host.layout.offset[SLOT_snapshot];
+    a.snap[n] = m->d_arena + m->host.dev.offset[SLOT_snapshot];
     a.sum[n] = v.d_sum;
     a.count[n] = v.floats;
     a.first[n] = blocks;

@@ -200,7 +200,7 @@ std::unique_ptr<MemberDev> prepare_member(const exo_config& cfg, int precision, 
 size_t member_bytes(const Engine& e, const MemberDev& md) {
   const Member& m = md.host;
   const size_t shared = m.tlayout.offset[TAB_zgw];
-  size_t bytes = m.layout.total + (m.tlayout.total - shared) + m.constants.size() * CON_COUNT +
+  size_t bytes = m.dev.total + (m.tlayout.total - shared) + m.constants.size() * CON_COUNT +
                  m.constants.size() * m.tlayout.count[TAB_bm1];
   if (!e.transform_tables.count(m.dims.ntru)) bytes += shared;
   return bytes * sizeof(float);
@@ -226,7 +226,10 @@ int admit_member(Engine& e, std::unique_ptr<MemberDev> md) {
       Member& m = md->host;
       const size_t nphase = m.constants.size();
       const size_t nbm1 = m.tlayout.count[TAB_bm1];
-      std::vector<float> arena(m.initial.begin(), m.initial.end());
+      std::vector<float> arena(m.dev.total, 0.0f);
+      for (int sl = 0; sl < SLOT_COUNT; sl++)
+        std::copy(m.initial.begin() + m.layout.offset[sl], m.initial.begin() + m.layout.offset[sl] + m.layout.floats[sl],
+                  arena.begin() + m.dev.offset[sl]);
       std::vector<float> tab(m.tables.begin(), m.tables.end());
       std::vector<float> consts(nphase * CON_COUNT), bm1(nphase * nbm1);
       for (size_t p = 0; p < nphase; p++) {
@@ -276,7 +279,7 @@ int admit_member(Engine& e, std::unique_ptr<MemberDev> md) {
       d.tab = md->d_tab;
       d.ltab = shared.d;
       for (int t = 0; t < 8; t++) d.lscale[t] = shared.lscale[t];
-      for (int sl = 0; sl < SLOT_COUNT; sl++) d.slot[sl] = (int)m.layout.offset[sl];
+      for (int sl = 0; sl < SLOT_COUNT; sl++) d.slot[sl] = (int)m.dev.offset[sl];
       for (int t = 0; t < TAB_COUNT; t++) d.tabo[t] = (int)m.tlayout.offset[t];
       for (int c = 0; c < CON_COUNT; c++) d.k[c] = consts[c];
       e.h_desc[desc] = d;
@@ -838,6 +841,20 @@ void rows_to_fourier(const float* rows, float* arena, size_t floats, size_t nfc)
     for (size_t r = 0; r < R; r++) arena[k * R + r] = rows[r * nfc + k];
 }
 
+// Whole-arena images between the reference's layout (checkpoints) and the
+// device's, whose slots start on 256-byte boundaries.
+void arena_from_device(const MemberDev* m, const float* device, float* image) {
+  const exo::Layout &lay = m->host.layout, &dev = m->host.dev;
+  for (int s = 0; s < SLOT_COUNT; s++)
+    std::copy(device + dev.offset[s], device + dev.offset[s] + lay.floats[s], image + lay.offset[s]);
+}
+
+void arena_to_device(const MemberDev* m, const float* image, float* device) {
+  const exo::Layout &lay = m->host.layout, &dev = m->host.dev;
+  for (int s = 0; s < SLOT_COUNT; s++)
+    std::copy(image + lay.offset[s], image + lay.offset[s] + lay.floats[s], device + dev.offset[s]);
+}
+
 // Rearranges every Fourier slot of a whole arena image in place; to_rows
 // picks the direction.
 void arena_fourier_rows(const MemberDev* m, float* image, bool to_rows) {
@@ -869,7 +886,7 @@ int exo_read_field(exo_engine* h, int id, const char* name, float* out, size_t f
   hipSetDevice(e.device);
   const bool fourier = m->host.layout.fourier[s];
   std::vector<float> staged(fourier ? floats : 0);
-  hipError_t err = hipMemcpyAsync(fourier ? staged.data() : out, m->d_arena + m->host.layout.offset[s],
+  hipError_t err = hipMemcpyAsync(fourier ? staged.data() : out, m->d_arena + m->host.dev.offset[s],
                                   floats * sizeof(float), hipMemcpyDeviceToHost, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
@@ -891,7 +908,7 @@ int exo_write_field(exo_engine* h, int id, const char* name, const float* values
   const bool fourier = m->host.layout.fourier[s];
   std::vector<float> staged(fourier ? floats : 0);
   if (fourier) rows_to_fourier(values, staged.data(), floats, (size_t)exo_nfc(m->host.dims.ntru));
-  hipError_t err = hipMemcpyAsync(m->d_arena + m->host.layout.offset[s], fourier ? staged.data() : values,
+  hipError_t err = hipMemcpyAsync(m->d_arena + m->host.dev.offset[s], fourier ? staged.data() : values,
                                   floats * sizeof(float), hipMemcpyHostToDevice, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
@@ -1037,7 +1054,7 @@ int exo_autotune(exo_engine* h, int trial_steps, double* tuned_ms) {
     HIP_OR_THROW(hipSetDevice(e.device));
     HIP_OR_THROW(hipStreamSynchronize(e.stream));
     for (Saved& s : saved) {
-      const size_t bytes = s.m->host.layout.total * sizeof(float);
+      const size_t bytes = s.m->host.dev.total * sizeof(float);
       HIP_OR_THROW(hipMalloc(&s.arena, bytes));
       HIP_OR_THROW(hipMemcpyAsync(s.arena, s.m->d_arena, bytes, hipMemcpyDeviceToDevice, e.stream));
     }
@@ -1097,7 +1114,7 @@ int exo_autotune(exo_engine* h, int trial_steps, double* tuned_ms) {
   hipStreamSynchronize(e.stream);
   for (Saved& s : saved) {
     if (s.arena) {
-      const hipError_t err = hipMemcpy(s.m->d_arena, s.arena, s.m->host.layout.total * sizeof(float),
+      const hipError_t err = hipMemcpy(s.m->d_arena, s.arena, s.m->host.dev.total * sizeof(float),
                                        hipMemcpyDeviceToDevice);
       if (err != hipSuccess && rc == EXO_OK) rc = fail(EXO_E_DEVICE, hipGetErrorString(err));
       hipFree(s.arena);
@@ -1241,10 +1258,11 @@ int exo_checkpoint_save(exo_engine* h, int id, void* out, size_t bytes) {
   const size_t start = (ml + 4 + json.size() + 3) / 4 * 4;
   Engine& e = h->e;
   hipSetDevice(e.device);
-  std::vector<float> image(m->host.layout.total);
-  hipError_t err = hipMemcpyAsync(image.data(), m->d_arena, image.size() * 4, hipMemcpyDeviceToHost, e.stream);
+  std::vector<float> device(m->host.dev.total), image(m->host.layout.total);
+  hipError_t err = hipMemcpyAsync(device.data(), m->d_arena, device.size() * 4, hipMemcpyDeviceToHost, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
+  arena_from_device(m, device.data(), image.data());
   arena_fourier_rows(m, image.data(), true);
   std::memcpy(p + start, image.data(), image.size() * 4);
   return EXO_OK;
@@ -1338,10 +1356,11 @@ int exo_checkpoint_restore(exo_engine* h, int id, const void* bytes, size_t size
   Engine& e = h->e;
   if (e.device < 0) return fail(EXO_E_STATE, "host-only engine has no device state");
   hipSetDevice(e.device);
-  std::vector<float> image(m->host.layout.total);
+  std::vector<float> image(m->host.layout.total), device(m->host.dev.total, 0.0f);
   std::memcpy(image.data(), p + start, image.size() * 4);
   arena_fourier_rows(m, image.data(), false);
-  hipError_t err = hipMemcpyAsync(m->d_arena, image.data(), image.size() * 4, hipMemcpyHostToDevice, e.stream);
+  arena_to_device(m, image.data(), device.data());
+  hipError_t err = hipMemcpyAsync(m->d_arena, device.data(), device.size() * 4, hipMemcpyHostToDevice, e.stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
   if (err != hipSuccess) return fail(EXO_E_DEVICE, hipGetErrorString(err));
   // A restart skips the start-up steps (gcm.worker.ts:110-130).

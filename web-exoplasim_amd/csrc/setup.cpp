@@ -89,6 +89,18 @@ Layout make_layout(const ExoDims& d, int nsnap) {
   return lay;
 }
 
+Layout device_layout(const Layout& ref) {
+  Layout d = ref;
+  size_t at = 0;
+  for (int s = 0; s < SLOT_COUNT; s++) {
+    at = (at + 63) & ~(size_t)63;
+    d.offset[s] = at;
+    at += ref.floats[s];
+  }
+  d.total = (at + 63) & ~(size_t)63;
+  return d;
+}
+
 TableLayout make_table_layout(const ExoDims& d) {
   const int T = d.ntru, NLAT = d.nlat, NLON = d.nlon, L = d.nlev;
   const int NCSP = exo_ncsp(T), NRSP = exo_nrsp(T), NFC = exo_nfc(T);
@@ -1075,6 +1087,7 @@ void build_member(const exo_config& cfg, Member& m, const exo_surface_field* fie
   m.dims = {T, NLAT, NLON, L};
   m.nsnap = exo_snapshot_count(L);
   m.layout = make_layout(m.dims, m.nsnap);
+  m.dev = device_layout(m.layout);
   m.tlayout = make_table_layout(m.dims);
   m.tables.assign(m.tlayout.total, 0.0);
   auto tab = [&](ExoTable t) { return &m.tables[m.tlayout.offset[t]]; };

@@ -50,7 +50,8 @@ struct Member {
   exo_config cfg{};
   ExoDims dims{};
   int nsnap = 0;
-  Layout layout;
+  Layout layout;      // the reference's arena layout (checkpoints, field offsets)
+  Layout dev;         // the device arena's: every slot on a 256-byte boundary
   TableLayout tlayout;
   std::vector<double> tables;  // FP64 tables, bm1 of the first phase
   // Phases in run order: start-up kits = n..1, then the main loop (kits 0).
@@ -80,6 +81,11 @@ void member_date(const Member& m, int64_t step, int out[5]);
 // latitudesDeg, grid/index.ts:32-82).
 std::vector<double> gaussian_degrees(int nlat);
 Layout make_layout(const ExoDims& d, int nsnap);
+// The device arena's layout: the slots of `ref` in the same order, each
+// starting on a 256-byte boundary, so that a wave's 64 consecutive floats
+// of a grid plane are two whole 128-byte cache lines (the reference's
+// offsets leave most slots a few floats off a line boundary).
+Layout device_layout(const Layout& ref);
 TableLayout make_table_layout(const ExoDims& d);
 // STEPPER.values for model step `step` into 16 doubles: radiation_step[5],
 // surface_step[7], carbon_step[1], snapshot_step[3]. carbon_step is 1 on
