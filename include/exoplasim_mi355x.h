@@ -208,6 +208,32 @@ int exo_atmosphere_resolve(const double *partial_pressures_bar, int pressure_and
                            double pressure_bar, double gas_constant, double *out_pressure_bar,
                            double *out_gas_constant, double *out_co2_ppm);
 
+/* GCM_NAMELIST.read (namelist/index.ts:196-233): reads `nkeys` keys from
+ * the text of one namelist file. `values` holds, key after key, each key's
+ * `lengths[k]` default numbers on entry and the read values on return: a
+ * "KEY = v1,v2,..." line (key case ignored, the file's last line for a key
+ * winning) writes its comma-separated numbers over the defaults from the
+ * front, each as JavaScript's Number() reads it with a 'd' exponent taken
+ * for 'e' (an empty part reads 0, a malformed one NaN); numbers beyond a
+ * key's length are dropped. A NULL text reads as an empty file. */
+int exo_namelist_read(const char *text, const char *const *keys, const int *lengths, int nkeys,
+                      double *values);
+
+/* The set-up's reading of a run's namelist files (setup/pumamod/
+ * index.ts:103-232, setup/modules/index.ts:41-305, setup/index.ts:
+ * 107-112), as the inverse of GCM_NAMELIST.edits (namelist/index.ts:
+ * 40-182): `count` files by name ("planet_namelist", "plasim_namelist",
+ * "radmod_namelist", ...; other names are ignored, an absent file reads
+ * as empty) and text. Every exo_config field that edits writes is set from
+ * the files, a key absent from them taking the module's default; the
+ * fields no namelist carries (the synthetic planet's, initial_state,
+ * clock_ms) keep the values *cfg holds on entry. A key the set-up reads
+ * that no field carries must hold the value this engine runs with (the
+ * base namelists' or the module's); any other value fails with EXO_E_ARG
+ * and a message naming the file and key. */
+int exo_config_from_namelists(int truncation, int levels, const char *const *names,
+                              const char *const *texts, int count, exo_config *cfg);
+
 /* Removes a member and frees its device memory. */
 int exo_remove_member(exo_engine *e, int member);
 

@@ -8,4 +8,5 @@ engine interface (runner.py).
 from .engine import (EXO_PRECISION_SPLIT, EXO_PRECISION_FP32, Engine,  # noqa: F401
                      ExoConfig, ExoError, constant_names, default_config,
                      balance_assess, boundary_files, boundary_read_sra, boundary_sra, library, native_patch,
-                     native_record, slot_names, snapshot_field_names, atmosphere_resolve)
+                     native_record, slot_names, snapshot_field_names, atmosphere_resolve,
+                     namelist_read, config_from_namelists)

@@ -160,6 +160,40 @@ int latitudes_for(int truncation) {
   throw std::runtime_error("T" + std::to_string(truncation) + " is not built");
 }
 
+}  // namespace
+
+PlasimArrays plasim_arrays(int truncation, int levels) {
+  // plasimmod's level-array defaults and readnl's resolution defaults
+  // before the namelist (setup/pumamod/index.ts:43-91, :113-127)
+  const double day = 86400, frcmod = 1;
+  const int L = levels;
+  PlasimArrays a;
+  a.nhdiff = 15;
+  a.ndel.assign(L, 2);
+  a.tdissd.assign(L, 0.2);
+  a.tdissz.assign(L, 1.1);
+  a.tdisst.assign(L, 5.6);
+  a.tdissq.assign(L, 0.1);
+  a.restim.assign(L, 0);
+  a.t0.assign(L, 250);
+  a.tfrc.assign(L, 0);
+  if (L == 10)
+    for (int l = 0; l < L; l++) a.tfrc[l] = (l == 0 ? 20 : l == 1 ? 100 : 0) * day * frcmod;
+  if (truncation == 42) {
+    a.nhdiff = 16;
+    for (int l = 0; l < L; l++) {
+      a.ndel[l] = 4;
+      a.tdissq[l] = 0.1 * day;
+      a.tdisst[l] = 0.76 * day;
+      a.tdissz[l] = 0.3 * day;
+      a.tdissd[l] = 0.06 * day;
+    }
+  }
+  return a;
+}
+
+namespace {
+
 // GCM_NAMELIST.edits (namelist/index.ts:40-182) applied over the T21 base
 // namelists, then PUMAMOD.read (setup/pumamod/index.ts:103-232).
 Pumamod pumamod_read(const exo_config& c) {
@@ -194,26 +228,17 @@ Pumamod pumamod_read(const exo_config& c) {
   }
   // plasim_namelist: readnl defaults, resolution defaults, base + edits.
   const double day = 86400;
-  double frcmod = 1;
-  p.ndel.assign(L, 2);
-  p.tdissd.assign(L, 0.2);
-  p.tdissz.assign(L, 1.1);
-  p.tdisst.assign(L, 5.6);
-  p.tdissq.assign(L, 0.1);
-  p.restim.assign(L, 0);
-  p.t0.assign(L, 250);
-  p.tfrc.assign(L, 0);
-  if (L == 10)
-    for (int l = 0; l < L; l++) p.tfrc[l] = (l == 0 ? 20 : l == 1 ? 100 : 0) * day * frcmod;
-  if (T == 42) {
-    p.nhdiff = 16;
-    for (int l = 0; l < L; l++) {
-      p.ndel[l] = 4;
-      p.tdissq[l] = 0.1 * day;
-      p.tdisst[l] = 0.76 * day;
-      p.tdissz[l] = 0.3 * day;
-      p.tdissd[l] = 0.06 * day;
-    }
+  {
+    PlasimArrays a = plasim_arrays(T, L);
+    p.nhdiff = a.nhdiff;
+    p.ndel = a.ndel;
+    p.tdissd = a.tdissd;
+    p.tdissz = a.tdissz;
+    p.tdisst = a.tdisst;
+    p.tdissq = a.tdissq;
+    p.restim = a.restim;
+    p.t0 = a.t0;
+    p.tfrc = a.tfrc;
   }
   p.kick = 1;
   p.nqspec = 1;

@@ -86,6 +86,14 @@ Layout make_layout(const ExoDims& d, int nsnap);
 // of a grid plane are two whole 128-byte cache lines (the reference's
 // offsets leave most slots a few floats off a line boundary).
 Layout device_layout(const Layout& ref);
+
+// plasim_namelist's level arrays and nhdiff at a resolution, before the
+// namelist is read (setup/pumamod/index.ts:43-91, :113-127)
+struct PlasimArrays {
+  int nhdiff;
+  std::vector<double> ndel, tdissd, tdissz, tdisst, tdissq, restim, t0, tfrc;
+};
+PlasimArrays plasim_arrays(int truncation, int levels);
 TableLayout make_table_layout(const ExoDims& d);
 // STEPPER.values for model step `step` into 16 doubles: radiation_step[5],
 // surface_step[7], carbon_step[1], snapshot_step[3]. carbon_step is 1 on

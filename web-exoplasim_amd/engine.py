@@ -108,6 +108,44 @@ def atmosphere_resolve(partial_pressures=None, mode="partialPressures", pressure
     return tuple(o.value for o in out)
 
 
+def namelist_read(text, defaults):
+    """GCM_NAMELIST.read: `defaults` ({key: number or list}) with the
+    values one namelist file's text sets, arrays filled from the front."""
+    lib = library()
+    keys = list(defaults)
+    lengths = [len(v) if isinstance(v, (list, tuple)) else 1 for v in defaults.values()]
+    flat = []
+    for v in defaults.values():
+        flat += [float(x) for x in v] if isinstance(v, (list, tuple)) else [float(v)]
+    karr = (ctypes.c_char_p * len(keys))(*[k.encode() for k in keys])
+    larr = (ctypes.c_int * len(keys))(*lengths)
+    varr = (ctypes.c_double * max(len(flat), 1))(*flat)
+    _check(lib.exo_namelist_read(None if text is None else text.encode(), karr, larr, len(keys), varr))
+    out, at = {}, 0
+    for k, n, v in zip(keys, lengths, defaults.values()):
+        out[k] = list(varr[at:at + n]) if isinstance(v, (list, tuple)) else varr[at]
+        at += n
+    return out
+
+
+def config_from_namelists(truncation, levels, files, base=None):
+    """exo_config_from_namelists: the ExoConfig a run's namelist files
+    ({name: text}) carry; fields no namelist holds come from `base`
+    (default_config() if None)."""
+    lib = library()
+    cfg = ExoConfig()
+    if base is None:
+        lib.exo_config_default(ctypes.byref(cfg))
+    else:
+        ctypes.memmove(ctypes.byref(cfg), ctypes.byref(base), ctypes.sizeof(ExoConfig))
+    names = [n.encode() for n in files]
+    texts = [None if t is None else t.encode() for t in files.values()]
+    narr = (ctypes.c_char_p * max(len(names), 1))(*names)
+    tarr = (ctypes.c_char_p * max(len(texts), 1))(*texts)
+    _check(lib.exo_config_from_namelists(truncation, levels, narr, tarr, len(names), ctypes.byref(cfg)))
+    return cfg
+
+
 class ExoRequest(ctypes.Structure):
     """exo_request: one run of the serving loop."""
 
@@ -193,6 +231,9 @@ def library():
         "exo_add_member_surface": (I, [P, ctypes.POINTER(ExoConfig), I, ctypes.POINTER(ExoSurfaceField), I,
                                        ctypes.POINTER(I)]),
         "exo_atmosphere_resolve": (I, [D, I, ctypes.c_double, ctypes.c_double, D, D, D]),
+        "exo_namelist_read": (I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(I), I, D]),
+        "exo_config_from_namelists": (I, [I, I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                          I, ctypes.POINTER(ExoConfig)]),
         "exo_snapshot_field_name": (ctypes.c_char_p, [I, I]),
         "exo_read_field": (I, [P, I, ctypes.c_char_p, F, S]),
         "exo_write_field": (I, [P, I, ctypes.c_char_p, F, S]),

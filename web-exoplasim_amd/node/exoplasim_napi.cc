@@ -138,9 +138,7 @@ const Field kConfigFields[] = {
 };
 #undef EXO_FIELD
 
-napi_value config_default(napi_env env, napi_callback_info) {
-  exo_config c;
-  exo_config_default(&c);
+napi_value config_object(napi_env env, const exo_config& c) {
   napi_value obj;
   NAPI_OK(napi_create_object(env, &obj));
   for (const Field& f : kConfigFields) {
@@ -160,6 +158,62 @@ napi_value config_default(napi_env env, napi_callback_info) {
     NAPI_OK(napi_set_named_property(env, obj, f.name, number(env, x)));
   }
   return obj;
+}
+
+napi_value config_default(napi_env env, napi_callback_info) {
+  exo_config c;
+  exo_config_default(&c);
+  return config_object(env, c);
+}
+
+std::string string_of(napi_env env, napi_value v) {
+  size_t len = 0;
+  if (napi_get_value_string_utf8(env, v, nullptr, 0, &len) != napi_ok) return std::string();
+  std::string s(len, '\0');
+  napi_get_value_string_utf8(env, v, &s[0], len + 1, &len);
+  return s;
+}
+
+bool config_from(napi_env env, napi_value obj, exo_config& c);
+
+// configFromNamelists(truncation, levels, {name: text}, base?) -> config:
+// exo_config_from_namelists; the fields no namelist carries come from
+// `base` (an exo_config object over the defaults) or the defaults.
+napi_value config_from_namelists(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  exo_config c;
+  napi_valuetype base_type = napi_undefined;
+  if (a.n > 3) napi_typeof(env, a.v[3], &base_type);
+  if (base_type == napi_object) {
+    if (!config_from(env, a.v[3], c)) return nullptr;
+  } else {
+    exo_config_default(&c);
+  }
+  napi_value keys;
+  uint32_t n = 0;
+  NAPI_OK(napi_get_property_names(env, a.v[2], &keys));
+  NAPI_OK(napi_get_array_length(env, keys, &n));
+  std::vector<std::string> names(n), texts(n);
+  std::vector<const char*> np(n), tp(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value k, v;
+    NAPI_OK(napi_get_element(env, keys, i, &k));
+    NAPI_OK(napi_get_property(env, a.v[2], k, &v));
+    napi_valuetype t;
+    NAPI_OK(napi_typeof(env, v, &t));
+    names[i] = string_of(env, k);
+    if (t != napi_string) {
+      napi_throw_type_error(env, nullptr, (names[i] + " must be the text of a namelist file").c_str());
+      return nullptr;
+    }
+    texts[i] = string_of(env, v);
+    np[i] = names[i].c_str();
+    tp[i] = texts[i].c_str();
+  }
+  EXO_OK_OR_THROW(exo_config_from_namelists(int_of(env, a.v[0]), int_of(env, a.v[1]), np.data(), tp.data(),
+                                            (int)n, &c));
+  return config_object(env, c);
 }
 
 // Defaults overlaid with the object's numeric properties; unknown keys throw.
@@ -771,7 +825,7 @@ napi_value boundary_sra(napi_env env, napi_callback_info info) {
 
 napi_value init(napi_env env, napi_value exports) {
   const std::pair<const char*, napi_callback> fns[] = {
-      {"configDefault", config_default}, {"create", create},
+      {"configDefault", config_default}, {"configFromNamelists", config_from_namelists}, {"create", create},
       {"destroy", destroy},              {"addMember", add_member},
       {"removeMember", remove_member},   {"step", step},
       {"sync", sync},                    {"setPublishEvery", set_publish_every},

@@ -21,6 +21,7 @@
 const fs = require("fs")
 const path = require("path")
 const addon = require(path.join(__dirname, "exoplasim_napi.node"))
+const NAMELIST = require(path.join(__dirname, "namelist.js"))
 
 // GcmConfig's fields (src/model/climate/gcm/types.ts:28-62).
 const CONFIG_KEYS = [
@@ -143,6 +144,33 @@ function engineConfig(config, planet) {
 	const merged = Object.assign(out, extra)
 	for (const key of Object.keys(merged)) if (merged[key] === undefined) delete merged[key]
 	return merged
+}
+
+// The base namelist files of a request (runner/webgpu/index.ts:44-58):
+// request.namelists, the namelist files of request.inputs, or public/gcm's
+// t<T>/namelists.json under request.publicDir; null when there are none.
+function baseNamelists(request) {
+	if (request.namelists) return request.namelists
+	const inputs = request.inputs && request.inputs.files
+	if (inputs && Object.keys(inputs).some((name) => name.endsWith("_namelist"))) return inputs
+	if (request.publicDir) {
+		const earth = request.config.earthSurfaceClimatology && request.resolution && request.resolution.gpu
+		const file = path.join(request.publicDir, earth ? request.resolution.gpu.inputs : `t${request.config.truncation}/namelists.json`)
+		if (fs.existsSync(file)) {
+			const files = JSON.parse(fs.readFileSync(file, "utf8")).files
+			return Object.fromEntries(Object.entries(files).filter(([name]) => name.endsWith("_namelist")))
+		}
+	}
+	return null
+}
+
+// A full GcmConfig through its namelist files, as the reference's set-up
+// receives it: GCM_NAMELIST.edits applied over the base files, read back
+// by exo_config_from_namelists. The fields no namelist carries (the
+// synthetic planet's, the kind of fresh start) come from engineConfig.
+function namelistConfig(config, planet, base) {
+	const direct = engineConfig(config, planet)
+	return addon.configFromNamelists(config.truncation, config.layers, NAMELIST.files(base, config), direct)
 }
 
 // Every resolution this engine runs (the shape of GCM_RUNNER.resolutions,
@@ -324,7 +352,9 @@ function start(request) {
 			status("loading")
 			state.engine = addon.create(request.device === undefined ? 0 : request.device, 1)
 			const precision = request.precision === "split" ? addon.PRECISION_SPLIT : addon.PRECISION_FP32
-			const config = engineConfig(request.config, request.planet)
+			const base = baseNamelists(request)
+			const toEngine = (c) => (base ? namelistConfig(c, request.planet, base) : engineConfig(c, request.planet))
+			const config = toEngine(request.config)
 			const fields = surfaceFields(state.engine, request)
 			// without a surface-temperature climatology or a synthetic SST the
 			// ocean starts as the reference's does (setup/surface/index.ts:
@@ -342,7 +372,7 @@ function start(request) {
 					state.engine,
 					state.id,
 					request.restart,
-					request.previousConfig ? engineConfig(request.previousConfig, request.planet) : null,
+					request.previousConfig ? toEngine(request.previousConfig) : null,
 				)
 			addon.setAveraging(state.engine, state.id, true)
 			if (request.averager) {
@@ -396,6 +426,8 @@ module.exports = {
 	start,
 	resolutions,
 	engineConfig,
+	namelistConfig,
+	baseNamelists,
 	addon,
 	readAverager,
 	loadAverager,
