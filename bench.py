@@ -45,7 +45,7 @@ def probe_steps(steps):
     return min(steps, 96)
 
 
-def run_gpu(exo, cfgs, steps, warmup, probes=(), precision=0, graphs=True, tune=False):
+def run_gpu(exo, cfgs, steps, warmup, probes=(), precision=0, graphs=True, tune=False, lanes=0):
     """Steps every config together; returns (wall s, device ms, probed).
     The timed steps replay the captured step graph (graphs=True) or launch
     every kernel (False); each kernel of `probes` then has its single
@@ -54,6 +54,8 @@ def run_gpu(exo, cfgs, steps, warmup, probes=(), precision=0, graphs=True, tune=
     eng = exo.Engine(0, len(cfgs))
     try:
         eng.set_step_graphs(graphs)
+        if lanes:
+            eng.set_step_lanes(lanes)
         ids = [eng.add_member(c, precision=precision) for c in cfgs]
         eng.step(warmup)
         eng.sync()
@@ -65,7 +67,7 @@ def run_gpu(exo, cfgs, steps, warmup, probes=(), precision=0, graphs=True, tune=
             ms = eng.autotune(trial_steps=8)
             classes = sorted({(c.truncation, c.levels) for c in cfgs})
             tuned = {"groups": {f"T{t} L{l}": eng.transform_group(t, l) for t, l in classes},
-                     "step_graphs": eng.step_graphs(),
+                     "step_graphs": eng.step_graphs(), "lanes": eng.step_lanes(),
                      "best_trial_ms_per_step": ms, "seconds": time.perf_counter() - t_tune}
         t0 = time.perf_counter()
         eng.step(steps)
@@ -225,6 +227,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default transform group size (4 members) instead of tuning it")
+    ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="streams a replayed step's members run on (0: the engine's default, or the "
+                         "autotuner's pick)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
     ap.add_argument("--cpu-cores", type=int, default=0, help="host threads of the CPU baseline (default: the "
                     "box's CPU share, OMP_NUM_THREADS)")
@@ -276,7 +281,7 @@ def main():
     secondary = [form_name(k) for k in secondary]
     probes = ([probe] if probe else []) + secondary
     wall, dev_ms, probed = run_gpu(exo, cfgs, args.steps, args.warmup, probes, precision,
-                                   tune=not args.no_autotune)
+                                   tune=not args.no_autotune, lanes=args.lanes)
     days = sum(args.steps / workloads.steps_per_day(c.timestep_minutes) for c in cfgs)
     value = days / wall
     line = {

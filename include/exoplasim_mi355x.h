@@ -279,8 +279,10 @@ int exo_transform_group(exo_engine *e, int truncation, int levels,
  * that hold two or more members), then restores every member's state,
  * step counters and phase as they were: the next exo_step continues
  * exactly as without the call. Nothing is published during the trials.
- * It then times the other launch mode (see exo_set_step_graphs) and keeps
- * the faster; a later exo_set_step_graphs overrides that choice.
+ * It then times the other lane count (see exo_set_step_lanes) and the
+ * other launch mode (see exo_set_step_graphs), keeping the faster of
+ * each; a later exo_set_step_lanes or exo_set_step_graphs overrides that
+ * choice.
  * tuned_ms (may be null) receives the best trial's device ms per step. */
 int exo_autotune(exo_engine *e, int trial_steps, double *tuned_ms);
 
@@ -292,6 +294,19 @@ int exo_autotune(exo_engine *e, int trial_steps, double *tuned_ms);
  * steps (stepper/index.ts:175-276); this is that batching's launch side. */
 int exo_set_step_graphs(exo_engine *e, int enable);
 int exo_step_graphs(exo_engine *e, int *enabled);
+
+/* Lanes of a replayed step, 1 to 4: n > 1 (2 by default) splits the
+ * active members into n sets of about equal work (grid points x levels)
+ * whose launches run on n streams, forked and joined inside the step
+ * graph, so that one set's latency-bound launches overlap the others'; 1
+ * runs them all on one stream. Results are bit-identical for every n
+ * (each member's launches keep their order and arithmetic); exo_autotune
+ * keeps the fastest count.
+ * Steps launched one by one always use one stream. No reference
+ * counterpart: the reference's one command buffer per batch has no
+ * concurrent branches. */
+int exo_set_step_lanes(exo_engine *e, int lanes);
+int exo_step_lanes(exo_engine *e, int *lanes);
 
 /* Blocks until queued steps have finished. Stands in for mapAsync. */
 int exo_sync(exo_engine *e);

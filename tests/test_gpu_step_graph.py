@@ -1,7 +1,8 @@
 """GPU test of the captured step graph: a batch stepped by graph replays
-holds the same state, bit for bit, as the same batch stepped by single
-launches, across start-up phases, snapshot publishing, averaging, eager
-work between calls and a change of members (which re-captures)."""
+(on one stream, or split over two concurrent streams) holds the same
+state, bit for bit, as the same batch stepped by single launches, across
+start-up phases, snapshot publishing, averaging, eager work between calls
+and a change of members (which re-captures)."""
 import numpy as np
 import pytest
 
@@ -14,13 +15,19 @@ def _planets(exo):
     return workloads.ensemble(exo, seed=11, mix=((21, 5, 2), (42, 10, 1)))
 
 
-def test_graph_replays_match_single_launches(exo):
+@pytest.mark.parametrize("lanes", [1, 2, 3])
+def test_graph_replays_match_single_launches(exo, lanes):
+    """With lanes = 2 the replayed step runs the T42 member on one stream
+    and the two T21 members on another, concurrently; with 3, each member
+    on a stream of its own."""
     cfgs = _planets(exo)
     extra = exo.default_config()
     engines = [exo.Engine(0, 4), exo.Engine(0, 4)]
     try:
         engines[0].set_step_graphs(False)
         engines[1].set_step_graphs(True)
+        engines[1].set_step_lanes(lanes)
+        assert engines[1].step_lanes() == lanes
         ids = []
         for e in engines:
             e.set_publish_every(2)

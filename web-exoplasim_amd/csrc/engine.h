@@ -36,7 +36,8 @@ extern "C" __global__ void k_spectrald_diffusion(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_spectrald_update(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_physics_in(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_fixer_column(const DevMember*, LaunchPlan);
-extern "C" __global__ void k_fixer_reduce(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_fixer_rows(const DevMember*, LaunchPlan);
+extern "C" __global__ void k_fixer_sum(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_surface_fluxes(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_shortwave(const DevMember*, LaunchPlan);
 extern "C" __global__ void k_longwave(const DevMember*, LaunchPlan);
@@ -170,6 +171,16 @@ struct Engine {
   // advances.
   bool graphs = true;
   uint64_t epoch = 0;
+  // Lanes of a captured step: with n > 1, the members split into n sets
+  // of about equal work whose steps run on n streams, forked and joined
+  // inside the graph, so one set's latency-bound launches overlap the
+  // others'. Each member's launches keep their order and arithmetic, so
+  // the results are the same for every n. exo_autotune picks by timing.
+  static constexpr int kMaxLanes = 4;
+  int lanes = 2;
+  hipStream_t lane_stream[kMaxLanes - 1] = {};
+  hipEvent_t lane_fork = nullptr;
+  hipEvent_t lane_join[kMaxLanes - 1] = {};
   // Members per group of the grouped transforms, per (truncation, levels);
   // kLegGroup where unset. exo_autotune picks them by timing; any value in
   // 1..kLegGroup gives bit-identical members, since every member's rows

@@ -182,6 +182,55 @@ void enqueue_step(Engine& e, const std::vector<int>& active) {
   s.coeff_levels(k_spectrald_update);
 }
 
+// The step of `active` in the engine's lanes (Engine::lanes): the members
+// dealt largest first to the lane with the least work so far (grid points
+// x levels), each lane keeping the members' order; lane 0 on the engine's
+// stream, lane k on lane stream k - 1, forked from and joined back to the
+// engine's stream.
+void enqueue_step_lanes(Engine& e, const std::vector<int>& active) {
+  const int n = std::min<int>(e.lanes, (int)active.size());
+  if (n < 2) {
+    enqueue_step(e, active);
+    return;
+  }
+  std::vector<int> order(active.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  const auto cost = [&](int i) {
+    const DevMember& d = e.h_desc[active[i]];
+    return (double)d.nhor * d.nlev;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  std::vector<int> lane_of(active.size(), 0);
+  double load[Engine::kMaxLanes] = {};
+  for (int i : order) {
+    int k = 0;
+    for (int j = 1; j < n; j++)
+      if (load[j] < load[k]) k = j;
+    lane_of[i] = k;
+    load[k] += cost(i);
+  }
+  std::vector<int> lane[Engine::kMaxLanes];
+  for (size_t i = 0; i < active.size(); i++) lane[lane_of[i]].push_back(active[i]);
+  HIP_OR_THROW(hipEventRecord(e.lane_fork, e.stream));
+  for (int k = 1; k < n; k++) HIP_OR_THROW(hipStreamWaitEvent(e.lane_stream[k - 1], e.lane_fork, 0));
+  enqueue_step(e, lane[0]);
+  hipStream_t main = e.stream;
+  try {
+    for (int k = 1; k < n; k++) {
+      e.stream = e.lane_stream[k - 1];
+      enqueue_step(e, lane[k]);
+    }
+  } catch (...) {
+    e.stream = main;
+    throw;
+  }
+  e.stream = main;
+  for (int k = 1; k < n; k++) {
+    HIP_OR_THROW(hipEventRecord(e.lane_join[k - 1], e.lane_stream[k - 1]));
+    HIP_OR_THROW(hipStreamWaitEvent(e.stream, e.lane_join[k - 1], 0));
+  }
+}
+
 }  // namespace exo
 
 namespace exo {
@@ -368,6 +417,11 @@ int exo_create(int device, int max_members, exo_engine** out) {
     e.device = device;
     HIP_OR_THROW(hipSetDevice(device));
     HIP_OR_THROW(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    HIP_OR_THROW(hipEventCreateWithFlags(&e.lane_fork, hipEventDisableTiming));
+    for (int k = 0; k < Engine::kMaxLanes - 1; k++) {
+      HIP_OR_THROW(hipStreamCreateWithFlags(&e.lane_stream[k], hipStreamNonBlocking));
+      HIP_OR_THROW(hipEventCreateWithFlags(&e.lane_join[k], hipEventDisableTiming));
+    }
     e.max_members = max_members;
     HIP_OR_THROW(hipMalloc(&e.d_desc, sizeof(DevMember) * max_members));
     e.h_desc.assign(max_members, DevMember{});
@@ -419,6 +473,11 @@ void exo_destroy(exo_engine* h) {
   if (e.t1) hipEventDestroy(e.t1);
   for (hipEvent_t ev : e.probe.begin) hipEventDestroy(ev);
   for (hipEvent_t ev : e.probe.end) hipEventDestroy(ev);
+  if (e.lane_fork) hipEventDestroy(e.lane_fork);
+  for (int k = 0; k < Engine::kMaxLanes - 1; k++) {
+    if (e.lane_join[k]) hipEventDestroy(e.lane_join[k]);
+    if (e.lane_stream[k]) hipStreamDestroy(e.lane_stream[k]);
+  }
   if (e.stream) hipStreamDestroy(e.stream);
   delete h;
 }
@@ -611,7 +670,7 @@ static int step_members(exo_engine* h, int steps, bool fixed) {
           hipGraph_t g = nullptr;
           try {
             exo::Stepper(e, desc).load_step_at(e.d_params, e.d_step_index, (int)per_step);
-            exo::enqueue_step(e, desc);
+            exo::enqueue_step_lanes(e, desc);
             if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
             hipLaunchKernelGGL(k_step_index, dim3(1), dim3(64), 0, e.stream, e.d_step_index, -1);
           } catch (...) {
@@ -681,7 +740,7 @@ const std::pair<const char*, const void*> kProbeKernels[] = {
     EXO_PROBE(k_leg_inv_split) EXO_PROBE(k_leg_fwd_split)
     EXO_PROBE(k_calcgp) EXO_PROBE(k_spectrala) EXO_PROBE(k_spectrald_diffusion) EXO_PROBE(k_spectrald_update)
     EXO_PROBE(k_mkdheat) EXO_PROBE(k_mean_dekin) EXO_PROBE(k_physics_in) EXO_PROBE(k_fixer_column)
-    EXO_PROBE(k_fixer_reduce) EXO_PROBE(k_surface_fluxes) EXO_PROBE(k_shortwave) EXO_PROBE(k_longwave)
+    EXO_PROBE(k_fixer_rows) EXO_PROBE(k_fixer_sum) EXO_PROBE(k_surface_fluxes) EXO_PROBE(k_shortwave) EXO_PROBE(k_longwave)
     EXO_PROBE(k_convection) EXO_PROBE(k_dca_limit) EXO_PROBE(k_rain) EXO_PROBE(k_clouds) EXO_PROBE(k_soil)
     EXO_PROBE(k_mkradv) EXO_PROBE(k_physics_c) EXO_PROBE(k_ice_sums) EXO_PROBE(k_physics_d) EXO_PROBE(k_pack)
 #undef EXO_PROBE
@@ -1097,6 +1156,23 @@ int exo_autotune(exo_engine* h, int trial_steps, double* tuned_ms) {
       e.group_cap[c.first] = best;
       e.epoch++;
     }
+    // Lanes of the captured step (bit-identical): every other count, the
+    // fastest kept.
+    if (e.graphs) {
+      int best = e.lanes;
+      for (int lanes = 1; lanes <= Engine::kMaxLanes; lanes++) {
+        if (lanes == best) continue;
+        e.lanes = lanes;
+        e.epoch++;
+        const double ms = trial();
+        if (ms < best_all) {
+          best_all = ms;
+          best = lanes;
+        }
+      }
+      e.lanes = best;
+      e.epoch++;
+    }
     // Launch mode: graph replays or single launches (bit-identical), the
     // faster of the two at the tuned sizes.
     const bool graphs = e.graphs;
@@ -1137,6 +1213,23 @@ int exo_step_graphs(exo_engine* h, int* enabled) {
 int exo_set_step_graphs(exo_engine* h, int enable) {
   if (!h) return fail(EXO_E_ARG, "null engine");
   h->e.graphs = enable != 0;
+  return EXO_OK;
+}
+
+int exo_step_lanes(exo_engine* h, int* lanes) {
+  if (!h || !lanes) return fail(EXO_E_ARG, "null argument");
+  *lanes = h->e.lanes;
+  return EXO_OK;
+}
+
+int exo_set_step_lanes(exo_engine* h, int lanes) {
+  if (!h) return fail(EXO_E_ARG, "null engine");
+  if (lanes < 1 || lanes > Engine::kMaxLanes)
+    return fail(EXO_E_ARG, "lanes must be 1 to " + std::to_string(Engine::kMaxLanes));
+  if (h->e.lanes != lanes) {
+    h->e.lanes = lanes;
+    h->e.epoch++;  // the captured step no longer matches
+  }
   return EXO_OK;
 }
 

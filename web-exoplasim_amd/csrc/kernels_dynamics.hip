@@ -441,6 +441,9 @@ extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __re
 // with the same tables; one workgroup per member and level computes it
 // (every other spectral coefficient of zsdef set to 0, as mean-mode
 // leaves it) and the heating.
+// The largest grid plane (T63: 96 x 192) over 1024 threads.
+constexpr int kDekinPts = 18;
+
 extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int l;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &l)];
@@ -453,40 +456,70 @@ extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember*
   const float* inv = M.ltab + M.tabo[TAB_fourier_inverse];
   const int half = nlat / 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // Fourier mode 0 of each row, weighted by its latitude's qc (mode (0, 0)
-  // is even about the equator: a southern row takes its northern twin's);
-  // waves take rows, lanes points. The planet's dissipated energy is summed
-  // in FP64 and rounded to FP32 once.
-  __shared__ double part[1024];
+  // Fourier mode 0 of each point's row, weighted by its latitude's qc
+  // (mode (0, 0) is even about the equator: a southern row takes its
+  // northern twin's). Each thread takes points h = t + 1024 j and issues
+  // all its loads before it sums; the planet's dissipated energy is summed
+  // in FP64 (thread, then wave butterfly, then the 16 waves in order) and
+  // rounded to FP32 once.
+  float prod[kDekinPts];
+#pragma unroll
+  for (int j = 0; j < kDekinPts; j++) {
+    const int h = threadIdx.x + 1024 * j;
+    float p = 0.0f;
+    if (h < NH) {
+      const int lat = h / nlon, x = h - lat * nlon;
+      const float w = qc[(lat < half ? lat : nlat - 1 - lat) * ncsp];
+      p = w * (zdekin[h] * fwd[x * nfc]);
+    }
+    prod[j] = p;
+  }
   double s = 0.0;
-  for (int lat = wave; lat < nlat; lat += 16) {
-    const float w = qc[(lat < half ? lat : nlat - 1 - lat) * ncsp];
-    for (int x = lane; x < nlon; x += 64) s += (double)(w * (zdekin[lat * nlon + x] * fwd[x * nfc]));
-  }
-  part[threadIdx.x] = s;
+#pragma unroll
+  for (int j = 0; j < kDekinPts; j++) s += (double)prod[j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  __shared__ double part[16];
+  __shared__ float total;
+  if (lane == 0) part[wave] = s;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; w++) t += part[w];
+    total = (float)t;
   }
-  const float S = (float)part[0];
+  __syncthreads();
+  const float S = total;
   // zsdef keeps only Re mode (0, 0)
   float* zsdef = A + M.slot[SLOT_zsdef] + l * nrsp;
   for (int c = threadIdx.x; c < nrsp; c += 1024) zsdef[c] = c == 0 ? S : 0.0f;
-  // back to the grid, then the heating (mkdheat stage 3)
+  // back to the grid, then the heating (mkdheat stage 3); the inputs of
+  // every point first, then the stores
   const float CT = M.k[CON_ct], WW = M.k[CON_ww], GA = M.k[CON_ga], ACPD = M.k[CON_acpd], ADV = M.k[CON_adv];
   const float dsig = M.tab[M.tabo[TAB_dsigma] + l];
   const float* dq = A + M.slot[SLOT_dq] + l * NH;
   const float* dp = A + M.slot[SLOT_dp];
   float* zdtdt = A + M.slot[SLOT_zdtdt] + l * NH;
-  for (int lat = wave; lat < nlat; lat += 16) {
-    const float f = qi[lat * ncsp] * S;
-    for (int x = lane; x < nlon; x += 64) {
-      const int h = lat * nlon + x;
-      const float e = f * inv[x];
-      zdekin[h] = e;
-      const float heat = 1.0f + ADV * dq[h];
-      const float dtdt = -e * 0.5f / ACPD / heat / dp[h] * GA / dsig;
+  float e[kDekinPts], dqv[kDekinPts], dpv[kDekinPts];
+#pragma unroll
+  for (int j = 0; j < kDekinPts; j++) {
+    const int h = threadIdx.x + 1024 * j;
+    e[j] = dqv[j] = 0.0f;
+    dpv[j] = 1.0f;
+    if (h < NH) {
+      const int lat = h / nlon, x = h - lat * nlon;
+      e[j] = (qi[lat * ncsp] * S) * inv[x];
+      dqv[j] = dq[h];
+      dpv[j] = dp[h];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kDekinPts; j++) {
+    const int h = threadIdx.x + 1024 * j;
+    if (h < NH) {
+      zdekin[h] = e[j];
+      const float heat = 1.0f + ADV * dqv[j];
+      const float dtdt = -e[j] * 0.5f / ACPD / heat / dpv[j] * GA / dsig;
       zdtdt[h] = dtdt / CT / WW;
     }
   }

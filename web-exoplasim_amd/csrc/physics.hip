@@ -7,9 +7,10 @@
 // each part run at a higher occupancy than their union:
 //
 //   k_fixer_column   fixer's column pass (physics/fixer/wgsl/column.wgsl)
-//   k_fixer_reduce   latitude rows (one wave per row, 16 waves) then the global sum,
-//                    one workgroup per member (fixer/wgsl/row.wgsl,
-//                    reduction/wgsl/sum.wgsl)
+//   k_fixer_rows     fixer's latitude rows, one wave per row
+//                    (fixer/wgsl/row.wgsl)
+//   k_fixer_sum      then its global sum, one workgroup per member
+//                    (reduction/wgsl/sum.wgsl)
 //   k_surface_fluxes fixer apply, surflx, vdiff
 //   k_shortwave      solang, mko3, swr
 //   k_longwave       lwr, radstep
@@ -2095,13 +2096,26 @@ __device__ __forceinline__ void physics_d(const DevMember& M, int h) {
 // The planet-wide reductions run one workgroup of kReduceThreads per member.
 constexpr int kReduceThreads = 1024;
 
+// grid points per thread of a planet-wide reduction (T63: 18432 points)
+constexpr int kReducePts = 18;
+
 __device__ __forceinline__ void block_sum2(const DevMember& M, const float* a, const float* b, float* out) {
   __shared__ double sa[kReduceThreads], sb[kReduceThreads];
-  double x = 0, y = 0;
-  for (int h = threadIdx.x; h < M.nhor; h += kReduceThreads) {
-    x += (double)a[h];
-    y += (double)b[h];
+  // every load issued before the sums, which run in point order
+  float va[kReducePts], vb[kReducePts];
+#pragma unroll
+  for (int j = 0; j < kReducePts; j++) {
+    const int h = threadIdx.x + kReduceThreads * j;
+    va[j] = h < M.nhor ? a[h] : 0.0f;
+    vb[j] = h < M.nhor ? b[h] : 0.0f;
   }
+  double x = 0, y = 0;
+#pragma unroll
+  for (int j = 0; j < kReducePts; j++)
+    if (threadIdx.x + kReduceThreads * j < M.nhor) {
+      x += (double)va[j];
+      y += (double)vb[j];
+    }
   sa[threadIdx.x] = x;
   sb[threadIdx.x] = y;
   __syncthreads();
@@ -2184,40 +2198,69 @@ extern "C" __global__ __launch_bounds__(64) void k_mkradv(const DevMember* __res
   mkradv(M, h);
 }
 
-// fixer's latitude pass (fixer/wgsl/row.wgsl:4-33), then its global sum:
-// one workgroup per member.
-extern "C" __global__ __launch_bounds__(kReduceThreads) void k_fixer_reduce(const DevMember* __restrict__ mems,
-                                                                             LaunchPlan plan) {
-  const DevMember& M = mems[plan.member[blockIdx.x]];
-  const int NH = M.nhor, NLON = M.nlon, L = M.nlev;
+// fixer's latitude pass (fixer/wgsl/row.wgsl:4-33): one wave per latitude
+// row, four rows a workgroup. Lanes sum strided shares, then a fixed
+// butterfly (the same order on every run), in FP64; a row that rescales
+// loads every level's values before it stores any.
+constexpr int kFixerRowPts = 3;  // longitudes per lane: 192 at T63
+constexpr int kMaxLevels = 10;
+
+extern "C" __global__ __launch_bounds__(256) void k_fixer_rows(const DevMember* __restrict__ mems, LaunchPlan plan) {
+  int tile;
+  const DevMember& M = mems[plan_find(plan, blockIdx.x, &tile)];
+  const int row = tile * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= M.nlat) return;
+  const int NH = M.nhor, NLON = M.nlon, first = row * NLON;
   float* fixq = M.arena + M.slot[SLOT_fixq];
   float* fixneg = M.arena + M.slot[SLOT_fixneg];
   float* fixpos = M.arena + M.slot[SLOT_fixpos];
-  // one wave per latitude row: lanes sum strided shares, then a fixed
-  // butterfly (the same order on every run), in FP64
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int row = wave; row < M.nlat; row += kReduceThreads / 64) {
-    const int first = row * NLON;
-    double zneg = 0, zpos = 0;
-    for (int j = lane; j < NLON; j += 64) {
-      zneg += (double)fixneg[first + j];
-      zpos += (double)fixpos[first + j];
+  float neg[kFixerRowPts], pos[kFixerRowPts];
+#pragma unroll
+  for (int k = 0; k < kFixerRowPts; k++) {
+    const int j = lane + 64 * k;
+    neg[k] = j < NLON ? fixneg[first + j] : 0.0f;
+    pos[k] = j < NLON ? fixpos[first + j] : 0.0f;
+  }
+  double zneg = 0, zpos = 0;
+#pragma unroll
+  for (int k = 0; k < kFixerRowPts; k++)
+    if (lane + 64 * k < NLON) {
+      zneg += (double)neg[k];
+      zpos += (double)pos[k];
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      zneg += __shfl_xor(zneg, o);
-      zpos += __shfl_xor(zpos, o);
-    }
-    if (zneg <= 0 || zpos < zneg) continue;
-    const float factor = (float)((zpos - zneg) / zpos);
-    for (int j = lane; j < NLON; j += 64) {
-      for (int l = 0; l < L; l++) fixq[l * NH + first + j] = fmaxf(fixq[l * NH + first + j] * factor, 0.0f);
-      fixneg[first + j] = 0;
-      fixpos[first + j] = fixpos[first + j] * factor;
-    }
+  for (int o = 32; o > 0; o >>= 1) {
+    zneg += __shfl_xor(zneg, o);
+    zpos += __shfl_xor(zpos, o);
   }
-  __syncthreads();
-  block_sum2(M, fixneg, fixpos, M.arena + M.slot[SLOT_fixsum]);
+  if (zneg <= 0 || zpos < zneg) return;
+  const float factor = (float)((zpos - zneg) / zpos);
+  const int L = M.nlev;
+  float q[kFixerRowPts][kMaxLevels];
+#pragma unroll
+  for (int k = 0; k < kFixerRowPts; k++)
+#pragma unroll
+    for (int l = 0; l < kMaxLevels; l++) {
+      const int j = lane + 64 * k;
+      q[k][l] = (j < NLON && l < L) ? fixq[l * NH + first + j] : 0.0f;
+    }
+#pragma unroll
+  for (int k = 0; k < kFixerRowPts; k++) {
+    const int j = lane + 64 * k;
+    if (j >= NLON) continue;
+#pragma unroll
+    for (int l = 0; l < kMaxLevels; l++)
+      if (l < L) fixq[l * NH + first + j] = fmaxf(q[k][l] * factor, 0.0f);
+    fixneg[first + j] = 0;
+    fixpos[first + j] = pos[k] * factor;
+  }
+}
+
+// fixer's global sum of the rows' results: one workgroup per member.
+extern "C" __global__ __launch_bounds__(kReduceThreads) void k_fixer_sum(const DevMember* __restrict__ mems,
+                                                                          LaunchPlan plan) {
+  const DevMember& M = mems[plan.member[blockIdx.x]];
+  block_sum2(M, M.arena + M.slot[SLOT_fixneg], M.arena + M.slot[SLOT_fixpos], M.arena + M.slot[SLOT_fixsum]);
 }
 
 // mkdca-limit.wgsl:5-22: dca_skip = 1 when any column gave up.
@@ -2227,7 +2270,11 @@ extern "C" __global__ __launch_bounds__(kReduceThreads) void k_dca_limit(const D
   __shared__ float part[kReduceThreads];
   const float* lim = M.arena + M.slot[SLOT_dca_limited];
   float flag = 0;
-  for (int h = threadIdx.x; h < M.nhor; h += kReduceThreads) flag = fmaxf(flag, lim[h]);
+#pragma unroll
+  for (int j = 0; j < kReducePts; j++) {
+    const int h = threadIdx.x + kReduceThreads * j;
+    if (h < M.nhor) flag = fmaxf(flag, lim[h]);
+  }
   part[threadIdx.x] = flag;
   __syncthreads();
   for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
@@ -2253,7 +2300,9 @@ void enqueue_physics(Engine& e, const std::vector<int>& active) {
   Stepper s(e, active);
   const LaunchPlan one = s.plan([](const DevMember&) { return 1; });
   s.column(k_fixer_column);
-  launch(e, k_fixer_reduce, dim3(one.n), dim3(kReduceThreads), e.d_desc, one);
+  const LaunchPlan rows = s.plan([](const DevMember& d) { return (d.nlat + 3) / 4; });
+  launch(e, k_fixer_rows, dim3(rows.first[rows.n]), dim3(256), e.d_desc, rows);
+  launch(e, k_fixer_sum, dim3(one.n), dim3(kReduceThreads), e.d_desc, one);
   s.column(k_surface_fluxes);
   s.column(k_shortwave);
   s.column(k_longwave);

@@ -252,6 +252,8 @@ def library():
         "exo_set_publish_every": (I, [P, I]),
         "exo_set_step_graphs": (I, [P, I]),
         "exo_step_graphs": (I, [P, ctypes.POINTER(ctypes.c_int)]),
+        "exo_set_step_lanes": (I, [P, I]),
+        "exo_step_lanes": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "exo_set_transform_group": (I, [P, I, I, I]),
         "exo_transform_group": (I, [P, I, I, ctypes.POINTER(ctypes.c_int)]),
         "exo_autotune": (I, [P, I, ctypes.POINTER(ctypes.c_double)]),
@@ -663,6 +665,18 @@ class Engine:
         """Replay steps from a captured HIP graph (True, the default) or
         launch their kernels one by one (False)."""
         _check(self.lib.exo_set_step_graphs(self.h, 1 if enable else 0))
+
+    def step_lanes(self):
+        """Streams a replayed step's members run on (1 to 4)."""
+        n = ctypes.c_int()
+        _check(self.lib.exo_step_lanes(self.h, ctypes.byref(n)))
+        return n.value
+
+    def set_step_lanes(self, lanes):
+        """Run a replayed step's members on one stream (1) or split over 2
+        to 4 concurrent streams (2 is the default); results are
+        bit-identical."""
+        _check(self.lib.exo_set_step_lanes(self.h, lanes))
 
     def set_publish_every(self, every):
         _check(self.lib.exo_set_publish_every(self.h, every))
