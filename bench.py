@@ -226,7 +226,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default transform group size (4 members) instead of tuning it")
+                    help="keep the default transform group size (4 members) and lanes instead of tuning them")
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="streams a replayed step's members run on (0: the engine's default, or the "
                          "autotuner's pick)")

@@ -265,7 +265,7 @@ int exo_set_publish_every(exo_engine *e, int every);
 
 /* Tile shape of the grouped transforms: how many members of one
  * (truncation, levels) class share a workgroup of the Legendre and zonal
- * transforms, 1..4 (default 4). Members compute bit-identically for every
+ * transforms, 1..8 (default 4). Members compute bit-identically for every
  * size. The reference compiles one pipeline per planet and has no such
  * choice (kernel/index.ts:17-122); these calls and exo_autotune serve the
  * batched engine. */
@@ -295,16 +295,16 @@ int exo_autotune(exo_engine *e, int trial_steps, double *tuned_ms);
 int exo_set_step_graphs(exo_engine *e, int enable);
 int exo_step_graphs(exo_engine *e, int *enabled);
 
-/* Lanes of a replayed step, 1 to 4: n > 1 (2 by default) splits the
- * active members into n sets of about equal work (grid points x levels)
- * whose launches run on n streams, forked and joined inside the step
- * graph, so that one set's latency-bound launches overlap the others'; 1
- * runs them all on one stream. Results are bit-identical for every n
- * (each member's launches keep their order and arithmetic); exo_autotune
- * keeps the fastest count.
- * Steps launched one by one always use one stream. No reference
- * counterpart: the reference's one command buffer per batch has no
- * concurrent branches. */
+/* Lanes of a step, 1 to 4: n > 1 (2 by default) splits the active
+ * members into n sets of about equal work (grid points x levels) whose
+ * launches run on n streams, forked from and joined back to the engine's
+ * stream (inside the step graph when one is replayed), so that one set's
+ * latency-bound launches overlap the others'; 1 runs them all on one
+ * stream. Results are bit-identical for every n (each member's launches
+ * keep their order and arithmetic); exo_autotune keeps the fastest count.
+ * While exo_probe_kernel times a kernel, steps run on one stream. No
+ * reference counterpart: the reference's one command buffer per batch has
+ * no concurrent branches. */
 int exo_set_step_lanes(exo_engine *e, int lanes);
 int exo_step_lanes(exo_engine *e, int *lanes);
 

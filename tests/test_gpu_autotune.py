@@ -26,7 +26,7 @@ def _run(exo, cfgs, steps, tune_at=None, groups=None):
             e.step(tune_at)
             ms = e.autotune(trial_steps=4)
             picked = {k: e.transform_group(*k) for k in ((21, 5), (42, 10))}
-            assert all(1 <= n <= 4 for n in picked.values())
+            assert all(1 <= n <= 8 for n in picked.values())
             assert e.step_graphs() in (True, False)
             steps -= tune_at
         e.step(steps)
@@ -37,7 +37,8 @@ def _run(exo, cfgs, steps, tune_at=None, groups=None):
     return arenas, snaps, counters, ms
 
 
-@pytest.mark.parametrize("groups", [{(21, 5): 1, (42, 10): 1}, {(21, 5): 3, (42, 10): 2}])
+@pytest.mark.parametrize("groups", [{(21, 5): 1, (42, 10): 1}, {(21, 5): 3, (42, 10): 2},
+                                    {(21, 5): 4, (42, 10): 4}])
 def test_group_sizes_are_bit_identical(exo, groups):
     cfgs = _planets(exo)
     ref = _run(exo, cfgs, 9)

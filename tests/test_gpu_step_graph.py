@@ -19,12 +19,14 @@ def _planets(exo):
 def test_graph_replays_match_single_launches(exo, lanes):
     """With lanes = 2 the replayed step runs the T42 member on one stream
     and the two T21 members on another, concurrently; with 3, each member
-    on a stream of its own."""
+    on a stream of its own. The single launches run on one stream, or in
+    two lanes against the one-stream replay."""
     cfgs = _planets(exo)
     extra = exo.default_config()
     engines = [exo.Engine(0, 4), exo.Engine(0, 4)]
     try:
         engines[0].set_step_graphs(False)
+        engines[0].set_step_lanes(2 if lanes == 1 else 1)
         engines[1].set_step_graphs(True)
         engines[1].set_step_lanes(lanes)
         assert engines[1].step_lanes() == lanes

@@ -47,8 +47,10 @@ __device__ __forceinline__ int plan_find(const LaunchPlan& p, int block, int* ti
   return p.member[lo];
 }
 
-// Members one Legendre workgroup serves at most (FP32 form).
-constexpr int kLegGroup = 4;
+// Members one Legendre workgroup serves at most (FP32 form), and the
+// group size where none is set or tuned.
+constexpr int kLegGroup = 8;
+constexpr int kLegGroupDefault = 4;
 
 // Members grouped for the transform kernels: group i is list[start[i] ..
 // start[i] + count[i]), members of one truncation and level count (one set

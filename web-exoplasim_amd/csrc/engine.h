@@ -182,13 +182,13 @@ struct Engine {
   hipEvent_t lane_fork = nullptr;
   hipEvent_t lane_join[kMaxLanes - 1] = {};
   // Members per group of the grouped transforms, per (truncation, levels);
-  // kLegGroup where unset. exo_autotune picks them by timing; any value in
+  // kLegGroupDefault where unset. exo_autotune picks them by timing; any value in
   // 1..kLegGroup gives bit-identical members, since every member's rows
   // are reduced on their own.
   std::map<std::pair<int, int>, int> group_cap;
   int group_cap_of(const DevMember& d) const {
     const auto it = group_cap.find({d.ntru, d.nlev});
-    return it == group_cap.end() ? kLegGroup : it->second;
+    return it == group_cap.end() ? kLegGroupDefault : it->second;
   }
   std::vector<int64_t> graph_key;
   hipGraphExec_t graph = nullptr;

@@ -692,7 +692,11 @@ static int step_members(exo_engine* h, int steps, bool fixed) {
         index_at = k + 1;
       } else {
         exo::Stepper(e, desc).load_step(d_rows + k * per_step);
-        exo::enqueue_step(e, desc);
+        // the kernel probe times single launches on the engine's stream
+        if (e.probe.kernel)
+          exo::enqueue_step(e, desc);
+        else
+          exo::enqueue_step_lanes(e, desc);
         if (!main_desc.empty()) exo::Stepper(e, main_desc).column(k_pack);
       }
       if (!avg_at[k].empty()) exo::enqueue_averages(e, avg_at[k], avg_step[k]);

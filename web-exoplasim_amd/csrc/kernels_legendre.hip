@@ -147,7 +147,7 @@ constexpr int kFwdGTiles = 2;
 constexpr int kInvRows = 40;   // staged input rows: inverse
 constexpr int kFwdRows = 30;   // staged input rows: forward
 // input rows of a whole group: members x fields x levels
-constexpr int kLegGroupRows = 128;
+constexpr int kLegGroupRows = 256;
 static_assert(kLegGroup * 3 * kMaxLev <= kLegGroupRows, "group input rows");
 
 __device__ __forceinline__ int group_member(const GroupPlan& plan, int g, int i) {
@@ -225,13 +225,13 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
   // where each input row of the group starts (member g, field f, level p
-  // at row (g nf + f) P + p), and the planetary vorticity it drops from
-  // Re mode (0, 1) (zeta rows of dv2uv)
+  // at row (g nf + f) P + p), and per member the planetary vorticity its
+  // zeta rows (field 1 of dv2uv) drop from Re mode (0, 1)
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
     rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp;
-    rowsub[row] = (job.plavor && f == 1) ? G.k[CON_plavor] : 0.0f;
+    if (fp == 0) rowsub[g] = job.plavor ? G.k[CON_plavor] : 0.0f;
   }
   // staged column col < RT: mode w of the pair's wavenumber, or padding
   const auto mode_of = [=](int col) {
@@ -248,7 +248,10 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
       const int w = mode_of(x >> 1);
       if (w < 0) return 0.0f;
       const float v = rowp[r0 + row][2 * w + (x & 1)];
-      return (w == 1 && (x & 1) == 0) ? v - rowsub[r0 + row] : v;
+      if (w != 1 || (x & 1) || nf == 1) return v;
+      // Re mode (0, 1) of a zeta row (field 1) loses the planetary vorticity
+      const int r = r0 + row, g = r / (2 * P);
+      return r - g * 2 * P >= P ? v - rowsub[g] : v;
     };
   };
   const auto raw_store = [&](int row, int x, float v) { raw[row * kPairRawK + x] = v; };
@@ -656,7 +659,7 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
     rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp;
-    rowsub[row] = (job.plavor && f == 1) ? G.k[CON_plavor] : 0.0f;
+    if (fp == 0) rowsub[g] = job.plavor ? G.k[CON_plavor] : 0.0f;
   }
   const auto mode_of = [=](int col) {
     if (col < Ra) return col < Ka ? wa + col : -1;
@@ -670,7 +673,10 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
       const int w = mode_of(x >> 1);
       if (w < 0) return 0.0f;
       const float v = rowp[r0 + row][2 * w + (x & 1)];
-      return (w == 1 && (x & 1) == 0) ? v - rowsub[r0 + row] : v;
+      if (w != 1 || (x & 1) || nf == 1) return v;
+      // Re mode (0, 1) of a zeta row (field 1) loses the planetary vorticity
+      const int r = r0 + row, g = r / (2 * P);
+      return r - g * 2 * P >= P ? v - rowsub[g] : v;
     };
   };
   const auto raw_store = [&](int row, int x, float v) { raw[row * kSplitRaw + x] = v; };
@@ -951,7 +957,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   __shared__ float As[2 * kPairAs];
   __shared__ float raw[kInvRows * kPairRawK];
   __shared__ const float* rowp[kInvGroupRows];
-  __shared__ float rowsub[kInvGroupRows];
+  __shared__ float rowsub[kLegGroup];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
@@ -969,7 +975,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   __shared__ __align__(16) float As[2 * kMaxLat * kSplitAs];
   __shared__ __align__(16) float raw[kSplitInvRows * kSplitRaw];
   __shared__ const float* rowp[kInvGroupRows];
-  __shared__ float rowsub[kInvGroupRows];
+  __shared__ float rowsub[kLegGroup];
   __shared__ float bscale[2 * kSplitInvRows];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
