@@ -86,6 +86,24 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
   return (run << 6) + (b & 7) * 8 + ((b >> 3) & 7);
 }
 
-#define SLOTP(M, name) ((M).arena + (M).slot[SLOT_##name])
-#define TABP(M, name) ((M).tab + (M).tabo[TAB_##name])
+// A descriptor's device pointers, read as pointers into the global address
+// space. Read as plain float*, they are generic, and every access through
+// them is a flat_* instruction, which the hardware counts against the LDS
+// and scalar-load counter as well as the vector-memory one, so each wait
+// for a scalar or LDS result also waits for every flat load in flight.
+// Read as global pointers, the accesses become global_* instructions.
+typedef __attribute__((address_space(1))) float exo_gfloat;
+typedef __attribute__((address_space(1))) const float exo_gcfloat;
+__device__ __forceinline__ float* dev_arena(const DevMember& M) {
+  return (float*)*static_cast<exo_gfloat* const*>(static_cast<const void*>(&M.arena));
+}
+__device__ __forceinline__ const float* dev_tab(const DevMember& M) {
+  return (const float*)*static_cast<exo_gcfloat* const*>(static_cast<const void*>(&M.tab));
+}
+__device__ __forceinline__ const float* dev_ltab(const DevMember& M) {
+  return (const float*)*static_cast<exo_gcfloat* const*>(static_cast<const void*>(&M.ltab));
+}
+
+#define SLOTP(M, name) (dev_arena(M) + (M).slot[SLOT_##name])
+#define TABP(M, name) (dev_tab(M) + (M).tabo[TAB_##name])
 #define KON(M, name) ((M).k[CON_##name])

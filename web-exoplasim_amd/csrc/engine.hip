@@ -77,7 +77,7 @@ __device__ __forceinline__ void store_step_value(const DevMember& M, int t, floa
     slot = SLOT_snapshot_step;
     at = t - 13;
   }
-  M.arena[M.slot[slot] + at] = v;
+  dev_arena(M)[M.slot[slot] + at] = v;
 }
 
 }  // namespace

@@ -394,7 +394,7 @@ __device__ __forceinline__ void dft_inv(const DevMember* __restrict__ mems, cons
   for (int j = 0; j < jobs.n; j++) per_member += rows_of(jobs.j[j]);
   const int total = plan.count[grp] * per_member;
   const int row = block * kRows + (threadIdx.x >> 1);
-  const float* src = M0.arena;
+  const float* src = dev_arena(M0);
   float* d = nullptr;
   int stride = 0;
   bool derived = false;
@@ -404,8 +404,8 @@ __device__ __forceinline__ void dft_inv(const DevMember* __restrict__ mems, cons
     const DevMember& G = mems[plan.list[plan.start[grp] + g]];
     const int plane_rows = (job.per_level ? M0.nlev : 1) * nlat;  // rows of one output
     const int o = r / plane_rows, pr = r - o * plane_rows;
-    src = G.arena + G.slot[(job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0] + pr;
-    d = G.arena + G.slot[o == 0 ? job.dst0 : job.dst1] + pr * nlon;
+    src = dev_arena(G) + G.slot[(job.kind == INV_DV2UV && o == 1) ? job.four1 : job.four0] + pr;
+    d = dev_arena(G) + G.slot[o == 0 ? job.dst0 : job.dst1] + pr * nlon;
     derived = job.kind == INV_DLAMBDA && o == 1;
     stride = plane_rows;
   }
@@ -424,15 +424,15 @@ __device__ __forceinline__ void dft_fwd(const DevMember* __restrict__ mems, cons
   for (int j = 0; j < jobs.n; j++) per_member += rows_of(jobs.j[j]);
   const int total = plan.count[grp] * per_member;
   const int row = block * kRows + (threadIdx.x >> 1);
-  const float* s = M0.arena;
+  const float* s = dev_arena(M0);
   float* d = nullptr;
   int stride = 0;
   if (row < total) {
     int g, r;
     const FwdJob& job = jobs.j[locate(jobs, per_member, row, rows_of, &g, &r)];
     const DevMember& G = mems[plan.list[plan.start[grp] + g]];
-    s = G.arena + G.slot[job.src] + r * nlon;
-    d = G.arena + G.slot[job.dst] + r;
+    s = dev_arena(G) + G.slot[job.src] + r * nlon;
+    d = dev_arena(G) + G.slot[job.dst] + r;
     stride = rows_of(job);
   }
   if ((threadIdx.x & 1) == 0) src[threadIdx.x >> 1] = s;

@@ -19,8 +19,8 @@ template <int NL>
 __device__ void calcgp_column(const DevMember& M, int h) {
   // dynamics/wgsl/calcgp.wgsl:14-147
   const int NHOR = M.nhor;
-  float* A = M.arena;
-  const float* T = M.tab;
+  float* A = dev_arena(M);
+  const float* T = dev_tab(M);
   auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NHOR + h]; };
   auto tab = [&](int t, int i) { return T[M.tabo[t] + i]; };
   const float PSURF = M.k[CON_psurf], WW = M.k[CON_ww], CV = M.k[CON_cv];
@@ -146,8 +146,8 @@ __device__ void spectrala_levels(const DevMember& M, int tile, LevelRows* lds) {
   const int NRSP = M.nrsp;
   const bool live = s < NRSP;
   const int sc = live ? s : 0;
-  float* A = M.arena;
-  const float* T = M.tab;
+  float* A = dev_arena(M);
+  const float* T = dev_tab(M);
   auto lev = [&](int slot) -> float& { return A[M.slot[slot] + l * NRSP + sc]; };
   auto tab = [&](int t, int i) { return T[M.tabo[t] + i]; };
   const float DELT = M.k[CON_delt], DELT2 = M.k[CON_delt2];
@@ -229,8 +229,8 @@ __device__ void spectrala_levels(const DevMember& M, int tile, LevelRows* lds) {
 __device__ void diffusion_coeff(const DevMember& M, int s, int l) {
   // dynamics/wgsl/spectrald-diffusion.wgsl:12-60
   const int NRSP = M.nrsp;
-  float* A = M.arena;
-  const float* T = M.tab;
+  float* A = dev_arena(M);
+  const float* T = dev_tab(M);
   auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
   const float DELT2 = M.k[CON_delt2], PLAVOR = M.k[CON_plavor];
   if (s < 2 && l == 0) A[M.slot[SLOT_spp] + s] = 0.0f;
@@ -277,7 +277,7 @@ __device__ void diffusion_coeff(const DevMember& M, int s, int l) {
 __device__ void update_coeff(const DevMember& M, int s, int l) {
   // dynamics/wgsl/spectrald-update.wgsl:12-45
   const int NRSP = M.nrsp;
-  float* A = M.arena;
+  float* A = dev_arena(M);
   auto lev = [&](int slot, int l) -> float& { return A[M.slot[slot] + l * NRSP + s]; };
   const float DELT2 = M.k[CON_delt2], PNU = M.k[CON_pnu];
   const bool FILTER = M.k[CON_filter] != 0.0f;
@@ -315,7 +315,7 @@ __device__ void update_coeff(const DevMember& M, int s, int l) {
 // gp itself, the level-0 thread stores it.
 __device__ void physics_in_point(const DevMember& M, int h, int l) {
   const int NHOR = M.nhor, NL = M.nlev;
-  float* A = M.arena;
+  float* A = dev_arena(M);
   const float PSURF = M.k[CON_psurf], CV = M.k[CON_cv], CT = M.k[CON_ct];
   const int at = l * NHOR + h;
   A[M.slot[SLOT_dudt] + at] = 0.0f;
@@ -325,14 +325,14 @@ __device__ void physics_in_point(const DevMember& M, int h, int l) {
   if (l >= NL) return;
   const float dp = PSURF * expf(A[M.slot[SLOT_gp] + h]);
   if (l == 0) A[M.slot[SLOT_dp] + h] = dp;
-  const float scale = sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
+  const float scale = sqrtf(dev_tab(M)[M.tabo[TAB_rcsq] + h]);
   A[M.slot[SLOT_gudt] + at] = 0.0f;
   A[M.slot[SLOT_gvdt] + at] = 0.0f;
   A[M.slot[SLOT_gtdt] + at] = 0.0f;
   A[M.slot[SLOT_gqdt] + at] = 0.0f;
   A[M.slot[SLOT_du] + at] = CV * A[M.slot[SLOT_gu] + at] * scale;
   A[M.slot[SLOT_dv] + at] = CV * A[M.slot[SLOT_gv] + at] * scale;
-  A[M.slot[SLOT_dt] + at] = CT * (A[M.slot[SLOT_gt] + at] + M.tab[M.tabo[TAB_t0] + l]);
+  A[M.slot[SLOT_dt] + at] = CT * (A[M.slot[SLOT_gt] + at] + dev_tab(M)[M.tabo[TAB_t0] + l]);
   A[M.slot[SLOT_dq] + at] = A[M.slot[SLOT_gq] + at] * PSURF / dp;
 }
 
@@ -406,12 +406,12 @@ extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __re
   const int i = tile * 64 + threadIdx.x;
   if (i >= M.nlev * M.nhor) return;
   const int l = i / M.nhor, h = i % M.nhor;
-  float* A = M.arena;
+  float* A = dev_arena(M);
   const float CV = M.k[CON_cv], CT = M.k[CON_ct], WW = M.k[CON_ww], GA = M.k[CON_ga];
   const float ACPD = M.k[CON_acpd], ADV = M.k[CON_adv], DELTSEC2 = M.k[CON_deltsec2];
   const float heat = 1.0f + ADV * A[M.slot[SLOT_dq] + i];
-  const float dsig = M.tab[M.tabo[TAB_dsigma] + l];
-  const float scale = CV * sqrtf(M.tab[M.tabo[TAB_rcsq] + h]);
+  const float dsig = dev_tab(M)[M.tabo[TAB_dsigma] + l];
+  const float scale = CV * sqrtf(dev_tab(M)[M.tabo[TAB_rcsq] + h]);
   const float u = A[M.slot[SLOT_zu] + i] * scale;
   const float v = A[M.slot[SLOT_zv] + i] * scale;
   {
@@ -448,12 +448,12 @@ extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember*
   int l;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &l)];
   const int NH = M.nhor, nlat = M.nlat, nlon = M.nlon, ncsp = M.ncsp, nfc = M.nfc, nrsp = M.nrsp;
-  float* A = M.arena;
+  float* A = dev_arena(M);
   float* zdekin = A + M.slot[SLOT_zdekin] + l * NH;
-  const float* qc = M.ltab + M.tabo[TAB_legendre_qc];
-  const float* qi = M.ltab + M.tabo[TAB_legendre_qi];
-  const float* fwd = M.ltab + M.tabo[TAB_fourier_forward];
-  const float* inv = M.ltab + M.tabo[TAB_fourier_inverse];
+  const float* qc = dev_ltab(M) + M.tabo[TAB_legendre_qc];
+  const float* qi = dev_ltab(M) + M.tabo[TAB_legendre_qi];
+  const float* fwd = dev_ltab(M) + M.tabo[TAB_fourier_forward];
+  const float* inv = dev_ltab(M) + M.tabo[TAB_fourier_inverse];
   const int half = nlat / 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // Fourier mode 0 of each point's row, weighted by its latitude's qc
@@ -496,7 +496,7 @@ extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember*
   // back to the grid, then the heating (mkdheat stage 3); the inputs of
   // every point first, then the stores
   const float CT = M.k[CON_ct], WW = M.k[CON_ww], GA = M.k[CON_ga], ACPD = M.k[CON_acpd], ADV = M.k[CON_adv];
-  const float dsig = M.tab[M.tabo[TAB_dsigma] + l];
+  const float dsig = dev_tab(M)[M.tabo[TAB_dsigma] + l];
   const float* dq = A + M.slot[SLOT_dq] + l * NH;
   const float* dp = A + M.slot[SLOT_dp];
   float* zdtdt = A + M.slot[SLOT_zdtdt] + l * NH;

@@ -203,7 +203,7 @@ __device__ __forceinline__ void store_cols(int rows, Store store, const float* v
 
 template <int P>
 __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
-                        float* As, float* raw, const float** rowp, float* rowsub) {
+                        float* As, float* raw, exo_gcfloat** rowp, float* rowsub) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1;
@@ -220,8 +220,8 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   const int chunk = max(1, min(count, kInvRows / (nf * P)));
   const int MT = nlat / 16;
   const int blocks = winds ? 2 : 1;
-  const float* q0 = M.ltab + M.tabo[winds ? (int)TAB_legendre_qu : job.table];
-  const float* q1 = M.ltab + M.tabo[TAB_legendre_qv];
+  const float* q0 = dev_ltab(M) + M.tabo[winds ? (int)TAB_legendre_qu : job.table];
+  const float* q1 = dev_ltab(M) + M.tabo[TAB_legendre_qv];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
   // where each input row of the group starts (member g, field f, level p
@@ -230,7 +230,7 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp;
+    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[f ? job.src1 : job.src0] + p * nrsp);
     if (fp == 0) rowsub[g] = job.plavor ? G.k[CON_plavor] : 0.0f;
   }
   // staged column col < RT: mode w of the pair's wavenumber, or padding
@@ -319,7 +319,7 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
             const int g = c / NCm;
             c -= g * NCm;
             const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-            float* base = G.arena + G.slot[winds && (c & 2) ? job.four1 : job.four0];
+            float* base = dev_arena(G) + G.slot[winds && (c & 2) ? job.four1 : job.four0];
             const int p = winds ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -368,7 +368,7 @@ __device__ __forceinline__ void fold_load(int rows, int nlat, int plane_stride, 
   for (int j = 0; j < kFwdFoldN; j++) {
     const int r = r0 + kFwdFoldS * j;
     if (ok && r < rows) {
-      const float* x = rowp(r) + part * plane_stride;
+      const auto x = rowp(r) + part * plane_stride;
       v[j][0] = x[l];
       v[j][1] = x[nlat - 1 - l];
     } else {
@@ -395,7 +395,7 @@ __device__ __forceinline__ void fold_store(int rows, int nlat, float* raw, const
 
 template <int P>
 __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, const LegJob& job, int m, float* Ah,
-                        float* raw, const float** rowp) {
+                        float* raw, exo_gcfloat** rowp) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, half = nlat / 2, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
@@ -429,7 +429,7 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = G.arena + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
+    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat);
   }
   // staged table row r: mode k of m, or -1 for padding
   const auto row_mode = [=](int r) {
@@ -444,7 +444,7 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
-        const float* q = M.ltab + M.tabo[tables[blk]];
+        const float* q = dev_ltab(M) + M.tabo[tables[blk]];
         load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
           const int k = row_mode(r);
           return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
@@ -501,7 +501,7 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
           const int g = c / NCm;
           c -= g * NCm;
           const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-          float* base = G.arena + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
+          float* base = dev_arena(G) + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
           const int p = uv ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
@@ -637,7 +637,7 @@ __device__ __forceinline__ void split_table(float* A, int rows, int stride, int 
 
 template <int P>
 __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
-                               float* As, float* raw, const float** rowp, float* rowsub, float* bscale) {
+                               float* As, float* raw, exo_gcfloat** rowp, float* rowsub, float* bscale) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1;
@@ -658,7 +658,7 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = G.arena + G.slot[f ? job.src1 : job.src0] + p * nrsp;
+    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[f ? job.src1 : job.src0] + p * nrsp);
     if (fp == 0) rowsub[g] = job.plavor ? G.k[CON_plavor] : 0.0f;
   }
   const auto mode_of = [=](int col) {
@@ -689,7 +689,7 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
 #pragma unroll
     for (int blk = 0; blk < 2; blk++)
       if (blk < blocks) {
-        const float* q = M.ltab + M.tabo[tabs[blk]];
+        const float* q = dev_ltab(M) + M.tabo[tabs[blk]];
         load_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col) {
           const int w = col < RT ? mode_of(col) : -1;
           return w >= 0 ? q[lat * ncsp + w] : 0.0f;
@@ -752,7 +752,7 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
             const int g = c / NCm;
             c -= g * NCm;
             const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-            float* base = G.arena + G.slot[winds && (c & 2) ? job.four1 : job.four0];
+            float* base = dev_arena(G) + G.slot[winds && (c & 2) ? job.four1 : job.four0];
             const int p = winds ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -816,7 +816,7 @@ __device__ __forceinline__ void split_fold_rows(float* raw, int rows, int half, 
 
 template <int P>
 __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const GroupPlan& plan, int grp, const LegJob& job, int m,
-                               float* Ah, float* raw, const float** rowp, float* bscale) {
+                               float* Ah, float* raw, exo_gcfloat** rowp, float* bscale) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, half = nlat / 2, ntp1 = M.ntru + 1, K = ntp1 - m, w0 = mode_first(m, ntp1);
@@ -845,7 +845,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = G.arena + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
+    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat);
   }
   const auto row_mode = [=](int r) {
     const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
@@ -857,7 +857,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
-        const float* q = M.ltab + M.tabo[tables[blk]];
+        const float* q = dev_ltab(M) + M.tabo[tables[blk]];
         load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
           const int k = row_mode(r);
           return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
@@ -918,7 +918,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
           const int g = c / NCm;
           c -= g * NCm;
           const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-          float* base = G.arena + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
+          float* base = dev_arena(G) + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
           const int p = uv ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
@@ -956,7 +956,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
                                                                        GroupPlan plan, InvJobs jobs) {
   __shared__ float As[2 * kPairAs];
   __shared__ float raw[kInvRows * kPairRawK];
-  __shared__ const float* rowp[kInvGroupRows];
+  __shared__ exo_gcfloat* rowp[kInvGroupRows];
   __shared__ float rowsub[kLegGroup];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
@@ -974,7 +974,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
                                                                              GroupPlan plan, InvJobs jobs) {
   __shared__ __align__(16) float As[2 * kMaxLat * kSplitAs];
   __shared__ __align__(16) float raw[kSplitInvRows * kSplitRaw];
-  __shared__ const float* rowp[kInvGroupRows];
+  __shared__ exo_gcfloat* rowp[kInvGroupRows];
   __shared__ float rowsub[kLegGroup];
   __shared__ float bscale[2 * kSplitInvRows];
   int tile;
@@ -991,7 +991,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
                                                                        GroupPlan plan, LegJobs jobs) {
   __shared__ float Ah[3 * kFwdAs];
   __shared__ float raw[kFwdRows * kRawLat];
-  __shared__ const float* rowp[kLegGroupRows];
+  __shared__ exo_gcfloat* rowp[kLegGroupRows];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
@@ -1008,7 +1008,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
                                                                              GroupPlan plan, LegJobs jobs) {
   __shared__ __align__(16) float Ah[3 * kMaxK * kSplitFwdAs];
   __shared__ __align__(16) float raw[kFwdRows * kRawLat];
-  __shared__ const float* rowp[kLegGroupRows];
+  __shared__ exo_gcfloat* rowp[kLegGroupRows];
   __shared__ float bscale[4 * kFwdRows];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);

@@ -32,8 +32,8 @@
 
 namespace {
 
-#define F(s) (M.arena + M.slot[SLOT_##s])
-#define T(s) (M.tab + M.tabo[TAB_##s])
+#define F(s) (dev_arena(M) + M.slot[SLOT_##s])
+#define T(s) (dev_tab(M) + M.tabo[TAB_##s])
 #define K(s) (M.k[CON_##s])
 #define LV(s, l, h) (F(s)[(l) * NH + (h)])
 
@@ -2211,9 +2211,9 @@ extern "C" __global__ __launch_bounds__(256) void k_fixer_rows(const DevMember* 
   const int row = tile * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= M.nlat) return;
   const int NH = M.nhor, NLON = M.nlon, first = row * NLON;
-  float* fixq = M.arena + M.slot[SLOT_fixq];
-  float* fixneg = M.arena + M.slot[SLOT_fixneg];
-  float* fixpos = M.arena + M.slot[SLOT_fixpos];
+  float* fixq = dev_arena(M) + M.slot[SLOT_fixq];
+  float* fixneg = dev_arena(M) + M.slot[SLOT_fixneg];
+  float* fixpos = dev_arena(M) + M.slot[SLOT_fixpos];
   float neg[kFixerRowPts], pos[kFixerRowPts];
 #pragma unroll
   for (int k = 0; k < kFixerRowPts; k++) {
@@ -2260,7 +2260,7 @@ extern "C" __global__ __launch_bounds__(256) void k_fixer_rows(const DevMember* 
 extern "C" __global__ __launch_bounds__(kReduceThreads) void k_fixer_sum(const DevMember* __restrict__ mems,
                                                                           LaunchPlan plan) {
   const DevMember& M = mems[plan.member[blockIdx.x]];
-  block_sum2(M, M.arena + M.slot[SLOT_fixneg], M.arena + M.slot[SLOT_fixpos], M.arena + M.slot[SLOT_fixsum]);
+  block_sum2(M, dev_arena(M) + M.slot[SLOT_fixneg], dev_arena(M) + M.slot[SLOT_fixpos], dev_arena(M) + M.slot[SLOT_fixsum]);
 }
 
 // mkdca-limit.wgsl:5-22: dca_skip = 1 when any column gave up.
@@ -2268,7 +2268,7 @@ extern "C" __global__ __launch_bounds__(kReduceThreads) void k_dca_limit(const D
                                                                           LaunchPlan plan) {
   const DevMember& M = mems[plan.member[blockIdx.x]];
   __shared__ float part[kReduceThreads];
-  const float* lim = M.arena + M.slot[SLOT_dca_limited];
+  const float* lim = dev_arena(M) + M.slot[SLOT_dca_limited];
   float flag = 0;
 #pragma unroll
   for (int j = 0; j < kReducePts; j++) {
@@ -2281,7 +2281,7 @@ extern "C" __global__ __launch_bounds__(kReduceThreads) void k_dca_limit(const D
     if ((int)threadIdx.x < s) part[threadIdx.x] = fmaxf(part[threadIdx.x], part[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) M.arena[M.slot[SLOT_dca_skip]] = part[0];
+  if (threadIdx.x == 0) dev_arena(M)[M.slot[SLOT_dca_skip]] = part[0];
 }
 
 // getiflx's global sums (physics/surface/index.ts:72-96), members whose
@@ -2290,8 +2290,8 @@ extern "C" __global__ __launch_bounds__(kReduceThreads) void k_ice_sums(const De
                                                                          LaunchPlan plan) {
   const DevMember& M = mems[plan.member[blockIdx.x]];
   if (M.k[CON_ice_coupled] == 0.0f) return;
-  block_sum2(M, M.arena + M.slot[SLOT_ice_part1], M.arena + M.slot[SLOT_ice_part2],
-             M.arena + M.slot[SLOT_ice_sums]);
+  block_sum2(M, dev_arena(M) + M.slot[SLOT_ice_part1], dev_arena(M) + M.slot[SLOT_ice_part2],
+             dev_arena(M) + M.slot[SLOT_ice_sums]);
 }
 
 namespace exo {

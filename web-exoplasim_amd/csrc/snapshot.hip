@@ -26,7 +26,7 @@ extern "C" __global__ __launch_bounds__(64) void k_pack(const DevMember* __restr
   const int h = tile * 64 + threadIdx.x;
   const int NH = M.nhor, L = M.nlev;
   if (h >= NH) return;
-  float* A = M.arena;
+  float* A = dev_arena(M);
   auto S = [&](int slot) { return A + M.slot[slot]; };
   auto put = [&](int index, float value) { A[M.slot[SLOT_snapshot] + index * NH + h] = value; };
   const float* ss = S(SLOT_snapshot_step);
@@ -132,10 +132,10 @@ extern "C" __global__ __launch_bounds__(256) void k_global_means(const DevMember
                                                                   double* __restrict__ out) {
   const DevMember& M = mems[plan.member[blockIdx.x]];
   const int NH = M.nhor, L = M.nlev;
-  const float* ts = M.arena + M.slot[SLOT_dt] + L * NH;
-  const float* sw = M.arena + M.slot[SLOT_dswfl];
-  const float* lw = M.arena + M.slot[SLOT_dlwfl];
-  const float* w = M.tab + M.tabo[TAB_zgw];
+  const float* ts = dev_arena(M) + M.slot[SLOT_dt] + L * NH;
+  const float* sw = dev_arena(M) + M.slot[SLOT_dswfl];
+  const float* lw = dev_arena(M) + M.slot[SLOT_dlwfl];
+  const float* w = dev_tab(M) + M.tabo[TAB_zgw];
   __shared__ double s0[256], s1[256], s2[256];
   double a = 0, b = 0, c = 0;
   for (int h = threadIdx.x; h < NH; h += 256) {
