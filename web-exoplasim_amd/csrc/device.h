@@ -55,12 +55,17 @@ constexpr int kLegGroupDefault = 4;
 // Members grouped for the transform kernels: group i is list[start[i] ..
 // start[i] + count[i]), members of one truncation and level count (one set
 // of shared tables); first[i] is its first workgroup, first[n] the total.
+// The members of a group share one arena layout (truncation and levels),
+// so a slot's offset is read once from the group's first descriptor;
+// arena[j] is the arena of list[j], so the transform kernels find a
+// member's rows without reading its descriptor.
 struct GroupPlan {
   int n;
   int first[EXO_MAX_MEMBERS + 1];
   int start[EXO_MAX_MEMBERS];
   int count[EXO_MAX_MEMBERS];
   int list[EXO_MAX_MEMBERS];
+  float* arena[EXO_MAX_MEMBERS];
 };
 
 __device__ __forceinline__ int group_find(const GroupPlan& p, int block, int* tile) {

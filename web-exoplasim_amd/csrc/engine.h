@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cstdint>
 #include <deque>
 #include <initializer_list>
@@ -274,7 +275,14 @@ class Stepper {
     for (const std::vector<int>& g : groups) {
       p.start[p.n] = listed;
       p.count[p.n] = (int)g.size();
-      for (int id : g) p.list[listed++] = id;
+      for (int id : g) {
+        const DevMember& m = e_.h_desc[id];
+        const DevMember& m0 = e_.h_desc[g[0]];
+        if (std::memcmp(m.slot, m0.slot, sizeof(m.slot)) != 0)
+          throw std::runtime_error("members of a transform group must share one arena layout");
+        p.arena[listed] = m.arena;
+        p.list[listed++] = id;
+      }
       p.first[p.n] = at;
       at += blocks(e_.h_desc[g[0]], (int)g.size());
       p.n++;

@@ -154,6 +154,16 @@ __device__ __forceinline__ int group_member(const GroupPlan& plan, int g, int i)
   return plan.list[plan.start[g] + i];
 }
 
+// Member i of group g: its arena (global address space), from the plan.
+__device__ __forceinline__ exo_gfloat* group_arena(const GroupPlan& plan, int g, int i) {
+  return (exo_gfloat*)plan.arena[plan.start[g] + i];
+}
+
+// The planetary vorticity a zeta row of member i drops from Re mode (0, 1).
+__device__ __forceinline__ float group_plavor(const DevMember* mems, const GroupPlan& plan, int g, int i) {
+  return mems[group_member(plan, g, i)].k[CON_plavor];
+}
+
 // ------------------------------------------------------------- inverse
 // A workgroup takes the wavenumber pair (m, T - m) of one job and member
 // group: K(m) + K(T - m) = T + 2 modes whatever m, so every workgroup does
@@ -203,7 +213,7 @@ __device__ __forceinline__ void store_cols(int rows, Store store, const float* v
 
 template <int P>
 __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
-                        float* As, float* raw, exo_gcfloat** rowp, float* rowsub) {
+                        float* As, float* raw, exo_gcfloat** rowp) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1;
@@ -225,13 +235,10 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
   // where each input row of the group starts (member g, field f, level p
-  // at row (g nf + f) P + p), and per member the planetary vorticity its
-  // zeta rows (field 1 of dv2uv) drop from Re mode (0, 1)
+  // at row (g nf + f) P + p)
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
-    const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[f ? job.src1 : job.src0] + p * nrsp);
-    if (fp == 0) rowsub[g] = job.plavor ? G.k[CON_plavor] : 0.0f;
+    rowp[row] = group_arena(plan, grp, g) + M.slot[f ? job.src1 : job.src0] + p * nrsp;
   }
   // staged column col < RT: mode w of the pair's wavenumber, or padding
   const auto mode_of = [=](int col) {
@@ -241,6 +248,8 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   };
   // the members' coefficients (zeta without the planetary vorticity in
   // Re mode (0, 1)), zeros in the padding, for the chunk from member c0
+  const GroupPlan* gp = &plan;  // the lambdas below copy a pointer, not the plan
+  const bool plavor = job.plavor;
   const auto raw_loader = [&](int c0) {
     const int r0 = c0 * nf * P;
     return [=](int row, int x) {
@@ -248,27 +257,29 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
       const int w = mode_of(x >> 1);
       if (w < 0) return 0.0f;
       const float v = rowp[r0 + row][2 * w + (x & 1)];
-      if (w != 1 || (x & 1) || nf == 1) return v;
+      if (w != 1 || (x & 1) || !plavor) return v;
       // Re mode (0, 1) of a zeta row (field 1) loses the planetary vorticity
       const int r = r0 + row, g = r / (2 * P);
-      return r - g * 2 * P >= P ? v - rowsub[g] : v;
+      return r - g * 2 * P >= P ? v - group_plavor(mems, *gp, grp, g) : v;
     };
   };
   const auto raw_store = [&](int row, int x, float v) { raw[row * kPairRawK + x] = v; };
-  __syncthreads();  // rowp
-  // one memory latency for the table columns of the pair (dv2uv: qu, qv)
-  // and the first chunk's inputs: every load is issued before any store
-  {
-    float va[2][kPairTabN], vr[kPairRawN];
+  // the table columns of the pair (dv2uv: qu, qv) need no row pointer:
+  // their loads are in flight across the barrier that publishes rowp, and
+  // the first chunk's inputs follow; every load is issued before any store
+  float va[2][kPairTabN];
 #pragma unroll
-    for (int blk = 0; blk < 2; blk++)
-      if (blk < blocks) {
-        const float* q = blk ? q1 : q0;
-        load_cols<kPairTabN, kPairK>(nlat, [&](int lat, int col) {
-          const int w = col < RT ? mode_of(col) : -1;
-          return w >= 0 ? q[lat * ncsp + w] : 0.0f;
-        }, va[blk]);
-      }
+  for (int blk = 0; blk < 2; blk++)
+    if (blk < blocks) {
+      const float* q = blk ? q1 : q0;
+      load_cols<kPairTabN, kPairK>(nlat, [&](int lat, int col) {
+        const int w = col < RT ? mode_of(col) : -1;
+        return w >= 0 ? q[lat * ncsp + w] : 0.0f;
+      }, va[blk]);
+    }
+  __syncthreads();  // rowp
+  {
+    float vr[kPairRawN];
     load_cols<kPairRawN, 2 * kPairK>(min(chunk, count) * nf * P, raw_loader(0), vr);
 #pragma unroll
     for (int blk = 0; blk < 2; blk++)
@@ -318,8 +329,7 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
           if (c < NC) {
             const int g = c / NCm;
             c -= g * NCm;
-            const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-            float* base = dev_arena(G) + G.slot[winds && (c & 2) ? job.four1 : job.four0];
+            exo_gfloat* base = group_arena(plan, grp, c0 + g) + M.slot[winds && (c & 2) ? job.four1 : job.four0];
             const int p = winds ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -428,28 +438,30 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   // latitude 0 of member g, field f, level p at row (g nf + f) P + p
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
-    const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat);
+    rowp[row] = group_arena(plan, grp, g) + M.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
   }
   // staged table row r: mode k of m, or -1 for padding
   const auto row_mode = [=](int r) {
     const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
     return k < K ? k : -1;
   };
-  __syncthreads();  // rowp
   // one memory latency for the northern half of each table,
-  // Ah[r][l] = q(l, w0 + k(r)), and the first chunk's inputs
-  {
-    float va[3][kFwdColTabN], vr[kFwdFoldN][2];
+  // Ah[r][l] = q(l, w0 + k(r)), and the first chunk's inputs; the tables
+  // need no row pointer, so their loads are in flight across the barrier
+  // that publishes rowp
+  float va[3][kFwdColTabN];
 #pragma unroll
-    for (int blk = 0; blk < 3; blk++)
-      if (blk < blocks) {
-        const float* q = dev_ltab(M) + M.tabo[tables[blk]];
-        load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
-          const int k = row_mode(r);
-          return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
-        }, va[blk]);
-      }
+  for (int blk = 0; blk < 3; blk++)
+    if (blk < blocks) {
+      const float* q = dev_ltab(M) + M.tabo[tables[blk]];
+      load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
+        const int k = row_mode(r);
+        return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
+      }, va[blk]);
+    }
+  __syncthreads();  // rowp
+  {
+    float vr[kFwdFoldN][2];
     fold_load(min(chunk, count) * nf * P, nlat, P * nlat, [&](int r) { return rowp[r]; }, vr);
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
@@ -500,8 +512,7 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
         if (c < NC) {
           const int g = c / NCm;
           c -= g * NCm;
-          const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-          float* base = dev_arena(G) + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
+          exo_gfloat* base = group_arena(plan, grp, c0 + g) + M.slot[uv && (c & 2) ? job.dst1 : job.dst0];
           const int p = uv ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
@@ -637,7 +648,7 @@ __device__ __forceinline__ void split_table(float* A, int rows, int stride, int 
 
 template <int P>
 __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
-                               float* As, float* raw, exo_gcfloat** rowp, float* rowsub, float* bscale) {
+                               float* As, float* raw, exo_gcfloat** rowp, float* bscale) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1;
@@ -657,15 +668,15 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
   const int i = lane & 15, kk = lane >> 4;
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
-    const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[f ? job.src1 : job.src0] + p * nrsp);
-    if (fp == 0) rowsub[g] = job.plavor ? G.k[CON_plavor] : 0.0f;
+    rowp[row] = group_arena(plan, grp, g) + M.slot[f ? job.src1 : job.src0] + p * nrsp;
   }
   const auto mode_of = [=](int col) {
     if (col < Ra) return col < Ka ? wa + col : -1;
     const int k = col - Ra;
     return k < Kb ? wb + k : -1;
   };
+  const GroupPlan* gp = &plan;  // the lambdas below copy a pointer, not the plan
+  const bool plavor = job.plavor;
   const auto raw_loader = [&](int c0) {
     const int r0 = c0 * nf * P;
     return [=](int row, int x) {
@@ -673,10 +684,10 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
       const int w = mode_of(x >> 1);
       if (w < 0) return 0.0f;
       const float v = rowp[r0 + row][2 * w + (x & 1)];
-      if (w != 1 || (x & 1) || nf == 1) return v;
+      if (w != 1 || (x & 1) || !plavor) return v;
       // Re mode (0, 1) of a zeta row (field 1) loses the planetary vorticity
       const int r = r0 + row, g = r / (2 * P);
-      return r - g * 2 * P >= P ? v - rowsub[g] : v;
+      return r - g * 2 * P >= P ? v - group_plavor(mems, *gp, grp, g) : v;
     };
   };
   const auto raw_store = [&](int row, int x, float v) { raw[row * kSplitRaw + x] = v; };
@@ -751,8 +762,7 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
           if (c < NC) {
             const int g = c / NCm;
             c -= g * NCm;
-            const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-            float* base = dev_arena(G) + G.slot[winds && (c & 2) ? job.four1 : job.four0];
+            exo_gfloat* base = group_arena(plan, grp, c0 + g) + M.slot[winds && (c & 2) ? job.four1 : job.four0];
             const int p = winds ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -844,8 +854,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
   const int i = lane & 15, kk = lane >> 4;
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
-    const DevMember& G = mems[group_member(plan, grp, g)];
-    rowp[row] = (exo_gcfloat*)(dev_arena(G) + G.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat);
+    rowp[row] = group_arena(plan, grp, g) + M.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
   }
   const auto row_mode = [=](int r) {
     const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
@@ -917,8 +926,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
         if (c < NC) {
           const int g = c / NCm;
           c -= g * NCm;
-          const DevMember& G = mems[group_member(plan, grp, c0 + g)];
-          float* base = dev_arena(G) + G.slot[uv && (c & 2) ? job.dst1 : job.dst0];
+          exo_gfloat* base = group_arena(plan, grp, c0 + g) + M.slot[uv && (c & 2) ? job.dst1 : job.dst0];
           const int p = uv ? c >> 2 : c >> 1, part = c & 1;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
@@ -957,7 +965,6 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   __shared__ float As[2 * kPairAs];
   __shared__ float raw[kInvRows * kPairRawK];
   __shared__ exo_gcfloat* rowp[kInvGroupRows];
-  __shared__ float rowsub[kLegGroup];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
@@ -965,7 +972,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   // rows share cache lines, go to one XCD (xcd_block)
   const int pairs = (M.ntru + 2) / 2;
   const InvJob job = jobs.j[job_of(jobs, tile / pairs)];
-#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, rowsub)
+#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
@@ -975,14 +982,13 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   __shared__ __align__(16) float As[2 * kMaxLat * kSplitAs];
   __shared__ __align__(16) float raw[kSplitInvRows * kSplitRaw];
   __shared__ exo_gcfloat* rowp[kInvGroupRows];
-  __shared__ float rowsub[kLegGroup];
   __shared__ float bscale[2 * kSplitInvRows];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int pairs = (M.ntru + 2) / 2;
   const InvJob job = jobs.j[job_of(jobs, tile / pairs)];
-#define CALL(PP) leg_inv_gsplit<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, rowsub, bscale)
+#define CALL(PP) leg_inv_gsplit<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, bscale)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
