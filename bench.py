@@ -65,6 +65,8 @@ def run_gpu(exo, cfgs, steps, warmup, probes=(), precision=0, graphs=True, tune=
             # state is left as it was
             t_tune = time.perf_counter()
             ms = eng.autotune(trial_steps=8)
+            if lanes:
+                eng.set_step_lanes(lanes)  # a lane count given on the command line stays
             classes = sorted({(c.truncation, c.levels) for c in cfgs})
             tuned = {"groups": {f"T{t} L{l}": eng.transform_group(t, l) for t, l in classes},
                      "step_graphs": eng.step_graphs(), "lanes": eng.step_lanes(),

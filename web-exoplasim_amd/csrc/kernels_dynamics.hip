@@ -441,42 +441,49 @@ extern "C" __global__ __launch_bounds__(64) void k_mkdheat(const DevMember* __re
 // with the same tables; one workgroup per member and level computes it
 // (every other spectral coefficient of zsdef set to 0, as mean-mode
 // leaves it) and the heating.
-// The largest grid plane (T63: 96 x 192) over 1024 threads.
-constexpr int kDekinPts = 18;
+// Rows per wave and longitudes per lane of the largest grid plane (T63:
+// 96 rows of 192 points over 16 waves of 64 lanes).
+constexpr int kDekinRows = 6, kDekinCols = 3;
 
 extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember* __restrict__ mems, LaunchPlan plan) {
   int l;
   const DevMember& M = mems[plan_find(plan, blockIdx.x, &l)];
-  const int NH = M.nhor, nlat = M.nlat, nlon = M.nlon, ncsp = M.ncsp, nfc = M.nfc, nrsp = M.nrsp;
+  const int NH = M.nhor, nlat = M.nlat, nlon = M.nlon, ncsp = M.ncsp, nrsp = M.nrsp;
   float* A = dev_arena(M);
   float* zdekin = A + M.slot[SLOT_zdekin] + l * NH;
   const float* qc = dev_ltab(M) + M.tabo[TAB_legendre_qc];
   const float* qi = dev_ltab(M) + M.tabo[TAB_legendre_qi];
-  const float* fwd = dev_ltab(M) + M.tabo[TAB_fourier_forward];
-  const float* inv = dev_ltab(M) + M.tabo[TAB_fourier_inverse];
+  // The m = 0 column of the forward DFT table is cos(0) / NLON and the m = 0
+  // row of the inverse table cos(0) = 1 at every longitude: one value each.
+  const float fwd0 = dev_ltab(M)[M.tabo[TAB_fourier_forward]];
+  const float inv0 = dev_ltab(M)[M.tabo[TAB_fourier_inverse]];
   const int half = nlat / 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // Wave w takes rows w, w + 16, ...; lane x takes longitudes x, x + 64, ...
+  // Points past the plane load its last row or longitude, unconditionally,
+  // and are neither summed nor stored.
+  const auto lat_of = [&](int r) { return min(wave + 16 * r, nlat - 1); };
+  const auto lon_of = [&](int c) { return min(lane + 64 * c, nlon - 1); };
+  const auto inside = [&](int r, int c) { return wave + 16 * r < nlat && lane + 64 * c < nlon; };
   // Fourier mode 0 of each point's row, weighted by its latitude's qc
   // (mode (0, 0) is even about the equator: a southern row takes its
-  // northern twin's). Each thread takes points h = t + 1024 j and issues
-  // all its loads before it sums; the planet's dissipated energy is summed
-  // in FP64 (thread, then wave butterfly, then the 16 waves in order) and
-  // rounded to FP32 once.
-  float prod[kDekinPts];
+  // northern twin's), every load issued before the sums. The planet's
+  // dissipated energy is summed in FP64 (thread, then wave butterfly, then
+  // the 16 waves in order) and rounded to FP32 once.
+  float prod[kDekinRows][kDekinCols];
 #pragma unroll
-  for (int j = 0; j < kDekinPts; j++) {
-    const int h = threadIdx.x + 1024 * j;
-    float p = 0.0f;
-    if (h < NH) {
-      const int lat = h / nlon, x = h - lat * nlon;
-      const float w = qc[(lat < half ? lat : nlat - 1 - lat) * ncsp];
-      p = w * (zdekin[h] * fwd[x * nfc]);
-    }
-    prod[j] = p;
+  for (int r = 0; r < kDekinRows; r++) {
+    const int lat = lat_of(r);
+    const float w = qc[(lat < half ? lat : nlat - 1 - lat) * ncsp];
+#pragma unroll
+    for (int c = 0; c < kDekinCols; c++) prod[r][c] = w * (zdekin[lat * nlon + lon_of(c)] * fwd0);
   }
   double s = 0.0;
 #pragma unroll
-  for (int j = 0; j < kDekinPts; j++) s += (double)prod[j];
+  for (int r = 0; r < kDekinRows; r++)
+#pragma unroll
+    for (int c = 0; c < kDekinCols; c++)
+      if (inside(r, c)) s += (double)prod[r][c];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   __shared__ double part[16];
@@ -500,27 +507,27 @@ extern "C" __global__ __launch_bounds__(1024) void k_mean_dekin(const DevMember*
   const float* dq = A + M.slot[SLOT_dq] + l * NH;
   const float* dp = A + M.slot[SLOT_dp];
   float* zdtdt = A + M.slot[SLOT_zdtdt] + l * NH;
-  float e[kDekinPts], dqv[kDekinPts], dpv[kDekinPts];
+  float f[kDekinRows], dqv[kDekinRows][kDekinCols], dpv[kDekinRows][kDekinCols];
 #pragma unroll
-  for (int j = 0; j < kDekinPts; j++) {
-    const int h = threadIdx.x + 1024 * j;
-    e[j] = dqv[j] = 0.0f;
-    dpv[j] = 1.0f;
-    if (h < NH) {
-      const int lat = h / nlon, x = h - lat * nlon;
-      e[j] = (qi[lat * ncsp] * S) * inv[x];
-      dqv[j] = dq[h];
-      dpv[j] = dp[h];
+  for (int r = 0; r < kDekinRows; r++) {
+    const int lat = lat_of(r);
+    f[r] = qi[lat * ncsp] * S;
+#pragma unroll
+    for (int c = 0; c < kDekinCols; c++) {
+      dqv[r][c] = dq[lat * nlon + lon_of(c)];
+      dpv[r][c] = dp[lat * nlon + lon_of(c)];
     }
   }
 #pragma unroll
-  for (int j = 0; j < kDekinPts; j++) {
-    const int h = threadIdx.x + 1024 * j;
-    if (h < NH) {
-      zdekin[h] = e[j];
-      const float heat = 1.0f + ADV * dqv[j];
-      const float dtdt = -e[j] * 0.5f / ACPD / heat / dpv[j] * GA / dsig;
-      zdtdt[h] = dtdt / CT / WW;
-    }
-  }
+  for (int r = 0; r < kDekinRows; r++)
+#pragma unroll
+    for (int c = 0; c < kDekinCols; c++)
+      if (inside(r, c)) {
+        const int h = (wave + 16 * r) * nlon + lane + 64 * c;
+        const float e = f[r] * inv0;
+        zdekin[h] = e;
+        const float heat = 1.0f + ADV * dqv[r][c];
+        const float dtdt = -e * 0.5f / ACPD / heat / dpv[r][c] * GA / dsig;
+        zdtdt[h] = dtdt / CT / WW;
+      }
 }

@@ -2101,13 +2101,14 @@ constexpr int kReducePts = 18;
 
 __device__ __forceinline__ void block_sum2(const DevMember& M, const float* a, const float* b, float* out) {
   __shared__ double sa[kReduceThreads], sb[kReduceThreads];
-  // every load issued before the sums, which run in point order
+  // every load issued before the sums, which run in point order (points
+  // past the grid load its last point, unconditionally, and are not summed)
   float va[kReducePts], vb[kReducePts];
 #pragma unroll
   for (int j = 0; j < kReducePts; j++) {
-    const int h = threadIdx.x + kReduceThreads * j;
-    va[j] = h < M.nhor ? a[h] : 0.0f;
-    vb[j] = h < M.nhor ? b[h] : 0.0f;
+    const int h = min((int)threadIdx.x + kReduceThreads * j, M.nhor - 1);
+    va[j] = a[h];
+    vb[j] = b[h];
   }
   double x = 0, y = 0;
 #pragma unroll
@@ -2214,12 +2215,13 @@ extern "C" __global__ __launch_bounds__(256) void k_fixer_rows(const DevMember* 
   float* fixq = dev_arena(M) + M.slot[SLOT_fixq];
   float* fixneg = dev_arena(M) + M.slot[SLOT_fixneg];
   float* fixpos = dev_arena(M) + M.slot[SLOT_fixpos];
+  // loads past the row's end read its last point and are not used
   float neg[kFixerRowPts], pos[kFixerRowPts];
 #pragma unroll
   for (int k = 0; k < kFixerRowPts; k++) {
-    const int j = lane + 64 * k;
-    neg[k] = j < NLON ? fixneg[first + j] : 0.0f;
-    pos[k] = j < NLON ? fixpos[first + j] : 0.0f;
+    const int j = min(lane + 64 * k, NLON - 1);
+    neg[k] = fixneg[first + j];
+    pos[k] = fixpos[first + j];
   }
   double zneg = 0, zpos = 0;
 #pragma unroll
@@ -2240,10 +2242,8 @@ extern "C" __global__ __launch_bounds__(256) void k_fixer_rows(const DevMember* 
 #pragma unroll
   for (int k = 0; k < kFixerRowPts; k++)
 #pragma unroll
-    for (int l = 0; l < kMaxLevels; l++) {
-      const int j = lane + 64 * k;
-      q[k][l] = (j < NLON && l < L) ? fixq[l * NH + first + j] : 0.0f;
-    }
+    for (int l = 0; l < kMaxLevels; l++)
+      q[k][l] = fixq[min(l, L - 1) * NH + first + min(lane + 64 * k, NLON - 1)];
 #pragma unroll
   for (int k = 0; k < kFixerRowPts; k++) {
     const int j = lane + 64 * k;
@@ -2271,10 +2271,7 @@ extern "C" __global__ __launch_bounds__(kReduceThreads) void k_dca_limit(const D
   const float* lim = dev_arena(M) + M.slot[SLOT_dca_limited];
   float flag = 0;
 #pragma unroll
-  for (int j = 0; j < kReducePts; j++) {
-    const int h = threadIdx.x + kReduceThreads * j;
-    if (h < M.nhor) flag = fmaxf(flag, lim[h]);
-  }
+  for (int j = 0; j < kReducePts; j++) flag = fmaxf(flag, lim[min((int)threadIdx.x + kReduceThreads * j, M.nhor - 1)]);
   part[threadIdx.x] = flag;
   __syncthreads();
   for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
