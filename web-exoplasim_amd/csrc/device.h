@@ -109,6 +109,17 @@ __device__ __forceinline__ const float* dev_ltab(const DevMember& M) {
   return (const float*)*static_cast<exo_gcfloat* const*>(static_cast<const void*>(&M.ltab));
 }
 
+// A workgroup barrier that orders LDS accesses only. __syncthreads() is a
+// workgroup-scope fence on every address space, so each wave first waits
+// for all of its global loads and stores in flight; here they stay in
+// flight across the barrier. For workgroups that share data through LDS
+// alone.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 #define SLOTP(M, name) (dev_arena(M) + (M).slot[SLOT_##name])
 #define TABP(M, name) (dev_tab(M) + (M).tabo[TAB_##name])
 #define KON(M, name) ((M).k[CON_##name])

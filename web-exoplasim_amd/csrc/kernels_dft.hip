@@ -310,7 +310,7 @@ __device__ __forceinline__ void inverse_row(const float* source, int stride, boo
       T[lane * kTs + 2 * n] = re[16 * c + n];
       T[lane * kTs + 2 * n + 1] = im[16 * c + n];
     }
-    __syncthreads();
+    lds_barrier();
     // row i's lanes 2i (first half of the row) and 2i + 1 (second half)
     const int h = lane >> 5, x = lane & 31;
 #pragma unroll 8
@@ -318,7 +318,7 @@ __device__ __forceinline__ void inverse_row(const float* source, int stride, boo
       gfloat* G = (gfloat*)uniform_ptr(dstp, i);
       if (G) G[h * M + 32 * c + x] = T[(2 * i + h) * kTs + x];
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
@@ -343,14 +343,14 @@ __device__ __forceinline__ void forward_row(const float* const* srcp, float* des
 #pragma unroll
       for (int i = 0; i < 16; i++) L[(h + i) * kLs + lane] = v[i];
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int j = 0; j < 16; j++) {
       const float2 z = *(const float2*)&L[r * kLs + 4 * j + 2 * p];
       re[slot_of<H>(16 * c + j)] = z.x;
       im[slot_of<H>(16 * c + j)] = z.y;
     }
-    __syncthreads();
+    lds_barrier();
   }
   fft<H, -1>(re, im);
   join_pair<H, -1>(re, im, p);
@@ -410,7 +410,7 @@ __device__ __forceinline__ void dft_inv(const DevMember* __restrict__ mems, cons
     stride = plane_rows;
   }
   if ((threadIdx.x & 1) == 0) dst[threadIdx.x >> 1] = d;
-  __syncthreads();
+  lds_barrier();
   inverse_row<M>(src, stride, derived, dst, trans);
 }
 
@@ -436,7 +436,7 @@ __device__ __forceinline__ void dft_fwd(const DevMember* __restrict__ mems, cons
     stride = rows_of(job);
   }
   if ((threadIdx.x & 1) == 0) src[threadIdx.x >> 1] = s;
-  __syncthreads();
+  lds_barrier();
   forward_row<M>(src, d, stride, stage);
 }
 

@@ -189,15 +189,17 @@ constexpr int kPairTabN = (kMaxLat + kPairTabS - 1) / kPairTabS;
 constexpr int kPairRawS = kGroupThreads / (2 * kPairK);
 constexpr int kPairRawN = (kInvRows + kPairRawS - 1) / kPairRawS;
 
+// Every lane loads, unconditionally: rows past the end (and the lanes past
+// S x C) load the last row, which store_cols then skips, so the loads
+// issue back to back (no branch and wait each) and nothing consumes them
+// before the stores. `load` must accept every row below `rows` (>= 1) and
+// every column below C.
 template <int N, int C, typename Load>
 __device__ __forceinline__ void load_cols(int rows, Load load, float* v) {
   constexpr int S = kGroupThreads / C;
   const int c = threadIdx.x % C, r0 = threadIdx.x / C;
 #pragma unroll
-  for (int j = 0; j < N; j++) {
-    const int r = r0 + S * j;
-    v[j] = (r0 < S && r < rows) ? load(r, c) : 0.0f;
-  }
+  for (int j = 0; j < N; j++) v[j] = load(min(r0 + S * j, rows - 1), c);
 }
 
 template <int N, int C, typename Store>
@@ -213,7 +215,7 @@ __device__ __forceinline__ void store_cols(int rows, Store store, const float* v
 
 template <int P>
 __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
-                        float* As, float* raw, exo_gcfloat** rowp) {
+                        float* As, float* raw, exo_gcfloat** rowp, float* plav) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1;
@@ -234,39 +236,15 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
   const float* q1 = dev_ltab(M) + M.tabo[TAB_legendre_qv];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
-  // where each input row of the group starts (member g, field f, level p
-  // at row (g nf + f) P + p)
-  if (threadIdx.x < count * nf * P) {
-    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
-    rowp[row] = group_arena(plan, grp, g) + M.slot[f ? job.src1 : job.src0] + p * nrsp;
-  }
   // staged column col < RT: mode w of the pair's wavenumber, or padding
   const auto mode_of = [=](int col) {
     if (col < Ra) return col < Ka ? wa + col : -1;
     const int k = col - Ra;
     return k < Kb ? wb + k : -1;
   };
-  // the members' coefficients (zeta without the planetary vorticity in
-  // Re mode (0, 1)), zeros in the padding, for the chunk from member c0
-  const GroupPlan* gp = &plan;  // the lambdas below copy a pointer, not the plan
-  const bool plavor = job.plavor;
-  const auto raw_loader = [&](int c0) {
-    const int r0 = c0 * nf * P;
-    return [=](int row, int x) {
-      if (x >= 2 * RT) return 0.0f;
-      const int w = mode_of(x >> 1);
-      if (w < 0) return 0.0f;
-      const float v = rowp[r0 + row][2 * w + (x & 1)];
-      if (w != 1 || (x & 1) || !plavor) return v;
-      // Re mode (0, 1) of a zeta row (field 1) loses the planetary vorticity
-      const int r = r0 + row, g = r / (2 * P);
-      return r - g * 2 * P >= P ? v - group_plavor(mems, *gp, grp, g) : v;
-    };
-  };
-  const auto raw_store = [&](int row, int x, float v) { raw[row * kPairRawK + x] = v; };
-  // the table columns of the pair (dv2uv: qu, qv) need no row pointer:
-  // their loads are in flight across the barrier that publishes rowp, and
-  // the first chunk's inputs follow; every load is issued before any store
+  // the table columns of the pair (dv2uv: qu, qv) need nothing from the
+  // members: their loads go first and stay in flight while the row
+  // pointers are set up and published
   float va[2][kPairTabN];
 #pragma unroll
   for (int blk = 0; blk < 2; blk++)
@@ -274,10 +252,40 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
       const float* q = blk ? q1 : q0;
       load_cols<kPairTabN, kPairK>(nlat, [&](int lat, int col) {
         const int w = col < RT ? mode_of(col) : -1;
-        return w >= 0 ? q[lat * ncsp + w] : 0.0f;
+        return q[lat * ncsp + max(w, 0)];
       }, va[blk]);
     }
-  __syncthreads();  // rowp
+  // where each input row of the group starts (member g, field f, level p
+  // at row (g nf + f) P + p)
+  if (threadIdx.x < count * nf * P) {
+    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
+    rowp[row] = group_arena(plan, grp, g) + M.slot[f ? job.src1 : job.src0] + p * nrsp;
+  }
+  // each member's planetary vorticity, for its zeta rows' Re mode (0, 1)
+  if (job.plavor && threadIdx.x < count) plav[threadIdx.x] = group_plavor(mems, plan, grp, threadIdx.x);
+  // the members' coefficients for the chunk from member c0 (the padding
+  // loads mode 0; the store zeroes it)
+  const auto raw_loader = [&](int c0) {
+    const int r0 = c0 * nf * P;
+    return [=](int row, int x) { return rowp[r0 + row][2 * max(mode_of(x >> 1), 0) + (x & 1)]; };
+  };
+  // staged as loaded, zeros in the padding, except Re mode (0, 1) (x = 2
+  // of pair 0) of a zeta row (field 1), which loses the member's planetary
+  // vorticity
+  const bool plavor = job.plavor && ma == 0;
+  const auto raw_storer = [&](int c0) {
+    const int r0 = c0 * nf * P;
+    return [=](int row, int x, float v) {
+      if (mode_of(x >> 1) < 0) v = 0.0f;
+      if (plavor && x == 2) {
+        const int r = r0 + row, g = r / (2 * P);
+        if (r - g * 2 * P >= P) v -= plav[g];
+      }
+      raw[row * kPairRawK + x] = v;
+    };
+  };
+  // the first chunk's inputs follow; every load is issued before any store
+  lds_barrier();  // rowp
   {
     float vr[kPairRawN];
     load_cols<kPairRawN, 2 * kPairK>(min(chunk, count) * nf * P, raw_loader(0), vr);
@@ -285,12 +293,14 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
     for (int blk = 0; blk < 2; blk++)
       if (blk < blocks) {
         float* A = As + blk * kPairAs;
-        store_cols<kPairTabN, kPairK>(nlat, [&](int lat, int col, float v) { if (col < RT) A[lat * as + col] = v; },
+        store_cols<kPairTabN, kPairK>(nlat, [&](int lat, int col, float v) {
+          if (col < RT) A[lat * as + col] = mode_of(col) >= 0 ? v : 0.0f;
+        },
                                       va[blk]);
       }
-    store_cols<kPairRawN, 2 * kPairK>(min(chunk, count) * nf * P, raw_store, vr);
+    store_cols<kPairRawN, 2 * kPairK>(min(chunk, count) * nf * P, raw_storer(0), vr);
   }
-  __syncthreads();
+  lds_barrier();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
@@ -341,9 +351,9 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
       }
     }
     if (n1 > 0) {
-      __syncthreads();  // every wave is done with this chunk's inputs
-      store_cols<kPairRawN, 2 * kPairK>(n1 * nf * P, raw_store, vn);
-      __syncthreads();
+      lds_barrier();  // every wave is done with this chunk's inputs
+      store_cols<kPairRawN, 2 * kPairK>(n1 * nf * P, raw_storer(c1), vn);
+      lds_barrier();
     }
   }
 }
@@ -374,16 +384,14 @@ __device__ __forceinline__ void fold_load(int rows, int nlat, int plane_stride, 
   const int c = threadIdx.x % kMaxLat, r0 = threadIdx.x / kMaxLat;
   const int part = c >= half, l = c - part * half;
   const bool ok = r0 < kFwdFoldS && c < 2 * half;
+  // every lane loads (past the rows or the pairs: the last row, latitude
+  // pair (0, nlat - 1)), which fold_store then skips
+  const int lc = ok ? l : 0, pc = ok ? part : 0;
 #pragma unroll
   for (int j = 0; j < kFwdFoldN; j++) {
-    const int r = r0 + kFwdFoldS * j;
-    if (ok && r < rows) {
-      const auto x = rowp(r) + part * plane_stride;
-      v[j][0] = x[l];
-      v[j][1] = x[nlat - 1 - l];
-    } else {
-      v[j][0] = v[j][1] = 0.0f;
-    }
+    const auto x = rowp(min(r0 + kFwdFoldS * j, rows - 1)) + pc * plane_stride;
+    v[j][0] = x[lc];
+    v[j][1] = x[nlat - 1 - lc];
   }
 }
 
@@ -434,12 +442,6 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   const int chunk = max(1, min(count, kFwdRows / (nf * P)));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
-  // where each input row of the group starts: Fourier coefficient m of
-  // latitude 0 of member g, field f, level p at row (g nf + f) P + p
-  if (threadIdx.x < count * nf * P) {
-    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
-    rowp[row] = group_arena(plan, grp, g) + M.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
-  }
   // staged table row r: mode k of m, or -1 for padding
   const auto row_mode = [=](int r) {
     const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
@@ -447,19 +449,23 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
   };
   // one memory latency for the northern half of each table,
   // Ah[r][l] = q(l, w0 + k(r)), and the first chunk's inputs; the tables
-  // need no row pointer, so their loads are in flight across the barrier
-  // that publishes rowp
+  // need nothing from the members, so their loads go first and stay in
+  // flight while the row pointers are set up and published
   float va[3][kFwdColTabN];
 #pragma unroll
   for (int blk = 0; blk < 3; blk++)
     if (blk < blocks) {
       const float* q = dev_ltab(M) + M.tabo[tables[blk]];
-      load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
-        const int k = row_mode(r);
-        return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
-      }, va[blk]);
+      load_cols<kFwdColTabN, 64>(half, [&](int l, int r) { return q[l * ncsp + w0 + max(row_mode(r), 0)]; },
+                                 va[blk]);
     }
-  __syncthreads();  // rowp
+  // where each input row of the group starts: Fourier coefficient m of
+  // latitude 0 of member g, field f, level p at row (g nf + f) P + p
+  if (threadIdx.x < count * nf * P) {
+    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
+    rowp[row] = group_arena(plan, grp, g) + M.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
+  }
+  lds_barrier();  // rowp
   {
     float vr[kFwdFoldN][2];
     fold_load(min(chunk, count) * nf * P, nlat, P * nlat, [&](int r) { return rowp[r]; }, vr);
@@ -467,12 +473,14 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         float* A = Ah + blk * kFwdAs;
-        store_cols<kFwdColTabN, 64>(half, [&](int l, int r, float v) { if (r < MT * 16) A[r * ash + l] = v; },
+        store_cols<kFwdColTabN, 64>(half, [&](int l, int r, float v) {
+          if (r < MT * 16) A[r * ash + l] = row_mode(r) >= 0 ? v : 0.0f;
+        },
                                     va[blk]);
       }
     fold_store(min(chunk, count) * nf * P, nlat, raw, vr);
   }
-  __syncthreads();
+  lds_barrier();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
@@ -523,9 +531,9 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
       }
     }
     if (n1 > 0) {
-      __syncthreads();  // every wave is done with this chunk's inputs
+      lds_barrier();  // every wave is done with this chunk's inputs
       fold_store(n1 * nf * P, nlat, raw, vn);
-      __syncthreads();
+      lds_barrier();
     }
   }
 }
@@ -648,7 +656,7 @@ __device__ __forceinline__ void split_table(float* A, int rows, int stride, int 
 
 template <int P>
 __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const GroupPlan& plan, int grp, const InvJob& job, int pair,
-                               float* As, float* raw, exo_gcfloat** rowp, float* bscale) {
+                               float* As, float* raw, exo_gcfloat** rowp, float* plav, float* bscale) {
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int count = plan.count[grp];
   const int nlat = M.nlat, ntp1 = M.ntru + 1;
@@ -670,31 +678,38 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     rowp[row] = group_arena(plan, grp, g) + M.slot[f ? job.src1 : job.src0] + p * nrsp;
   }
+  // each member's planetary vorticity, for its zeta rows' Re mode (0, 1)
+  if (job.plavor && threadIdx.x < count) plav[threadIdx.x] = group_plavor(mems, plan, grp, threadIdx.x);
   const auto mode_of = [=](int col) {
     if (col < Ra) return col < Ka ? wa + col : -1;
     const int k = col - Ra;
     return k < Kb ? wb + k : -1;
   };
-  const GroupPlan* gp = &plan;  // the lambdas below copy a pointer, not the plan
-  const bool plavor = job.plavor;
+  // the members' coefficients for the chunk from member c0 (the padding
+  // loads mode 0; the store zeroes it)
   const auto raw_loader = [&](int c0) {
     const int r0 = c0 * nf * P;
-    return [=](int row, int x) {
-      if (x >= 2 * RT) return 0.0f;
-      const int w = mode_of(x >> 1);
-      if (w < 0) return 0.0f;
-      const float v = rowp[r0 + row][2 * w + (x & 1)];
-      if (w != 1 || (x & 1) || !plavor) return v;
-      // Re mode (0, 1) of a zeta row (field 1) loses the planetary vorticity
-      const int r = r0 + row, g = r / (2 * P);
-      return r - g * 2 * P >= P ? v - group_plavor(mems, *gp, grp, g) : v;
+    return [=](int row, int x) { return rowp[r0 + row][2 * max(mode_of(x >> 1), 0) + (x & 1)]; };
+  };
+  // staged as loaded, zeros in the padding, except Re mode (0, 1) (x = 2
+  // of pair 0) of a zeta row (field 1), which loses the member's planetary
+  // vorticity
+  const bool plavor = job.plavor && ma == 0;
+  const auto raw_storer = [&](int c0) {
+    const int r0 = c0 * nf * P;
+    return [=](int row, int x, float v) {
+      if (mode_of(x >> 1) < 0) v = 0.0f;
+      if (plavor && x == 2) {
+        const int r = r0 + row, g = r / (2 * P);
+        if (r - g * 2 * P >= P) v -= plav[g];
+      }
+      raw[row * kSplitRaw + x] = v;
     };
   };
-  const auto raw_store = [&](int row, int x, float v) { raw[row * kSplitRaw + x] = v; };
   const auto split_inputs = [&](int rows) {
     split_rows(raw, rows, kSplitRaw, 2 * RT, 2 * kSplitPairK, bscale, [](int v) { return v; });
   };
-  __syncthreads();  // rowp
+  lds_barrier();  // rowp
   {
     float va[2][kSplitTabN], vr[kSplitRawN];
 #pragma unroll
@@ -703,7 +718,7 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
         const float* q = dev_ltab(M) + M.tabo[tabs[blk]];
         load_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col) {
           const int w = col < RT ? mode_of(col) : -1;
-          return w >= 0 ? q[lat * ncsp + w] : 0.0f;
+          return q[lat * ncsp + max(w, 0)];
         }, va[blk]);
       }
     load_cols<kSplitRawN, 2 * kSplitPairK>(min(chunk, count) * nf * P, raw_loader(0), vr);
@@ -711,16 +726,18 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
     for (int blk = 0; blk < 2; blk++)
       if (blk < blocks) {
         float* A = As + blk * kMaxLat * kSplitAs;
-        store_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col, float v) { A[lat * kSplitAs + col] = v; },
+        store_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col, float v) {
+          A[lat * kSplitAs + col] = (col < RT && mode_of(col) >= 0) ? v : 0.0f;
+        },
                                             va[blk]);
       }
-    store_cols<kSplitRawN, 2 * kSplitPairK>(min(chunk, count) * nf * P, raw_store, vr);
+    store_cols<kSplitRawN, 2 * kSplitPairK>(min(chunk, count) * nf * P, raw_storer(0), vr);
   }
-  __syncthreads();
+  lds_barrier();
   for (int blk = 0; blk < blocks; blk++)
     split_table(As + blk * kMaxLat * kSplitAs, nlat, kSplitAs, RT, M.lscale[tabs[blk] - TAB_legendre_qc]);
   split_inputs(min(chunk, count) * nf * P);
-  __syncthreads();
+  lds_barrier();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
@@ -774,11 +791,11 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
       }
     }
     if (n1 > 0) {
-      __syncthreads();  // every wave is done with this chunk's inputs
-      store_cols<kSplitRawN, 2 * kSplitPairK>(n1 * nf * P, raw_store, vn);
-      __syncthreads();
+      lds_barrier();  // every wave is done with this chunk's inputs
+      store_cols<kSplitRawN, 2 * kSplitPairK>(n1 * nf * P, raw_storer(c1), vn);
+      lds_barrier();
       split_inputs(n1 * nf * P);
-      __syncthreads();
+      lds_barrier();
     }
   }
 }
@@ -860,7 +877,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
     const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
     return k < K ? k : -1;
   };
-  __syncthreads();  // rowp
+  lds_barrier();  // rowp
   {
     float va[3][kFwdColTabN], vr[kFwdFoldN][2];
 #pragma unroll
@@ -869,7 +886,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
         const float* q = dev_ltab(M) + M.tabo[tables[blk]];
         load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
           const int k = row_mode(r);
-          return k >= 0 ? q[l * ncsp + w0 + k] : 0.0f;
+          return q[l * ncsp + w0 + max(k, 0)];
         }, va[blk]);
       }
     fold_load(min(chunk, count) * nf * P, nlat, P * nlat, [&](int r) { return rowp[r]; }, vr);
@@ -877,16 +894,18 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
         float* A = Ah + blk * kMaxK * kSplitFwdAs;
-        store_cols<kFwdColTabN, 64>(half, [&](int l, int r, float v) { if (r < MT * 16) A[r * kSplitFwdAs + l] = v; },
+        store_cols<kFwdColTabN, 64>(half, [&](int l, int r, float v) {
+          if (r < MT * 16) A[r * kSplitFwdAs + l] = row_mode(r) >= 0 ? v : 0.0f;
+        },
                                     va[blk]);
       }
     fold_store(min(chunk, count) * nf * P, nlat, raw, vr);
   }
-  __syncthreads();
+  lds_barrier();
   for (int blk = 0; blk < blocks; blk++)
     split_table(Ah + blk * kMaxK * kSplitFwdAs, MT * 16, kSplitFwdAs, half, M.lscale[tables[blk] - TAB_legendre_qc]);
   split_fold_rows(raw, min(chunk, count) * nf * P, half, bscale);
-  __syncthreads();
+  lds_barrier();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
@@ -937,11 +956,11 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
       }
     }
     if (n1 > 0) {
-      __syncthreads();  // every wave is done with this chunk's inputs
+      lds_barrier();  // every wave is done with this chunk's inputs
       fold_store(n1 * nf * P, nlat, raw, vn);
-      __syncthreads();
+      lds_barrier();
       split_fold_rows(raw, n1 * nf * P, half, bscale);
-      __syncthreads();
+      lds_barrier();
     }
   }
 }
@@ -965,6 +984,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   __shared__ float As[2 * kPairAs];
   __shared__ float raw[kInvRows * kPairRawK];
   __shared__ exo_gcfloat* rowp[kInvGroupRows];
+  __shared__ float plav[kLegGroup];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
@@ -972,7 +992,7 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   // rows share cache lines, go to one XCD (xcd_block)
   const int pairs = (M.ntru + 2) / 2;
   const InvJob job = jobs.j[job_of(jobs, tile / pairs)];
-#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp)
+#define CALL(PP) leg_inv<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, plav)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
@@ -982,13 +1002,14 @@ extern "C" __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_wav
   __shared__ __align__(16) float As[2 * kMaxLat * kSplitAs];
   __shared__ __align__(16) float raw[kSplitInvRows * kSplitRaw];
   __shared__ exo_gcfloat* rowp[kInvGroupRows];
+  __shared__ float plav[kLegGroup];
   __shared__ float bscale[2 * kSplitInvRows];
   int tile;
   const int grp = group_find(plan, xcd_block(blockIdx.x, gridDim.x), &tile);
   const DevMember& M = mems[group_member(plan, grp, 0)];
   const int pairs = (M.ntru + 2) / 2;
   const InvJob job = jobs.j[job_of(jobs, tile / pairs)];
-#define CALL(PP) leg_inv_gsplit<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, bscale)
+#define CALL(PP) leg_inv_gsplit<PP>(mems, plan, grp, job, tile % pairs, As, raw, rowp, plav, bscale)
   EXO_P_DISPATCH(job.per_level ? M.nlev : 1, CALL);
 #undef CALL
 }
