@@ -30,11 +30,12 @@
  * (asymptotes, inverse) and the storm's outflow ratio (T_s over the
  * environment's temperature at the level of neutral buoyancy).
  *
- * Parity unpinned: two coefficients no property fixes, the slope of the
- * CO2 absorptivity's logarithmic branch (0.0546 per decade of path above
- * 1; its value at the branch point is pinned) and the exponent of the
- * ozone absorption's ultraviolet term (0.805). They are held only to the
- * GPU kernels written from the same reading of the reference WGSL.
+ * The two coefficients no property fixes, the slope of the CO2
+ * absorptivity's logarithmic branch (0.0546 per decade of path above 1)
+ * and the exponent of the ozone absorption's ultraviolet term (0.805),
+ * are pinned to the literals read out of lwr.wgsl:122 and swr.wgsl:36
+ * (tests/golden/make_fit_constants.py): the oracle's rise per decade and
+ * its large-path log-log slope must recover them.
  */
 #define EXO_ORACLE_INTERNAL
 #include <tgmath.h>

@@ -38,6 +38,12 @@ wrong branch point, a swapped argument, a dropped term) breaks:
   level of neutral buoyancy, and the intensity is Bister and Emanuel's
   sqrt(Ck/Cd (T_s/T_o) (CAPE* - CAPE)).
 
+* the two coefficients no property fixes, the CO2 absorptivity's slope per
+  decade above its branch point and the ozone absorption's ultraviolet
+  exponent: the oracle's rise per decade and its large-path log-log slope
+  recover the literals read out of the reference's WGSL text
+  (tests/golden/make_fit_constants.py, lwr.wgsl:122, swr.wgsl:36).
+
 FP64 oracle: the identities hold to rounding, continuity to the size of a
 step of the dense grid.
 """
@@ -345,3 +351,35 @@ def test_storm_outflow_ratio_and_intensity(member):
         assert vmax == pytest.approx(math.sqrt(run.constant("ckcd") * ratio * (capes - cape)), rel=1e-12)
         checked += 1
     assert checked >= 5, checked
+
+
+def _fit_constants():
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "radiation_fit_constants.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def test_co2_log_branch_slope_is_the_reference_literal(member):
+    """Above the branch point the CO2 absorptivity is C log10(u) + offset:
+    its rise over each decade of path is C, which must equal the literal
+    read from lwr.wgsl:122 (tests/golden/make_fit_constants.py)."""
+    run, _, _ = _run(member)
+    c = _fit_constants()["lwr_co2_log_slope"]["value"]
+    a = run.radiation_fit("lwr_co2", np.array([2.0, 20.0, 200.0, 2000.0]))
+    assert np.allclose(np.diff(a), c, rtol=0, atol=1e-12), (np.diff(a), c)
+
+
+def test_ozone_uv_exponent_is_the_reference_literal(member):
+    """For large ozone paths the ultraviolet term 1.082 x / (1 + 138.6 x)^E
+    dominates the ozone absorption (the other two terms fall as 1/x and
+    1/x^2), so the absorption grows as x^(1 - E): the log-log slope between
+    two large paths recovers E, which must equal the literal read from
+    swr.wgsl:36 (tests/golden/make_fit_constants.py)."""
+    run, _, _ = _run(member)
+    e = _fit_constants()["swr_o3_uv_exponent"]["value"]
+    a = run.radiation_fit("swr_o3", np.array([1e12, 1e13]))
+    measured = 1.0 - math.log10(a[1] / a[0])
+    assert abs(measured - e) < 1e-7, (measured, e)
