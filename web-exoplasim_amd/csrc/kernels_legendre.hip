@@ -34,6 +34,7 @@
 namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
 
 constexpr int kMaxLat = 96, kMaxK = 64, kMaxLev = 10;
 // output tiles per wave: inverse dv2uv at T63 L10 is 6 x 3 tiles
@@ -371,8 +372,6 @@ __device__ void leg_inv(const DevMember* mems, const GroupPlan& plan, int grp, c
 // per pass); inputs rows x (2 parts x 48 latitude pairs) (5 rows per pass),
 // each thread loading a latitude and its mirror
 constexpr int kFwdColTabN = (kMaxLat / 2 + kGroupThreads / 64 - 1) / (kGroupThreads / 64);
-// (the split form stages unfolded rows: 2 NLAT <= 192 columns, 2 rows a pass)
-constexpr int kFwdColRawN = (kFwdRows + kGroupThreads / (2 * kMaxLat) - 1) / (kGroupThreads / (2 * kMaxLat));
 constexpr int kFwdFoldS = kGroupThreads / kMaxLat;
 constexpr int kFwdFoldN = (kFwdRows + kFwdFoldS - 1) / kFwdFoldS;
 
@@ -549,23 +548,46 @@ __device__ void leg_fwd(const DevMember* mems, const GroupPlan& plan, int grp, c
 // [2^12, 2^13), well inside FP16's range; the sums are unscaled exactly.
 //
 // The workgroups are those of the FP32 form (member groups sharing the
-// tables; inverse per wavenumber pair, forward per wavenumber): the table
-// columns and each chunk's inputs are staged in FP32 as there, then each
-// staged row is split in place by one wave into groups of 4 values,
-// 4 hi halves then 4 lo halves, so a lane's MFMA operands (4 consecutive
+// tables; inverse per wavenumber pair, forward per wavenumber). Operands
+// are split as they are staged: table values scaled and stored split,
+// input rows loaded one wave per row so that the row's scale is one wave
+// reduction, then stored split. Each group of 4 values is stored as 4 hi
+// halves then 4 lo halves, so a lane's MFMA operands (4 consecutive
 // reduction values, hi and lo) are one 16-byte LDS read. A wavenumber's
 // reduction runs are padded with zeros to multiples of 16, the MFMA depth.
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
-// Power of two putting a largest magnitude in [2^12, 2^13); 1 for zero.
-__device__ __forceinline__ float pow2_scale(float maxabs) {
-  if (!(maxabs > 0.0f)) return 1.0f;
-  int e;
-  frexpf(maxabs, &e);
-  e = e < -100 ? -100 : e;
-  return ldexpf(1.0f, 13 - e);
+// Magnitudes are reduced as bit patterns: for non-negative floats the
+// integer order is the numeric order, and integer max needs no NaN
+// canonicalisation.
+__device__ __forceinline__ int abs_bits(float x) { return __float_as_int(x) & 0x7fffffff; }
+
+// Power of two putting a largest magnitude (as bits) in [2^12, 2^13): with
+// biased exponent E, x in [2^(E-127), 2^(E-126)) takes 2^(139-E), E held
+// to >= 26 (tiny rows get 2^113); 1 for zero.
+__device__ __forceinline__ float pow2_scale_bits(int b) {
+  if (b == 0) return 1.0f;
+  const int e = max(b >> 23, 26);
+  return __int_as_float((266 - e) << 23);
+}
+
+// Largest of b over the 64 lanes, in every lane: DPP within quads, half
+// rows and rows (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror), then the four rows' values read out (v_readlane). No LDS
+// round trip, unlike a shuffle.
+template <int Ctrl>
+__device__ __forceinline__ int dpp_step_max(int b) {
+  return max(b, __builtin_amdgcn_mov_dpp(b, Ctrl, 0xF, 0xF, true));
+}
+__device__ __forceinline__ int wave_max_bits(int b) {
+  b = dpp_step_max<0xB1>(b);
+  b = dpp_step_max<0x4E>(b);
+  b = dpp_step_max<0x141>(b);
+  b = dpp_step_max<0x140>(b);
+  return max(max(__builtin_amdgcn_readlane(b, 0), __builtin_amdgcn_readlane(b, 16)),
+             max(__builtin_amdgcn_readlane(b, 32), __builtin_amdgcn_readlane(b, 48)));
 }
 
 // x (already scaled) as hi and lo halves at value index v of a split row.
@@ -601,55 +623,63 @@ constexpr int kSplitRawN =
     (kSplitInvRows + kGroupThreads / (2 * kSplitPairK) - 1) / (kGroupThreads / (2 * kSplitPairK));
 static_assert(2 * kSplitPairK <= kSplitRaw && kSplitPairK <= kSplitAs, "split rows");
 
-// One wave per staged input row (rows r = wave, wave + 8, ...): the row's
-// `width` FP32 values x = 2 v + part (at stride 1) become, in place, its
-// per-part scale in scale[2 r + part] and split values at halves
-// part * part_halves + group layout of v (vmap(v) reorders v, if asked).
-template <typename VMap>
-__device__ __forceinline__ void split_rows(float* raw, int rows, int stride, int width, int part_halves,
-                                           float* scale, VMap vmap) {
+// Split staging: the split is done at the LDS store, so no pass re-reads
+// the staged FP32 values. Table values are scaled by the table's scale and
+// stored split where they land. Input rows are loaded one wave per row
+// (rows wave, wave + 8, ...): each lane holds columns lane and lane + 64
+// of its row, both parts, so the row's largest magnitude per part is one
+// wave reduction, and
+// the values are stored split with that scale. Every load is unconditional
+// (rows past the end load the last row, padding loads a valid element that
+// the store zeroes), so a chunk's loads are all in flight before any use.
+constexpr int kSplitInvRowsPerWave = (kSplitInvRows + kGroupWaves - 1) / kGroupWaves;  // 3
+static_assert(kSplitPairK <= 128, "two columns a lane");
+constexpr int kSplitFwdRowsPerWave = (kFwdRows + kGroupWaves - 1) / kGroupWaves;       // 4
+
+// Lane values: columns col = lane + 64 j (j < 2), both parts, v[jr][2 j +
+// part]; `load(row, col)` gives both parts of a column.
+template <typename Load>
+__device__ __forceinline__ void inv_split_load(int rows, Load load, float (*v)[4]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int r = wave; r < rows; r += kGroupWaves) {
-    float* row = raw + r * stride;
-    float v[3];
-    float mx = 0.0f;
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const int x = lane + 64 * j;
-      v[j] = x < width ? row[x] : 0.0f;
-      mx = fmaxf(mx, fabsf(v[j]));
-    }
-    // lanes of one parity hold one part
+  for (int jr = 0; jr < kSplitInvRowsPerWave; jr++) {
+    const int r = min(wave + kGroupWaves * jr, rows - 1);
 #pragma unroll
-    for (int o = 32; o > 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    const float sc = pow2_scale(mx);
-    if (lane < 2) scale[2 * r + lane] = sc;
-    _Float16* h = reinterpret_cast<_Float16*>(row);
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const int x = lane + 64 * j;
-      if (x < width) put_split(h + (x & 1) * part_halves, vmap(x >> 1), v[j] * sc);
+    for (int j = 0; j < 2; j++) {
+      const f2 x = load(r, lane + 64 * j);
+      v[jr][2 * j] = x[0];
+      v[jr][2 * j + 1] = x[1];
     }
   }
 }
 
-// One wave per table row (rows r = wave, wave + 8, ...): `width` FP32
-// values times the table scale, split in place.
-__device__ __forceinline__ void split_table(float* A, int rows, int stride, int width, float sa) {
+// Row r's columns col < width (`fix` giving the staged value of value x =
+// 2 col + part) as split halves part * 2 kSplitPairK + group layout of col,
+// and the per-part scale in scale[2 r + part].
+template <typename Fix>
+__device__ __forceinline__ void inv_split_store(int rows, int width, Fix fix, const float (*v)[4], float* raw,
+                                                float* scale) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int r = wave; r < rows; r += kGroupWaves) {
-    float* row = A + r * stride;
-    float v[2];
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int x = lane + 64 * j;
-      v[j] = x < width ? row[x] * sa : 0.0f;
-    }
-    _Float16* h = reinterpret_cast<_Float16*>(row);
+  for (int jr = 0; jr < kSplitInvRowsPerWave; jr++) {
+    const int r = wave + kGroupWaves * jr;
+    if (r < rows) {
+      float x[4];
+      int mx[2] = {0, 0};
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int x = lane + 64 * j;
-      if (x < width) put_split(h, x, v[j]);
+      for (int k = 0; k < 4; k++) {
+        const int c = lane + 64 * (k >> 1), part = k & 1;
+        x[k] = c < width ? fix(r, 2 * c + part, v[jr][k]) : 0.0f;
+        mx[part] = max(mx[part], abs_bits(x[k]));
+      }
+      const float sc[2] = {pow2_scale_bits(wave_max_bits(mx[0])), pow2_scale_bits(wave_max_bits(mx[1]))};
+      if (lane < 2) scale[2 * r + lane] = lane ? sc[1] : sc[0];
+      _Float16* h = reinterpret_cast<_Float16*>(raw + r * kSplitRaw);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int c = lane + 64 * (k >> 1), part = k & 1;
+        if (c < width) put_split(h + part * (2 * kSplitPairK), c, x[k] * sc[part]);
+      }
     }
   }
 }
@@ -674,28 +704,42 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
   const int tabs[2] = {winds ? (int)TAB_legendre_qu : job.table, (int)TAB_legendre_qv};
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
+  const auto mode_of = [=](int col) {
+    if (col < Ra) return col < Ka ? wa + col : -1;
+    const int k = col - Ra;
+    return k < Kb ? wb + k : -1;
+  };
+  // the table columns of the pair need nothing from the members: their
+  // loads go first and stay in flight while the row pointers are set up
+  float va[2][kSplitTabN];
+#pragma unroll
+  for (int blk = 0; blk < 2; blk++)
+    if (blk < blocks) {
+      const float* q = dev_ltab(M) + M.tabo[tabs[blk]];
+      load_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col) {
+        const int w = col < RT ? mode_of(col) : -1;
+        return q[lat * ncsp + max(w, 0)];
+      }, va[blk]);
+    }
   if (threadIdx.x < count * nf * P) {
     const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
     rowp[row] = group_arena(plan, grp, g) + M.slot[f ? job.src1 : job.src0] + p * nrsp;
   }
   // each member's planetary vorticity, for its zeta rows' Re mode (0, 1)
   if (job.plavor && threadIdx.x < count) plav[threadIdx.x] = group_plavor(mems, plan, grp, threadIdx.x);
-  const auto mode_of = [=](int col) {
-    if (col < Ra) return col < Ka ? wa + col : -1;
-    const int k = col - Ra;
-    return k < Kb ? wb + k : -1;
-  };
-  // the members' coefficients for the chunk from member c0 (the padding
-  // loads mode 0; the store zeroes it)
+  // the members' coefficients for the chunk from member c0, value x = 2 col
+  // + part of a row (columns past the pair load mode 0; the store zeroes it)
   const auto raw_loader = [&](int c0) {
     const int r0 = c0 * nf * P;
-    return [=](int row, int x) { return rowp[r0 + row][2 * max(mode_of(x >> 1), 0) + (x & 1)]; };
+    return [=](int row, int col) {
+      return *reinterpret_cast<const __attribute__((address_space(1))) f2*>(rowp[r0 + row] + 2 * max(mode_of(col), 0));
+    };
   };
   // staged as loaded, zeros in the padding, except Re mode (0, 1) (x = 2
   // of pair 0) of a zeta row (field 1), which loses the member's planetary
   // vorticity
   const bool plavor = job.plavor && ma == 0;
-  const auto raw_storer = [&](int c0) {
+  const auto raw_fix = [&](int c0) {
     const int r0 = c0 * nf * P;
     return [=](int row, int x, float v) {
       if (mode_of(x >> 1) < 0) v = 0.0f;
@@ -703,47 +747,33 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
         const int r = r0 + row, g = r / (2 * P);
         if (r - g * 2 * P >= P) v -= plav[g];
       }
-      raw[row * kSplitRaw + x] = v;
+      return v;
     };
   };
-  const auto split_inputs = [&](int rows) {
-    split_rows(raw, rows, kSplitRaw, 2 * RT, 2 * kSplitPairK, bscale, [](int v) { return v; });
-  };
-  lds_barrier();  // rowp
+  lds_barrier();  // rowp, plav
   {
-    float va[2][kSplitTabN], vr[kSplitRawN];
+    float vr[kSplitInvRowsPerWave][4];
+    const int rows = min(chunk, count) * nf * P;
+    inv_split_load(rows, raw_loader(0), vr);
 #pragma unroll
     for (int blk = 0; blk < 2; blk++)
       if (blk < blocks) {
-        const float* q = dev_ltab(M) + M.tabo[tabs[blk]];
-        load_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col) {
-          const int w = col < RT ? mode_of(col) : -1;
-          return q[lat * ncsp + max(w, 0)];
-        }, va[blk]);
-      }
-    load_cols<kSplitRawN, 2 * kSplitPairK>(min(chunk, count) * nf * P, raw_loader(0), vr);
-#pragma unroll
-    for (int blk = 0; blk < 2; blk++)
-      if (blk < blocks) {
-        float* A = As + blk * kMaxLat * kSplitAs;
+        _Float16* A = reinterpret_cast<_Float16*>(As + blk * kMaxLat * kSplitAs);
+        const float sa = M.lscale[tabs[blk] - TAB_legendre_qc];
         store_cols<kSplitTabN, kSplitPairK>(nlat, [&](int lat, int col, float v) {
-          A[lat * kSplitAs + col] = (col < RT && mode_of(col) >= 0) ? v : 0.0f;
+          put_split(A + lat * (2 * kSplitAs), col, (col < RT && mode_of(col) >= 0) ? v * sa : 0.0f);
         },
                                             va[blk]);
       }
-    store_cols<kSplitRawN, 2 * kSplitPairK>(min(chunk, count) * nf * P, raw_storer(0), vr);
+    inv_split_store(rows, RT, raw_fix(0), vr, raw, bscale);
   }
-  lds_barrier();
-  for (int blk = 0; blk < blocks; blk++)
-    split_table(As + blk * kMaxLat * kSplitAs, nlat, kSplitAs, RT, M.lscale[tabs[blk] - TAB_legendre_qc]);
-  split_inputs(min(chunk, count) * nf * P);
   lds_barrier();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
     const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
-    float vn[kSplitRawN];
-    if (n1 > 0) load_cols<kSplitRawN, 2 * kSplitPairK>(n1 * nf * P, raw_loader(c1), vn);
+    float vn[kSplitInvRowsPerWave][4];
+    if (n1 > 0) inv_split_load(n1 * nf * P, raw_loader(c1), vn);
     const _Float16* rawh = reinterpret_cast<const _Float16*>(raw);
     for (int h = 0; h <= two; h++) {
       const int m = h ? mb : ma, col0 = h ? Ra : 0, R = h ? Rb : Ra;
@@ -792,9 +822,7 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
     }
     if (n1 > 0) {
       lds_barrier();  // every wave is done with this chunk's inputs
-      store_cols<kSplitRawN, 2 * kSplitPairK>(n1 * nf * P, raw_storer(c1), vn);
-      lds_barrier();
-      split_inputs(n1 * nf * P);
+      inv_split_store(n1 * nf * P, RT, raw_fix(c1), vn, raw, bscale);
       lds_barrier();
     }
   }
@@ -812,31 +840,48 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
 constexpr int kSplitFwdAs = 68;
 static_assert(kMaxLat / 2 <= kSplitFwdAs && 2 * kMaxLat <= kRawLat, "split forward rows");
 
-// One wave per (row, E or O) run pair: scale[4 row + 2 eo + part] and the
-// split values of both parts, in place.
-__device__ __forceinline__ void split_fold_rows(float* raw, int rows, int half, float* scale) {
+// One wave per staged row r (rows wave, wave + 8, ...): lane l < half loads
+// latitude l and its mirror nlat - 1 - l of both parts (part p at p
+// plane_stride), clamped as inv_split_load is.
+template <typename Rows>
+__device__ __forceinline__ void fwd_split_load(int rows, int nlat, int plane_stride, Rows rowp, float (*v)[4]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int rr = wave; rr < 2 * rows; rr += kGroupWaves) {
-    const int r = rr >> 1, eo = rr & 1;
-    float* row = raw + r * kRawLat + eo * kMaxLat;
-    float v[2];
-    float mx = 0.0f;
+  const int l = lane < (nlat >> 1) ? lane : 0;
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int x = lane + 64 * j;
-      v[j] = x < 2 * half ? row[x] : 0.0f;
-      mx = fmaxf(mx, fabsf(v[j]));
-    }
-    // lanes of one parity hold one part
+  for (int jr = 0; jr < kSplitFwdRowsPerWave; jr++) {
+    const auto x = rowp(min(wave + kGroupWaves * jr, rows - 1));
+    v[jr][0] = x[l];
+    v[jr][1] = x[nlat - 1 - l];
+    v[jr][2] = x[plane_stride + l];
+    v[jr][3] = x[plane_stride + nlat - 1 - l];
+  }
+}
+
+// Row r's folded runs E = north + south and O = north - south of each part,
+// each with its own scale scale[4 r + 2 eo + part], stored split: run (eo,
+// part) at halves eo 2 kMaxLat + part kMaxLat of the row.
+__device__ __forceinline__ void fwd_split_store(int rows, int nlat, const float (*v)[4], float* raw, float* scale) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool ok = lane < (nlat >> 1);
 #pragma unroll
-    for (int o = 32; o > 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    const float sc = pow2_scale(mx);
-    if (lane < 2) scale[4 * r + 2 * eo + lane] = sc;
-    _Float16* h = reinterpret_cast<_Float16*>(row);
+  for (int jr = 0; jr < kSplitFwdRowsPerWave; jr++) {
+    const int r = wave + kGroupWaves * jr;
+    if (r < rows) {
+      // k = 2 eo + part
+      float x[4];
+      x[0] = ok ? v[jr][0] + v[jr][1] : 0.0f;
+      x[1] = ok ? v[jr][2] + v[jr][3] : 0.0f;
+      x[2] = ok ? v[jr][0] - v[jr][1] : 0.0f;
+      x[3] = ok ? v[jr][2] - v[jr][3] : 0.0f;
+      float sc[4];
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int x = lane + 64 * j;
-      if (x < 2 * half) put_split(h + (x & 1) * kMaxLat, x >> 1, v[j] * sc);
+      for (int k = 0; k < 4; k++) sc[k] = pow2_scale_bits(wave_max_bits(abs_bits(x[k])));
+      if (lane < 4) scale[4 * r + lane] = lane == 0 ? sc[0] : lane == 1 ? sc[1] : lane == 2 ? sc[2] : sc[3];
+      _Float16* h = reinterpret_cast<_Float16*>(raw + r * kRawLat);
+      if (ok) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) put_split(h + (k >> 1) * (2 * kMaxLat) + (k & 1) * kMaxLat, lane, x[k] * sc[k]);
+      }
     }
   }
 }
@@ -869,49 +914,50 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
   const int chunk = max(1, min(count, kFwdRows / (nf * P)));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kk = lane >> 4;
-  if (threadIdx.x < count * nf * P) {
-    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
-    rowp[row] = group_arena(plan, grp, g) + M.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
-  }
   const auto row_mode = [=](int r) {
     const int k = r < re16 ? 2 * r : 2 * (r - re16) + 1;
     return k < K ? k : -1;
   };
+  // the tables need nothing from the members: their loads go first and stay
+  // in flight while the row pointers are set up
+  float va[3][kFwdColTabN];
+#pragma unroll
+  for (int blk = 0; blk < 3; blk++)
+    if (blk < blocks) {
+      const float* q = dev_ltab(M) + M.tabo[tables[blk]];
+      load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
+        const int k = row_mode(r);
+        return q[l * ncsp + w0 + max(k, 0)];
+      }, va[blk]);
+    }
+  if (threadIdx.x < count * nf * P) {
+    const int row = threadIdx.x, g = row / (nf * P), fp = row - g * nf * P, f = fp / P, p = fp - f * P;
+    rowp[row] = group_arena(plan, grp, g) + M.slot[fields[f]] + 2 * m * (P * nlat) + p * nlat;
+  }
   lds_barrier();  // rowp
   {
-    float va[3][kFwdColTabN], vr[kFwdFoldN][2];
+    float vr[kSplitFwdRowsPerWave][4];
+    const int rows = min(chunk, count) * nf * P;
+    fwd_split_load(rows, nlat, P * nlat, [&](int r) { return rowp[r]; }, vr);
 #pragma unroll
     for (int blk = 0; blk < 3; blk++)
       if (blk < blocks) {
-        const float* q = dev_ltab(M) + M.tabo[tables[blk]];
-        load_cols<kFwdColTabN, 64>(half, [&](int l, int r) {
-          const int k = row_mode(r);
-          return q[l * ncsp + w0 + max(k, 0)];
-        }, va[blk]);
-      }
-    fold_load(min(chunk, count) * nf * P, nlat, P * nlat, [&](int r) { return rowp[r]; }, vr);
-#pragma unroll
-    for (int blk = 0; blk < 3; blk++)
-      if (blk < blocks) {
-        float* A = Ah + blk * kMaxK * kSplitFwdAs;
+        _Float16* A = reinterpret_cast<_Float16*>(Ah + blk * kMaxK * kSplitFwdAs);
+        const float sa = M.lscale[tables[blk] - TAB_legendre_qc];
         store_cols<kFwdColTabN, 64>(half, [&](int l, int r, float v) {
-          if (r < MT * 16) A[r * kSplitFwdAs + l] = row_mode(r) >= 0 ? v : 0.0f;
+          if (r < MT * 16) put_split(A + r * (2 * kSplitFwdAs), l, row_mode(r) >= 0 ? v * sa : 0.0f);
         },
                                     va[blk]);
       }
-    fold_store(min(chunk, count) * nf * P, nlat, raw, vr);
+    fwd_split_store(rows, nlat, vr, raw, bscale);
   }
-  lds_barrier();
-  for (int blk = 0; blk < blocks; blk++)
-    split_table(Ah + blk * kMaxK * kSplitFwdAs, MT * 16, kSplitFwdAs, half, M.lscale[tables[blk] - TAB_legendre_qc]);
-  split_fold_rows(raw, min(chunk, count) * nf * P, half, bscale);
   lds_barrier();
   for (int c0 = 0; c0 < count; c0 += chunk) {
     const int n = min(chunk, count - c0);
     const int NC = n * NCm, NT = (NC + 15) / 16;
     const int c1 = c0 + chunk, n1 = c1 < count ? min(chunk, count - c1) : 0;
-    float vn[kFwdFoldN][2];
-    if (n1 > 0) fold_load(n1 * nf * P, nlat, P * nlat, [&](int r) { return rowp[c1 * nf * P + r]; }, vn);
+    float vn[kSplitFwdRowsPerWave][4];
+    if (n1 > 0) fwd_split_load(n1 * nf * P, nlat, P * nlat, [&](int r) { return rowp[c1 * nf * P + r]; }, vn);
     const _Float16* rawh = reinterpret_cast<const _Float16*>(raw);
     f4 acc[kFwdGTiles];
 #pragma unroll
@@ -957,9 +1003,7 @@ __device__ __forceinline__ void leg_fwd_gsplit(const DevMember* mems, const Grou
     }
     if (n1 > 0) {
       lds_barrier();  // every wave is done with this chunk's inputs
-      fold_store(n1 * nf * P, nlat, raw, vn);
-      lds_barrier();
-      split_fold_rows(raw, n1 * nf * P, half, bscale);
+      fwd_split_store(n1 * nf * P, nlat, vn, raw, bscale);
       lds_barrier();
     }
   }
