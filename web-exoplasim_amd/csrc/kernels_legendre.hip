@@ -590,6 +590,21 @@ __device__ __forceinline__ int wave_max_bits(int b) {
              max(__builtin_amdgcn_readlane(b, 32), __builtin_amdgcn_readlane(b, 48)));
 }
 
+// Largest of b over each group of `lpr` lanes (16, 32 or 64, aligned), in
+// every lane of the group: the DPP steps leave each 16-lane row's largest
+// in its lanes, and the rows' values combine by v_readlane.
+__device__ __forceinline__ int rows_max_bits(int b, int lpr) {
+  b = dpp_step_max<0xB1>(b);
+  b = dpp_step_max<0x4E>(b);
+  b = dpp_step_max<0x141>(b);
+  b = dpp_step_max<0x140>(b);
+  if (lpr == 16) return b;
+  const int r0 = __builtin_amdgcn_readlane(b, 0), r1 = __builtin_amdgcn_readlane(b, 16);
+  const int r2 = __builtin_amdgcn_readlane(b, 32), r3 = __builtin_amdgcn_readlane(b, 48);
+  if (lpr == 32) return (threadIdx.x & 63) < 32 ? max(r0, r1) : max(r2, r3);
+  return max(max(r0, r1), max(r2, r3));
+}
+
 // x (already scaled) as hi and lo halves at value index v of a split row.
 __device__ __forceinline__ void put_split(_Float16* row, int v, float x) {
   const _Float16 h = (_Float16)x;
@@ -840,16 +855,24 @@ __device__ __forceinline__ void leg_inv_gsplit(const DevMember* mems, const Grou
 constexpr int kSplitFwdAs = 68;
 static_assert(kMaxLat / 2 <= kSplitFwdAs && 2 * kMaxLat <= kRawLat, "split forward rows");
 
-// One wave per staged row r (rows wave, wave + 8, ...): lane l < half loads
-// latitude l and its mirror nlat - 1 - l of both parts (part p at p
-// plane_stride), clamped as inv_split_load is.
+// A wave stages 64 / lpr rows a pass, lpr = 16, 32 or 64 lanes a row (the
+// northern latitudes rounded up: T21, T42, T63): pass j of wave w takes rows
+// w + 8 (j q + s), s = lane / lpr, q = 64 / lpr. Lane l = lane % lpr < half
+// loads latitude l and its mirror nlat - 1 - l of both parts (part p at p
+// plane_stride); every load is clamped (rows past the end load the last
+// row, lanes past half latitude 0) and issued before any use.
+__device__ __forceinline__ int fwd_lanes_per_row(int nlat) {
+  return nlat <= 32 ? 16 : nlat <= 64 ? 32 : 64;
+}
+
 template <typename Rows>
 __device__ __forceinline__ void fwd_split_load(int rows, int nlat, int plane_stride, Rows rowp, float (*v)[4]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int l = lane < (nlat >> 1) ? lane : 0;
+  const int lpr = fwd_lanes_per_row(nlat), q = 64 / lpr, sub = lane / lpr, l0 = lane & (lpr - 1);
+  const int l = l0 < (nlat >> 1) ? l0 : 0;
 #pragma unroll
   for (int jr = 0; jr < kSplitFwdRowsPerWave; jr++) {
-    const auto x = rowp(min(wave + kGroupWaves * jr, rows - 1));
+    const auto x = rowp(min(wave + kGroupWaves * (jr * q + sub), rows - 1));
     v[jr][0] = x[l];
     v[jr][1] = x[nlat - 1 - l];
     v[jr][2] = x[plane_stride + l];
@@ -857,16 +880,19 @@ __device__ __forceinline__ void fwd_split_load(int rows, int nlat, int plane_str
   }
 }
 
-// Row r's folded runs E = north + south and O = north - south of each part,
-// each with its own scale scale[4 r + 2 eo + part], stored split: run (eo,
-// part) at halves eo 2 kMaxLat + part kMaxLat of the row.
+// Each staged row's folded runs E = north + south and O = north - south of
+// each part, each with its own scale scale[4 r + 2 eo + part], stored
+// split: run (eo, part) at halves eo 2 kMaxLat + part kMaxLat of the row.
+// The arithmetic runs in every lane (the reductions need them all); the
+// stores are guarded.
 __device__ __forceinline__ void fwd_split_store(int rows, int nlat, const float (*v)[4], float* raw, float* scale) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const bool ok = lane < (nlat >> 1);
+  const int lpr = fwd_lanes_per_row(nlat), q = 64 / lpr, sub = lane / lpr, l = lane & (lpr - 1);
+  const bool ok = l < (nlat >> 1);
 #pragma unroll
   for (int jr = 0; jr < kSplitFwdRowsPerWave; jr++) {
-    const int r = wave + kGroupWaves * jr;
-    if (r < rows) {
+    if (wave + kGroupWaves * jr * q < rows) {
+      const int r = wave + kGroupWaves * (jr * q + sub);
       // k = 2 eo + part
       float x[4];
       x[0] = ok ? v[jr][0] + v[jr][1] : 0.0f;
@@ -875,12 +901,14 @@ __device__ __forceinline__ void fwd_split_store(int rows, int nlat, const float 
       x[3] = ok ? v[jr][2] - v[jr][3] : 0.0f;
       float sc[4];
 #pragma unroll
-      for (int k = 0; k < 4; k++) sc[k] = pow2_scale_bits(wave_max_bits(abs_bits(x[k])));
-      if (lane < 4) scale[4 * r + lane] = lane == 0 ? sc[0] : lane == 1 ? sc[1] : lane == 2 ? sc[2] : sc[3];
-      _Float16* h = reinterpret_cast<_Float16*>(raw + r * kRawLat);
-      if (ok) {
+      for (int k = 0; k < 4; k++) sc[k] = pow2_scale_bits(rows_max_bits(abs_bits(x[k]), lpr));
+      if (r < rows) {
+        if (l < 4) scale[4 * r + l] = l == 0 ? sc[0] : l == 1 ? sc[1] : l == 2 ? sc[2] : sc[3];
+        _Float16* h = reinterpret_cast<_Float16*>(raw + r * kRawLat);
+        if (ok) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) put_split(h + (k >> 1) * (2 * kMaxLat) + (k & 1) * kMaxLat, lane, x[k] * sc[k]);
+          for (int k = 0; k < 4; k++) put_split(h + (k >> 1) * (2 * kMaxLat) + (k & 1) * kMaxLat, l, x[k] * sc[k]);
+        }
       }
     }
   }
